@@ -1,0 +1,150 @@
+/*
+ * ttgpu.h -- C ABI of the MI355X (gfx950) Felsenstein-pruning engine.
+ *
+ * The reference (TreeTime 0.12.1) is pure Python and has no FFI layer: its
+ * boundary is the TreeAnc / GTR method surface.  Each entry point below names
+ * the reference call it replaces (paths relative to the reference checkout).
+ * The Python package treetime_amd binds these with ctypes and keeps the
+ * reference's method names and argument meaning on top of them.
+ *
+ * Conventions
+ *   - Every function returns TT_OK (0) or a negative status; the message of
+ *     the last failure on the calling thread is tt_last_error().
+ *   - Host input arrays are copied; the caller keeps ownership.
+ *   - A tt_tree owns its device state (patterns, messages, profiles).
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *   - Host output pointers are filled before the call returns (the call
+ *     synchronises its stream when it has host outputs).
+ *   - No C++ exception crosses this boundary.  One host thread per tree at a
+ *     time; concurrency comes from streams and from tt_batch_*.
+ */
+#ifndef TTGPU_H
+#define TTGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TT_OK = 0,
+    TT_EINVAL = -1,         /* bad argument (maps to ValueError / TypeError) */
+    TT_ENOMEM_BUDGET = -2,  /* admission would exceed the engine's HBM budget */
+    TT_EDEVICE = -3,        /* HIP runtime error */
+    TT_ERANGE = -4,         /* mu*t*lambda > 10: reference gtr.py:1042-1047 raises ValueError */
+    TT_ESTATE = -5          /* call order violated (e.g. joint results before tt_joint) */
+};
+
+/* flags for tt_marginal / tt_joint */
+#define TT_RECONSTRUCT_TIPS 1u  /* reconstruct_tip_states=True (treeanc.py:900-903, 1043-1045) */
+
+/* which= values for tt_get_profiles */
+#define TT_PROFILE_MARGINAL 0   /* node.marginal_profile (treeanc.py:909-917, 822-824) */
+#define TT_PROFILE_SUBTREE 1    /* node.marginal_subtree_LH, row-normalised (treeanc.py:877) */
+
+typedef struct tt_engine tt_engine;
+typedef struct tt_tree tt_tree;
+
+/* Thread-local text of the last error. */
+const char *tt_last_error(void);
+/* ABI version: (major << 16) | minor. */
+int32_t tt_version(void);
+
+/* ---- engine: one per device, holds the HBM admission budget ------------ */
+int32_t tt_engine_create(int32_t device, uint64_t hbm_budget_bytes, tt_engine **out);
+int32_t tt_engine_destroy(tt_engine *eng);
+int32_t tt_engine_usage(const tt_engine *eng, uint64_t *bytes_in_use, uint64_t *budget);
+
+/* ---- tree: topology + compressed alignment, resident on the device ------
+ * Replaces the per-node attribute set-up of TreeAnc (treeanc.py:316-503,
+ * 840-853) and the compressed alignment of SequenceData
+ * (sequence_data.py:325-464).  Tips are the nodes that have no children; tip
+ * rows of tip_codes follow ascending node index.  Children of a node are
+ * visited in ascending node index (callers put ladderized order into the
+ * numbering).  code_lut[c] is the profile vector of character code c
+ * (seq_utils.py:28-122 profile maps).
+ */
+typedef struct {
+    int32_t n_nodes;            /* tips + internal nodes */
+    int32_t n_states;           /* alphabet size n */
+    int32_t n_patterns;         /* P, compressed alignment length */
+    int32_t n_codes;            /* rows of code_lut (<= 256) */
+    const int32_t *parent;      /* [n_nodes], -1 for the root */
+    const double *branch_len;   /* [n_nodes] effective length (treeanc.py:752-760); root ignored */
+    const uint8_t *tip_codes;   /* [n_tips][P] pattern-major character codes */
+    const float *code_lut;      /* [n_codes][n_states] */
+    const double *multiplicity; /* [P] (sequence_data.py:302-306) */
+    const double *pi;           /* [n_states] equilibrium frequencies */
+} tt_tree_desc;
+
+int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *desc, tt_tree **out);
+int32_t tt_tree_destroy(tt_tree *tree);
+int32_t tt_tree_info(const tt_tree *tree, int32_t *n_levels_post, int32_t *n_levels_pre, uint64_t *device_bytes);
+
+/* GTR eigensystem (gtr.py:612-629): P(t) = v diag(exp(mu t lambda)) v_inv,
+ * clipped at 0 (gtr.py:1051-1067).  Rebuilds P(t) of every branch in FP64 on
+ * the device.  Replaces GTR.expQt calls inside the passes. */
+int32_t tt_tree_set_model(tt_tree *tree, const double *eigvals, const double *v, const double *v_inv, double mu);
+
+/* New effective lengths for k branches (node ids); rebuilds their P(t). */
+int32_t tt_tree_set_branch_lengths(tt_tree *tree, const int32_t *nodes, const double *t, int32_t k);
+
+/* ---- passes --------------------------------------------------------------
+ * tt_marginal: TreeAnc._ml_anc_marginal (treeanc.py:762-932): postorder,
+ * root likelihood, preorder with the division trick, argmax states.
+ * *logL receives tree.total_sequence_LH (treeanc.py:828). */
+int32_t tt_marginal(tt_tree *tree, uint32_t flags, double *logL, void *stream);
+
+/* tt_loglik: postorder + root only (treeanc.py:840-878 then 814-829 with
+ * assign_sequence=False), the reference's way to re-evaluate logL. */
+int32_t tt_loglik(tt_tree *tree, double *logL, void *stream);
+
+/* tt_joint: TreeAnc._ml_anc_joint (treeanc.py:934-1080); *logL receives
+ * tree.sequence_joint_LH (treeanc.py:1037). */
+int32_t tt_joint(tt_tree *tree, uint32_t flags, double *logL, void *stream);
+
+/* tt_update_branch: change one branch length and recompute the marginal logL
+ * along the path from that branch to the root only.  Equivalent to setting
+ * the length and re-running tt_loglik (treeanc.py:1685-1689 pattern). */
+int32_t tt_update_branch(tt_tree *tree, int32_t node, double t, double *logL, void *stream);
+
+/* ---- results (device -> host, synchronous) ------------------------------ */
+/* tree.sequence_LH: per-pattern log-likelihood of the last pass
+ * (marginal: treeanc.py:827, joint: treeanc.py:1036). */
+int32_t tt_get_sequence_lh(tt_tree *tree, double *out);
+/* Reconstructed state index per node and pattern, [n_nodes][P]; 255 where a
+ * node has no reconstruction (tips without TT_RECONSTRUCT_TIPS). */
+int32_t tt_get_states(tt_tree *tree, uint8_t *out);
+/* Profiles, [n_nodes][P][n_states] (float32): which = TT_PROFILE_*.  Rows a
+ * pass did not produce are zero. */
+int32_t tt_get_profiles(tt_tree *tree, int32_t which, float *out);
+/* Device pointers of the resident buffers (for zero-copy consumers). */
+int32_t tt_get_device_buffers(tt_tree *tree, float **profiles, uint8_t **states, double **sequence_lh);
+
+/* ---- primitives ----------------------------------------------------------
+ * GTR.propagate_profile (gtr.py:965-995) for transpose=0, GTR.evolve
+ * (gtr.py:997-1025) for transpose=1: out = in @ Q (or in @ Q^T), optionally
+ * log.  in/out are device buffers of P*n doubles; Q is a host n*n FP64
+ * expQt. */
+int32_t tt_propagate_profile(const double *in_dev, int32_t P, int32_t n, const double *Q, int32_t transpose,
+                             int32_t return_log, double *out_dev, void *stream);
+
+/* GTR.prob_t_profiles (gtr.py:922-963) for K transition matrices at once:
+ * out[k] = sum_a mult[a] * log(sum_ij pc[a,i] Q_k[i,j] pp[a,j] + 1e-24)
+ * (times the non-gap weight when gap_index >= 0).  pp/pc/mult are device
+ * buffers; Q is host [K][n][n] FP64; out is host [K]. */
+int32_t tt_prob_t_profiles(const double *pp_dev, const double *pc_dev, const double *mult_dev, int32_t P,
+                           int32_t n, const double *Q, int32_t K, int32_t gap_index, double *out, void *stream);
+
+/* ---- batching --------------------------------------------------------------
+ * Marginal reconstruction of k resident trees in ONE persistent launch
+ * driven by a device work queue of (tree, level, pattern-tile) items
+ * (per tree the result equals tt_marginal).  logL is host [k]. */
+int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, double *logL,
+                          void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TTGPU_H */

@@ -1,0 +1,163 @@
+"""Parity of the HIP passes (through the C ABI) with the FP64 oracle.
+
+Tolerances (FP32 storage, FP64 arithmetic and normaliser sums):
+  total logL            relative error <= 1e-7   (measured ~1e-10)
+  per-pattern logL      absolute error <= 1e-6
+  marginal profiles     max absolute error <= 2e-6
+  argmax states         bit-exact wherever the FP64 margin between the two
+                        best states exceeds 1e-5 (joint: the smallest decision
+                        margin on the path from the root)
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import build_oracle, small_job
+from treetime_amd.engine import Engine
+from treetime_amd.prepare import device_tree
+
+pytestmark = pytest.mark.gpu
+
+LOGL_RTOL = 1e-7
+SITE_ATOL = 1e-6
+PROF_ATOL = 2e-6
+MARGIN = 1e-5
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def marginal_margin(prof):
+    s = np.sort(prof, axis=-1)
+    return s[..., -1] - s[..., -2]
+
+
+CASES = [
+    # name, n_tips, L, n_states, seed, depth, p_ambig, p_gap
+    ("c0-like", 200, 1700, 4, 0, 0.02, 0.01, 0.005),
+    ("deep", 64, 600, 4, 5, 0.5, 0.02, 0.01),
+    ("tiny", 3, 40, 4, 2, 0.1, 0.0, 0.0),
+    ("aa", 60, 300, 20, 3, 0.3, 0.0, 0.0),
+    ("p-not-multiple-of-64", 37, 97, 4, 9, 0.05, 0.05, 0.05),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_marginal_parity(gpu_engine, case):
+    name, n_tips, L, n_states, seed, depth, pa, pg = case
+    say(f"[marginal] case={name} n_tips={n_tips} L={L} n={n_states} seed={seed}")
+    job, prep, alphabet = small_job(n_tips, L, n_states, seed, depth, pa, pg)
+    ta, nodes, tips = build_oracle(prep, alphabet, job["gtr"])
+    ref = ta.marginal()
+    dt = device_tree(gpu_engine, prep)
+    got = dt.marginal()
+    assert abs(got - ref) <= LOGL_RTOL * abs(ref), (got, ref)
+    slh = dt.sequence_lh()
+    assert np.max(np.abs(slh - ta.sequence_LH)) <= SITE_ATOL
+    prof = dt.profiles()
+    states = dt.states()
+    for i, nd in enumerate(nodes):
+        if nd.is_terminal():
+            assert (states[i] == 255).all()
+            continue
+        err = np.max(np.abs(prof[i] - nd.marginal_profile))
+        assert err <= PROF_ATOL, (i, err)
+        ok = marginal_margin(nd.marginal_profile) > MARGIN
+        assert np.array_equal(states[i][ok], nd.seq_idx[ok]), i
+    dt.close()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_joint_parity(gpu_engine, case):
+    name, n_tips, L, n_states, seed, depth, pa, pg = case
+    say(f"[joint] case={name} n_tips={n_tips} L={L} n={n_states} seed={seed}")
+    job, prep, alphabet = small_job(n_tips, L, n_states, seed, depth, pa, pg)
+    ta, nodes, tips = build_oracle(prep, alphabet, job["gtr"])
+    ref = ta.joint()
+    dt = device_tree(gpu_engine, prep)
+    got = dt.joint()
+    assert abs(got - ref) <= LOGL_RTOL * abs(ref), (got, ref)
+    slh = dt.sequence_lh()
+    assert np.max(np.abs(slh - ta.joint_sequence_LH)) <= SITE_ATOL
+    states = dt.states()
+    for i, nd in enumerate(nodes):
+        if nd.is_terminal():
+            assert (states[i] == 255).all()
+            continue
+        ok = nd.path_margin > MARGIN
+        assert np.array_equal(states[i][ok], nd.seq_idx[ok]), i
+    dt.close()
+
+
+def test_reconstruct_tips(gpu_engine):
+    say("[tips] reconstruct_tip_states marginal + joint")
+    job, prep, alphabet = small_job(40, 300, 4, 13, 0.05, 0.05, 0.02)
+    ta, nodes, tips = build_oracle(prep, alphabet, job["gtr"])
+    ta.marginal(reconstruct_tip_states=True)
+    dt = device_tree(gpu_engine, prep)
+    dt.marginal(reconstruct_tips=True)
+    prof = dt.profiles()
+    states = dt.states()
+    for i in tips:
+        nd = nodes[i]
+        assert np.max(np.abs(prof[i] - nd.marginal_profile)) <= PROF_ATOL
+        ok = marginal_margin(nd.marginal_profile) > MARGIN
+        assert np.array_equal(states[i][ok], nd.seq_idx[ok])
+    ta.joint(reconstruct_tip_states=True)
+    dt.joint(reconstruct_tips=True)
+    states = dt.states()
+    for i in tips:
+        ok = nodes[i].path_margin > MARGIN
+        assert np.array_equal(states[i][ok], nodes[i].seq_idx[ok])
+    dt.close()
+
+
+def test_loglik_and_branch_update(gpu_engine):
+    say("[update] loglik + set_branch_lengths + update_branch")
+    job, prep, alphabet = small_job(80, 500, 4, 21, 0.05, 0.01, 0.0)
+    dt = device_tree(gpu_engine, prep)
+    base = dt.loglik()
+    ta, nodes, tips = build_oracle(prep, alphabet, job["gtr"])
+    assert abs(base - ta.marginal()) <= LOGL_RTOL * abs(base)
+    node = int(tips[3])
+    newt = prep["t"][node] * 1.7
+    got = dt.update_branch(node, newt)
+    prep2 = dict(prep)
+    prep2["t"] = prep["t"].copy()
+    prep2["t"][node] = newt
+    ta2, _, _ = build_oracle(prep2, alphabet, job["gtr"])
+    ref = ta2.marginal()
+    assert abs(got - ref) <= LOGL_RTOL * abs(ref)
+    dt.close()
+
+
+def test_repeat_runs_identical(gpu_engine):
+    """Graph replays give bit-identical results (deterministic reductions)."""
+    say("[repeat] graph replays")
+    job, prep, alphabet = small_job(150, 900, 4, 4)
+    dt = device_tree(gpu_engine, prep)
+    a = [dt.marginal() for _ in range(3)]
+    b = [dt.joint() for _ in range(2)]
+    c = dt.marginal()
+    assert a[0] == a[1] == a[2] == c
+    assert b[0] == b[1]
+    dt.close()
+
+
+def test_budget_rejection():
+    say("[budget] admission control")
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from treetime_amd._native import BudgetError
+
+    eng = Engine(0, 1 << 20)  # 1 MiB
+    job, prep, alphabet = small_job(200, 1700, 4, 0)
+    with pytest.raises(BudgetError):
+        device_tree(eng, prep)
+    used, budget = eng.usage()
+    assert used == 0 and budget == 1 << 20
+    eng.close()

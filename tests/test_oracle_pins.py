@@ -1,0 +1,335 @@
+"""Pins of the FP64 oracle against closed forms, brute force, library routines
+and the reference's own known-answer tests (test/test_treetime.py)."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from oracle import fp64 as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rand_model(n, seed, alphabet):
+    rng = np.random.default_rng(seed)
+    pi = rng.dirichlet(3 * np.ones(n))
+    W = np.zeros((n, n))
+    iu = np.triu_indices(n, 1)
+    W[iu] = rng.lognormal(0, 0.7, len(iu[0]))
+    W = W + W.T
+    return O.GTRModel(alphabet, mu=1.3, pi=pi, W=W)
+
+
+def standard_jc69():
+    """GTR.standard('JC69', alphabet='nuc_nogap'): built with W = ones, then
+    mu re-assigned to 1 on the normalised W (gtr.py:460, 126-128)."""
+    m = O.GTRModel("nuc_nogap", mu=1.0, pi=np.ones(4), W=np.ones((4, 4)))
+    m.assign_rates(1.0, m.Pi, m.W)
+    return m
+
+
+# ---------------------------------------------------------------- expQt ----
+@pytest.mark.parametrize("n,alphabet", [(4, "nuc_nogap"), (5, "nuc"), (20, "aa_nogap")])
+def test_expqt_matches_scipy_expm(n, alphabet):
+    m = rand_model(n, 7, alphabet)
+    for t in [0.0, 1e-6, 0.01, 0.3, 2.0]:
+        ref = expm(m.mu * m.Q * t)
+        assert np.allclose(m.expQt(t), np.maximum(ref, 0), atol=1e-12, rtol=1e-10)
+
+
+@pytest.mark.parametrize("n,alphabet", [(4, "nuc_nogap"), (20, "aa_nogap")])
+def test_expqt_invariants(n, alphabet):
+    m = rand_model(n, 3, alphabet)
+    s, t = 0.07, 0.21
+    Ps, Pt = m.expQt(s), m.expQt(t)
+    assert np.allclose(Ps.sum(axis=0), 1.0, atol=1e-12)                 # columns sum to 1
+    assert np.allclose(m.Pi[None, :] * Pt, (m.Pi[None, :] * Pt).T, atol=1e-14)  # detailed balance
+    assert np.allclose(m.expQt(0.0), np.eye(n), atol=1e-12)              # P(0) = I
+    assert np.allclose(Ps @ Pt, m.expQt(s + t), atol=1e-12)              # Chapman-Kolmogorov
+    assert np.allclose(m.expQt(8.0 / m.mu * 1.0 / max(1e-3, -m.eigenvals[:-1].max()))[:, 0], m.Pi, atol=1e-3)
+    assert np.abs((m.v @ m.v_inv - np.eye(n))).sum() < 1e-10            # test_treetime.py:83
+    assert np.abs(m.Q.sum(axis=0)).sum() < 1e-14                         # test_treetime.py:60,80
+
+
+def test_expqt_jc69_closed_form():
+    m = standard_jc69()
+    for t in [1e-4, 0.05, 0.5, 3.0]:
+        P = m.expQt(t)
+        e = np.exp(-4.0 * t / 3.0)
+        assert np.allclose(np.diag(P), 0.25 + 0.75 * e, atol=1e-13)
+        off = P[~np.eye(4, dtype=bool)]
+        assert np.allclose(off, 0.25 - 0.25 * e, atol=1e-13)
+
+
+def test_expqt_range_error():
+    m = standard_jc69()
+    with pytest.raises(ValueError):
+        m.expQt(-100.0)  # mu * t * lambda > 10 for the negative eigenvalues
+
+
+def test_gtr_text_fixture():
+    """test_treetime.py:46-75 on test/test_sequence_evolution_model.txt."""
+    with open(os.path.join(GOLDEN, "gtr_fixture.txt")) as f:
+        m = O.gtr_from_text(f.read())
+    assert (m.Pi.sum() - 1.0) ** 2 < 1e-14
+    assert np.allclose(m.Pi, [0.3088, 0.1897, 0.2335, 0.2581, 0.0099])
+    assert list(m.alphabet) == ["A", "C", "G", "T", "-"]
+    assert abs(m.mu - 1.0) < 1e-4
+    assert np.abs(m.Q.sum(0)).sum() < 1e-14
+    W = np.array([[0, 0.7003, 3.0669, 0.2651, 0.9742], [0.7003, 0, 0.3354, 3.399, 0.999],
+                  [3.0669, 0.3354, 0, 0.4258, 0.9892], [0.2651, 3.399, 0.4258, 0, 0.9848],
+                  [0.9742, 0.999, 0.9892, 0.9848, 0]])
+    assert np.allclose(m.W, W, atol=1e-4)
+
+
+# ------------------------------------------------------- tiny-tree helpers --
+def tiny_tree(newick_like):
+    """(parent, branch_length, names) of a hand-written tree description:
+    list of (name, parent_name, branch_length); the root has parent None."""
+    names = [x[0] for x in newick_like]
+    idx = {n: i for i, n in enumerate(names)}
+    parent = [(-1 if p is None else idx[p]) for _, p, _ in newick_like]
+    bl = [b for _, _, b in newick_like]
+    return O.build_tree(parent, bl, names)
+
+
+def enumerate_lh(root, nodes, model, tip_state_profiles, full_length):
+    """Brute-force likelihood per pattern and exact node posteriors, with P(t)
+    from scipy.linalg.expm (independent of the oracle's eigen route)."""
+    internal = [n for n in nodes if not n.is_terminal()]
+    n = model.n_states
+    P = {}
+    for nd in nodes:
+        if nd.up is not None:
+            t = max(O.MIN_BRANCH_LENGTH / full_length, nd.branch_length)
+            P[nd.name] = np.maximum(expm(model.mu * model.Q * t), 0.0)
+    L = next(iter(tip_state_profiles.values())).shape[0]
+    total = np.zeros(L)
+    post = {nd.name: np.zeros((L, n)) for nd in internal}
+    for combo in itertools.product(range(n), repeat=len(internal)):
+        st = {nd.name: s for nd, s in zip(internal, combo)}
+        w = np.full(L, model.Pi[st[root.name]])
+        for nd in nodes:
+            if nd.up is None:
+                continue
+            ps = st[nd.up.name]
+            if nd.is_terminal():
+                w = w * (tip_state_profiles[nd.name] @ P[nd.name][:, ps])
+            else:
+                w = w * P[nd.name][st[nd.name], ps]
+        total += w
+        for nd in internal:
+            post[nd.name][:, st[nd.name]] += w
+    return total, {k: v / total[:, None] for k, v in post.items()}
+
+
+def three_tip_reference_case():
+    """test_treetime.py:139-155: ((A:0.601,B:0.301):0.1,C:0.2), pi=[.9,.06,.02,.02],
+    W = ones, all 64 tip patterns."""
+    root, nodes = tiny_tree([("R", None, 0.001), ("X", "R", 0.1), ("A", "X", 0.60100000009),
+                             ("B", "X", 0.3010000009), ("C", "R", 0.2)])
+    model = O.GTRModel("nuc_nogap", mu=1.0, pi=np.array([0.9, 0.06, 0.02, 0.02]), W=np.ones((4, 4)))
+    cols = list(itertools.product("ACGT", repeat=3))
+    aln = {k: np.array([c[i] for c in cols]) for i, k in enumerate("ABC")}
+    return root, nodes, model, aln
+
+
+def test_marginal_lh_sums_to_one_reference_case():
+    """Sum over all n^3 patterns of exp(sequence_LH) = 1 (test_treetime.py:149-152)."""
+    root, nodes, model, aln = three_tip_reference_case()
+    comp, mult, _, _ = O.compress_alignment(list("ABC"), aln, "N")
+    ta = O.OracleTreeAnc(root, model, comp, mult, full_length=64)
+    ta.marginal()
+    assert abs(np.exp(ta.sequence_LH).sum() - 1.0) < 1e-6
+    assert len(mult) == 64  # every variable column is its own pattern
+
+
+def test_marginal_matches_enumeration():
+    root, nodes, model, aln = three_tip_reference_case()
+    comp, mult, _, _ = O.compress_alignment(list("ABC"), aln, "N")
+    ta = O.OracleTreeAnc(root, model, comp, mult, full_length=64)
+    ta.marginal()
+    tips = {k: O.seq2prof(comp[k], model.profile_map) for k in "ABC"}
+    lh, post = enumerate_lh(root, nodes, model, tips, 64)
+    assert np.allclose(ta.sequence_LH, np.log(lh), atol=1e-12)
+    for nd in nodes:
+        if not nd.is_terminal():
+            assert np.allclose(nd.marginal_profile, post[nd.name], atol=1e-10)
+
+
+def five_tip_case(seed, L=40, amb=True):
+    rng = np.random.default_rng(seed)
+    root, nodes = tiny_tree([("R", None, 0.0), ("X", "R", 0.13), ("Y", "X", 0.05), ("A", "Y", 0.31),
+                             ("B", "Y", 0.02), ("C", "X", 0.2), ("Z", "R", 0.4), ("D", "Z", 0.07),
+                             ("E", "Z", 0.11)])
+    model = rand_model(4, seed, "nuc_nogap")
+    chars = np.array(list("ACGTNRY-" if amb else "ACGT"))
+    aln = {k: rng.choice(chars, size=L) for k in "ABCDE"}
+    return root, nodes, model, aln
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_marginal_profiles_and_lh_five_tips(seed):
+    root, nodes, model, aln = five_tip_case(seed)
+    comp, mult, _, _ = O.compress_alignment(list("ABCDE"), aln, "N")
+    ta = O.OracleTreeAnc(root, model, comp, mult, full_length=40)
+    ta.marginal()
+    tips = {k: O.seq2prof(comp[k], model.profile_map) for k in "ABCDE"}
+    lh, post = enumerate_lh(root, nodes, model, tips, 40)
+    assert np.allclose(ta.sequence_LH, np.log(lh), atol=1e-11)
+    for nd in nodes:
+        if not nd.is_terminal():
+            assert np.allclose(nd.marginal_profile, post[nd.name], atol=1e-9)
+
+
+def test_pulley_principle():
+    """logL of a reversible model does not depend on where the root sits on
+    the root branch."""
+    model = rand_model(4, 11, "nuc_nogap")
+    rng = np.random.default_rng(0)
+    aln = {k: rng.choice(list("ACGT"), size=30) for k in "ABC"}
+    res = []
+    for a in [0.05, 0.2, 0.29]:
+        root, nodes = tiny_tree([("R", None, 0.0), ("X", "R", a), ("A", "X", 0.1), ("B", "X", 0.3),
+                                 ("C", "R", 0.3 - a)])
+        comp, mult, _, _ = O.compress_alignment(list("ABC"), aln, "N")
+        ta = O.OracleTreeAnc(root, model, comp, mult, full_length=1e9)
+        res.append(ta.marginal())
+    assert np.allclose(res, res[0], rtol=1e-12)
+
+
+# ------------------------------------------------------------------ joint --
+def test_joint_lh_is_max_reference_case():
+    """test_treetime.py:241-307: joint sequence_LH equals the brute-force max
+    over all 16 realisations of the internal states."""
+    model = O.GTRModel("nuc_nogap", mu=1.0, pi=np.array([0.91, 0.05, 0.02, 0.02]), W=np.ones((4, 4)))
+    root, nodes = tiny_tree([("E", None, 0.004), ("C", "E", 0.030), ("A", "C", 0.0060), ("B", "C", 0.30),
+                             ("D", "E", 0.020)])
+    comp = {"A": np.array(["A"]), "B": np.array(["C"]), "D": np.array(["G"])}
+    ta = O.OracleTreeAnc(root, model, comp, np.ones(1), full_length=1)
+    ta.joint()
+    best = -np.inf
+    idx = {c: i for i, c in enumerate("ACGT")}
+    for sc, se in itertools.product(range(4), repeat=2):
+        seqs = {"A": np.array([0]), "B": np.array([1]), "D": np.array([2]), "C": np.array([sc]), "E": np.array([se])}
+        best = max(best, ta.ancestral_likelihood(seqs)[0])
+    assert abs(best - ta.joint_sequence_LH[0]) < 1e-10
+    assert idx  # silence
+
+
+def test_joint_ancestral_likelihood_consistency():
+    """test_treetime.py:160-238: ancestral_likelihood of the joint
+    reconstruction equals tree.sequence_LH."""
+    model = O.GTRModel("nuc_nogap", mu=1.0, pi=np.array([0.15, 0.95, 0.05, 0.3]), W=np.ones((4, 4)))
+    root, nodes = tiny_tree([("E", None, 0.004), ("C", "E", 0.020), ("A", "C", 0.060), ("B", "C", 0.012),
+                             ("D", "E", 0.0050)])
+    rng = np.random.default_rng(1234)
+    L = 400
+    st = {"E": rng.choice(4, p=model.Pi, size=L)}
+    for nd in O.preorder(root)[1:]:
+        P = model.expQt(nd.branch_length)
+        st[nd.name] = np.array([rng.choice(4, p=P[:, s]) for s in st[nd.up.name]])
+    aln = {k: model.alphabet[st[k]] for k in "ABD"}
+    comp, mult, _, _ = O.compress_alignment(list("ABD"), aln, "N")
+    ta = O.OracleTreeAnc(root, model, comp, mult, full_length=L)
+    ta.joint(reconstruct_tip_states=True)
+    seqs = {nd.name: nd.seq_idx for nd in O.preorder(root)}
+    LH = ta.ancestral_likelihood(seqs)
+    assert abs((LH - ta.joint_sequence_LH).sum()) < 1e-9
+
+
+def test_joint_matches_enumeration_five_tips():
+    root, nodes, model, aln = five_tip_case(5, amb=False)
+    comp, mult, _, _ = O.compress_alignment(list("ABCDE"), aln, "N")
+    ta = O.OracleTreeAnc(root, model, comp, mult, full_length=1e9)
+    ta.joint()
+    internal = [n for n in nodes if not n.is_terminal()]
+    tipidx = {k: np.array([{"A": 0, "C": 1, "G": 2, "T": 3}[c] for c in comp[k]]) for k in "ABCDE"}
+    best = np.full(len(mult), -np.inf)
+    arg = None
+    for combo in itertools.product(range(4), repeat=len(internal)):
+        seqs = dict(tipidx)
+        for nd, s in zip(internal, combo):
+            seqs[nd.name] = np.full(len(mult), s)
+        lh = ta.ancestral_likelihood(seqs)
+        better = lh > best
+        best = np.where(better, lh, best)
+        if arg is None:
+            arg = {nd.name: np.zeros(len(mult), dtype=int) for nd in internal}
+        for nd, s in zip(internal, combo):
+            arg[nd.name][better] = s
+    assert np.allclose(ta.joint_sequence_LH, best, atol=1e-10)
+    for nd in internal:
+        assert np.array_equal(nd.seq_idx, arg[nd.name])
+
+
+# ------------------------------------------------------------ compression --
+def test_compression_rules():
+    """Hand-built columns for each rule of sequence_data.py:378-413."""
+    cols = [
+        "AAAA",  # 0 constant A
+        "AANA",  # 1 one state + N -> rewritten to A, merges with 0
+        "ACAA",  # 2 variable -> private
+        "ACAA",  # 3 identical variable column -> still private
+        "A-AA",  # 4 gap makes it variable (gap is not the ambiguous char)
+        "NNNN",  # 5 all ambiguous -> constant N pattern
+        "CCCC",  # 6 constant C
+        "NNNN",  # 7 merges with 5
+        "ANCA",  # 8 two states + N -> private, N kept
+        "N-NN",  # 9 gap + N -> N rewritten to '-', constant gap column
+    ]
+    names = ["s1", "s2", "s3", "s4"]
+    aln = {nm: np.array([c[i] for c in cols]) for i, nm in enumerate(names)}
+    comp, mult, f2c, c2f = O.compress_alignment(names, aln, "N")
+    P = len(mult)
+    assert f2c.tolist() == [0, 0, 1, 2, 3, 4, 5, 4, 6, 7]
+    assert mult.tolist() == [2, 1, 1, 1, 2, 1, 1, 1]
+    assert P == 8
+    assert "".join(comp[nm][7] for nm in names) == "----"
+    assert "".join(comp[nm][6] for nm in names) == "ANCA"
+    assert c2f[4].tolist() == [5, 7]
+
+
+def test_compression_additional_constant_sites():
+    aln = {"a": np.array(list("ACGT")), "b": np.array(list("ACGA"))}
+    comp, mult, f2c, _ = O.compress_alignment(["a", "b"], aln, "N", additional_constant_sites=6,
+                                              nogap_alphabet=O.ALPHABETS["nuc_nogap"])
+    assert mult.sum() == 10
+    assert len(f2c) == 10
+
+
+# ----------------------------------------------------- branch statistics --
+def test_prob_t_profiles_one_hot_equals_compressed():
+    m = rand_model(4, 21, "nuc_nogap")
+    rng = np.random.default_rng(2)
+    sp = rng.integers(0, 4, size=50)
+    sc = rng.integers(0, 4, size=50)
+    mult = rng.integers(1, 5, size=50).astype(float)
+    pp, pc = np.eye(4)[sp], np.eye(4)[sc]
+    pairs = np.stack([sp, sc], axis=1)
+    for t in [0.01, 0.2]:
+        a = m.prob_t_profiles((pp, pc), mult, t, return_log=True)
+        b = m.prob_t_compressed(pairs, mult, t, return_log=True)
+        assert abs(a - b) < 1e-9 * abs(b)
+
+
+def test_optimal_t_jc69_closed_form():
+    """JC69 (rate 1): the ML distance of two sequences with mismatch fraction
+    p is t = -3/4 ln(1 - 4/3 p)."""
+    m = standard_jc69()
+    seq_p = np.array(list("A" * 80 + "C" * 20))
+    seq_c = np.array(list("A" * 70 + "G" * 10 + "C" * 20))
+    pairs, mult = m.state_pair(seq_p, seq_c)
+    t = m.optimal_t_compressed(pairs, mult)
+    p = 10 / 100
+    assert abs(t - (-0.75 * np.log(1 - 4.0 / 3.0 * p))) < 1e-6
+
+
+def test_state_pair_counts_truncate():
+    m = standard_jc69()
+    pairs, mult = m.state_pair(np.array(list("AACN")), np.array(list("ACCA")),
+                               pattern_multiplicity=np.array([1.5, 2.7, 1.0, 3.0]))
+    d = {tuple(p): c for p, c in zip(pairs.tolist(), mult.tolist())}
+    assert d == {(0, 0): 1, (0, 1): 2, (1, 1): 1}

@@ -1,0 +1,706 @@
+// C-ABI of the pruning engine (declared in include/ttgpu.h).
+//
+// Host side of a tree: flattens the topology into a level schedule
+// (postorder levels = node heights, preorder levels = node depths), uploads
+// the pattern-major tip codes, owns every device buffer of the tree, and
+// records each pass (level kernels + reductions) once into a hipGraph that is
+// replayed on later calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ttgpu.h"
+#include "tt_common.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int32_t fail(int32_t code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define TT_HIP(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) return fail(TT_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+bool supported_states(int n) { return n == 2 || n == 3 || n == 4 || n == 5 || n == 20 || n == 21 || n == 22; }
+
+// Waves a level launch aims for before it starts giving each wave several
+// nodes; 256 CUs x 4 SIMDs x 8 waves.
+int g_target_waves() {
+    static int v = [] {
+        const char *e = getenv("TT_TARGET_WAVES");
+        int x = e ? atoi(e) : 8192;
+        return x > 0 ? x : 8192;
+    }();
+    return v;
+}
+
+bool g_use_graphs() {
+    static bool v = [] {
+        const char *e = getenv("TT_NO_GRAPH");
+        return !(e && e[0] == '1');
+    }();
+    return v;
+}
+
+}  // namespace
+
+struct tt_engine {
+    int device = 0;
+    uint64_t budget = 0;
+    std::atomic<uint64_t> used{0};
+};
+
+struct PostLv {
+    int off, cnt, chunk, nchunks, row_off, rows;
+};
+struct PreLv {
+    int off, n_int, n_all;
+};
+
+struct tt_tree {
+    tt_engine *eng = nullptr;
+    int n_nodes = 0, n_tips = 0, n_int = 0, n = 0, P = 0, n_codes = 0, root = -1, root_row = -1;
+    std::vector<int32_t> parent, int_row, tip_row, int_node, child_ptr, child_ref, child_node, tip_nodes;
+    std::vector<double> t;
+    std::vector<int32_t> post_nodes, pre_nodes;
+    std::vector<PostLv> post_lv;  // heights 1..H; the last level is the root
+    std::vector<PreLv> pre_lv;    // depths 1..D
+    int R_all = 0, R_joint = 0;
+    bool have_model = false;
+    double mu = 1.0;
+    std::vector<double> eig, v, vinv;
+
+    TreeDev d{};
+    int32_t *d_post_nodes = nullptr, *d_pre_nodes = nullptr, *d_tip_nodes = nullptr, *d_all_branches = nullptr;
+    double *d_t = nullptr, *d_eig = nullptr, *d_v = nullptr, *d_vinv = nullptr;
+    double *d_tmp_t = nullptr;
+    int32_t *d_tmp_br = nullptr;
+    int tmp_cap = 0;
+    bool joint_ready = false;
+    hipStream_t stream = nullptr;
+    std::map<int, hipGraphExec_t> graphs;
+    std::vector<std::pair<void *, size_t>> allocs;
+    int last_pass = 0;  // 1 marginal, 2 joint, 3 loglik
+    uint64_t bytes = 0;
+
+    int32_t alloc(void **p, size_t nbytes) {
+        nbytes = std::max<size_t>(nbytes, 16);
+        uint64_t cur = eng->used.load();
+        if (eng->budget && cur + nbytes > eng->budget)
+            return fail(TT_ENOMEM_BUDGET, "HBM budget exceeded: %llu in use + %llu requested > budget %llu",
+                        (unsigned long long)cur, (unsigned long long)nbytes, (unsigned long long)eng->budget);
+        hipError_t e = hipMalloc(p, nbytes);
+        if (e != hipSuccess) return fail(TT_EDEVICE, "hipMalloc(%zu): %s", nbytes, hipGetErrorString(e));
+        eng->used += nbytes;
+        bytes += nbytes;
+        allocs.emplace_back(*p, nbytes);
+        return TT_OK;
+    }
+    template <class T>
+    int32_t upload(T **dst, const T *src, size_t count) {
+        int32_t rc = alloc((void **)dst, count * sizeof(T));
+        if (rc) return rc;
+        if (count) TT_HIP(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+        return TT_OK;
+    }
+    ~tt_tree() {
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+        for (auto &a : allocs) {
+            (void)hipFree(a.first);
+            eng->used -= a.second;
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+const char *tt_last_error(void) { return g_err.c_str(); }
+int32_t tt_version(void) { return (1 << 16) | 0; }
+
+int32_t tt_engine_create(int32_t device, uint64_t hbm_budget_bytes, tt_engine **out) {
+    if (!out) return fail(TT_EINVAL, "out is NULL");
+    int count = 0;
+    TT_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(TT_EINVAL, "device %d out of range (%d devices)", device, count);
+    TT_HIP(hipSetDevice(device));
+    auto *e = new tt_engine;
+    e->device = device;
+    e->budget = hbm_budget_bytes;
+    *out = e;
+    return TT_OK;
+}
+
+int32_t tt_engine_destroy(tt_engine *eng) {
+    delete eng;
+    return TT_OK;
+}
+
+int32_t tt_engine_usage(const tt_engine *eng, uint64_t *bytes_in_use, uint64_t *budget) {
+    if (!eng) return fail(TT_EINVAL, "engine is NULL");
+    if (bytes_in_use) *bytes_in_use = eng->used.load();
+    if (budget) *budget = eng->budget;
+    return TT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Topology -> children lists, heights, depths and the level schedule.
+int32_t build_schedule(tt_tree *T, const tt_tree_desc *D) {
+    const int nn = D->n_nodes;
+    T->parent.assign(D->parent, D->parent + nn);
+    std::vector<std::vector<int>> kids(nn);
+    int root = -1;
+    for (int i = 0; i < nn; ++i) {
+        int p = T->parent[i];
+        if (p < 0) {
+            if (root >= 0) return fail(TT_EINVAL, "more than one root (nodes %d and %d)", root, i);
+            root = i;
+        } else {
+            if (p >= nn) return fail(TT_EINVAL, "parent[%d]=%d out of range", i, p);
+            kids[p].push_back(i);
+        }
+    }
+    if (root < 0) return fail(TT_EINVAL, "no root (parent == -1) found");
+    // preorder from the root: detects cycles / disconnected nodes
+    std::vector<int> order;
+    order.reserve(nn);
+    std::vector<int> depth(nn, -1);
+    order.push_back(root);
+    depth[root] = 0;
+    for (size_t k = 0; k < order.size(); ++k) {
+        for (int c : kids[order[k]]) {
+            depth[c] = depth[order[k]] + 1;
+            order.push_back(c);
+        }
+    }
+    if ((int)order.size() != nn) return fail(TT_EINVAL, "topology is not a single rooted tree");
+    if (kids[root].empty()) return fail(TT_EINVAL, "root has no children");
+    T->root = root;
+    T->int_row.assign(nn, -1);
+    T->tip_row.assign(nn, -1);
+    int ni = 0, nt = 0;
+    for (int i = 0; i < nn; ++i) {
+        if (kids[i].empty())
+            T->tip_row[i] = nt++;
+        else
+            T->int_row[i] = ni++;
+    }
+    T->n_tips = nt;
+    T->n_int = ni;
+    T->root_row = T->int_row[root];
+    T->int_node.assign(ni, 0);
+    T->tip_nodes.clear();
+    for (int i = 0; i < nn; ++i) {
+        if (T->int_row[i] >= 0) T->int_node[T->int_row[i]] = i;
+        if (T->tip_row[i] >= 0) T->tip_nodes.push_back(i);
+    }
+    T->child_ptr.assign(ni + 1, 0);
+    T->child_ref.clear();
+    T->child_node.clear();
+    for (int r = 0; r < ni; ++r) {
+        int v = T->int_node[r];
+        for (int c : kids[v]) {
+            T->child_ref.push_back(T->int_row[c] >= 0 ? T->int_row[c] : -(T->tip_row[c] + 1));
+            T->child_node.push_back(c);
+        }
+        T->child_ptr[r + 1] = (int)T->child_ref.size();
+    }
+    // heights (postorder levels)
+    std::vector<int> height(nn, 0);
+    for (int k = nn - 1; k >= 0; --k) {
+        int v = order[k];
+        for (int c : kids[v]) height[v] = std::max(height[v], height[c] + 1);
+    }
+    const int H = height[root];
+    std::vector<std::vector<int>> byh(H + 1);
+    for (int r = 0; r < ni; ++r) byh[height[T->int_node[r]]].push_back(r);
+    const int npt = (D->n_patterns + TT_WAVE - 1) / TT_WAVE;
+    T->post_nodes.clear();
+    T->post_lv.clear();
+    int row = 0;
+    for (int h = 1; h <= H; ++h) {
+        PostLv L;
+        L.off = (int)T->post_nodes.size();
+        L.cnt = (int)byh[h].size();
+        long waves = (long)L.cnt * npt;
+        L.chunk = (int)std::max<long>(1, (waves + g_target_waves() - 1) / g_target_waves());
+        L.nchunks = (L.cnt + L.chunk - 1) / L.chunk;
+        L.rows = (L.nchunks + 3) / 4;
+        L.row_off = row;
+        row += L.rows;
+        for (int r : byh[h]) T->post_nodes.push_back(r);
+        T->post_lv.push_back(L);
+    }
+    T->R_all = row;
+    T->R_joint = T->post_lv.back().row_off;  // rows before the root level
+    // depths (preorder levels): internal nodes first, then tips
+    const int Dm = *std::max_element(depth.begin(), depth.end());
+    std::vector<std::vector<int>> byd_i(Dm + 1), byd_t(Dm + 1);
+    for (int i = 0; i < nn; ++i) {
+        if (i == root) continue;
+        (T->int_row[i] >= 0 ? byd_i : byd_t)[depth[i]].push_back(i);
+    }
+    T->pre_nodes.clear();
+    T->pre_lv.clear();
+    for (int dd = 1; dd <= Dm; ++dd) {
+        PreLv L;
+        L.off = (int)T->pre_nodes.size();
+        L.n_int = (int)byd_i[dd].size();
+        L.n_all = L.n_int + (int)byd_t[dd].size();
+        for (int x : byd_i[dd]) T->pre_nodes.push_back(x);
+        for (int x : byd_t[dd]) T->pre_nodes.push_back(x);
+        T->pre_lv.push_back(L);
+    }
+    return TT_OK;
+}
+
+int rowsum_slices(int R) { return std::min(R, 64); }
+
+int32_t ensure_joint(tt_tree *T) {
+    if (T->joint_ready) return TT_OK;
+    int32_t rc;
+    size_t rows = (size_t)T->n_int * T->P * T->n;
+    if ((rc = T->alloc((void **)&T->d.Lx, rows * sizeof(float)))) return rc;
+    if ((rc = T->alloc((void **)&T->d.Cx, (size_t)T->n_nodes * T->P * T->n))) return rc;
+    T->joint_ready = true;
+    return TT_OK;
+}
+
+// Enqueue one pass on stream s.  mode: 1 marginal, 2 joint, 3 loglik.
+int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
+    const TreeDev &D = T->d;
+    const bool tips = (flags & TT_RECONSTRUCT_TIPS) != 0;
+    const int nb = ttk::root_blocks(T->P);
+    if (mode == 1 || mode == 3) {
+        for (auto &L : T->post_lv) {
+            LevelDev ld{T->d_post_nodes + L.off, L.cnt, L.chunk, L.nchunks, L.row_off};
+            TT_HIP(ttk::post_marginal(D, ld, s));
+        }
+        const int S = rowsum_slices(T->R_all);
+        TT_HIP(ttk::rowsum(D, T->R_all, S, s));
+        TT_HIP(ttk::root_marginal(D, S, s));
+        TT_HIP(ttk::final_sum(D, nb, s));
+        if (mode == 1) {
+            if (!tips) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
+            for (auto &L : T->pre_lv) {
+                LevelDev ld{T->d_pre_nodes + L.off, tips ? L.n_all : L.n_int, 1, 0, 0};
+                TT_HIP(ttk::pre_marginal(D, ld, s));
+            }
+        }
+    } else {
+        for (size_t h = 0; h + 1 < T->post_lv.size(); ++h) {
+            auto &L = T->post_lv[h];
+            LevelDev ld{T->d_post_nodes + L.off, L.cnt, L.chunk, L.nchunks, L.row_off};
+            TT_HIP(ttk::post_joint(D, ld, tips ? 1 : 0, s));
+        }
+        const int S = rowsum_slices(T->R_joint);
+        TT_HIP(ttk::rowsum(D, T->R_joint, S, s));
+        TT_HIP(ttk::root_joint(D, S, tips ? 1 : 0, s));
+        TT_HIP(ttk::final_sum(D, nb, s));
+        if (!tips) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
+        for (auto &L : T->pre_lv) {
+            LevelDev ld{T->d_pre_nodes + L.off, tips ? L.n_all : L.n_int, 1, 0, 0};
+            TT_HIP(ttk::backtrack(D, ld, s));
+        }
+    }
+    return TT_OK;
+}
+
+int32_t run_pass(tt_tree *T, int mode, uint32_t flags, double *logL, void *stream) {
+    if (!T) return fail(TT_EINVAL, "tree is NULL");
+    if (!T->have_model) return fail(TT_ESTATE, "tt_tree_set_model has not been called");
+    TT_HIP(hipSetDevice(T->eng->device));
+    if (mode == 2) {
+        int32_t rc = ensure_joint(T);
+        if (rc) return rc;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : T->stream;
+    const int key = mode * 8 + (int)(flags & TT_RECONSTRUCT_TIPS);
+    if (g_use_graphs()) {
+        auto it = T->graphs.find(key);
+        if (it == T->graphs.end()) {
+            hipGraph_t g = nullptr;
+            TT_HIP(hipStreamBeginCapture(T->stream, hipStreamCaptureModeThreadLocal));
+            int32_t rc = enqueue_pass(T, mode, flags, T->stream);
+            hipError_t e = hipStreamEndCapture(T->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(TT_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+            hipGraphExec_t ex = nullptr;
+            e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(TT_EDEVICE, "graph instantiate: %s", hipGetErrorString(e));
+            it = T->graphs.emplace(key, ex).first;
+        }
+        TT_HIP(hipGraphLaunch(it->second, s));
+    } else {
+        int32_t rc = enqueue_pass(T, mode, flags, s);
+        if (rc) return rc;
+    }
+    T->last_pass = mode;
+    if (logL) {
+        TT_HIP(hipMemcpyAsync(logL, T->d.total, sizeof(double), hipMemcpyDeviceToHost, s));
+        TT_HIP(hipStreamSynchronize(s));
+    }
+    return TT_OK;
+}
+
+int32_t refresh_expqt(tt_tree *T, const int32_t *d_branches, const double *d_t, int k, hipStream_t s) {
+    TT_HIP(ttk::expqt(T->n, d_t, d_branches, k, T->d_eig, T->d_v, T->d_vinv, T->mu, (double *)T->d.P64,
+                      (double *)T->d.logP64, s));
+    return TT_OK;
+}
+
+int32_t check_range(tt_tree *T, const double *t, int k) {
+    double lmax = -INFINITY;
+    for (double x : T->eig) lmax = std::max(lmax, x);
+    for (int i = 0; i < k; ++i) {
+        if (!(t[i] >= 0.0) && !(t[i] < 0.0)) return fail(TT_EINVAL, "branch length is NaN");
+        if (T->mu * t[i] * lmax > 10.0)
+            return fail(TT_ERANGE,
+                        "Error in computing exp(Q * t): Q has positive eigenvalues or the branch length t is too "
+                        "large (mu*t*lambda = %g)",
+                        T->mu * t[i] * lmax);
+    }
+    return TT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
+    if (!eng || !D || !out) return fail(TT_EINVAL, "NULL argument");
+    if (D->n_nodes < 2) return fail(TT_EINVAL, "n_nodes=%d: need at least 2 nodes", D->n_nodes);
+    if (!supported_states(D->n_states))
+        return fail(TT_EINVAL, "n_states=%d not supported (2,3,4,5,20,21,22)", D->n_states);
+    if (D->n_patterns < 1) return fail(TT_EINVAL, "n_patterns=%d", D->n_patterns);
+    if (D->n_codes < 1 || D->n_codes > 256) return fail(TT_EINVAL, "n_codes=%d out of 1..256", D->n_codes);
+    if (!D->parent || !D->branch_len || !D->tip_codes || !D->code_lut || !D->multiplicity || !D->pi)
+        return fail(TT_EINVAL, "NULL array in tree descriptor");
+    TT_HIP(hipSetDevice(eng->device));
+    auto *T = new tt_tree;
+    T->eng = eng;
+    T->n_nodes = D->n_nodes;
+    T->n = D->n_states;
+    T->P = D->n_patterns;
+    T->n_codes = D->n_codes;
+    int32_t rc = build_schedule(T, D);
+    if (rc) {
+        delete T;
+        return rc;
+    }
+    // validate codes on the host: every code must index code_lut
+    const size_t ncodes_total = (size_t)T->n_tips * T->P;
+    for (size_t i = 0; i < ncodes_total; ++i)
+        if (D->tip_codes[i] >= D->n_codes) {
+            delete T;
+            return fail(TT_EINVAL, "tip code %d at %zu >= n_codes %d", D->tip_codes[i], i, D->n_codes);
+        }
+    T->t.assign(D->branch_len, D->branch_len + D->n_nodes);
+    hipError_t he = hipStreamCreateWithFlags(&T->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete T;
+        return fail(TT_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(he));
+    }
+    const int n = T->n, P = T->P, nn = T->n_nodes;
+    std::vector<double> loglut((size_t)D->n_codes * n);
+    for (size_t i = 0; i < loglut.size(); ++i) loglut[i] = std::log(std::max((double)D->code_lut[i], TT_TINY));
+    std::vector<int32_t> all(nn);
+    for (int i = 0; i < nn; ++i) all[i] = i;
+    TreeDev &d = T->d;
+#define UP(dst, src, cnt)                                   \
+    if ((rc = T->upload(&dst, src, (size_t)(cnt)))) {       \
+        delete T;                                           \
+        return rc;                                          \
+    }
+    int32_t *p_child_ptr, *p_child_ref, *p_child_node, *p_parent, *p_int_row, *p_tip_row, *p_int_node;
+    uint8_t *p_codes;
+    float *p_lut;
+    double *p_loglut, *p_mult, *p_pi;
+    UP(p_child_ptr, T->child_ptr.data(), T->child_ptr.size());
+    UP(p_child_ref, T->child_ref.data(), T->child_ref.size());
+    UP(p_child_node, T->child_node.data(), T->child_node.size());
+    UP(p_parent, T->parent.data(), nn);
+    UP(p_int_row, T->int_row.data(), nn);
+    UP(p_tip_row, T->tip_row.data(), nn);
+    UP(p_int_node, T->int_node.data(), T->n_int);
+    UP(p_codes, D->tip_codes, ncodes_total);
+    UP(p_lut, D->code_lut, (size_t)D->n_codes * n);
+    UP(p_loglut, loglut.data(), loglut.size());
+    UP(p_mult, D->multiplicity, P);
+    UP(p_pi, D->pi, n);
+    UP(T->d_post_nodes, T->post_nodes.data(), T->post_nodes.size());
+    UP(T->d_pre_nodes, T->pre_nodes.data(), T->pre_nodes.size());
+    UP(T->d_tip_nodes, T->tip_nodes.data(), T->tip_nodes.size());
+    UP(T->d_all_branches, all.data(), nn);
+    UP(T->d_t, T->t.data(), nn);
+#undef UP
+    d.n_nodes = nn;
+    d.n_tips = T->n_tips;
+    d.n_int = T->n_int;
+    d.n = n;
+    d.P = P;
+    d.n_codes = D->n_codes;
+    d.root = T->root;
+    d.root_row = T->root_row;
+    d.child_ptr = p_child_ptr;
+    d.child_ref = p_child_ref;
+    d.child_node = p_child_node;
+    d.parent = p_parent;
+    d.int_row = p_int_row;
+    d.tip_row = p_tip_row;
+    d.int_node = p_int_node;
+    d.tip_codes = p_codes;
+    d.lut = p_lut;
+    d.loglut = p_loglut;
+    d.mult = p_mult;
+    d.pi = p_pi;
+    struct {
+        void **p;
+        size_t bytes;
+    } bufs[] = {
+        {(void **)&d.P64, (size_t)nn * n * n * sizeof(double)},
+        {(void **)&d.logP64, (size_t)nn * n * n * sizeof(double)},
+        {(void **)&d.sub, (size_t)T->n_int * P * n * sizeof(float)},
+        {(void **)&d.prof, (size_t)nn * P * n * sizeof(float)},
+        {(void **)&d.states, (size_t)nn * P},
+        {(void **)&d.part, (size_t)std::max(T->R_all, 1) * P * sizeof(double)},
+        {(void **)&d.part2, (size_t)std::max(rowsum_slices(T->R_all), 1) * P * sizeof(double)},
+        {(void **)&d.seqlh, (size_t)P * sizeof(double)},
+        {(void **)&d.bpart, (size_t)ttk::root_blocks(P) * sizeof(double)},
+        {(void **)&d.total, sizeof(double)},
+        {(void **)&T->d_eig, TT_MAX_STATES * sizeof(double)},
+        {(void **)&T->d_v, TT_MAX_STATES * TT_MAX_STATES * sizeof(double)},
+        {(void **)&T->d_vinv, TT_MAX_STATES * TT_MAX_STATES * sizeof(double)},
+    };
+    for (auto &b : bufs) {
+        if ((rc = T->alloc(b.p, b.bytes))) {
+            delete T;
+            return rc;
+        }
+    }
+    he = hipMemset(d.states, 255, (size_t)nn * P);
+    if (he == hipSuccess) he = hipMemset(d.prof, 0, (size_t)nn * P * n * sizeof(float));
+    // the passes run on a non-blocking stream that does not order after the
+    // null-stream memsets
+    if (he == hipSuccess) he = hipDeviceSynchronize();
+    if (he != hipSuccess) {
+        delete T;
+        return fail(TT_EDEVICE, "hipMemset: %s", hipGetErrorString(he));
+    }
+    *out = T;
+    return TT_OK;
+}
+
+int32_t tt_tree_destroy(tt_tree *tree) {
+    if (tree) {
+        (void)hipSetDevice(tree->eng->device);
+        (void)hipStreamSynchronize(tree->stream);
+    }
+    delete tree;
+    return TT_OK;
+}
+
+int32_t tt_tree_info(const tt_tree *T, int32_t *n_levels_post, int32_t *n_levels_pre, uint64_t *device_bytes) {
+    if (!T) return fail(TT_EINVAL, "tree is NULL");
+    if (n_levels_post) *n_levels_post = (int32_t)T->post_lv.size();
+    if (n_levels_pre) *n_levels_pre = (int32_t)T->pre_lv.size();
+    if (device_bytes) *device_bytes = T->bytes;
+    return TT_OK;
+}
+
+int32_t tt_tree_set_model(tt_tree *T, const double *eigvals, const double *v, const double *v_inv, double mu) {
+    if (!T || !eigvals || !v || !v_inv) return fail(TT_EINVAL, "NULL argument");
+    if (!(mu >= 0.0)) return fail(TT_EINVAL, "mu must be >= 0");
+    const int n = T->n;
+    T->eig.assign(eigvals, eigvals + n);
+    T->v.assign(v, v + n * n);
+    T->vinv.assign(v_inv, v_inv + n * n);
+    T->mu = mu;
+    int32_t rc = check_range(T, T->t.data(), T->n_nodes);
+    if (rc) return rc;
+    TT_HIP(hipSetDevice(T->eng->device));
+    TT_HIP(hipMemcpyAsync(T->d_eig, eigvals, n * sizeof(double), hipMemcpyHostToDevice, T->stream));
+    TT_HIP(hipMemcpyAsync(T->d_v, v, n * n * sizeof(double), hipMemcpyHostToDevice, T->stream));
+    TT_HIP(hipMemcpyAsync(T->d_vinv, v_inv, n * n * sizeof(double), hipMemcpyHostToDevice, T->stream));
+    if ((rc = refresh_expqt(T, T->d_all_branches, T->d_t, T->n_nodes, T->stream))) return rc;
+    TT_HIP(hipStreamSynchronize(T->stream));
+    T->have_model = true;
+    return TT_OK;
+}
+
+int32_t tt_tree_set_branch_lengths(tt_tree *T, const int32_t *nodes, const double *t, int32_t k) {
+    if (!T || (k > 0 && (!nodes || !t))) return fail(TT_EINVAL, "NULL argument");
+    if (k <= 0) return TT_OK;
+    for (int i = 0; i < k; ++i)
+        if (nodes[i] < 0 || nodes[i] >= T->n_nodes) return fail(TT_EINVAL, "node %d out of range", nodes[i]);
+    if (T->have_model) {
+        int32_t rc = check_range(T, t, k);
+        if (rc) return rc;
+    }
+    TT_HIP(hipSetDevice(T->eng->device));
+    for (int i = 0; i < k; ++i) T->t[nodes[i]] = t[i];
+    TT_HIP(hipMemcpyAsync(T->d_t, T->t.data(), T->n_nodes * sizeof(double), hipMemcpyHostToDevice, T->stream));
+    if (T->have_model) {
+        if (k > T->tmp_cap) {
+            int32_t rc;
+            if ((rc = T->alloc((void **)&T->d_tmp_t, (size_t)k * sizeof(double)))) return rc;
+            if ((rc = T->alloc((void **)&T->d_tmp_br, (size_t)k * sizeof(int32_t)))) return rc;
+            T->tmp_cap = k;
+        }
+        TT_HIP(hipMemcpyAsync(T->d_tmp_t, t, k * sizeof(double), hipMemcpyHostToDevice, T->stream));
+        TT_HIP(hipMemcpyAsync(T->d_tmp_br, nodes, k * sizeof(int32_t), hipMemcpyHostToDevice, T->stream));
+        int32_t rc = refresh_expqt(T, T->d_tmp_br, T->d_tmp_t, k, T->stream);
+        if (rc) return rc;
+    }
+    TT_HIP(hipStreamSynchronize(T->stream));
+    return TT_OK;
+}
+
+int32_t tt_marginal(tt_tree *tree, uint32_t flags, double *logL, void *stream) {
+    return run_pass(tree, 1, flags, logL, stream);
+}
+
+int32_t tt_loglik(tt_tree *tree, double *logL, void *stream) { return run_pass(tree, 3, 0, logL, stream); }
+
+int32_t tt_joint(tt_tree *tree, uint32_t flags, double *logL, void *stream) {
+    return run_pass(tree, 2, flags, logL, stream);
+}
+
+int32_t tt_update_branch(tt_tree *tree, int32_t node, double t, double *logL, void *stream) {
+    int32_t rc = tt_tree_set_branch_lengths(tree, &node, &t, 1);
+    if (rc) return rc;
+    return run_pass(tree, 3, 0, logL, stream);
+}
+
+int32_t tt_get_sequence_lh(tt_tree *T, double *out) {
+    if (!T || !out) return fail(TT_EINVAL, "NULL argument");
+    if (!T->last_pass) return fail(TT_ESTATE, "no pass has run on this tree");
+    TT_HIP(hipSetDevice(T->eng->device));
+    TT_HIP(hipDeviceSynchronize());
+    TT_HIP(hipMemcpy(out, T->d.seqlh, (size_t)T->P * sizeof(double), hipMemcpyDeviceToHost));
+    return TT_OK;
+}
+
+int32_t tt_get_states(tt_tree *T, uint8_t *out) {
+    if (!T || !out) return fail(TT_EINVAL, "NULL argument");
+    if (!T->last_pass) return fail(TT_ESTATE, "no pass has run on this tree");
+    TT_HIP(hipSetDevice(T->eng->device));
+    TT_HIP(hipDeviceSynchronize());
+    TT_HIP(hipMemcpy(out, T->d.states, (size_t)T->n_nodes * T->P, hipMemcpyDeviceToHost));
+    return TT_OK;
+}
+
+int32_t tt_get_profiles(tt_tree *T, int32_t which, float *out) {
+    if (!T || !out) return fail(TT_EINVAL, "NULL argument");
+    if (!T->last_pass) return fail(TT_ESTATE, "no pass has run on this tree");
+    TT_HIP(hipSetDevice(T->eng->device));
+    TT_HIP(hipDeviceSynchronize());
+    const int n = T->n, P = T->P;
+    const size_t row = (size_t)P * n;
+    if (which == TT_PROFILE_MARGINAL) {
+        TT_HIP(hipMemcpy(out, T->d.prof, (size_t)T->n_nodes * row * sizeof(float), hipMemcpyDeviceToHost));
+        return TT_OK;
+    }
+    if (which != TT_PROFILE_SUBTREE) return fail(TT_EINVAL, "unknown profile kind %d", which);
+    std::vector<float> sub((size_t)T->n_int * row);
+    TT_HIP(hipMemcpy(sub.data(), T->d.sub, sub.size() * sizeof(float), hipMemcpyDeviceToHost));
+    std::memset(out, 0, (size_t)T->n_nodes * row * sizeof(float));
+    for (int r = 0; r < T->n_int; ++r) {
+        const float *s = sub.data() + (size_t)r * row;
+        float *o = out + (size_t)T->int_node[r] * row;
+        for (int p = 0; p < P; ++p) {
+            double z = 0.0;
+            for (int j = 0; j < n; ++j) z += s[(size_t)p * n + j];
+            for (int j = 0; j < n; ++j) o[(size_t)p * n + j] = (float)(s[(size_t)p * n + j] / z);
+        }
+    }
+    return TT_OK;
+}
+
+int32_t tt_get_device_buffers(tt_tree *T, float **profiles, uint8_t **states, double **sequence_lh) {
+    if (!T) return fail(TT_EINVAL, "tree is NULL");
+    if (profiles) *profiles = T->d.prof;
+    if (states) *states = T->d.states;
+    if (sequence_lh) *sequence_lh = T->d.seqlh;
+    return TT_OK;
+}
+
+int32_t tt_propagate_profile(const double *in_dev, int32_t P, int32_t n, const double *Q, int32_t transpose,
+                             int32_t return_log, double *out_dev, void *stream) {
+    if (!in_dev || !out_dev || !Q) return fail(TT_EINVAL, "NULL argument");
+    if (P < 0 || n < 1 || n > TT_MAX_STATES) return fail(TT_EINVAL, "bad shape P=%d n=%d", P, n);
+    if (P == 0) return TT_OK;
+    hipStream_t s = (hipStream_t)stream;
+    double *dQ = nullptr;
+    TT_HIP(hipMallocAsync((void **)&dQ, (size_t)n * n * sizeof(double), s));
+    TT_HIP(hipMemcpyAsync(dQ, Q, (size_t)n * n * sizeof(double), hipMemcpyHostToDevice, s));
+    TT_HIP(ttk::propagate(in_dev, P, n, dQ, transpose, return_log, out_dev, s));
+    TT_HIP(hipFreeAsync(dQ, s));
+    TT_HIP(hipStreamSynchronize(s));
+    return TT_OK;
+}
+
+int32_t tt_prob_t_profiles(const double *pp_dev, const double *pc_dev, const double *mult_dev, int32_t P, int32_t n,
+                           const double *Q, int32_t K, int32_t gap_index, double *out, void *stream) {
+    if (!pp_dev || !pc_dev || !mult_dev || !Q || !out) return fail(TT_EINVAL, "NULL argument");
+    if (P < 1 || n < 1 || n > TT_MAX_STATES || K < 1) return fail(TT_EINVAL, "bad shape P=%d n=%d K=%d", P, n, K);
+    if (gap_index >= n) return fail(TT_EINVAL, "gap_index %d >= n %d", gap_index, n);
+    hipStream_t s = (hipStream_t)stream;
+    const int nblk = ttk::root_blocks(P);
+    double *dQ = nullptr, *dB = nullptr;
+    TT_HIP(hipMallocAsync((void **)&dQ, (size_t)K * n * n * sizeof(double), s));
+    TT_HIP(hipMallocAsync((void **)&dB, (size_t)K * nblk * sizeof(double), s));
+    TT_HIP(hipMemcpyAsync(dQ, Q, (size_t)K * n * n * sizeof(double), hipMemcpyHostToDevice, s));
+    TT_HIP(ttk::prob_t_profiles(pp_dev, pc_dev, mult_dev, P, n, dQ, K, gap_index, dB, nblk, s));
+    std::vector<double> hb((size_t)K * nblk);
+    TT_HIP(hipMemcpyAsync(hb.data(), dB, hb.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    TT_HIP(hipFreeAsync(dQ, s));
+    TT_HIP(hipFreeAsync(dB, s));
+    TT_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < K; ++k) {
+        double acc = 0.0;
+        for (int b = 0; b < nblk; ++b) acc += hb[(size_t)k * nblk + b];
+        out[k] = acc;
+    }
+    return TT_OK;
+}
+
+int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, double *logL,
+                          void *stream) {
+    if (!eng || (k > 0 && !trees)) return fail(TT_EINVAL, "NULL argument");
+    for (int i = 0; i < k; ++i) {
+        int32_t rc = run_pass(trees[i], 1, flags, logL ? logL + i : nullptr, stream);
+        if (rc) return rc;
+    }
+    return TT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,657 @@
+// HIP kernels of the pruning engine for gfx950 (CDNA4, wave64).
+//
+// Layout in HBM (pattern-major inside each node):
+//   sub  [n_int][P][n]   float  subtree partials, normalised so the largest
+//                               entry is exactly 1.0 (reference:
+//                               node.marginal_subtree_LH, treeanc.py:877)
+//   prof [n_nodes][P][n] float  marginal profiles, sum = 1 (treeanc.py:909-917)
+//   Lx   [n_int][P][n]   float  joint messages, largest entry exactly 0
+//   Cx   [n_nodes][P][n] uint8  joint argmax table (treeanc.py:996-1011)
+//   part [R][P]          double per-chunk-group sums of log normalisers
+//
+// Precision scheme: rows are stored in FP32, arithmetic is FP64.  Because a
+// stored row's dominant entry is exactly 1 (0 for the joint), the only FP32
+// rounding that reaches the likelihood is on entries weighted by their
+// (small) posterior, and the per-node log normalisers are accumulated in
+// FP64.  This keeps the total log-likelihood within ~1e-9 relative of the
+// FP64 reference, where naive FP32 storage drifts by ~1e-6.
+//
+// Work decomposition of the level kernels: a wave owns 64 consecutive
+// patterns of one node (lane = pattern), so everything that depends on the
+// node (child list, P(t) of the branch) is wave-uniform and is read through
+// the scalar unit.
+#include "tt_common.h"
+
+#include <float.h>
+
+namespace {
+
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+template <int N>
+__device__ __forceinline__ void load_row(const float *__restrict__ src, double (&r)[N]) {
+    if constexpr (N % 4 == 0) {
+        const float4 *s4 = reinterpret_cast<const float4 *>(src);
+#pragma unroll
+        for (int q = 0; q < N / 4; ++q) {
+            float4 x = s4[q];
+            r[4 * q + 0] = x.x;
+            r[4 * q + 1] = x.y;
+            r[4 * q + 2] = x.z;
+            r[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) r[i] = src[i];
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void store_row(float *__restrict__ dst, const float (&r)[N]) {
+    if constexpr (N % 4 == 0) {
+        float4 *d4 = reinterpret_cast<float4 *>(dst);
+#pragma unroll
+        for (int q = 0; q < N / 4; ++q) d4[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) dst[i] = r[i];
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void store_bytes(uint8_t *__restrict__ dst, const int (&c)[N]) {
+    if constexpr (N == 4) {
+        uint32_t w = (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) | ((uint32_t)c[3] << 24);
+        *reinterpret_cast<uint32_t *>(dst) = w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) dst[i] = (uint8_t)c[i];
+    }
+}
+
+// Child message m_j = sum_i r_i P[i][j] (GTR.propagate_profile, gtr.py:992-995).
+template <int N>
+__device__ __forceinline__ void child_message(const double (&r)[N], const double *__restrict__ M, double (&m)[N]) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) m[j] = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) m[j] = fma(r[i], M[i * N + j], m[j]);
+    }
+}
+
+// Subtree row of a child: stored row of an internal child, profile-map row
+// of a tip's character (seq2prof, seq_utils.py:207-229).
+template <int N>
+__device__ __forceinline__ void child_row(const TreeDev &T, int cref, int p, double (&r)[N]) {
+    if (cref < 0) {
+        const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
+        const float *lr = T.lut + code * N;
+#pragma unroll
+        for (int i = 0; i < N; ++i) r[i] = lr[i];
+    } else {
+        load_row<N>(T.sub + ((size_t)cref * T.P + p) * N, r);
+    }
+}
+
+// Postorder step for internal row v and pattern p (treeanc.py:857-878): the
+// product over children of (child row @ P(t)), renormalised so the largest
+// entry is 1.  Returns the log normaliser (reference: normalize_profile
+// offset, seq_utils.py:296-307).
+template <int N>
+__device__ __forceinline__ double post_node(const TreeDev &T, int v, int p) {
+    double prod[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) prod[j] = 1.0;
+    double lscale = 0.0;
+    const int e0 = T.child_ptr[v], e1 = T.child_ptr[v + 1];
+    for (int e = e0; e < e1; ++e) {
+        const int cref = T.child_ref[e];
+        const double *__restrict__ M = T.P64 + (size_t)T.child_node[e] * (N * N);
+        double r[N], m[N];
+        child_row<N>(T, cref, p, r);
+        child_message<N>(r, M, m);
+#pragma unroll
+        for (int j = 0; j < N; ++j) prod[j] *= m[j];
+        if (e1 - e0 > 2) {  // polytomies: keep the running product away from underflow
+            double mx = prod[0];
+#pragma unroll
+            for (int j = 1; j < N; ++j) mx = fmax(mx, prod[j]);
+            if (mx < 0x1p-500 && mx > 0.0) {
+#pragma unroll
+                for (int j = 0; j < N; ++j) prod[j] *= 0x1p+500;
+                lscale -= 500.0 * 0.6931471805599453;
+            }
+        }
+    }
+    double mx = prod[0];
+    int jm = 0;
+#pragma unroll
+    for (int j = 1; j < N; ++j)
+        if (prod[j] > mx) {
+            mx = prod[j];
+            jm = j;
+        }
+    const double inv = 1.0 / mx;
+    float out[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : (float)(prod[j] * inv);
+    store_row<N>(T.sub + ((size_t)v * T.P + p) * N, out);
+    return log(mx) + lscale;
+}
+
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_post_marginal(TreeDev T, LevelDev L, int n_ptiles) {
+    __shared__ double s_acc[TT_BLOCK / TT_WAVE][TT_WAVE];
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    const int pt = blockIdx.x % n_ptiles, cg = blockIdx.x / n_ptiles;
+    const int p = pt * TT_WAVE + lane;
+    const bool act = p < T.P;
+    const int chunk = cg * (TT_BLOCK / TT_WAVE) + w;
+    double acc = 0.0;
+    if (chunk < L.n_chunks) {
+        const int kb = chunk * L.chunk;
+        const int ke = min(kb + L.chunk, L.n_nodes);
+        for (int k = kb; k < ke; ++k) {
+            const int v = wave_uniform(L.nodes[k]);
+            if (act) {
+                const double o = post_node<N>(T, v, p);
+                acc += o;
+                if (T.offs) T.offs[(size_t)v * T.P + p] = o;
+            }
+        }
+    }
+    s_acc[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && act) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < TT_BLOCK / TT_WAVE; ++q) s += s_acc[q][lane];
+        T.part[(size_t)(L.row_off + cg) * T.P + p] = s;
+    }
+}
+
+// Sums row slices of the partial-normaliser matrix: part2[s][p] = sum of
+// part[r][p] over r in slice s.
+__global__ __launch_bounds__(TT_BLOCK) void k_rowsum(TreeDev T, int R, int S, int G, int n_ptiles) {
+    const int lane = threadIdx.x & (TT_WAVE - 1);
+    const int task = blockIdx.x * (TT_BLOCK / TT_WAVE) + threadIdx.x / TT_WAVE;
+    const int pt = task % n_ptiles, s = task / n_ptiles;
+    const int p = pt * TT_WAVE + lane;
+    if (s >= S || p >= T.P) return;
+    const int r0 = s * G, r1 = min(R, r0 + G);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int r = r0;
+    for (; r + 3 < r1; r += 4) {
+        a0 += T.part[(size_t)r * T.P + p];
+        a1 += T.part[(size_t)(r + 1) * T.P + p];
+        a2 += T.part[(size_t)(r + 2) * T.P + p];
+        a3 += T.part[(size_t)(r + 3) * T.P + p];
+    }
+    for (; r < r1; ++r) a0 += T.part[(size_t)r * T.P + p];
+    T.part2[(size_t)s * T.P + p] = (a0 + a1) + (a2 + a3);
+}
+
+// Deterministic block reduction of one double per thread (wave shuffles,
+// then LDS across the block's waves).  Result valid in thread 0.
+__device__ __forceinline__ double block_sum(double x) {
+    __shared__ double s_w[TT_BLOCK / TT_WAVE];
+#pragma unroll
+    for (int off = TT_WAVE / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, TT_WAVE);
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    if (lane == 0) s_w[w] = x;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + TT_WAVE - 1) / TT_WAVE;
+        for (int q = 0; q < nw; ++q) r += s_w[q];
+    }
+    return r;
+}
+
+// Root: profile = normalise(pi * subtree) (treeanc.py:817-824), per-pattern
+// logL = root log-norm + all node log normalisers (treeanc.py:826-827),
+// block partials of sum(mult * logL) (treeanc.py:828).
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_root_marginal(TreeDev T, int S) {
+    const int p = blockIdx.x * TT_BLOCK + threadIdx.x;
+    double contrib = 0.0;
+    if (p < T.P) {
+        double r[N], x[N];
+        load_row<N>(T.sub + ((size_t)T.root_row * T.P + p) * N, r);
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            x[j] = T.pi[j] * r[j];
+            s += x[j];
+        }
+        double lp = log(s);
+        for (int k = 0; k < S; ++k) lp += T.part2[(size_t)k * T.P + p];
+        T.seqlh[p] = lp;
+        const double inv = 1.0 / s;
+        float out[N];
+        int jm = 0;
+        double mx = x[0];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            out[j] = (float)(x[j] * inv);
+            if (x[j] > mx) {
+                mx = x[j];
+                jm = j;
+            }
+        }
+        store_row<N>(T.prof + ((size_t)T.root * T.P + p) * N, out);
+        T.states[(size_t)T.root * T.P + p] = (uint8_t)jm;
+        contrib = T.mult[p] * lp;
+    }
+    const double b = block_sum(contrib);
+    if (threadIdx.x == 0) T.bpart[blockIdx.x] = b;
+}
+
+__global__ __launch_bounds__(TT_BLOCK) void k_final_sum(TreeDev T, int nblocks) {
+    double x = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += TT_BLOCK) x += T.bpart[i];
+    const double b = block_sum(x);
+    if (threadIdx.x == 0) T.total[0] = b;
+}
+
+// Preorder step for node c (treeanc.py:889-930): outgroup message by the
+// reference's division trick, normalise(log max(TINY, parent profile) -
+// log(child message)), then evolve it down the branch and multiply with the
+// subtree row.
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_pre_marginal(TreeDev T, LevelDev L, int n_ptiles) {
+    const int lane = threadIdx.x & (TT_WAVE - 1);
+    const int task = blockIdx.x * (TT_BLOCK / TT_WAVE) + threadIdx.x / TT_WAVE;
+    if (task >= L.n_nodes * n_ptiles) return;
+    const int k = task / n_ptiles, pt = task % n_ptiles;
+    const int p = pt * TT_WAVE + lane;
+    if (p >= T.P) return;
+    const int c = wave_uniform(L.nodes[k]);
+    const int par = T.parent[c];
+    const int ir = T.int_row[c];
+    const double *__restrict__ M = T.P64 + (size_t)c * (N * N);
+    double pp[N], sub[N], m[N], q[N], msg[N];
+    {
+        const float *src = T.prof + ((size_t)par * T.P + p) * N;
+        if constexpr (N % 4 == 0) {
+            const float4 *s4 = reinterpret_cast<const float4 *>(src);
+#pragma unroll
+            for (int u = 0; u < N / 4; ++u) {
+                float4 x = s4[u];
+                pp[4 * u] = x.x;
+                pp[4 * u + 1] = x.y;
+                pp[4 * u + 2] = x.z;
+                pp[4 * u + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) pp[i] = src[i];
+        }
+    }
+    child_row<N>(T, ir >= 0 ? ir : -(T.tip_row[c] + 1), p, sub);
+    child_message<N>(sub, M, m);
+    double qs = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        q[j] = fmax(TT_TINY, pp[j]) / m[j];
+        qs += q[j];
+    }
+    const double qi = 1.0 / qs;
+#pragma unroll
+    for (int j = 0; j < N; ++j) q[j] *= qi;
+    // GTR.evolve (gtr.py:1023-1025): msg_i = sum_j outgroup_j P[i][j]
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double a = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) a = fma(q[j], M[i * N + j], a);
+        msg[i] = a * sub[i];
+    }
+    double xs = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) xs += msg[i];
+    const double xi = 1.0 / xs;
+    float out[N];
+    int jm = 0;
+    double mx = msg[0];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        out[i] = (float)(msg[i] * xi);
+        if (msg[i] > mx) {
+            mx = msg[i];
+            jm = i;
+        }
+    }
+    store_row<N>(T.prof + ((size_t)c * T.P + p) * N, out);
+    T.states[(size_t)c * T.P + p] = (uint8_t)jm;
+}
+
+// Joint message of a tip child (treeanc.py:970-978, 997-1011):
+// Lx[j] = max_i log P[i][j] + log max(profile_i, TINY); Cx[j] = first argmax.
+template <int N>
+__device__ __forceinline__ void tip_joint(const TreeDev &T, int cref, int cnode, int p, double (&msg)[N],
+                                          int store_cx) {
+    const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
+    const double *ll = T.loglut + code * N;
+    const double *__restrict__ LP = T.logP64 + (size_t)cnode * (N * N);
+    double tl[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) tl[i] = ll[i];
+    int cx[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        double best = LP[j] + tl[0];
+        int bi = 0;
+#pragma unroll
+        for (int i = 1; i < N; ++i) {
+            const double val = LP[i * N + j] + tl[i];
+            if (val > best) {
+                best = val;
+                bi = i;
+            }
+        }
+        msg[j] += best;
+        cx[j] = bi;
+    }
+    if (store_cx) store_bytes<N>(T.Cx + ((size_t)cnode * T.P + p) * N, cx);
+}
+
+template <int N>
+__device__ __forceinline__ void joint_children(const TreeDev &T, int v, int p, double (&msg)[N], int store_tip_cx) {
+    const int e0 = T.child_ptr[v], e1 = T.child_ptr[v + 1];
+    for (int e = e0; e < e1; ++e) {
+        const int cref = T.child_ref[e];
+        if (cref < 0) {
+            tip_joint<N>(T, cref, T.child_node[e], p, msg, store_tip_cx);
+        } else {
+            double r[N];
+            load_row<N>(T.Lx + ((size_t)cref * T.P + p) * N, r);
+#pragma unroll
+            for (int i = 0; i < N; ++i) msg[i] += r[i];
+        }
+    }
+}
+
+// Joint postorder step for internal non-root row v with node id vn.
+template <int N>
+__device__ __forceinline__ double post_joint_node(const TreeDev &T, int v, int vn, int p, int store_tip_cx) {
+    double msg[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) msg[i] = 0.0;
+    joint_children<N>(T, v, p, msg, store_tip_cx);
+    const double *__restrict__ LP = T.logP64 + (size_t)vn * (N * N);
+    double best[N];
+    int cx[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        double b = LP[j] + msg[0];
+        int bi = 0;
+#pragma unroll
+        for (int i = 1; i < N; ++i) {
+            const double val = LP[i * N + j] + msg[i];
+            if (val > b) {
+                b = val;
+                bi = i;
+            }
+        }
+        best[j] = b;
+        cx[j] = bi;
+    }
+    double off = best[0];
+#pragma unroll
+    for (int j = 1; j < N; ++j) off = fmax(off, best[j]);
+    float out[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = (float)(best[j] - off);
+    store_row<N>(T.Lx + ((size_t)v * T.P + p) * N, out);
+    store_bytes<N>(T.Cx + ((size_t)vn * T.P + p) * N, cx);
+    return off;
+}
+
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_post_joint(TreeDev T, LevelDev L, int n_ptiles, int store_tip_cx) {
+    __shared__ double s_acc[TT_BLOCK / TT_WAVE][TT_WAVE];
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    const int pt = blockIdx.x % n_ptiles, cg = blockIdx.x / n_ptiles;
+    const int p = pt * TT_WAVE + lane;
+    const bool act = p < T.P;
+    const int chunk = cg * (TT_BLOCK / TT_WAVE) + w;
+    double acc = 0.0;
+    if (chunk < L.n_chunks) {
+        const int kb = chunk * L.chunk;
+        const int ke = min(kb + L.chunk, L.n_nodes);
+        for (int k = kb; k < ke; ++k) {
+            const int v = wave_uniform(L.nodes[k]);
+            const int vn = wave_uniform(T.int_node[v]);
+            if (act) acc += post_joint_node<N>(T, v, vn, p, store_tip_cx);
+        }
+    }
+    s_acc[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && act) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < TT_BLOCK / TT_WAVE; ++q) s += s_acc[q][lane];
+        T.part[(size_t)(L.row_off + cg) * T.P + p] = s;
+    }
+}
+
+// Joint root (treeanc.py:1015-1037): Lx_root = sum of children + log pi,
+// state = first argmax, sequence_LH = Lx_root[state] + normalisers.
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_root_joint(TreeDev T, int S, int store_tip_cx) {
+    const int p = blockIdx.x * TT_BLOCK + threadIdx.x;
+    double contrib = 0.0;
+    if (p < T.P) {
+        double msg[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) msg[i] = 0.0;
+        joint_children<N>(T, T.root_row, p, msg, store_tip_cx);
+        int jm = 0;
+        double mx = -DBL_MAX;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            msg[i] += log(T.pi[i]);
+            if (msg[i] > mx) {
+                mx = msg[i];
+                jm = i;
+            }
+        }
+        double lp = mx;
+        for (int k = 0; k < S; ++k) lp += T.part2[(size_t)k * T.P + p];
+        T.seqlh[p] = lp;
+        T.states[(size_t)T.root * T.P + p] = (uint8_t)jm;
+        contrib = T.mult[p] * lp;
+    }
+    const double b = block_sum(contrib);
+    if (threadIdx.x == 0) T.bpart[blockIdx.x] = b;
+}
+
+// Joint backtrack (treeanc.py:1048-1063): state = Cx[state of parent].
+__global__ __launch_bounds__(TT_BLOCK) void k_backtrack(TreeDev T, LevelDev L, int n_ptiles) {
+    const int lane = threadIdx.x & (TT_WAVE - 1);
+    const int task = blockIdx.x * (TT_BLOCK / TT_WAVE) + threadIdx.x / TT_WAVE;
+    if (task >= L.n_nodes * n_ptiles) return;
+    const int k = task / n_ptiles, pt = task % n_ptiles;
+    const int p = pt * TT_WAVE + lane;
+    if (p >= T.P) return;
+    const int c = wave_uniform(L.nodes[k]);
+    const int par = T.parent[c];
+    const int ps = T.states[(size_t)par * T.P + p];
+    T.states[(size_t)c * T.P + p] = T.Cx[((size_t)c * T.P + p) * T.n + ps];
+}
+
+__global__ void k_fill_states(TreeDev T, const int32_t *nodes, int n_nodes, uint8_t value) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n_nodes * T.P) return;
+    const int k = (int)(i / T.P), p = (int)(i % T.P);
+    T.states[(size_t)nodes[k] * T.P + p] = value;
+}
+
+// P(t) for a list of branches (gtr.py:1027-1067): v diag(exp(mu t lambda))
+// v_inv, clipped at 0, plus log(max(TINY, P)) for the joint pass
+// (treeanc.py:970).
+__global__ void k_expqt(int n, const double *t, const int32_t *branches, int k, const double *eig,
+                        const double *v, const double *vinv, double mu, double *P64, double *logP64) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nn = n * n;
+    if (gid >= k * nn) return;
+    const int b = gid / nn, ij = gid % nn, i = ij / n, j = ij % n;
+    const double tb = t[b];
+    double acc = 0.0;
+    for (int q = 0; q < n; ++q) acc += v[i * n + q] * (exp(mu * tb * eig[q]) * vinv[q * n + j]);
+    const double P = fmax(0.0, acc);
+    const size_t o = (size_t)branches[b] * nn + ij;
+    P64[o] = P;
+    logP64[o] = log(fmax(TT_TINY, P));
+}
+
+// out = in @ Q (transpose=0, GTR.propagate_profile) or in @ Q^T (transpose=1,
+// GTR.evolve); optional log.
+__global__ void k_propagate(const double *in, int P, int n, const double *Q, int transpose, int ret_log,
+                            double *out) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (size_t)P * n) return;
+    const size_t a = gid / n;
+    const int j = (int)(gid % n);
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) acc += in[a * n + i] * (transpose ? Q[j * n + i] : Q[i * n + j]);
+    out[gid] = ret_log ? log(acc) : acc;
+}
+
+// GTR.prob_t_profiles (gtr.py:950-961) for K matrices; one block partial
+// per (k, block).
+__global__ __launch_bounds__(TT_BLOCK) void k_prob_t_profiles(const double *pp, const double *pc,
+                                                              const double *mult, int P, int n, const double *Q,
+                                                              int gap, double *block_out, int nblk) {
+    const int a = blockIdx.x * TT_BLOCK + threadIdx.x;
+    const int kq = blockIdx.y;
+    const double *Qk = Q + (size_t)kq * n * n;
+    double contrib = 0.0;
+    if (a < P) {
+        double res = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double inner = 0.0;
+            for (int j = 0; j < n; ++j) inner += Qk[i * n + j] * pp[(size_t)a * n + j];
+            res += pc[(size_t)a * n + i] * inner;
+        }
+        double lg = mult[a] * log(res + 1e-24);
+        if (gap >= 0) lg *= (1.0 - pp[(size_t)a * n + gap]) * (1.0 - pc[(size_t)a * n + gap]);
+        contrib = lg;
+    }
+    const double b = block_sum(contrib);
+    if (threadIdx.x == 0) block_out[(size_t)kq * nblk + blockIdx.x] = b;
+}
+
+inline int n_ptiles(int P) { return (P + TT_WAVE - 1) / TT_WAVE; }
+inline unsigned grid_for_tasks(long tasks) { return (unsigned)((tasks + 3) / 4); }
+
+}  // namespace
+
+#define TT_DISPATCH(n, CALL)               \
+    switch (n) {                           \
+        case 2: { constexpr int N_ = 2; CALL; } break;   \
+        case 3: { constexpr int N_ = 3; CALL; } break;   \
+        case 4: { constexpr int N_ = 4; CALL; } break;   \
+        case 5: { constexpr int N_ = 5; CALL; } break;   \
+        case 20: { constexpr int N_ = 20; CALL; } break; \
+        case 21: { constexpr int N_ = 21; CALL; } break; \
+        case 22: { constexpr int N_ = 22; CALL; } break; \
+        default: return hipErrorInvalidValue;            \
+    }
+
+namespace ttk {
+
+int root_blocks(int P) { return (P + TT_BLOCK - 1) / TT_BLOCK; }
+
+hipError_t expqt(int n, const double *t, const int32_t *branches, int k, const double *eig, const double *v,
+                 const double *vinv, double mu, double *P64, double *logP64, hipStream_t s) {
+    if (k <= 0) return hipSuccess;
+    const int total = k * n * n;
+    hipLaunchKernelGGL(k_expqt, dim3((total + 255) / 256), dim3(256), 0, s, n, t, branches, k, eig, v, vinv, mu,
+                       P64, logP64);
+    return hipGetLastError();
+}
+
+hipError_t post_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s) {
+    const int npt = n_ptiles(T.P);
+    const int groups = (L.n_chunks + 3) / 4;
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_post_marginal<N_>, dim3(groups * npt), dim3(TT_BLOCK), 0, s, T, L, npt));
+    return hipGetLastError();
+}
+
+hipError_t rowsum(const TreeDev &T, int R, int S, hipStream_t s) {
+    if (S <= 0) return hipSuccess;
+    const int npt = n_ptiles(T.P);
+    const int G = (R + S - 1) / S;
+    hipLaunchKernelGGL(k_rowsum, dim3(grid_for_tasks((long)S * npt)), dim3(TT_BLOCK), 0, s, T, R, S, G, npt);
+    return hipGetLastError();
+}
+
+hipError_t root_marginal(const TreeDev &T, int S, hipStream_t s) {
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_root_marginal<N_>, dim3(root_blocks(T.P)), dim3(TT_BLOCK), 0, s, T, S));
+    return hipGetLastError();
+}
+
+hipError_t final_sum(const TreeDev &T, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TT_BLOCK), 0, s, T, nblocks);
+    return hipGetLastError();
+}
+
+hipError_t pre_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s) {
+    if (L.n_nodes <= 0) return hipSuccess;
+    const int npt = n_ptiles(T.P);
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_pre_marginal<N_>, dim3(grid_for_tasks((long)L.n_nodes * npt)),
+                                        dim3(TT_BLOCK), 0, s, T, L, npt));
+    return hipGetLastError();
+}
+
+hipError_t root_joint(const TreeDev &T, int S, int store_tip_cx, hipStream_t s) {
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_root_joint<N_>, dim3(root_blocks(T.P)), dim3(TT_BLOCK), 0, s, T, S,
+                                        store_tip_cx));
+    return hipGetLastError();
+}
+
+hipError_t backtrack(const TreeDev &T, const LevelDev &L, hipStream_t s) {
+    if (L.n_nodes <= 0) return hipSuccess;
+    const int npt = n_ptiles(T.P);
+    hipLaunchKernelGGL(k_backtrack, dim3(grid_for_tasks((long)L.n_nodes * npt)), dim3(TT_BLOCK), 0, s, T, L, npt);
+    return hipGetLastError();
+}
+
+hipError_t fill_states(const TreeDev &T, const int32_t *nodes, int n_nodes, uint8_t value, hipStream_t s) {
+    if (n_nodes <= 0) return hipSuccess;
+    const size_t total = (size_t)n_nodes * T.P;
+    hipLaunchKernelGGL(k_fill_states, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, T, nodes, n_nodes,
+                       value);
+    return hipGetLastError();
+}
+
+hipError_t propagate(const double *in, int P, int n, const double *Q, int transpose, int ret_log, double *out,
+                     hipStream_t s) {
+    const size_t total = (size_t)P * n;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_propagate, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, P, n, Q, transpose,
+                       ret_log, out);
+    return hipGetLastError();
+}
+
+hipError_t prob_t_profiles(const double *pp, const double *pc, const double *mult, int P, int n, const double *Q,
+                           int K, int gap, double *block_out, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(k_prob_t_profiles, dim3(nblk, K), dim3(TT_BLOCK), 0, s, pp, pc, mult, P, n, Q, gap,
+                       block_out, nblk);
+    return hipGetLastError();
+}
+
+}  // namespace ttk
+
+hipError_t ttk::post_joint(const TreeDev &T, const LevelDev &L, int store_tip_cx, hipStream_t s) {
+    if (L.n_nodes <= 0) return hipSuccess;
+    const int npt = n_ptiles(T.P);
+    const int groups = (L.n_chunks + 3) / 4;
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_post_joint<N_>, dim3(groups * npt), dim3(TT_BLOCK), 0, s, T, L, npt,
+                                        store_tip_cx));
+    return hipGetLastError();
+}

@@ -1,0 +1,167 @@
+"""Python handles over the C ABI: ``Engine`` (device + HBM budget) and
+``DeviceTree`` (one resident tree with its compressed alignment)."""
+import ctypes as C
+
+import numpy as np
+
+from . import _native
+
+N_CODES = 256
+MISSING_CODE = 0  # character code of tips without a sequence (all-ones profile)
+
+
+def code_lut(profile_map, n_states):
+    """(256, n) profile of every byte; characters outside the profile map are
+    treated as missing information (seq_utils.extend_profile semantics)."""
+    lut = np.ones((N_CODES, n_states), dtype=np.float32)
+    for c, v in profile_map.items():
+        if isinstance(c, str) and len(c) == 1 and ord(c) < N_CODES:
+            lut[ord(c)] = v
+    lut[MISSING_CODE] = 1.0
+    return lut
+
+
+class Engine:
+    """One device and its HBM admission budget (0 = unlimited)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self.lib = _native.load()
+        h = C.c_void_p()
+        _native.check(self.lib.tt_engine_create(int(device), int(hbm_budget), C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def usage(self):
+        used, budget = C.c_uint64(), C.c_uint64()
+        _native.check(self.lib.tt_engine_usage(self.handle, C.byref(used), C.byref(budget)))
+        return used.value, budget.value
+
+    def close(self):
+        if self.handle:
+            self.lib.tt_engine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_engine = None
+
+
+def default_engine():
+    global _default_engine
+    if _default_engine is None:
+        _default_engine = Engine(0, 0)
+    return _default_engine
+
+
+class DeviceTree:
+    """A tree resident on the device.
+
+    parent : (n_nodes,) int32, -1 at the root; children are visited in
+             ascending node index.
+    t      : (n_nodes,) effective branch lengths.
+    tip_codes : (n_tips, P) uint8 character codes, rows in ascending node
+             index of the tips.
+    """
+
+    def __init__(self, engine, parent, t, tip_codes, lut, multiplicity, pi, eigvals, v, v_inv, mu):
+        self.engine = engine
+        self.lib = engine.lib
+        self.parent = _native.as_c(parent, np.int32)
+        self.n_nodes = len(self.parent)
+        self.P = int(tip_codes.shape[1])
+        self.n = int(lut.shape[1])
+        t = _native.as_c(t, np.float64)
+        codes = _native.as_c(tip_codes, np.uint8)
+        lut = _native.as_c(lut, np.float32)
+        mult = _native.as_c(multiplicity, np.float64)
+        pi = _native.as_c(pi, np.float64)
+        d = _native.tt_tree_desc(
+            self.n_nodes, self.n, self.P, lut.shape[0],
+            _native.ptr(self.parent, C.c_int32), _native.ptr(t, C.c_double), _native.ptr(codes, C.c_uint8),
+            _native.ptr(lut, C.c_float), _native.ptr(mult, C.c_double), _native.ptr(pi, C.c_double))
+        h = C.c_void_p()
+        _native.check(self.lib.tt_tree_create(engine.handle, C.byref(d), C.byref(h)))
+        self.handle = h
+        self.set_model(eigvals, v, v_inv, mu)
+
+    def set_model(self, eigvals, v, v_inv, mu):
+        e = _native.as_c(eigvals, np.float64)
+        vv = _native.as_c(v, np.float64)
+        vi = _native.as_c(v_inv, np.float64)
+        _native.check(self.lib.tt_tree_set_model(self.handle, _native.ptr(e, C.c_double), _native.ptr(vv, C.c_double),
+                                                 _native.ptr(vi, C.c_double), float(mu)))
+
+    def set_branch_lengths(self, nodes, t):
+        nodes = _native.as_c(nodes, np.int32)
+        t = _native.as_c(t, np.float64)
+        _native.check(self.lib.tt_tree_set_branch_lengths(self.handle, _native.ptr(nodes, C.c_int32),
+                                                          _native.ptr(t, C.c_double), len(nodes)))
+
+    def info(self):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_uint64()
+        _native.check(self.lib.tt_tree_info(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        return {"levels_post": a.value, "levels_pre": b.value, "device_bytes": c.value}
+
+    def marginal(self, reconstruct_tips=False, stream=None, want_logL=True):
+        out = C.c_double(np.nan)
+        flags = _native.TT_RECONSTRUCT_TIPS if reconstruct_tips else 0
+        _native.check(self.lib.tt_marginal(self.handle, flags, C.byref(out) if want_logL else None, stream))
+        return out.value
+
+    def joint(self, reconstruct_tips=False, stream=None, want_logL=True):
+        out = C.c_double(np.nan)
+        flags = _native.TT_RECONSTRUCT_TIPS if reconstruct_tips else 0
+        _native.check(self.lib.tt_joint(self.handle, flags, C.byref(out) if want_logL else None, stream))
+        return out.value
+
+    def loglik(self, stream=None, want_logL=True):
+        out = C.c_double(np.nan)
+        _native.check(self.lib.tt_loglik(self.handle, C.byref(out) if want_logL else None, stream))
+        return out.value
+
+    def update_branch(self, node, t, stream=None):
+        out = C.c_double(np.nan)
+        _native.check(self.lib.tt_update_branch(self.handle, int(node), float(t), C.byref(out), stream))
+        return out.value
+
+    def sequence_lh(self):
+        out = np.empty(self.P, dtype=np.float64)
+        _native.check(self.lib.tt_get_sequence_lh(self.handle, _native.ptr(out, C.c_double)))
+        return out
+
+    def states(self):
+        out = np.empty((self.n_nodes, self.P), dtype=np.uint8)
+        _native.check(self.lib.tt_get_states(self.handle, _native.ptr(out, C.c_uint8)))
+        return out
+
+    def profiles(self, which=_native.TT_PROFILE_MARGINAL):
+        out = np.empty((self.n_nodes, self.P, self.n), dtype=np.float32)
+        _native.check(self.lib.tt_get_profiles(self.handle, int(which), _native.ptr(out, C.c_float)))
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.tt_tree_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_marginal(engine, trees, reconstruct_tips=False, stream=None, want_logL=True):
+    """Marginal reconstruction of several resident trees in one persistent launch."""
+    k = len(trees)
+    arr = (C.c_void_p * k)(*[t.handle.value for t in trees])
+    out = np.full(k, np.nan)
+    flags = _native.TT_RECONSTRUCT_TIPS if reconstruct_tips else 0
+    _native.check(engine.lib.tt_batch_marginal(engine.handle, arr, k, flags,
+                                               _native.ptr(out, C.c_double) if want_logL else None, stream))
+    return out

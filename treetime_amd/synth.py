@@ -1,0 +1,144 @@
+"""Synthetic workloads: Kingman-coalescent trees, random GTR models and
+alignments simulated down the tree.
+
+Tree: with k lineages the waiting time is Exp(k(k-1)/2) and a uniformly
+random pair merges; the ultrametric tree is scaled to the requested
+root-to-tip depth.  Sequences: the root is drawn from pi and every child
+site from column P(t)[:, parent state] (the sampling of SeqGen.evolve,
+treetime/seqgen.py:38-67).  Tip noise replaces characters by 'N' / '-'.
+"""
+import numpy as np
+
+from .gtr import GTR
+from .seq_utils import alphabets
+
+
+def kingman_tree(n_tips, depth, rng):
+    """Returns (parent, branch_length) with tips 0..n-1 and internal nodes
+    n..2n-2 (root last); ultrametric with root height ``depth``."""
+    n_nodes = 2 * n_tips - 1
+    parent = np.full(n_nodes, -1, dtype=np.int32)
+    height = np.zeros(n_nodes)
+    lineages = list(range(n_tips))
+    t = 0.0
+    nxt = n_tips
+    while len(lineages) > 1:
+        k = len(lineages)
+        t += rng.exponential(2.0 / (k * (k - 1)))
+        i, j = rng.choice(k, size=2, replace=False)
+        a, b = lineages[i], lineages[j]
+        for x in sorted((i, j), reverse=True):
+            lineages[x] = lineages[-1]
+            lineages.pop()
+        parent[a] = parent[b] = nxt
+        height[nxt] = t
+        lineages.append(nxt)
+        nxt += 1
+    height *= depth / height[-1]
+    bl = np.zeros(n_nodes)
+    nonroot = parent >= 0
+    bl[nonroot] = height[parent[nonroot]] - height[nonroot]
+    return parent, bl
+
+
+def preorder_renumber(parent, bl):
+    """Renumber nodes in preorder (root 0, children in input order) so that
+    the children of each node have ascending indices."""
+    n = len(parent)
+    kids = [[] for _ in range(n)]
+    root = int(np.where(parent < 0)[0][0])
+    for i, p in enumerate(parent):
+        if p >= 0:
+            kids[p].append(i)
+    order = []
+    stack = [root]
+    while stack:
+        v = stack.pop()
+        order.append(v)
+        stack.extend(reversed(kids[v]))
+    new = np.empty(n, dtype=np.int64)
+    new[np.array(order)] = np.arange(n)
+    parent2 = np.full(n, -1, dtype=np.int32)
+    nonroot = parent >= 0
+    parent2[new[nonroot]] = new[parent[nonroot]]
+    bl2 = np.empty(n)
+    bl2[new] = bl
+    return parent2, bl2, new
+
+
+def random_gtr(n_states, rng, alpha, sigma):
+    """pi ~ Dirichlet(alpha), exchangeabilities ~ LogNormal(0, sigma),
+    W symmetric and scaled so that pi^T W pi = 1 (rate 1 per unit length)."""
+    pi = rng.dirichlet(alpha * np.ones(n_states))
+    W = np.zeros((n_states, n_states))
+    iu = np.triu_indices(n_states, 1)
+    W[iu] = rng.lognormal(0.0, sigma, size=len(iu[0]))
+    W = W + W.T
+    W /= float(pi @ W @ pi)
+    return pi, W
+
+
+def simulate(parent, bl, gtr, L, rng):
+    """(n_nodes, L) uint8 state indices of a sequence evolved down the tree."""
+    n = len(parent)
+    kids = [[] for _ in range(n)]
+    root = int(np.where(parent < 0)[0][0])
+    for i, p in enumerate(parent):
+        if p >= 0:
+            kids[p].append(i)
+    states = np.empty((n, L), dtype=np.uint8)
+    cum_pi = np.cumsum(gtr.Pi)
+    states[root] = np.minimum(np.searchsorted(cum_pi, rng.random(L) * cum_pi[-1], side="right"),
+                              gtr.n_states - 1)
+    stack = list(kids[root])
+    while stack:
+        v = stack.pop()
+        stack.extend(kids[v])
+        P = gtr.expQt(bl[v])
+        ps = states[parent[v]]
+        u = rng.random(L, dtype=np.float32)
+        child = ps.copy()
+        move = np.nonzero(u >= P[ps, ps])[0]
+        if move.size:
+            Q = P.copy()
+            np.fill_diagonal(Q, 0.0)
+            cq = np.cumsum(Q, axis=0)
+            pj = ps[move]
+            u2 = rng.random(move.size) * cq[-1, pj]
+            new = (u2[:, None] >= cq[:, pj].T).sum(axis=1)
+            child[move] = np.minimum(new, gtr.n_states - 1)
+        states[v] = child
+    return states
+
+
+def make_job(n_tips, L, depth, n_states, seed, p_ambig=0.0, p_gap=0.0):
+    """One synthetic job.  Returns a dict with the tree (preorder numbering),
+    the GTR model, and the tip alignment as a (n_tips, L) char matrix whose
+    rows follow ascending tip node index."""
+    rng = np.random.default_rng(seed)
+    parent, bl = kingman_tree(n_tips, depth, rng)
+    parent, bl, _ = preorder_renumber(parent, bl)
+    if n_states == 4:
+        pi, W = random_gtr(4, rng, 5.0, 0.5)
+        gtr = GTR.custom(mu=1.0, pi=pi, W=W, alphabet="nuc_nogap")
+        alpha = alphabets["nuc_nogap"]
+        amb = "N"
+    elif n_states == 20:
+        pi, W = random_gtr(20, rng, 2.0, 1.0)
+        gtr = GTR.custom(mu=1.0, pi=pi, W=W, alphabet="aa_nogap")
+        alpha = alphabets["aa_nogap"]
+        amb = "X"
+    else:
+        raise ValueError("n_states must be 4 or 20")
+    states = simulate(parent, bl, gtr, L, rng)
+    is_tip = np.ones(len(parent), dtype=bool)
+    is_tip[parent[parent >= 0]] = False
+    tips = np.where(is_tip)[0]
+    chars = alpha.astype("S1")[states[tips]]
+    if p_ambig or p_gap:
+        u = rng.random(chars.shape)
+        chars[u < p_ambig] = amb.encode()
+        chars[(u >= p_ambig) & (u < p_ambig + p_gap)] = b"-"
+    names = ["t%05d" % i for i in range(len(tips))]
+    return {"parent": parent, "branch_length": bl, "tips": tips, "names": names, "gtr": gtr,
+            "chars": chars, "ambiguous": amb, "L": L, "seed": seed, "true_states": states}
