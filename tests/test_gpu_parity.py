@@ -161,3 +161,33 @@ def test_budget_rejection():
     used, budget = eng.usage()
     assert used == 0 and budget == 1 << 20
     eng.close()
+
+
+@pytest.mark.parametrize("tips_flag", [False, True])
+def test_batch_matches_plain(gpu_engine, tips_flag):
+    """One persistent launch over uneven trees equals tt_marginal per tree:
+    identical profiles and states, logL within 1e-12 (summation order only)."""
+    from treetime_amd.engine import batch_marginal
+
+    say(f"[batch] 6 uneven trees, reconstruct_tips={tips_flag}")
+    specs = [(50, 1000, 1), (300, 200, 2), (7, 65, 3), (120, 3000, 4), (3, 10, 5), (900, 640, 6)]
+    jobs = [small_job(n, L, 4, seed) for n, L, seed in specs]
+    trees = [device_tree(gpu_engine, prep) for _, prep, _ in jobs]
+    plain = [t.marginal(reconstruct_tips=tips_flag) for t in trees]
+    plain_prof = [t.profiles() for t in trees]
+    plain_states = [t.states() for t in trees]
+    plain_slh = [t.sequence_lh() for t in trees]
+    for rep in range(2):
+        got = batch_marginal(gpu_engine, trees, reconstruct_tips=tips_flag)
+        for i, t in enumerate(trees):
+            assert abs(got[i] - plain[i]) <= 1e-12 * abs(plain[i]), (i, got[i], plain[i])
+            assert np.array_equal(t.states(), plain_states[i])
+            assert np.array_equal(t.profiles(), plain_prof[i])
+            assert np.allclose(t.sequence_lh(), plain_slh[i], rtol=0, atol=1e-12)
+    # oracle check of one member
+    job, prep, alphabet = jobs[3]
+    ta, nodes, tips = build_oracle(prep, alphabet, job["gtr"])
+    ref = ta.marginal()
+    assert abs(got[3] - ref) <= LOGL_RTOL * abs(ref)
+    for t in trees:
+        t.close()

@@ -65,10 +65,36 @@ bool g_use_graphs() {
 
 }  // namespace
 
+struct BatchPlan;
+
 struct tt_engine {
     int device = 0;
     uint64_t budget = 0;
     std::atomic<uint64_t> used{0};
+    std::mutex mu;
+    std::vector<BatchPlan *> plans;
+};
+
+// A recorded batched launch over a fixed list of trees.
+struct BatchPlan {
+    tt_engine *eng = nullptr;
+    std::vector<tt_tree *> trees;
+    uint32_t flags = 0;
+    std::vector<BatchItem> items;
+    BatchTreeDev *d_trees = nullptr;
+    BatchItem *d_items = nullptr;
+    int *d_queue = nullptr;
+    double *d_item_out = nullptr;
+    size_t bytes = 0;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    ~BatchPlan() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (void *q : {(void *)d_trees, (void *)d_items, (void *)d_queue, (void *)d_item_out})
+            if (q) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+        eng->used -= bytes;
+    }
 };
 
 struct PostLv {
@@ -93,6 +119,7 @@ struct tt_tree {
 
     TreeDev d{};
     int32_t *d_post_nodes = nullptr, *d_pre_nodes = nullptr, *d_tip_nodes = nullptr, *d_all_branches = nullptr;
+    int32_t *d_post_off = nullptr, *d_pre_off = nullptr, *d_pre_nint = nullptr;
     double *d_t = nullptr, *d_eig = nullptr, *d_v = nullptr, *d_vinv = nullptr;
     double *d_tmp_t = nullptr;
     int32_t *d_tmp_br = nullptr;
@@ -153,6 +180,10 @@ int32_t tt_engine_create(int32_t device, uint64_t hbm_budget_bytes, tt_engine **
 }
 
 int32_t tt_engine_destroy(tt_engine *eng) {
+    if (eng) {
+        (void)hipDeviceSynchronize();
+        for (BatchPlan *p : eng->plans) delete p;
+    }
     delete eng;
     return TT_OK;
 }
@@ -461,6 +492,20 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     UP(T->d_tip_nodes, T->tip_nodes.data(), T->tip_nodes.size());
     UP(T->d_all_branches, all.data(), nn);
     UP(T->d_t, T->t.data(), nn);
+    {
+        std::vector<int32_t> post_off, pre_off, pre_nint;
+        for (auto &L : T->post_lv) post_off.push_back(L.off);
+        post_off.push_back((int32_t)T->post_nodes.size());
+        for (auto &L : T->pre_lv) {
+            pre_off.push_back(L.off);
+            pre_nint.push_back(L.n_int);
+        }
+        pre_off.push_back((int32_t)T->pre_nodes.size());
+        if (pre_nint.empty()) pre_nint.push_back(0);
+        UP(T->d_post_off, post_off.data(), post_off.size());
+        UP(T->d_pre_off, pre_off.data(), pre_off.size());
+        UP(T->d_pre_nint, pre_nint.data(), pre_nint.size());
+    }
 #undef UP
     d.n_nodes = nn;
     d.n_tips = T->n_tips;
@@ -522,7 +567,18 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
 int32_t tt_tree_destroy(tt_tree *tree) {
     if (tree) {
         (void)hipSetDevice(tree->eng->device);
-        (void)hipStreamSynchronize(tree->stream);
+        (void)hipDeviceSynchronize();
+        tt_engine *eng = tree->eng;
+        std::lock_guard<std::mutex> g(eng->mu);
+        auto &pl = eng->plans;
+        for (size_t i = 0; i < pl.size();) {
+            if (std::find(pl[i]->trees.begin(), pl[i]->trees.end(), tree) != pl[i]->trees.end()) {
+                delete pl[i];
+                pl.erase(pl.begin() + i);
+            } else {
+                ++i;
+            }
+        }
     }
     delete tree;
     return TT_OK;
@@ -696,9 +752,108 @@ int32_t tt_prob_t_profiles(const double *pp_dev, const double *pc_dev, const dou
 int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, double *logL,
                           void *stream) {
     if (!eng || (k > 0 && !trees)) return fail(TT_EINVAL, "NULL argument");
+    if (k <= 0) return TT_OK;
+    const int n = trees[0] ? trees[0]->n : 0;
     for (int i = 0; i < k; ++i) {
-        int32_t rc = run_pass(trees[i], 1, flags, logL ? logL + i : nullptr, stream);
-        if (rc) return rc;
+        tt_tree *T = trees[i];
+        if (!T) return fail(TT_EINVAL, "trees[%d] is NULL", i);
+        if (T->eng != eng) return fail(TT_EINVAL, "trees[%d] belongs to another engine", i);
+        if (T->n != n) return fail(TT_EINVAL, "trees[%d] has %d states, trees[0] has %d", i, T->n, n);
+        if (!T->have_model) return fail(TT_ESTATE, "trees[%d]: tt_tree_set_model has not been called", i);
+    }
+    TT_HIP(hipSetDevice(eng->device));
+    const uint32_t fl = flags & TT_RECONSTRUCT_TIPS;
+    std::vector<tt_tree *> key(trees, trees + k);
+    BatchPlan *plan = nullptr;
+    {
+        std::lock_guard<std::mutex> g(eng->mu);
+        for (BatchPlan *p : eng->plans)
+            if (p->flags == fl && p->trees == key) plan = p;
+    }
+    if (!plan) {
+        auto *P = new BatchPlan;
+        P->eng = eng;
+        P->trees = key;
+        P->flags = fl;
+        // items, longest tree first (LPT), tiles in ascending order within a tree
+        std::vector<std::pair<long, int>> order;
+        for (int i = 0; i < k; ++i) {
+            long pre = (long)trees[i]->pre_nodes.size();
+            if (!fl) pre -= trees[i]->n_tips;
+            order.push_back({(long)trees[i]->n_int + pre, i});
+        }
+        std::stable_sort(order.begin(), order.end(), [](auto &a, auto &b) { return a.first > b.first; });
+        for (auto &o : order) {
+            const int tiles = (trees[o.second]->P + TT_WAVE - 1) / TT_WAVE;
+            for (int t = 0; t < tiles; ++t) P->items.push_back({o.second, t});
+        }
+        std::vector<BatchTreeDev> bt(k);
+        for (int i = 0; i < k; ++i) {
+            tt_tree *T = trees[i];
+            bt[i].T = T->d;
+            bt[i].post_nodes = T->d_post_nodes;
+            bt[i].post_off = T->d_post_off;
+            bt[i].pre_nodes = T->d_pre_nodes;
+            bt[i].pre_off = T->d_pre_off;
+            bt[i].pre_nint = T->d_pre_nint;
+            bt[i].tip_nodes = T->d_tip_nodes;
+            bt[i].H = (int32_t)T->post_lv.size();
+            bt[i].D = (int32_t)T->pre_lv.size();
+        }
+        const size_t nb_trees = bt.size() * sizeof(BatchTreeDev), nb_items = P->items.size() * sizeof(BatchItem),
+                     nb_out = P->items.size() * sizeof(double);
+        P->bytes = nb_trees + nb_items + nb_out + 256;
+        if (eng->budget && eng->used.load() + P->bytes > eng->budget) {
+            P->bytes = 0;
+            delete P;
+            return fail(TT_ENOMEM_BUDGET, "HBM budget exceeded by the batch plan");
+        }
+        eng->used += P->bytes;
+        hipError_t e = hipMalloc((void **)&P->d_trees, nb_trees);
+        if (e == hipSuccess) e = hipMalloc((void **)&P->d_items, nb_items);
+        if (e == hipSuccess) e = hipMalloc((void **)&P->d_queue, 256);
+        if (e == hipSuccess) e = hipMalloc((void **)&P->d_item_out, nb_out);
+        if (e == hipSuccess) e = hipMemcpy(P->d_trees, bt.data(), nb_trees, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(P->d_items, P->items.data(), nb_items, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete P;
+            return fail(TT_EDEVICE, "batch plan set-up: %s", hipGetErrorString(e));
+        }
+        if (g_use_graphs()) {
+            hipGraph_t g = nullptr;
+            e = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal);
+            hipError_t e2 = hipSuccess;
+            if (e == hipSuccess)
+                e2 = ttk::batch_marginal(n, P->d_trees, P->d_items, (int)P->items.size(), P->d_queue,
+                                         P->d_item_out, fl ? 1 : 0, P->stream);
+            if (e == hipSuccess) e = hipStreamEndCapture(P->stream, &g);
+            if (e == hipSuccess) e = e2;
+            if (e == hipSuccess) e = hipGraphInstantiate(&P->graph, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                delete P;
+                return fail(TT_EDEVICE, "batch graph capture: %s", hipGetErrorString(e));
+            }
+        }
+        std::lock_guard<std::mutex> g(eng->mu);
+        eng->plans.push_back(P);
+        plan = P;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : plan->stream;
+    if (plan->graph) {
+        TT_HIP(hipGraphLaunch(plan->graph, s));
+    } else {
+        TT_HIP(ttk::batch_marginal(n, plan->d_trees, plan->d_items, (int)plan->items.size(), plan->d_queue,
+                                   plan->d_item_out, fl ? 1 : 0, s));
+    }
+    for (int i = 0; i < k; ++i) trees[i]->last_pass = 1;
+    if (logL) {
+        std::vector<double> out(plan->items.size());
+        TT_HIP(hipMemcpyAsync(out.data(), plan->d_item_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        TT_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < k; ++i) logL[i] = 0.0;
+        for (size_t j = 0; j < out.size(); ++j) logL[plan->items[j].tree] += out[j];
     }
     return TT_OK;
 }

@@ -1,0 +1,117 @@
+// Persistent batched marginal reconstruction of many trees in one launch.
+//
+// Work queue: items (tree, 64-pattern tile), sorted on the host longest
+// first.  A persistent block of 4 waves pops an item with one atomic, then
+// walks that tile through the tree's postorder levels (heights), the root and
+// the preorder levels (depths); the 4 waves split the nodes of each level and
+// meet at a block barrier between levels.  Patterns are independent, so an
+// item never waits for another block: there are no cross-block dependencies
+// and no spin-waits, and the critical path of the largest tree is about
+// (internal nodes / 4) node steps per pass.
+//
+// Per (node, pattern) arithmetic is the same device code as the level
+// kernels (tt_device.h), so profiles and states equal tt_marginal's; the
+// per-pattern log normalisers are summed in registers instead of in partial
+// rows.
+#include "tt_device.h"
+
+namespace {
+using namespace ttd;
+
+constexpr int kWaves = TT_BLOCK / TT_WAVE;
+
+__global__ void k_reset_queue(int *queue) { *queue = 0; }
+
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal(const BatchTreeDev *__restrict__ trees,
+                                                             const BatchItem *__restrict__ items, int n_items,
+                                                             int *queue, double *item_out, int tips) {
+    __shared__ int s_item;
+    __shared__ double s_acc[kWaves][TT_WAVE];
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= n_items) break;
+        const BatchItem item = items[it];
+        const BatchTreeDev &B = trees[item.tree];
+        const TreeDev &T = B.T;
+        const int p = item.tile * TT_WAVE + lane;
+        const bool act = p < T.P;
+        // postorder: heights 1..H, the last level is the root
+        double acc = 0.0;
+        for (int h = 0; h < B.H; ++h) {
+            const int kb = B.post_off[h], ke = B.post_off[h + 1];
+            for (int k = kb + w; k < ke; k += kWaves) {
+                const int v = wave_uniform(B.post_nodes[k]);
+                if (act) acc += post_node<N>(T, v, p);
+            }
+            __syncthreads();
+        }
+        // root: per-pattern logL from the 4 waves' normaliser sums
+        s_acc[w][lane] = acc;
+        __syncthreads();
+        double contrib = 0.0;
+        if (w == 0 && act) {
+            double off = 0.0;
+#pragma unroll
+            for (int q = 0; q < kWaves; ++q) off += s_acc[q][lane];
+            contrib = root_step<N>(T, p, off);
+        }
+        const double tot = block_sum(contrib);
+        if (threadIdx.x == 0) item_out[it] = tot;
+        if (!tips && act)
+            for (int k = w; k < T.n_tips; k += kWaves) T.states[(size_t)B.tip_nodes[k] * T.P + p] = 255;
+        __syncthreads();
+        // preorder: depths 1..D (tips only when reconstructing them)
+        for (int d = 0; d < B.D; ++d) {
+            const int kb = B.pre_off[d];
+            const int ke = tips ? B.pre_off[d + 1] : kb + B.pre_nint[d];
+            for (int k = kb + w; k < ke; k += kWaves) {
+                const int c = wave_uniform(B.pre_nodes[k]);
+                if (act) pre_node<N>(T, c, p);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int N>
+int persistent_grid() {
+    static int grid = [] {
+        int dev = 0, cus = 0, per_cu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N>, TT_BLOCK, 0);
+        return std::max(1, cus * std::max(1, per_cu));
+    }();
+    return grid;
+}
+
+}  // namespace
+
+#define TT_BATCH_DISPATCH(n, CALL)                        \
+    switch (n) {                                          \
+        case 2: { constexpr int N_ = 2; CALL; } break;   \
+        case 3: { constexpr int N_ = 3; CALL; } break;   \
+        case 4: { constexpr int N_ = 4; CALL; } break;   \
+        case 5: { constexpr int N_ = 5; CALL; } break;   \
+        case 20: { constexpr int N_ = 20; CALL; } break; \
+        case 21: { constexpr int N_ = 21; CALL; } break; \
+        case 22: { constexpr int N_ = 22; CALL; } break; \
+        default: return hipErrorInvalidValue;            \
+    }
+
+hipError_t ttk::batch_marginal(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
+                               double *item_out, int tips, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, queue);
+    TT_BATCH_DISPATCH(n, {
+        const int grid = std::min(persistent_grid<N_>(), n_items);
+        hipLaunchKernelGGL(k_batch_marginal<N_>, dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items, queue,
+                           item_out, tips);
+    });
+    return hipGetLastError();
+}

@@ -68,3 +68,26 @@ hipError_t prob_t_profiles(const double *pp, const double *pc, const double *mul
                            int K, int gap, double *block_out, int nblk, hipStream_t s);
 int root_blocks(int P);
 }  // namespace ttk
+
+// A tree as seen by the persistent batched kernel: the tree buffers plus its
+// level schedule.
+struct BatchTreeDev {
+    TreeDev T;
+    const int32_t *post_nodes;  // internal rows, grouped by height 1..H (root last)
+    const int32_t *post_off;    // [H + 1]
+    const int32_t *pre_nodes;   // non-root node ids grouped by depth 1..D, internal first
+    const int32_t *pre_off;     // [D + 1]
+    const int32_t *pre_nint;    // [D] internal nodes per depth
+    const int32_t *tip_nodes;   // [n_tips]
+    int32_t H, D;
+};
+
+// One work item of the batched launch: 64 patterns (tile) of one tree.
+struct BatchItem {
+    int32_t tree, tile;
+};
+
+namespace ttk {
+hipError_t batch_marginal(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
+                          double *item_out, int tips, hipStream_t s);
+}  // namespace ttk
