@@ -1,0 +1,254 @@
+#!/usr/bin/env python3
+"""Throughput of marginal ancestral reconstruction on one MI355X.
+
+Default workload: 256 independent nucleotide jobs (tips log-uniform in
+[50, 4000], sites log-uniform in [1000, 30000], depth 0.02, 1% N / 0.5% gaps,
+own seed per job), all resident in HBM, reconstructed in ONE persistent
+launch per step (tt_batch_marginal).  A step = one full marginal pass
+(postorder, root likelihood, preorder with argmax states) over all jobs.
+
+Metric: pattern x branch updates per second, U = P * (nodes - 1) per job.
+Printed beside it: the plain path (one tt_marginal graph per tree, trees in
+sequence), the roofline of the batched kernel from the algorithmic bytes per
+update, and the FP64 numpy oracle timed on the host as the CPU baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mix|large-5000|...]
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD datasheet
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="mix")
+    ap.add_argument("--jobs", type=int, default=256)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workers", type=int, default=0)
+    return ap.parse_args()
+
+
+def n_workers(req):
+    if req:
+        return req
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def gen_preps(specs, workers):
+    from treetime_amd.synth import prepared_job
+
+    if workers <= 1 or len(specs) == 1:
+        return [prepared_job(s) for s in specs]
+    # big jobs first so the pool's tail is short
+    order = sorted(range(len(specs)), key=lambda i: -specs[i]["n_tips"] * specs[i]["L"])
+    ctx = mp.get_context("fork")  # no GPU has been touched yet in this process
+    with ctx.Pool(workers) as pool:
+        res = pool.map(prepared_job, [specs[i] for i in order], chunksize=1)
+    out = [None] * len(specs)
+    for i, r in zip(order, res):
+        out[i] = r
+    return out
+
+
+def tree_arrays(prep):
+    parent = prep["parent"]
+    is_tip = np.ones(len(parent), dtype=bool)
+    is_tip[parent[parent >= 0]] = False
+    return parent, is_tip
+
+
+def updates(prep):
+    return prep["tip_codes"].shape[1] * (len(prep["parent"]) - 1)
+
+
+def algorithmic_bytes(prep):
+    """Minimal compulsory HBM bytes of one FP32 marginal pass: per pattern,
+    18n+7 per internal-child branch, 2n+3 per tip branch, plus 4 (weights)
+    and 4n (root) per pattern."""
+    n = prep["n_states"]
+    parent, is_tip = tree_arrays(prep)
+    nonroot = parent >= 0
+    n_tip_br = int((is_tip & nonroot).sum())
+    n_int_br = int((~is_tip & nonroot).sum())
+    P = prep["tip_codes"].shape[1]
+    return P * (n_int_br * (18 * n + 7) + n_tip_br * (2 * n + 3) + 4 + 4 * n)
+
+
+def oracle_marginal(prep):
+    """FP64 oracle (CPU baseline / checker) on one prepared job."""
+    from oracle import fp64 as O
+
+    parent, is_tip = tree_arrays(prep)
+    names = ["n%d" % i for i in range(len(parent))]
+    root, nodes = O.build_tree(parent, prep["t"], names)
+    alpha = {4: "nuc_nogap", 20: "aa_nogap"}[prep["n_states"]]
+    chars = prep["tip_codes"].view("S1").astype("U1")
+    tips = np.where(is_tip)[0]
+    comp = {names[v]: chars[k] for k, v in enumerate(tips)}
+    g = O.GTRModel(alpha, mu=prep["mu"], pi=prep["pi"], W=prep["W"])
+    ta = O.OracleTreeAnc(root, g, comp, prep["multiplicity"], full_length=1e300)
+    return ta.marginal()
+
+
+def cpu_baseline(preps, seconds):
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    done_u, t0, used = 0, time.perf_counter(), []
+    for i in sorted(range(len(preps)), key=lambda i: updates(preps[i])):
+        oracle_marginal(preps[i])
+        done_u += updates(preps[i])
+        used.append(i)
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    return done_u / dt, dt, used
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described, not built",
+                          "n_gpus": args.gpus}))
+        log("bench.py: only --gpus 1 is supported (one MI355X per box)")
+        return 2
+    from treetime_amd import synth
+
+    if args.workload == "mix":
+        specs = [synth.mix_job_spec(j) for j in range(args.jobs)]
+        wname = f"mix{args.jobs}-nuc-marginal-batched"
+    else:
+        n_tips, L, depth, n_states, seed, pa, pg = synth.WORKLOADS[args.workload]
+        specs = [dict(n_tips=n_tips, L=L, depth=depth, n_states=n_states, seed=seed, p_ambig=pa, p_gap=pg)]
+        wname = args.workload
+    t0 = time.perf_counter()
+    preps = gen_preps(specs, n_workers(args.workers))
+    log(f"[bench] generated {len(preps)} jobs in {time.perf_counter() - t0:.1f}s")
+
+    import torch
+
+    from treetime_amd.engine import Engine, batch_marginal
+    from treetime_amd.prepare import device_tree
+
+    eng = Engine(0, 0)
+    t0 = time.perf_counter()
+    trees = [device_tree(eng, p) for p in preps]
+    torch.cuda.synchronize()
+    log(f"[bench] {len(trees)} trees resident ({eng.usage()[0] / 2**30:.1f} GiB) in {time.perf_counter() - t0:.1f}s")
+    U = sum(updates(p) for p in preps)
+    B = sum(algorithmic_bytes(p) for p in preps)
+    P_total = sum(p["tip_codes"].shape[1] for p in preps)
+
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    # correctness: batched logL equals the plain pass per tree
+    plain_logl = np.array([t.marginal(stream=sp) for t in trees])
+    batch_logl = batch_marginal(eng, trees, stream=sp)
+    rel = float(np.max(np.abs(batch_logl - plain_logl) / np.abs(plain_logl)))
+    log(f"[bench] batched vs plain logL max rel diff {rel:.2e}")
+    small = int(np.argmin([updates(p) for p in preps]))
+    ref = oracle_marginal(preps[small])
+    err_fp64 = abs(plain_logl[small] - ref) / abs(ref)
+    log(f"[bench] job {small}: logL rel err vs FP64 oracle {err_fp64:.2e}")
+
+    # batched: warmup + timed
+    for _ in range(args.warmup):
+        batch_marginal(eng, trees, stream=sp, want_logL=False)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        batch_marginal(eng, trees, stream=sp, want_logL=False)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms = ev0.elapsed_time(ev1) / args.steps
+    log(f"[bench] batched: {ms:.3f} ms/step (events), wall {wall / args.steps * 1e3:.3f} ms/step")
+
+    # plain: one tree per launch sequence, trees in sequence
+    for t in trees:
+        t.marginal(stream=sp, want_logL=False)
+    torch.cuda.synchronize()
+    p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    p0.record(stream)
+    for _ in range(args.plain_steps):
+        for t in trees:
+            t.marginal(stream=sp, want_logL=False)
+    p1.record(stream)
+    torch.cuda.synchronize()
+    ms_plain = p0.elapsed_time(p1) / args.plain_steps
+    log(f"[bench] plain: {ms_plain:.3f} ms/step")
+
+    cpu = None
+    if not args.no_cpu:
+        rate, secs, used = cpu_baseline(preps, args.cpu_seconds)
+        cpu = {"value": rate, "unit": "pattern*branch updates/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 numpy oracle marginal pass on {len(used)} of {len(preps)} jobs (smallest first), "
+                         f"{secs:.1f}s on {platform.processor() or platform.machine()}"}
+        log(f"[bench] cpu baseline {rate:.3e} U/s over {secs:.1f}s")
+
+    value = U / (ms / 1e3)
+    achieved = B / (ms / 1e3) / 1e9
+    out = {
+        "metric": "pattern*branch updates/s (marginal reconstruction)",
+        "value": value,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": wname,
+            "jobs": len(preps),
+            "patterns_total": int(P_total),
+            "updates_per_step": int(U),
+            "storage": "f32 rows, f64 arithmetic and normaliser sums",
+            "plain_ms_per_step": ms_plain,
+            "plain_value": U / (ms_plain / 1e3),
+            "batched_vs_plain": ms_plain / ms,
+            "batched_vs_plain_logL_max_rel_diff": rel,
+            "logL_rel_err_vs_fp64_oracle": err_fp64,
+            "wall_ms_per_step": wall / args.steps * 1e3,
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": int(B)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    for t in trees:
+        t.close()
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

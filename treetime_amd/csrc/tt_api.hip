@@ -328,7 +328,7 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
     const int nb = ttk::root_blocks(T->P);
     if (mode == 1 || mode == 3) {
         for (auto &L : T->post_lv) {
-            LevelDev ld{T->d_post_nodes + L.off, L.cnt, L.chunk, L.nchunks, L.row_off};
+            LevelDev ld{(decltype(LevelDev::nodes))(T->d_post_nodes + L.off), L.cnt, L.chunk, L.nchunks, L.row_off};
             TT_HIP(ttk::post_marginal(D, ld, s));
         }
         const int S = rowsum_slices(T->R_all);
@@ -338,14 +338,14 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
         if (mode == 1) {
             if (!tips) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
             for (auto &L : T->pre_lv) {
-                LevelDev ld{T->d_pre_nodes + L.off, tips ? L.n_all : L.n_int, 1, 0, 0};
+                LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off), tips ? L.n_all : L.n_int, 1, 0, 0};
                 TT_HIP(ttk::pre_marginal(D, ld, s));
             }
         }
     } else {
         for (size_t h = 0; h + 1 < T->post_lv.size(); ++h) {
             auto &L = T->post_lv[h];
-            LevelDev ld{T->d_post_nodes + L.off, L.cnt, L.chunk, L.nchunks, L.row_off};
+            LevelDev ld{(decltype(LevelDev::nodes))(T->d_post_nodes + L.off), L.cnt, L.chunk, L.nchunks, L.row_off};
             TT_HIP(ttk::post_joint(D, ld, tips ? 1 : 0, s));
         }
         const int S = rowsum_slices(T->R_joint);
@@ -354,7 +354,7 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
         TT_HIP(ttk::final_sum(D, nb, s));
         if (!tips) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
         for (auto &L : T->pre_lv) {
-            LevelDev ld{T->d_pre_nodes + L.off, tips ? L.n_all : L.n_int, 1, 0, 0};
+            LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off), tips ? L.n_all : L.n_int, 1, 0, 0};
             TT_HIP(ttk::backtrack(D, ld, s));
         }
     }
@@ -515,18 +515,18 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     d.n_codes = D->n_codes;
     d.root = T->root;
     d.root_row = T->root_row;
-    d.child_ptr = p_child_ptr;
-    d.child_ref = p_child_ref;
-    d.child_node = p_child_node;
-    d.parent = p_parent;
-    d.int_row = p_int_row;
-    d.tip_row = p_tip_row;
-    d.int_node = p_int_node;
-    d.tip_codes = p_codes;
-    d.lut = p_lut;
-    d.loglut = p_loglut;
-    d.mult = p_mult;
-    d.pi = p_pi;
+    d.child_ptr = (decltype(d.child_ptr))p_child_ptr;
+    d.child_ref = (decltype(d.child_ref))p_child_ref;
+    d.child_node = (decltype(d.child_node))p_child_node;
+    d.parent = (decltype(d.parent))p_parent;
+    d.int_row = (decltype(d.int_row))p_int_row;
+    d.tip_row = (decltype(d.tip_row))p_tip_row;
+    d.int_node = (decltype(d.int_node))p_int_node;
+    d.tip_codes = (decltype(d.tip_codes))p_codes;
+    d.lut = (decltype(d.lut))p_lut;
+    d.loglut = (decltype(d.loglut))p_loglut;
+    d.mult = (decltype(d.mult))p_mult;
+    d.pi = (decltype(d.pi))p_pi;
     struct {
         void **p;
         size_t bytes;
@@ -791,12 +791,12 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
         for (int i = 0; i < k; ++i) {
             tt_tree *T = trees[i];
             bt[i].T = T->d;
-            bt[i].post_nodes = T->d_post_nodes;
-            bt[i].post_off = T->d_post_off;
-            bt[i].pre_nodes = T->d_pre_nodes;
-            bt[i].pre_off = T->d_pre_off;
-            bt[i].pre_nint = T->d_pre_nint;
-            bt[i].tip_nodes = T->d_tip_nodes;
+            bt[i].post_nodes = (decltype(bt[i].post_nodes))T->d_post_nodes;
+            bt[i].post_off = (decltype(bt[i].post_off))T->d_post_off;
+            bt[i].pre_nodes = (decltype(bt[i].pre_nodes))T->d_pre_nodes;
+            bt[i].pre_off = (decltype(bt[i].pre_off))T->d_pre_off;
+            bt[i].pre_nint = (decltype(bt[i].pre_nint))T->d_pre_nint;
+            bt[i].tip_nodes = (decltype(bt[i].tip_nodes))T->d_tip_nodes;
             bt[i].H = (int32_t)T->post_lv.size();
             bt[i].D = (int32_t)T->pre_lv.size();
         }

@@ -15,6 +15,8 @@
 // rows.
 #include "tt_device.h"
 
+#include <cstdlib>
+
 namespace {
 using namespace ttd;
 
@@ -23,35 +25,46 @@ constexpr int kWaves = TT_BLOCK / TT_WAVE;
 __global__ void k_reset_queue(int *queue) { *queue = 0; }
 
 template <int N>
-__global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal(const BatchTreeDev *__restrict__ trees,
-                                                             const BatchItem *__restrict__ items, int n_items,
+__global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal(const BatchTreeDev *__restrict__ trees_arg,
+                                                             const BatchItem *__restrict__ items_arg, int n_items,
                                                              int *queue, double *item_out, int tips) {
+    // tree records and items are read-only: scalar loads through the constant address space
+    const BatchTreeDev TT_C *trees = (const BatchTreeDev TT_C *)trees_arg;
+    const BatchItem TT_C *items = (const BatchItem TT_C *)items_arg;
     __shared__ int s_item;
     __shared__ double s_acc[kWaves][TT_WAVE];
+    __shared__ float s_lut[256 * N];
     const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    int cur_tree = -1;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
         __syncthreads();
         const int it = s_item;
-        __syncthreads();
         if (it >= n_items) break;
         const BatchItem item = items[it];
-        const BatchTreeDev &B = trees[item.tree];
-        const TreeDev &T = B.T;
+        const BatchTreeDev TT_C &B = trees[item.tree];
+        const TreeDev TT_C &T = B.T;
+        if (item.tree != cur_tree) {  // stage the tree's profile map in LDS
+            for (int i = threadIdx.x; i < 256 * N; i += TT_BLOCK) s_lut[i] = T.lut[i];
+            cur_tree = item.tree;
+        }
+        __syncthreads();
         const int p = item.tile * TT_WAVE + lane;
         const bool act = p < T.P;
         // postorder: heights 1..H, the last level is the root
-        double acc = 0.0;
+        LogAcc acc;
         for (int h = 0; h < B.H; ++h) {
             const int kb = B.post_off[h], ke = B.post_off[h + 1];
-            for (int k = kb + w; k < ke; k += kWaves) {
-                const int v = wave_uniform(B.post_nodes[k]);
-                if (act) acc += post_node<N>(T, v, p);
+            for (int k = kb + w; k < ke; k += 2 * kWaves) {
+                const bool has1 = k + kWaves < ke;
+                const int v0 = wave_uniform(B.post_nodes[k]);
+                const int v1 = has1 ? wave_uniform(B.post_nodes[k + kWaves]) : v0;
+                if (act) post_pair<N>(T, v0, v1, has1, p, s_lut, acc);
             }
             __syncthreads();
         }
-        // root: per-pattern logL from the 4 waves' normaliser sums
-        s_acc[w][lane] = acc;
+        // root: per-pattern logL from the 4 waves' normaliser products
+        s_acc[w][lane] = act ? acc.log_value() : 0.0;
         __syncthreads();
         double contrib = 0.0;
         if (w == 0 && act) {
@@ -69,9 +82,11 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal(const BatchTreeDev 
         for (int d = 0; d < B.D; ++d) {
             const int kb = B.pre_off[d];
             const int ke = tips ? B.pre_off[d + 1] : kb + B.pre_nint[d];
-            for (int k = kb + w; k < ke; k += kWaves) {
-                const int c = wave_uniform(B.pre_nodes[k]);
-                if (act) pre_node<N>(T, c, p);
+            for (int k = kb + w; k < ke; k += 2 * kWaves) {
+                const bool has1 = k + kWaves < ke;
+                const int c0 = wave_uniform(B.pre_nodes[k]);
+                const int c1 = has1 ? wave_uniform(B.pre_nodes[k + kWaves]) : c0;
+                if (act) pre_pair<N>(T, c0, c1, has1, p, s_lut);
             }
             __syncthreads();
         }
@@ -85,7 +100,9 @@ int persistent_grid() {
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N>, TT_BLOCK, 0);
-        return std::max(1, cus * std::max(1, per_cu));
+        const char *e = getenv("TT_BATCH_BLOCKS_PER_CU");
+        if (e && atoi(e) > 0) per_cu = atoi(e);
+        return std::max(1, cus * std::max(2, per_cu));
     }();
     return grid;
 }

@@ -8,41 +8,57 @@
 #define TT_TINY 1e-12
 #define TT_MAX_STATES 32
 
+// Device buffers of a tree live in global memory; saying so lets the device
+// compiler emit global (not flat) loads for pointers read from memory, e.g.
+// the per-tree records of the batched kernel.  Host code sees plain pointers
+// with the same layout.
+//
+// Read-only tree structure (child lists, P(t) matrices, level lists) is read
+// through the constant address space: uniform loads from it become scalar
+// loads into SGPRs, and branches on them become scalar branches.  Nothing
+// behind a TT_C pointer is written while a pass runs.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define TT_G __attribute__((address_space(1)))
+#define TT_C __attribute__((address_space(4)))
+#else
+#define TT_G
+#define TT_C
+#endif
+
 // Child reference encoding used in the CSR child lists:
 //   ref >= 0  -> internal row (index into the [n_int][P][n] row buffers)
 //   ref <  0  -> tip row (-ref - 1) of the [n_tips][P] code matrix
 struct TreeDev {
     int32_t n_nodes, n_tips, n_int, n, P, n_codes, root, root_row;
-    const int32_t *child_ptr;   // [n_int + 1]
-    const int32_t *child_ref;   // [n_edges]
-    const int32_t *child_node;  // [n_edges] node id of the child (branch index)
-    const int32_t *parent;      // [n_nodes] node ids, -1 for the root
-    const int32_t *int_row;     // [n_nodes] internal row or -1
-    const int32_t *tip_row;     // [n_nodes] tip row or -1
-    const int32_t *int_node;    // [n_int] node id of each internal row
-    const uint8_t *tip_codes;   // [n_tips][P]
-    const float *lut;           // [n_codes][n]   profile of each character code
-    const double *loglut;       // [n_codes][n]   log(max(profile, TINY)) for the joint pass
-    const double *mult;         // [P]
-    const double *pi;           // [n]
-    const double *P64;          // [n_nodes][n][n]  expQt per branch, FP64
-    const double *logP64;       // [n_nodes][n][n]  log(max(TINY, expQt))
-    float *sub;                 // [n_int][P][n]   subtree partials, max entry = 1
-    float *prof;                // [n_nodes][P][n] marginal profiles (sum = 1)
-    uint8_t *states;            // [n_nodes][P]
-    float *Lx;                  // [n_int][P][n]   joint messages, max entry = 0
-    uint8_t *Cx;                // [n_nodes][P][n] joint argmax table
-    double *part;               // [R][P]          per-chunk-group FP64 log-normaliser sums
-    double *part2;              // [S][P]          row-slice sums
-    double *seqlh;              // [P]
-    double *bpart;              // [nblocks]       block partials of sum(mult * seqlh)
-    double *total;              // [1]
-    double *offs;               // [n_int][P] per-node log normalisers (only when kept)
+    const int32_t TT_C *child_ptr;   // [n_int + 1]
+    const int32_t TT_C *child_ref;   // [n_edges]
+    const int32_t TT_C *child_node;  // [n_edges] node id of the child (branch index)
+    const int32_t TT_C *parent;      // [n_nodes] node ids, -1 for the root
+    const int32_t TT_C *int_row;     // [n_nodes] internal row or -1
+    const int32_t TT_C *tip_row;     // [n_nodes] tip row or -1
+    const int32_t TT_C *int_node;    // [n_int] node id of each internal row
+    const uint8_t TT_C *tip_codes;   // [n_tips][P]
+    const float TT_C *lut;           // [n_codes][n]   profile of each character code
+    const double TT_C *loglut;       // [n_codes][n]   log(max(profile, TINY)) for the joint pass
+    const double TT_C *mult;         // [P]
+    const double TT_C *pi;           // [n]
+    const double TT_C *P64;          // [n_nodes][n][n]  expQt per branch, FP64
+    const double TT_C *logP64;       // [n_nodes][n][n]  log(max(TINY, expQt))
+    float TT_G *sub;                 // [n_int][P][n]   subtree partials, max entry = 1
+    float TT_G *prof;                // [n_nodes][P][n] marginal profiles (sum = 1)
+    uint8_t TT_G *states;            // [n_nodes][P]
+    float TT_G *Lx;                  // [n_int][P][n]   joint messages, max entry = 0
+    uint8_t TT_G *Cx;                // [n_nodes][P][n] joint argmax table
+    double TT_G *part;               // [R][P]          per-chunk-group FP64 log-normaliser sums
+    double TT_G *part2;              // [S][P]          row-slice sums
+    double TT_G *seqlh;              // [P]
+    double TT_G *bpart;              // [nblocks]       block partials of sum(mult * seqlh)
+    double TT_G *total;              // [1]
 };
 
 // One level of a level-scheduled pass.
 struct LevelDev {
-    const int32_t *nodes;  // internal rows (post) or node ids (pre)
+    const int32_t TT_C *nodes;  // internal rows (post) or node ids (pre)
     int32_t n_nodes;       // nodes in this launch
     int32_t chunk;         // nodes per wave (post)
     int32_t n_chunks;
@@ -73,12 +89,12 @@ int root_blocks(int P);
 // level schedule.
 struct BatchTreeDev {
     TreeDev T;
-    const int32_t *post_nodes;  // internal rows, grouped by height 1..H (root last)
-    const int32_t *post_off;    // [H + 1]
-    const int32_t *pre_nodes;   // non-root node ids grouped by depth 1..D, internal first
-    const int32_t *pre_off;     // [D + 1]
-    const int32_t *pre_nint;    // [D] internal nodes per depth
-    const int32_t *tip_nodes;   // [n_tips]
+    const int32_t TT_C *post_nodes;  // internal rows, grouped by height 1..H (root last)
+    const int32_t TT_C *post_off;    // [H + 1]
+    const int32_t TT_C *pre_nodes;   // non-root node ids grouped by depth 1..D, internal first
+    const int32_t TT_C *pre_off;     // [D + 1]
+    const int32_t TT_C *pre_nint;    // [D] internal nodes per depth
+    const int32_t TT_C *tip_nodes;   // [n_tips]
     int32_t H, D;
 };
 

@@ -1,6 +1,10 @@
 // Device-side building blocks shared by the level kernels (tt_kernels.hip)
 // and the persistent batched kernel (tt_batch.hip): one (node, pattern) step
 // of each pass, so both launch shapes run identical arithmetic.
+//
+// Memory-level parallelism: a wave handles two nodes per step; for binary
+// nodes every child load of both nodes is issued before any arithmetic
+// (fetch / finish split), so one wave keeps four row loads in flight.
 #pragma once
 #include "tt_common.h"
 
@@ -10,13 +14,13 @@ namespace ttd {
 
 __device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
-template <int N>
-__device__ __forceinline__ void load_row(const float *__restrict__ src, double (&r)[N]) {
+template <int N, class Ptr>
+__device__ __forceinline__ void load_rowf(Ptr src, float (&r)[N]) {
     if constexpr (N % 4 == 0) {
-        const float4 *s4 = reinterpret_cast<const float4 *>(src);
+        const float4 TT_G *s4 = reinterpret_cast<const float4 TT_G *>(src);
 #pragma unroll
         for (int q = 0; q < N / 4; ++q) {
-            float4 x = s4[q];
+            const float4 x = s4[q];
             r[4 * q + 0] = x.x;
             r[4 * q + 1] = x.y;
             r[4 * q + 2] = x.z;
@@ -28,10 +32,18 @@ __device__ __forceinline__ void load_row(const float *__restrict__ src, double (
     }
 }
 
-template <int N>
-__device__ __forceinline__ void store_row(float *__restrict__ dst, const float (&r)[N]) {
+template <int N, class Ptr>
+__device__ __forceinline__ void load_row(Ptr src, double (&r)[N]) {
+    float f[N];
+    load_rowf<N>(src, f);
+#pragma unroll
+    for (int i = 0; i < N; ++i) r[i] = f[i];
+}
+
+template <int N, class Ptr>
+__device__ __forceinline__ void store_row(Ptr dst, const float (&r)[N]) {
     if constexpr (N % 4 == 0) {
-        float4 *d4 = reinterpret_cast<float4 *>(dst);
+        float4 TT_G *d4 = reinterpret_cast<float4 TT_G *>(dst);
 #pragma unroll
         for (int q = 0; q < N / 4; ++q) d4[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
     } else {
@@ -40,11 +52,11 @@ __device__ __forceinline__ void store_row(float *__restrict__ dst, const float (
     }
 }
 
-template <int N>
-__device__ __forceinline__ void store_bytes(uint8_t *__restrict__ dst, const int (&c)[N]) {
+template <int N, class Ptr>
+__device__ __forceinline__ void store_bytes(Ptr dst, const int (&c)[N]) {
     if constexpr (N == 4) {
         uint32_t w = (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) | ((uint32_t)c[3] << 24);
-        *reinterpret_cast<uint32_t *>(dst) = w;
+        *reinterpret_cast<uint32_t TT_G *>(dst) = w;
     } else {
 #pragma unroll
         for (int i = 0; i < N; ++i) dst[i] = (uint8_t)c[i];
@@ -52,8 +64,8 @@ __device__ __forceinline__ void store_bytes(uint8_t *__restrict__ dst, const int
 }
 
 // Child message m_j = sum_i r_i P[i][j] (GTR.propagate_profile, gtr.py:992-995).
-template <int N>
-__device__ __forceinline__ void child_message(const double (&r)[N], const double *__restrict__ M, double (&m)[N]) {
+template <int N, class Ptr>
+__device__ __forceinline__ void child_message(const double (&r)[N], Ptr M, double (&m)[N]) {
 #pragma unroll
     for (int j = 0; j < N; ++j) m[j] = 0.0;
 #pragma unroll
@@ -63,50 +75,32 @@ __device__ __forceinline__ void child_message(const double (&r)[N], const double
     }
 }
 
-// Subtree row of a child: stored row of an internal child, profile-map row
-// of a tip's character (seq2prof, seq_utils.py:207-229).
-template <int N>
-__device__ __forceinline__ void child_row(const TreeDev &T, int cref, int p, double (&r)[N]) {
-    if (cref < 0) {
-        const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
-        const float *lr = T.lut + code * N;
-#pragma unroll
-        for (int i = 0; i < N; ++i) r[i] = lr[i];
-    } else {
-        load_row<N>(T.sub + ((size_t)cref * T.P + p) * N, r);
-    }
-}
-
-// Postorder step for internal row v and pattern p (treeanc.py:857-878): the
-// product over children of (child row @ P(t)), renormalised so the largest
-// entry is 1.  Returns the log normaliser (reference: normalize_profile
-// offset, seq_utils.py:296-307).
-template <int N>
-__device__ __forceinline__ double post_node(const TreeDev &T, int v, int p) {
-    double prod[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j) prod[j] = 1.0;
-    double lscale = 0.0;
-    const int e0 = T.child_ptr[v], e1 = T.child_ptr[v + 1];
-    for (int e = e0; e < e1; ++e) {
-        const int cref = T.child_ref[e];
-        const double *__restrict__ M = T.P64 + (size_t)T.child_node[e] * (N * N);
-        double r[N], m[N];
-        child_row<N>(T, cref, p, r);
-        child_message<N>(r, M, m);
-#pragma unroll
-        for (int j = 0; j < N; ++j) prod[j] *= m[j];
-        if (e1 - e0 > 2) {  // polytomies: keep the running product away from underflow
-            double mx = prod[0];
-#pragma unroll
-            for (int j = 1; j < N; ++j) mx = fmax(mx, prod[j]);
-            if (mx < 0x1p-500 && mx > 0.0) {
-#pragma unroll
-                for (int j = 0; j < N; ++j) prod[j] *= 0x1p+500;
-                lscale -= 500.0 * 0.6931471805599453;
-            }
+// Running product of per-node normalisers, value = m * 2^e.  Summing logs
+// node by node costs one FP64 log per (node, pattern); the product needs one
+// multiply and a rare rescale, and a single log at the end.
+struct LogAcc {
+    double m = 1.0;
+    int e = 0;
+    __device__ __forceinline__ void mul(double x, int ex = 0) {
+        m *= x;
+        e += ex;
+        if (m < 0x1p-500) {
+            m *= 0x1p+500;
+            e -= 500;
+        } else if (m > 0x1p+500) {
+            m *= 0x1p-500;
+            e += 500;
         }
     }
+    __device__ __forceinline__ double log_value() const { return log(m) + (double)e * 0.6931471805599453094; }
+};
+
+// Renormalise a node's product so its largest entry is exactly 1, store the
+// row, fold the normaliser into ``acc`` (normalize_profile offset,
+// seq_utils.py:296-307).
+template <int N>
+__device__ __forceinline__ void finish_post(const TreeDev &T, int v, int p, const double (&prod)[N], int ex,
+                                            LogAcc &acc) {
     double mx = prod[0];
     int jm = 0;
 #pragma unroll
@@ -120,7 +114,106 @@ __device__ __forceinline__ double post_node(const TreeDev &T, int v, int p) {
 #pragma unroll
     for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : (float)(prod[j] * inv);
     store_row<N>(T.sub + ((size_t)v * T.P + p) * N, out);
-    return log(mx) + lscale;
+    acc.mul(mx, ex);
+}
+
+// Postorder step for internal row v, any number of children
+// (treeanc.py:857-878): product over children of (child row @ P(t)).
+template <int N, class LP>
+__device__ __forceinline__ void post_node(const TreeDev &T, int v, int p, LP lut, LogAcc &acc) {
+    double prod[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) prod[j] = 1.0;
+    int ex = 0;
+    const int e0 = wave_uniform(T.child_ptr[v]), e1 = wave_uniform(T.child_ptr[v + 1]);
+    for (int e = e0; e < e1; ++e) {
+        const int cref = wave_uniform(T.child_ref[e]);
+        const int cnode = wave_uniform(T.child_node[e]);
+        double r[N], m[N];
+        if (cref < 0) {
+            const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
+#pragma unroll
+            for (int i = 0; i < N; ++i) r[i] = lut[code * N + i];
+        } else {
+            load_row<N>(T.sub + ((size_t)cref * T.P + p) * N, r);
+        }
+        child_message<N>(r, T.P64 + (size_t)cnode * (N * N), m);
+#pragma unroll
+        for (int j = 0; j < N; ++j) prod[j] *= m[j];
+        if (e1 - e0 > 2) {  // polytomies: keep the running product away from underflow
+            double mx = prod[0];
+#pragma unroll
+            for (int j = 1; j < N; ++j) mx = fmax(mx, prod[j]);
+            if (mx < 0x1p-500 && mx > 0.0) {
+#pragma unroll
+                for (int j = 0; j < N; ++j) prod[j] *= 0x1p+500;
+                ex -= 500;
+            }
+        }
+    }
+    finish_post<N>(T, v, p, prod, ex, acc);
+}
+
+// Binary fast path, split in a fetch phase (global loads only) and a finish
+// phase (arithmetic), so the loads of several nodes overlap.
+template <int N>
+struct BinRaw {
+    float row[2][N];
+    int code[2];
+};
+
+template <int N>
+__device__ __forceinline__ void bin_fetch(const TreeDev &T, int e0, int p, BinRaw<N> &R) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int cref = T.child_ref[e0 + c];
+        if (cref < 0) {
+            R.code[c] = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
+        } else {
+            R.code[c] = 0;
+            load_rowf<N>(T.sub + ((size_t)cref * T.P + p) * N, R.row[c]);
+        }
+    }
+}
+
+template <int N, class LP>
+__device__ __forceinline__ void bin_finish(const TreeDev &T, int v, int e0, int p, const BinRaw<N> &R,
+                                           LP lut, LogAcc &acc) {
+    double prod[N];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int cref = T.child_ref[e0 + c];
+        double r[N], m[N];
+        if (cref < 0) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) r[i] = lut[R.code[c] * N + i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) r[i] = R.row[c][i];
+        }
+        child_message<N>(r, T.P64 + (size_t)T.child_node[e0 + c] * (N * N), m);
+#pragma unroll
+        for (int j = 0; j < N; ++j) prod[j] = c ? prod[j] * m[j] : m[j];
+    }
+    finish_post<N>(T, v, p, prod, 0, acc);
+}
+
+// Two postorder steps (rows v0 and, if has1, v1) for lane pattern p.
+template <int N, class LP>
+__device__ __forceinline__ void post_pair(const TreeDev &T, int v0, int v1, bool has1, int p, LP lut,
+                                          LogAcc &acc) {
+    const int a0 = T.child_ptr[v0], a1 = T.child_ptr[v0 + 1];
+    const int b0 = T.child_ptr[v1], b1 = T.child_ptr[v1 + 1];
+    if (a1 - a0 == 2 && b1 - b0 == 2) {
+        BinRaw<N> RA, RB;  // both fetched unconditionally so they stay in registers
+        bin_fetch<N>(T, a0, p, RA);
+        bin_fetch<N>(T, b0, p, RB);
+        bin_finish<N>(T, v0, a0, p, RA, lut, acc);
+        if (has1) bin_finish<N>(T, v1, b0, p, RB, lut, acc);
+    } else {
+        post_node<N>(T, v0, p, lut, acc);
+        if (has1) post_node<N>(T, v1, p, lut, acc);
+    }
 }
 
 // Root (treeanc.py:817-828) for pattern p: profile = normalise(pi * subtree),
@@ -158,49 +251,53 @@ __device__ __forceinline__ double root_step(const TreeDev &T, int p, double offs
 
 // Preorder step for node c (treeanc.py:889-930): outgroup message by the
 // reference's division trick, normalise(log max(TINY, parent profile) -
-// log(child message)), then evolve it down the branch and multiply with the
-// subtree row.
+// log(child message)), evolved down the branch (GTR.evolve,
+// gtr.py:1023-1025) and multiplied with the subtree row.
 template <int N>
-__device__ __forceinline__ void pre_node(const TreeDev &T, int c, int p) {
+struct PreRaw {
+    float pp[N];
+    float sub[N];
+    int code;
+};
+
+template <int N>
+__device__ __forceinline__ void pre_fetch(const TreeDev &T, int c, int p, PreRaw<N> &R) {
     const int par = T.parent[c];
     const int ir = T.int_row[c];
-    const double *__restrict__ M = T.P64 + (size_t)c * (N * N);
-    double pp[N], sub[N], m[N], q[N], msg[N];
-    {
-        const float *src = T.prof + ((size_t)par * T.P + p) * N;
-        if constexpr (N % 4 == 0) {
-            const float4 *s4 = reinterpret_cast<const float4 *>(src);
-#pragma unroll
-            for (int u = 0; u < N / 4; ++u) {
-                float4 x = s4[u];
-                pp[4 * u] = x.x;
-                pp[4 * u + 1] = x.y;
-                pp[4 * u + 2] = x.z;
-                pp[4 * u + 3] = x.w;
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < N; ++i) pp[i] = src[i];
-        }
+    load_rowf<N>(T.prof + ((size_t)par * T.P + p) * N, R.pp);
+    if (ir >= 0) {
+        R.code = 0;
+        load_rowf<N>(T.sub + ((size_t)ir * T.P + p) * N, R.sub);
+    } else {
+        R.code = T.tip_codes[(size_t)T.tip_row[c] * T.P + p];
     }
-    child_row<N>(T, ir >= 0 ? ir : -(T.tip_row[c] + 1), p, sub);
+}
+
+template <int N, class LP>
+__device__ __forceinline__ void pre_finish(const TreeDev &T, int c, int p, const PreRaw<N> &R, LP lut) {
+    const auto M = T.P64 + (size_t)c * (N * N);
+    double sub[N], m[N], q[N], msg[N];
+    if (T.int_row[c] >= 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) sub[i] = R.sub[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) sub[i] = lut[R.code * N + i];
+    }
     child_message<N>(sub, M, m);
     double qs = 0.0;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        q[j] = fmax(TT_TINY, pp[j]) / m[j];
+        q[j] = fmax(TT_TINY, (double)R.pp[j]) / m[j];
         qs += q[j];
     }
     const double qi = 1.0 / qs;
-#pragma unroll
-    for (int j = 0; j < N; ++j) q[j] *= qi;
-    // GTR.evolve (gtr.py:1023-1025): msg_i = sum_j outgroup_j P[i][j]
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         double a = 0.0;
 #pragma unroll
         for (int j = 0; j < N; ++j) a = fma(q[j], M[i * N + j], a);
-        msg[i] = a * sub[i];
+        msg[i] = a * qi * sub[i];
     }
     double xs = 0.0;
 #pragma unroll
@@ -219,6 +316,22 @@ __device__ __forceinline__ void pre_node(const TreeDev &T, int c, int p) {
     }
     store_row<N>(T.prof + ((size_t)c * T.P + p) * N, out);
     T.states[(size_t)c * T.P + p] = (uint8_t)jm;
+}
+
+template <int N, class LP>
+__device__ __forceinline__ void pre_node(const TreeDev &T, int c, int p, LP lut) {
+    PreRaw<N> R;
+    pre_fetch<N>(T, c, p, R);
+    pre_finish<N>(T, c, p, R, lut);
+}
+
+template <int N, class LP>
+__device__ __forceinline__ void pre_pair(const TreeDev &T, int c0, int c1, bool has1, int p, LP lut) {
+    PreRaw<N> RA, RB;
+    pre_fetch<N>(T, c0, p, RA);
+    pre_fetch<N>(T, c1, p, RB);
+    pre_finish<N>(T, c0, p, RA, lut);
+    if (has1) pre_finish<N>(T, c1, p, RB, lut);
 }
 
 // Deterministic block reduction of one double per thread (wave shuffles,

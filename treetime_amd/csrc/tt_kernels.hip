@@ -33,20 +33,18 @@ __global__ __launch_bounds__(TT_BLOCK) void k_post_marginal(TreeDev T, LevelDev 
     const int p = pt * TT_WAVE + lane;
     const bool act = p < T.P;
     const int chunk = cg * (TT_BLOCK / TT_WAVE) + w;
-    double acc = 0.0;
+    LogAcc acc;
     if (chunk < L.n_chunks) {
         const int kb = chunk * L.chunk;
         const int ke = min(kb + L.chunk, L.n_nodes);
-        for (int k = kb; k < ke; ++k) {
-            const int v = wave_uniform(L.nodes[k]);
-            if (act) {
-                const double o = post_node<N>(T, v, p);
-                acc += o;
-                if (T.offs) T.offs[(size_t)v * T.P + p] = o;
-            }
+        for (int k = kb; k < ke; k += 2) {
+            const bool has1 = k + 1 < ke;
+            const int v0 = wave_uniform(L.nodes[k]);
+            const int v1 = has1 ? wave_uniform(L.nodes[k + 1]) : v0;
+            if (act) post_pair<N>(T, v0, v1, has1, p, T.lut, acc);
         }
     }
-    s_acc[w][lane] = acc;
+    s_acc[w][lane] = act ? acc.log_value() : 0.0;
     __syncthreads();
     if (w == 0 && act) {
         double s = 0.0;
@@ -100,15 +98,20 @@ __global__ __launch_bounds__(TT_BLOCK) void k_final_sum(TreeDev T, int nblocks) 
     if (threadIdx.x == 0) T.total[0] = b;
 }
 
+// Preorder level: a wave handles two nodes of the level for 64 patterns.
 template <int N>
 __global__ __launch_bounds__(TT_BLOCK) void k_pre_marginal(TreeDev T, LevelDev L, int n_ptiles) {
     const int lane = threadIdx.x & (TT_WAVE - 1);
     const int task = blockIdx.x * (TT_BLOCK / TT_WAVE) + threadIdx.x / TT_WAVE;
-    if (task >= L.n_nodes * n_ptiles) return;
-    const int k = task / n_ptiles, pt = task % n_ptiles;
+    const int n_pairs = (L.n_nodes + 1) / 2;
+    if (task >= n_pairs * n_ptiles) return;
+    const int k = 2 * (task / n_ptiles), pt = task % n_ptiles;
     const int p = pt * TT_WAVE + lane;
     if (p >= T.P) return;
-    pre_node<N>(T, wave_uniform(L.nodes[k]), p);
+    const bool has1 = k + 1 < L.n_nodes;
+    const int c0 = wave_uniform(L.nodes[k]);
+    const int c1 = has1 ? wave_uniform(L.nodes[k + 1]) : c0;
+    pre_pair<N>(T, c0, c1, has1, p, T.lut);
 }
 
 // Joint message of a tip child (treeanc.py:970-978, 997-1011):
@@ -117,8 +120,8 @@ template <int N>
 __device__ __forceinline__ void tip_joint(const TreeDev &T, int cref, int cnode, int p, double (&msg)[N],
                                           int store_cx) {
     const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
-    const double *ll = T.loglut + code * N;
-    const double *__restrict__ LP = T.logP64 + (size_t)cnode * (N * N);
+    const auto ll = T.loglut + code * N;
+    const auto LP = T.logP64 + (size_t)cnode * (N * N);
     double tl[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) tl[i] = ll[i];
@@ -164,7 +167,7 @@ __device__ __forceinline__ double post_joint_node(const TreeDev &T, int v, int v
 #pragma unroll
     for (int i = 0; i < N; ++i) msg[i] = 0.0;
     joint_children<N>(T, v, p, msg, store_tip_cx);
-    const double *__restrict__ LP = T.logP64 + (size_t)vn * (N * N);
+    const auto LP = T.logP64 + (size_t)vn * (N * N);
     double best[N];
     int cx[N];
 #pragma unroll
@@ -386,7 +389,7 @@ hipError_t final_sum(const TreeDev &T, int nblocks, hipStream_t s) {
 hipError_t pre_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s) {
     if (L.n_nodes <= 0) return hipSuccess;
     const int npt = n_ptiles(T.P);
-    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_pre_marginal<N_>, dim3(grid_for_tasks((long)L.n_nodes * npt)),
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_pre_marginal<N_>, dim3(grid_for_tasks((long)((L.n_nodes + 1) / 2) * npt)),
                                         dim3(TT_BLOCK), 0, s, T, L, npt));
     return hipGetLastError();
 }

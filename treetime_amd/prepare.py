@@ -60,6 +60,8 @@ def prepare(parent, branch_length, tip_chars, gtr, ambiguous, do_fill_overhangs=
         "v": gtr.v,
         "v_inv": gtr.v_inv,
         "mu": gtr.mu,
+        "W": gtr.W,
+        "n_states": gtr.n_states,
         "full_length": L,
     }
 

@@ -142,3 +142,42 @@ def make_job(n_tips, L, depth, n_states, seed, p_ambig=0.0, p_gap=0.0):
     names = ["t%05d" % i for i in range(len(tips))]
     return {"parent": parent, "branch_length": bl, "tips": tips, "names": names, "gtr": gtr,
             "chars": chars, "ambiguous": amb, "L": L, "seed": seed, "true_states": states}
+
+
+# ---------------------------------------------------------------------------
+# named workloads
+# ---------------------------------------------------------------------------
+def log_uniform_int(rng, lo, hi):
+    return int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+
+
+def mix_job_spec(j, base_seed=3000):
+    """Job j of the uneven nucleotide mix: tips log-uniform in [50, 4000],
+    sites log-uniform in [1000, 30000], depth 0.02, 1% N and 0.5% gaps at
+    tips, own seed."""
+    seed = base_seed + j
+    rng = np.random.default_rng(seed)
+    n_tips = log_uniform_int(rng, 50, 4000)
+    L = log_uniform_int(rng, 1000, 30000)
+    return dict(n_tips=n_tips, L=L, depth=0.02, n_states=4, seed=seed, p_ambig=0.01, p_gap=0.005)
+
+
+WORKLOADS = {
+    # name: (n_tips, L, depth, n_states, seed, p_ambig, p_gap)
+    "plain-200": (200, 1700, 0.02, 4, 0, 0.01, 0.005),
+    "large-5000": (5000, 30000, 1e-3, 4, 1000, 0.02, 0.0),
+    "large-5000-deep": (5000, 30000, 1e-2, 4, 1001, 0.02, 0.0),
+    "aa-1000": (1000, 5000, 0.3, 20, 2000, 0.0, 0.0),
+}
+
+
+def prepared_job(spec):
+    """Worker-friendly: generate + prepare one job; only numpy arrays out."""
+    from .prepare import prepare
+
+    job = make_job(spec["n_tips"], spec["L"], spec["depth"], spec["n_states"], spec["seed"],
+                   spec.get("p_ambig", 0.0), spec.get("p_gap", 0.0))
+    prep = prepare(job["parent"], job["branch_length"], job["chars"], job["gtr"], job["ambiguous"])
+    prep.pop("full_to_compressed", None)
+    prep["spec"] = spec
+    return prep
