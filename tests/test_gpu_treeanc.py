@@ -1,0 +1,164 @@
+"""The reference's TreeAnc tests (test/test_treetime.py), re-expressed through
+the drop-in facade; every pass runs on the GPU."""
+import itertools
+import sys
+from collections import defaultdict
+
+import numpy as np
+import pytest
+
+import treetime_amd as tt
+from oracle import fp64 as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _need_gpu():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def test_ancestral_lh_normalization():
+    """test_treetime.py:139-157: the marginal LH over all 4^3 tip patterns sums to 1."""
+    say("[treeanc] LH normalisation")
+    tree = tt.Tree.from_newick("((A:0.60100000009,B:0.3010000009):0.1,C:0.2):0.001;")
+    cols = list(itertools.product("ACGT", repeat=3))
+    aln = {k: "".join(c[i] for c in cols) for i, k in enumerate("ABC")}
+    g = tt.GTR.custom(alphabet=np.array(["A", "C", "G", "T"]), pi=np.array([0.9, 0.06, 0.02, 0.02]), W=np.ones((4, 4)))
+    t = tt.TreeAnc(gtr=g, tree=tree, aln=aln, rng_seed=1234, verbose=0)
+    t.reconstruct_anc("ml", marginal=True, debug=True)
+    lhsum = np.exp(t.sequence_LH(pos=np.arange(4**3))).sum()
+    say(f"  sum exp(LH) - 1 = {lhsum - 1:.2e}")
+    assert np.abs(lhsum - 1.0) < 1e-6
+    assert t.optimize_branch_len() == "success"
+
+
+def test_seq_joint_lh_is_max():
+    """test_treetime.py:241-307: joint sequence_LH equals the brute-force max."""
+    say("[treeanc] joint LH is max")
+    g = tt.GTR.custom(alphabet=np.array(["A", "C", "G", "T"]), pi=np.array([0.91, 0.05, 0.02, 0.02]),
+                      W=np.ones((4, 4)))
+    nwk = "((A:.0060,B:.30)C:.030,D:.020)E:.004;"
+    t1 = tt.TreeAnc(gtr=g, tree=tt.Tree.from_newick(nwk), aln={"A": "A", "B": "C", "D": "G"}, verbose=0)
+    t1.reconstruct_anc(method="ml", marginal=False, debug=True)
+    real = t1.tree.sequence_LH
+    aln = {"A": "A" * 16, "B": "C" * 16, "D": "G" * 16, "C": "AAAACCCCGGGGTTTT", "E": "ACGTACGTACGTACGT"}
+    t2 = tt.TreeAnc(gtr=g, tree=tt.Tree.from_newick(nwk), aln=aln, verbose=0)
+    ref = t2.ancestral_likelihood()
+    say(f"  |max - joint| = {abs(ref.max() - real[0]):.2e}")
+    assert abs(ref.max() - real[0]) < 1e-10
+
+
+def test_seq_joint_reconstruction_correct():
+    """test_treetime.py:160-238: joint reconstruction of simulated sequences;
+    ancestral_likelihood of the result equals tree.sequence_LH."""
+    say("[treeanc] joint reconstruction correct")
+    g = tt.GTR.custom(alphabet=np.array(["A", "C", "G", "T"]), pi=np.array([0.15, 0.95, 0.05, 0.3]),
+                      W=np.ones((4, 4)))
+    tree = tt.Tree.from_newick("((A:.060,B:.01200)C:.020,D:.0050)E:.004;")
+    rng = np.random.default_rng(1234)
+    L = 400
+    seqs = {"E": rng.choice(g.alphabet, p=g.Pi, size=L)}
+    order = list(tree.find_clades(order="preorder"))
+    parent = {}
+    for n in order:
+        for c in n.clades:
+            parent[c.name] = n.name
+    mut_count = 0
+    for n in order[1:]:
+        P = g.expQt(n.branch_length)
+        ps = seqs[parent[n.name]]
+        idx = np.array([rng.choice(4, p=P[:, g.state_index[s]]) for s in ps])
+        seqs[n.name] = g.alphabet[idx]
+        mut_count += int((seqs[n.name] != ps).sum())
+    aln = {k: "".join(seqs[k]) for k in "ABD"}
+    t = tt.TreeAnc(gtr=g, tree=tree, aln=aln, verbose=0)
+    t.infer_ancestral_sequences(final=True, debug=True, reconstruct_tip_states=True)
+    diff = sum(int((n.sequence != seqs[n.name]).sum()) for n in t.tree.find_clades() if n.up is not None)
+    assert diff / L < 2 * (1.0 * mut_count / L) ** 2 + 0.05
+    LH = t.ancestral_likelihood()
+    say(f"  sum(LH - sequence_LH) = {(LH - t.tree.sequence_LH).sum():.2e}")
+    assert abs((LH - t.tree.sequence_LH).sum()) < 1e-9
+
+
+def test_facade_matches_oracle_and_attributes():
+    """Facade marginal + joint on a simulated alignment against the oracle
+    run on the facade's own (ladderized) tree."""
+    say("[treeanc] facade vs oracle")
+    from treetime_amd import synth
+
+    job = synth.make_job(80, 500, 0.05, 4, 31, 0.02, 0.01)
+    parent, bl = job["parent"], job["branch_length"]
+    n = len(parent)
+    clades = [tt.Clade(branch_length=float(bl[i])) for i in range(n)]
+    for i, p in enumerate(parent):
+        if p >= 0:
+            clades[p].clades.append(clades[i])
+    for k, v in enumerate(job["tips"]):
+        clades[v].name = job["names"][k]
+    aln = {job["names"][k]: job["chars"][k].tobytes().decode() for k in range(len(job["tips"]))}
+    t = tt.TreeAnc(tree=tt.Tree(clades[0]), aln=aln, gtr=job["gtr"], verbose=0)
+    nd = t.infer_ancestral_sequences(marginal=True)
+    assert nd > 0
+    # oracle on the same ladderized topology and compressed alignment
+    nodes = list(t.tree.find_clades(order="preorder"))
+    idx = {id(x): i for i, x in enumerate(nodes)}
+    par = [-1] * len(nodes)
+    for i, x in enumerate(nodes):
+        for c in x.clades:
+            par[idx[id(c)]] = i
+    root, onodes = O.build_tree(par, [x.branch_length for x in nodes], [x.name for x in nodes])
+    g = t.gtr
+    ta = O.OracleTreeAnc(root, O.GTRModel("nuc_nogap", mu=g.mu, pi=g.Pi, W=g.W), t.data.compressed_alignment,
+                         t.data.multiplicity(), full_length=t.data.full_length)
+    ref = ta.marginal()
+    err = abs(t.sequence_LH() - ref) / abs(ref)
+    say(f"  facade marginal logL rel err {err:.2e}")
+    assert err < 1e-7
+    for x, o in zip(nodes, onodes):
+        if not x.is_terminal():
+            assert np.max(np.abs(x.marginal_profile - o.marginal_profile)) < 2e-6
+            assert x.cseq.shape == (t.data.compressed_length,)
+    refj = ta.joint()
+    t.infer_ancestral_sequences(marginal=False)
+    assert abs(t.tree.sequence_joint_LH - refj) / abs(refj) < 1e-7
+    full = t.get_reconstructed_alignment()
+    assert len(full[t.tree.root.name]) == 500
+    # a changed branch length is picked up on the next pass
+    some = nodes[5]
+    some.branch_length *= 3
+    t.infer_ancestral_sequences(marginal=True)
+    ta2_root, ta2_nodes = O.build_tree(par, [x.branch_length for x in nodes], [x.name for x in nodes])
+    ta2 = O.OracleTreeAnc(ta2_root, O.GTRModel("nuc_nogap", mu=g.mu, pi=g.Pi, W=g.W), t.data.compressed_alignment,
+                          t.data.multiplicity(), full_length=t.data.full_length)
+    ref2 = ta2.marginal()
+    assert abs(t.sequence_LH() - ref2) / abs(ref2) < 1e-7
+
+
+def test_gtr_primitives_on_device():
+    """GTR.propagate_profile / evolve / prob_t_profiles through the C ABI
+    against their FP64 numpy definitions (gtr.py:922-1025)."""
+    say("[gtr] device primitives")
+    g = tt.GTR.standard("JC69", alphabet="nuc")
+    rng = np.random.default_rng(3)
+    prof = rng.random((1000, 5))
+    Q = g.expQt(0.13)
+    assert np.allclose(g.propagate_profile(prof, 0.13), prof @ Q, rtol=1e-13)
+    assert np.allclose(g.propagate_profile(prof, 0.13, return_log=True), np.log(prof @ Q), rtol=1e-12)
+    assert np.allclose(g.evolve(prof, 0.13), prof @ Q.T, rtol=1e-13)
+    pp, pc = rng.random((700, 5)), rng.random((700, 5))
+    mult = rng.integers(1, 4, size=700).astype(float)
+    for t in [0.0, 0.05, 0.4, -1.0]:
+        og = O.GTRModel("nuc", mu=g.mu, pi=g.Pi, W=g.W)
+        ref = og.prob_t_profiles((pp, pc), mult, t, return_log=True)
+        got = g.prob_t_profiles((pp, pc), mult, t, return_log=True)
+        assert abs(got - ref) <= 1e-11 * max(1.0, abs(ref))
+    many = g.prob_t_profiles((pp, pc), mult, [0.01, 0.02, 0.03], return_log=True)
+    assert many.shape == (3,)

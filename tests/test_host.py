@@ -1,0 +1,158 @@
+"""CPU tests of the host-side logic: compression, Newick, facade set-up,
+error types, synthetic generators and the level schedule inputs."""
+import numpy as np
+import pytest
+
+import treetime_amd as tt
+from oracle import fp64 as O
+from treetime_amd import synth
+from treetime_amd.prepare import effective_lengths, fill_overhangs
+from treetime_amd.sequence_data import compress_columns
+
+
+def _codes(cols):
+    return np.array([[ord(c[i]) for c in cols] for i in range(len(cols[0]))], dtype=np.uint8)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_vectorised_compression_equals_oracle(seed):
+    rng = np.random.default_rng(seed)
+    N, L = 9, 300
+    chars = np.array(list("ACGTN-R"))
+    A = rng.choice(chars, size=(N, L), p=[0.3, 0.3, 0.2, 0.1, 0.05, 0.03, 0.02])
+    # plenty of constant and near-constant columns
+    A[:, ::3] = "A"
+    A[rng.integers(0, N, size=L // 7), np.arange(0, L, 7)[: L // 7]] = "N"
+    names = ["s%d" % i for i in range(N)]
+    comp, mult, f2c, _ = O.compress_alignment(names, {nm: A[i] for i, nm in enumerate(names)}, "N")
+    pat, c = compress_columns(A.view(np.uint32).astype(np.uint8), "N")
+    assert np.array_equal(pat, f2c)
+    assert np.array_equal(np.bincount(pat), mult)
+    for i, nm in enumerate(names):
+        assert "".join(c[i].view("S1").astype("U1")) == "".join(comp[nm])
+
+
+def test_compression_rules_vectorised():
+    cols = ["AAAA", "AANA", "ACAA", "ACAA", "A-AA", "NNNN", "CCCC", "NNNN", "ANCA", "N-NN"]
+    pat, c = compress_columns(_codes(cols), "N")
+    assert pat.tolist() == [0, 0, 1, 2, 3, 4, 5, 4, 6, 7]
+    assert "".join(chr(x) for x in c[:, 7]) == "----"
+
+
+def test_sequence_data_additional_constant_sites():
+    d = tt.sequence_data.SequenceData({"a": "ACGTACGT", "b": "ACGAACGT", "c": "ACGTTCGT"}, sequence_length=20)
+    assert d.full_length == 20 and d.multiplicity().sum() == 20
+    assert d.additional_constant_sites == 12
+
+
+def test_fill_overhangs_matches_reference_rule():
+    rows = ["--AC-G--", "--------", "ACGT", "A--T"]
+    for r in rows:
+        got = fill_overhangs(np.array([[ord(x) for x in r]], dtype=np.uint8), "N")
+        ref = O.fill_overhangs(np.array(list(r)), "N")
+        assert "".join(chr(x) for x in got[0]) == "".join(ref)
+
+
+def test_newick_roundtrip_and_ladderize():
+    t = tt.Tree.from_newick("((A:0.1,(B:0.2,C:0.3)0.95:0.4)X:0.5,D:0.6);")
+    assert [c.name for c in t.get_terminals()] == ["A", "B", "C", "D"]
+    inner = t.root.clades[0].clades[1]
+    assert inner.name is None and inner.confidence == 0.95
+    t.ladderize()
+    assert [c.name for c in t.get_terminals()] == ["D", "A", "B", "C"]
+    t2 = tt.Tree.from_newick(t.to_newick())
+    assert [c.name for c in t2.get_terminals()] == ["D", "A", "B", "C"]
+    assert abs(t2.total_branch_length() - 2.1) < 1e-12
+
+
+def test_postorder_preorder_orders():
+    t = tt.Tree.from_newick("((A,B)X,(C,D)Y)R;")
+    assert [n.name for n in t.find_clades("postorder")] == ["A", "B", "X", "C", "D", "Y", "R"]
+    assert [n.name for n in t.find_clades("preorder")] == ["R", "X", "A", "B", "Y", "C", "D"]
+    assert [n.name for n in t.find_clades("level")] == ["R", "X", "Y", "A", "B", "C", "D"]
+
+
+def test_treeanc_setup_errors():
+    with pytest.raises(tt.MissingDataError):
+        tt.TreeAnc(tree=tt.Tree.from_newick("(A:0.1,B:0.2);"), aln={"A": "AC", "B": "AC"}, verbose=0)
+    with pytest.raises(TypeError):
+        tt.TreeAnc(tree=None)
+    t = tt.TreeAnc(tree=tt.Tree.from_newick("((A:0.1,B:0.2):0.1,C:0.3);"), aln={"A": "ACG", "B": "ACT", "C": "ACG"},
+                   gtr="JC69", verbose=0)
+    with pytest.raises(tt.UnknownMethodError):
+        t.infer_ancestral_sequences(method="bogus")
+    with pytest.raises(NotImplementedError):
+        t.infer_ancestral_sequences(method="fitch")
+    assert [n.name for n in t.tree.get_nonterminals()] == ["NODE_0000000", "NODE_0000001"]
+    with pytest.raises(tt.MissingDataError):
+        tt.TreeAnc(tree=tt.Tree.from_newick("((A:0.1,B:0.2):0.1,(C:0.3,D:0.1):0.1);"), aln={"A": "ACG"}, verbose=0)
+
+
+def test_gtr_host_matches_oracle():
+    rng = np.random.default_rng(9)
+    for alpha, n in [("nuc_nogap", 4), ("nuc", 5), ("aa_nogap", 20)]:
+        pi = rng.dirichlet(np.ones(n) * 3)
+        W = rng.lognormal(0, 0.5, (n, n))
+        W = W + W.T
+        g = tt.GTR.custom(mu=0.7, pi=pi, W=W, alphabet=alpha)
+        o = O.GTRModel(alpha, mu=0.7, pi=pi, W=W)
+        assert abs(g.mu - o.mu) < 1e-14 and np.allclose(g.W, o.W) and np.allclose(g.Pi, o.Pi)
+        for t in [0.0, 0.01, 0.7]:
+            assert np.allclose(g.expQt(t), o.expQt(t), atol=1e-14)
+        with pytest.raises(ValueError):
+            g.expQt(-1e6)
+
+
+def test_gtr_standard_models():
+    jc = tt.GTR.standard("JC69", alphabet="nuc_nogap")
+    assert abs(jc.average_rate() - 1.0) < 1e-12
+    P = jc.expQt(0.3)
+    assert np.allclose(np.diag(P), 0.25 + 0.75 * np.exp(-0.4))
+    for m in ["K80", "F81", "HKY85", "T92", "TN93"]:
+        g = tt.GTR.standard(m)
+        assert abs(g.Q.sum(0)).sum() < 1e-12
+    with pytest.raises(KeyError):
+        tt.GTR.standard("nope")
+
+
+def test_gtr_from_file_fixture():
+    import os
+
+    g = tt.GTR.from_file(os.path.join(os.path.dirname(__file__), "golden", "gtr_fixture.txt"))
+    assert np.allclose(g.Pi, [0.3088, 0.1897, 0.2335, 0.2581, 0.0099])
+    assert abs(g.mu - 1.0) < 1e-4
+
+
+def test_state_pair_matches_oracle():
+    g = tt.GTR.standard("JC69", alphabet="nuc")
+    o = O.GTRModel("nuc", mu=1.0, pi=np.ones(5), W=np.ones((5, 5)))
+    rng = np.random.default_rng(1)
+    a = rng.choice(list("ACGT-N"), 200)
+    b = rng.choice(list("ACGT-N"), 200)
+    m = rng.random(200) * 3
+    for ig in [False, True]:
+        p1, c1 = g.state_pair(a, b, m, ignore_gaps=ig)
+        p2, c2 = o.state_pair(a, b, m, ignore_gaps=ig)
+        d1 = {tuple(x): y for x, y in zip(p1.tolist(), c1.tolist())}
+        d2 = {tuple(x): y for x, y in zip(p2.tolist(), c2.tolist())}
+        assert d1 == d2
+
+
+def test_kingman_and_simulation_shapes():
+    rng = np.random.default_rng(0)
+    parent, bl = synth.kingman_tree(50, 0.02, rng)
+    assert (parent >= 0).sum() == 98 and (bl >= 0).all()
+    p2, b2, _ = synth.preorder_renumber(parent, bl)
+    assert p2[0] == -1 and all(p2[i] < i for i in range(1, len(p2)))
+    job = synth.make_job(30, 200, 0.05, 4, 3, 0.01, 0.005)
+    assert job["chars"].shape == (30, 200)
+    # ultrametric: every tip at depth 0.05
+    depth = np.zeros(len(job["parent"]))
+    for i in range(1, len(job["parent"])):
+        depth[i] = depth[job["parent"][i]] + job["branch_length"][i]
+    assert np.allclose(depth[job["tips"]], 0.05)
+
+
+def test_effective_lengths_floor():
+    t = effective_lengths([0.0, 1e-9, 0.3], 1000)
+    assert np.allclose(t, [1e-6, 1e-6, 0.3])

@@ -315,7 +315,7 @@ int32_t ensure_joint(tt_tree *T) {
     if (T->joint_ready) return TT_OK;
     int32_t rc;
     size_t rows = (size_t)T->n_int * T->P * T->n;
-    if ((rc = T->alloc((void **)&T->d.Lx, rows * sizeof(float)))) return rc;
+    if ((rc = T->alloc((void **)&T->d.Lx, rows * sizeof(double)))) return rc;
     if ((rc = T->alloc((void **)&T->d.Cx, (size_t)T->n_nodes * T->P * T->n))) return rc;
     T->joint_ready = true;
     return TT_OK;
@@ -404,7 +404,7 @@ int32_t run_pass(tt_tree *T, int mode, uint32_t flags, double *logL, void *strea
 
 int32_t refresh_expqt(tt_tree *T, const int32_t *d_branches, const double *d_t, int k, hipStream_t s) {
     TT_HIP(ttk::expqt(T->n, d_t, d_branches, k, T->d_eig, T->d_v, T->d_vinv, T->mu, (double *)T->d.P64,
-                      (double *)T->d.logP64, s));
+                      (double *)T->d.logP64, (float *)T->d.P32, (float *)T->d.D32, s));
     return TT_OK;
 }
 
@@ -533,6 +533,8 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     } bufs[] = {
         {(void **)&d.P64, (size_t)nn * n * n * sizeof(double)},
         {(void **)&d.logP64, (size_t)nn * n * n * sizeof(double)},
+        {(void **)&d.P32, (size_t)nn * n * n * sizeof(float)},
+        {(void **)&d.D32, (size_t)nn * n * n * sizeof(float)},
         {(void **)&d.sub, (size_t)T->n_int * P * n * sizeof(float)},
         {(void **)&d.prof, (size_t)nn * P * n * sizeof(float)},
         {(void **)&d.states, (size_t)nn * P},

@@ -44,10 +44,12 @@ struct TreeDev {
     const double TT_C *pi;           // [n]
     const double TT_C *P64;          // [n_nodes][n][n]  expQt per branch, FP64
     const double TT_C *logP64;       // [n_nodes][n][n]  log(max(TINY, expQt))
+    const float TT_C *P32;           // [n_nodes][n][n]  expQt, FP32
+    const float TT_C *D32;           // [n_nodes][n][n]  expQt - I (computed in FP64), FP32
     float TT_G *sub;                 // [n_int][P][n]   subtree partials, max entry = 1
     float TT_G *prof;                // [n_nodes][P][n] marginal profiles (sum = 1)
     uint8_t TT_G *states;            // [n_nodes][P]
-    float TT_G *Lx;                  // [n_int][P][n]   joint messages, max entry = 0
+    double TT_G *Lx;                 // [n_int][P][n]   joint messages, max entry = 0 (FP64)
     uint8_t TT_G *Cx;                // [n_nodes][P][n] joint argmax table
     double TT_G *part;               // [R][P]          per-chunk-group FP64 log-normaliser sums
     double TT_G *part2;              // [S][P]          row-slice sums
@@ -68,7 +70,7 @@ struct LevelDev {
 namespace ttk {
 // launchers (tt_kernels.hip); n is the alphabet size
 hipError_t expqt(int n, const double *t, const int32_t *branches, int k, const double *eig, const double *v,
-                 const double *vinv, double mu, double *P64, double *logP64, hipStream_t s);
+                 const double *vinv, double mu, double *P64, double *logP64, float *P32, float *D32, hipStream_t s);
 hipError_t post_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s);
 hipError_t rowsum(const TreeDev &T, int R, int S, hipStream_t s);
 hipError_t root_marginal(const TreeDev &T, int S, hipStream_t s);

@@ -75,15 +75,36 @@ __device__ __forceinline__ void child_message(const double (&r)[N], Ptr M, doubl
     }
 }
 
-// Running product of per-node normalisers, value = m * 2^e.  Summing logs
-// node by node costs one FP64 log per (node, pattern); the product needs one
-// multiply and a rare rescale, and a single log at the end.
+// ---- postorder: FP32 arithmetic with deficit messages ---------------------
+//
+// For child c and state j the message is m_cj = r_cj + d_cj with
+// d_cj = sum_i r_ci (P - I)_ij, where P - I is built in FP64 and rounded once
+// (D32).  The deficit d is small on short branches and keeps its relative
+// precision in FP32; r + d is then formed exactly in FP64, so the per-node
+// normaliser of a nearly constant pattern is accurate to ~1e-7 of its own
+// (small) distance from 1 instead of 6e-8 absolute.  Normalisers multiply
+// into an FP64 running product (LogAcc).
+
+template <int N, class MP>
+__device__ __forceinline__ void deficit(const float (&r)[N], MP D, float (&d)[N]) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) d[j] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) d[j] = fmaf(r[i], D[i * N + j], d[j]);
+    }
+}
+
+// Running product of per-node normalisers, value = m * 2^e, FP64.  A node
+// contributes prod_c (r_c + d_c) at its selected state, each factor formed
+// exactly in FP64 from the FP32 row entry r and deficit d; one FP64 log per
+// pattern at the end replaces a log per (node, child, pattern).
 struct LogAcc {
     double m = 1.0;
     int e = 0;
-    __device__ __forceinline__ void mul(double x, int ex = 0) {
+    __device__ __forceinline__ void mul(double x) {
         m *= x;
-        e += ex;
         if (m < 0x1p-500) {
             m *= 0x1p+500;
             e -= 500;
@@ -95,63 +116,96 @@ struct LogAcc {
     __device__ __forceinline__ double log_value() const { return log(m) + (double)e * 0.6931471805599453094; }
 };
 
-// Renormalise a node's product so its largest entry is exactly 1, store the
-// row, fold the normaliser into ``acc`` (normalize_profile offset,
-// seq_utils.py:296-307).
+__device__ __forceinline__ double msg64(float r, float d) { return (double)r + (double)d; }
+
+// log of an FP32 message for the per-state sums of the polytomy path (those
+// only shape the stored row; the normaliser itself goes through LogAcc)
+__device__ __forceinline__ float log_msg(float r, float d) { return r == 1.0f ? log1pf(d) : logf(r + d); }
+
 template <int N>
-__device__ __forceinline__ void finish_post(const TreeDev &T, int v, int p, const double (&prod)[N], int ex,
-                                            LogAcc &acc) {
-    double mx = prod[0];
+__device__ __forceinline__ int argmax_first(const float (&x)[N]) {
     int jm = 0;
+    float mx = x[0];
 #pragma unroll
     for (int j = 1; j < N; ++j)
-        if (prod[j] > mx) {
-            mx = prod[j];
+        if (x[j] > mx) {
+            mx = x[j];
             jm = j;
         }
-    const double inv = 1.0 / mx;
-    float out[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : (float)(prod[j] * inv);
-    store_row<N>(T.sub + ((size_t)v * T.P + p) * N, out);
-    acc.mul(mx, ex);
+    return jm;
 }
 
-// Postorder step for internal row v, any number of children
-// (treeanc.py:857-878): product over children of (child row @ P(t)).
+// Store row v normalised by its entry jm (value xm), which is stored as
+// exactly 1.
+template <int N>
+__device__ __forceinline__ void store_post(const TreeDev &T, int v, int p, const float (&x)[N], int jm, float xm) {
+    // correctly rounded quotient: these entries feed the parents' messages,
+    // and a biased approximate reciprocal would bias variable patterns' logL
+    float out[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : x[j] / xm;
+    store_row<N>(T.sub + ((size_t)v * T.P + p) * N, out);
+}
+
+// Subtree row of child edge e: stored row of an internal child or the
+// profile-map row of a tip's character (seq2prof, seq_utils.py:207-229).
+template <int N, class LP>
+__device__ __forceinline__ void child_row(const TreeDev &T, int cref, int p, LP lut, float (&r)[N]) {
+    if (cref < 0) {
+        const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
+#pragma unroll
+        for (int i = 0; i < N; ++i) r[i] = lut[code * N + i];
+    } else {
+        load_rowf<N>(T.sub + ((size_t)cref * T.P + p) * N, r);
+    }
+}
+
+// Postorder step for internal row v with any number of children
+// (treeanc.py:857-878), in log space per state as the reference does.
 template <int N, class LP>
 __device__ __forceinline__ void post_node(const TreeDev &T, int v, int p, LP lut, LogAcc &acc) {
-    double prod[N];
+    float L[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) prod[j] = 1.0;
-    int ex = 0;
+    for (int j = 0; j < N; ++j) L[j] = 0.0f;
     const int e0 = wave_uniform(T.child_ptr[v]), e1 = wave_uniform(T.child_ptr[v + 1]);
     for (int e = e0; e < e1; ++e) {
         const int cref = wave_uniform(T.child_ref[e]);
         const int cnode = wave_uniform(T.child_node[e]);
-        double r[N], m[N];
-        if (cref < 0) {
-            const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
+        float r[N], d[N];
+        child_row<N>(T, cref, p, lut, r);
+        deficit<N>(r, T.D32 + (size_t)cnode * (N * N), d);
 #pragma unroll
-            for (int i = 0; i < N; ++i) r[i] = lut[code * N + i];
-        } else {
-            load_row<N>(T.sub + ((size_t)cref * T.P + p) * N, r);
-        }
-        child_message<N>(r, T.P64 + (size_t)cnode * (N * N), m);
-#pragma unroll
-        for (int j = 0; j < N; ++j) prod[j] *= m[j];
-        if (e1 - e0 > 2) {  // polytomies: keep the running product away from underflow
-            double mx = prod[0];
-#pragma unroll
-            for (int j = 1; j < N; ++j) mx = fmax(mx, prod[j]);
-            if (mx < 0x1p-500 && mx > 0.0) {
-#pragma unroll
-                for (int j = 0; j < N; ++j) prod[j] *= 0x1p+500;
-                ex -= 500;
-            }
-        }
+        for (int j = 0; j < N; ++j) L[j] += log_msg(r[j], d[j]);
     }
-    finish_post<N>(T, v, p, prod, ex, acc);
+    // first maximum, value carried along (no dynamic register indexing)
+    int jm = 0;
+    float Lm = L[0];
+#pragma unroll
+    for (int j = 1; j < N; ++j)
+        if (L[j] > Lm) {
+            Lm = L[j];
+            jm = j;
+        }
+    float out[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : expf(L[j] - Lm);
+    // normaliser at the selected state, children revisited (cache hits)
+    for (int e = e0; e < e1; ++e) {
+        const int cref = wave_uniform(T.child_ref[e]);
+        const int cnode = wave_uniform(T.child_node[e]);
+        float r[N], d[N];
+        child_row<N>(T, cref, p, lut, r);
+        deficit<N>(r, T.D32 + (size_t)cnode * (N * N), d);
+        float rj = r[0], dj = d[0];
+#pragma unroll
+        for (int j = 1; j < N; ++j)
+            if (j == jm) {
+                rj = r[j];
+                dj = d[j];
+            }
+        acc.mul(msg64(rj, dj));
+    }
+    store_row<N>(T.sub + ((size_t)v * T.P + p) * N, out);
 }
 
 // Binary fast path, split in a fetch phase (global loads only) and a finish
@@ -166,7 +220,7 @@ template <int N>
 __device__ __forceinline__ void bin_fetch(const TreeDev &T, int e0, int p, BinRaw<N> &R) {
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        const int cref = T.child_ref[e0 + c];
+        const int cref = wave_uniform(T.child_ref[e0 + c]);
         if (cref < 0) {
             R.code[c] = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
         } else {
@@ -177,33 +231,47 @@ __device__ __forceinline__ void bin_fetch(const TreeDev &T, int e0, int p, BinRa
 }
 
 template <int N, class LP>
-__device__ __forceinline__ void bin_finish(const TreeDev &T, int v, int e0, int p, const BinRaw<N> &R,
-                                           LP lut, LogAcc &acc) {
-    double prod[N];
+__device__ __forceinline__ void bin_finish(const TreeDev &T, int v, int e0, int p, const BinRaw<N> &R, LP lut,
+                                           LogAcc &acc) {
+    float r[2][N], d[2][N], x[N];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        const int cref = T.child_ref[e0 + c];
-        double r[N], m[N];
+        const int cref = wave_uniform(T.child_ref[e0 + c]);
+        const int cnode = wave_uniform(T.child_node[e0 + c]);
         if (cref < 0) {
 #pragma unroll
-            for (int i = 0; i < N; ++i) r[i] = lut[R.code[c] * N + i];
+            for (int i = 0; i < N; ++i) r[c][i] = lut[R.code[c] * N + i];
         } else {
 #pragma unroll
-            for (int i = 0; i < N; ++i) r[i] = R.row[c][i];
+            for (int i = 0; i < N; ++i) r[c][i] = R.row[c][i];
         }
-        child_message<N>(r, T.P64 + (size_t)T.child_node[e0 + c] * (N * N), m);
-#pragma unroll
-        for (int j = 0; j < N; ++j) prod[j] = c ? prod[j] * m[j] : m[j];
+        deficit<N>(r[c], T.D32 + (size_t)cnode * (N * N), d[c]);
     }
-    finish_post<N>(T, v, p, prod, 0, acc);
+#pragma unroll
+    for (int j = 0; j < N; ++j) x[j] = (r[0][j] + d[0][j]) * (r[1][j] + d[1][j]);
+    // first maximum; the children's (row, deficit) at that state ride along
+    int jm = 0;
+    float xm = x[0], r0 = r[0][0], d0 = d[0][0], r1 = r[1][0], d1 = d[1][0];
+#pragma unroll
+    for (int j = 1; j < N; ++j)
+        if (x[j] > xm) {
+            xm = x[j];
+            jm = j;
+            r0 = r[0][j];
+            d0 = d[0][j];
+            r1 = r[1][j];
+            d1 = d[1][j];
+        }
+    acc.mul(msg64(r0, d0) * msg64(r1, d1));
+    store_post<N>(T, v, p, x, jm, xm);
 }
 
 // Two postorder steps (rows v0 and, if has1, v1) for lane pattern p.
 template <int N, class LP>
 __device__ __forceinline__ void post_pair(const TreeDev &T, int v0, int v1, bool has1, int p, LP lut,
                                           LogAcc &acc) {
-    const int a0 = T.child_ptr[v0], a1 = T.child_ptr[v0 + 1];
-    const int b0 = T.child_ptr[v1], b1 = T.child_ptr[v1 + 1];
+    const int a0 = wave_uniform(T.child_ptr[v0]), a1 = wave_uniform(T.child_ptr[v0 + 1]);
+    const int b0 = wave_uniform(T.child_ptr[v1]), b1 = wave_uniform(T.child_ptr[v1 + 1]);
     if (a1 - a0 == 2 && b1 - b0 == 2) {
         BinRaw<N> RA, RB;  // both fetched unconditionally so they stay in registers
         bin_fetch<N>(T, a0, p, RA);
@@ -249,10 +317,12 @@ __device__ __forceinline__ double root_step(const TreeDev &T, int p, double offs
     return T.mult[p] * lp;
 }
 
-// Preorder step for node c (treeanc.py:889-930): outgroup message by the
-// reference's division trick, normalise(log max(TINY, parent profile) -
+// Preorder step for node c (treeanc.py:889-930), FP32: outgroup message by
+// the reference's division trick, normalise(log max(TINY, parent profile) -
 // log(child message)), evolved down the branch (GTR.evolve,
-// gtr.py:1023-1025) and multiplied with the subtree row.
+// gtr.py:1023-1025) and multiplied with the subtree row.  The outgroup's own
+// normalisation is a per-pattern constant that cancels in the final
+// normalisation, so it is not computed.
 template <int N>
 struct PreRaw {
     float pp[N];
@@ -262,58 +332,51 @@ struct PreRaw {
 
 template <int N>
 __device__ __forceinline__ void pre_fetch(const TreeDev &T, int c, int p, PreRaw<N> &R) {
-    const int par = T.parent[c];
-    const int ir = T.int_row[c];
+    const int par = wave_uniform(T.parent[c]);
+    const int ir = wave_uniform(T.int_row[c]);
     load_rowf<N>(T.prof + ((size_t)par * T.P + p) * N, R.pp);
     if (ir >= 0) {
         R.code = 0;
         load_rowf<N>(T.sub + ((size_t)ir * T.P + p) * N, R.sub);
     } else {
-        R.code = T.tip_codes[(size_t)T.tip_row[c] * T.P + p];
+        R.code = T.tip_codes[(size_t)wave_uniform(T.tip_row[c]) * T.P + p];
     }
 }
 
 template <int N, class LP>
 __device__ __forceinline__ void pre_finish(const TreeDev &T, int c, int p, const PreRaw<N> &R, LP lut) {
-    const auto M = T.P64 + (size_t)c * (N * N);
-    double sub[N], m[N], q[N], msg[N];
-    if (T.int_row[c] >= 0) {
+    const auto M = T.P32 + (size_t)c * (N * N);
+    float sub[N], m[N], q[N], msg[N];
+    if (wave_uniform(T.int_row[c]) >= 0) {
 #pragma unroll
         for (int i = 0; i < N; ++i) sub[i] = R.sub[i];
     } else {
 #pragma unroll
         for (int i = 0; i < N; ++i) sub[i] = lut[R.code * N + i];
     }
-    child_message<N>(sub, M, m);
-    double qs = 0.0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-        q[j] = fmax(TT_TINY, (double)R.pp[j]) / m[j];
-        qs += q[j];
-    }
-    const double qi = 1.0 / qs;
+    for (int j = 0; j < N; ++j) m[j] = 0.0f;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        double a = 0.0;
 #pragma unroll
-        for (int j = 0; j < N; ++j) a = fma(q[j], M[i * N + j], a);
-        msg[i] = a * qi * sub[i];
+        for (int j = 0; j < N; ++j) m[j] = fmaf(sub[i], M[i * N + j], m[j]);
     }
-    double xs = 0.0;
 #pragma unroll
-    for (int i = 0; i < N; ++i) xs += msg[i];
-    const double xi = 1.0 / xs;
+    for (int j = 0; j < N; ++j) q[j] = fmaxf((float)TT_TINY, R.pp[j]) * __builtin_amdgcn_rcpf(m[j]);
+    float xs = 0.0f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        float a = 0.0f;
+#pragma unroll
+        for (int j = 0; j < N; ++j) a = fmaf(q[j], M[i * N + j], a);
+        msg[i] = a * sub[i];
+        xs += msg[i];
+    }
+    const float xi = __builtin_amdgcn_rcpf(xs);
     float out[N];
-    int jm = 0;
-    double mx = msg[0];
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        out[i] = (float)(msg[i] * xi);
-        if (msg[i] > mx) {
-            mx = msg[i];
-            jm = i;
-        }
-    }
+    for (int i = 0; i < N; ++i) out[i] = msg[i] * xi;
+    const int jm = argmax_first<N>(msg);
     store_row<N>(T.prof + ((size_t)c * T.P + p) * N, out);
     T.states[(size_t)c * T.P + p] = (uint8_t)jm;
 }

@@ -5,7 +5,7 @@
 //                               entry is exactly 1.0 (reference:
 //                               node.marginal_subtree_LH, treeanc.py:877)
 //   prof [n_nodes][P][n] float  marginal profiles, sum = 1 (treeanc.py:909-917)
-//   Lx   [n_int][P][n]   float  joint messages, largest entry exactly 0
+//   Lx   [n_int][P][n]   double joint messages, largest entry exactly 0
 //   Cx   [n_nodes][P][n] uint8  joint argmax table (treeanc.py:996-1011)
 //   part [R][P]          double per-chunk-group sums of log normalisers
 //
@@ -152,8 +152,10 @@ __device__ __forceinline__ void joint_children(const TreeDev &T, int v, int p, d
         if (cref < 0) {
             tip_joint<N>(T, cref, T.child_node[e], p, msg, store_tip_cx);
         } else {
+            const auto src = T.Lx + ((size_t)cref * T.P + p) * N;
             double r[N];
-            load_row<N>(T.Lx + ((size_t)cref * T.P + p) * N, r);
+#pragma unroll
+            for (int i = 0; i < N; ++i) r[i] = src[i];
 #pragma unroll
             for (int i = 0; i < N; ++i) msg[i] += r[i];
         }
@@ -188,10 +190,9 @@ __device__ __forceinline__ double post_joint_node(const TreeDev &T, int v, int v
     double off = best[0];
 #pragma unroll
     for (int j = 1; j < N; ++j) off = fmax(off, best[j]);
-    float out[N];
+    const auto dst = T.Lx + ((size_t)v * T.P + p) * N;
 #pragma unroll
-    for (int j = 0; j < N; ++j) out[j] = (float)(best[j] - off);
-    store_row<N>(T.Lx + ((size_t)v * T.P + p) * N, out);
+    for (int j = 0; j < N; ++j) dst[j] = best[j] - off;
     store_bytes<N>(T.Cx + ((size_t)vn * T.P + p) * N, cx);
     return off;
 }
@@ -280,7 +281,8 @@ __global__ void k_fill_states(TreeDev T, const int32_t *nodes, int n_nodes, uint
 // v_inv, clipped at 0, plus log(max(TINY, P)) for the joint pass
 // (treeanc.py:970).
 __global__ void k_expqt(int n, const double *t, const int32_t *branches, int k, const double *eig,
-                        const double *v, const double *vinv, double mu, double *P64, double *logP64) {
+                        const double *v, const double *vinv, double mu, double *P64, double *logP64, float *P32,
+                        float *D32) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int nn = n * n;
     if (gid >= k * nn) return;
@@ -292,6 +294,8 @@ __global__ void k_expqt(int n, const double *t, const int32_t *branches, int k, 
     const size_t o = (size_t)branches[b] * nn + ij;
     P64[o] = P;
     logP64[o] = log(fmax(TT_TINY, P));
+    P32[o] = (float)P;
+    D32[o] = (float)(P - (i == j ? 1.0 : 0.0));  // deficit form keeps the diagonal's small part exact
 }
 
 // out = in @ Q (transpose=0, GTR.propagate_profile) or in @ Q^T (transpose=1,
@@ -353,11 +357,11 @@ namespace ttk {
 int root_blocks(int P) { return (P + TT_BLOCK - 1) / TT_BLOCK; }
 
 hipError_t expqt(int n, const double *t, const int32_t *branches, int k, const double *eig, const double *v,
-                 const double *vinv, double mu, double *P64, double *logP64, hipStream_t s) {
+                 const double *vinv, double mu, double *P64, double *logP64, float *P32, float *D32, hipStream_t s) {
     if (k <= 0) return hipSuccess;
     const int total = k * n * n;
     hipLaunchKernelGGL(k_expqt, dim3((total + 255) / 256), dim3(256), 0, s, n, t, branches, k, eig, v, vinv, mu,
-                       P64, logP64);
+                       P64, logP64, P32, D32);
     return hipGetLastError();
 }
 

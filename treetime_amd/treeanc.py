@@ -1,0 +1,455 @@
+"""TreeAnc: the reference's ancestral-reconstruction surface on the HIP engine.
+
+Method names, arguments, return values, node/tree attributes and exceptions
+follow treetime/treeanc.py (TreeTime 0.12.1):
+
+* ``infer_ancestral_sequences`` / ``reconstruct_anc`` (treeanc.py:512-570)
+  run the marginal (treeanc.py:762-932) or joint (934-1080) pass on the GPU
+  through ``libttgpu.so`` and write ``node.marginal_profile``,
+  ``node.marginal_subtree_LH``, ``node._cseq``, ``tree.sequence_LH``,
+  ``tree.total_sequence_LH`` / ``tree.sequence_joint_LH`` as the reference
+  does.
+* ``sequence_LH`` (691-716), ``ancestral_likelihood`` (718-750),
+  ``optimize_branch_len`` / ``optimize_branch_lengths_joint`` (1168-1270).
+
+Fitch parsimony and GTR inference are outside this engine
+(``NotImplementedError``).
+"""
+import sys
+import time
+
+import numpy as np
+
+from . import config as ttconf
+from .engine import MISSING_CODE, DeviceTree, code_lut, default_engine
+from .errors import MissingDataError, UnknownMethodError
+from .gtr import GTR
+from .seq_utils import extend_profile, seq2prof
+from .sequence_data import SequenceData
+from .tree import Clade, Tree
+
+
+def _compressed_sequence(node):
+    """treeanc.py:14-25."""
+    tt = node.tt
+    if node.name in tt.data.compressed_alignment and not tt.reconstructed_tip_sequences:
+        return tt.data.compressed_alignment[node.name]
+    if hasattr(node, "_cseq"):
+        return node._cseq
+    if node.is_terminal():
+        return None
+    raise ValueError("Ancestral sequences are not yet inferred")
+
+
+def _mutations(node):
+    """treeanc.py:28-42."""
+    tt = node.tt
+    if node.up is None:
+        return []
+    if (not tt.reconstructed_tip_sequences) and node.name in tt.data.aln:
+        return tt.data.differences(node.up.cseq, tt.data.aln[node.name], seq2_compressed=False)
+    if node.is_terminal() and node.name not in tt.data.aln:
+        return []
+    return tt.data.differences(node.up.cseq, node.cseq)
+
+
+Clade.cseq = property(_compressed_sequence)
+Clade.mutations = property(_mutations)
+Clade.sequence = property(lambda x: x.tt.sequence(x, as_string=False))
+
+
+class TreeAnc:
+    """Tree + alignment + GTR model with GPU ancestral reconstruction."""
+
+    def __init__(self, tree=None, aln=None, gtr=None, fill_overhangs=True, ref=None, verbose=ttconf.VERBOSE,
+                 ignore_gaps=True, convert_upper=True, seq_multiplicity=None, log=None, compress=True, seq_len=None,
+                 ignore_missing_alns=False, keep_node_order=False, rng_seed=None, engine=None, **kwargs):
+        if tree is None:
+            raise TypeError("TreeAnc requires a tree!")
+        self.t_start = time.time()
+        self.verbose = verbose
+        self.log = log
+        self.ok = False
+        self.data = None
+        self.log_messages = set()
+        self.use_mutation_length = False
+        self.ignore_gaps = ignore_gaps
+        self.reconstructed_tip_sequences = False
+        self.sequence_reconstruction = None
+        self.ignore_missing_alns = ignore_missing_alns
+        self.keep_node_order = keep_node_order
+        self.rng = np.random.default_rng(seed=rng_seed)
+        self.engine = engine
+        self._device = None
+        self._tree = None
+        self.tree = tree
+        self._gtr = None
+        self.set_gtr(gtr or "JC69", **kwargs)
+        self.data = SequenceData(aln, ref=ref, logger=self.logger, compress=compress, convert_upper=convert_upper,
+                                 fill_overhangs=fill_overhangs, ambiguous=self.gtr.ambiguous, sequence_length=seq_len,
+                                 seq_multiplicity=seq_multiplicity)
+        if self.data.aln and self.tree:
+            self._check_alignment_tree_gtr_consistency()
+
+    # ------------------------------------------------------------------ setup
+    def logger(self, msg, level, warn=False, only_once=False):
+        if only_once and msg in self.log_messages:
+            return
+        self.log_messages.add(msg)
+        if level < self.verbose or (warn and level <= self.verbose):
+            dt = time.time() - self.t_start
+            print(("\n" if level < 2 else "") + format(dt, "4.2f") + "\t" + level * "-" + msg, file=sys.stdout)
+
+    @property
+    def gtr(self):
+        return self._gtr
+
+    @gtr.setter
+    def gtr(self, value):
+        if not isinstance(value, GTR):
+            raise TypeError("GTR instance expected")
+        self._gtr = value
+        self._drop_device()
+
+    def set_gtr(self, in_gtr, **kwargs):
+        if isinstance(in_gtr, str):
+            self._gtr = GTR.standard(model=in_gtr, **kwargs)
+        elif isinstance(in_gtr, GTR):
+            self._gtr = in_gtr
+        else:
+            raise TypeError("Cannot set GTR model in TreeAnc class: GTR or string expected")
+        self._gtr.logger = self.logger
+        self._drop_device()
+
+    @property
+    def aln(self):
+        return self.data.aln
+
+    @aln.setter
+    def aln(self, in_aln):
+        self.data.aln = in_aln
+        self._drop_device()
+        if self.tree:
+            self._check_alignment_tree_gtr_consistency()
+
+    @property
+    def tree(self):
+        return self._tree
+
+    @tree.setter
+    def tree(self, in_tree):
+        """treeanc.py:316-371: accepts a Tree (or Bio.Phylo tree) or a Newick
+        file name / string."""
+        import os
+
+        if isinstance(in_tree, str):
+            if os.path.isfile(in_tree):
+                with open(in_tree) as f:
+                    in_tree = f.read()
+            try:
+                in_tree = Tree.from_newick(in_tree)
+            except Exception as e:
+                raise MissingDataError("TreeAnc: could not load tree! input was " + str(in_tree)[:80]) from e
+        if not hasattr(in_tree, "root"):
+            raise MissingDataError("TreeAnc: could not load tree! input was " + str(in_tree))
+        self._tree = in_tree
+        if self._tree.count_terminals() < 3:
+            raise MissingDataError("TreeAnc: tree in %s as only %d tips. Please check your tree!"
+                                   % (str(in_tree), self._tree.count_terminals()))
+        long_branch = False
+        for node in self._tree.find_clades():
+            node.branch_length = node.branch_length if node.branch_length else 0.0
+            long_branch |= node.branch_length > ttconf.MAX_BRANCH_LENGTH
+            if hasattr(node, "_cseq"):
+                del node._cseq
+            node.original_length = node.branch_length
+            node.mutation_length = node.branch_length
+        if long_branch:
+            self.logger("WARNING: TreeTime has detected branches that are longer than %d." % ttconf.MAX_BRANCH_LENGTH,
+                        0, warn=True)
+        self.prepare_tree()
+        if self.data:
+            self._check_alignment_tree_gtr_consistency()
+
+    def prepare_tree(self):
+        """treeanc.py:446-459."""
+        self.sequence_reconstruction = False
+        self.tree.root.branch_length = 0.001
+        self.tree.root.mask = None
+        self.tree.root.mutation_length = self.tree.root.branch_length
+        if not self.keep_node_order:
+            self.tree.ladderize()
+        self._prepare_nodes()
+        self._leaves_lookup = {node.name: node for node in self.tree.get_terminals()}
+        self._drop_device()
+
+    def _prepare_nodes(self):
+        """treeanc.py:461-493: NODE_%07d names, up links, masks."""
+        self.tree.root.up = None
+        self.tree.root.tt = self
+        names = {n.name for n in self.tree.find_clades() if n.name}
+        count = 0
+        for clade in self.tree.get_nonterminals(order="preorder"):
+            if clade.name is None:
+                tmp = "NODE_" + format(count, "07d")
+                while tmp in names:
+                    count += 1
+                    tmp = "NODE_" + format(count, "07d")
+                clade.name = tmp
+                names.add(tmp)
+            count += 1
+            for c in clade.clades:
+                c.up = clade
+                c.tt = self
+        for clade in self.tree.find_clades(order="postorder"):
+            if not hasattr(clade, "mask"):
+                clade.mask = None
+        self.tree.root.dist2root = 0.0
+        for clade in self.tree.get_nonterminals(order="preorder"):
+            for c in clade.clades:
+                c.dist2root = clade.dist2root + c.mutation_length
+
+    @property
+    def leaves_lookup(self):
+        return self._leaves_lookup
+
+    @property
+    def one_mutation(self):
+        return 1.0 / self.data.full_length if self.data.full_length else np.nan
+
+    @property
+    def seq_len(self):
+        return self.data.full_length
+
+    @property
+    def sequence_length(self):
+        return self.data.full_length
+
+    def _check_alignment_tree_gtr_consistency(self):
+        """treeanc.py:395-444."""
+        failed = 0
+        for leaf in self.tree.get_terminals():
+            leaf.count = self.data.seq_multiplicity.get(leaf.name, 1.0)
+            if leaf.name not in self.data.compressed_alignment:
+                self.logger("***WARNING: NO SEQUENCE FOR LEAF: '%s'" % leaf.name, 0, warn=True)
+                failed += 1
+                if not self.ignore_missing_alns and failed > self.tree.count_terminals() / 3:
+                    raise MissingDataError("TreeAnc._check_alignment_tree_gtr_consistency: At least 30\\% terminal "
+                                           "nodes cannot be assigned a sequence!")
+        extend_profile(self.gtr, self.data.aln.values(), logger=self.logger)
+        self.ok = True
+        self._drop_device()
+
+    # --------------------------------------------------------- device mirror
+    def _drop_device(self):
+        if getattr(self, "_device", None) is not None:
+            self._device.close()
+        self._device = None
+        self._nodes = None
+
+    def _branch_length_to_gtr(self, node):
+        """treeanc.py:752-760."""
+        bl = node.mutation_length if self.use_mutation_length else node.branch_length
+        return max(ttconf.MIN_BRANCH_LENGTH * self.one_mutation, bl)
+
+    def _device_tree(self):
+        """Flatten the tree (preorder numbering, children in list order) and
+        make it resident on the device, or refresh its branch lengths."""
+        nodes = list(self.tree.find_clades(order="preorder"))
+        t = np.array([self._branch_length_to_gtr(n) for n in nodes])
+        if self._device is not None and self._nodes is not None and len(self._nodes) == len(nodes) and \
+                all(a is b for a, b in zip(self._nodes, nodes)):
+            changed = np.nonzero(t != self._t)[0]
+            if len(changed):
+                self._device.set_branch_lengths(changed.astype(np.int32), t[changed])
+                self._t = t
+            return self._device
+        self._drop_device()
+        idx = {id(n): i for i, n in enumerate(nodes)}
+        parent = np.full(len(nodes), -1, dtype=np.int32)
+        for i, n in enumerate(nodes):
+            for c in n.clades:
+                parent[idx[id(c)]] = i
+        tips = [n for n in nodes if not n.clades]
+        P = self.data.compressed_length
+        row = {k: i for i, k in enumerate(self.data.sequence_names)}
+        codes = np.full((len(tips), P), MISSING_CODE, dtype=np.uint8)
+        for k, leaf in enumerate(tips):
+            if leaf.name in row:
+                codes[k] = self.data.compressed_codes[row[leaf.name]]
+        g = self.gtr
+        eng = self.engine or default_engine()
+        self._device = DeviceTree(eng, parent, t, codes, code_lut(g.profile_map, g.n_states), self.data.multiplicity(),
+                                  g.Pi, g.eigenvals, g.v, g.v_inv, g.mu)
+        self._nodes = nodes
+        self._t = t
+        self._gtr_state = (g.mu, g.Pi.copy(), g.W.copy())
+        return self._device
+
+    def _sync_model(self, dev):
+        g = self.gtr
+        if not (self._gtr_state[0] == g.mu and np.array_equal(self._gtr_state[1], g.Pi) and
+                np.array_equal(self._gtr_state[2], g.W)):
+            self._drop_device()
+            return self._device_tree()
+        return dev
+
+    # ------------------------------------------------------- reconstruction
+    def reconstruct_anc(self, *args, **kwargs):
+        return self.infer_ancestral_sequences(*args, **kwargs)
+
+    def infer_ancestral_sequences(self, method="probabilistic", infer_gtr=False, marginal=False,
+                                  reconstruct_tip_states=False, **kwargs):
+        """treeanc.py:516-570; returns the number of changed characters."""
+        if not self.ok:
+            raise MissingDataError("TreeAnc.infer_ancestral_sequences: ERROR, sequences or tree are missing")
+        m = method.lower()
+        if m in ["ml", "probabilistic"]:
+            run = self._ml_anc_marginal if marginal else self._ml_anc_joint
+        elif m in ["fitch", "parsimony"]:
+            raise NotImplementedError("Fitch parsimony is not part of the GPU pruning engine")
+        else:
+            raise UnknownMethodError("Reconstruction method needs to be in ['ml', 'probabilistic', 'fitch', "
+                                     "'parsimony'], got '{}'".format(method))
+        if infer_gtr:
+            raise NotImplementedError("GTR inference is not part of the GPU pruning engine")
+        return run(reconstruct_tip_states=reconstruct_tip_states, **kwargs)
+
+    def _assign_sequences(self, dev, reconstruct_tip_states, sample_root=False):
+        states = dev.states()
+        alpha = self.gtr.alphabet
+        L = self.data.compressed_length
+        n_diff = 0
+        for i, node in enumerate(self._nodes):
+            if node.is_terminal() and not reconstruct_tip_states:
+                continue
+            seq = alpha[states[i]]
+            if self.sequence_reconstruction:
+                n_diff += int((seq != node.cseq).sum())
+            else:
+                n_diff += L
+            node._cseq = seq
+        return n_diff
+
+    def _ml_anc_marginal(self, sample_from_profile=False, reconstruct_tip_states=False, debug=False, **kwargs):
+        """treeanc.py:762-812."""
+        if sample_from_profile:
+            raise NotImplementedError("sampling from profiles is not part of the GPU pruning engine")
+        dev = self._sync_model(self._device_tree())
+        total = dev.marginal(reconstruct_tips=reconstruct_tip_states)
+        prof = dev.profiles()
+        sub = dev.profiles(which=1)
+        tree = self.tree
+        tree.sequence_LH = dev.sequence_lh()
+        tree.total_sequence_LH = total
+        tree.sequence_marginal_LH = total
+        for i, node in enumerate(self._nodes):
+            if node.is_terminal():
+                if node.name in self.data.compressed_alignment:
+                    node.marginal_subtree_LH = seq2prof(self.data.compressed_alignment[node.name],
+                                                        self.gtr.profile_map)
+                else:
+                    node.marginal_subtree_LH = np.ones((self.data.compressed_length, self.gtr.n_states))
+                if reconstruct_tip_states:
+                    node.marginal_profile = prof[i].astype(float)
+            else:
+                node.marginal_subtree_LH = sub[i].astype(float)
+                node.marginal_profile = prof[i].astype(float)
+        n_diff = self._assign_sequences(dev, reconstruct_tip_states)
+        self.reconstructed_tip_sequences = reconstruct_tip_states
+        self.sequence_reconstruction = "marginal"
+        return n_diff
+
+    def _ml_anc_joint(self, sample_from_profile=False, reconstruct_tip_states=False, debug=False, **kwargs):
+        """treeanc.py:934-1080."""
+        if sample_from_profile:
+            raise NotImplementedError("sampling from profiles is not part of the GPU pruning engine")
+        dev = self._sync_model(self._device_tree())
+        total = dev.joint(reconstruct_tips=reconstruct_tip_states)
+        self.tree.sequence_LH = dev.sequence_lh()
+        self.tree.sequence_joint_LH = total
+        n_diff = self._assign_sequences(dev, reconstruct_tip_states)
+        self.reconstructed_tip_sequences = reconstruct_tip_states
+        self.sequence_reconstruction = "joint"
+        return n_diff
+
+    # ---------------------------------------------------------- likelihood
+    def sequence_LH(self, pos=None, full_sequence=False):
+        """treeanc.py:691-716."""
+        if not hasattr(self.tree, "total_sequence_LH"):
+            self.logger("TreeAnc.sequence_LH: you need to run marginal ancestral inference first!", 1)
+            self.infer_ancestral_sequences(marginal=True)
+        if pos is not None:
+            cpos = self.data.full_to_compressed_sequence_map[pos] if full_sequence else pos
+            return self.tree.sequence_LH[cpos]
+        return self.tree.total_sequence_LH
+
+    def ancestral_likelihood(self):
+        """treeanc.py:718-750: log-likelihood of the current sequences."""
+        log_lh = np.zeros(self.data.multiplicity().shape[0])
+        idx = self.gtr.state_index
+        for node in self.tree.find_clades(order="postorder"):
+            if node.up is None:
+                prof = seq2prof(node.cseq, self.gtr.profile_map) * self.gtr.Pi
+                log_lh += np.log(prof.sum(axis=1))
+                continue
+            pairs = np.array([(idx[a], idx[b]) for a, b in zip(node.up.cseq, node.cseq)])
+            lq = np.log(self.gtr.expQt(node.branch_length))
+            log_lh += lq[pairs[:, 1], pairs[:, 0]]
+        return log_lh
+
+    # ------------------------------------------------------ branch lengths
+    def optimize_branch_len(self, **kwargs):
+        return self.optimize_branch_lengths_joint(**kwargs)
+
+    def optimize_branch_len_joint(self, **kwargs):
+        return self.optimize_branch_lengths_joint(**kwargs)
+
+    def optimize_branch_lengths_joint(self, **kwargs):
+        """treeanc.py:1176-1242: per branch, Brent on the state-pair counts of
+        the current reconstruction."""
+        if self.tree is None or self.data.aln is None:
+            raise MissingDataError("TreeAnc.optimize_branch_length: ERROR, alignment or tree are missing.")
+        store_old = kwargs.get("store_old", False)
+        max_bl = 0
+        for node in self.tree.find_clades(order="postorder"):
+            if node.up is None:
+                continue
+            if store_old:
+                node._old_length = node.branch_length
+            new_len = max(0, self.optimal_branch_length(node))
+            node.branch_length = new_len
+            node.mutation_length = new_len
+            max_bl = max(max_bl, new_len)
+        if max_bl > 0.15:
+            self.logger("TreeAnc.optimize_branch_lengths_joint: THIS TREE HAS LONG BRANCHES.", 0, warn=True)
+        self.tree.root.up = None
+        self.tree.root.dist2root = 0.0
+        self._prepare_nodes()
+        return ttconf.SUCCESS
+
+    def optimal_branch_length(self, node):
+        """treeanc.py:1244-1270."""
+        if node.up is None:
+            return self.one_mutation
+        if node.cseq is None and node.is_terminal():
+            raise MissingDataError("TreeAnc.optimal_branch_length: terminal node alignments required; sequence is "
+                                   "missing for leaf: '%s'." % node.name)
+        pairs, mult = self.gtr.state_pair(node.up.cseq, node.cseq, pattern_multiplicity=self.data.multiplicity(),
+                                          ignore_gaps=self.ignore_gaps)
+        node.branch_state = {"pair": pairs, "multiplicity": mult}
+        return self.gtr.optimal_t_compressed(pairs, mult)
+
+    # ---------------------------------------------------------------- output
+    def sequence(self, node, as_string=True):
+        seq = node.cseq
+        if seq is None:
+            return None
+        return self.data.compressed_to_full_sequence(seq, as_string=as_string)
+
+    def get_reconstructed_alignment(self, reconstruct_tip_states=False):
+        """Full-length sequences of every node as a dict name -> str."""
+        if not self.sequence_reconstruction:
+            self.infer_ancestral_sequences("probabilistic", marginal=True, reconstruct_tip_states=reconstruct_tip_states)
+        return {n.name: self.sequence(n, as_string=True) for n in self.tree.find_clades()
+                if n.cseq is not None}
