@@ -106,8 +106,24 @@ int32_t tt_joint(tt_tree *tree, uint32_t flags, double *logL, void *stream);
 
 /* tt_update_branch: change one branch length and recompute the marginal logL
  * along the path from that branch to the root only.  Equivalent to setting
- * the length and re-running tt_loglik (treeanc.py:1685-1689 pattern). */
+ * the length and re-running tt_loglik (treeanc.py:1685-1689 pattern).  The
+ * first call switches the tree to update mode (the postorder then also keeps
+ * each node's log normaliser) and runs one full pass; later calls touch only
+ * the root path.  Profiles and states of earlier passes become stale. */
 int32_t tt_update_branch(tt_tree *tree, int32_t node, double t, double *logL, void *stream);
+
+/* Free the joint-pass buffers (Lx, Cx) and return them to the engine's
+ * budget; the next tt_joint allocates them again.  No reference counterpart:
+ * it lets a server admit requests against an HBM budget. */
+int32_t tt_tree_trim(tt_tree *tree);
+
+/* 1 when all work queued on the tree's own stream (passes launched with
+ * stream == NULL) has finished, 0 while it runs; for asynchronous callers. */
+int32_t tt_tree_query(tt_tree *tree);
+
+/* tree.total_sequence_LH / sequence_joint_LH of the last pass (waits for the
+ * tree's own stream). */
+int32_t tt_get_total(tt_tree *tree, double *logL);
 
 /* ---- results (device -> host, synchronous) ------------------------------ */
 /* tree.sequence_LH: per-pattern log-likelihood of the last pass
@@ -116,8 +132,8 @@ int32_t tt_get_sequence_lh(tt_tree *tree, double *out);
 /* Reconstructed state index per node and pattern, [n_nodes][P]; 255 where a
  * node has no reconstruction (tips without TT_RECONSTRUCT_TIPS). */
 int32_t tt_get_states(tt_tree *tree, uint8_t *out);
-/* Profiles, [n_nodes][P][n_states] (float32): which = TT_PROFILE_*.  Rows a
- * pass did not produce are zero. */
+/* Profiles, [n_nodes][P][n_states] (float32): which = TT_PROFILE_*.  Rows the
+ * last pass did not produce hold an earlier pass's values (or zero). */
 int32_t tt_get_profiles(tt_tree *tree, int32_t which, float *out);
 /* Device pointers of the resident buffers (for zero-copy consumers). */
 int32_t tt_get_device_buffers(tt_tree *tree, float **profiles, uint8_t **states, double **sequence_lh);

@@ -80,6 +80,9 @@ SIGNATURES = {
     "tt_loglik": (_I32, [_P, _PD, _P]),
     "tt_joint": (_I32, [_P, _U32, _PD, _P]),
     "tt_update_branch": (_I32, [_P, _I32, _D, _PD, _P]),
+    "tt_tree_trim": (_I32, [_P]),
+    "tt_tree_query": (_I32, [_P]),
+    "tt_get_total": (_I32, [_P, _PD]),
     "tt_get_sequence_lh": (_I32, [_P, _PD]),
     "tt_get_states": (_I32, [_P, _PU8]),
     "tt_get_profiles": (_I32, [_P, _I32, _PF]),

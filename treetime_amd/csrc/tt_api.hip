@@ -125,6 +125,8 @@ struct tt_tree {
     int32_t *d_tmp_br = nullptr;
     int tmp_cap = 0;
     bool joint_ready = false;
+    bool update_mode = false;  // postorder also stores per-node normalisers (d.offs)
+    bool offs_valid = false;   // d.offs and the subtree rows match the current lengths
     hipStream_t stream = nullptr;
     std::map<int, hipGraphExec_t> graphs;
     std::vector<std::pair<void *, size_t>> allocs;
@@ -395,10 +397,24 @@ int32_t run_pass(tt_tree *T, int mode, uint32_t flags, double *logL, void *strea
         if (rc) return rc;
     }
     T->last_pass = mode;
+    if (T->update_mode && (mode == 1 || mode == 3)) T->offs_valid = true;
     if (logL) {
         TT_HIP(hipMemcpyAsync(logL, T->d.total, sizeof(double), hipMemcpyDeviceToHost, s));
         TT_HIP(hipStreamSynchronize(s));
     }
+    return TT_OK;
+}
+
+// Switch a tree to update mode: per-node normalisers are kept so that a
+// branch change can be re-evaluated along its root path only.
+int32_t enable_update_mode(tt_tree *T) {
+    if (T->update_mode) return TT_OK;
+    int32_t rc;
+    if ((rc = T->alloc((void **)&T->d.offs, (size_t)T->n_int * T->P * sizeof(double)))) return rc;
+    for (auto &g : T->graphs) (void)hipGraphExecDestroy(g.second);  // recorded without d.offs
+    T->graphs.clear();
+    T->update_mode = true;
+    T->offs_valid = false;
     return TT_OK;
 }
 
@@ -602,6 +618,7 @@ int32_t tt_tree_set_model(tt_tree *T, const double *eigvals, const double *v, co
     T->v.assign(v, v + n * n);
     T->vinv.assign(v_inv, v_inv + n * n);
     T->mu = mu;
+    T->offs_valid = false;
     int32_t rc = check_range(T, T->t.data(), T->n_nodes);
     if (rc) return rc;
     TT_HIP(hipSetDevice(T->eng->device));
@@ -614,7 +631,18 @@ int32_t tt_tree_set_model(tt_tree *T, const double *eigvals, const double *v, co
     return TT_OK;
 }
 
+namespace {
+int32_t set_lengths(tt_tree *T, const int32_t *nodes, const double *t, int32_t k);
+}
+
 int32_t tt_tree_set_branch_lengths(tt_tree *T, const int32_t *nodes, const double *t, int32_t k) {
+    int32_t rc = set_lengths(T, nodes, t, k);
+    if (rc == TT_OK && k > 0) T->offs_valid = false;
+    return rc;
+}
+
+namespace {
+int32_t set_lengths(tt_tree *T, const int32_t *nodes, const double *t, int32_t k) {
     if (!T || (k > 0 && (!nodes || !t))) return fail(TT_EINVAL, "NULL argument");
     if (k <= 0) return TT_OK;
     for (int i = 0; i < k; ++i)
@@ -641,6 +669,7 @@ int32_t tt_tree_set_branch_lengths(tt_tree *T, const int32_t *nodes, const doubl
     TT_HIP(hipStreamSynchronize(T->stream));
     return TT_OK;
 }
+}  // namespace
 
 int32_t tt_marginal(tt_tree *tree, uint32_t flags, double *logL, void *stream) {
     return run_pass(tree, 1, flags, logL, stream);
@@ -652,10 +681,81 @@ int32_t tt_joint(tt_tree *tree, uint32_t flags, double *logL, void *stream) {
     return run_pass(tree, 2, flags, logL, stream);
 }
 
-int32_t tt_update_branch(tt_tree *tree, int32_t node, double t, double *logL, void *stream) {
-    int32_t rc = tt_tree_set_branch_lengths(tree, &node, &t, 1);
+int32_t tt_update_branch(tt_tree *T, int32_t node, double t, double *logL, void *stream) {
+    if (!T) return fail(TT_EINVAL, "tree is NULL");
+    if (node < 0 || node >= T->n_nodes) return fail(TT_EINVAL, "node %d out of range", node);
+    if (node == T->root) return fail(TT_EINVAL, "the root has no branch");
+    if (!T->have_model) return fail(TT_ESTATE, "tt_tree_set_model has not been called");
+    TT_HIP(hipSetDevice(T->eng->device));
+    int32_t rc = enable_update_mode(T);
     if (rc) return rc;
-    return run_pass(tree, 3, 0, logL, stream);
+    if (!T->offs_valid) {  // first call: set the length, full pass that keeps normalisers
+        if ((rc = set_lengths(T, &node, &t, 1))) return rc;
+        return run_pass(T, 3, 0, logL, stream);
+    }
+    if ((rc = check_range(T, &t, 1))) return rc;
+    // asynchronous on the stream: new length, its P(t), the root path, the total
+    hipStream_t s = stream ? (hipStream_t)stream : T->stream;
+    T->t[node] = t;
+    TT_HIP(hipMemcpyAsync(T->d_t + node, &T->t[node], sizeof(double), hipMemcpyHostToDevice, s));
+    TT_HIP(ttk::expqt(T->n, T->d_t + node, T->d_all_branches + node, 1, T->d_eig, T->d_v, T->d_vinv, T->mu,
+                      (double *)T->d.P64, (double *)T->d.logP64, (float *)T->d.P32, (float *)T->d.D32, s));
+    TT_HIP(ttk::path_update(T->d, node, s));
+    TT_HIP(ttk::final_sum(T->d, ttk::root_blocks(T->P), s));
+    T->last_pass = 3;
+    if (logL) {
+        TT_HIP(hipMemcpyAsync(logL, T->d.total, sizeof(double), hipMemcpyDeviceToHost, s));
+        TT_HIP(hipStreamSynchronize(s));
+    }
+    return TT_OK;
+}
+
+int32_t tt_tree_trim(tt_tree *T) {
+    if (!T) return fail(TT_EINVAL, "tree is NULL");
+    if (!T->joint_ready) return TT_OK;
+    TT_HIP(hipSetDevice(T->eng->device));
+    TT_HIP(hipStreamSynchronize(T->stream));
+    TT_HIP(hipDeviceSynchronize());
+    for (auto it = T->graphs.begin(); it != T->graphs.end();) {  // joint graphs hold the freed pointers
+        if (it->first / 8 == 2) {
+            (void)hipGraphExecDestroy(it->second);
+            it = T->graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    for (void *q : {(void *)T->d.Lx, (void *)T->d.Cx}) {
+        for (size_t i = 0; i < T->allocs.size(); ++i)
+            if (T->allocs[i].first == q) {
+                (void)hipFree(q);
+                T->eng->used -= T->allocs[i].second;
+                T->bytes -= T->allocs[i].second;
+                T->allocs.erase(T->allocs.begin() + i);
+                break;
+            }
+    }
+    T->d.Lx = nullptr;
+    T->d.Cx = nullptr;
+    T->joint_ready = false;
+    if (T->last_pass == 2) T->last_pass = 0;
+    return TT_OK;
+}
+
+int32_t tt_tree_query(tt_tree *T) {
+    if (!T) return fail(TT_EINVAL, "tree is NULL");
+    hipError_t e = hipStreamQuery(T->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return fail(TT_EDEVICE, "hipStreamQuery: %s", hipGetErrorString(e));
+}
+
+int32_t tt_get_total(tt_tree *T, double *logL) {
+    if (!T || !logL) return fail(TT_EINVAL, "NULL argument");
+    if (!T->last_pass) return fail(TT_ESTATE, "no pass has run on this tree");
+    TT_HIP(hipSetDevice(T->eng->device));
+    TT_HIP(hipStreamSynchronize(T->stream));
+    TT_HIP(hipMemcpy(logL, T->d.total, sizeof(double), hipMemcpyDeviceToHost));
+    return TT_OK;
 }
 
 int32_t tt_get_sequence_lh(tt_tree *T, double *out) {

@@ -56,6 +56,7 @@ struct TreeDev {
     double TT_G *seqlh;              // [P]
     double TT_G *bpart;              // [nblocks]       block partials of sum(mult * seqlh)
     double TT_G *total;              // [1]
+    double TT_G *offs;               // [n_int][P] per-node log normaliser, written only in update mode
 };
 
 // One level of a level-scheduled pass.
@@ -79,6 +80,7 @@ hipError_t pre_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s);
 hipError_t post_joint(const TreeDev &T, const LevelDev &L, int store_tip_cx, hipStream_t s);
 hipError_t root_joint(const TreeDev &T, int S, int store_tip_cx, hipStream_t s);
 hipError_t backtrack(const TreeDev &T, const LevelDev &L, hipStream_t s);
+hipError_t path_update(const TreeDev &T, int node, hipStream_t s);
 hipError_t fill_states(const TreeDev &T, const int32_t *nodes, int n_nodes, uint8_t value, hipStream_t s);
 hipError_t propagate(const double *in, int P, int n, const double *Q, int transpose, int ret_log, double *out,
                      hipStream_t s);

@@ -190,6 +190,7 @@ __device__ __forceinline__ void post_node(const TreeDev &T, int v, int p, LP lut
 #pragma unroll
     for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : expf(L[j] - Lm);
     // normaliser at the selected state, children revisited (cache hits)
+    double lnode = 0.0;
     for (int e = e0; e < e1; ++e) {
         const int cref = wave_uniform(T.child_ref[e]);
         const int cnode = wave_uniform(T.child_node[e]);
@@ -204,7 +205,9 @@ __device__ __forceinline__ void post_node(const TreeDev &T, int v, int p, LP lut
                 dj = d[j];
             }
         acc.mul(msg64(rj, dj));
+        if (T.offs) lnode += log(msg64(rj, dj));
     }
+    if (T.offs) T.offs[(size_t)v * T.P + p] = lnode;
     store_row<N>(T.sub + ((size_t)v * T.P + p) * N, out);
 }
 
@@ -262,7 +265,9 @@ __device__ __forceinline__ void bin_finish(const TreeDev &T, int v, int e0, int 
             r1 = r[1][j];
             d1 = d[1][j];
         }
-    acc.mul(msg64(r0, d0) * msg64(r1, d1));
+    const double f = msg64(r0, d0) * msg64(r1, d1);
+    acc.mul(f);
+    if (T.offs) T.offs[(size_t)v * T.P + p] = log(f);  // update mode: keep the node's own normaliser
     store_post<N>(T, v, p, x, jm, xm);
 }
 

@@ -270,6 +270,42 @@ __global__ __launch_bounds__(TT_BLOCK) void k_backtrack(TreeDev T, LevelDev L, i
     T.states[(size_t)c * T.P + p] = T.Cx[((size_t)c * T.P + p) * T.n + ps];
 }
 
+// Branch re-evaluation along one root path (update mode).  Starting at the
+// changed branch's parent, every internal node up to the root is recomputed
+// from its children (the changed one was just rewritten by this thread), and
+// sequence_LH moves by the change of those nodes' log normalisers and of the
+// root term.  Equals a full postorder + root (treeanc.py:840-878, 814-829)
+// for a tree whose other rows are current.
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_path_update(TreeDev T, int node) {
+    const int p = blockIdx.x * TT_BLOCK + threadIdx.x;
+    double contrib = 0.0;
+    if (p < T.P) {
+        auto root_term = [&]() {
+            float r[N];
+            load_rowf<N>(T.sub + ((size_t)T.root_row * T.P + p) * N, r);
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) s += T.pi[j] * (double)r[j];
+            return log(s);
+        };
+        const double root_old = root_term();
+        double dl = 0.0;
+        LogAcc unused;
+        for (int a = wave_uniform(T.parent[node]); a >= 0; a = wave_uniform(T.parent[a])) {
+            const int v = wave_uniform(T.int_row[a]);
+            const double old = T.offs[(size_t)v * T.P + p];
+            post_pair<N>(T, v, v, false, p, T.lut, unused);
+            dl += T.offs[(size_t)v * T.P + p] - old;
+        }
+        const double lp = T.seqlh[p] + dl + (root_term() - root_old);
+        T.seqlh[p] = lp;
+        contrib = T.mult[p] * lp;
+    }
+    const double b = block_sum(contrib);
+    if (threadIdx.x == 0) T.bpart[blockIdx.x] = b;
+}
+
 __global__ void k_fill_states(TreeDev T, const int32_t *nodes, int n_nodes, uint8_t value) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)n_nodes * T.P) return;
@@ -408,6 +444,11 @@ hipError_t backtrack(const TreeDev &T, const LevelDev &L, hipStream_t s) {
     if (L.n_nodes <= 0) return hipSuccess;
     const int npt = n_ptiles(T.P);
     hipLaunchKernelGGL(k_backtrack, dim3(grid_for_tasks((long)L.n_nodes * npt)), dim3(TT_BLOCK), 0, s, T, L, npt);
+    return hipGetLastError();
+}
+
+hipError_t path_update(const TreeDev &T, int node, hipStream_t s) {
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_path_update<N_>, dim3(root_blocks(T.P)), dim3(TT_BLOCK), 0, s, T, node));
     return hipGetLastError();
 }
 

@@ -87,6 +87,8 @@ class DeviceTree:
         h = C.c_void_p()
         _native.check(self.lib.tt_tree_create(engine.handle, C.byref(d), C.byref(h)))
         self.handle = h
+        self.t_cur = t.copy()        # current effective branch lengths (host mirror)
+        self.joint_ready = False     # joint buffers allocated on the device
         self.set_model(eigvals, v, v_inv, mu)
 
     def set_model(self, eigvals, v, v_inv, mu):
@@ -96,9 +98,21 @@ class DeviceTree:
         _native.check(self.lib.tt_tree_set_model(self.handle, _native.ptr(e, C.c_double), _native.ptr(vv, C.c_double),
                                                  _native.ptr(vi, C.c_double), float(mu)))
 
+    def nonroot_nodes(self):
+        return np.nonzero(self.parent >= 0)[0]
+
+    def path_len(self, node):
+        """Internal nodes from the branch's parent up to the root."""
+        k, a = 0, int(self.parent[node])
+        while a >= 0:
+            k += 1
+            a = int(self.parent[a])
+        return k
+
     def set_branch_lengths(self, nodes, t):
         nodes = _native.as_c(nodes, np.int32)
         t = _native.as_c(t, np.float64)
+        self.t_cur[nodes] = t
         _native.check(self.lib.tt_tree_set_branch_lengths(self.handle, _native.ptr(nodes, C.c_int32),
                                                           _native.ptr(t, C.c_double), len(nodes)))
 
@@ -117,6 +131,7 @@ class DeviceTree:
         out = C.c_double(np.nan)
         flags = _native.TT_RECONSTRUCT_TIPS if reconstruct_tips else 0
         _native.check(self.lib.tt_joint(self.handle, flags, C.byref(out) if want_logL else None, stream))
+        self.joint_ready = True
         return out.value
 
     def loglik(self, stream=None, want_logL=True):
@@ -124,9 +139,34 @@ class DeviceTree:
         _native.check(self.lib.tt_loglik(self.handle, C.byref(out) if want_logL else None, stream))
         return out.value
 
-    def update_branch(self, node, t, stream=None):
+    def update_branch(self, node, t, stream=None, want_logL=True):
+        """Set one branch length and re-evaluate logL along its root path."""
         out = C.c_double(np.nan)
-        _native.check(self.lib.tt_update_branch(self.handle, int(node), float(t), C.byref(out), stream))
+        _native.check(self.lib.tt_update_branch(self.handle, int(node), float(t),
+                                                C.byref(out) if want_logL else None, stream))
+        self.t_cur[int(node)] = t
+        return out.value
+
+    def trim(self):
+        """Free the joint-pass buffers (returned to the engine budget)."""
+        _native.check(self.lib.tt_tree_trim(self.handle))
+        self.joint_ready = False
+
+    def joint_bytes(self):
+        """Device bytes a joint pass allocates on first use."""
+        n_int = int(np.count_nonzero(np.bincount(self.parent[self.parent >= 0], minlength=self.n_nodes)))
+        return n_int * self.P * self.n * 8 + self.n_nodes * self.P * self.n
+
+    def query(self):
+        """True when the work queued on the tree's own stream has finished."""
+        rc = self.lib.tt_tree_query(self.handle)
+        if rc < 0:
+            _native.check(rc)
+        return rc == 1
+
+    def total(self):
+        out = C.c_double(np.nan)
+        _native.check(self.lib.tt_get_total(self.handle, C.byref(out)))
         return out.value
 
     def sequence_lh(self):
