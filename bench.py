@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workers", type=int, default=0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=1000.0, help="request arrivals per second")
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-pool", type=int, default=64)
     return ap.parse_args()
 
 
@@ -127,6 +131,33 @@ def cpu_baseline(preps, seconds):
     return done_u / dt, dt, used
 
 
+def serve_bench(eng, pool, args):
+    """Poisson request stream over a resident pool: 70% branch re-evaluation
+    (root path only), 20% marginal, 10% joint; HBM budget 200 GB.  The same
+    stream is replayed with every branch request recomputed from scratch."""
+    from treetime_amd.serve import Request, Server, make_stream, summarize
+
+    for t in pool:  # switch to update mode (keeps per-node normalisers)
+        node = int(t.nonroot_nodes()[0])
+        t.update_branch(node, float(t.t_cur[node]))
+    rng = np.random.default_rng(4000)
+    reqs = make_stream(pool, args.serve_requests, args.serve_rate, rng)
+
+    def replay(scratch):
+        rr = [Request(r.kind, r.tree, r.arrival, r.node, r.scale, r.rid) for r in reqs]
+        srv = Server(pool, engine=eng, slots=16, hbm_budget=200 * 10**9, scratch=scratch)
+        w0 = time.perf_counter()
+        done = srv.run(rr, realtime=True)
+        return summarize(done, time.perf_counter() - w0)
+
+    rp = replay(False)
+    sc = replay(True)
+    log(f"[bench] serving @ {args.serve_rate:.0f} req/s: p50 {rp['p50_ms']:.2f} ms p99 {rp['p99_ms']:.2f} ms "
+        f"({rp['throughput_req_s']:.0f} req/s); from scratch p50 {sc['p50_ms']:.2f} p99 {sc['p99_ms']:.2f}")
+    return {"pool": len(pool), "rate_req_s": args.serve_rate, "mix": "70% branch / 20% marginal / 10% joint",
+            "hbm_budget_GB": 200, "root_path": rp, "from_scratch": sc}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -202,6 +233,10 @@ def main():
     ms_plain = p0.elapsed_time(p1) / args.plain_steps
     log(f"[bench] plain: {ms_plain:.3f} ms/step")
 
+    serving = None
+    if not args.no_serve and len(trees) > 1:
+        serving = serve_bench(eng, trees[: args.serve_pool], args)
+
     cpu = None
     if not args.no_cpu:
         rate, secs, used = cpu_baseline(preps, args.cpu_seconds)
@@ -242,6 +277,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": int(B)},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     for t in trees:

@@ -557,6 +557,7 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
         {(void **)&d.part, (size_t)std::max(T->R_all, 1) * P * sizeof(double)},
         {(void **)&d.part2, (size_t)std::max(rowsum_slices(T->R_all), 1) * P * sizeof(double)},
         {(void **)&d.seqlh, (size_t)P * sizeof(double)},
+        {(void **)&d.jseqlh, (size_t)P * sizeof(double)},
         {(void **)&d.bpart, (size_t)ttk::root_blocks(P) * sizeof(double)},
         {(void **)&d.total, sizeof(double)},
         {(void **)&T->d_eig, TT_MAX_STATES * sizeof(double)},
@@ -763,7 +764,8 @@ int32_t tt_get_sequence_lh(tt_tree *T, double *out) {
     if (!T->last_pass) return fail(TT_ESTATE, "no pass has run on this tree");
     TT_HIP(hipSetDevice(T->eng->device));
     TT_HIP(hipDeviceSynchronize());
-    TT_HIP(hipMemcpy(out, T->d.seqlh, (size_t)T->P * sizeof(double), hipMemcpyDeviceToHost));
+    TT_HIP(hipMemcpy(out, T->last_pass == 2 ? T->d.jseqlh : T->d.seqlh, (size_t)T->P * sizeof(double),
+                     hipMemcpyDeviceToHost));
     return TT_OK;
 }
 

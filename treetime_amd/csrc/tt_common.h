@@ -53,7 +53,8 @@ struct TreeDev {
     uint8_t TT_G *Cx;                // [n_nodes][P][n] joint argmax table
     double TT_G *part;               // [R][P]          per-chunk-group FP64 log-normaliser sums
     double TT_G *part2;              // [S][P]          row-slice sums
-    double TT_G *seqlh;              // [P]
+    double TT_G *seqlh;              // [P]  marginal per-pattern logL (root-path updates start here)
+    double TT_G *jseqlh;             // [P]  joint per-pattern logL
     double TT_G *bpart;              // [nblocks]       block partials of sum(mult * seqlh)
     double TT_G *total;              // [1]
     double TT_G *offs;               // [n_int][P] per-node log normaliser, written only in update mode
@@ -77,6 +78,8 @@ hipError_t rowsum(const TreeDev &T, int R, int S, hipStream_t s);
 hipError_t root_marginal(const TreeDev &T, int S, hipStream_t s);
 hipError_t final_sum(const TreeDev &T, int nblocks, hipStream_t s);
 hipError_t pre_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s);
+hipError_t post_marginal20(const TreeDev &T, const LevelDev &L, hipStream_t s);  // MFMA, n = 20
+hipError_t pre_marginal20(const TreeDev &T, const LevelDev &L, hipStream_t s);   // MFMA, n = 20
 hipError_t post_joint(const TreeDev &T, const LevelDev &L, int store_tip_cx, hipStream_t s);
 hipError_t root_joint(const TreeDev &T, int S, int store_tip_cx, hipStream_t s);
 hipError_t backtrack(const TreeDev &T, const LevelDev &L, hipStream_t s);
@@ -87,6 +90,7 @@ hipError_t propagate(const double *in, int P, int n, const double *Q, int transp
 hipError_t prob_t_profiles(const double *pp, const double *pc, const double *mult, int P, int n, const double *Q,
                            int K, int gap, double *block_out, int nblk, hipStream_t s);
 int root_blocks(int P);
+bool mfma_enabled();
 }  // namespace ttk
 
 // A tree as seen by the persistent batched kernel: the tree buffers plus its

@@ -22,6 +22,8 @@
 // the scalar unit.
 #include "tt_device.h"
 
+#include <cstdlib>
+
 namespace {
 using namespace ttd;
 
@@ -122,6 +124,7 @@ __device__ __forceinline__ void tip_joint(const TreeDev &T, int cref, int cnode,
     const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
     const auto ll = T.loglut + code * N;
     const auto LP = T.logP64 + (size_t)cnode * (N * N);
+    const auto cxo = T.Cx + ((size_t)cnode * T.P + p) * N;
     double tl[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) tl[i] = ll[i];
@@ -139,9 +142,15 @@ __device__ __forceinline__ void tip_joint(const TreeDev &T, int cref, int cnode,
             }
         }
         msg[j] += best;
-        cx[j] = bi;
+        if constexpr (N > 8) {
+            if (store_cx) cxo[j] = (uint8_t)bi;  // large alphabets: no packed argmax row in registers
+        } else {
+            cx[j] = bi;
+        }
     }
-    if (store_cx) store_bytes<N>(T.Cx + ((size_t)cnode * T.P + p) * N, cx);
+    if constexpr (N <= 8) {
+        if (store_cx) store_bytes<N>(cxo, cx);
+    }
 }
 
 template <int N>
@@ -162,7 +171,10 @@ __device__ __forceinline__ void joint_children(const TreeDev &T, int v, int p, d
     }
 }
 
-// Joint postorder step for internal non-root row v with node id vn.
+// Joint postorder step for internal non-root row v with node id vn.  The
+// stored message is shifted by an offset (its maximum for small alphabets,
+// its first entry for large ones, where rows are written out state by state
+// to keep register pressure low); the offset goes to the FP64 normaliser sum.
 template <int N>
 __device__ __forceinline__ double post_joint_node(const TreeDev &T, int v, int vn, int p, int store_tip_cx) {
     double msg[N];
@@ -170,31 +182,53 @@ __device__ __forceinline__ double post_joint_node(const TreeDev &T, int v, int v
     for (int i = 0; i < N; ++i) msg[i] = 0.0;
     joint_children<N>(T, v, p, msg, store_tip_cx);
     const auto LP = T.logP64 + (size_t)vn * (N * N);
-    double best[N];
-    int cx[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        double b = LP[j] + msg[0];
-        int bi = 0;
-#pragma unroll
-        for (int i = 1; i < N; ++i) {
-            const double val = LP[i * N + j] + msg[i];
-            if (val > b) {
-                b = val;
-                bi = i;
-            }
-        }
-        best[j] = b;
-        cx[j] = bi;
-    }
-    double off = best[0];
-#pragma unroll
-    for (int j = 1; j < N; ++j) off = fmax(off, best[j]);
     const auto dst = T.Lx + ((size_t)v * T.P + p) * N;
+    const auto cxo = T.Cx + ((size_t)vn * T.P + p) * N;
+    if constexpr (N > 8) {
+        double off = 0.0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) dst[j] = best[j] - off;
-    store_bytes<N>(T.Cx + ((size_t)vn * T.P + p) * N, cx);
-    return off;
+        for (int j = 0; j < N; ++j) {
+            double b = LP[j] + msg[0];
+            int bi = 0;
+#pragma unroll
+            for (int i = 1; i < N; ++i) {
+                const double val = LP[i * N + j] + msg[i];
+                if (val > b) {
+                    b = val;
+                    bi = i;
+                }
+            }
+            if (j == 0) off = b;
+            dst[j] = b - off;
+            cxo[j] = (uint8_t)bi;
+        }
+        return off;
+    } else {
+        double best[N];
+        int cx[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            double b = LP[j] + msg[0];
+            int bi = 0;
+#pragma unroll
+            for (int i = 1; i < N; ++i) {
+                const double val = LP[i * N + j] + msg[i];
+                if (val > b) {
+                    b = val;
+                    bi = i;
+                }
+            }
+            best[j] = b;
+            cx[j] = bi;
+        }
+        double off = best[0];
+#pragma unroll
+        for (int j = 1; j < N; ++j) off = fmax(off, best[j]);
+#pragma unroll
+        for (int j = 0; j < N; ++j) dst[j] = best[j] - off;
+        store_bytes<N>(cxo, cx);
+        return off;
+    }
 }
 
 template <int N>
@@ -248,7 +282,7 @@ __global__ __launch_bounds__(TT_BLOCK) void k_root_joint(TreeDev T, int S, int s
         }
         double lp = mx;
         for (int k = 0; k < S; ++k) lp += T.part2[(size_t)k * T.P + p];
-        T.seqlh[p] = lp;
+        T.jseqlh[p] = lp;
         T.states[(size_t)T.root * T.P + p] = (uint8_t)jm;
         contrib = T.mult[p] * lp;
     }
@@ -392,6 +426,16 @@ namespace ttk {
 
 int root_blocks(int P) { return (P + TT_BLOCK - 1) / TT_BLOCK; }
 
+// The 20-state passes use the MFMA kernels unless TT_NO_MFMA=1 (then the
+// VALU template instance runs; both are parity-tested).
+bool mfma_enabled() {
+    static bool v = [] {
+        const char *e = getenv("TT_NO_MFMA");
+        return !(e && e[0] == '1');
+    }();
+    return v;
+}
+
 hipError_t expqt(int n, const double *t, const int32_t *branches, int k, const double *eig, const double *v,
                  const double *vinv, double mu, double *P64, double *logP64, float *P32, float *D32, hipStream_t s) {
     if (k <= 0) return hipSuccess;
@@ -402,6 +446,7 @@ hipError_t expqt(int n, const double *t, const int32_t *branches, int k, const d
 }
 
 hipError_t post_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s) {
+    if (T.n == 20 && mfma_enabled()) return post_marginal20(T, L, s);
     const int npt = n_ptiles(T.P);
     const int groups = (L.n_chunks + 3) / 4;
     TT_DISPATCH(T.n, hipLaunchKernelGGL(k_post_marginal<N_>, dim3(groups * npt), dim3(TT_BLOCK), 0, s, T, L, npt));
@@ -428,6 +473,7 @@ hipError_t final_sum(const TreeDev &T, int nblocks, hipStream_t s) {
 
 hipError_t pre_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s) {
     if (L.n_nodes <= 0) return hipSuccess;
+    if (T.n == 20 && mfma_enabled()) return pre_marginal20(T, L, s);
     const int npt = n_ptiles(T.P);
     TT_DISPATCH(T.n, hipLaunchKernelGGL(k_pre_marginal<N_>, dim3(grid_for_tasks((long)((L.n_nodes + 1) / 2) * npt)),
                                         dim3(TT_BLOCK), 0, s, T, L, npt));

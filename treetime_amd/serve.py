@@ -13,9 +13,11 @@ Scheduling:
 * at most ``slots`` requests are in flight; a request that arrives while
   others run is admitted as soon as a slot and its memory are free;
 * memory admission: a joint request allocates the tree's joint buffers on
-  first use; it is admitted only if the engine's budget has room, and the
-  buffers are returned (``trim``) when it completes.  A request that can
-  never fit fails alone; the others proceed.
+  first use; it is admitted only if the engine's budget has room.  When it
+  has not, joint buffers of idle trees are returned (``trim``), least
+  recently used first; if that is still not enough the request waits for
+  running requests, and a request that can never fit fails alone while the
+  others proceed.
 
 ``scratch=True`` recomputes a branch request with a full postorder instead
 of the root path (the baseline the root-path update is measured against).
@@ -57,6 +59,7 @@ class Server:
         self.hbm_budget = hbm_budget
         self.scratch = scratch
         self.clock = clock
+        self.last_use = [0.0] * len(self.trees)
 
     # -- memory ------------------------------------------------------------
     def _used(self):
@@ -72,6 +75,16 @@ class Server:
         if self.hbm_budget is None:
             return True
         return self._used() + in_flight_need + self._need(req) <= self.hbm_budget
+
+    def _evict(self, req, in_flight_need, inflight):
+        """Trim joint buffers of idle trees (LRU) until ``req`` fits."""
+        idle = [k for k, t in enumerate(self.trees)
+                if getattr(t, "joint_ready", False) and k not in inflight and k != req.tree]
+        for k in sorted(idle, key=lambda k: self.last_use[k]):
+            if self._fits(req, in_flight_need):
+                break
+            self.trees[k].trim()
+        return self._fits(req, in_flight_need)
 
     # -- launch / complete ---------------------------------------------------
     def _launch(self, req):
@@ -100,10 +113,7 @@ class Server:
             raise ValueError("unknown request kind %r" % req.kind)
 
     def _complete(self, req):
-        tree = self.trees[req.tree]
-        req.result = tree.total()
-        if req.kind == "joint" and self.hbm_budget is not None:
-            tree.trim()
+        req.result = self.trees[req.tree].total()
 
     # -- main loop -------------------------------------------------------------
     def run(self, requests, realtime=True):
@@ -132,7 +142,7 @@ class Server:
                     break
                 if r.tree in blocked:
                     continue
-                if not self._fits(r, need_inflight):
+                if not self._fits(r, need_inflight) and not self._evict(r, need_inflight, inflight):
                     if not inflight and self._need(r) + self._used() > (self.hbm_budget or 0):
                         pending.remove(r)      # can never fit: fail this request alone
                         r.error = "rejected: exceeds the HBM budget"
@@ -142,6 +152,7 @@ class Server:
                     continue
                 pending.remove(r)
                 r.t_admit = self.clock() - t0
+                self.last_use[r.tree] = r.t_admit
                 try:
                     self._launch(r)
                     inflight[r.tree] = r
