@@ -49,6 +49,10 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="request arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-pool", type=int, default=64)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES between two device buffers 5 times (HBM counter calibration)")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "batch_marginal_mix256_traffic.md"),
+                    help="measured HBM bytes per launch of the batched kernel (from rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -158,8 +162,38 @@ def serve_bench(eng, pool, args):
             "hbm_budget_GB": 200, "root_path": rp, "from_scratch": sc}
 
 
+def copy_probe(nbytes):
+    """A plain device-to-device copy of known size: what FETCH_SIZE and
+    WRITE_SIZE read for it scales the counters of the batched kernel."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] copied {nbytes} bytes 5 times")
+    return 0
+
+
+def measured_traffic(path, workload):
+    """HBM bytes per launch recorded for ``workload`` in a profile summary
+    (line 'traffic_bytes_per_launch <workload> <bytes>'), else None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == "traffic_bytes_per_launch" and w[1] == workload:
+                    return float(w[2])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     args = parse()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described, not built",
                           "n_gpus": args.gpus}))
@@ -180,7 +214,7 @@ def main():
 
     import torch
 
-    from treetime_amd.engine import Engine, batch_marginal
+    from treetime_amd.engine import Engine, batch_info, batch_marginal
     from treetime_amd.prepare import device_tree
 
     eng = Engine(0, 0)
@@ -199,6 +233,8 @@ def main():
     batch_logl = batch_marginal(eng, trees, stream=sp)
     rel = float(np.max(np.abs(batch_logl - plain_logl) / np.abs(plain_logl)))
     log(f"[bench] batched vs plain logL max rel diff {rel:.2e}")
+    shape = batch_info(eng, trees)
+    log(f"[bench] autotuned launch shape {shape}")
     small = int(np.argmin([updates(p) for p in preps]))
     ref = oracle_marginal(preps[small])
     err_fp64 = abs(plain_logl[small] - ref) / abs(ref)
@@ -220,18 +256,20 @@ def main():
     log(f"[bench] batched: {ms:.3f} ms/step (events), wall {wall / args.steps * 1e3:.3f} ms/step")
 
     # plain: one tree per launch sequence, trees in sequence
-    for t in trees:
-        t.marginal(stream=sp, want_logL=False)
-    torch.cuda.synchronize()
-    p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    p0.record(stream)
-    for _ in range(args.plain_steps):
+    ms_plain = float("nan")
+    if args.plain_steps > 0:
         for t in trees:
             t.marginal(stream=sp, want_logL=False)
-    p1.record(stream)
-    torch.cuda.synchronize()
-    ms_plain = p0.elapsed_time(p1) / args.plain_steps
-    log(f"[bench] plain: {ms_plain:.3f} ms/step")
+        torch.cuda.synchronize()
+        p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        p0.record(stream)
+        for _ in range(args.plain_steps):
+            for t in trees:
+                t.marginal(stream=sp, want_logL=False)
+        p1.record(stream)
+        torch.cuda.synchronize()
+        ms_plain = p0.elapsed_time(p1) / args.plain_steps
+        log(f"[bench] plain: {ms_plain:.3f} ms/step")
 
     serving = None
     if not args.no_serve and len(trees) > 1:
@@ -258,23 +296,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32",
         "data": "synthetic",
         "config": {
             "workload": wname,
             "jobs": len(preps),
             "patterns_total": int(P_total),
             "updates_per_step": int(U),
-            "storage": "f32 rows, f64 arithmetic and normaliser sums",
+            "precision": "f32 rows, messages and preorder; f64 normaliser products, root and final sum",
             "plain_ms_per_step": ms_plain,
             "plain_value": U / (ms_plain / 1e3),
             "batched_vs_plain": ms_plain / ms,
             "batched_vs_plain_logL_max_rel_diff": rel,
             "logL_rel_err_vs_fp64_oracle": err_fp64,
             "wall_ms_per_step": wall / args.steps * 1e3,
+            "launch_shape": shape,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.traffic_file, wname),
                      "algorithmic_bytes_per_launch": int(B)},
         "cpu_baseline": cpu,
         "serving": serving,

@@ -112,6 +112,29 @@ int32_t tt_joint(tt_tree *tree, uint32_t flags, double *logL, void *stream);
  * the root path.  Profiles and states of earlier passes become stale. */
 int32_t tt_update_branch(tt_tree *tree, int32_t node, double t, double *logL, void *stream);
 
+/* tt_optimize_branch_lengths: one sweep of joint branch-length optimisation
+ * over every branch, from the states of the last marginal or joint
+ * reconstruction.  Replaces TreeAnc.optimize_branch_lengths_joint /
+ * optimize_branch_len (treeanc.py:1168-1242) with, per branch,
+ * optimal_branch_length (:1244-1270), add_branch_state (:1148-1163),
+ * GTR.state_pair (gtr.py:631-712) and GTR.optimal_t_compressed (:816-920).
+ *   code_state[256]: alphabet index of each character code, used for tips
+ *       whose states were not reconstructed; -1 = not counted (the reference
+ *       skips characters outside the alphabet for n < 10 and counts them as
+ *       state 1 for n >= 10 -- the caller encodes that).
+ *   gap_state: state excluded from the counts (ignore_gaps), -1 for none.
+ *   max_t, tol: the Brent bracket is (-sqrt(max_t), sqrt(hamming),
+ *       sqrt(max_t)) in s = sqrt(t) (config.py MAX_BRANCH_LENGTH), tol 1e-10.
+ * Outputs (host, [n_nodes], root entries 0): t_opt = s*s at the minimum;
+ * status 0 ok, 1 the bracket condition fails (scipy raises ValueError in the
+ * reference; t_opt NaN), 2 no convergence (t_opt = Hamming distance, as the
+ * reference falls back); pair_counts (nullable) [n_nodes][n][n] FP64
+ * multiplicity-weighted (parent, child) counts before truncation.
+ * Branch lengths of the tree are not changed (call
+ * tt_tree_set_branch_lengths with the accepted values). */
+int32_t tt_optimize_branch_lengths(tt_tree *tree, const int16_t *code_state, int32_t gap_state, double max_t,
+                                   double tol, double *t_opt, int32_t *status, double *pair_counts, void *stream);
+
 /* Free the joint-pass buffers (Lx, Cx) and return them to the engine's
  * budget; the next tt_joint allocates them again.  No reference counterpart:
  * it lets a server admit requests against an HBM budget. */
@@ -155,10 +178,22 @@ int32_t tt_prob_t_profiles(const double *pp_dev, const double *pc_dev, const dou
 
 /* ---- batching --------------------------------------------------------------
  * Marginal reconstruction of k resident trees in ONE persistent launch
- * driven by a device work queue of (tree, level, pattern-tile) items
- * (per tree the result equals tt_marginal).  logL is host [k]. */
+ * driven by a device work queue of (tree, pattern-tile) items; each item
+ * walks its tile through every level of its tree (per tree the result
+ * equals tt_marginal: states and profiles identical, logL to rounding).
+ * logL is host [k].  No reference counterpart (the reference runs one tree
+ * at a time, treeanc.py:762-932).  The first call for a list of trees and
+ * flags builds a plan: items longest tree first, the launch shape chosen by
+ * timing the candidates (autotuner), one captured graph. */
 int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, double *logL,
                           void *stream);
+
+/* Launch shape the autotuner chose for the plan of (trees, flags): one wave
+ * (1) or one 4-wave block (0) per item, resident blocks per CU, and the
+ * measured time per launch while tuning (0 if not timed).  TT_ESTATE when
+ * no plan exists yet.  No reference counterpart. */
+int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, int32_t *wave_items,
+                      int32_t *blocks_per_cu, double *tuned_ms);
 
 #ifdef __cplusplus
 }

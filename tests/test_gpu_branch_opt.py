@@ -1,0 +1,155 @@
+"""Joint branch-length optimisation on the GPU (tt_optimize_branch_lengths)
+against the oracle's state_pair + optimal_t_compressed (scipy Brent) per
+branch, and through the TreeAnc facade.
+
+Tolerances: state-pair counts are exact (integer-valued multiplicities).
+The optimum's position is ill-conditioned: near it f(s) - f* ~ f''/2 ds^2, so
+FP64 rounding of f (~1e-16 |f|, here summation order) hides ds up to
+~sqrt(2e-16 |f| / f''), a few 1e-9 in s for one mutation in 800 sites --
+both Brent searches stop anywhere in that flat window.  So the objective
+values at the two optima must agree to 1e-12 relative (both are optimal to
+FP64 precision), and the lengths to 1e-6 relative + 1e-10 absolute."""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import oracle_gtr, small_job
+from treetime_amd.prepare import device_tree
+
+pytestmark = pytest.mark.gpu
+
+T_RTOL, T_ATOL = 1e-6, 1e-10
+F_RTOL = 1e-12
+
+
+def same_optimum(og, pairs, mult, t_dev, t_ref):
+    """Objective (gtr.py:714-747) at both optima agrees to F_RTOL."""
+    if not len(pairs):
+        return True
+    f_dev = -og.prob_t_compressed(pairs, mult, t_dev, return_log=True)
+    f_ref = -og.prob_t_compressed(pairs, mult, t_ref, return_log=True)
+    return abs(f_dev - f_ref) <= F_RTOL * max(1.0, abs(f_ref))
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def code_states(alphabet_chars):
+    other = -1 if len(alphabet_chars) < 10 else 1
+    out = np.full(256, other, dtype=np.int16)
+    for i, a in enumerate(alphabet_chars):
+        out[ord(a)] = i
+    return out
+
+
+def oracle_sweep(prep, job, alphabet, states, tips_reconstructed):
+    og = oracle_gtr(job["gtr"], alphabet)
+    alpha = np.array([str(a) for a in og.alphabet])
+    parent = prep["parent"]
+    n = len(parent)
+    is_tip = np.ones(n, dtype=bool)
+    is_tip[parent[parent >= 0]] = False
+    tip_rank = np.cumsum(is_tip) - 1
+    chars = prep["tip_codes"].view("S1").astype("U1")
+
+    def seq(v):
+        if is_tip[v] and not tips_reconstructed:
+            return chars[tip_rank[v]]
+        return alpha[states[v]]
+
+    out = {}
+    for v in range(n):
+        if parent[v] < 0:
+            continue
+        pairs, mult = og.state_pair(seq(parent[v]), seq(v), pattern_multiplicity=prep["multiplicity"])
+        cnt = np.zeros((og.n_states, og.n_states))
+        if len(pairs):
+            cnt[pairs[:, 0], pairs[:, 1]] = mult
+        out[v] = (og.optimal_t_compressed(pairs, mult), cnt, pairs, mult)
+    return out, alpha, og
+
+
+@pytest.mark.parametrize("n_tips,L,n_states,seed,joint,tips", [
+    (60, 800, 4, 21, True, False),
+    (60, 800, 4, 22, False, False),
+    (40, 500, 4, 23, True, True),
+    (30, 300, 20, 24, True, False),
+])
+def test_sweep_matches_oracle(gpu_engine, n_tips, L, n_states, seed, joint, tips):
+    say(f"[branch-opt] tips={n_tips} L={L} n={n_states} seed={seed} joint={joint} reconstruct_tips={tips}")
+    job, prep, alphabet = small_job(n_tips, L, n_states, seed, depth=0.05)
+    dev = device_tree(gpu_engine, prep)
+    if joint:
+        dev.joint(reconstruct_tips=tips)
+    else:
+        dev.marginal(reconstruct_tips=tips)
+    states = dev.states()
+    ref, alpha, og = oracle_sweep(prep, job, alphabet, states, tips)
+    t, st, cnt = dev.optimize_branch_lengths(code_states(list(alpha)), -1, 4.0, 1e-10)
+    worst = 0.0
+    for v, (t_ref, c_ref, pairs, mult) in ref.items():
+        assert st[v] == 0, (v, st[v])
+        assert np.array_equal(np.trunc(cnt[v]), c_ref), v
+        assert same_optimum(og, pairs, mult, t[v], t_ref), (v, t[v], t_ref)
+        err = abs(t[v] - t_ref)
+        if t_ref > 1e-6:
+            worst = max(worst, err / t_ref)
+        assert err <= T_RTOL * t_ref + T_ATOL, (v, t[v], t_ref)
+    say(f"  {len(ref)} branches, max rel diff of optimal t {worst:.2e} (t > 1e-6)")
+    dev.close()
+
+
+def test_sweep_requires_reconstruction(gpu_engine):
+    from treetime_amd.errors import MissingDataError
+
+    job, prep, alphabet = small_job(20, 200, 4, 31)
+    dev = device_tree(gpu_engine, prep)
+    dev.loglik()
+    with pytest.raises(MissingDataError):
+        dev.optimize_branch_lengths(code_states(list("ACGT")))
+    dev.close()
+
+
+def test_facade_optimize_matches_oracle():
+    """TreeAnc.optimize_branch_len after a joint reconstruction equals the
+    per-branch oracle optimum (treeanc.py:1176-1270), and branch_state holds
+    the oracle's state-pair counts."""
+    import treetime_amd as tt
+    from oracle import fp64 as O
+    from treetime_amd import synth
+
+    job = synth.make_job(60, 700, 0.05, 4, 43, 0.02, 0.01)
+    parent, bl = job["parent"], job["branch_length"]
+    clades = [tt.Clade(branch_length=float(bl[i])) for i in range(len(parent))]
+    for i, p in enumerate(parent):
+        if p >= 0:
+            clades[p].clades.append(clades[i])
+    for k, v in enumerate(job["tips"]):
+        clades[v].name = job["names"][k]
+    aln = {job["names"][k]: job["chars"][k].tobytes().decode() for k in range(len(job["tips"]))}
+    ta = tt.TreeAnc(tree=tt.Tree(clades[0]), aln=aln, gtr=job["gtr"], verbose=0)
+    ta.infer_ancestral_sequences(marginal=False)
+    seqs = {id(c): c.cseq.copy() for c in ta.tree.find_clades()}
+    assert ta.optimize_branch_len(store_old=True) == "success"
+    g = ta.gtr
+    og = O.GTRModel("nuc_nogap", mu=g.mu, pi=g.Pi, W=g.W)
+    mult = ta.data.multiplicity()
+    worst = 0.0
+    for c in ta.tree.find_clades(order="postorder"):
+        if c.up is None:
+            continue
+        pairs, m = og.state_pair(seqs[id(c.up)], seqs[id(c)], pattern_multiplicity=mult)
+        t_ref = max(0, og.optimal_t_compressed(pairs, m))
+        assert same_optimum(og, pairs, m, c.branch_length, t_ref), (c.name, c.branch_length, t_ref)
+        assert abs(c.branch_length - t_ref) <= T_RTOL * t_ref + T_ATOL, (c.name, c.branch_length, t_ref)
+        assert np.array_equal(c.branch_state["pair"], pairs)
+        assert np.array_equal(c.branch_state["multiplicity"], m)
+        assert c.mutation_length == c.branch_length
+        if t_ref > 1e-6:
+            worst = max(worst, abs(c.branch_length - t_ref) / t_ref)
+    say(f"[branch-opt] facade: max rel diff {worst:.2e} (t > 1e-6)")
+    # the new lengths are used by the next reconstruction
+    ta.infer_ancestral_sequences(marginal=True)
+    assert np.isfinite(ta.sequence_LH())

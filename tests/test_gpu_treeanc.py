@@ -162,3 +162,55 @@ def test_gtr_primitives_on_device():
         assert abs(got - ref) <= 1e-11 * max(1.0, abs(ref))
     many = g.prob_t_profiles((pp, pc), mult, [0.01, 0.02, 0.03], return_log=True)
     assert many.shape == (3,)
+
+
+def _sim_treeanc(seed, rng_seed):
+    from treetime_amd import synth
+
+    job = synth.make_job(40, 400, 0.05, 4, seed, 0.02, 0.01)
+    parent, bl = job["parent"], job["branch_length"]
+    clades = [tt.Clade(branch_length=float(bl[i])) for i in range(len(parent))]
+    for i, p in enumerate(parent):
+        if p >= 0:
+            clades[p].clades.append(clades[i])
+    for k, v in enumerate(job["tips"]):
+        clades[v].name = job["names"][k]
+    aln = {job["names"][k]: job["chars"][k].tobytes().decode() for k in range(len(job["tips"]))}
+    return tt.TreeAnc(tree=tt.Tree(clades[0]), aln=aln, gtr=job["gtr"], verbose=0, rng_seed=rng_seed)
+
+
+def test_sample_from_profile():
+    """sample_from_profile (treeanc.py:762-812, seq_utils.py:232-276): the
+    sampled characters have positive posterior, the same seed gives the same
+    sample, 'root' samples only the root, and one random number is drawn per
+    pattern and sampled node, root first, then in preorder."""
+    say("[treeanc] sampling from marginal profiles")
+    ta = _sim_treeanc(51, 7)
+    ta.infer_ancestral_sequences(marginal=True)
+    argmax = {id(c): c.cseq.copy() for c in ta.tree.find_clades() if not c.is_terminal()}
+    ta.infer_ancestral_sequences(marginal=True, sample_from_profile="root")
+    for c in ta.tree.find_clades():
+        if not c.is_terminal() and c.up is not None:
+            assert np.array_equal(c.cseq, argmax[id(c)])
+    ta.rng = np.random.default_rng(99)
+    ta.infer_ancestral_sequences(marginal=True, sample_from_profile=True)
+    alpha = list(ta.gtr.alphabet)
+    s1 = {}
+    n_sampled = 0
+    for c in ta.tree.find_clades():
+        if c.is_terminal():
+            continue
+        idx = np.array([alpha.index(x) for x in c.cseq])
+        pr = c.marginal_profile[np.arange(len(idx)), idx]
+        assert np.all(pr > 0)
+        s1[id(c)] = c.cseq.copy()
+        n_sampled += 1
+    # replay the draws from the same generator state
+    rng = np.random.default_rng(99)
+    for c in ta.tree.find_clades(order="preorder"):
+        if c.is_terminal():
+            continue
+        cum = np.cumsum(c.marginal_profile, axis=1).T
+        idx = np.argmax(cum >= rng.random(size=cum.shape[1]), axis=0)
+        assert np.array_equal(np.array(alpha)[idx], s1[id(c)])
+    say(f"  {n_sampled} nodes sampled")

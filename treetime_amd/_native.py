@@ -63,6 +63,7 @@ _PD = C.POINTER(C.c_double)
 _PF = C.POINTER(C.c_float)
 _PI32 = C.POINTER(C.c_int32)
 _PU8 = C.POINTER(C.c_uint8)
+_PI16 = C.POINTER(C.c_int16)
 
 # name -> (restype, argtypes); every symbol the header declares
 SIGNATURES = {
@@ -80,6 +81,7 @@ SIGNATURES = {
     "tt_loglik": (_I32, [_P, _PD, _P]),
     "tt_joint": (_I32, [_P, _U32, _PD, _P]),
     "tt_update_branch": (_I32, [_P, _I32, _D, _PD, _P]),
+    "tt_optimize_branch_lengths": (_I32, [_P, _PI16, _I32, _D, _D, _PD, _PI32, _PD, _P]),
     "tt_tree_trim": (_I32, [_P]),
     "tt_tree_query": (_I32, [_P]),
     "tt_get_total": (_I32, [_P, _PD]),
@@ -90,6 +92,7 @@ SIGNATURES = {
     "tt_propagate_profile": (_I32, [_P, _I32, _I32, _PD, _I32, _I32, _P, _P]),
     "tt_prob_t_profiles": (_I32, [_P, _P, _P, _I32, _I32, _PD, _I32, _I32, _PD, _P]),
     "tt_batch_marginal": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PD, _P]),
+    "tt_batch_info": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PI32, _PI32, _PD]),
 }
 
 _lib = None

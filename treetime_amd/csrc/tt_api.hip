@@ -88,6 +88,8 @@ struct BatchPlan {
     size_t bytes = 0;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
+    BatchShape shape{1, 1};
+    double tuned_ms = 0.0;  // time per launch of the chosen shape while tuning (0: not timed)
     ~BatchPlan() {
         if (graph) (void)hipGraphExecDestroy(graph);
         for (void *q : {(void *)d_trees, (void *)d_items, (void *)d_queue, (void *)d_item_out})
@@ -131,6 +133,10 @@ struct tt_tree {
     std::map<int, hipGraphExec_t> graphs;
     std::vector<std::pair<void *, size_t>> allocs;
     int last_pass = 0;  // 1 marginal, 2 joint, 3 loglik
+    bool have_states = false;  // d.states holds a marginal or joint reconstruction
+    int16_t *d_code_state = nullptr;  // branch optimisation buffers, allocated on first use
+    double *d_pair_counts = nullptr, *d_t_opt = nullptr;
+    int32_t *d_opt_status = nullptr;
     uint64_t bytes = 0;
 
     int32_t alloc(void **p, size_t nbytes) {
@@ -397,6 +403,7 @@ int32_t run_pass(tt_tree *T, int mode, uint32_t flags, double *logL, void *strea
         if (rc) return rc;
     }
     T->last_pass = mode;
+    if (mode == 1 || mode == 2) T->have_states = true;
     if (T->update_mode && (mode == 1 || mode == 3)) T->offs_valid = true;
     if (logL) {
         TT_HIP(hipMemcpyAsync(logL, T->d.total, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -435,6 +442,67 @@ int32_t check_range(tt_tree *T, const double *t, int k) {
                         "large (mu*t*lambda = %g)",
                         T->mu * t[i] * lmax);
     }
+    return TT_OK;
+}
+
+// Autotuner of the batched launch: every candidate shape (item = one wave or
+// one 4-wave block; resident blocks per CU 1, 2 or the occupancy limit) runs
+// once to warm up and twice timed with HIP events on the plan's own trees;
+// the fastest is kept in the plan.  TT_BATCH_MODE=block|wave and
+// TT_BATCH_BLOCKS_PER_CU=k restrict the candidates; TT_AUTOTUNE=0 skips the
+// timing and takes the first candidate (one block per item at the occupancy
+// limit).  Shapes differ only in how items are spread over the chip and in
+// the order the per-node normalisers of an item are multiplied, so states
+// and profiles are identical for every shape and logL agrees to rounding.
+int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
+    std::vector<BatchShape> cand;
+    const char *mode = getenv("TT_BATCH_MODE");
+    const char *bpc = getenv("TT_BATCH_BLOCKS_PER_CU");
+    const char *tune = getenv("TT_AUTOTUNE");
+    const bool log = getenv("TT_AUTOTUNE_LOG") != nullptr;  // print every candidate's time to stderr
+    for (int w : {0, 1}) {
+        if (mode && mode[0] && ((mode[0] == 'b') != (w == 0))) continue;
+        const int occ = ttk::batch_max_blocks_per_cu(n, w);
+        std::vector<int> per_cu;
+        if (bpc && atoi(bpc) > 0) {
+            per_cu.push_back(atoi(bpc));
+        } else {
+            per_cu.push_back(occ);
+            for (int x : {2, 1})
+                if (x < occ) per_cu.push_back(x);
+        }
+        for (int x : per_cu) cand.push_back({w, x});
+    }
+    if (cand.empty()) return fail(TT_EINVAL, "TT_BATCH_MODE=%s selects no launch shape", mode);
+    P->shape = cand[0];
+    P->tuned_ms = 0.0;
+    if (cand.size() == 1 || (tune && tune[0] == '0')) return TT_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TT_HIP(hipEventCreate(&e0));
+    TT_HIP(hipEventCreate(&e1));
+    double best = INFINITY;
+    hipError_t e = hipSuccess;
+    for (const BatchShape &c : cand) {
+        const int ni = (int)P->items.size();
+        e = ttk::batch_marginal(n, P->d_trees, P->d_items, ni, P->d_queue, P->d_item_out, tips, c, s);
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int r = 0; r < 2 && e == hipSuccess; ++r)
+            e = ttk::batch_marginal(n, P->d_trees, P->d_items, ni, P->d_queue, P->d_item_out, tips, c, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) break;
+        if (log) fprintf(stderr, "tt autotune: wave_items=%d blocks_per_cu=%d %.3f ms\n", c.wave_items, c.blocks_per_cu, ms / 2.0);
+        if (ms / 2.0 < best) {
+            best = ms / 2.0;
+            P->shape = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(TT_EDEVICE, "batch autotune: %s", hipGetErrorString(e));
+    P->tuned_ms = best;
     return TT_OK;
 }
 
@@ -711,6 +779,38 @@ int32_t tt_update_branch(tt_tree *T, int32_t node, double t, double *logL, void 
     return TT_OK;
 }
 
+int32_t tt_optimize_branch_lengths(tt_tree *T, const int16_t *code_state, int32_t gap_state, double max_t,
+                                   double tol, double *t_opt, int32_t *status, double *pair_counts, void *stream) {
+    if (!T || !code_state || !t_opt || !status) return fail(TT_EINVAL, "NULL argument");
+    if (!T->have_model) return fail(TT_ESTATE, "tt_tree_set_model has not been called");
+    if (!T->have_states) return fail(TT_ESTATE, "no marginal or joint reconstruction has run on this tree");
+    if (gap_state < -1 || gap_state >= T->n) return fail(TT_EINVAL, "gap_state %d outside [-1, %d)", gap_state, T->n);
+    if (!(max_t > 0.0) || !(tol >= 0.0)) return fail(TT_EINVAL, "need max_t > 0 and tol >= 0");
+    for (int c = 0; c < 256; ++c)
+        if (code_state[c] < -1 || code_state[c] >= T->n)
+            return fail(TT_EINVAL, "code_state[%d] = %d outside [-1, %d)", c, code_state[c], T->n);
+    TT_HIP(hipSetDevice(T->eng->device));
+    int32_t rc;
+    const size_t nn = (size_t)T->n * T->n;
+    if (!T->d_code_state) {
+        if ((rc = T->alloc((void **)&T->d_code_state, 256 * sizeof(int16_t)))) return rc;
+        if ((rc = T->alloc((void **)&T->d_pair_counts, (size_t)T->n_nodes * nn * sizeof(double)))) return rc;
+        if ((rc = T->alloc((void **)&T->d_t_opt, (size_t)T->n_nodes * sizeof(double)))) return rc;
+        if ((rc = T->alloc((void **)&T->d_opt_status, (size_t)T->n_nodes * sizeof(int32_t)))) return rc;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : T->stream;
+    TT_HIP(hipMemcpyAsync(T->d_code_state, code_state, 256 * sizeof(int16_t), hipMemcpyHostToDevice, s));
+    TT_HIP(ttk::branch_opt(T->d, T->d_code_state, gap_state, T->d_eig, T->d_v, T->d_vinv, T->mu, max_t, tol,
+                           T->d_pair_counts, T->d_t_opt, T->d_opt_status, s));
+    TT_HIP(hipMemcpyAsync(t_opt, T->d_t_opt, (size_t)T->n_nodes * sizeof(double), hipMemcpyDeviceToHost, s));
+    TT_HIP(hipMemcpyAsync(status, T->d_opt_status, (size_t)T->n_nodes * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (pair_counts)
+        TT_HIP(hipMemcpyAsync(pair_counts, T->d_pair_counts, (size_t)T->n_nodes * nn * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+    TT_HIP(hipStreamSynchronize(s));
+    return TT_OK;
+}
+
 int32_t tt_tree_trim(tt_tree *T) {
     if (!T) return fail(TT_EINVAL, "tree is NULL");
     if (!T->joint_ready) return TT_OK;
@@ -853,6 +953,21 @@ int32_t tt_prob_t_profiles(const double *pp_dev, const double *pc_dev, const dou
     return TT_OK;
 }
 
+int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, int32_t *wave_items,
+                      int32_t *blocks_per_cu, double *tuned_ms) {
+    if (!eng || (k > 0 && !trees)) return fail(TT_EINVAL, "NULL argument");
+    std::vector<tt_tree *> key(trees, trees + std::max(k, 0));
+    std::lock_guard<std::mutex> g(eng->mu);
+    for (BatchPlan *p : eng->plans)
+        if (p->flags == (flags & TT_RECONSTRUCT_TIPS) && p->trees == key) {
+            if (wave_items) *wave_items = p->shape.wave_items;
+            if (blocks_per_cu) *blocks_per_cu = p->shape.blocks_per_cu;
+            if (tuned_ms) *tuned_ms = p->tuned_ms;
+            return TT_OK;
+        }
+    return fail(TT_ESTATE, "no batch plan for these trees and flags");
+}
+
 int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, double *logL,
                           void *stream) {
     if (!eng || (k > 0 && !trees)) return fail(TT_EINVAL, "NULL argument");
@@ -924,13 +1039,17 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
             delete P;
             return fail(TT_EDEVICE, "batch plan set-up: %s", hipGetErrorString(e));
         }
+        if (int32_t rc = tune_batch(P, n, fl ? 1 : 0, P->stream)) {
+            delete P;
+            return rc;
+        }
         if (g_use_graphs()) {
             hipGraph_t g = nullptr;
             e = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal);
             hipError_t e2 = hipSuccess;
             if (e == hipSuccess)
                 e2 = ttk::batch_marginal(n, P->d_trees, P->d_items, (int)P->items.size(), P->d_queue,
-                                         P->d_item_out, fl ? 1 : 0, P->stream);
+                                         P->d_item_out, fl ? 1 : 0, P->shape, P->stream);
             if (e == hipSuccess) e = hipStreamEndCapture(P->stream, &g);
             if (e == hipSuccess) e = e2;
             if (e == hipSuccess) e = hipGraphInstantiate(&P->graph, g, nullptr, nullptr, 0);
@@ -949,9 +1068,12 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
         TT_HIP(hipGraphLaunch(plan->graph, s));
     } else {
         TT_HIP(ttk::batch_marginal(n, plan->d_trees, plan->d_items, (int)plan->items.size(), plan->d_queue,
-                                   plan->d_item_out, fl ? 1 : 0, s));
+                                   plan->d_item_out, fl ? 1 : 0, plan->shape, s));
     }
-    for (int i = 0; i < k; ++i) trees[i]->last_pass = 1;
+    for (int i = 0; i < k; ++i) {
+        trees[i]->last_pass = 1;
+        trees[i]->have_states = true;
+    }
     if (logL) {
         std::vector<double> out(plan->items.size());
         TT_HIP(hipMemcpyAsync(out.data(), plan->d_item_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -93,18 +93,77 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal(const BatchTreeDev 
     }
 }
 
+// Wave-granular variant: every wave pops its own item and walks all of the
+// tile's levels alone (program order replaces the block barriers, so sparse
+// deep levels cost one step instead of idling three waves at a barrier).
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int off = TT_WAVE / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, TT_WAVE);
+    return x;
+}
+
 template <int N>
-int persistent_grid() {
-    static int grid = [] {
-        int dev = 0, cus = 0, per_cu = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+__global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal_w(const BatchTreeDev *__restrict__ trees_arg,
+                                                               const BatchItem *__restrict__ items_arg,
+                                                               int n_items, int *queue, double *item_out, int tips) {
+    const BatchTreeDev TT_C *trees = (const BatchTreeDev TT_C *)trees_arg;
+    const BatchItem TT_C *items = (const BatchItem TT_C *)items_arg;
+    __shared__ float s_lut[kWaves][256 * N];
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    float *lut = s_lut[w];
+    int cur_tree = -1;
+    for (;;) {
+        int it = 0;
+        if (lane == 0) it = atomicAdd(queue, 1);
+        it = __shfl(it, 0, TT_WAVE);
+        if (it >= n_items) break;
+        const BatchItem item = items[it];
+        const BatchTreeDev TT_C &B = trees[item.tree];
+        const TreeDev TT_C &T = B.T;
+        if (item.tree != cur_tree) {  // stage the tree's profile map in this wave's LDS
+            for (int i = lane; i < 256 * N; i += TT_WAVE) lut[i] = T.lut[i];
+            cur_tree = item.tree;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int p = item.tile * TT_WAVE + lane;
+        const bool act = p < T.P;
+        LogAcc acc;
+        const int n_post = B.post_off[B.H];
+        for (int k = 0; k < n_post; k += 2) {  // level order: children before parents
+            const bool has1 = k + 1 < n_post;
+            const int v0 = wave_uniform(B.post_nodes[k]);
+            const int v1 = has1 ? wave_uniform(B.post_nodes[k + 1]) : v0;
+            if (act) post_pair<N>(T, v0, v1, has1, p, lut, acc);
+        }
+        double contrib = 0.0;
+        if (act) contrib = root_step<N>(T, p, acc.log_value());
+        const double tot = wave_sum(contrib);
+        if (lane == 0) item_out[it] = tot;
+        if (!tips && act)
+            for (int k = 0; k < T.n_tips; ++k) T.states[(size_t)B.tip_nodes[k] * T.P + p] = 255;
+        for (int d = 0; d < B.D; ++d) {  // depth order: parents before children
+            const int kb = B.pre_off[d];
+            const int ke = tips ? B.pre_off[d + 1] : kb + B.pre_nint[d];
+            for (int k = kb; k < ke; k += 2) {
+                const bool has1 = k + 1 < ke;
+                const int c0 = wave_uniform(B.pre_nodes[k]);
+                const int c1 = has1 ? wave_uniform(B.pre_nodes[k + 1]) : c0;
+                if (act) pre_pair<N>(T, c0, c1, has1, p, lut);
+            }
+        }
+    }
+}
+
+template <int N>
+int max_blocks_per_cu(int wave_items) {
+    int per_cu = 0;
+    if (wave_items)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal_w<N>, TT_BLOCK, 0);
+    else
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N>, TT_BLOCK, 0);
-        const char *e = getenv("TT_BATCH_BLOCKS_PER_CU");
-        if (e && atoi(e) > 0) per_cu = atoi(e);
-        return std::max(1, cus * std::max(2, per_cu));
-    }();
-    return grid;
+    return std::max(1, per_cu);
 }
 
 }  // namespace
@@ -121,14 +180,39 @@ int persistent_grid() {
         default: return hipErrorInvalidValue;            \
     }
 
+int ttk::device_cus() {
+    static int cus = [] {
+        int dev = 0, c = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
+        return std::max(1, c);
+    }();
+    return cus;
+}
+
+int ttk::batch_max_blocks_per_cu(int n, int wave_items) {
+    auto f = [&]() -> int {
+        TT_BATCH_DISPATCH(n, return max_blocks_per_cu<N_>(wave_items));
+        return 0;
+    };
+    return f();
+}
+
 hipError_t ttk::batch_marginal(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
-                               double *item_out, int tips, hipStream_t s) {
+                               double *item_out, int tips, const BatchShape &shape, hipStream_t s) {
     if (n_items <= 0) return hipSuccess;
+    const int resident = device_cus() * std::max(1, shape.blocks_per_cu);
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, queue);
     TT_BATCH_DISPATCH(n, {
-        const int grid = std::min(persistent_grid<N_>(), n_items);
-        hipLaunchKernelGGL(k_batch_marginal<N_>, dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items, queue,
-                           item_out, tips);
+        if (shape.wave_items) {
+            const int grid = std::min(resident, (n_items + kWaves - 1) / kWaves);
+            hipLaunchKernelGGL(k_batch_marginal_w<N_>, dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items,
+                               queue, item_out, tips);
+        } else {
+            const int grid = std::min(resident, n_items);
+            hipLaunchKernelGGL(k_batch_marginal<N_>, dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items,
+                               queue, item_out, tips);
+        }
     });
     return hipGetLastError();
 }

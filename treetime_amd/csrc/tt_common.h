@@ -91,6 +91,10 @@ hipError_t prob_t_profiles(const double *pp, const double *pc, const double *mul
                            int K, int gap, double *block_out, int nblk, hipStream_t s);
 int root_blocks(int P);
 bool mfma_enabled();
+// joint branch-length optimisation (tt_branch.hip)
+hipError_t branch_opt(const TreeDev &T, const int16_t *code_state, int gap, const double *eig, const double *v,
+                      const double *vinv, double mu, double max_t, double tol, double *counts, double *t_out,
+                      int32_t *status, hipStream_t s);
 }  // namespace ttk
 
 // A tree as seen by the persistent batched kernel: the tree buffers plus its
@@ -111,7 +115,17 @@ struct BatchItem {
     int32_t tree, tile;
 };
 
+// Launch shape of the persistent batched kernel, picked per plan by the
+// autotuner in tt_api.hip.
+struct BatchShape {
+    int32_t wave_items;     // 1: one wave per item; 0: one 4-wave block per item
+    int32_t blocks_per_cu;  // resident blocks per CU the grid is sized for
+};
+
 namespace ttk {
 hipError_t batch_marginal(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
-                          double *item_out, int tips, hipStream_t s);
+                          double *item_out, int tips, const BatchShape &shape, hipStream_t s);
+// occupancy limit (blocks per CU) of the batched kernel for n states and item shape; 0 if n unsupported
+int batch_max_blocks_per_cu(int n, int wave_items);
+int device_cus();
 }  // namespace ttk

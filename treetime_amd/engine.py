@@ -147,6 +147,24 @@ class DeviceTree:
         self.t_cur[int(node)] = t
         return out.value
 
+    def optimize_branch_lengths(self, code_state, gap_state=-1, max_t=4.0, tol=1e-10, stream=None,
+                                with_counts=True):
+        """One sweep of joint branch-length optimisation from the states of
+        the last reconstruction (tt_optimize_branch_lengths).  Returns
+        (t_opt, status, pair_counts or None); the tree's lengths are left as
+        they are."""
+        cs = _native.as_c(code_state, np.int16)
+        if cs.shape != (N_CODES,):
+            raise ValueError("code_state must have 256 entries")
+        t = np.empty(self.n_nodes, dtype=np.float64)
+        st = np.empty(self.n_nodes, dtype=np.int32)
+        cnt = np.empty((self.n_nodes, self.n, self.n), dtype=np.float64) if with_counts else None
+        _native.check(self.lib.tt_optimize_branch_lengths(
+            self.handle, _native.ptr(cs, C.c_int16), int(gap_state), float(max_t), float(tol),
+            _native.ptr(t, C.c_double), _native.ptr(st, C.c_int32),
+            _native.ptr(cnt, C.c_double) if with_counts else None, stream))
+        return t, st, cnt
+
     def trim(self):
         """Free the joint-pass buffers (returned to the engine budget)."""
         _native.check(self.lib.tt_tree_trim(self.handle))
@@ -205,3 +223,14 @@ def batch_marginal(engine, trees, reconstruct_tips=False, stream=None, want_logL
     _native.check(engine.lib.tt_batch_marginal(engine.handle, arr, k, flags,
                                                _native.ptr(out, C.c_double) if want_logL else None, stream))
     return out
+
+
+def batch_info(engine, trees, reconstruct_tips=False):
+    """Launch shape chosen by the autotuner for this list of trees:
+    {"wave_items", "blocks_per_cu", "tuned_ms"}."""
+    k = len(trees)
+    arr = (C.c_void_p * k)(*[t.handle.value for t in trees])
+    w, b, ms = C.c_int32(), C.c_int32(), C.c_double()
+    flags = _native.TT_RECONSTRUCT_TIPS if reconstruct_tips else 0
+    _native.check(engine.lib.tt_batch_info(engine.handle, arr, k, flags, C.byref(w), C.byref(b), C.byref(ms)))
+    return {"wave_items": w.value, "blocks_per_cu": b.value, "tuned_ms": ms.value}

@@ -315,7 +315,12 @@ class TreeAnc:
             raise NotImplementedError("GTR inference is not part of the GPU pruning engine")
         return run(reconstruct_tip_states=reconstruct_tip_states, **kwargs)
 
-    def _assign_sequences(self, dev, reconstruct_tip_states, sample_root=False):
+    def _assign_sequences(self, dev, reconstruct_tip_states, sample_root=False, sample_other=False, prof=None):
+        """Write node._cseq from the device states (argmax, first maximum) in
+        preorder; nodes flagged for sampling draw their states from their
+        marginal profile with this TreeAnc's generator instead, one
+        ``rng.random(P)`` per sampled node in the reference's order
+        (seq_utils.py:232-276, treeanc.py:830-838, 918-930)."""
         states = dev.states()
         alpha = self.gtr.alphabet
         L = self.data.compressed_length
@@ -323,8 +328,16 @@ class TreeAnc:
         for i, node in enumerate(self._nodes):
             if node.is_terminal() and not reconstruct_tip_states:
                 continue
-            seq = alpha[states[i]]
-            if self.sequence_reconstruction:
+            sample = sample_root if node.up is None else sample_other
+            if sample:
+                cum = np.cumsum(prof[i].astype(float), axis=1).T
+                idx = np.argmax(cum >= self.rng.random(size=cum.shape[1]), axis=0)
+            else:
+                idx = states[i]
+            seq = alpha[idx]
+            if node.up is None:
+                pass  # the root's characters are not counted in N_diff (treeanc.py:830-838)
+            elif self.sequence_reconstruction:
                 n_diff += int((seq != node.cseq).sum())
             else:
                 n_diff += L
@@ -333,8 +346,12 @@ class TreeAnc:
 
     def _ml_anc_marginal(self, sample_from_profile=False, reconstruct_tip_states=False, debug=False, **kwargs):
         """treeanc.py:762-812."""
-        if sample_from_profile:
-            raise NotImplementedError("sampling from profiles is not part of the GPU pruning engine")
+        if sample_from_profile == "root":
+            sample_root, sample_other = True, False
+        elif isinstance(sample_from_profile, bool):
+            sample_root = sample_other = sample_from_profile
+        else:
+            raise ValueError("sample_from_profile must be True, False or 'root'")
         dev = self._sync_model(self._device_tree())
         total = dev.marginal(reconstruct_tips=reconstruct_tip_states)
         prof = dev.profiles()
@@ -355,7 +372,7 @@ class TreeAnc:
             else:
                 node.marginal_subtree_LH = sub[i].astype(float)
                 node.marginal_profile = prof[i].astype(float)
-        n_diff = self._assign_sequences(dev, reconstruct_tip_states)
+        n_diff = self._assign_sequences(dev, reconstruct_tip_states, sample_root, sample_other, prof)
         self.reconstructed_tip_sequences = reconstruct_tip_states
         self.sequence_reconstruction = "marginal"
         return n_diff
@@ -406,18 +423,23 @@ class TreeAnc:
         return self.optimize_branch_lengths_joint(**kwargs)
 
     def optimize_branch_lengths_joint(self, **kwargs):
-        """treeanc.py:1176-1242: per branch, Brent on the state-pair counts of
-        the current reconstruction."""
+        """treeanc.py:1176-1242: one sweep over all branches, each set to the
+        maximum-likelihood length given the states of the current
+        reconstruction at its ends.  All branches are optimised together on
+        the device (tt_optimize_branch_lengths); the results are applied in
+        postorder as the reference does, so an error at one branch leaves the
+        branches before it updated."""
         if self.tree is None or self.data.aln is None:
             raise MissingDataError("TreeAnc.optimize_branch_length: ERROR, alignment or tree are missing.")
         store_old = kwargs.get("store_old", False)
+        res = self._branch_optima()
         max_bl = 0
         for node in self.tree.find_clades(order="postorder"):
             if node.up is None:
                 continue
             if store_old:
                 node._old_length = node.branch_length
-            new_len = max(0, self.optimal_branch_length(node))
+            new_len = max(0, self._take_optimum(node, res))
             node.branch_length = new_len
             node.mutation_length = new_len
             max_bl = max(max_bl, new_len)
@@ -428,17 +450,54 @@ class TreeAnc:
         self._prepare_nodes()
         return ttconf.SUCCESS
 
-    def optimal_branch_length(self, node):
-        """treeanc.py:1244-1270."""
-        if node.up is None:
-            return self.one_mutation
+    def _code_states(self):
+        """Alphabet index of every character code for gtr.py:672-712: characters
+        outside the alphabet are not counted for n < 10 and count as state 1
+        for larger alphabets (the reference's np.ones_like default)."""
+        alpha = [str(a) for a in self.gtr.alphabet]
+        other = -1 if len(alpha) < 10 else 1
+        out = np.full(256, other, dtype=np.int16)
+        for i, a in enumerate(alpha):
+            if len(a) == 1 and ord(a) < 256:
+                out[ord(a)] = i
+        return out
+
+    def _branch_optima(self):
+        """Device sweep: optimal length, status and state-pair counts of every
+        branch, indexed by node."""
+        dev = self._device
+        if dev is None or self._nodes is None or not self.sequence_reconstruction:
+            raise MissingDataError("TreeAnc.optimize_branch_length: ancestral sequences must be reconstructed "
+                                   "first (infer_ancestral_sequences)")
+        nodes = list(self.tree.find_clades(order="preorder"))
+        if len(nodes) != len(self._nodes) or any(a is not b for a, b in zip(nodes, self._nodes)):
+            raise MissingDataError("TreeAnc.optimize_branch_length: the tree changed since the last "
+                                   "reconstruction; reconstruct ancestral sequences again")
+        g = self.gtr
+        gap = g.gap_index if (self.ignore_gaps and g.gap_index is not None) else -1
+        t, st, cnt = dev.optimize_branch_lengths(self._code_states(), gap, ttconf.MAX_BRANCH_LENGTH, 1e-10)
+        return {id(n): (t[i], st[i], cnt[i]) for i, n in enumerate(nodes)}
+
+    def _take_optimum(self, node, res):
+        """treeanc.py:1244-1270 for one node, from the device sweep."""
         if node.cseq is None and node.is_terminal():
             raise MissingDataError("TreeAnc.optimal_branch_length: terminal node alignments required; sequence is "
                                    "missing for leaf: '%s'." % node.name)
-        pairs, mult = self.gtr.state_pair(node.up.cseq, node.cseq, pattern_multiplicity=self.data.multiplicity(),
-                                          ignore_gaps=self.ignore_gaps)
-        node.branch_state = {"pair": pairs, "multiplicity": mult}
-        return self.gtr.optimal_t_compressed(pairs, mult)
+        t, st, cnt = res[id(node)]
+        nz = np.argwhere(cnt != 0)
+        node.branch_state = {"pair": nz.astype(int), "multiplicity": cnt[cnt != 0].astype(int)}
+        if st == 1:
+            raise ValueError("Bracketing values (xa, xb, xc) do not fulfill this requirement: "
+                             "(f(xb) < f(xa)) and (f(xb) < f(xc))")
+        if t > 0.9 * ttconf.MAX_BRANCH_LENGTH:
+            self.logger("WARNING: GTR.optimal_t_compressed -- The branch length seems to be very long!", 4, warn=True)
+        return float(t)
+
+    def optimal_branch_length(self, node):
+        """treeanc.py:1244-1270 (one branch; the sweep above does all at once)."""
+        if node.up is None:
+            return self.one_mutation
+        return max(0.0, self._take_optimum(node, self._branch_optima()))
 
     # ---------------------------------------------------------------- output
     def sequence(self, node, as_string=True):
