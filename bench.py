@@ -176,6 +176,24 @@ def copy_probe(nbytes):
     return 0
 
 
+def copy_ceiling(torch, nbytes=2 << 30, reps=5):
+    """Device-to-device copy rate (read + write bytes per second, GB/s): the
+    measured ceiling beside the datasheet peak."""
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    rate = 2 * nbytes * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return rate
+
+
 def measured_traffic(path, workload):
     """HBM bytes per launch recorded for ``workload`` in a profile summary
     (line 'traffic_bytes_per_launch <workload> <bytes>'), else None."""
@@ -226,6 +244,8 @@ def main():
     B = sum(algorithmic_bytes(p) for p in preps)
     P_total = sum(p["tip_codes"].shape[1] for p in preps)
 
+    ceiling = copy_ceiling(torch)
+    log(f"[bench] device copy ceiling {ceiling:.0f} GB/s (read + write)")
     stream = torch.cuda.Stream()
     sp = stream.cuda_stream
     # correctness: batched logL equals the plain pass per tree
@@ -314,6 +334,7 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.traffic_file, wname),
+                     "copy_ceiling": ceiling, "frac_of_copy_ceiling": achieved / ceiling,
                      "algorithmic_bytes_per_launch": int(B)},
         "cpu_baseline": cpu,
         "serving": serving,

@@ -189,11 +189,12 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
                           void *stream);
 
 /* Launch shape the autotuner chose for the plan of (trees, flags): one wave
- * (1) or one 4-wave block (0) per item, resident blocks per CU, and the
- * measured time per launch while tuning (0 if not timed).  TT_ESTATE when
- * no plan exists yet.  No reference counterpart. */
+ * (1) or one 4-wave block (0) per item, 64-pattern tiles per item, resident
+ * blocks per CU, and the measured time per launch while tuning (0 if not
+ * timed).  Any output pointer may be NULL.  TT_ESTATE when no plan exists
+ * yet.  No reference counterpart. */
 int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, int32_t *wave_items,
-                      int32_t *blocks_per_cu, double *tuned_ms);
+                      int32_t *tiles, int32_t *blocks_per_cu, double *tuned_ms);
 
 #ifdef __cplusplus
 }

@@ -201,3 +201,38 @@ def test_batch_matches_plain(gpu_engine, tips_flag):
     assert abs(got[3] - ref) <= LOGL_RTOL * abs(ref)
     for t in trees:
         t.close()
+
+
+def test_tip_states_reset_after_reconstruction(gpu_engine):
+    """Tip rows of the states hold 255 after a pass without tip
+    reconstruction, also when an earlier pass (plain or batched) filled them."""
+    from treetime_amd.engine import batch_marginal
+
+    say("[tips] tip states reset to 255")
+    jobs = [small_job(n, L, 4, seed) for n, L, seed in [(30, 200, 61), (70, 500, 62)]]
+    trees = [device_tree(gpu_engine, prep) for _, prep, _ in jobs]
+
+    def tip_rows(t):
+        st = t.states()
+        is_tip = np.ones(t.n_nodes, dtype=bool)
+        is_tip[t.parent[t.parent >= 0]] = False
+        return st[is_tip]
+
+    t0 = trees[0]
+    t0.marginal(reconstruct_tips=True)
+    assert (tip_rows(t0) < 4).all()
+    t0.marginal()
+    assert (tip_rows(t0) == 255).all()
+    t0.joint(reconstruct_tips=True)
+    assert (tip_rows(t0) < 4).all()
+    t0.joint()
+    assert (tip_rows(t0) == 255).all()
+    batch_marginal(gpu_engine, trees, reconstruct_tips=True)
+    assert all((tip_rows(t) < 4).all() for t in trees)
+    batch_marginal(gpu_engine, trees)
+    assert all((tip_rows(t) == 255).all() for t in trees)
+    trees[1].marginal(reconstruct_tips=True)
+    batch_marginal(gpu_engine, trees)
+    assert all((tip_rows(t) == 255).all() for t in trees)
+    for t in trees:
+        t.close()

@@ -80,19 +80,26 @@ struct BatchPlan {
     tt_engine *eng = nullptr;
     std::vector<tt_tree *> trees;
     uint32_t flags = 0;
-    std::vector<BatchItem> items;
+    std::vector<BatchItem> items1, items2;  // items of one and of two 64-pattern tiles
+    std::vector<BatchTreeDev> host_trees;   // host copy of d_trees
     BatchTreeDev *d_trees = nullptr;
-    BatchItem *d_items = nullptr;
+    BatchItem *d_items1 = nullptr, *d_items2 = nullptr;
     int *d_queue = nullptr;
     double *d_item_out = nullptr;
     size_t bytes = 0;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
-    BatchShape shape{1, 1};
+    BatchShape shape{0, 1, 1, 0};
     double tuned_ms = 0.0;  // time per launch of the chosen shape while tuning (0: not timed)
+    const std::vector<BatchItem> &items_for(const BatchShape &c) const { return c.tiles == 2 ? items2 : items1; }
+    const BatchItem *d_items_for(const BatchShape &c) const { return c.tiles == 2 ? d_items2 : d_items1; }
+    hipError_t launch(int n, int tips, const BatchShape &c, hipStream_t s) const {
+        return ttk::batch_marginal(n, d_trees, d_items_for(c), (int)items_for(c).size(), d_queue, d_item_out, tips,
+                                   c, s);
+    }
     ~BatchPlan() {
         if (graph) (void)hipGraphExecDestroy(graph);
-        for (void *q : {(void *)d_trees, (void *)d_items, (void *)d_queue, (void *)d_item_out})
+        for (void *q : {(void *)d_trees, (void *)d_items1, (void *)d_items2, (void *)d_queue, (void *)d_item_out})
             if (q) (void)hipFree(q);
         if (stream) (void)hipStreamDestroy(stream);
         eng->used -= bytes;
@@ -134,6 +141,7 @@ struct tt_tree {
     std::vector<std::pair<void *, size_t>> allocs;
     int last_pass = 0;  // 1 marginal, 2 joint, 3 loglik
     bool have_states = false;  // d.states holds a marginal or joint reconstruction
+    bool tips_dirty = false;   // tip rows of d.states hold reconstructed states (else 255)
     int16_t *d_code_state = nullptr;  // branch optimisation buffers, allocated on first use
     double *d_pair_counts = nullptr, *d_t_opt = nullptr;
     int32_t *d_opt_status = nullptr;
@@ -330,9 +338,14 @@ int32_t ensure_joint(tt_tree *T) {
 }
 
 // Enqueue one pass on stream s.  mode: 1 marginal, 2 joint, 3 loglik.
+// flag of enqueue_pass (not part of the ABI): reset the tip rows of the
+// states to 255 because an earlier pass reconstructed them
+constexpr uint32_t kFillTips = 0x100u;
+
 int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
     const TreeDev &D = T->d;
     const bool tips = (flags & TT_RECONSTRUCT_TIPS) != 0;
+    const bool fill = (flags & kFillTips) != 0;
     const int nb = ttk::root_blocks(T->P);
     if (mode == 1 || mode == 3) {
         for (auto &L : T->post_lv) {
@@ -344,7 +357,7 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
         TT_HIP(ttk::root_marginal(D, S, s));
         TT_HIP(ttk::final_sum(D, nb, s));
         if (mode == 1) {
-            if (!tips) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
+            if (fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
             for (auto &L : T->pre_lv) {
                 LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off), tips ? L.n_all : L.n_int, 1, 0, 0};
                 TT_HIP(ttk::pre_marginal(D, ld, s));
@@ -360,7 +373,7 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
         TT_HIP(ttk::rowsum(D, T->R_joint, S, s));
         TT_HIP(ttk::root_joint(D, S, tips ? 1 : 0, s));
         TT_HIP(ttk::final_sum(D, nb, s));
-        if (!tips) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
+        if (fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
         for (auto &L : T->pre_lv) {
             LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off), tips ? L.n_all : L.n_int, 1, 0, 0};
             TT_HIP(ttk::backtrack(D, ld, s));
@@ -378,7 +391,10 @@ int32_t run_pass(tt_tree *T, int mode, uint32_t flags, double *logL, void *strea
         if (rc) return rc;
     }
     hipStream_t s = stream ? (hipStream_t)stream : T->stream;
-    const int key = mode * 8 + (int)(flags & TT_RECONSTRUCT_TIPS);
+    const bool tips = (flags & TT_RECONSTRUCT_TIPS) != 0;
+    const bool fill = !tips && T->tips_dirty && (mode == 1 || mode == 2);
+    flags = (flags & TT_RECONSTRUCT_TIPS) | (fill ? kFillTips : 0u);
+    const int key = mode * 8 + (tips ? 1 : 0) + (fill ? 2 : 0);
     if (g_use_graphs()) {
         auto it = T->graphs.find(key);
         if (it == T->graphs.end()) {
@@ -403,7 +419,10 @@ int32_t run_pass(tt_tree *T, int mode, uint32_t flags, double *logL, void *strea
         if (rc) return rc;
     }
     T->last_pass = mode;
-    if (mode == 1 || mode == 2) T->have_states = true;
+    if (mode == 1 || mode == 2) {
+        T->have_states = true;
+        T->tips_dirty = tips;
+    }
     if (T->update_mode && (mode == 1 || mode == 3)) T->offs_valid = true;
     if (logL) {
         TT_HIP(hipMemcpyAsync(logL, T->d.total, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -445,13 +464,15 @@ int32_t check_range(tt_tree *T, const double *t, int k) {
     return TT_OK;
 }
 
-// Autotuner of the batched launch: every candidate shape (item = one wave or
-// one 4-wave block; resident blocks per CU 1, 2 or the occupancy limit) runs
+// Autotuner of the batched launch: every candidate shape (item = one 4-wave
+// block over two 64-pattern tiles with the compiler's register budget or one
+// fitting 5 waves per SIMD, one 4-wave block over one tile, or one wave over
+// one tile; resident blocks per CU at the occupancy limit or half of it) runs
 // once to warm up and twice timed with HIP events on the plan's own trees;
 // the fastest is kept in the plan.  TT_BATCH_MODE=block|wave and
 // TT_BATCH_BLOCKS_PER_CU=k restrict the candidates; TT_AUTOTUNE=0 skips the
-// timing and takes the first candidate (one block per item at the occupancy
-// limit).  Shapes differ only in how items are spread over the chip and in
+// timing and takes the first candidate (one block per two-tile item, 5 waves
+// per SIMD, the fastest on the 256-tree nucleotide mix).  Shapes differ only in how items are spread over the chip and in
 // the order the per-node normalisers of an item are multiplied, so states
 // and profiles are identical for every shape and logL agrees to rounding.
 int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
@@ -460,18 +481,21 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     const char *bpc = getenv("TT_BATCH_BLOCKS_PER_CU");
     const char *tune = getenv("TT_AUTOTUNE");
     const bool log = getenv("TT_AUTOTUNE_LOG") != nullptr;  // print every candidate's time to stderr
-    for (int w : {0, 1}) {
-        if (mode && mode[0] && ((mode[0] == 'b') != (w == 0))) continue;
-        const int occ = ttk::batch_max_blocks_per_cu(n, w);
+    const BatchShape kinds[] = {{0, 0, 2, 5}, {0, 0, 2, 0}, {0, 0, 1, 0}, {1, 0, 1, 0}};
+    for (BatchShape kd : kinds) {
+        if (mode && mode[0] && ((mode[0] == 'b') != (kd.wave_items == 0))) continue;
+        const int occ = ttk::batch_max_blocks_per_cu(n, kd);
         std::vector<int> per_cu;
         if (bpc && atoi(bpc) > 0) {
             per_cu.push_back(atoi(bpc));
         } else {
             per_cu.push_back(occ);
-            for (int x : {2, 1})
-                if (x < occ) per_cu.push_back(x);
+            if (occ > 1) per_cu.push_back(occ / 2);
         }
-        for (int x : per_cu) cand.push_back({w, x});
+        for (int x : per_cu) {
+            kd.blocks_per_cu = x;
+            cand.push_back(kd);
+        }
     }
     if (cand.empty()) return fail(TT_EINVAL, "TT_BATCH_MODE=%s selects no launch shape", mode);
     P->shape = cand[0];
@@ -483,17 +507,17 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     double best = INFINITY;
     hipError_t e = hipSuccess;
     for (const BatchShape &c : cand) {
-        const int ni = (int)P->items.size();
-        e = ttk::batch_marginal(n, P->d_trees, P->d_items, ni, P->d_queue, P->d_item_out, tips, c, s);
+        e = P->launch(n, tips, c, s);
         if (e == hipSuccess) e = hipEventRecord(e0, s);
-        for (int r = 0; r < 2 && e == hipSuccess; ++r)
-            e = ttk::batch_marginal(n, P->d_trees, P->d_items, ni, P->d_queue, P->d_item_out, tips, c, s);
+        for (int r = 0; r < 2 && e == hipSuccess; ++r) e = P->launch(n, tips, c, s);
         if (e == hipSuccess) e = hipEventRecord(e1, s);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
         if (e != hipSuccess) break;
-        if (log) fprintf(stderr, "tt autotune: wave_items=%d blocks_per_cu=%d %.3f ms\n", c.wave_items, c.blocks_per_cu, ms / 2.0);
+        if (log)
+            fprintf(stderr, "tt autotune: wave_items=%d tiles=%d waves_per_simd=%d blocks_per_cu=%d %.3f ms\n",
+                    c.wave_items, c.tiles, c.waves_per_simd, c.blocks_per_cu, ms / 2.0);
         if (ms / 2.0 < best) {
             best = ms / 2.0;
             P->shape = c;
@@ -954,13 +978,14 @@ int32_t tt_prob_t_profiles(const double *pp_dev, const double *pc_dev, const dou
 }
 
 int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, int32_t *wave_items,
-                      int32_t *blocks_per_cu, double *tuned_ms) {
+                      int32_t *tiles, int32_t *blocks_per_cu, double *tuned_ms) {
     if (!eng || (k > 0 && !trees)) return fail(TT_EINVAL, "NULL argument");
     std::vector<tt_tree *> key(trees, trees + std::max(k, 0));
     std::lock_guard<std::mutex> g(eng->mu);
     for (BatchPlan *p : eng->plans)
         if (p->flags == (flags & TT_RECONSTRUCT_TIPS) && p->trees == key) {
             if (wave_items) *wave_items = p->shape.wave_items;
+            if (tiles) *tiles = p->shape.tiles;
             if (blocks_per_cu) *blocks_per_cu = p->shape.blocks_per_cu;
             if (tuned_ms) *tuned_ms = p->tuned_ms;
             return TT_OK;
@@ -1004,7 +1029,8 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
         std::stable_sort(order.begin(), order.end(), [](auto &a, auto &b) { return a.first > b.first; });
         for (auto &o : order) {
             const int tiles = (trees[o.second]->P + TT_WAVE - 1) / TT_WAVE;
-            for (int t = 0; t < tiles; ++t) P->items.push_back({o.second, t});
+            for (int t = 0; t < tiles; ++t) P->items1.push_back({o.second, t});
+            for (int t = 0; t < tiles; t += 2) P->items2.push_back({o.second, t});
         }
         std::vector<BatchTreeDev> bt(k);
         for (int i = 0; i < k; ++i) {
@@ -1018,10 +1044,11 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
             bt[i].tip_nodes = (decltype(bt[i].tip_nodes))T->d_tip_nodes;
             bt[i].H = (int32_t)T->post_lv.size();
             bt[i].D = (int32_t)T->pre_lv.size();
+            bt[i].fill_tips = 0;
         }
-        const size_t nb_trees = bt.size() * sizeof(BatchTreeDev), nb_items = P->items.size() * sizeof(BatchItem),
-                     nb_out = P->items.size() * sizeof(double);
-        P->bytes = nb_trees + nb_items + nb_out + 256;
+        const size_t nb_trees = bt.size() * sizeof(BatchTreeDev), nb_items1 = P->items1.size() * sizeof(BatchItem),
+                     nb_items2 = P->items2.size() * sizeof(BatchItem), nb_out = P->items1.size() * sizeof(double);
+        P->bytes = nb_trees + nb_items1 + nb_items2 + nb_out + 256;
         if (eng->budget && eng->used.load() + P->bytes > eng->budget) {
             P->bytes = 0;
             delete P;
@@ -1029,11 +1056,14 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
         }
         eng->used += P->bytes;
         hipError_t e = hipMalloc((void **)&P->d_trees, nb_trees);
-        if (e == hipSuccess) e = hipMalloc((void **)&P->d_items, nb_items);
+        if (e == hipSuccess) e = hipMalloc((void **)&P->d_items1, nb_items1);
+        if (e == hipSuccess) e = hipMalloc((void **)&P->d_items2, nb_items2);
         if (e == hipSuccess) e = hipMalloc((void **)&P->d_queue, 256);
         if (e == hipSuccess) e = hipMalloc((void **)&P->d_item_out, nb_out);
         if (e == hipSuccess) e = hipMemcpy(P->d_trees, bt.data(), nb_trees, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(P->d_items, P->items.data(), nb_items, hipMemcpyHostToDevice);
+        P->host_trees = bt;
+        if (e == hipSuccess) e = hipMemcpy(P->d_items1, P->items1.data(), nb_items1, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(P->d_items2, P->items2.data(), nb_items2, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
             delete P;
@@ -1048,8 +1078,7 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
             e = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal);
             hipError_t e2 = hipSuccess;
             if (e == hipSuccess)
-                e2 = ttk::batch_marginal(n, P->d_trees, P->d_items, (int)P->items.size(), P->d_queue,
-                                         P->d_item_out, fl ? 1 : 0, P->shape, P->stream);
+                e2 = P->launch(n, fl ? 1 : 0, P->shape, P->stream);
             if (e == hipSuccess) e = hipStreamEndCapture(P->stream, &g);
             if (e == hipSuccess) e = e2;
             if (e == hipSuccess) e = hipGraphInstantiate(&P->graph, g, nullptr, nullptr, 0);
@@ -1064,22 +1093,35 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
         plan = P;
     }
     hipStream_t s = stream ? (hipStream_t)stream : plan->stream;
+    // tip rows that an earlier pass reconstructed are reset to 255 (read by the kernel at run time)
+    bool changed = false;
+    for (int i = 0; i < k; ++i) {
+        const int32_t want = (!fl && trees[i]->tips_dirty) ? 1 : 0;
+        if (plan->host_trees[i].fill_tips != want) {
+            plan->host_trees[i].fill_tips = want;
+            changed = true;
+        }
+    }
+    if (changed)
+        TT_HIP(hipMemcpyAsync(plan->d_trees, plan->host_trees.data(), plan->host_trees.size() * sizeof(BatchTreeDev),
+                              hipMemcpyHostToDevice, s));
     if (plan->graph) {
         TT_HIP(hipGraphLaunch(plan->graph, s));
     } else {
-        TT_HIP(ttk::batch_marginal(n, plan->d_trees, plan->d_items, (int)plan->items.size(), plan->d_queue,
-                                   plan->d_item_out, fl ? 1 : 0, plan->shape, s));
+        TT_HIP(plan->launch(n, fl ? 1 : 0, plan->shape, s));
     }
     for (int i = 0; i < k; ++i) {
         trees[i]->last_pass = 1;
         trees[i]->have_states = true;
+        trees[i]->tips_dirty = fl != 0;
     }
     if (logL) {
-        std::vector<double> out(plan->items.size());
+        const std::vector<BatchItem> &items = plan->items_for(plan->shape);
+        std::vector<double> out(items.size());
         TT_HIP(hipMemcpyAsync(out.data(), plan->d_item_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         TT_HIP(hipStreamSynchronize(s));
         for (int i = 0; i < k; ++i) logL[i] = 0.0;
-        for (size_t j = 0; j < out.size(); ++j) logL[plan->items[j].tree] += out[j];
+        for (size_t j = 0; j < out.size(); ++j) logL[items[j].tree] += out[j];
     }
     return TT_OK;
 }

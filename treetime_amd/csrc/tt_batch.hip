@@ -24,15 +24,79 @@ constexpr int kWaves = TT_BLOCK / TT_WAVE;
 
 __global__ void k_reset_queue(int *queue) { *queue = 0; }
 
-template <int N>
-__global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal(const BatchTreeDev *__restrict__ trees_arg,
+// TILES 64-pattern tiles per item: every lane carries TILES patterns of the
+// same nodes, so each node step keeps 2 * TILES row fetches in flight and the
+// wave-uniform work (node ids, child lists, P(t)) is shared by all of them.
+// The second tile's state is held in separately named variables (no arrays
+// of register structs, which the compiler may place in scratch).
+template <int N, int TL, class LP>
+__device__ __forceinline__ void post_pair_t(const TreeDev &T, int v0, int v1, bool has1, int p0, int p1, bool act0,
+                                            bool act1, LP lut, LogAcc &acc0, LogAcc &acc1) {
+    const int a0 = wave_uniform(T.child_ptr[v0]), a1 = wave_uniform(T.child_ptr[v0 + 1]);
+    const int b0 = wave_uniform(T.child_ptr[v1]), b1 = wave_uniform(T.child_ptr[v1 + 1]);
+    if (a1 - a0 == 2 && b1 - b0 == 2) {
+        BinRaw<N> RA0, RB0, RA1, RB1;
+        if (act0) {
+            bin_fetch<N>(T, a0, p0, RA0);
+            bin_fetch<N>(T, b0, p0, RB0);
+        }
+        if (TL == 2 && act1) {
+            bin_fetch<N>(T, a0, p1, RA1);
+            bin_fetch<N>(T, b0, p1, RB1);
+        }
+        if (act0) {
+            bin_finish<N>(T, v0, a0, p0, RA0, lut, acc0);
+            if (has1) bin_finish<N>(T, v1, b0, p0, RB0, lut, acc0);
+        }
+        if (TL == 2 && act1) {
+            bin_finish<N>(T, v0, a0, p1, RA1, lut, acc1);
+            if (has1) bin_finish<N>(T, v1, b0, p1, RB1, lut, acc1);
+        }
+    } else {
+        if (act0) {
+            post_node<N>(T, v0, p0, lut, acc0);
+            if (has1) post_node<N>(T, v1, p0, lut, acc0);
+        }
+        if (TL == 2 && act1) {
+            post_node<N>(T, v0, p1, lut, acc1);
+            if (has1) post_node<N>(T, v1, p1, lut, acc1);
+        }
+    }
+}
+
+template <int N, int TL, class LP>
+__device__ __forceinline__ void pre_pair_t(const TreeDev &T, int c0, int c1, bool has1, int p0, int p1, bool act0,
+                                           bool act1, LP lut) {
+    PreRaw<N> RA0, RB0, RA1, RB1;
+    if (act0) {
+        pre_fetch<N>(T, c0, p0, RA0);
+        pre_fetch<N>(T, c1, p0, RB0);
+    }
+    if (TL == 2 && act1) {
+        pre_fetch<N>(T, c0, p1, RA1);
+        pre_fetch<N>(T, c1, p1, RB1);
+    }
+    if (act0) {
+        pre_finish<N>(T, c0, p0, RA0, lut);
+        if (has1) pre_finish<N>(T, c1, p0, RB0, lut);
+    }
+    if (TL == 2 && act1) {
+        pre_finish<N>(T, c0, p1, RA1, lut);
+        if (has1) pre_finish<N>(T, c1, p1, RB1, lut);
+    }
+}
+
+// WPE: waves per SIMD the register allocator must allow (1 = no constraint);
+// the two-tile kernel needs 100 VGPRs (4 waves), at 5 it spills a few.
+template <int N, int TL, int WPE>
+__global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_batch_marginal(const BatchTreeDev *__restrict__ trees_arg,
                                                              const BatchItem *__restrict__ items_arg, int n_items,
                                                              int *queue, double *item_out, int tips) {
     // tree records and items are read-only: scalar loads through the constant address space
     const BatchTreeDev TT_C *trees = (const BatchTreeDev TT_C *)trees_arg;
     const BatchItem TT_C *items = (const BatchItem TT_C *)items_arg;
     __shared__ int s_item;
-    __shared__ double s_acc[kWaves][TT_WAVE];
+    __shared__ double s_acc[kWaves][TL][TT_WAVE];
     __shared__ float s_lut[256 * N];
     const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
     int cur_tree = -1;
@@ -49,34 +113,44 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal(const BatchTreeDev 
             cur_tree = item.tree;
         }
         __syncthreads();
-        const int p = item.tile * TT_WAVE + lane;
-        const bool act = p < T.P;
+        const int p0 = item.tile * TT_WAVE + lane;  // item.tile counts 64-pattern tiles
+        const int p1 = p0 + TT_WAVE;
+        const bool act0 = p0 < T.P, act1 = TL == 2 && p1 < T.P;
         // postorder: heights 1..H, the last level is the root
-        LogAcc acc;
+        LogAcc acc0, acc1;
         for (int h = 0; h < B.H; ++h) {
             const int kb = B.post_off[h], ke = B.post_off[h + 1];
             for (int k = kb + w; k < ke; k += 2 * kWaves) {
                 const bool has1 = k + kWaves < ke;
                 const int v0 = wave_uniform(B.post_nodes[k]);
                 const int v1 = has1 ? wave_uniform(B.post_nodes[k + kWaves]) : v0;
-                if (act) post_pair<N>(T, v0, v1, has1, p, s_lut, acc);
+                post_pair_t<N, TL>(T, v0, v1, has1, p0, p1, act0, act1, s_lut, acc0, acc1);
             }
             __syncthreads();
         }
         // root: per-pattern logL from the 4 waves' normaliser products
-        s_acc[w][lane] = act ? acc.log_value() : 0.0;
+        s_acc[w][0][lane] = act0 ? acc0.log_value() : 0.0;
+        if (TL == 2) s_acc[w][TL - 1][lane] = act1 ? acc1.log_value() : 0.0;
         __syncthreads();
         double contrib = 0.0;
-        if (w == 0 && act) {
-            double off = 0.0;
+        if (w == 0) {
+            double off0 = 0.0, off1 = 0.0;
 #pragma unroll
-            for (int q = 0; q < kWaves; ++q) off += s_acc[q][lane];
-            contrib = root_step<N>(T, p, off);
+            for (int q = 0; q < kWaves; ++q) {
+                off0 += s_acc[q][0][lane];
+                off1 += s_acc[q][TL - 1][lane];
+            }
+            if (act0) contrib += root_step<N>(T, p0, off0);
+            if (act1) contrib += root_step<N>(T, p1, off1);
         }
         const double tot = block_sum(contrib);
         if (threadIdx.x == 0) item_out[it] = tot;
-        if (!tips && act)
-            for (int k = w; k < T.n_tips; k += kWaves) T.states[(size_t)B.tip_nodes[k] * T.P + p] = 255;
+        if (!tips && B.fill_tips)
+            for (int k = w; k < T.n_tips; k += kWaves) {
+                const size_t row = (size_t)B.tip_nodes[k] * T.P;
+                if (act0) T.states[row + p0] = 255;
+                if (act1) T.states[row + p1] = 255;
+            }
         __syncthreads();
         // preorder: depths 1..D (tips only when reconstructing them)
         for (int d = 0; d < B.D; ++d) {
@@ -86,7 +160,7 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal(const BatchTreeDev 
                 const bool has1 = k + kWaves < ke;
                 const int c0 = wave_uniform(B.pre_nodes[k]);
                 const int c1 = has1 ? wave_uniform(B.pre_nodes[k + kWaves]) : c0;
-                if (act) pre_pair<N>(T, c0, c1, has1, p, s_lut);
+                pre_pair_t<N, TL>(T, c0, c1, has1, p0, p1, act0, act1, s_lut);
             }
             __syncthreads();
         }
@@ -130,18 +204,20 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal_w(const BatchTreeDe
         const int p = item.tile * TT_WAVE + lane;
         const bool act = p < T.P;
         LogAcc acc;
-        const int n_post = B.post_off[B.H];
-        for (int k = 0; k < n_post; k += 2) {  // level order: children before parents
-            const bool has1 = k + 1 < n_post;
-            const int v0 = wave_uniform(B.post_nodes[k]);
-            const int v1 = has1 ? wave_uniform(B.post_nodes[k + 1]) : v0;
-            if (act) post_pair<N>(T, v0, v1, has1, p, lut, acc);
+        for (int h = 0; h < B.H; ++h) {  // a pair never spans two heights: a parent may follow its child
+            const int kb = B.post_off[h], ke = B.post_off[h + 1];
+            for (int k = kb; k < ke; k += 2) {
+                const bool has1 = k + 1 < ke;
+                const int v0 = wave_uniform(B.post_nodes[k]);
+                const int v1 = has1 ? wave_uniform(B.post_nodes[k + 1]) : v0;
+                if (act) post_pair<N>(T, v0, v1, has1, p, lut, acc);
+            }
         }
         double contrib = 0.0;
         if (act) contrib = root_step<N>(T, p, acc.log_value());
         const double tot = wave_sum(contrib);
         if (lane == 0) item_out[it] = tot;
-        if (!tips && act)
+        if (!tips && B.fill_tips && act)
             for (int k = 0; k < T.n_tips; ++k) T.states[(size_t)B.tip_nodes[k] * T.P + p] = 255;
         for (int d = 0; d < B.D; ++d) {  // depth order: parents before children
             const int kb = B.pre_off[d];
@@ -157,12 +233,16 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal_w(const BatchTreeDe
 }
 
 template <int N>
-int max_blocks_per_cu(int wave_items) {
+int max_blocks_per_cu(const BatchShape &c) {
     int per_cu = 0;
-    if (wave_items)
+    if (c.wave_items)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal_w<N>, TT_BLOCK, 0);
+    else if (c.tiles == 2 && c.waves_per_simd == 5)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N, 2, 5>, TT_BLOCK, 0);
+    else if (c.tiles == 2)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N, 2, 1>, TT_BLOCK, 0);
     else
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N>, TT_BLOCK, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N, 1, 1>, TT_BLOCK, 0);
     return std::max(1, per_cu);
 }
 
@@ -190,9 +270,9 @@ int ttk::device_cus() {
     return cus;
 }
 
-int ttk::batch_max_blocks_per_cu(int n, int wave_items) {
+int ttk::batch_max_blocks_per_cu(int n, const BatchShape &c) {
     auto f = [&]() -> int {
-        TT_BATCH_DISPATCH(n, return max_blocks_per_cu<N_>(wave_items));
+        TT_BATCH_DISPATCH(n, return max_blocks_per_cu<N_>(c));
         return 0;
     };
     return f();
@@ -208,10 +288,18 @@ hipError_t ttk::batch_marginal(int n, const BatchTreeDev *trees, const BatchItem
             const int grid = std::min(resident, (n_items + kWaves - 1) / kWaves);
             hipLaunchKernelGGL(k_batch_marginal_w<N_>, dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items,
                                queue, item_out, tips);
+        } else if (shape.tiles == 2 && shape.waves_per_simd == 5) {
+            const int grid = std::min(resident, n_items);
+            hipLaunchKernelGGL((k_batch_marginal<N_, 2, 5>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items,
+                               n_items, queue, item_out, tips);
+        } else if (shape.tiles == 2) {
+            const int grid = std::min(resident, n_items);
+            hipLaunchKernelGGL((k_batch_marginal<N_, 2, 1>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items,
+                               n_items, queue, item_out, tips);
         } else {
             const int grid = std::min(resident, n_items);
-            hipLaunchKernelGGL(k_batch_marginal<N_>, dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items,
-                               queue, item_out, tips);
+            hipLaunchKernelGGL((k_batch_marginal<N_, 1, 1>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items,
+                               n_items, queue, item_out, tips);
         }
     });
     return hipGetLastError();

@@ -108,9 +108,11 @@ struct BatchTreeDev {
     const int32_t TT_C *pre_nint;    // [D] internal nodes per depth
     const int32_t TT_C *tip_nodes;   // [n_tips]
     int32_t H, D;
+    int32_t fill_tips;               // reset tip states to 255 (an earlier pass reconstructed them)
 };
 
-// One work item of the batched launch: 64 patterns (tile) of one tree.
+// One work item of the batched launch: patterns [64 tile, 64 (tile + tiles))
+// of one tree, tiles per item given by the launch shape.
 struct BatchItem {
     int32_t tree, tile;
 };
@@ -120,12 +122,14 @@ struct BatchItem {
 struct BatchShape {
     int32_t wave_items;     // 1: one wave per item; 0: one 4-wave block per item
     int32_t blocks_per_cu;  // resident blocks per CU the grid is sized for
+    int32_t tiles;          // 64-pattern tiles per item (block items: 1 or 2; wave items: 1)
+    int32_t waves_per_simd; // register budget of the kernel: 0 = compiler's choice, 5 = fit 5 waves per SIMD
 };
 
 namespace ttk {
 hipError_t batch_marginal(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
                           double *item_out, int tips, const BatchShape &shape, hipStream_t s);
 // occupancy limit (blocks per CU) of the batched kernel for n states and item shape; 0 if n unsupported
-int batch_max_blocks_per_cu(int n, int wave_items);
+int batch_max_blocks_per_cu(int n, const BatchShape &shape);
 int device_cus();
 }  // namespace ttk

@@ -227,10 +227,11 @@ def batch_marginal(engine, trees, reconstruct_tips=False, stream=None, want_logL
 
 def batch_info(engine, trees, reconstruct_tips=False):
     """Launch shape chosen by the autotuner for this list of trees:
-    {"wave_items", "blocks_per_cu", "tuned_ms"}."""
+    {"wave_items", "tiles", "blocks_per_cu", "tuned_ms"}."""
     k = len(trees)
     arr = (C.c_void_p * k)(*[t.handle.value for t in trees])
-    w, b, ms = C.c_int32(), C.c_int32(), C.c_double()
+    w, tl, b, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
     flags = _native.TT_RECONSTRUCT_TIPS if reconstruct_tips else 0
-    _native.check(engine.lib.tt_batch_info(engine.handle, arr, k, flags, C.byref(w), C.byref(b), C.byref(ms)))
-    return {"wave_items": w.value, "blocks_per_cu": b.value, "tuned_ms": ms.value}
+    _native.check(engine.lib.tt_batch_info(engine.handle, arr, k, flags, C.byref(w), C.byref(tl), C.byref(b),
+                                           C.byref(ms)))
+    return {"wave_items": w.value, "tiles": tl.value, "blocks_per_cu": b.value, "tuned_ms": ms.value}
