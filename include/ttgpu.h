@@ -188,9 +188,10 @@ int32_t tt_prob_t_profiles(const double *pp_dev, const double *pc_dev, const dou
 int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, double *logL,
                           void *stream);
 
-/* Launch shape the autotuner chose for the plan of (trees, flags): one wave
- * (1) or one 4-wave block (0) per item, 64-pattern tiles per item, resident
- * blocks per CU, and the measured time per launch while tuning (0 if not
+/* Launch shape the autotuner chose for the plan of (trees, flags): one
+ * 4-wave block (0) or one wave (1) per item of the persistent kernel, or each
+ * tree's own level-kernel pass in turn (2); 64-pattern tiles per item;
+ * resident blocks per CU; the measured time per launch while tuning (0 if not
  * timed).  Any output pointer may be NULL.  TT_ESTATE when no plan exists
  * yet.  No reference counterpart. */
 int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, int32_t *wave_items,

@@ -464,12 +464,28 @@ int32_t check_range(tt_tree *T, const double *t, int k) {
     return TT_OK;
 }
 
+// wave_items value of the shape that runs every tree's own level-kernel
+// graph in turn instead of the persistent kernel (best for very few, very
+// large trees, whose items would each walk thousands of levels alone)
+constexpr int kPerTreeGraphs = 2;
+
+int32_t launch_plan(BatchPlan *P, int n, int tips, const BatchShape &c, hipStream_t s) {
+    if (c.wave_items == kPerTreeGraphs) {
+        for (tt_tree *T : P->trees)
+            if (int32_t rc = run_pass(T, 1, tips ? TT_RECONSTRUCT_TIPS : 0u, nullptr, s)) return rc;
+        return TT_OK;
+    }
+    TT_HIP(P->launch(n, tips, c, s));
+    return TT_OK;
+}
+
 // Autotuner of the batched launch: every candidate shape (item = one 4-wave
 // block over two 64-pattern tiles with the compiler's register budget or one
 // fitting 5 waves per SIMD, one 4-wave block over one tile, or one wave over
-// one tile; resident blocks per CU at the occupancy limit or half of it) runs
+// one tile; resident blocks per CU at the occupancy limit or half of it; or
+// each tree's own level-kernel graph in turn) runs
 // once to warm up and twice timed with HIP events on the plan's own trees;
-// the fastest is kept in the plan.  TT_BATCH_MODE=block|wave and
+// the fastest is kept in the plan.  TT_BATCH_MODE=block|wave|plain and
 // TT_BATCH_BLOCKS_PER_CU=k restrict the candidates; TT_AUTOTUNE=0 skips the
 // timing and takes the first candidate (one block per two-tile item, 5 waves
 // per SIMD, the fastest on the 256-tree nucleotide mix).  Shapes differ only in how items are spread over the chip and in
@@ -481,9 +497,14 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     const char *bpc = getenv("TT_BATCH_BLOCKS_PER_CU");
     const char *tune = getenv("TT_AUTOTUNE");
     const bool log = getenv("TT_AUTOTUNE_LOG") != nullptr;  // print every candidate's time to stderr
+    auto allowed = [&](int wave_items) {
+        if (!mode || !mode[0]) return true;
+        const int want = mode[0] == 'b' ? 0 : mode[0] == 'w' ? 1 : mode[0] == 'p' ? kPerTreeGraphs : -1;
+        return wave_items == want;
+    };
     const BatchShape kinds[] = {{0, 0, 2, 5}, {0, 0, 2, 0}, {0, 0, 1, 0}, {1, 0, 1, 0}};
     for (BatchShape kd : kinds) {
-        if (mode && mode[0] && ((mode[0] == 'b') != (kd.wave_items == 0))) continue;
+        if (!allowed(kd.wave_items)) continue;
         const int occ = ttk::batch_max_blocks_per_cu(n, kd);
         std::vector<int> per_cu;
         if (bpc && atoi(bpc) > 0) {
@@ -497,6 +518,7 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
             cand.push_back(kd);
         }
     }
+    if (allowed(kPerTreeGraphs) && !(bpc && atoi(bpc) > 0)) cand.push_back({kPerTreeGraphs, 0, 1, 0});
     if (cand.empty()) return fail(TT_EINVAL, "TT_BATCH_MODE=%s selects no launch shape", mode);
     P->shape = cand[0];
     P->tuned_ms = 0.0;
@@ -506,10 +528,13 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     TT_HIP(hipEventCreate(&e1));
     double best = INFINITY;
     hipError_t e = hipSuccess;
+    int32_t rc = TT_OK;
     for (const BatchShape &c : cand) {
-        e = P->launch(n, tips, c, s);
-        if (e == hipSuccess) e = hipEventRecord(e0, s);
-        for (int r = 0; r < 2 && e == hipSuccess; ++r) e = P->launch(n, tips, c, s);
+        rc = launch_plan(P, n, tips, c, s);
+        if (rc) break;
+        e = hipEventRecord(e0, s);
+        for (int r = 0; r < 2 && e == hipSuccess && !rc; ++r) rc = launch_plan(P, n, tips, c, s);
+        if (rc) break;
         if (e == hipSuccess) e = hipEventRecord(e1, s);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;
@@ -525,6 +550,7 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (rc) return rc;
     if (e != hipSuccess) return fail(TT_EDEVICE, "batch autotune: %s", hipGetErrorString(e));
     P->tuned_ms = best;
     return TT_OK;
@@ -1073,7 +1099,7 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
             delete P;
             return rc;
         }
-        if (g_use_graphs()) {
+        if (g_use_graphs() && P->shape.wave_items != kPerTreeGraphs) {
             hipGraph_t g = nullptr;
             e = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal);
             hipError_t e2 = hipSuccess;
@@ -1107,15 +1133,19 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
                               hipMemcpyHostToDevice, s));
     if (plan->graph) {
         TT_HIP(hipGraphLaunch(plan->graph, s));
-    } else {
-        TT_HIP(plan->launch(n, fl ? 1 : 0, plan->shape, s));
+    } else if (int32_t rc = launch_plan(plan, n, fl ? 1 : 0, plan->shape, s)) {
+        return rc;
     }
     for (int i = 0; i < k; ++i) {
         trees[i]->last_pass = 1;
         trees[i]->have_states = true;
         trees[i]->tips_dirty = fl != 0;
     }
-    if (logL) {
+    if (logL && plan->shape.wave_items == kPerTreeGraphs) {
+        for (int i = 0; i < k; ++i)
+            TT_HIP(hipMemcpyAsync(&logL[i], trees[i]->d.total, sizeof(double), hipMemcpyDeviceToHost, s));
+        TT_HIP(hipStreamSynchronize(s));
+    } else if (logL) {
         const std::vector<BatchItem> &items = plan->items_for(plan->shape);
         std::vector<double> out(items.size());
         TT_HIP(hipMemcpyAsync(out.data(), plan->d_item_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
