@@ -135,6 +135,33 @@ int32_t tt_update_branch(tt_tree *tree, int32_t node, double t, double *logL, vo
 int32_t tt_optimize_branch_lengths(tt_tree *tree, const int16_t *code_state, int32_t gap_state, double max_t,
                                    double tol, double *t_opt, int32_t *status, double *pair_counts, void *stream);
 
+/* tt_optimize_branch_lengths_marginal: the optimum of every branch given the
+ * marginal profiles at its ends, from the last marginal pass.  Replaces
+ * TreeAnc.optimal_marginal_branch_length (treeanc.py:1272-1295) for all
+ * branches, i.e. GTR.optimal_t_compressed(profiles=True) (gtr.py:816-920) on
+ * prob_t_profiles (gtr.py:922-963) with pc = marginal_subtree_LH and
+ * pp = marginal_outgroup_LH (treeanc.py:1122-1146); Brent in s = sqrt(t) on
+ * -logP(s^2) + exp(s^4 / 1e4), bracket (-sqrt(max_t), sqrt(hamming),
+ * sqrt(max_t)), tolerance tol.
+ *   gap_state: the alphabet's gap state (prob_t_profiles weighs patterns by
+ *       the non-gap probability), -1 for none.
+ * Outputs (host): t_opt [n_nodes] (root 0), status [n_nodes] as in
+ * tt_optimize_branch_lengths; root_pair (nullable, [2]): when the root has
+ * exactly two children, the optimum of their summed length with
+ * pp = normalise(second child's subtree * pi), pc = first child's subtree
+ * (optimize_tree_marginal, treeanc.py:1318-1331), and its status (-1 when
+ * the root is not bifurcating).  TT_ESTATE unless the last pass that touched
+ * the profiles was tt_marginal / tt_batch_marginal with the current lengths. */
+int32_t tt_optimize_branch_lengths_marginal(tt_tree *tree, int32_t gap_state, double max_t, double tol,
+                                            double *t_opt, int32_t *status, double *root_pair, void *stream);
+
+/* tt_branch_profile_lh: GTR.prob_t_profiles(return_log=True) of every branch
+ * (profile pair as above) at K branch lengths t[K]: out[n_nodes][K] (host),
+ * root rows 0, t < 0 gives -1e10 (gtr.py:946-947).  The grid evaluation of
+ * BranchLenInterpolator in marginal mode (branch_len_interpolator.py:104-110). */
+int32_t tt_branch_profile_lh(tt_tree *tree, int32_t gap_state, const double *t, int32_t K, double *out,
+                             void *stream);
+
 /* Free the joint-pass buffers (Lx, Cx) and return them to the engine's
  * budget; the next tt_joint allocates them again.  No reference counterpart:
  * it lets a server admit requests against an HBM budget. */

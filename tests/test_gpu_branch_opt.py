@@ -153,3 +153,126 @@ def test_facade_optimize_matches_oracle():
     # the new lengths are used by the next reconstruction
     ta.infer_ancestral_sequences(marginal=True)
     assert np.isfinite(ta.sequence_LH())
+
+
+# ---------------------------------------------------------------- marginal
+# The device profiles are FP32 (<= 2e-7 absolute from the FP64 oracle's), so
+# the objective differs from the oracle's by ~1e-7 relative and its optimum
+# moves by up to ~1e-5 relative on flat optima: lengths agree to 1e-4
+# relative + 1e-8 absolute, and the device optimum is optimal for the
+# oracle's own objective to 1e-9 relative.
+TM_RTOL, TM_ATOL, FM_RTOL = 1e-4, 1e-8, 1e-9
+
+
+def _profile_objective(og, pp, pc, mult, t):
+    s = np.sqrt(t)
+    return -og.prob_t_profiles((pp, pc), mult, t, return_log=True) + np.exp(s ** 4 / 10000)
+
+
+@pytest.mark.parametrize("n_tips,L,seed", [(50, 600, 71), (25, 300, 72)])
+def test_marginal_sweep_matches_oracle(gpu_engine, n_tips, L, seed):
+    from helpers import build_oracle
+    from oracle.fp64 import normalize_profile
+
+    say(f"[branch-opt marginal] tips={n_tips} L={L} seed={seed}")
+    job, prep, alphabet = small_job(n_tips, L, 4, seed, depth=0.05)
+    dev = device_tree(gpu_engine, prep)
+    dev.marginal()
+    t, st, pair = dev.optimize_branch_lengths_marginal(-1, 4.0, 1e-10)
+    ta, nodes, tips = build_oracle(prep, alphabet, job["gtr"])
+    ta.marginal()
+    og = oracle_gtr(job["gtr"], alphabet)
+    mult = prep["multiplicity"]
+    worst = 0.0
+    for v, nd in enumerate(nodes):
+        if nd.up is None:
+            continue
+        pp, pc = nd.marginal_outgroup_LH, nd.marginal_subtree_LH
+        t_ref = og.optimal_t_compressed((pp, pc), mult, profiles=True, tol=1e-10)
+        assert st[v] == 0, (v, st[v])
+        f_dev = _profile_objective(og, pp, pc, mult, t[v])
+        f_ref = _profile_objective(og, pp, pc, mult, t_ref)
+        assert f_dev - f_ref <= FM_RTOL * abs(f_ref), (v, f_dev, f_ref)
+        assert abs(t[v] - t_ref) <= TM_RTOL * t_ref + TM_ATOL, (v, t[v], t_ref)
+        if t_ref > 1e-6:
+            worst = max(worst, abs(t[v] - t_ref) / t_ref)
+    # the bifurcating root's pair (treeanc.py:1318-1331)
+    root = [nd for nd in nodes if nd.up is None][0]
+    assert len(root.clades) == 2 and pair[1] == 0
+    n1, n2 = root.clades
+    pp = normalize_profile(n2.marginal_subtree_LH * og.Pi)[0]
+    pc = n1.marginal_subtree_LH
+    t_ref = og.optimal_t_compressed((pp, pc), mult, profiles=True, tol=1e-10)
+    assert abs(pair[0] - t_ref) <= TM_RTOL * t_ref + TM_ATOL, (pair, t_ref)
+    say(f"  {len(nodes) - 1} branches + root pair, max rel diff of optimal t {worst:.2e} (t > 1e-6)")
+    # grid of prob_t_profiles values (BranchLenInterpolator, marginal mode)
+    grid = np.array([-0.1, 0.0, 1e-4, 0.003, 0.05, 0.4])
+    lh = dev.branch_profile_lh(grid)
+    for v, nd in enumerate(nodes):
+        if nd.up is None:
+            assert np.all(lh[v] == 0)
+            continue
+        ref = [og.prob_t_profiles((nd.marginal_outgroup_LH, nd.marginal_subtree_LH), mult, x, return_log=True)
+               for x in grid]
+        assert np.allclose(lh[v], ref, rtol=1e-6, atol=1e-6), (v, lh[v], ref)
+    dev.close()
+
+
+def test_marginal_sweep_requires_fresh_profiles(gpu_engine):
+    from treetime_amd.errors import MissingDataError
+
+    job, prep, alphabet = small_job(20, 200, 4, 73)
+    dev = device_tree(gpu_engine, prep)
+    dev.joint()
+    with pytest.raises(MissingDataError):
+        dev.optimize_branch_lengths_marginal()
+    dev.marginal()
+    dev.optimize_branch_lengths_marginal()
+    dev.set_branch_lengths(np.array([1], np.int32), np.array([0.01]))
+    with pytest.raises(MissingDataError):
+        dev.branch_profile_lh([0.01])
+    dev.close()
+
+
+def test_facade_optimize_tree_marginal():
+    """TreeAnc.optimize_tree_marginal (treeanc.py:1297-1358): the likelihood
+    does not decrease, and optimal_marginal_branch_length equals the
+    oracle's optimum on the facade's own profiles."""
+    import treetime_amd as tt
+    from oracle import fp64 as O
+    from treetime_amd import synth
+
+    job = synth.make_job(40, 500, 0.05, 4, 75, 0.02, 0.01)
+    parent, bl = job["parent"], job["branch_length"]
+    clades = [tt.Clade(branch_length=float(bl[i]) * 1.5) for i in range(len(parent))]
+    for i, p in enumerate(parent):
+        if p >= 0:
+            clades[p].clades.append(clades[i])
+    for k, v in enumerate(job["tips"]):
+        clades[v].name = job["names"][k]
+    aln = {job["names"][k]: job["chars"][k].tobytes().decode() for k in range(len(job["tips"]))}
+    ta = tt.TreeAnc(tree=tt.Tree(clades[0]), aln=aln, gtr=job["gtr"], verbose=0)
+    ta.infer_ancestral_sequences(marginal=True)
+    lh0 = ta.sequence_LH()
+    # one branch against the oracle run on the facade's tree and alignment
+    nodes = list(ta.tree.find_clades(order="preorder"))
+    idx = {id(x): i for i, x in enumerate(nodes)}
+    par = [-1] * len(nodes)
+    for i, x in enumerate(nodes):
+        for c in x.clades:
+            par[idx[id(c)]] = i
+    root, onodes = O.build_tree(par, [ta._branch_length_to_gtr(x) for x in nodes], [x.name for x in nodes])
+    g = ta.gtr
+    og = O.GTRModel("nuc_nogap", mu=g.mu, pi=g.Pi, W=g.W)
+    orc = O.OracleTreeAnc(root, og, ta.data.compressed_alignment, ta.data.multiplicity(),
+                          full_length=ta.data.full_length)
+    orc.marginal()
+    k = 7
+    got = ta.optimal_marginal_branch_length(nodes[k])
+    ref = og.optimal_t_compressed((onodes[k].marginal_outgroup_LH, onodes[k].marginal_subtree_LH),
+                                  ta.data.multiplicity(), profiles=True, tol=1e-10)
+    assert abs(got - ref) <= TM_RTOL * ref + TM_ATOL, (got, ref)
+    assert ta.optimize_tree_marginal(max_iter=4) == "success"
+    lh1 = ta.sequence_LH()
+    say(f"[branch-opt marginal] facade: logL {lh0:.3f} -> {lh1:.3f}")
+    assert lh1 >= lh0 - 1e-6

@@ -142,6 +142,9 @@ struct tt_tree {
     int last_pass = 0;  // 1 marginal, 2 joint, 3 loglik
     bool have_states = false;  // d.states holds a marginal or joint reconstruction
     bool tips_dirty = false;   // tip rows of d.states hold reconstructed states (else 255)
+    bool profiles_fresh = false;  // d.prof and d.sub belong to the current lengths (last marginal pass)
+    double *d_grid_t = nullptr, *d_grid_out = nullptr;
+    int grid_cap = 0;
     int16_t *d_code_state = nullptr;  // branch optimisation buffers, allocated on first use
     double *d_pair_counts = nullptr, *d_t_opt = nullptr;
     int32_t *d_opt_status = nullptr;
@@ -423,6 +426,8 @@ int32_t run_pass(tt_tree *T, int mode, uint32_t flags, double *logL, void *strea
         T->have_states = true;
         T->tips_dirty = tips;
     }
+    if (mode == 1) T->profiles_fresh = true;
+    if (mode == 3) T->profiles_fresh = false;  // subtree rows rewritten, profiles not
     if (T->update_mode && (mode == 1 || mode == 3)) T->offs_valid = true;
     if (logL) {
         TT_HIP(hipMemcpyAsync(logL, T->d.total, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -461,6 +466,16 @@ int32_t check_range(tt_tree *T, const double *t, int k) {
                         "large (mu*t*lambda = %g)",
                         T->mu * t[i] * lmax);
     }
+    return TT_OK;
+}
+
+// Per-branch outputs of the branch-length sweeps: [n_nodes + 1] (the extra
+// entry is the bifurcating root's pair of children).
+int32_t ensure_opt_out(tt_tree *T) {
+    if (T->d_t_opt) return TT_OK;
+    int32_t rc;
+    if ((rc = T->alloc((void **)&T->d_t_opt, (size_t)(T->n_nodes + 1) * sizeof(double)))) return rc;
+    if ((rc = T->alloc((void **)&T->d_opt_status, (size_t)(T->n_nodes + 1) * sizeof(int32_t)))) return rc;
     return TT_OK;
 }
 
@@ -505,6 +520,7 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     const BatchShape kinds[] = {{0, 0, 2, 5}, {0, 0, 2, 0}, {0, 0, 1, 0}, {1, 0, 1, 0}};
     for (BatchShape kd : kinds) {
         if (!allowed(kd.wave_items)) continue;
+        if (!ttk::batch_all_shapes(n) && !(kd.wave_items == 0 && kd.tiles == 1)) continue;
         const int occ = ttk::batch_max_blocks_per_cu(n, kd);
         std::vector<int> per_cu;
         if (bpc && atoi(bpc) > 0) {
@@ -738,6 +754,7 @@ int32_t tt_tree_set_model(tt_tree *T, const double *eigvals, const double *v, co
     T->vinv.assign(v_inv, v_inv + n * n);
     T->mu = mu;
     T->offs_valid = false;
+    T->profiles_fresh = false;
     int32_t rc = check_range(T, T->t.data(), T->n_nodes);
     if (rc) return rc;
     TT_HIP(hipSetDevice(T->eng->device));
@@ -756,7 +773,10 @@ int32_t set_lengths(tt_tree *T, const int32_t *nodes, const double *t, int32_t k
 
 int32_t tt_tree_set_branch_lengths(tt_tree *T, const int32_t *nodes, const double *t, int32_t k) {
     int32_t rc = set_lengths(T, nodes, t, k);
-    if (rc == TT_OK && k > 0) T->offs_valid = false;
+    if (rc == TT_OK && k > 0) {
+        T->offs_valid = false;
+        T->profiles_fresh = false;
+    }
     return rc;
 }
 
@@ -822,6 +842,7 @@ int32_t tt_update_branch(tt_tree *T, int32_t node, double t, double *logL, void 
     TT_HIP(ttk::path_update(T->d, node, s));
     TT_HIP(ttk::final_sum(T->d, ttk::root_blocks(T->P), s));
     T->last_pass = 3;
+    T->profiles_fresh = false;
     if (logL) {
         TT_HIP(hipMemcpyAsync(logL, T->d.total, sizeof(double), hipMemcpyDeviceToHost, s));
         TT_HIP(hipStreamSynchronize(s));
@@ -842,11 +863,10 @@ int32_t tt_optimize_branch_lengths(tt_tree *T, const int16_t *code_state, int32_
     TT_HIP(hipSetDevice(T->eng->device));
     int32_t rc;
     const size_t nn = (size_t)T->n * T->n;
+    if ((rc = ensure_opt_out(T))) return rc;
     if (!T->d_code_state) {
         if ((rc = T->alloc((void **)&T->d_code_state, 256 * sizeof(int16_t)))) return rc;
         if ((rc = T->alloc((void **)&T->d_pair_counts, (size_t)T->n_nodes * nn * sizeof(double)))) return rc;
-        if ((rc = T->alloc((void **)&T->d_t_opt, (size_t)T->n_nodes * sizeof(double)))) return rc;
-        if ((rc = T->alloc((void **)&T->d_opt_status, (size_t)T->n_nodes * sizeof(int32_t)))) return rc;
     }
     hipStream_t s = stream ? (hipStream_t)stream : T->stream;
     TT_HIP(hipMemcpyAsync(T->d_code_state, code_state, 256 * sizeof(int16_t), hipMemcpyHostToDevice, s));
@@ -857,6 +877,66 @@ int32_t tt_optimize_branch_lengths(tt_tree *T, const int16_t *code_state, int32_
     if (pair_counts)
         TT_HIP(hipMemcpyAsync(pair_counts, T->d_pair_counts, (size_t)T->n_nodes * nn * sizeof(double),
                               hipMemcpyDeviceToHost, s));
+    TT_HIP(hipStreamSynchronize(s));
+    return TT_OK;
+}
+
+int32_t tt_optimize_branch_lengths_marginal(tt_tree *T, int32_t gap_state, double max_t, double tol, double *t_opt,
+                                            int32_t *status, double *root_pair, void *stream) {
+    if (!T || !t_opt || !status) return fail(TT_EINVAL, "NULL argument");
+    if (!T->have_model) return fail(TT_ESTATE, "tt_tree_set_model has not been called");
+    if (!T->profiles_fresh)
+        return fail(TT_ESTATE, "marginal ancestral inference needs to be performed first (profiles are stale)");
+    if (gap_state < -1 || gap_state >= T->n) return fail(TT_EINVAL, "gap_state %d outside [-1, %d)", gap_state, T->n);
+    if (!(max_t > 0.0) || !(tol >= 0.0)) return fail(TT_EINVAL, "need max_t > 0 and tol >= 0");
+    TT_HIP(hipSetDevice(T->eng->device));
+    int32_t rc;
+    if ((rc = ensure_opt_out(T))) return rc;
+    // children of a bifurcating root, in child order (treeanc.py:1318-1320)
+    int n1 = -1, n2 = -1;
+    const int rr = T->int_row[T->root];
+    if (rr >= 0 && T->child_ptr[rr + 1] - T->child_ptr[rr] == 2) {
+        n1 = T->child_node[T->child_ptr[rr]];
+        n2 = T->child_node[T->child_ptr[rr] + 1];
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : T->stream;
+    TT_HIP(ttk::branch_opt_marginal(T->d, gap_state, T->d_eig, T->d_v, T->d_vinv, T->mu, max_t, tol, n1, n2,
+                                    T->d_t_opt, T->d_opt_status, s));
+    TT_HIP(hipMemcpyAsync(t_opt, T->d_t_opt, (size_t)T->n_nodes * sizeof(double), hipMemcpyDeviceToHost, s));
+    TT_HIP(hipMemcpyAsync(status, T->d_opt_status, (size_t)T->n_nodes * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    double pair_t = NAN;
+    int32_t pair_st = -1;
+    if (n1 >= 0) {
+        TT_HIP(hipMemcpyAsync(&pair_t, T->d_t_opt + T->n_nodes, sizeof(double), hipMemcpyDeviceToHost, s));
+        TT_HIP(hipMemcpyAsync(&pair_st, T->d_opt_status + T->n_nodes, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    }
+    TT_HIP(hipStreamSynchronize(s));
+    if (root_pair) {
+        root_pair[0] = pair_t;
+        root_pair[1] = (double)pair_st;
+    }
+    return TT_OK;
+}
+
+int32_t tt_branch_profile_lh(tt_tree *T, int32_t gap_state, const double *t, int32_t K, double *out, void *stream) {
+    if (!T || (K > 0 && (!t || !out))) return fail(TT_EINVAL, "NULL argument");
+    if (K <= 0) return TT_OK;
+    if (!T->have_model) return fail(TT_ESTATE, "tt_tree_set_model has not been called");
+    if (!T->profiles_fresh)
+        return fail(TT_ESTATE, "marginal ancestral inference needs to be performed first (profiles are stale)");
+    if (gap_state < -1 || gap_state >= T->n) return fail(TT_EINVAL, "gap_state %d outside [-1, %d)", gap_state, T->n);
+    TT_HIP(hipSetDevice(T->eng->device));
+    int32_t rc;
+    if (K > T->grid_cap) {
+        if ((rc = T->alloc((void **)&T->d_grid_t, (size_t)K * sizeof(double)))) return rc;
+        if ((rc = T->alloc((void **)&T->d_grid_out, (size_t)K * T->n_nodes * sizeof(double)))) return rc;
+        T->grid_cap = K;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : T->stream;
+    TT_HIP(hipMemcpyAsync(T->d_grid_t, t, (size_t)K * sizeof(double), hipMemcpyHostToDevice, s));
+    TT_HIP(ttk::branch_profile_lh(T->d, gap_state, T->d_eig, T->d_v, T->d_vinv, T->mu, T->d_grid_t, K,
+                                  T->d_grid_out, s));
+    TT_HIP(hipMemcpyAsync(out, T->d_grid_out, (size_t)K * T->n_nodes * sizeof(double), hipMemcpyDeviceToHost, s));
     TT_HIP(hipStreamSynchronize(s));
     return TT_OK;
 }
@@ -1140,6 +1220,7 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
         trees[i]->last_pass = 1;
         trees[i]->have_states = true;
         trees[i]->tips_dirty = fl != 0;
+        trees[i]->profiles_fresh = true;
     }
     if (logL && plan->shape.wave_items == kPerTreeGraphs) {
         for (int i = 0; i < k; ++i)

@@ -232,17 +232,25 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal_w(const BatchTreeDe
     }
 }
 
+// Wide alphabets (n = 20: 256 VGPRs with spills already at one tile) get only
+// the one-tile block kernel; the other shapes are instantiated for n <= 8.
+constexpr bool all_shapes(int n) { return n <= 8; }
+
 template <int N>
 int max_blocks_per_cu(const BatchShape &c) {
     int per_cu = 0;
-    if (c.wave_items)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal_w<N>, TT_BLOCK, 0);
-    else if (c.tiles == 2 && c.waves_per_simd == 5)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N, 2, 5>, TT_BLOCK, 0);
-    else if (c.tiles == 2)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N, 2, 1>, TT_BLOCK, 0);
-    else
+    if constexpr (!all_shapes(N)) {
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N, 1, 1>, TT_BLOCK, 0);
+    } else {
+        if (c.wave_items)
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal_w<N>, TT_BLOCK, 0);
+        else if (c.tiles == 2 && c.waves_per_simd == 5)
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N, 2, 5>, TT_BLOCK, 0);
+        else if (c.tiles == 2)
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N, 2, 1>, TT_BLOCK, 0);
+        else
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_marginal<N, 1, 1>, TT_BLOCK, 0);
+    }
     return std::max(1, per_cu);
 }
 
@@ -270,6 +278,8 @@ int ttk::device_cus() {
     return cus;
 }
 
+bool ttk::batch_all_shapes(int n) { return all_shapes(n); }
+
 int ttk::batch_max_blocks_per_cu(int n, const BatchShape &c) {
     auto f = [&]() -> int {
         TT_BATCH_DISPATCH(n, return max_blocks_per_cu<N_>(c));
@@ -284,7 +294,11 @@ hipError_t ttk::batch_marginal(int n, const BatchTreeDev *trees, const BatchItem
     const int resident = device_cus() * std::max(1, shape.blocks_per_cu);
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, queue);
     TT_BATCH_DISPATCH(n, {
-        if (shape.wave_items) {
+        if constexpr (!all_shapes(N_)) {
+            const int grid = std::min(resident, n_items);
+            hipLaunchKernelGGL((k_batch_marginal<N_, 1, 1>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items,
+                               n_items, queue, item_out, tips);
+        } else if (shape.wave_items) {
             const int grid = std::min(resident, (n_items + kWaves - 1) / kWaves);
             hipLaunchKernelGGL(k_batch_marginal_w<N_>, dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items,
                                queue, item_out, tips);

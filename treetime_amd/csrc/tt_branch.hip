@@ -34,6 +34,83 @@ constexpr double kAbsTol = 1.0e-11;
 constexpr int kMaxIter = 500;
 constexpr double kSuperTiny = 1.0e-24;  // config.py SUPERTINY_NUMBER
 
+// Minimise f in s from the bracket (xa, xb, xc); every caller thread runs the
+// same iterates (f returns identical bits in all of them).  st: 0 ok, 1 the
+// bracket condition fails, 2 no convergence.
+template <class F>
+__device__ inline double brent_minimize(F &f, double xa, double xb, double xc, double tol, int &st) {
+    st = 0;
+    double x = xb;
+    if (!(xa < xb && xb < xc)) {
+        st = 1;
+        return x;
+    }
+    const double fa = f(xa), fb = f(xb), fc = f(xc);
+    if (!(fb < fa && fb < fc)) {
+        st = 1;
+        return x;
+    }
+    double a = xa, b = xc;
+    double w = xb, vv = xb;
+    double fx = fb, fw = fb, fv = fb;
+    double dx = 0.0, step = 0.0;
+    int it = 0;
+    for (; it < kMaxIter; ++it) {
+        const double tol1 = tol * fabs(x) + kAbsTol, tol2 = 2.0 * tol1;
+        const double mid = 0.5 * (a + b);
+        if (fabs(x - mid) < tol2 - 0.5 * (b - a)) break;
+        bool golden = fabs(dx) <= tol1;
+        if (!golden) {
+            // parabola through (x, fx), (w, fw), (vv, fv)
+            const double r = (x - w) * (fx - fv);
+            double q = (x - vv) * (fx - fw);
+            double p = (x - vv) * q - (x - w) * r;
+            q = 2.0 * (q - r);
+            if (q > 0.0) p = -p;
+            q = fabs(q);
+            const double dx_prev = dx;
+            dx = step;
+            if (p > q * (a - x) && p < q * (b - x) && fabs(p) < fabs(0.5 * q * dx_prev)) {
+                step = p * 1.0 / q;
+                const double u = x + step;
+                if (u - a < tol2 || b - u < tol2) step = mid - x >= 0 ? tol1 : -tol1;
+            } else {
+                golden = true;
+            }
+        }
+        if (golden) {
+            dx = x >= mid ? a - x : b - x;
+            step = kGoldenFrac * dx;
+        }
+        const double u = fabs(step) < tol1 ? (step >= 0 ? x + tol1 : x - tol1) : x + step;
+        const double fu = f(u);
+        if (fu > fx) {
+            if (u < x) a = u;
+            else b = u;
+            if (fu <= fw || w == x) {
+                vv = w;
+                fv = fw;
+                w = u;
+                fw = fu;
+            } else if (fu <= fv || vv == x || vv == w) {
+                vv = u;
+                fv = fu;
+            }
+        } else {
+            if (u >= x) a = x;
+            else b = x;
+            vv = w;
+            fv = fw;
+            w = x;
+            fw = fx;
+            x = u;
+            fx = fu;
+        }
+    }
+    if (it >= kMaxIter || isnan(x) || isnan(fx)) st = 2;
+    return x;
+}
+
 __device__ inline double wave_total(double x) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
@@ -133,80 +210,215 @@ __global__ __launch_bounds__(TT_BLOCK) void k_branch_opt(TreeDev T, const int16_
     diff = wave_total(diff);
     const double hamming = diff / all;  // NaN without any counted pair
 
-    const double xa = -sqrt(max_t), xb = sqrt(hamming), xc = sqrt(max_t);
     int st = 0;
-    double x = xb;
-    if (!(xa < xb && xb < xc)) {
-        st = 1;
-    } else {
-        const double fa = f(xa), fb = f(xb), fc = f(xc);
-        if (!(fb < fa && fb < fc)) {
-            st = 1;
-        } else {
-            double a = xa, b = xc;
-            double w = xb, vv = xb;
-            double fx = fb, fw = fb, fv = fb;
-            double dx = 0.0, step = 0.0;
-            int it = 0;
-            for (; it < kMaxIter; ++it) {
-                const double tol1 = tol * fabs(x) + kAbsTol, tol2 = 2.0 * tol1;
-                const double mid = 0.5 * (a + b);
-                if (fabs(x - mid) < tol2 - 0.5 * (b - a)) break;
-                bool golden = fabs(dx) <= tol1;
-                if (!golden) {
-                    // parabola through (x, fx), (w, fw), (vv, fv)
-                    const double r = (x - w) * (fx - fv);
-                    double q = (x - vv) * (fx - fw);
-                    double p = (x - vv) * q - (x - w) * r;
-                    q = 2.0 * (q - r);
-                    if (q > 0.0) p = -p;
-                    q = fabs(q);
-                    const double dx_prev = dx;
-                    dx = step;
-                    if (p > q * (a - x) && p < q * (b - x) && fabs(p) < fabs(0.5 * q * dx_prev)) {
-                        step = p * 1.0 / q;
-                        const double u = x + step;
-                        if (u - a < tol2 || b - u < tol2) step = mid - x >= 0 ? tol1 : -tol1;
-                    } else {
-                        golden = true;
-                    }
-                }
-                if (golden) {
-                    dx = x >= mid ? a - x : b - x;
-                    step = kGoldenFrac * dx;
-                }
-                const double u = fabs(step) < tol1 ? (step >= 0 ? x + tol1 : x - tol1) : x + step;
-                const double fu = f(u);
-                if (fu > fx) {
-                    if (u < x) a = u;
-                    else b = u;
-                    if (fu <= fw || w == x) {
-                        vv = w;
-                        fv = fw;
-                        w = u;
-                        fw = fu;
-                    } else if (fu <= fv || vv == x || vv == w) {
-                        vv = u;
-                        fv = fu;
-                    }
-                } else {
-                    if (u >= x) a = x;
-                    else b = x;
-                    vv = w;
-                    fv = fw;
-                    w = x;
-                    fw = fx;
-                    x = u;
-                    fx = fu;
-                }
-            }
-            if (it >= kMaxIter || isnan(x) || isnan(fx)) st = 2;
-        }
-    }
+    const double x = brent_minimize(f, -sqrt(max_t), sqrt(hamming), sqrt(max_t), tol, st);
     if (lane == 0) {
         // failed searches fall back to the Hamming distance (gtr.py:916-918)
         t_out[node] = st == 0 ? x * x : (st == 2 ? hamming : NAN);
         status[node] = st;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Marginal branch lengths (treeanc.py:1272-1295 optimal_marginal_branch_length,
+// :1297-1358 optimize_tree_marginal): per branch, Brent in s on
+//   f(s) = -sum_p mult_p w_p log(pc_p . P(s^2) pp_p + 1e-24) + exp(s^4 / 1e4)
+// (gtr.py:816-920 with profiles=True, gtr.py:922-963 prob_t_profiles), where
+// pc is the child's marginal_subtree_LH (tips: the character's profile,
+// internal nodes: the subtree row normalised to sum 1) and pp its
+// marginal_outgroup_LH = normalise(max(TINY, parent profile) / m), m the
+// child's message to the parent at the current length (treeanc.py:895-899);
+// w_p = (1 - pp_gap)(1 - pc_gap) when the alphabet has a gap, else 1.  For
+// the children n1, n2 of a bifurcating root the reference optimises their
+// summed length with pc = n1's subtree and pp = normalise(n2's subtree * pi)
+// (treeanc.py:1318-1331); that pair is the extra workgroup n_nodes.
+//
+// One workgroup per branch.  Every evaluation rebuilds P(t) in LDS (FP64,
+// clipped at 0) and streams the branch's patterns (profiles recomputed from
+// the FP32 rows, arithmetic in FP64); the per-pattern terms are summed in a
+// fixed order (per thread, then a fixed LDS tree), so every thread holds the
+// same bits and runs the same Brent iterates.
+
+constexpr int kOptThreads = TT_BLOCK;
+
+__device__ inline double block_total(double x, double *s_red) {
+    s_red[threadIdx.x] = x;
+    __syncthreads();
+    for (int w = kOptThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s_red[threadIdx.x] += s_red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double t = s_red[0];
+    __syncthreads();
+    return t;
+}
+
+// Branch profiles of pattern p: pc (child side) and pp (parent side), FP64.
+template <int N>
+__device__ inline void branch_profiles(const TreeDev &T, int node, int pair_other, int p, double (&pc)[N],
+                                       double (&pp)[N]) {
+    const int ir = T.int_row[node];
+    if (ir >= 0) {
+        const float TT_G *r = T.sub + ((size_t)ir * T.P + p) * N;
+        double z = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            pc[i] = (double)r[i];
+            z += pc[i];
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) pc[i] /= z;
+    } else {
+        const int code = T.tip_codes[(size_t)T.tip_row[node] * T.P + p];
+#pragma unroll
+        for (int i = 0; i < N; ++i) pc[i] = (double)T.lut[code * N + i];
+    }
+    double z = 0.0;
+    if (pair_other >= 0) {  // bifurcating root: the sibling's subtree times pi
+        const int jr = T.int_row[pair_other];
+        if (jr >= 0) {
+            const float TT_G *r = T.sub + ((size_t)jr * T.P + p) * N;
+#pragma unroll
+            for (int j = 0; j < N; ++j) pp[j] = (double)r[j] * T.pi[j];
+        } else {
+            const int code = T.tip_codes[(size_t)T.tip_row[pair_other] * T.P + p];
+#pragma unroll
+            for (int j = 0; j < N; ++j) pp[j] = (double)T.lut[code * N + j] * T.pi[j];
+        }
+    } else {  // outgroup: max(TINY, parent profile) / (child message at the current length)
+        const float TT_G *pr = T.prof + ((size_t)T.parent[node] * T.P + p) * N;
+        const double TT_C *M = T.P64 + (size_t)node * (N * N);
+        double r[N];
+        if (ir >= 0) {
+            const float TT_G *sr = T.sub + ((size_t)ir * T.P + p) * N;
+#pragma unroll
+            for (int i = 0; i < N; ++i) r[i] = (double)sr[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) r[i] = pc[i];
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            double m = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) m = fma(r[i], M[i * N + j], m);
+            pp[j] = fmax(TT_TINY, (double)pr[j]) / m;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) z += pp[j];
+#pragma unroll
+    for (int j = 0; j < N; ++j) pp[j] /= z;
+}
+
+// P(t) = v diag(exp(mu t lambda)) v_inv clipped at 0, into LDS (gtr.py:1027-1067).
+template <int N>
+__device__ inline void build_expqt(double t, const double *eig, const double *v, const double *vinv, double mu,
+                                   double *s_P) {
+    const double mt = mu * t;
+    for (int k = threadIdx.x; k < N * N; k += blockDim.x) {
+        const int i = k / N, j = k % N;
+        double a = 0.0;
+        for (int m = 0; m < N; ++m) a = fma(v[i * N + m], exp(mt * eig[m]) * vinv[m * N + j], a);
+        s_P[k] = fmax(0.0, a);
+    }
+    __syncthreads();
+}
+
+// sum_p mult_p w_p log(pc_p . P pp_p + 1e-24) over this branch's patterns.
+template <int N>
+__device__ inline double profile_loglik(const TreeDev &T, int node, int other, int gap, const double *s_P,
+                                        double *s_red) {
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < T.P; p += blockDim.x) {
+        double pc[N], pp[N];
+        branch_profiles<N>(T, node, other, p, pc, pp);
+        double res = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double row = 0.0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) row = fma(s_P[i * N + j], pp[j], row);
+            res = fma(pc[i], row, res);
+        }
+        double w = T.mult[p];
+        if (gap >= 0) w *= (1.0 - pp[gap]) * (1.0 - pc[gap]);
+        acc = fma(w, log(res + kSuperTiny), acc);
+    }
+    return block_total(acc, s_red);
+}
+
+template <int N>
+__global__ __launch_bounds__(kOptThreads) void k_branch_opt_marginal(TreeDev T, int gap, const double *eig,
+                                                                     const double *v, const double *vinv, double mu,
+                                                                     double max_t, double tol, int pair_n1,
+                                                                     int pair_n2, double *t_out, int32_t *status) {
+    __shared__ double s_P[N * N];
+    __shared__ double s_red[kOptThreads];
+    const int b = blockIdx.x;
+    int node = b, other = -1;
+    if (b == T.n_nodes) {  // the bifurcating root's pair of children
+        if (pair_n1 < 0) return;
+        node = pair_n1;
+        other = pair_n2;
+    } else if (T.parent[b] < 0) {
+        if (threadIdx.x == 0) {
+            t_out[b] = 0.0;
+            status[b] = 0;
+        }
+        return;
+    }
+    // Hamming distance of the profile pair (gtr.py:860-863)
+    double same = 0.0, all = 0.0;
+    for (int p = threadIdx.x; p < T.P; p += blockDim.x) {
+        double pc[N], pp[N];
+        branch_profiles<N>(T, node, other, p, pc, pp);
+        double d = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) d = fma(pp[j], pc[j], d);
+        same = fma(T.mult[p], d, same);
+        all += T.mult[p];
+    }
+    same = block_total(same, s_red);
+    all = block_total(all, s_red);
+    const double hamming = 1.0 - same / all;
+
+    auto f = [&](double s) {
+        build_expqt<N>(s * s, eig, v, vinv, mu, s_P);
+        const double lp = profile_loglik<N>(T, node, other, gap, s_P, s_red);
+        return -lp + exp(s * s * s * s / 10000.0);
+    };
+    int st = 0;
+    const double x = brent_minimize(f, -sqrt(max_t), sqrt(hamming), sqrt(max_t), tol, st);
+    if (threadIdx.x == 0) {
+        t_out[b] = st == 0 ? x * x : (st == 2 ? hamming : NAN);
+        status[b] = st;
+    }
+}
+
+// prob_t_profiles (return_log=True) of every branch at K lengths t[k]
+// (BranchLenInterpolator grids, branch_len_interpolator.py:104-110).
+template <int N>
+__global__ __launch_bounds__(kOptThreads) void k_branch_profile_lh(TreeDev T, int gap, const double *eig,
+                                                                   const double *v, const double *vinv, double mu,
+                                                                   const double *t, int K, double *out) {
+    __shared__ double s_P[N * N];
+    __shared__ double s_red[kOptThreads];
+    const int node = blockIdx.x;
+    if (T.parent[node] < 0) {
+        for (int k = threadIdx.x; k < K; k += blockDim.x) out[(size_t)node * K + k] = 0.0;
+        return;
+    }
+    for (int k = 0; k < K; ++k) {
+        const double tk = t[k];
+        double lp;
+        if (tk < 0.0) {
+            lp = -1e10;  // config.py BIG_NUMBER (gtr.py:946-947)
+        } else {
+            build_expqt<N>(tk, eig, v, vinv, mu, s_P);
+            lp = profile_loglik<N>(T, node, -1, gap, s_P, s_red);
+        }
+        if (threadIdx.x == 0) out[(size_t)node * K + k] = lp;
     }
 }
 
@@ -225,6 +437,22 @@ __global__ __launch_bounds__(TT_BLOCK) void k_branch_opt(TreeDev T, const int16_
     }
 
 namespace ttk {
+hipError_t branch_opt_marginal(const TreeDev &T, int gap, const double *eig, const double *v, const double *vinv,
+                               double mu, double max_t, double tol, int pair_n1, int pair_n2, double *t_out,
+                               int32_t *status, hipStream_t s) {
+    TT_DISPATCH_ALL(T.n, hipLaunchKernelGGL(k_branch_opt_marginal<N_>, dim3((unsigned)T.n_nodes + 1),
+                                            dim3(kOptThreads), 0, s, T, gap, eig, v, vinv, mu, max_t, tol, pair_n1,
+                                            pair_n2, t_out, status));
+    return hipGetLastError();
+}
+
+hipError_t branch_profile_lh(const TreeDev &T, int gap, const double *eig, const double *v, const double *vinv,
+                             double mu, const double *t, int K, double *out, hipStream_t s) {
+    TT_DISPATCH_ALL(T.n, hipLaunchKernelGGL(k_branch_profile_lh<N_>, dim3((unsigned)T.n_nodes), dim3(kOptThreads), 0,
+                                            s, T, gap, eig, v, vinv, mu, t, K, out));
+    return hipGetLastError();
+}
+
 hipError_t branch_opt(const TreeDev &T, const int16_t *code_state, int gap, const double *eig, const double *v,
                       const double *vinv, double mu, double max_t, double tol, double *counts, double *t_out,
                       int32_t *status, hipStream_t s) {

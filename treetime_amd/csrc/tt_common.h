@@ -95,6 +95,13 @@ bool mfma_enabled();
 hipError_t branch_opt(const TreeDev &T, const int16_t *code_state, int gap, const double *eig, const double *v,
                       const double *vinv, double mu, double max_t, double tol, double *counts, double *t_out,
                       int32_t *status, hipStream_t s);
+// marginal branch lengths: Brent on the profile pair of every branch (+ the root pair as entry n_nodes)
+hipError_t branch_opt_marginal(const TreeDev &T, int gap, const double *eig, const double *v, const double *vinv,
+                               double mu, double max_t, double tol, int pair_n1, int pair_n2, double *t_out,
+                               int32_t *status, hipStream_t s);
+// prob_t_profiles of every branch at K lengths, out [n_nodes][K]
+hipError_t branch_profile_lh(const TreeDev &T, int gap, const double *eig, const double *v, const double *vinv,
+                             double mu, const double *t, int K, double *out, hipStream_t s);
 }  // namespace ttk
 
 // A tree as seen by the persistent batched kernel: the tree buffers plus its
@@ -131,5 +138,7 @@ hipError_t batch_marginal(int n, const BatchTreeDev *trees, const BatchItem *ite
                           double *item_out, int tips, const BatchShape &shape, hipStream_t s);
 // occupancy limit (blocks per CU) of the batched kernel for n states and item shape; 0 if n unsupported
 int batch_max_blocks_per_cu(int n, const BatchShape &shape);
+// false when only the one-tile block kernel exists for n states
+bool batch_all_shapes(int n);
 int device_cus();
 }  // namespace ttk

@@ -165,6 +165,28 @@ class DeviceTree:
             _native.ptr(cnt, C.c_double) if with_counts else None, stream))
         return t, st, cnt
 
+    def optimize_branch_lengths_marginal(self, gap_state=-1, max_t=4.0, tol=1e-10, stream=None):
+        """Optimal length of every branch from the profile pairs of the last
+        marginal pass (tt_optimize_branch_lengths_marginal).  Returns
+        (t_opt, status, (t_pair, status_pair)) -- the last for the summed
+        length of a bifurcating root's two children, (nan, -1) otherwise."""
+        t = np.empty(self.n_nodes, dtype=np.float64)
+        st = np.empty(self.n_nodes, dtype=np.int32)
+        pair = np.empty(2, dtype=np.float64)
+        _native.check(self.lib.tt_optimize_branch_lengths_marginal(
+            self.handle, int(gap_state), float(max_t), float(tol), _native.ptr(t, C.c_double),
+            _native.ptr(st, C.c_int32), _native.ptr(pair, C.c_double), stream))
+        return t, st, (float(pair[0]), int(pair[1]))
+
+    def branch_profile_lh(self, t, gap_state=-1, stream=None):
+        """prob_t_profiles(return_log=True) of every branch at the lengths t:
+        (n_nodes, K)."""
+        tt = _native.as_c(np.atleast_1d(t), np.float64)
+        out = np.empty((self.n_nodes, len(tt)), dtype=np.float64)
+        _native.check(self.lib.tt_branch_profile_lh(self.handle, int(gap_state), _native.ptr(tt, C.c_double),
+                                                    len(tt), _native.ptr(out, C.c_double), stream))
+        return out
+
     def trim(self):
         """Free the joint-pass buffers (returned to the engine budget)."""
         _native.check(self.lib.tt_tree_trim(self.handle))

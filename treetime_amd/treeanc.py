@@ -499,6 +499,94 @@ class TreeAnc:
             return self.one_mutation
         return max(0.0, self._take_optimum(node, self._branch_optima()))
 
+    # ------------------------------------------------- marginal branch lengths
+    def _marginal_optima(self, tol):
+        """Device sweep over all branches (tt_optimize_branch_lengths_marginal)
+        with the profiles of the last marginal pass: {id(node): (t, status)}
+        and (t, status) of a bifurcating root's pair of children."""
+        dev = self._device
+        if dev is None or self._nodes is None or self.sequence_reconstruction != "marginal":
+            raise MissingDataError("marginal ancestral inference needs to be performed first!")
+        nodes = list(self.tree.find_clades(order="preorder"))
+        if len(nodes) != len(self._nodes) or any(a is not b for a, b in zip(nodes, self._nodes)):
+            raise MissingDataError("the tree changed since the last marginal reconstruction")
+        g = self.gtr
+        gap = g.gap_index if g.gap_index is not None else -1
+        t, st, pair = dev.optimize_branch_lengths_marginal(gap, ttconf.MAX_BRANCH_LENGTH, tol)
+        return {id(n): (float(t[i]), int(st[i])) for i, n in enumerate(nodes)}, pair
+
+    @staticmethod
+    def _checked(t, st):
+        if st == 1:
+            raise ValueError("Bracketing values (xa, xb, xc) do not fulfill this requirement: "
+                             "(f(xb) < f(xa)) and (f(xb) < f(xc))")
+        return t
+
+    def optimal_marginal_branch_length(self, node, tol=1e-10):
+        """treeanc.py:1272-1295 (the device computes every branch at once)."""
+        if node.up is None:
+            return self.one_mutation
+        res, _ = self._marginal_optima(tol)
+        return self._checked(*res[id(node)])
+
+    def optimize_tree_marginal(self, max_iter=10, infer_gtr=False, pc=1.0, damping=0.75, LHtol=0.1,
+                               site_specific_gtr=False, **kwargs):
+        """treeanc.py:1297-1358: iterate (all branch optima from the current
+        marginal profiles, damped update, marginal reconstruction) until the
+        likelihood gain drops below LHtol."""
+        if infer_gtr:
+            raise NotImplementedError("GTR inference is not part of the GPU pruning engine")
+        self.infer_ancestral_sequences(marginal=True, **kwargs)
+        oldLH = self.sequence_LH()
+        self.logger("TreeAnc.optimize_tree_marginal: initial, LH=%1.2f, total branch_length %1.4f"
+                    % (oldLH, self.tree.total_branch_length()), 2)
+        for i in range(max_iter):
+            old_bl = self.tree.total_branch_length()
+            tol = 1e-8 + 0.01 ** (i + 1)
+            res, pair = self._marginal_optima(tol)
+            keep = damping ** (i + 1)
+            for n in self.tree.find_clades():
+                if n.up is None:
+                    continue
+                if n.up.up is None and len(n.up.clades) == 2:
+                    # children of a bifurcating root (visited twice, as in the reference)
+                    n1, n2 = n.up.clades
+                    total_bl = n1.branch_length + n2.branch_length
+                    bl_ratio = n1.branch_length / total_bl
+                    new_bl = self._checked(*pair)
+                    update_val = new_bl * (1 - keep) + total_bl * keep
+                    n1.branch_length = update_val * bl_ratio
+                    n2.branch_length = update_val * (1 - bl_ratio)
+                    n1.mutation_length = n1.branch_length
+                    n2.mutation_length = n2.branch_length
+                else:
+                    new_val = self._checked(*res[id(n)])
+                    n.branch_length = new_val * (1 - keep) + n.branch_length * keep
+                    n.mutation_length = n.branch_length
+            self.infer_ancestral_sequences(marginal=True, **kwargs)
+            LH = self.sequence_LH()
+            deltaLH = LH - oldLH
+            oldLH = LH
+            dbl = self.tree.total_branch_length() - old_bl
+            self.logger("TreeAnc.optimize_tree_marginal: iteration %d, LH=%1.2f (%1.2f), delta branch_length=%1.4f, "
+                        "total branch_length %1.4f" % (i, LH, deltaLH, dbl, self.tree.total_branch_length()), 2)
+            if deltaLH < LHtol:
+                self.logger("TreeAnc.optimize_tree_marginal: deltaLH=%f, stopping iteration." % deltaLH, 1)
+                break
+        return ttconf.SUCCESS
+
+    def branch_profile_lh(self, grid):
+        """prob_t_profiles(return_log=True) of every branch on the lengths
+        ``grid`` (the marginal-mode BranchLenInterpolator grid,
+        branch_len_interpolator.py:104-110), from the last marginal pass:
+        {node: array(len(grid))}."""
+        if self._device is None or self.sequence_reconstruction != "marginal":
+            raise MissingDataError("marginal ancestral inference needs to be performed first!")
+        g = self.gtr
+        out = self._device.branch_profile_lh(np.asarray(grid, dtype=float), g.gap_index if g.gap_index is not None
+                                             else -1)
+        return {n: out[i] for i, n in enumerate(self._nodes) if n.up is not None}
+
     # ---------------------------------------------------------------- output
     def sequence(self, node, as_string=True):
         seq = node.cseq
