@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="request arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-pool", type=int, default=64)
+    ap.add_argument("--branch-opt", type=int, default=0, metavar="K",
+                    help="also time the device branch-length sweeps (joint and marginal) on the first K trees "
+                         "against the FP64 oracle's per-branch Brent")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES between two device buffers 5 times (HBM counter calibration)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "batch_marginal_mix256_traffic.md"),
@@ -108,6 +111,11 @@ def algorithmic_bytes(prep):
 
 def oracle_marginal(prep):
     """FP64 oracle (CPU baseline / checker) on one prepared job."""
+    return oracle_tree(prep).marginal()
+
+
+def oracle_tree(prep):
+    """The FP64 oracle's TreeAnc over one prepared job."""
     from oracle import fp64 as O
 
     parent, is_tip = tree_arrays(prep)
@@ -118,8 +126,7 @@ def oracle_marginal(prep):
     tips = np.where(is_tip)[0]
     comp = {names[v]: chars[k] for k, v in enumerate(tips)}
     g = O.GTRModel(alpha, mu=prep["mu"], pi=prep["pi"], W=prep["W"])
-    ta = O.OracleTreeAnc(root, g, comp, prep["multiplicity"], full_length=1e300)
-    return ta.marginal()
+    return O.OracleTreeAnc(root, g, comp, prep["multiplicity"], full_length=1e300)
 
 
 def cpu_baseline(preps, seconds):
@@ -174,6 +181,52 @@ def copy_probe(nbytes):
     torch.cuda.synchronize()
     log(f"[bench] copied {nbytes} bytes 5 times")
     return 0
+
+
+def branch_opt_bench(trees, preps, torch):
+    """Device branch-length sweeps (all branches, one launch each) against
+    the FP64 oracle's scipy Brent on a sample of branches (host, 1 core)."""
+    from oracle import fp64 as O
+
+    n_br = sum(t.n_nodes - 1 for t in trees)
+    codes = np.full(256, -1, dtype=np.int16)
+    for i, ch in enumerate("ACGT"):
+        codes[ord(ch)] = i
+    for t in trees:
+        t.marginal(want_logL=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in trees:
+        t.optimize_branch_lengths_marginal(-1, 4.0, 1e-10)
+    dt_m = time.perf_counter() - t0
+    dt_j = float("nan")
+    if trees[0].n == 4:
+        for t in trees:
+            t.joint(want_logL=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in trees:
+            t.optimize_branch_lengths(codes, -1, 4.0, 1e-10, with_counts=False)
+        dt_j = time.perf_counter() - t0
+    log(f"[bench] branch sweeps on {len(trees)} trees ({n_br} branches): marginal {dt_m * 1e3:.1f} ms "
+        f"({n_br / dt_m:.3e} branches/s), joint {dt_j * 1e3:.1f} ms ({n_br / dt_j:.3e} branches/s)")
+    # oracle: marginal profiles of the first tree, then Brent per branch for ~10 s
+    prep = preps[0]
+    ta = oracle_tree(prep)
+    ta.marginal()
+    og = ta.gtr
+    done, t0 = 0, time.perf_counter()
+    for nd in O.preorder(ta.root):
+        if nd.up is None:
+            continue
+        og.optimal_t_compressed((nd.marginal_outgroup_LH, nd.marginal_subtree_LH), prep["multiplicity"],
+                                profiles=True, tol=1e-10)
+        done += 1
+        if time.perf_counter() - t0 > 10:
+            break
+    dt = time.perf_counter() - t0
+    log(f"[bench] oracle marginal Brent: {done} branches of tree 0 (P={prep['tip_codes'].shape[1]}) in {dt:.1f}s "
+        f"= {done / dt:.2f} branches/s (1 core)")
 
 
 def copy_ceiling(torch, nbytes=2 << 30, reps=5):
@@ -290,6 +343,9 @@ def main():
         torch.cuda.synchronize()
         ms_plain = p0.elapsed_time(p1) / args.plain_steps
         log(f"[bench] plain: {ms_plain:.3f} ms/step")
+
+    if args.branch_opt:
+        branch_opt_bench(trees[: args.branch_opt], preps[: args.branch_opt], torch)
 
     serving = None
     if not args.no_serve and len(trees) > 1:

@@ -112,6 +112,10 @@ struct PostLv {
 struct PreLv {
     int off, n_int, n_all;
 };
+// consecutive narrow levels launched as one run kernel: levels [l0, l1)
+struct LvRun {
+    int l0, l1;
+};
 
 struct tt_tree {
     tt_engine *eng = nullptr;
@@ -121,6 +125,7 @@ struct tt_tree {
     std::vector<int32_t> post_nodes, pre_nodes;
     std::vector<PostLv> post_lv;  // heights 1..H; the last level is the root
     std::vector<PreLv> pre_lv;    // depths 1..D
+    std::vector<LvRun> post_runs, pre_runs[2];  // pre_runs[tips]
     int R_all = 0, R_joint = 0;
     bool have_model = false;
     double mu = 1.0;
@@ -325,6 +330,24 @@ int32_t build_schedule(tt_tree *T, const tt_tree_desc *D) {
         for (int x : byd_t[dd]) T->pre_nodes.push_back(x);
         T->pre_lv.push_back(L);
     }
+    // runs of narrow levels: a level kernel costs a launch (several us), a
+    // level inside a run one or two node steps of a 4-wave block
+    const char *nr = getenv("TT_NO_RUNS");
+    if (!(nr && nr[0] == '1')) {
+        const int narrow = (D->n_states == 20 && ttk::mfma_enabled()) ? 8 : 16;
+        auto runs = [&](int n_levels, auto width, std::vector<LvRun> &out) {
+            out.clear();
+            for (int l = 0; l < n_levels;) {
+                int e = l;
+                while (e < n_levels && width(e) <= narrow) ++e;
+                if (e - l >= 2) out.push_back({l, e});
+                l = e > l ? e : l + 1;
+            }
+        };
+        runs((int)T->post_lv.size(), [&](int l) { return T->post_lv[l].cnt; }, T->post_runs);
+        runs((int)T->pre_lv.size(), [&](int l) { return T->pre_lv[l].n_int; }, T->pre_runs[0]);
+        runs((int)T->pre_lv.size(), [&](int l) { return T->pre_lv[l].n_all; }, T->pre_runs[1]);
+    }
     return TT_OK;
 }
 
@@ -351,7 +374,18 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
     const bool fill = (flags & kFillTips) != 0;
     const int nb = ttk::root_blocks(T->P);
     if (mode == 1 || mode == 3) {
-        for (auto &L : T->post_lv) {
+        size_t ri = 0;
+        for (size_t h = 0; h < T->post_lv.size();) {
+            if (ri < T->post_runs.size() && T->post_runs[ri].l0 == (int)h) {
+                const LvRun &r = T->post_runs[ri++];
+                const PostLv &last = T->post_lv[r.l1 - 1];
+                RunDev rd{(decltype(RunDev::nodes))T->d_post_nodes, (decltype(RunDev::off))T->d_post_off, nullptr,
+                          r.l0, r.l1, T->post_lv[r.l0].row_off, last.row_off + last.rows, 0};
+                TT_HIP(ttk::post_run(D, rd, s));
+                h = r.l1;
+                continue;
+            }
+            const PostLv &L = T->post_lv[h++];
             LevelDev ld{(decltype(LevelDev::nodes))(T->d_post_nodes + L.off), L.cnt, L.chunk, L.nchunks, L.row_off};
             TT_HIP(ttk::post_marginal(D, ld, s));
         }
@@ -361,7 +395,18 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
         TT_HIP(ttk::final_sum(D, nb, s));
         if (mode == 1) {
             if (fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
-            for (auto &L : T->pre_lv) {
+            const std::vector<LvRun> &runs = T->pre_runs[tips ? 1 : 0];
+            size_t ri = 0;
+            for (size_t d = 0; d < T->pre_lv.size();) {
+                if (ri < runs.size() && runs[ri].l0 == (int)d) {
+                    const LvRun &r = runs[ri++];
+                    RunDev rd{(decltype(RunDev::nodes))T->d_pre_nodes, (decltype(RunDev::off))T->d_pre_off,
+                              (decltype(RunDev::nint))T->d_pre_nint, r.l0, r.l1, 0, 0, tips ? 1 : 0};
+                    TT_HIP(ttk::pre_run(D, rd, s));
+                    d = r.l1;
+                    continue;
+                }
+                const PreLv &L = T->pre_lv[d++];
                 LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off), tips ? L.n_all : L.n_int, 1, 0, 0};
                 TT_HIP(ttk::pre_marginal(D, ld, s));
             }

@@ -69,6 +69,17 @@ struct LevelDev {
     int32_t row_off;       // first partial row of this level (post)
 };
 
+// A run of consecutive narrow levels walked in one launch (one block per
+// pattern tile, a block barrier between levels).
+struct RunDev {
+    const int32_t TT_C *nodes;  // post: internal rows by height; pre: node ids by depth
+    const int32_t TT_C *off;    // level offsets into nodes
+    const int32_t TT_C *nint;   // pre: internal nodes per depth (listed first); unused for post
+    int32_t l0, l1;             // levels [l0, l1)
+    int32_t row_first, row_end; // post: partial rows of these levels; the first gets the run's sum
+    int32_t tips;               // pre: also the tips of each depth
+};
+
 namespace ttk {
 // launchers (tt_kernels.hip); n is the alphabet size
 hipError_t expqt(int n, const double *t, const int32_t *branches, int k, const double *eig, const double *v,
@@ -78,6 +89,10 @@ hipError_t rowsum(const TreeDev &T, int R, int S, hipStream_t s);
 hipError_t root_marginal(const TreeDev &T, int S, hipStream_t s);
 hipError_t final_sum(const TreeDev &T, int nblocks, hipStream_t s);
 hipError_t pre_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s);
+hipError_t post_run(const TreeDev &T, const RunDev &R, hipStream_t s);
+hipError_t pre_run(const TreeDev &T, const RunDev &R, hipStream_t s);
+hipError_t post_run20(const TreeDev &T, const RunDev &R, hipStream_t s);  // MFMA, n = 20
+hipError_t pre_run20(const TreeDev &T, const RunDev &R, hipStream_t s);   // MFMA, n = 20
 hipError_t post_marginal20(const TreeDev &T, const LevelDev &L, hipStream_t s);  // MFMA, n = 20
 hipError_t pre_marginal20(const TreeDev &T, const LevelDev &L, hipStream_t s);   // MFMA, n = 20
 hipError_t post_joint(const TreeDev &T, const LevelDev &L, int store_tip_cx, hipStream_t s);

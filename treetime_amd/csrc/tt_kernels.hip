@@ -116,6 +116,60 @@ __global__ __launch_bounds__(TT_BLOCK) void k_pre_marginal(TreeDev T, LevelDev L
     pre_pair<N>(T, c0, c1, has1, p, T.lut);
 }
 
+// Runs of narrow levels (few nodes each, typically near the root): one launch
+// walks all of them.  A block of 4 waves takes one 64-pattern tile through
+// every level of the run -- the level's nodes split over the waves, two per
+// step -- with a block barrier between levels, so a narrow level costs one
+// or two node steps instead of a kernel launch.  Same per-(node, pattern)
+// arithmetic as the level kernels.
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_post_run(TreeDev T, RunDev R) {
+    constexpr int kW = TT_BLOCK / TT_WAVE;
+    __shared__ double s_acc[kW][TT_WAVE];
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    const int p = blockIdx.x * TT_WAVE + lane;
+    const bool act = p < T.P;
+    LogAcc acc;
+    for (int l = R.l0; l < R.l1; ++l) {
+        const int kb = R.off[l], ke = R.off[l + 1];
+        for (int k = kb + w; k < ke; k += 2 * kW) {
+            const bool has1 = k + kW < ke;
+            const int v0 = wave_uniform(R.nodes[k]);
+            const int v1 = has1 ? wave_uniform(R.nodes[k + kW]) : v0;
+            if (act) post_pair<N>(T, v0, v1, has1, p, T.lut, acc);
+        }
+        __syncthreads();
+    }
+    s_acc[w][lane] = act ? acc.log_value() : 0.0;
+    __syncthreads();
+    if (w == 0 && act) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < kW; ++q) s += s_acc[q][lane];
+        T.part[(size_t)R.row_first * T.P + p] = s;
+        for (int r = R.row_first + 1; r < R.row_end; ++r) T.part[(size_t)r * T.P + p] = 0.0;
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_pre_run(TreeDev T, RunDev R) {
+    constexpr int kW = TT_BLOCK / TT_WAVE;
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    const int p = blockIdx.x * TT_WAVE + lane;
+    const bool act = p < T.P;
+    for (int l = R.l0; l < R.l1; ++l) {
+        const int kb = R.off[l];
+        const int ke = R.tips ? R.off[l + 1] : kb + R.nint[l];
+        for (int k = kb + w; k < ke; k += 2 * kW) {
+            const bool has1 = k + kW < ke;
+            const int c0 = wave_uniform(R.nodes[k]);
+            const int c1 = has1 ? wave_uniform(R.nodes[k + kW]) : c0;
+            if (act) pre_pair<N>(T, c0, c1, has1, p, T.lut);
+        }
+        __syncthreads();
+    }
+}
+
 // Joint message of a tip child (treeanc.py:970-978, 997-1011):
 // Lx[j] = max_i log P[i][j] + log max(profile_i, TINY); Cx[j] = first argmax.
 template <int N>
@@ -468,6 +522,18 @@ hipError_t root_marginal(const TreeDev &T, int S, hipStream_t s) {
 
 hipError_t final_sum(const TreeDev &T, int nblocks, hipStream_t s) {
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TT_BLOCK), 0, s, T, nblocks);
+    return hipGetLastError();
+}
+
+hipError_t post_run(const TreeDev &T, const RunDev &R, hipStream_t s) {
+    if (T.n == 20 && mfma_enabled()) return post_run20(T, R, s);
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_post_run<N_>, dim3(n_ptiles(T.P)), dim3(TT_BLOCK), 0, s, T, R));
+    return hipGetLastError();
+}
+
+hipError_t pre_run(const TreeDev &T, const RunDev &R, hipStream_t s) {
+    if (T.n == 20 && mfma_enabled()) return pre_run20(T, R, s);
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_pre_run<N_>, dim3(n_ptiles(T.P)), dim3(TT_BLOCK), 0, s, T, R));
     return hipGetLastError();
 }
 

@@ -2,13 +2,20 @@
 //
 // A wave handles one node for 32 patterns.  The child message is computed
 // transposed, states x patterns = D^T (32 x 20, rows 20..31 zero) times
-// R^T (20 x 32 patterns), K = 20 in ten K=2 steps.  In the 32x32 accumulator
-// lane l holds column (pattern) l % 32 and the 16 rows
+// R^T (20 x 32 patterns).  In the 32x32 accumulator lane l holds column
+// (pattern) l % 32 and the 16 rows
 //   state(r, h) = 8 * (r / 4) + 4 * h + r % 4,  h = l / 32,
 // so the two lanes l and l ^ 32 together own one pattern's 20 states, and
 // per-pattern reductions (argmax, sums) are a within-lane pass plus one
 // exchange with the partner lane.  Operand layouts: lane l supplies
 // A[row l % 32][k = l / 32] and B[k = l / 32][col l % 32].
+//
+// Every vector lives in that accumulator layout, including the inputs: step
+// s of the contraction takes input state state(s, h) from lane half h, i.e.
+// the value the lane already holds in slot s.  Half 0 holds 12 real states
+// and half 1 holds 8 (its slots 8..11 are padding, zero), so a contraction
+// is 12 MFMA steps with no data exchanged between lanes, and a lane loads
+// only its own states of a row (3 or 2 float4).
 //
 // Arithmetic follows tt_device.h: deficit messages m = r + d with
 // d = r (P - I), normaliser factor formed in FP64 and multiplied into a
@@ -24,54 +31,55 @@ constexpr int NS = 20;
 
 __device__ __forceinline__ int acc_state(int r, int h) { return (r >> 2) * 8 + h * 4 + (r & 3); }
 
-// Full 20-state row of child edge (cref) for pattern p (zeros when !act).
+constexpr int kSteps = 12;  // accumulator slots that hold real states in half 0
+
+template <class Ptr>
+__device__ __forceinline__ void load_groups(Ptr row, int h, float (&o)[16]);
+
+// This lane's states of the subtree row of child edge cref for pattern p,
+// in accumulator layout (zeros for padding and when !act).
 template <class LP>
-__device__ __forceinline__ void row20(const TreeDev &T, int cref, int p, bool act, LP lut, float (&row)[NS]) {
+__device__ __forceinline__ void child_slots(const TreeDev &T, int cref, int p, bool act, int h, LP lut,
+                                            float (&xo)[16]) {
+    if (!act) {
 #pragma unroll
-    for (int i = 0; i < NS; ++i) row[i] = 0.0f;
-    if (!act) return;
+        for (int r = 0; r < 16; ++r) xo[r] = 0.0f;
+        return;
+    }
     if (cref < 0) {
         const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
 #pragma unroll
-        for (int i = 0; i < NS; ++i) row[i] = lut[code * NS + i];
+        for (int r = 0; r < 16; ++r) {
+            const int st = acc_state(r, h);
+            xo[r] = st < NS ? lut[code * NS + (st < NS ? st : 0)] : 0.0f;
+        }
     } else {
-        load_rowf<NS>(T.sub + ((size_t)cref * T.P + p) * NS, row);
+        load_groups(T.sub + ((size_t)cref * T.P + p) * NS, h, xo);
     }
 }
 
-// Row entries at this lane's accumulator states (0 for padding rows).
-__device__ __forceinline__ void to_acc_layout(const float (&row)[NS], int h, float (&ro)[16]) {
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int s0 = acc_state(r, 0), s1 = acc_state(r, 1);
-        const float v0 = s0 < NS ? row[s0 < NS ? s0 : 0] : 0.0f;
-        const float v1 = s1 < NS ? row[s1 < NS ? s1 : 0] : 0.0f;
-        ro[r] = h ? v1 : v0;
-    }
-}
-
-// acc = (row as B operand) contracted with A[i][k] = M[k * NS + i] (i = this
-// lane's row l % 32, k the input state): out_i = sum_k row_k M[k][i].
+// out_i = sum_k x_k M[k][i] (A[i][k] = M[k * NS + i]).
 template <class MP>
-__device__ __forceinline__ v16f contract_in(const float (&row)[NS], int h, int i, MP M) {
+__device__ __forceinline__ v16f contract_in(const float (&xo)[16], int h, int i, MP M) {
     v16f acc = {0.0f};
 #pragma unroll
-    for (int s = 0; s < NS / 2; ++s) {
-        const float b = h ? row[2 * s + 1] : row[2 * s];
-        const float a = (i < NS) ? M[(2 * s + h) * NS + (i < NS ? i : 0)] : 0.0f;
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    for (int s = 0; s < kSteps; ++s) {
+        const int k = acc_state(s, h);
+        const float a = (i < NS && k < NS) ? M[(k < NS ? k : 0) * NS + (i < NS ? i : 0)] : 0.0f;
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, xo[s], acc, 0, 0, 0);
     }
     return acc;
 }
 
-// Same with A[i][k] = M[i * NS + k]: out_i = sum_k M[i][k] x_k.
+// out_i = sum_k M[i][k] x_k (A[i][k] = M[i * NS + k]).
 template <class MP>
-__device__ __forceinline__ v16f contract_out(const float (&xb)[NS / 2], int i, int h, MP M) {
+__device__ __forceinline__ v16f contract_out(const float (&xo)[16], int h, int i, MP M) {
     v16f acc = {0.0f};
 #pragma unroll
-    for (int s = 0; s < NS / 2; ++s) {
-        const float a = (i < NS) ? M[(i < NS ? i : 0) * NS + 2 * s + h] : 0.0f;
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, xb[s], acc, 0, 0, 0);
+    for (int s = 0; s < kSteps; ++s) {
+        const int k = acc_state(s, h);
+        const float a = (i < NS && k < NS) ? M[(i < NS ? i : 0) * NS + (k < NS ? k : 0)] : 0.0f;
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, xo[s], acc, 0, 0, 0);
     }
     return acc;
 }
@@ -142,13 +150,11 @@ __device__ __forceinline__ void post_node20(const TreeDev &T, int v, int p, bool
                                             LogAcc &acc) {
     const int e0 = wave_uniform(T.child_ptr[v]), e1 = wave_uniform(T.child_ptr[v + 1]);
     if (e1 - e0 == 2) {
-        float ro0[16], ro1[16], row[NS];
-        row20(T, wave_uniform(T.child_ref[e0]), p, act, lut, row);
-        to_acc_layout(row, h, ro0);
-        const v16f d0 = contract_in(row, h, i, T.D32 + (size_t)wave_uniform(T.child_node[e0]) * (NS * NS));
-        row20(T, wave_uniform(T.child_ref[e0 + 1]), p, act, lut, row);
-        to_acc_layout(row, h, ro1);
-        const v16f d1 = contract_in(row, h, i, T.D32 + (size_t)wave_uniform(T.child_node[e0 + 1]) * (NS * NS));
+        float ro0[16], ro1[16];  // both children's rows fetched before any arithmetic
+        child_slots(T, wave_uniform(T.child_ref[e0]), p, act, h, lut, ro0);
+        child_slots(T, wave_uniform(T.child_ref[e0 + 1]), p, act, h, lut, ro1);
+        const v16f d0 = contract_in(ro0, h, i, T.D32 + (size_t)wave_uniform(T.child_node[e0]) * (NS * NS));
+        const v16f d1 = contract_in(ro1, h, i, T.D32 + (size_t)wave_uniform(T.child_node[e0 + 1]) * (NS * NS));
         float x[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) x[r] = (ro0[r] + d0[r]) * (ro1[r] + d1[r]);
@@ -183,10 +189,9 @@ __device__ __forceinline__ void post_node20(const TreeDev &T, int v, int p, bool
 #pragma unroll
     for (int r = 0; r < 16; ++r) L[r] = 0.0f;
     for (int e = e0; e < e1; ++e) {
-        float row[NS], ro[16];
-        row20(T, wave_uniform(T.child_ref[e]), p, act, lut, row);
-        to_acc_layout(row, h, ro);
-        const v16f d = contract_in(row, h, i, T.D32 + (size_t)wave_uniform(T.child_node[e]) * (NS * NS));
+        float ro[16];
+        child_slots(T, wave_uniform(T.child_ref[e]), p, act, h, lut, ro);
+        const v16f d = contract_in(ro, h, i, T.D32 + (size_t)wave_uniform(T.child_node[e]) * (NS * NS));
 #pragma unroll
         for (int r = 0; r < 16; ++r) L[r] += (acc_state(r, h) < NS) ? log_msg(ro[r], d[r]) : 0.0f;
     }
@@ -201,7 +206,7 @@ __device__ __forceinline__ void post_node20(const TreeDev &T, int v, int p, bool
     if (act) store_groups(T.sub + ((size_t)v * T.P + p) * NS, h, o);
 }
 
-__global__ __launch_bounds__(TT_BLOCK) void k_post_marginal20(TreeDev T, LevelDev L, int n_ptiles32) {
+__global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_post_marginal20(TreeDev T, LevelDev L, int n_ptiles32) {
     __shared__ double s_acc[TT_BLOCK / TT_WAVE][TT_WAVE];
     const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
     const int h = lane >> 5, i = lane & 31;
@@ -226,14 +231,12 @@ __global__ __launch_bounds__(TT_BLOCK) void k_post_marginal20(TreeDev T, LevelDe
 }
 
 // Preorder step (treeanc.py:889-930) of node c for this wave's 32 patterns.
-__device__ __forceinline__ void pre_node20(const TreeDev &T, int c, int p, bool act, int h, int i,
-                                           float *lds /* [32][NS] of this wave */) {
+__device__ __forceinline__ void pre_node20(const TreeDev &T, int c, int p, bool act, int h, int i) {
     const int par = wave_uniform(T.parent[c]);
     const int ir = wave_uniform(T.int_row[c]);
     const auto P32 = T.P32 + (size_t)c * (NS * NS);
-    float row[NS], ro[16], pp[16];
-    row20(T, ir >= 0 ? ir : -(wave_uniform(T.tip_row[c]) + 1), p, act, T.lut, row);
-    to_acc_layout(row, h, ro);
+    float ro[16], pp[16];
+    child_slots(T, ir >= 0 ? ir : -(wave_uniform(T.tip_row[c]) + 1), p, act, h, T.lut, ro);
     if (act) {
         load_groups(T.prof + ((size_t)par * T.P + p) * NS, h, pp);
     } else {
@@ -241,21 +244,13 @@ __device__ __forceinline__ void pre_node20(const TreeDev &T, int c, int p, bool 
         for (int r = 0; r < 16; ++r) pp[r] = 0.0f;
     }
     // child message m_j = sum_i row_i P[i][j]
-    const v16f m = contract_in(row, h, i, P32);
+    const v16f m = contract_in(ro, h, i, P32);
     // outgroup q_j = max(TINY, parent_j) / m_j (its normalisation cancels)
+    float q[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int st = acc_state(r, h);
-        if (st < NS) lds[i * NS + (st < NS ? st : 0)] = fmaxf((float)TT_TINY, pp[r]) / m[r];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    float qb[NS / 2];
-#pragma unroll
-    for (int s = 0; s < NS / 2; ++s) qb[s] = lds[i * NS + 2 * s + h];
-    // evolve: msg_k = sum_j q_j P[k][j]
-    const v16f msg = contract_out(qb, i, h, P32);
+    for (int r = 0; r < 16; ++r) q[r] = acc_state(r, h) < NS ? fmaxf((float)TT_TINY, pp[r]) / m[r] : 0.0f;
+    // evolve: msg_k = sum_j P[k][j] q_j
+    const v16f msg = contract_out(q, h, i, P32);
     float x[16];
     float part = 0.0f;
 #pragma unroll
@@ -274,23 +269,72 @@ __device__ __forceinline__ void pre_node20(const TreeDev &T, int c, int p, bool 
         store_groups(T.prof + ((size_t)c * T.P + p) * NS, h, o);
         if (h == 0) T.states[(size_t)c * T.P + p] = (uint8_t)jm;
     }
-    __builtin_amdgcn_wave_barrier();  // lds reused by the next node of this wave
 }
 
 __global__ __launch_bounds__(TT_BLOCK) void k_pre_marginal20(TreeDev T, LevelDev L, int n_ptiles32) {
-    __shared__ float s_q[TT_BLOCK / TT_WAVE][32 * NS];
     const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
     const int h = lane >> 5, i = lane & 31;
     const int task = blockIdx.x * (TT_BLOCK / TT_WAVE) + w;
     if (task >= L.n_nodes * n_ptiles32) return;  // whole wave exits together
     const int k = task / n_ptiles32, pt = task % n_ptiles32;
     const int p = pt * 32 + i;
-    pre_node20(T, wave_uniform(L.nodes[k]), p, p < T.P, h, i, s_q[w]);
+    pre_node20(T, wave_uniform(L.nodes[k]), p, p < T.P, h, i);
+}
+
+// Runs of narrow levels in one launch (see k_post_run): a block of 4 waves
+// takes one 32-pattern tile through every level of the run, the level's
+// nodes split over the waves, a block barrier between levels.
+__global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_post_run20(TreeDev T, RunDev R) {
+    constexpr int kW = TT_BLOCK / TT_WAVE;
+    __shared__ double s_acc[kW][TT_WAVE];
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    const int h = lane >> 5, i = lane & 31;
+    const int p = blockIdx.x * 32 + i;
+    const bool act = p < T.P;
+    LogAcc acc;
+    for (int l = R.l0; l < R.l1; ++l) {
+        const int kb = R.off[l], ke = R.off[l + 1];
+        for (int k = kb + w; k < ke; k += kW) post_node20(T, wave_uniform(R.nodes[k]), p, act, h, i, T.lut, acc);
+        __syncthreads();
+    }
+    s_acc[w][lane] = (act && h == 0) ? acc.log_value() : 0.0;
+    __syncthreads();
+    if (w == 0 && act && h == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < kW; ++q) s += s_acc[q][lane];
+        T.part[(size_t)R.row_first * T.P + p] = s;
+        for (int r = R.row_first + 1; r < R.row_end; ++r) T.part[(size_t)r * T.P + p] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(TT_BLOCK) void k_pre_run20(TreeDev T, RunDev R) {
+    constexpr int kW = TT_BLOCK / TT_WAVE;
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    const int h = lane >> 5, i = lane & 31;
+    const int p = blockIdx.x * 32 + i;
+    const bool act = p < T.P;
+    for (int l = R.l0; l < R.l1; ++l) {
+        const int kb = R.off[l];
+        const int ke = R.tips ? R.off[l + 1] : kb + R.nint[l];
+        for (int k = kb + w; k < ke; k += kW) pre_node20(T, wave_uniform(R.nodes[k]), p, act, h, i);
+        __syncthreads();
+    }
 }
 
 }  // namespace
 
 namespace ttk {
+
+hipError_t post_run20(const TreeDev &T, const RunDev &R, hipStream_t s) {
+    hipLaunchKernelGGL(k_post_run20, dim3((T.P + 31) / 32), dim3(TT_BLOCK), 0, s, T, R);
+    return hipGetLastError();
+}
+
+hipError_t pre_run20(const TreeDev &T, const RunDev &R, hipStream_t s) {
+    hipLaunchKernelGGL(k_pre_run20, dim3((T.P + 31) / 32), dim3(TT_BLOCK), 0, s, T, R);
+    return hipGetLastError();
+}
 
 hipError_t post_marginal20(const TreeDev &T, const LevelDev &L, hipStream_t s) {
     const int npt = (T.P + 31) / 32;
