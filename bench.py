@@ -230,21 +230,27 @@ def branch_opt_bench(trees, preps, torch):
 
 
 def copy_ceiling(torch, nbytes=2 << 30, reps=5):
-    """Device-to-device copy rate (read + write bytes per second, GB/s): the
-    measured ceiling beside the datasheet peak."""
+    """Streaming rates of plain device kernels (GB/s, bytes moved per second):
+    copy (read + write), read-only (sum) and write-only (fill) -- the
+    measured ceilings beside the datasheet peak."""
     a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
-    b.copy_(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    rate = 2 * nbytes * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
+
+    def rate(fn, moved):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return moved * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
+
+    out = {"copy": rate(lambda: b.copy_(a), 2 * nbytes), "read": rate(lambda: a.sum(), nbytes),
+           "write": rate(lambda: b.fill_(1.0), nbytes)}
     del a, b
     torch.cuda.empty_cache()
-    return rate
+    return out
 
 
 def measured_traffic(path, workload):
@@ -297,8 +303,10 @@ def main():
     B = sum(algorithmic_bytes(p) for p in preps)
     P_total = sum(p["tip_codes"].shape[1] for p in preps)
 
-    ceiling = copy_ceiling(torch)
-    log(f"[bench] device copy ceiling {ceiling:.0f} GB/s (read + write)")
+    ceilings = copy_ceiling(torch)
+    ceiling = ceilings["copy"]
+    log(f"[bench] streaming ceilings (GB/s): copy {ceilings['copy']:.0f}, read {ceilings['read']:.0f}, "
+        f"write {ceilings['write']:.0f}")
     stream = torch.cuda.Stream()
     sp = stream.cuda_stream
     # correctness: batched logL equals the plain pass per tree
@@ -391,6 +399,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.traffic_file, wname),
                      "copy_ceiling": ceiling, "frac_of_copy_ceiling": achieved / ceiling,
+                     "streaming_ceilings": ceilings,
                      "algorithmic_bytes_per_launch": int(B)},
         "cpu_baseline": cpu,
         "serving": serving,

@@ -35,3 +35,46 @@ def small_job(n_tips, L, n_states=4, seed=1, depth=0.02, p_ambig=0.01, p_gap=0.0
     prep = prepare(job["parent"], job["branch_length"], job["chars"], job["gtr"], job["ambiguous"])
     alphabet = "nuc_nogap" if n_states == 4 else "aa_nogap"
     return job, prep, alphabet
+
+
+def collapse_job(job, frac, seed, root_extra=True):
+    """Multifurcating variant of a synthetic job: a fraction ``frac`` of the
+    internal non-root nodes is removed, their children attached to the
+    nearest kept ancestor with the removed branch length added (a polytomy
+    wherever a node is removed); ``root_extra`` also removes one internal
+    child of the root, giving the root three or more children.  Tip rows
+    keep their order."""
+    parent, bl = job["parent"], job["branch_length"].astype(float).copy()
+    n = len(parent)
+    is_tip = np.ones(n, dtype=bool)
+    is_tip[parent[parent >= 0]] = False
+    internal = [v for v in range(n) if not is_tip[v] and parent[v] >= 0]
+    rng = np.random.default_rng(seed)
+    drop = set(int(x) for x in rng.choice(internal, size=int(frac * len(internal)), replace=False))
+    if root_extra:
+        root = int(np.where(parent < 0)[0][0])
+        kids = [v for v in internal if parent[v] == root]
+        if kids:
+            drop.add(kids[0])
+    keep = [v for v in range(n) if v not in drop]
+    idx = {v: i for i, v in enumerate(keep)}
+    new_parent = np.full(len(keep), -1, dtype=np.int32)
+    new_bl = np.zeros(len(keep))
+    for v in keep:
+        p, add = int(parent[v]), 0.0
+        while p >= 0 and p in drop:
+            add += bl[p]
+            p = int(parent[p])
+        new_parent[idx[v]] = idx[p] if p >= 0 else -1
+        new_bl[idx[v]] = bl[v] + add
+    out = dict(job)
+    out["parent"], out["branch_length"] = new_parent, new_bl
+    out["tips"] = np.array([idx[int(v)] for v in job["tips"]])
+    return out
+
+
+def polytomy_job(n_tips, L, n_states=4, seed=1, frac=0.3, depth=0.05, p_ambig=0.01, p_gap=0.005):
+    job = collapse_job(synth.make_job(n_tips, L, depth, n_states, seed, p_ambig, p_gap), frac, seed + 1)
+    prep = prepare(job["parent"], job["branch_length"], job["chars"], job["gtr"], job["ambiguous"])
+    alphabet = "nuc_nogap" if n_states == 4 else "aa_nogap"
+    return job, prep, alphabet

@@ -169,13 +169,16 @@ def _profile_objective(og, pp, pc, mult, t):
     return -og.prob_t_profiles((pp, pc), mult, t, return_log=True) + np.exp(s ** 4 / 10000)
 
 
-@pytest.mark.parametrize("n_tips,L,seed", [(50, 600, 71), (25, 300, 72)])
-def test_marginal_sweep_matches_oracle(gpu_engine, n_tips, L, seed):
-    from helpers import build_oracle
+@pytest.mark.parametrize("n_tips,L,seed,poly", [(50, 600, 71, False), (25, 300, 72, False), (60, 500, 74, True)])
+def test_marginal_sweep_matches_oracle(gpu_engine, n_tips, L, seed, poly):
+    from helpers import build_oracle, polytomy_job
     from oracle.fp64 import normalize_profile
 
-    say(f"[branch-opt marginal] tips={n_tips} L={L} seed={seed}")
-    job, prep, alphabet = small_job(n_tips, L, 4, seed, depth=0.05)
+    say(f"[branch-opt marginal] tips={n_tips} L={L} seed={seed} multifurcating={poly}")
+    if poly:
+        job, prep, alphabet = polytomy_job(n_tips, L, 4, seed, 0.3)
+    else:
+        job, prep, alphabet = small_job(n_tips, L, 4, seed, depth=0.05)
     dev = device_tree(gpu_engine, prep)
     dev.marginal()
     t, st, pair = dev.optimize_branch_lengths_marginal(-1, 4.0, 1e-10)
@@ -198,12 +201,15 @@ def test_marginal_sweep_matches_oracle(gpu_engine, n_tips, L, seed):
             worst = max(worst, abs(t[v] - t_ref) / t_ref)
     # the bifurcating root's pair (treeanc.py:1318-1331)
     root = [nd for nd in nodes if nd.up is None][0]
-    assert len(root.clades) == 2 and pair[1] == 0
-    n1, n2 = root.clades
-    pp = normalize_profile(n2.marginal_subtree_LH * og.Pi)[0]
-    pc = n1.marginal_subtree_LH
-    t_ref = og.optimal_t_compressed((pp, pc), mult, profiles=True, tol=1e-10)
-    assert abs(pair[0] - t_ref) <= TM_RTOL * t_ref + TM_ATOL, (pair, t_ref)
+    if len(root.clades) == 2:
+        assert pair[1] == 0
+        n1, n2 = root.clades
+        pp = normalize_profile(n2.marginal_subtree_LH * og.Pi)[0]
+        pc = n1.marginal_subtree_LH
+        t_ref = og.optimal_t_compressed((pp, pc), mult, profiles=True, tol=1e-10)
+        assert abs(pair[0] - t_ref) <= TM_RTOL * t_ref + TM_ATOL, (pair, t_ref)
+    else:
+        assert pair[1] == -1
     say(f"  {len(nodes) - 1} branches + root pair, max rel diff of optimal t {worst:.2e} (t > 1e-6)")
     # grid of prob_t_profiles values (BranchLenInterpolator, marginal mode)
     grid = np.array([-0.1, 0.0, 1e-4, 0.003, 0.05, 0.4])
@@ -212,9 +218,14 @@ def test_marginal_sweep_matches_oracle(gpu_engine, n_tips, L, seed):
         if nd.up is None:
             assert np.all(lh[v] == 0)
             continue
-        ref = [og.prob_t_profiles((nd.marginal_outgroup_LH, nd.marginal_subtree_LH), mult, x, return_log=True)
-               for x in grid]
-        assert np.allclose(lh[v], ref, rtol=1e-6, atol=1e-6), (v, lh[v], ref)
+        ref = np.array([og.prob_t_profiles((nd.marginal_outgroup_LH, nd.marginal_subtree_LH), mult, x,
+                                           return_log=True) for x in grid])
+        # t > 0: 1e-6.  t = 0 reduces to pc . pp, which for a tip whose
+        # character has an outgroup probability near the reference's 1e-12
+        # clamp (treeanc.py:895-896) depends on which side of the clamp the
+        # FP32 and FP64 profiles fall: 1e-3 there.
+        assert np.allclose(lh[v][grid != 0], ref[grid != 0], rtol=1e-6, atol=1e-6), (v, lh[v], ref)
+        assert np.allclose(lh[v][grid == 0], ref[grid == 0], rtol=1e-3), (v, lh[v], ref)
     dev.close()
 
 

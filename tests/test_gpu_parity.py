@@ -13,7 +13,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import build_oracle, small_job
+from helpers import build_oracle, polytomy_job, small_job
 from treetime_amd.engine import Engine
 from treetime_amd.prepare import device_tree
 
@@ -46,14 +46,24 @@ CASES = [
     ("aa", 60, 300, 20, 3, 0.3, 0.0, 0.0),
     ("p-not-multiple-of-64", 37, 97, 4, 9, 0.05, 0.05, 0.05),
     ("big", 2000, 3000, 4, 11, 0.02, 0.01, 0.005),
+    # multifurcating trees (30% of internal nodes collapsed, root with >= 3 children)
+    ("poly-nuc", 150, 900, 4, 31, 0.05, 0.01, 0.005),
+    ("poly-aa", 50, 300, 20, 32, 0.3, 0.0, 0.0),
+    ("poly-big", 1500, 1200, 4, 33, 0.02, 0.01, 0.005),
 ]
+
+
+def make_case(name, n_tips, L, n_states, seed, depth, pa, pg):
+    if name.startswith("poly"):
+        return polytomy_job(n_tips, L, n_states, seed, 0.3, depth, pa, pg)
+    return small_job(n_tips, L, n_states, seed, depth, pa, pg)
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_marginal_parity(gpu_engine, case):
     name, n_tips, L, n_states, seed, depth, pa, pg = case
     say(f"[marginal] case={name} n_tips={n_tips} L={L} n={n_states} seed={seed}")
-    job, prep, alphabet = small_job(n_tips, L, n_states, seed, depth, pa, pg)
+    job, prep, alphabet = make_case(*case)
     ta, nodes, tips = build_oracle(prep, alphabet, job["gtr"])
     ref = ta.marginal()
     dt = device_tree(gpu_engine, prep)
@@ -82,7 +92,7 @@ def test_marginal_parity(gpu_engine, case):
 def test_joint_parity(gpu_engine, case):
     name, n_tips, L, n_states, seed, depth, pa, pg = case
     say(f"[joint] case={name} n_tips={n_tips} L={L} n={n_states} seed={seed}")
-    job, prep, alphabet = small_job(n_tips, L, n_states, seed, depth, pa, pg)
+    job, prep, alphabet = make_case(*case)
     ta, nodes, tips = build_oracle(prep, alphabet, job["gtr"])
     ref = ta.joint()
     dt = device_tree(gpu_engine, prep)
@@ -179,9 +189,9 @@ def test_batch_matches_plain(gpu_engine, tips_flag):
     identical profiles and states, logL within 1e-12 (summation order only)."""
     from treetime_amd.engine import batch_marginal
 
-    say(f"[batch] 6 uneven trees, reconstruct_tips={tips_flag}")
+    say(f"[batch] 7 uneven trees (one multifurcating), reconstruct_tips={tips_flag}")
     specs = [(50, 1000, 1), (300, 200, 2), (7, 65, 3), (120, 3000, 4), (3, 10, 5), (900, 640, 6)]
-    jobs = [small_job(n, L, 4, seed) for n, L, seed in specs]
+    jobs = [small_job(n, L, 4, seed) for n, L, seed in specs] + [polytomy_job(200, 700, 4, 7, 0.3)]
     trees = [device_tree(gpu_engine, prep) for _, prep, _ in jobs]
     plain = [t.marginal(reconstruct_tips=tips_flag) for t in trees]
     plain_prof = [t.profiles() for t in trees]

@@ -53,13 +53,14 @@ __device__ __forceinline__ void post_pair_t(const TreeDev &T, int v0, int v1, bo
             if (has1) bin_finish<N>(T, v1, b0, p1, RB1, lut, acc1);
         }
     } else {
+        // each node by its own arity (see post_pair)
         if (act0) {
-            post_node<N>(T, v0, p0, lut, acc0);
-            if (has1) post_node<N>(T, v1, p0, lut, acc0);
+            post_one<N>(T, v0, a0, a1 - a0 == 2, p0, lut, acc0);
+            if (has1) post_one<N>(T, v1, b0, b1 - b0 == 2, p0, lut, acc0);
         }
         if (TL == 2 && act1) {
-            post_node<N>(T, v0, p1, lut, acc1);
-            if (has1) post_node<N>(T, v1, p1, lut, acc1);
+            post_one<N>(T, v0, a0, a1 - a0 == 2, p1, lut, acc1);
+            if (has1) post_one<N>(T, v1, b0, b1 - b0 == 2, p1, lut, acc1);
         }
     }
 }

@@ -271,6 +271,18 @@ __device__ __forceinline__ void bin_finish(const TreeDev &T, int v, int e0, int 
     store_post<N>(T, v, p, x, jm, xm);
 }
 
+// One postorder step: the binary path for two children, else post_node.
+template <int N, class LP>
+__device__ __forceinline__ void post_one(const TreeDev &T, int v, int e0, bool binary, int p, LP lut, LogAcc &acc) {
+    if (binary) {
+        BinRaw<N> R;
+        bin_fetch<N>(T, e0, p, R);
+        bin_finish<N>(T, v, e0, p, R, lut, acc);
+    } else {
+        post_node<N>(T, v, p, lut, acc);
+    }
+}
+
 // Two postorder steps (rows v0 and, if has1, v1) for lane pattern p.
 template <int N, class LP>
 __device__ __forceinline__ void post_pair(const TreeDev &T, int v0, int v1, bool has1, int p, LP lut,
@@ -284,8 +296,10 @@ __device__ __forceinline__ void post_pair(const TreeDev &T, int v0, int v1, bool
         bin_finish<N>(T, v0, a0, p, RA, lut, acc);
         if (has1) bin_finish<N>(T, v1, b0, p, RB, lut, acc);
     } else {
-        post_node<N>(T, v0, p, lut, acc);
-        if (has1) post_node<N>(T, v1, p, lut, acc);
+        // a node's arithmetic depends only on its own arity, never on the
+        // node it is paired with (results identical for every schedule)
+        post_one<N>(T, v0, a0, a1 - a0 == 2, p, lut, acc);
+        if (has1) post_one<N>(T, v1, b0, b1 - b0 == 2, p, lut, acc);
     }
 }
 
