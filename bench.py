@@ -231,8 +231,8 @@ def branch_opt_bench(trees, preps, torch):
 
 def copy_ceiling(torch, nbytes=2 << 30, reps=5):
     """Streaming rates of plain device kernels (GB/s, bytes moved per second):
-    copy (read + write), read-only (sum) and write-only (fill) -- the
-    measured ceilings beside the datasheet peak."""
+    copy (read + write) and write-only (fill) -- the measured ceilings beside
+    the datasheet peak."""
     a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
 
@@ -246,8 +246,7 @@ def copy_ceiling(torch, nbytes=2 << 30, reps=5):
         torch.cuda.synchronize()
         return moved * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
 
-    out = {"copy": rate(lambda: b.copy_(a), 2 * nbytes), "read": rate(lambda: a.sum(), nbytes),
-           "write": rate(lambda: b.fill_(1.0), nbytes)}
+    out = {"copy": rate(lambda: b.copy_(a), 2 * nbytes), "write": rate(lambda: b.fill_(1.0), nbytes)}
     del a, b
     torch.cuda.empty_cache()
     return out
@@ -305,8 +304,7 @@ def main():
 
     ceilings = copy_ceiling(torch)
     ceiling = ceilings["copy"]
-    log(f"[bench] streaming ceilings (GB/s): copy {ceilings['copy']:.0f}, read {ceilings['read']:.0f}, "
-        f"write {ceilings['write']:.0f}")
+    log(f"[bench] streaming ceilings (GB/s): copy {ceilings['copy']:.0f}, write {ceilings['write']:.0f}")
     stream = torch.cuda.Stream()
     sp = stream.cuda_stream
     # correctness: batched logL equals the plain pass per tree
