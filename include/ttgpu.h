@@ -53,6 +53,7 @@ int32_t tt_version(void);
 
 /* ---- engine: one per device, holds the HBM admission budget ------------ */
 int32_t tt_engine_create(int32_t device, uint64_t hbm_budget_bytes, tt_engine **out);
+/* TT_ESTATE (nothing released) while trees created on the engine are alive. */
 int32_t tt_engine_destroy(tt_engine *eng);
 int32_t tt_engine_usage(const tt_engine *eng, uint64_t *bytes_in_use, uint64_t *budget);
 

@@ -246,3 +246,32 @@ def test_tip_states_reset_after_reconstruction(gpu_engine):
     assert all((tip_rows(t) == 255).all() for t in trees)
     for t in trees:
         t.close()
+
+
+def test_engine_lifetime_and_growing_updates():
+    """An engine with open trees refuses to close; branch-length updates of
+    growing size (temporary buffers re-allocated) give the oracle's logL."""
+    from treetime_amd.errors import MissingDataError
+
+    say("[engine] lifetime, growing updates")
+    eng = Engine(0, 0)
+    job, prep, alphabet = small_job(60, 400, 4, 81)
+    dt = device_tree(eng, prep)
+    with pytest.raises(MissingDataError):
+        eng.close()
+    t = prep["t"].copy()
+    rng = np.random.default_rng(5)
+    for k in (1, 7, 40, len(t) - 1):
+        nodes = rng.choice(np.nonzero(prep["parent"] >= 0)[0], size=k, replace=False).astype(np.int32)
+        t[nodes] *= rng.uniform(0.5, 2.0, size=k)
+        dt.set_branch_lengths(nodes, t[nodes])
+        got = dt.loglik()
+        p2 = dict(prep)
+        p2["t"] = t
+        ta, _, _ = build_oracle(p2, alphabet, job["gtr"])
+        ref = ta.marginal()
+        assert abs(got - ref) <= LOGL_RTOL * abs(ref), (k, got, ref)
+    used = eng.usage()[0]
+    dt.close()
+    assert eng.usage()[0] < used
+    eng.close()

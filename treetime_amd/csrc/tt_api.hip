@@ -73,6 +73,7 @@ struct tt_engine {
     std::atomic<uint64_t> used{0};
     std::mutex mu;
     std::vector<BatchPlan *> plans;
+    std::atomic<int> live_trees{0};
 };
 
 // A recorded batched launch over a fixed list of trees.
@@ -168,6 +169,18 @@ struct tt_tree {
         allocs.emplace_back(*p, nbytes);
         return TT_OK;
     }
+    // free one allocation made by alloc() (returns its bytes to the budget)
+    void release(void *p) {
+        if (!p) return;
+        for (size_t i = 0; i < allocs.size(); ++i)
+            if (allocs[i].first == p) {
+                (void)hipFree(p);
+                eng->used -= allocs[i].second;
+                bytes -= allocs[i].second;
+                allocs.erase(allocs.begin() + i);
+                return;
+            }
+    }
     template <class T>
     int32_t upload(T **dst, const T *src, size_t count) {
         int32_t rc = alloc((void **)dst, count * sizeof(T));
@@ -204,6 +217,8 @@ int32_t tt_engine_create(int32_t device, uint64_t hbm_budget_bytes, tt_engine **
 }
 
 int32_t tt_engine_destroy(tt_engine *eng) {
+    if (eng && eng->live_trees.load() > 0)
+        return fail(TT_ESTATE, "engine still has %d trees; destroy them first", eng->live_trees.load());
     if (eng) {
         (void)hipDeviceSynchronize();
         for (BatchPlan *p : eng->plans) delete p;
@@ -758,6 +773,7 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
         delete T;
         return fail(TT_EDEVICE, "hipMemset: %s", hipGetErrorString(he));
     }
+    eng->live_trees += 1;
     *out = T;
     return TT_OK;
 }
@@ -767,6 +783,7 @@ int32_t tt_tree_destroy(tt_tree *tree) {
         (void)hipSetDevice(tree->eng->device);
         (void)hipDeviceSynchronize();
         tt_engine *eng = tree->eng;
+        eng->live_trees -= 1;
         std::lock_guard<std::mutex> g(eng->mu);
         auto &pl = eng->plans;
         for (size_t i = 0; i < pl.size();) {
@@ -840,6 +857,12 @@ int32_t set_lengths(tt_tree *T, const int32_t *nodes, const double *t, int32_t k
     TT_HIP(hipMemcpyAsync(T->d_t, T->t.data(), T->n_nodes * sizeof(double), hipMemcpyHostToDevice, T->stream));
     if (T->have_model) {
         if (k > T->tmp_cap) {
+            TT_HIP(hipStreamSynchronize(T->stream));  // earlier copies may still read the old buffers
+            T->release(T->d_tmp_t);
+            T->release(T->d_tmp_br);
+            T->d_tmp_t = nullptr;
+            T->d_tmp_br = nullptr;
+            T->tmp_cap = 0;
             int32_t rc;
             if ((rc = T->alloc((void **)&T->d_tmp_t, (size_t)k * sizeof(double)))) return rc;
             if ((rc = T->alloc((void **)&T->d_tmp_br, (size_t)k * sizeof(int32_t)))) return rc;
@@ -973,6 +996,11 @@ int32_t tt_branch_profile_lh(tt_tree *T, int32_t gap_state, const double *t, int
     TT_HIP(hipSetDevice(T->eng->device));
     int32_t rc;
     if (K > T->grid_cap) {
+        TT_HIP(hipStreamSynchronize(T->stream));
+        T->release(T->d_grid_t);
+        T->release(T->d_grid_out);
+        T->d_grid_t = T->d_grid_out = nullptr;
+        T->grid_cap = 0;
         if ((rc = T->alloc((void **)&T->d_grid_t, (size_t)K * sizeof(double)))) return rc;
         if ((rc = T->alloc((void **)&T->d_grid_out, (size_t)K * T->n_nodes * sizeof(double)))) return rc;
         T->grid_cap = K;

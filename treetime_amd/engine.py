@@ -37,8 +37,10 @@ class Engine:
         return used.value, budget.value
 
     def close(self):
+        """Release the engine; fails (MissingDataError) while trees created
+        on it are still open."""
         if self.handle:
-            self.lib.tt_engine_destroy(self.handle)
+            _native.check(self.lib.tt_engine_destroy(self.handle))
             self.handle = None
 
     def __del__(self):
