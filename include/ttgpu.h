@@ -163,6 +163,27 @@ int32_t tt_optimize_branch_lengths_marginal(tt_tree *tree, int32_t gap_state, do
 int32_t tt_branch_profile_lh(tt_tree *tree, int32_t gap_state, const double *t, int32_t K, double *out,
                              void *stream);
 
+/* tt_branch_state_counts: the multiplicity-weighted (parent state, child
+ * state) counts of every branch from the states of the last reconstruction,
+ * pair_counts (host) [n_nodes][n][n] -- the counts step of
+ * tt_optimize_branch_lengths without the search.  Replaces the per-branch
+ * counting that infer_gtr does for a joint reconstruction
+ * (treeanc.py:1573-1589, c.mutations and c.cseq) and GTR.state_pair
+ * (gtr.py:631-712); code_state and gap_state as there. */
+int32_t tt_branch_state_counts(tt_tree *tree, const int16_t *code_state, int32_t gap_state, double *pair_counts,
+                               void *stream);
+
+/* tt_mutation_counts: GTR-inference statistics of a marginal reconstruction
+ * (treeanc.py:1556-1572 with get_branch_mutation_matrix, :1085-1120): per
+ * branch and pattern the joint distribution of (child state i, parent state
+ * j), M_ij = pc_i pp_j (P(t)_ij + 1e-24) / Z, summed with the pattern
+ * multiplicities into nij[i][j] (host, [n][n]) and the time in each state
+ * Ti[i] (host, [n]: half the branch length for the parent's and half for the
+ * child's state); per_branch (nullable, host [n_nodes][n*n + n]) gets each
+ * branch's share.  Alphabets of up to 5 states; TT_ESTATE unless the profiles
+ * are those of the current lengths. */
+int32_t tt_mutation_counts(tt_tree *tree, double *nij, double *Ti, double *per_branch, void *stream);
+
 /* Free the joint-pass buffers (Lx, Cx) and return them to the engine's
  * budget; the next tt_joint allocates them again.  No reference counterpart:
  * it lets a server admit requests against an HBM budget. */

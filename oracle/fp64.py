@@ -305,6 +305,42 @@ class GTRModel:
         return new_len
 
 
+def gtr_infer(alphabet, nij, Ti, root_state, fixed_pi=None, pc=1.0, gap_limit=0.01):
+    """treetime/gtr.py:491-599 (GTR.infer): fixed point of
+    n_ij + pc = mu pi_i W_ij (T_j + pc + root_state); returns a GTRModel."""
+    nij = np.array(nij, dtype=float)
+    Ti = np.asarray(Ti, dtype=float)
+    root_state = np.asarray(root_state, dtype=float)
+    probe = GTRModel(alphabet)
+    gap = probe.gap_index
+    pcm = pc * np.ones_like(nij)
+    np.fill_diagonal(pcm, 0.0)
+    np.fill_diagonal(nij, 0.0)
+    pi = np.ones_like(Ti) if fixed_pi is None else np.array(fixed_pi, dtype=float)
+    pi /= pi.sum()
+    old = np.zeros_like(Ti)
+    W = np.ones_like(nij)
+    mu = (nij.sum() + pc) / (Ti.sum() + pc)
+    it = 0
+    while np.linalg.norm(old - pi) > 1e-5 and it < 40:
+        it += 1
+        old = pi.copy()
+        W = (nij + nij.T + 2 * pcm) / mu / (np.outer(pi, Ti) + np.outer(Ti, pi) + TINY_NUMBER + 2 * pcm)
+        np.fill_diagonal(W, 0)
+        W = W / avg_transition(W, pi, gap_index=gap)
+        if fixed_pi is None:
+            pi = (np.sum(nij + pcm, axis=1) + root_state) / (
+                TINY_NUMBER + mu * W.dot(Ti) + root_state.sum() + np.sum(pcm, axis=1))
+            pi /= pi.sum()
+            mu = (nij.sum() + pc) / (np.sum(pi * W.dot(Ti)) + pc)
+        else:
+            mu = (nij.sum() + pc) / (np.sum(pi * W.dot(pi)) * Ti.sum() + pc)
+    if gap is not None and pi[gap] < gap_limit:
+        pi[gap] = gap_limit
+        pi /= pi.sum()
+    return GTRModel(alphabet, mu=mu, pi=pi, W=W)
+
+
 def gtr_from_text(text):
     """Parse the reference's GTR text format (treetime/gtr.py:174-233)."""
     lines = text.splitlines()
@@ -570,3 +606,29 @@ class OracleTreeAnc:
             lq = np.log(self.gtr.expQt(node.branch_length))
             out += lq[seq_idx[node.name], seq_idx[node.up.name]]
         return out
+
+    # -- GTR inference statistics ------------------------------------------------
+    def branch_mutation_matrix(self, node):
+        """treetime/treeanc.py:1085-1120 (get_branch_mutation_matrix, compressed):
+        (P, n, n) joint distribution of (child state i, parent state j) per
+        pattern, after a marginal pass."""
+        pp, pc = node.marginal_outgroup_LH, node.marginal_subtree_LH
+        E = self.gtr.expQt(self.branch_t(node)) + SUPERTINY_NUMBER
+        M = np.einsum("ai,aj,ij->aij", pc, pp, E)
+        return M / M.sum(axis=2).sum(axis=1)[:, None, None]
+
+    def mutation_counts_marginal(self):
+        """treetime/treeanc.py:1556-1572 summed over patterns: n_ij (i derived,
+        j ancestral) and the time spent in each state, T_i."""
+        n = self.gtr.n_states
+        nij, Ti = np.zeros((n, n)), np.zeros(n)
+        for node in preorder(self.root):
+            if node.up is None:
+                continue
+            M = self.branch_mutation_matrix(node)
+            t = self.branch_t(node)
+            w = self.mult[:, None]
+            Ti += 0.5 * t * (M.sum(axis=1) * w).sum(axis=0)
+            Ti += 0.5 * t * (M.sum(axis=2) * w).sum(axis=0)
+            nij += (M * self.mult[:, None, None]).sum(axis=0)
+        return nij, Ti

@@ -156,3 +156,20 @@ def test_kingman_and_simulation_shapes():
 def test_effective_lengths_floor():
     t = effective_lengths([0.0, 1e-9, 0.3], 1000)
     assert np.allclose(t, [1e-6, 1e-6, 0.3])
+
+
+def test_gtr_infer_matches_oracle():
+    """The facade's GTR.infer (host fixed point, gtr.py:491-599) equals the
+    oracle's restatement on the same statistics."""
+    from treetime_amd.gtr import GTR
+
+    rng = np.random.default_rng(9)
+    for alphabet, n in [("nuc_nogap", 4), ("nuc", 5)]:
+        nij = rng.uniform(0, 50, (n, n))
+        Ti = rng.uniform(10, 100, n)
+        root = rng.uniform(0, 30, n)
+        g = GTR.infer(nij, Ti, root, pc=1.0, alphabet=alphabet)
+        o = O.gtr_infer(alphabet, nij, Ti, root, pc=1.0)
+        assert np.allclose(g.W, o.W, rtol=1e-12, atol=0)
+        assert np.allclose(g.Pi, o.Pi, rtol=1e-12, atol=0)
+        assert abs(g.mu - o.mu) <= 1e-12 * o.mu

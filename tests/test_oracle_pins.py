@@ -333,3 +333,59 @@ def test_state_pair_counts_truncate():
                                pattern_multiplicity=np.array([1.5, 2.7, 1.0, 3.0]))
     d = {tuple(p): c for p, c in zip(pairs.tolist(), mult.tolist())}
     assert d == {(0, 0): 1, (0, 1): 2, (1, 1): 1}
+
+
+# ------------------------------------------------- GTR inference statistics --
+def enumerate_pair_posterior(root, nodes, model, tip_state_profiles, full_length, child_name):
+    """Brute force: P(child state i, parent state j | data) per pattern for the
+    branch above ``child_name`` (an internal node), P(t) from scipy expm."""
+    internal = [n for n in nodes if not n.is_terminal()]
+    n = model.n_states
+    P = {nd.name: np.maximum(expm(model.mu * model.Q * max(O.MIN_BRANCH_LENGTH / full_length, nd.branch_length)), 0)
+         for nd in nodes if nd.up is not None}
+    L = next(iter(tip_state_profiles.values())).shape[0]
+    child = [nd for nd in nodes if nd.name == child_name][0]
+    pair = np.zeros((L, n, n))
+    for combo in itertools.product(range(n), repeat=len(internal)):
+        st = {nd.name: s for nd, s in zip(internal, combo)}
+        w = np.full(L, model.Pi[st[root.name]])
+        for nd in nodes:
+            if nd.up is None:
+                continue
+            ps = st[nd.up.name]
+            w = w * (tip_state_profiles[nd.name] @ P[nd.name][:, ps] if nd.is_terminal() else P[nd.name][st[nd.name], ps])
+        pair[:, st[child.name], st[child.up.name]] += w
+    return pair / pair.sum(axis=(1, 2))[:, None, None]
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_branch_mutation_matrix_is_pair_posterior(seed):
+    """get_branch_mutation_matrix (treeanc.py:1085-1120) of an internal branch
+    equals the exact joint posterior of its end states by enumeration."""
+    root, nodes, model, aln = five_tip_case(seed, amb=False)
+    comp, mult, _, _ = O.compress_alignment(list("ABCDE"), aln, "N")
+    ta = O.OracleTreeAnc(root, model, comp, mult, full_length=40)
+    ta.marginal()
+    tips = {k: O.seq2prof(comp[k], model.profile_map) for k in "ABCDE"}
+    for name in ["X", "Y", "Z"]:
+        nd = [x for x in nodes if x.name == name][0]
+        ref = enumerate_pair_posterior(root, nodes, model, tips, 40, name)
+        assert np.allclose(ta.branch_mutation_matrix(nd), ref, atol=1e-10), name
+
+
+def test_gtr_infer_recovers_model():
+    """GTR.infer (gtr.py:491-599) on exact expected counts
+    n_ij = mu pi_i W_ij T_j without pseudocounts returns (mu, pi, W) to the
+    precision of its stopping rule (|d pi| < 1e-5)."""
+    rng = np.random.default_rng(3)
+    for alphabet, n in [("nuc_nogap", 4), ("aa_nogap", 20)]:
+        pi = rng.dirichlet(np.ones(n) * 5)
+        W = rng.lognormal(0, 0.5, (n, n))
+        W = (W + W.T) / 2
+        g0 = O.GTRModel(alphabet, mu=1.3, pi=pi, W=W)
+        T = rng.uniform(100, 1000, n)
+        nij = g0.mu * g0.Pi[:, None] * g0.W * T[None, :]
+        g = O.gtr_infer(alphabet, nij, T, np.zeros(n), pc=0.0)
+        assert np.max(np.abs(g.Pi - g0.Pi)) < 1e-4
+        assert np.max(np.abs(g.W - g0.W) / g0.W.max()) < 1e-3
+        assert abs(g.mu - g0.mu) < 1e-4 * g0.mu

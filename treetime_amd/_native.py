@@ -84,6 +84,8 @@ SIGNATURES = {
     "tt_optimize_branch_lengths": (_I32, [_P, _PI16, _I32, _D, _D, _PD, _PI32, _PD, _P]),
     "tt_optimize_branch_lengths_marginal": (_I32, [_P, _I32, _D, _D, _PD, _PI32, _PD, _P]),
     "tt_branch_profile_lh": (_I32, [_P, _I32, _PD, _I32, _PD, _P]),
+    "tt_branch_state_counts": (_I32, [_P, _PI16, _I32, _PD, _P]),
+    "tt_mutation_counts": (_I32, [_P, _PD, _PD, _PD, _P]),
     "tt_tree_trim": (_I32, [_P]),
     "tt_tree_query": (_I32, [_P]),
     "tt_get_total": (_I32, [_P, _PD]),

@@ -150,6 +150,7 @@ struct tt_tree {
     bool tips_dirty = false;   // tip rows of d.states hold reconstructed states (else 255)
     bool profiles_fresh = false;  // d.prof and d.sub belong to the current lengths (last marginal pass)
     double *d_grid_t = nullptr, *d_grid_out = nullptr;
+    double *d_mut_counts = nullptr;
     int grid_cap = 0;
     int16_t *d_code_state = nullptr;  // branch optimisation buffers, allocated on first use
     double *d_pair_counts = nullptr, *d_t_opt = nullptr;
@@ -939,7 +940,7 @@ int32_t tt_optimize_branch_lengths(tt_tree *T, const int16_t *code_state, int32_
     hipStream_t s = stream ? (hipStream_t)stream : T->stream;
     TT_HIP(hipMemcpyAsync(T->d_code_state, code_state, 256 * sizeof(int16_t), hipMemcpyHostToDevice, s));
     TT_HIP(ttk::branch_opt(T->d, T->d_code_state, gap_state, T->d_eig, T->d_v, T->d_vinv, T->mu, max_t, tol,
-                           T->d_pair_counts, T->d_t_opt, T->d_opt_status, s));
+                           T->d_pair_counts, T->d_t_opt, T->d_opt_status, 1, s));
     TT_HIP(hipMemcpyAsync(t_opt, T->d_t_opt, (size_t)T->n_nodes * sizeof(double), hipMemcpyDeviceToHost, s));
     TT_HIP(hipMemcpyAsync(status, T->d_opt_status, (size_t)T->n_nodes * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     if (pair_counts)
@@ -1011,6 +1012,60 @@ int32_t tt_branch_profile_lh(tt_tree *T, int32_t gap_state, const double *t, int
                                   T->d_grid_out, s));
     TT_HIP(hipMemcpyAsync(out, T->d_grid_out, (size_t)K * T->n_nodes * sizeof(double), hipMemcpyDeviceToHost, s));
     TT_HIP(hipStreamSynchronize(s));
+    return TT_OK;
+}
+
+int32_t tt_branch_state_counts(tt_tree *T, const int16_t *code_state, int32_t gap_state, double *pair_counts,
+                               void *stream) {
+    if (!T || !code_state || !pair_counts) return fail(TT_EINVAL, "NULL argument");
+    if (!T->have_states) return fail(TT_ESTATE, "no marginal or joint reconstruction has run on this tree");
+    if (gap_state < -1 || gap_state >= T->n) return fail(TT_EINVAL, "gap_state %d outside [-1, %d)", gap_state, T->n);
+    for (int c = 0; c < 256; ++c)
+        if (code_state[c] < -1 || code_state[c] >= T->n)
+            return fail(TT_EINVAL, "code_state[%d] = %d outside [-1, %d)", c, code_state[c], T->n);
+    TT_HIP(hipSetDevice(T->eng->device));
+    int32_t rc;
+    const size_t nn = (size_t)T->n * T->n;
+    if ((rc = ensure_opt_out(T))) return rc;
+    if (!T->d_code_state) {
+        if ((rc = T->alloc((void **)&T->d_code_state, 256 * sizeof(int16_t)))) return rc;
+        if ((rc = T->alloc((void **)&T->d_pair_counts, (size_t)T->n_nodes * nn * sizeof(double)))) return rc;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : T->stream;
+    TT_HIP(hipMemcpyAsync(T->d_code_state, code_state, 256 * sizeof(int16_t), hipMemcpyHostToDevice, s));
+    TT_HIP(ttk::branch_opt(T->d, T->d_code_state, gap_state, T->d_eig, T->d_v, T->d_vinv, T->mu, 1.0, 0.0,
+                           T->d_pair_counts, T->d_t_opt, T->d_opt_status, 0, s));
+    TT_HIP(hipMemcpyAsync(pair_counts, T->d_pair_counts, (size_t)T->n_nodes * nn * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    TT_HIP(hipStreamSynchronize(s));
+    return TT_OK;
+}
+
+int32_t tt_mutation_counts(tt_tree *T, double *nij, double *Ti, double *per_branch, void *stream) {
+    if (!T || !nij || !Ti) return fail(TT_EINVAL, "NULL argument");
+    if (T->n > 5) return fail(TT_EINVAL, "mutation counts are built for alphabets of up to 5 states (n = %d)", T->n);
+    if (!T->profiles_fresh)
+        return fail(TT_ESTATE, "marginal ancestral inference needs to be performed first (profiles are stale)");
+    TT_HIP(hipSetDevice(T->eng->device));
+    const int n = T->n, w = n * n + n;
+    if (!T->d_mut_counts) {
+        int32_t rc = T->alloc((void **)&T->d_mut_counts, (size_t)T->n_nodes * w * sizeof(double));
+        if (rc) return rc;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : T->stream;
+    TT_HIP(ttk::mutation_counts(T->d, T->d_t, T->d_mut_counts, s));
+    std::vector<double> h((size_t)T->n_nodes * w);
+    TT_HIP(hipMemcpyAsync(h.data(), T->d_mut_counts, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    TT_HIP(hipStreamSynchronize(s));
+    // sums over branches in node order
+    for (int k = 0; k < n * n; ++k) nij[k] = 0.0;
+    for (int k = 0; k < n; ++k) Ti[k] = 0.0;
+    for (int c = 0; c < T->n_nodes; ++c) {
+        const double *o = h.data() + (size_t)c * w;
+        for (int k = 0; k < n * n; ++k) nij[k] += o[k];
+        for (int k = 0; k < n; ++k) Ti[k] += o[n * n + k];
+    }
+    if (per_branch) std::memcpy(per_branch, h.data(), h.size() * sizeof(double));
     return TT_OK;
 }
 

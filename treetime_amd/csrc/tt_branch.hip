@@ -153,16 +153,17 @@ template <int N>
 __global__ __launch_bounds__(TT_BLOCK) void k_branch_opt(TreeDev T, const int16_t *code_state, int gap,
                                                         const double *eig, const double *v, const double *vinv,
                                                         double mu, double max_t, double tol, double *counts,
-                                                        double *t_out, int32_t *status) {
+                                                        double *t_out, int32_t *status, int search) {
     constexpr int NN = N * N;
     __shared__ double s_cnt[NN];
     const int node = blockIdx.x;
     const int par = T.parent[node];
     if (par < 0) {  // the root has no branch
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0 && search) {
             t_out[node] = 0.0;
             status[node] = 0;
         }
+        for (int k = threadIdx.x; k < NN; k += blockDim.x) counts[(size_t)node * NN + k] = 0.0;
         return;
     }
     for (int k = threadIdx.x; k < NN; k += blockDim.x) s_cnt[k] = 0.0;
@@ -185,7 +186,7 @@ __global__ __launch_bounds__(TT_BLOCK) void k_branch_opt(TreeDev T, const int16_
     }
     __syncthreads();
     for (int k = threadIdx.x; k < NN; k += blockDim.x) counts[(size_t)node * NN + k] = s_cnt[k];
-    if (threadIdx.x >= TT_WAVE) return;
+    if (threadIdx.x >= TT_WAVE || !search) return;  // counts only
 
     // 2. Brent in s on wave 0 (gtr.py:816-920)
     const int lane = threadIdx.x;
@@ -422,6 +423,70 @@ __global__ __launch_bounds__(kOptThreads) void k_branch_profile_lh(TreeDev T, in
     }
 }
 
+// ---------------------------------------------------------------------------
+// GTR inference counts from the marginal profiles (treeanc.py:1085-1120
+// get_branch_mutation_matrix, summed as in infer_gtr, treeanc.py:1556-1572):
+// for the branch above node c and pattern p,
+//   M_ij = pc_i pp_j (P(t_c)_ij + 1e-24) / Z,  Z = sum_ij pc_i pp_j (P_ij + 1e-24)
+// (i the child's state, j the parent's), and per branch
+//   n_ij += mult_p M_ij,
+//   T_j  += t_c / 2 * mult_p sum_i M_ij,   T_i += t_c / 2 * mult_p sum_j M_ij.
+// One workgroup per branch; every thread sums its patterns in registers,
+// then the block reduces each entry in a fixed order.  Alphabets up to 5
+// states (n^2 + n accumulators per thread).
+template <int N>
+__global__ __launch_bounds__(kOptThreads) void k_mutation_counts(TreeDev T, const double *t_eff, double *out) {
+    constexpr int NN = N * N;
+    __shared__ double s_red[kOptThreads];
+    const int c = blockIdx.x;
+    double *o = out + (size_t)c * (NN + N);
+    if (T.parent[c] < 0) {
+        for (int k = threadIdx.x; k < NN + N; k += blockDim.x) o[k] = 0.0;
+        return;
+    }
+    const double TT_C *E = T.P64 + (size_t)c * NN;
+    const double half_t = 0.5 * t_eff[c];
+    double n_acc[NN], t_acc[N];
+#pragma unroll
+    for (int k = 0; k < NN; ++k) n_acc[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) t_acc[k] = 0.0;
+    for (int p = threadIdx.x; p < T.P; p += blockDim.x) {
+        double pc[N], pp[N];
+        branch_profiles<N>(T, c, -1, p, pc, pp);
+        double m[NN], z = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                m[i * N + j] = pc[i] * pp[j] * (E[i * N + j] + kSuperTiny);
+                z += m[i * N + j];
+            }
+        const double w = T.mult[p] / z;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double row = 0.0, col = 0.0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                n_acc[i * N + j] = fma(m[i * N + j], w, n_acc[i * N + j]);
+                row += m[i * N + j];  // sum over the parent state j: child state i
+                col += m[j * N + i];  // sum over the child state: parent state i
+            }
+            t_acc[i] = fma(half_t * w, row + col, t_acc[i]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NN; ++k) {
+        const double v = block_total(n_acc[k], s_red);
+        if (threadIdx.x == 0) o[k] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double v = block_total(t_acc[k], s_red);
+        if (threadIdx.x == 0) o[NN + k] = v;
+    }
+}
+
 }  // namespace
 
 #define TT_DISPATCH_ALL(n, CALL)                          \
@@ -437,6 +502,17 @@ __global__ __launch_bounds__(kOptThreads) void k_branch_profile_lh(TreeDev T, in
     }
 
 namespace ttk {
+hipError_t mutation_counts(const TreeDev &T, const double *t_eff, double *out, hipStream_t s) {
+    switch (T.n) {
+        case 2: hipLaunchKernelGGL(k_mutation_counts<2>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
+        case 3: hipLaunchKernelGGL(k_mutation_counts<3>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
+        case 4: hipLaunchKernelGGL(k_mutation_counts<4>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
+        case 5: hipLaunchKernelGGL(k_mutation_counts<5>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t branch_opt_marginal(const TreeDev &T, int gap, const double *eig, const double *v, const double *vinv,
                                double mu, double max_t, double tol, int pair_n1, int pair_n2, double *t_out,
                                int32_t *status, hipStream_t s) {
@@ -455,9 +531,10 @@ hipError_t branch_profile_lh(const TreeDev &T, int gap, const double *eig, const
 
 hipError_t branch_opt(const TreeDev &T, const int16_t *code_state, int gap, const double *eig, const double *v,
                       const double *vinv, double mu, double max_t, double tol, double *counts, double *t_out,
-                      int32_t *status, hipStream_t s) {
+                      int32_t *status, int search, hipStream_t s) {
     TT_DISPATCH_ALL(T.n, hipLaunchKernelGGL(k_branch_opt<N_>, dim3((unsigned)T.n_nodes), dim3(TT_BLOCK), 0, s, T,
-                                            code_state, gap, eig, v, vinv, mu, max_t, tol, counts, t_out, status));
+                                            code_state, gap, eig, v, vinv, mu, max_t, tol, counts, t_out, status,
+                                            search));
     return hipGetLastError();
 }
 }  // namespace ttk

@@ -109,11 +109,13 @@ bool mfma_enabled();
 // joint branch-length optimisation (tt_branch.hip)
 hipError_t branch_opt(const TreeDev &T, const int16_t *code_state, int gap, const double *eig, const double *v,
                       const double *vinv, double mu, double max_t, double tol, double *counts, double *t_out,
-                      int32_t *status, hipStream_t s);
+                      int32_t *status, int search, hipStream_t s);
 // marginal branch lengths: Brent on the profile pair of every branch (+ the root pair as entry n_nodes)
 hipError_t branch_opt_marginal(const TreeDev &T, int gap, const double *eig, const double *v, const double *vinv,
                                double mu, double max_t, double tol, int pair_n1, int pair_n2, double *t_out,
                                int32_t *status, hipStream_t s);
+// GTR inference counts per branch, out [n_nodes][n*n + n] (n_ij then T_i), n <= 5
+hipError_t mutation_counts(const TreeDev &T, const double *t_eff, double *out, hipStream_t s);
 // prob_t_profiles of every branch at K lengths, out [n_nodes][K]
 hipError_t branch_profile_lh(const TreeDev &T, int gap, const double *eig, const double *v, const double *vinv,
                              double mu, const double *t, int K, double *out, hipStream_t s);

@@ -189,6 +189,25 @@ class DeviceTree:
                                                     len(tt), _native.ptr(out, C.c_double), stream))
         return out
 
+    def branch_state_counts(self, code_state, gap_state=-1, stream=None):
+        """(n_nodes, n, n) multiplicity-weighted (parent, child) state counts
+        of the last reconstruction (tt_branch_state_counts)."""
+        cs = _native.as_c(code_state, np.int16)
+        out = np.empty((self.n_nodes, self.n, self.n), dtype=np.float64)
+        _native.check(self.lib.tt_branch_state_counts(self.handle, _native.ptr(cs, C.c_int16), int(gap_state),
+                                                      _native.ptr(out, C.c_double), stream))
+        return out
+
+    def mutation_counts(self, per_branch=False, stream=None):
+        """GTR-inference statistics of the last marginal pass
+        (tt_mutation_counts): (nij (n, n), Ti (n,)[, per-branch (n_nodes, n*n + n)])."""
+        nij = np.empty((self.n, self.n), dtype=np.float64)
+        Ti = np.empty(self.n, dtype=np.float64)
+        pb = np.empty((self.n_nodes, self.n * self.n + self.n), dtype=np.float64) if per_branch else None
+        _native.check(self.lib.tt_mutation_counts(self.handle, _native.ptr(nij, C.c_double), _native.ptr(Ti, C.c_double),
+                                                  _native.ptr(pb, C.c_double) if per_branch else None, stream))
+        return (nij, Ti, pb) if per_branch else (nij, Ti)
+
     def trim(self):
         """Free the joint-pass buffers (returned to the engine budget)."""
         _native.check(self.lib.tt_tree_trim(self.handle))

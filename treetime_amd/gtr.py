@@ -175,6 +175,51 @@ class GTR:
         return g
 
     @classmethod
+    def infer(cls, nij, Ti, root_state, fixed_pi=None, pc=1.0, gap_limit=0.01, **kwargs):
+        """Model from transition counts and state times (gtr.py:491-599):
+        the fixed point of n_ij + pc = mu pi_i W_ij (T_j + pc + root_state),
+        W symmetric, iterated until pi moves by less than 1e-5 (at most 40
+        rounds).  The n x n statistics come from the device
+        (TreeAnc.infer_gtr); the iteration is small and runs on the host."""
+        gtr = cls(**kwargs)
+        gtr.logger("GTR: model inference ", 1)
+        nij = np.array(nij, dtype=float)
+        Ti = np.asarray(Ti, dtype=float)
+        root_state = np.asarray(root_state, dtype=float)
+        pc_mat = pc * np.ones_like(nij)
+        np.fill_diagonal(pc_mat, 0.0)
+        np.fill_diagonal(nij, 0.0)
+        pi = np.ones_like(Ti) if fixed_pi is None else np.array(fixed_pi, dtype=float)
+        pi /= pi.sum()
+        pi_old = np.zeros_like(Ti)
+        W = np.ones_like(nij)
+        mu = (nij.sum() + pc) / (Ti.sum() + pc)
+        rounds = 0
+        while np.linalg.norm(pi_old - pi) > 1e-5 and rounds < 40:
+            rounds += 1
+            pi_old = pi.copy()
+            W = (nij + nij.T + 2 * pc_mat) / mu / (np.outer(pi, Ti) + np.outer(Ti, pi) + ttconf.TINY_NUMBER
+                                                     + 2 * pc_mat)
+            np.fill_diagonal(W, 0)
+            W = W / avg_transition(W, pi, gap_index=gtr.gap_index)
+            if fixed_pi is None:
+                pi = (np.sum(nij + pc_mat, axis=1) + root_state) / (
+                    ttconf.TINY_NUMBER + mu * np.dot(W, Ti) + root_state.sum() + np.sum(pc_mat, axis=1))
+                pi /= pi.sum()
+                mu = (nij.sum() + pc) / (np.sum(pi * (W.dot(Ti))) + pc)
+            else:
+                mu = (nij.sum() + pc) / (np.sum(pi * (W.dot(pi))) * Ti.sum() + pc)
+        if rounds >= 40:
+            gtr.logger("WARNING: maximum number of iterations has been reached in GTR inference", 3, warn=True)
+        if gtr.gap_index is not None and pi[gtr.gap_index] < gap_limit:
+            gtr.logger("The model allows for gaps which are estimated to occur at a low fraction of %1.3e; gap "
+                       "fraction will be set to %1.4f" % (pi[gtr.gap_index], gap_limit), 2, warn=True)
+            pi[gtr.gap_index] = gap_limit
+            pi /= pi.sum()
+        gtr.assign_rates(mu=mu, W=W, pi=pi)
+        return gtr
+
+    @classmethod
     def random(cls, mu=1.0, alphabet="nuc", rng=None):
         rng = rng if rng is not None else np.random.default_rng()
         n = len(alphabets[alphabet_synonyms[alphabet]])

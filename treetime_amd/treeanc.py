@@ -12,7 +12,10 @@ follow treetime/treeanc.py (TreeTime 0.12.1):
 * ``sequence_LH`` (691-716), ``ancestral_likelihood`` (718-750),
   ``optimize_branch_len`` / ``optimize_branch_lengths_joint`` (1168-1270).
 
-Fitch parsimony and GTR inference are outside this engine
+* ``infer_gtr`` (1500-1632) with the counts from the device; marginal
+  branch lengths ``optimize_tree_marginal`` (1297-1358).
+
+Fitch parsimony and site-specific models are outside this engine
 (``NotImplementedError``).
 """
 import sys
@@ -312,7 +315,7 @@ class TreeAnc:
             raise UnknownMethodError("Reconstruction method needs to be in ['ml', 'probabilistic', 'fitch', "
                                      "'parsimony'], got '{}'".format(method))
         if infer_gtr:
-            raise NotImplementedError("GTR inference is not part of the GPU pruning engine")
+            self.infer_gtr(marginal=marginal, **kwargs)
         return run(reconstruct_tip_states=reconstruct_tip_states, **kwargs)
 
     def _assign_sequences(self, dev, reconstruct_tip_states, sample_root=False, sample_other=False, prof=None):
@@ -450,12 +453,14 @@ class TreeAnc:
         self._prepare_nodes()
         return ttconf.SUCCESS
 
-    def _code_states(self):
+    def _code_states(self, skip_unknown=False):
         """Alphabet index of every character code for gtr.py:672-712: characters
         outside the alphabet are not counted for n < 10 and count as state 1
-        for larger alphabets (the reference's np.ones_like default)."""
+        for larger alphabets (the reference's np.ones_like default);
+        ``skip_unknown`` never counts them (infer_gtr's state_index lookup,
+        treeanc.py:1574-1578)."""
         alpha = [str(a) for a in self.gtr.alphabet]
-        other = -1 if len(alpha) < 10 else 1
+        other = -1 if (len(alpha) < 10 or skip_unknown) else 1
         out = np.full(256, other, dtype=np.int16)
         for i, a in enumerate(alpha):
             if len(a) == 1 and ord(a) < 256:
@@ -498,6 +503,53 @@ class TreeAnc:
         if node.up is None:
             return self.one_mutation
         return max(0.0, self._take_optimum(node, self._branch_optima()))
+
+    # ----------------------------------------------------------- GTR inference
+    def infer_gtr(self, marginal=False, site_specific=False, normalized_rate=True, fixed_pi=None, pc=5.0, **kwargs):
+        """treeanc.py:1500-1632: transition counts n_ij and state times T_i
+        of the current reconstruction (computed on the device for every
+        branch at once), then GTR.infer on the host.  Marginal: expected
+        counts from each branch's profile pair (tt_mutation_counts);
+        joint: the (parent, child) state counts of every branch
+        (tt_branch_state_counts)."""
+        if site_specific:
+            raise NotImplementedError("site-specific GTR models are not part of the GPU pruning engine")
+        if not self.ok:
+            raise MissingDataError("TreeAnc.infer_gtr: ERROR, sequences or tree are missing")
+        if marginal and self.sequence_reconstruction != "marginal":
+            self._ml_anc_marginal(**kwargs)
+        elif not self.sequence_reconstruction:
+            self._ml_anc_joint(**kwargs)
+        g = self.gtr
+        n = g.n_states
+        dev = self._device
+        mult = self.data.multiplicity()
+        self.logger("TreeAnc.infer_gtr: counting mutations...", 2)
+        if marginal:
+            nij, Ti = dev.mutation_counts()
+        else:
+            t = np.array([self._branch_length_to_gtr(x) if x.up is not None else 0.0 for x in self._nodes])
+            C = dev.branch_state_counts(self._code_states(skip_unknown=True), -1)  # C[node][parent j][child i]
+            nij = np.zeros((n, n))
+            Ti = np.zeros(n)
+            off = ~np.eye(n, dtype=bool)
+            for k, x in enumerate(self._nodes):
+                if x.up is None:
+                    continue
+                c = C[k]
+                nij += (c * off).T                      # i derived (child), j ancestral (parent)
+                Ti += t[k] * c.sum(axis=0)              # time in the child's state
+                Ti += 0.5 * t[k] * (c * off).sum(axis=1)  # mutations: half the branch in the parent state
+                Ti -= 0.5 * t[k] * (c * off).sum(axis=0)  # ... and not in the child state
+        root_idx = self._nodes.index(self.tree.root)
+        root_states = dev.states()[root_idx]
+        root_state = np.bincount(root_states, weights=mult, minlength=n)[:n]
+        self._gtr = GTR.infer(nij, Ti, root_state, fixed_pi=fixed_pi, pc=pc, alphabet=g.alphabet,
+                              logger=self.logger, prof_map=g.profile_map)
+        if normalized_rate:
+            self.logger("TreeAnc.infer_gtr: setting overall rate to 1.0...", 2)
+            self._gtr.mu = 1.0
+        return self._gtr
 
     # ------------------------------------------------- marginal branch lengths
     def _marginal_optima(self, tol):
