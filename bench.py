@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=1000.0, help="request arrivals per second")
+    ap.add_argument("--serve-rate", default="250,1000,4000,16000",
+                    help="request arrival rates (per second) to sweep, comma-separated")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-pool", type=int, default=64)
     ap.add_argument("--branch-opt", type=int, default=0, metavar="K",
@@ -151,22 +152,27 @@ def serve_bench(eng, pool, args):
     for t in pool:  # switch to update mode (keeps per-node normalisers)
         node = int(t.nonroot_nodes()[0])
         t.update_branch(node, float(t.t_cur[node]))
-    rng = np.random.default_rng(4000)
-    reqs = make_stream(pool, args.serve_requests, args.serve_rate, rng)
+    rates = [float(x) for x in str(args.serve_rate).split(",") if x.strip()]
+    out = {"pool": len(pool), "mix": "70% branch / 20% marginal / 10% joint", "hbm_budget_GB": 200,
+           "requests_per_rate": args.serve_requests, "sweep": []}
+    for rate in rates:
+        rng = np.random.default_rng(4000)
+        reqs = make_stream(pool, args.serve_requests, rate, rng)
 
-    def replay(scratch):
-        rr = [Request(r.kind, r.tree, r.arrival, r.node, r.scale, r.rid) for r in reqs]
-        srv = Server(pool, engine=eng, slots=16, hbm_budget=200 * 10**9, scratch=scratch)
-        w0 = time.perf_counter()
-        done = srv.run(rr, realtime=True)
-        return summarize(done, time.perf_counter() - w0)
+        def replay(scratch):
+            rr = [Request(r.kind, r.tree, r.arrival, r.node, r.scale, r.rid) for r in reqs]
+            srv = Server(pool, engine=eng, slots=16, hbm_budget=200 * 10**9, scratch=scratch)
+            w0 = time.perf_counter()
+            done = srv.run(rr, realtime=True)
+            return summarize(done, time.perf_counter() - w0)
 
-    rp = replay(False)
-    sc = replay(True)
-    log(f"[bench] serving @ {args.serve_rate:.0f} req/s: p50 {rp['p50_ms']:.2f} ms p99 {rp['p99_ms']:.2f} ms "
-        f"({rp['throughput_req_s']:.0f} req/s); from scratch p50 {sc['p50_ms']:.2f} p99 {sc['p99_ms']:.2f}")
-    return {"pool": len(pool), "rate_req_s": args.serve_rate, "mix": "70% branch / 20% marginal / 10% joint",
-            "hbm_budget_GB": 200, "root_path": rp, "from_scratch": sc}
+        rp = replay(False)
+        sc = replay(True)
+        log(f"[bench] serving @ {rate:.0f} req/s offered: p50 {rp['p50_ms']:.2f} ms p99 {rp['p99_ms']:.2f} ms "
+            f"({rp['throughput_req_s']:.0f} req/s served); from scratch p50 {sc['p50_ms']:.2f} p99 {sc['p99_ms']:.2f} "
+            f"({sc['throughput_req_s']:.0f} req/s)")
+        out["sweep"].append({"rate_req_s": rate, "root_path": rp, "from_scratch": sc})
+    return out
 
 
 def copy_probe(nbytes):
