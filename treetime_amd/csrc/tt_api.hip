@@ -367,7 +367,14 @@ int32_t build_schedule(tt_tree *T, const tt_tree_desc *D) {
     return TT_OK;
 }
 
-int rowsum_slices(int R) { return std::min(R, 64); }
+// Two-stage sum of the R partial rows: k_rowsum adds G = ceil(R / S) rows
+// into each of S slices, the root kernel adds the S slices; S ~ sqrt(R)
+// keeps both chains short.
+int rowsum_slices(int R) {
+    int s = 1;
+    while (s * s < R) ++s;
+    return std::min(R, s);
+}
 
 int32_t ensure_joint(tt_tree *T) {
     if (T->joint_ready) return TT_OK;
