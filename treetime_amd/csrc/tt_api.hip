@@ -135,6 +135,7 @@ struct tt_tree {
     TreeDev d{};
     int32_t *d_post_nodes = nullptr, *d_pre_nodes = nullptr, *d_tip_nodes = nullptr, *d_all_branches = nullptr;
     int32_t *d_post_off = nullptr, *d_pre_off = nullptr, *d_pre_nint = nullptr;
+    int32_t *d_post_desc = nullptr, *d_pre_desc = nullptr;
     double *d_t = nullptr, *d_eig = nullptr, *d_v = nullptr, *d_vinv = nullptr;
     double *d_tmp_t = nullptr;
     int32_t *d_tmp_br = nullptr;
@@ -402,14 +403,17 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
             if (ri < T->post_runs.size() && T->post_runs[ri].l0 == (int)h) {
                 const LvRun &r = T->post_runs[ri++];
                 const PostLv &last = T->post_lv[r.l1 - 1];
-                RunDev rd{(decltype(RunDev::nodes))T->d_post_nodes, (decltype(RunDev::off))T->d_post_off, nullptr,
+                RunDev rd{(decltype(RunDev::nodes))T->d_post_nodes, (decltype(RunDev::desc))T->d_post_desc,
+                          (decltype(RunDev::off))T->d_post_off, nullptr,
                           r.l0, r.l1, T->post_lv[r.l0].row_off, last.row_off + last.rows, 0};
                 TT_HIP(ttk::post_run(D, rd, s));
                 h = r.l1;
                 continue;
             }
             const PostLv &L = T->post_lv[h++];
-            LevelDev ld{(decltype(LevelDev::nodes))(T->d_post_nodes + L.off), L.cnt, L.chunk, L.nchunks, L.row_off};
+            LevelDev ld{(decltype(LevelDev::nodes))(T->d_post_nodes + L.off),
+                        (decltype(LevelDev::desc))(T->d_post_desc + kPostDesc * L.off), L.cnt, L.chunk, L.nchunks,
+                        L.row_off};
             TT_HIP(ttk::post_marginal(D, ld, s));
         }
         const int S = rowsum_slices(T->R_all);
@@ -423,21 +427,26 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
             for (size_t d = 0; d < T->pre_lv.size();) {
                 if (ri < runs.size() && runs[ri].l0 == (int)d) {
                     const LvRun &r = runs[ri++];
-                    RunDev rd{(decltype(RunDev::nodes))T->d_pre_nodes, (decltype(RunDev::off))T->d_pre_off,
+                    RunDev rd{(decltype(RunDev::nodes))T->d_pre_nodes, (decltype(RunDev::desc))T->d_pre_desc,
+                              (decltype(RunDev::off))T->d_pre_off,
                               (decltype(RunDev::nint))T->d_pre_nint, r.l0, r.l1, 0, 0, tips ? 1 : 0};
                     TT_HIP(ttk::pre_run(D, rd, s));
                     d = r.l1;
                     continue;
                 }
                 const PreLv &L = T->pre_lv[d++];
-                LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off), tips ? L.n_all : L.n_int, 1, 0, 0};
+                LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off),
+                            (decltype(LevelDev::desc))(T->d_pre_desc + kPreDesc * L.off), tips ? L.n_all : L.n_int, 1,
+                            0, 0};
                 TT_HIP(ttk::pre_marginal(D, ld, s));
             }
         }
     } else {
         for (size_t h = 0; h + 1 < T->post_lv.size(); ++h) {
             auto &L = T->post_lv[h];
-            LevelDev ld{(decltype(LevelDev::nodes))(T->d_post_nodes + L.off), L.cnt, L.chunk, L.nchunks, L.row_off};
+            LevelDev ld{(decltype(LevelDev::nodes))(T->d_post_nodes + L.off),
+                        (decltype(LevelDev::desc))(T->d_post_desc + kPostDesc * L.off), L.cnt, L.chunk, L.nchunks,
+                        L.row_off};
             TT_HIP(ttk::post_joint(D, ld, tips ? 1 : 0, s));
         }
         const int S = rowsum_slices(T->R_joint);
@@ -446,7 +455,9 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
         TT_HIP(ttk::final_sum(D, nb, s));
         if (fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
         for (auto &L : T->pre_lv) {
-            LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off), tips ? L.n_all : L.n_int, 1, 0, 0};
+            LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off),
+                            (decltype(LevelDev::desc))(T->d_pre_desc + kPreDesc * L.off), tips ? L.n_all : L.n_int, 1,
+                            0, 0};
             TT_HIP(ttk::backtrack(D, ld, s));
         }
     }
@@ -720,6 +731,19 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
         }
         pre_off.push_back((int32_t)T->pre_nodes.size());
         if (pre_nint.empty()) pre_nint.push_back(0);
+        // node descriptors of the level lists (kPostDesc / kPreDesc ints per entry)
+        std::vector<int32_t> post_desc, pre_desc;
+        for (int r : T->post_nodes) {
+            const int e0 = T->child_ptr[r], nc = T->child_ptr[r + 1] - e0;
+            const bool bin = nc == 2;
+            post_desc.insert(post_desc.end(), {r, e0, nc, bin ? T->child_ref[e0] : 0, bin ? T->child_ref[e0 + 1] : 0,
+                                               bin ? T->child_node[e0] : 0, bin ? T->child_node[e0 + 1] : 0, 0});
+        }
+        for (int c : T->pre_nodes) pre_desc.insert(pre_desc.end(), {c, T->parent[c], T->int_row[c], T->tip_row[c]});
+        if (post_desc.empty()) post_desc.assign(kPostDesc, 0);
+        if (pre_desc.empty()) pre_desc.assign(kPreDesc, 0);
+        UP(T->d_post_desc, post_desc.data(), post_desc.size());
+        UP(T->d_pre_desc, pre_desc.data(), pre_desc.size());
         UP(T->d_post_off, post_off.data(), post_off.size());
         UP(T->d_pre_off, pre_off.data(), pre_off.size());
         UP(T->d_pre_nint, pre_nint.data(), pre_nint.size());
@@ -1278,6 +1302,8 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
             tt_tree *T = trees[i];
             bt[i].T = T->d;
             bt[i].post_nodes = (decltype(bt[i].post_nodes))T->d_post_nodes;
+            bt[i].post_desc = (decltype(bt[i].post_desc))T->d_post_desc;
+            bt[i].pre_desc = (decltype(bt[i].pre_desc))T->d_pre_desc;
             bt[i].post_off = (decltype(bt[i].post_off))T->d_post_off;
             bt[i].pre_nodes = (decltype(bt[i].pre_nodes))T->d_pre_nodes;
             bt[i].pre_off = (decltype(bt[i].pre_off))T->d_pre_off;

@@ -30,60 +30,59 @@ __global__ void k_reset_queue(int *queue) { *queue = 0; }
 // The second tile's state is held in separately named variables (no arrays
 // of register structs, which the compiler may place in scratch).
 template <int N, int TL, class LP>
-__device__ __forceinline__ void post_pair_t(const TreeDev &T, int v0, int v1, bool has1, int p0, int p1, bool act0,
-                                            bool act1, LP lut, LogAcc &acc0, LogAcc &acc1) {
-    const int a0 = wave_uniform(T.child_ptr[v0]), a1 = wave_uniform(T.child_ptr[v0 + 1]);
-    const int b0 = wave_uniform(T.child_ptr[v1]), b1 = wave_uniform(T.child_ptr[v1 + 1]);
-    if (a1 - a0 == 2 && b1 - b0 == 2) {
+__device__ __forceinline__ void post_pair_t(const TreeDev &T, const PostDesc &d0, const PostDesc &d1, bool has1,
+                                            int p0, int p1, bool act0, bool act1, LP lut, LogAcc &acc0,
+                                            LogAcc &acc1) {
+    if (d0.nchild == 2 && d1.nchild == 2) {
         BinRaw<N> RA0, RB0, RA1, RB1;
         if (act0) {
-            bin_fetch<N>(T, a0, p0, RA0);
-            bin_fetch<N>(T, b0, p0, RB0);
+            bin_fetch_d<N>(T, d0, p0, RA0);
+            bin_fetch_d<N>(T, d1, p0, RB0);
         }
         if (TL == 2 && act1) {
-            bin_fetch<N>(T, a0, p1, RA1);
-            bin_fetch<N>(T, b0, p1, RB1);
+            bin_fetch_d<N>(T, d0, p1, RA1);
+            bin_fetch_d<N>(T, d1, p1, RB1);
         }
         if (act0) {
-            bin_finish<N>(T, v0, a0, p0, RA0, lut, acc0);
-            if (has1) bin_finish<N>(T, v1, b0, p0, RB0, lut, acc0);
+            bin_finish_d<N>(T, d0, p0, RA0, lut, acc0);
+            if (has1) bin_finish_d<N>(T, d1, p0, RB0, lut, acc0);
         }
         if (TL == 2 && act1) {
-            bin_finish<N>(T, v0, a0, p1, RA1, lut, acc1);
-            if (has1) bin_finish<N>(T, v1, b0, p1, RB1, lut, acc1);
+            bin_finish_d<N>(T, d0, p1, RA1, lut, acc1);
+            if (has1) bin_finish_d<N>(T, d1, p1, RB1, lut, acc1);
         }
     } else {
         // each node by its own arity (see post_pair)
         if (act0) {
-            post_one<N>(T, v0, a0, a1 - a0 == 2, p0, lut, acc0);
-            if (has1) post_one<N>(T, v1, b0, b1 - b0 == 2, p0, lut, acc0);
+            post_one_d<N>(T, d0, p0, lut, acc0);
+            if (has1) post_one_d<N>(T, d1, p0, lut, acc0);
         }
         if (TL == 2 && act1) {
-            post_one<N>(T, v0, a0, a1 - a0 == 2, p1, lut, acc1);
-            if (has1) post_one<N>(T, v1, b0, b1 - b0 == 2, p1, lut, acc1);
+            post_one_d<N>(T, d0, p1, lut, acc1);
+            if (has1) post_one_d<N>(T, d1, p1, lut, acc1);
         }
     }
 }
 
 template <int N, int TL, class LP>
-__device__ __forceinline__ void pre_pair_t(const TreeDev &T, int c0, int c1, bool has1, int p0, int p1, bool act0,
-                                           bool act1, LP lut) {
+__device__ __forceinline__ void pre_pair_t(const TreeDev &T, const PreDesc &q0, const PreDesc &q1, bool has1, int p0,
+                                           int p1, bool act0, bool act1, LP lut) {
     PreRaw<N> RA0, RB0, RA1, RB1;
     if (act0) {
-        pre_fetch<N>(T, c0, p0, RA0);
-        pre_fetch<N>(T, c1, p0, RB0);
+        pre_fetch_d<N>(T, q0, p0, RA0);
+        pre_fetch_d<N>(T, q1, p0, RB0);
     }
     if (TL == 2 && act1) {
-        pre_fetch<N>(T, c0, p1, RA1);
-        pre_fetch<N>(T, c1, p1, RB1);
+        pre_fetch_d<N>(T, q0, p1, RA1);
+        pre_fetch_d<N>(T, q1, p1, RB1);
     }
     if (act0) {
-        pre_finish<N>(T, c0, p0, RA0, lut);
-        if (has1) pre_finish<N>(T, c1, p0, RB0, lut);
+        pre_finish_ir<N>(T, q0.c, q0.ir, p0, RA0, lut);
+        if (has1) pre_finish_ir<N>(T, q1.c, q1.ir, p0, RB0, lut);
     }
     if (TL == 2 && act1) {
-        pre_finish<N>(T, c0, p1, RA1, lut);
-        if (has1) pre_finish<N>(T, c1, p1, RB1, lut);
+        pre_finish_ir<N>(T, q0.c, q0.ir, p1, RA1, lut);
+        if (has1) pre_finish_ir<N>(T, q1.c, q1.ir, p1, RB1, lut);
     }
 }
 
@@ -121,11 +120,11 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
         LogAcc acc0, acc1;
         for (int h = 0; h < B.H; ++h) {
             const int kb = B.post_off[h], ke = B.post_off[h + 1];
-            for (int k = kb + w; k < ke; k += 2 * kWaves) {
+            for (int k = wave_uniform(kb + w); k < ke; k += 2 * kWaves) {
                 const bool has1 = k + kWaves < ke;
-                const int v0 = wave_uniform(B.post_nodes[k]);
-                const int v1 = has1 ? wave_uniform(B.post_nodes[k + kWaves]) : v0;
-                post_pair_t<N, TL>(T, v0, v1, has1, p0, p1, act0, act1, s_lut, acc0, acc1);
+                const PostDesc d0 = load_post_desc(B.post_desc + kPostDesc * k);
+                const PostDesc d1 = has1 ? load_post_desc(B.post_desc + kPostDesc * (k + kWaves)) : d0;
+                post_pair_t<N, TL>(T, d0, d1, has1, p0, p1, act0, act1, s_lut, acc0, acc1);
             }
             __syncthreads();
         }
@@ -157,11 +156,11 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
         for (int d = 0; d < B.D; ++d) {
             const int kb = B.pre_off[d];
             const int ke = tips ? B.pre_off[d + 1] : kb + B.pre_nint[d];
-            for (int k = kb + w; k < ke; k += 2 * kWaves) {
+            for (int k = wave_uniform(kb + w); k < ke; k += 2 * kWaves) {
                 const bool has1 = k + kWaves < ke;
-                const int c0 = wave_uniform(B.pre_nodes[k]);
-                const int c1 = has1 ? wave_uniform(B.pre_nodes[k + kWaves]) : c0;
-                pre_pair_t<N, TL>(T, c0, c1, has1, p0, p1, act0, act1, s_lut);
+                const PreDesc q0 = load_pre_desc(B.pre_desc + kPreDesc * k);
+                const PreDesc q1 = has1 ? load_pre_desc(B.pre_desc + kPreDesc * (k + kWaves)) : q0;
+                pre_pair_t<N, TL>(T, q0, q1, has1, p0, p1, act0, act1, s_lut);
             }
             __syncthreads();
         }

@@ -61,8 +61,17 @@ struct TreeDev {
 };
 
 // One level of a level-scheduled pass.
+// Node descriptors, one per entry of the level lists, so a node step needs
+// one scalar load instead of a chain (node -> child range -> children):
+//   post entry k: desc[8k..8k+7] = {row, first child edge, children,
+//                                   child ref 0, child ref 1, child node 0, child node 1, 0}
+//   pre entry k:  desc[4k..4k+3] = {node, parent, internal row or -1, tip row or -1}
+constexpr int kPostDesc = 8;
+constexpr int kPreDesc = 4;
+
 struct LevelDev {
     const int32_t TT_C *nodes;  // internal rows (post) or node ids (pre)
+    const int32_t TT_C *desc;   // descriptors of these entries (post or pre layout)
     int32_t n_nodes;       // nodes in this launch
     int32_t chunk;         // nodes per wave (post)
     int32_t n_chunks;
@@ -73,6 +82,7 @@ struct LevelDev {
 // pattern tile, a block barrier between levels).
 struct RunDev {
     const int32_t TT_C *nodes;  // post: internal rows by height; pre: node ids by depth
+    const int32_t TT_C *desc;   // descriptors of the whole level list (post or pre layout)
     const int32_t TT_C *off;    // level offsets into nodes
     const int32_t TT_C *nint;   // pre: internal nodes per depth (listed first); unused for post
     int32_t l0, l1;             // levels [l0, l1)
@@ -126,6 +136,8 @@ hipError_t branch_profile_lh(const TreeDev &T, int gap, const double *eig, const
 struct BatchTreeDev {
     TreeDev T;
     const int32_t TT_C *post_nodes;  // internal rows, grouped by height 1..H (root last)
+    const int32_t TT_C *post_desc;   // descriptors of post_nodes
+    const int32_t TT_C *pre_desc;    // descriptors of pre_nodes
     const int32_t TT_C *post_off;    // [H + 1]
     const int32_t TT_C *pre_nodes;   // non-root node ids grouped by depth 1..D, internal first
     const int32_t TT_C *pre_off;     // [D + 1]

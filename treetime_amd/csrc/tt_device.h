@@ -220,10 +220,10 @@ struct BinRaw {
 };
 
 template <int N>
-__device__ __forceinline__ void bin_fetch(const TreeDev &T, int e0, int p, BinRaw<N> &R) {
+__device__ __forceinline__ void bin_fetch_refs(const TreeDev &T, int cref0, int cref1, int p, BinRaw<N> &R) {
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        const int cref = wave_uniform(T.child_ref[e0 + c]);
+        const int cref = c ? cref1 : cref0;
         if (cref < 0) {
             R.code[c] = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
         } else {
@@ -233,14 +233,19 @@ __device__ __forceinline__ void bin_fetch(const TreeDev &T, int e0, int p, BinRa
     }
 }
 
+template <int N>
+__device__ __forceinline__ void bin_fetch(const TreeDev &T, int e0, int p, BinRaw<N> &R) {
+    bin_fetch_refs<N>(T, wave_uniform(T.child_ref[e0]), wave_uniform(T.child_ref[e0 + 1]), p, R);
+}
+
 template <int N, class LP>
-__device__ __forceinline__ void bin_finish(const TreeDev &T, int v, int e0, int p, const BinRaw<N> &R, LP lut,
-                                           LogAcc &acc) {
+__device__ __forceinline__ void bin_finish_refs(const TreeDev &T, int v, int cref0, int cref1, int cnode0, int cnode1,
+                                                int p, const BinRaw<N> &R, LP lut, LogAcc &acc) {
     float r[2][N], d[2][N], x[N];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        const int cref = wave_uniform(T.child_ref[e0 + c]);
-        const int cnode = wave_uniform(T.child_node[e0 + c]);
+        const int cref = c ? cref1 : cref0;
+        const int cnode = c ? cnode1 : cnode0;
         if (cref < 0) {
 #pragma unroll
             for (int i = 0; i < N; ++i) r[c][i] = lut[R.code[c] * N + i];
@@ -269,6 +274,32 @@ __device__ __forceinline__ void bin_finish(const TreeDev &T, int v, int e0, int 
     acc.mul(f);
     if (T.offs) T.offs[(size_t)v * T.P + p] = log(f);  // update mode: keep the node's own normaliser
     store_post<N>(T, v, p, x, jm, xm);
+}
+
+template <int N, class LP>
+__device__ __forceinline__ void bin_finish(const TreeDev &T, int v, int e0, int p, const BinRaw<N> &R, LP lut,
+                                           LogAcc &acc) {
+    bin_finish_refs<N>(T, v, wave_uniform(T.child_ref[e0]), wave_uniform(T.child_ref[e0 + 1]),
+                       wave_uniform(T.child_node[e0]), wave_uniform(T.child_node[e0 + 1]), p, R, lut, acc);
+}
+
+// A postorder descriptor (kPostDesc ints) read with one scalar load.
+struct PostDesc {
+    int v, e0, nchild, cref0, cref1, cnode0, cnode1;
+};
+__device__ __forceinline__ PostDesc load_post_desc(const int32_t TT_C *d) {
+    return PostDesc{d[0], d[1], d[2], d[3], d[4], d[5], d[6]};
+}
+
+template <int N>
+__device__ __forceinline__ void bin_fetch_d(const TreeDev &T, const PostDesc &d, int p, BinRaw<N> &R) {
+    bin_fetch_refs<N>(T, d.cref0, d.cref1, p, R);
+}
+
+template <int N, class LP>
+__device__ __forceinline__ void bin_finish_d(const TreeDev &T, const PostDesc &d, int p, const BinRaw<N> &R, LP lut,
+                                             LogAcc &acc) {
+    bin_finish_refs<N>(T, d.v, d.cref0, d.cref1, d.cnode0, d.cnode1, p, R, lut, acc);
 }
 
 // One postorder step: the binary path for two children, else post_node.
@@ -300,6 +331,33 @@ __device__ __forceinline__ void post_pair(const TreeDev &T, int v0, int v1, bool
         // node it is paired with (results identical for every schedule)
         post_one<N>(T, v0, a0, a1 - a0 == 2, p, lut, acc);
         if (has1) post_one<N>(T, v1, b0, b1 - b0 == 2, p, lut, acc);
+    }
+}
+
+// post_pair from descriptors (no dependent scalar loads of the tree structure).
+template <int N, class LP>
+__device__ __forceinline__ void post_one_d(const TreeDev &T, const PostDesc &d, int p, LP lut, LogAcc &acc) {
+    if (d.nchild == 2) {
+        BinRaw<N> R;
+        bin_fetch_d<N>(T, d, p, R);
+        bin_finish_d<N>(T, d, p, R, lut, acc);
+    } else {
+        post_node<N>(T, d.v, p, lut, acc);
+    }
+}
+
+template <int N, class LP>
+__device__ __forceinline__ void post_pair_d(const TreeDev &T, const PostDesc &d0, const PostDesc &d1, bool has1, int p,
+                                            LP lut, LogAcc &acc) {
+    if (d0.nchild == 2 && d1.nchild == 2) {
+        BinRaw<N> RA, RB;  // both fetched unconditionally so they stay in registers
+        bin_fetch_d<N>(T, d0, p, RA);
+        bin_fetch_d<N>(T, d1, p, RB);
+        bin_finish_d<N>(T, d0, p, RA, lut, acc);
+        if (has1) bin_finish_d<N>(T, d1, p, RB, lut, acc);
+    } else {
+        post_one_d<N>(T, d0, p, lut, acc);
+        if (has1) post_one_d<N>(T, d1, p, lut, acc);
     }
 }
 
@@ -349,24 +407,33 @@ struct PreRaw {
     int code;
 };
 
+struct PreDesc {
+    int c, par, ir, tr;
+};
+__device__ __forceinline__ PreDesc load_pre_desc(const int32_t TT_C *d) { return PreDesc{d[0], d[1], d[2], d[3]}; }
+
 template <int N>
-__device__ __forceinline__ void pre_fetch(const TreeDev &T, int c, int p, PreRaw<N> &R) {
-    const int par = wave_uniform(T.parent[c]);
-    const int ir = wave_uniform(T.int_row[c]);
-    load_rowf<N>(T.prof + ((size_t)par * T.P + p) * N, R.pp);
-    if (ir >= 0) {
+__device__ __forceinline__ void pre_fetch_d(const TreeDev &T, const PreDesc &q, int p, PreRaw<N> &R) {
+    load_rowf<N>(T.prof + ((size_t)q.par * T.P + p) * N, R.pp);
+    if (q.ir >= 0) {
         R.code = 0;
-        load_rowf<N>(T.sub + ((size_t)ir * T.P + p) * N, R.sub);
+        load_rowf<N>(T.sub + ((size_t)q.ir * T.P + p) * N, R.sub);
     } else {
-        R.code = T.tip_codes[(size_t)wave_uniform(T.tip_row[c]) * T.P + p];
+        R.code = T.tip_codes[(size_t)q.tr * T.P + p];
     }
 }
 
+template <int N>
+__device__ __forceinline__ void pre_fetch(const TreeDev &T, int c, int p, PreRaw<N> &R) {
+    pre_fetch_d<N>(T, PreDesc{c, wave_uniform(T.parent[c]), wave_uniform(T.int_row[c]), wave_uniform(T.tip_row[c])},
+                   p, R);
+}
+
 template <int N, class LP>
-__device__ __forceinline__ void pre_finish(const TreeDev &T, int c, int p, const PreRaw<N> &R, LP lut) {
+__device__ __forceinline__ void pre_finish_ir(const TreeDev &T, int c, int ir, int p, const PreRaw<N> &R, LP lut) {
     const auto M = T.P32 + (size_t)c * (N * N);
     float sub[N], m[N], q[N], msg[N];
-    if (wave_uniform(T.int_row[c]) >= 0) {
+    if (ir >= 0) {
 #pragma unroll
         for (int i = 0; i < N; ++i) sub[i] = R.sub[i];
     } else {
@@ -398,6 +465,21 @@ __device__ __forceinline__ void pre_finish(const TreeDev &T, int c, int p, const
     const int jm = argmax_first<N>(msg);
     store_row<N>(T.prof + ((size_t)c * T.P + p) * N, out);
     T.states[(size_t)c * T.P + p] = (uint8_t)jm;
+}
+
+template <int N, class LP>
+__device__ __forceinline__ void pre_finish(const TreeDev &T, int c, int p, const PreRaw<N> &R, LP lut) {
+    pre_finish_ir<N>(T, c, wave_uniform(T.int_row[c]), p, R, lut);
+}
+
+template <int N, class LP>
+__device__ __forceinline__ void pre_pair_d(const TreeDev &T, const PreDesc &q0, const PreDesc &q1, bool has1, int p,
+                                           LP lut) {
+    PreRaw<N> RA, RB;
+    pre_fetch_d<N>(T, q0, p, RA);
+    pre_fetch_d<N>(T, q1, p, RB);
+    pre_finish_ir<N>(T, q0.c, q0.ir, p, RA, lut);
+    if (has1) pre_finish_ir<N>(T, q1.c, q1.ir, p, RB, lut);
 }
 
 template <int N, class LP>

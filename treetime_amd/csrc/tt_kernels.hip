@@ -39,11 +39,11 @@ __global__ __launch_bounds__(TT_BLOCK) void k_post_marginal(TreeDev T, LevelDev 
     if (chunk < L.n_chunks) {
         const int kb = chunk * L.chunk;
         const int ke = min(kb + L.chunk, L.n_nodes);
-        for (int k = kb; k < ke; k += 2) {
+        for (int k = wave_uniform(kb); k < ke; k += 2) {
             const bool has1 = k + 1 < ke;
-            const int v0 = wave_uniform(L.nodes[k]);
-            const int v1 = has1 ? wave_uniform(L.nodes[k + 1]) : v0;
-            if (act) post_pair<N>(T, v0, v1, has1, p, T.lut, acc);
+            const PostDesc d0 = load_post_desc(L.desc + kPostDesc * k);
+            const PostDesc d1 = has1 ? load_post_desc(L.desc + kPostDesc * (k + 1)) : d0;
+            if (act) post_pair_d<N>(T, d0, d1, has1, p, T.lut, acc);
         }
     }
     s_acc[w][lane] = act ? acc.log_value() : 0.0;
@@ -111,9 +111,10 @@ __global__ __launch_bounds__(TT_BLOCK) void k_pre_marginal(TreeDev T, LevelDev L
     const int p = pt * TT_WAVE + lane;
     if (p >= T.P) return;
     const bool has1 = k + 1 < L.n_nodes;
-    const int c0 = wave_uniform(L.nodes[k]);
-    const int c1 = has1 ? wave_uniform(L.nodes[k + 1]) : c0;
-    pre_pair<N>(T, c0, c1, has1, p, T.lut);
+    const int ku = wave_uniform(k);
+    const PreDesc q0 = load_pre_desc(L.desc + kPreDesc * ku);
+    const PreDesc q1 = has1 ? load_pre_desc(L.desc + kPreDesc * (ku + 1)) : q0;
+    pre_pair_d<N>(T, q0, q1, has1, p, T.lut);
 }
 
 // Runs of narrow levels (few nodes each, typically near the root): one launch
@@ -132,11 +133,11 @@ __global__ __launch_bounds__(TT_BLOCK) void k_post_run(TreeDev T, RunDev R) {
     LogAcc acc;
     for (int l = R.l0; l < R.l1; ++l) {
         const int kb = R.off[l], ke = R.off[l + 1];
-        for (int k = kb + w; k < ke; k += 2 * kW) {
+        for (int k = wave_uniform(kb + w); k < ke; k += 2 * kW) {
             const bool has1 = k + kW < ke;
-            const int v0 = wave_uniform(R.nodes[k]);
-            const int v1 = has1 ? wave_uniform(R.nodes[k + kW]) : v0;
-            if (act) post_pair<N>(T, v0, v1, has1, p, T.lut, acc);
+            const PostDesc d0 = load_post_desc(R.desc + kPostDesc * k);
+            const PostDesc d1 = has1 ? load_post_desc(R.desc + kPostDesc * (k + kW)) : d0;
+            if (act) post_pair_d<N>(T, d0, d1, has1, p, T.lut, acc);
         }
         __syncthreads();
     }
@@ -160,11 +161,11 @@ __global__ __launch_bounds__(TT_BLOCK) void k_pre_run(TreeDev T, RunDev R) {
     for (int l = R.l0; l < R.l1; ++l) {
         const int kb = R.off[l];
         const int ke = R.tips ? R.off[l + 1] : kb + R.nint[l];
-        for (int k = kb + w; k < ke; k += 2 * kW) {
+        for (int k = wave_uniform(kb + w); k < ke; k += 2 * kW) {
             const bool has1 = k + kW < ke;
-            const int c0 = wave_uniform(R.nodes[k]);
-            const int c1 = has1 ? wave_uniform(R.nodes[k + kW]) : c0;
-            if (act) pre_pair<N>(T, c0, c1, has1, p, T.lut);
+            const PreDesc q0 = load_pre_desc(R.desc + kPreDesc * k);
+            const PreDesc q1 = has1 ? load_pre_desc(R.desc + kPreDesc * (k + kW)) : q0;
+            if (act) pre_pair_d<N>(T, q0, q1, has1, p, T.lut);
         }
         __syncthreads();
     }
