@@ -240,11 +240,37 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
 /* Launch shape the autotuner chose for the plan of (trees, flags): one
  * 4-wave block (0) or one wave (1) per item of the persistent kernel, or each
  * tree's own level-kernel pass in turn (2); 64-pattern tiles per item;
- * resident blocks per CU; the measured time per launch while tuning (0 if not
- * timed).  Any output pointer may be NULL.  TT_ESTATE when no plan exists
- * yet.  No reference counterpart. */
+ * resident blocks per CU; the waves per SIMD the kernel's register budget was
+ * compiled for (0 = the compiler's own budget); the measured time per launch
+ * while tuning (0 if not timed).  Any output pointer may be NULL.  TT_ESTATE
+ * when no plan exists yet.  No reference counterpart. */
 int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, int32_t *wave_items,
-                      int32_t *tiles, int32_t *blocks_per_cu, double *tuned_ms);
+                      int32_t *tiles, int32_t *blocks_per_cu, int32_t *waves_per_simd, double *tuned_ms);
+
+/* ---- alignment compression -------------------------------------------------
+ * Replaces SequenceData.make_compressed_alignment (sequence_data.py:325-464,
+ * dense alignments) and, with TT_FILL_OVERHANGS, the overhang fill of
+ * seq2array (seq_utils.py:196-202).  `chars` holds n_seqs rows of row_stride
+ * bytes (column c of sequence s at s * row_stride + c, n_seqs * row_stride
+ * bytes in all).  ambiguous: the ambiguous character, or -1 for none.
+ * Outputs: *n_patterns = P; pattern_of_column[n_columns] =
+ * full_to_compressed_sequence_map (sequence_data.py:453-455);
+ * patterns[n_seqs][P] (row stride P) = the compressed alignment, ambiguous
+ * characters of constant columns rewritten (sequence_data.py:387-394);
+ * multiplicity[P] (sequence_data.py:445-447); char_counts[256] (host,
+ * nullable) = occurrences of every byte after the fill, from which the caller
+ * plans additional constant sites (sequence_data.py:415-442).  patterns and
+ * multiplicity must hold n_seqs * capacity and capacity entries; TT_ERANGE
+ * (with *n_patterns set, nothing else written) when P > capacity.
+ * TT_COMPRESS_DEVICE: chars, pattern_of_column, patterns and multiplicity
+ * are device pointers (otherwise host).  Returns when every output is
+ * written.  Serialised per engine (one shared scratch buffer). */
+#define TT_FILL_OVERHANGS 1u
+#define TT_COMPRESS_DEVICE 2u
+int32_t tt_compress_alignment(tt_engine *eng, const uint8_t *chars, int32_t n_seqs, int64_t n_columns,
+                              int64_t row_stride, int32_t ambiguous, uint32_t flags, int64_t capacity,
+                              int64_t *n_patterns, int32_t *pattern_of_column, uint8_t *patterns,
+                              double *multiplicity, int64_t *char_counts, void *stream);
 
 #ifdef __cplusplus
 }

@@ -96,7 +96,7 @@ SIGNATURES = {
     "tt_propagate_profile": (_I32, [_P, _I32, _I32, _PD, _I32, _I32, _P, _P]),
     "tt_prob_t_profiles": (_I32, [_P, _P, _P, _I32, _I32, _PD, _I32, _I32, _PD, _P]),
     "tt_batch_marginal": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PD, _P]),
-    "tt_batch_info": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PI32, _PI32, _PI32, _PD]),
+    "tt_batch_info": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PI32, _PI32, _PI32, _PI32, _PD]),
 }
 
 _lib = None

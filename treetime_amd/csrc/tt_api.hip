@@ -74,6 +74,11 @@ struct tt_engine {
     std::mutex mu;
     std::vector<BatchPlan *> plans;
     std::atomic<int> live_trees{0};
+    // grow-only device buffers of tt_compress_alignment: scratch, and the
+    // staging of host inputs and outputs; guarded by cmu
+    std::mutex cmu;
+    void *cscratch = nullptr, *cstage = nullptr;
+    size_t cscratch_bytes = 0, cstage_bytes = 0;
 };
 
 // A recorded batched launch over a fixed list of trees.
@@ -224,6 +229,8 @@ int32_t tt_engine_destroy(tt_engine *eng) {
     if (eng) {
         (void)hipDeviceSynchronize();
         for (BatchPlan *p : eng->plans) delete p;
+        if (eng->cscratch) (void)hipFree(eng->cscratch);
+        if (eng->cstage) (void)hipFree(eng->cstage);
     }
     delete eng;
     return TT_OK;
@@ -581,8 +588,8 @@ int32_t launch_plan(BatchPlan *P, int n, int tips, const BatchShape &c, hipStrea
 // once to warm up and twice timed with HIP events on the plan's own trees;
 // the fastest is kept in the plan.  TT_BATCH_MODE=block|wave|plain and
 // TT_BATCH_BLOCKS_PER_CU=k restrict the candidates; TT_AUTOTUNE=0 skips the
-// timing and takes the first candidate (one block per two-tile item, 5 waves
-// per SIMD, the fastest on the 256-tree nucleotide mix).  Shapes differ only in how items are spread over the chip and in
+// timing and takes the first candidate (one block per two-tile item with the
+// compiler's register budget, the fastest on the 256-tree nucleotide mix).  Shapes differ only in how items are spread over the chip and in
 // the order the per-node normalisers of an item are multiplied, so states
 // and profiles are identical for every shape and logL agrees to rounding.
 int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
@@ -596,7 +603,7 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
         const int want = mode[0] == 'b' ? 0 : mode[0] == 'w' ? 1 : mode[0] == 'p' ? kPerTreeGraphs : -1;
         return wave_items == want;
     };
-    const BatchShape kinds[] = {{0, 0, 2, 5}, {0, 0, 2, 0}, {0, 0, 1, 0}, {1, 0, 1, 0}};
+    const BatchShape kinds[] = {{0, 0, 2, 0}, {0, 0, 2, 5}, {0, 0, 1, 0}, {1, 0, 1, 0}};
     for (BatchShape kd : kinds) {
         if (!allowed(kd.wave_items)) continue;
         if (!ttk::batch_all_shapes(n) && !(kd.wave_items == 0 && kd.tiles == 1)) continue;
@@ -1243,7 +1250,7 @@ int32_t tt_prob_t_profiles(const double *pp_dev, const double *pc_dev, const dou
 }
 
 int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, int32_t *wave_items,
-                      int32_t *tiles, int32_t *blocks_per_cu, double *tuned_ms) {
+                      int32_t *tiles, int32_t *blocks_per_cu, int32_t *waves_per_simd, double *tuned_ms) {
     if (!eng || (k > 0 && !trees)) return fail(TT_EINVAL, "NULL argument");
     std::vector<tt_tree *> key(trees, trees + std::max(k, 0));
     std::lock_guard<std::mutex> g(eng->mu);
@@ -1252,6 +1259,7 @@ int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t
             if (wave_items) *wave_items = p->shape.wave_items;
             if (tiles) *tiles = p->shape.tiles;
             if (blocks_per_cu) *blocks_per_cu = p->shape.blocks_per_cu;
+            if (waves_per_simd) *waves_per_simd = p->shape.waves_per_simd;
             if (tuned_ms) *tuned_ms = p->tuned_ms;
             return TT_OK;
         }
@@ -1395,6 +1403,108 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
         for (int i = 0; i < k; ++i) logL[i] = 0.0;
         for (size_t j = 0; j < out.size(); ++j) logL[items[j].tree] += out[j];
     }
+    return TT_OK;
+}
+
+namespace {
+
+// Grow one of the engine's compression buffers to at least `need` bytes
+// (contents are not kept), counting it against the HBM budget.
+int32_t grow(tt_engine *eng, void **buf, size_t *have, size_t need) {
+    if (need <= *have) return TT_OK;
+    need = std::max<size_t>(need, 256);
+    const uint64_t cur = eng->used.load() - *have;
+    if (eng->budget && cur + need > eng->budget)
+        return fail(TT_ENOMEM_BUDGET, "HBM budget exceeded: %llu in use + %llu requested > budget %llu",
+                    (unsigned long long)cur, (unsigned long long)need, (unsigned long long)eng->budget);
+    if (*buf) TT_HIP(hipFree(*buf));
+    eng->used -= *have;
+    *buf = nullptr;
+    *have = 0;
+    TT_HIP(hipMalloc(buf, need));
+    *have = need;
+    eng->used += need;
+    return TT_OK;
+}
+
+size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+int32_t tt_compress_alignment(tt_engine *eng, const uint8_t *chars, int32_t n_seqs, int64_t n_columns,
+                              int64_t row_stride, int32_t ambiguous, uint32_t flags, int64_t capacity,
+                              int64_t *n_patterns, int32_t *pattern_of_column, uint8_t *patterns,
+                              double *multiplicity, int64_t *char_counts, void *stream) {
+    if (!eng || !n_patterns || (n_columns > 0 && (!chars || !pattern_of_column)))
+        return fail(TT_EINVAL, "NULL argument");
+    if (n_seqs < 1) return fail(TT_EINVAL, "n_seqs=%d: need at least one sequence", n_seqs);
+    if (n_columns < 0 || n_columns > INT32_MAX / 2)
+        return fail(TT_EINVAL, "n_columns=%lld out of range [0, %d]", (long long)n_columns, INT32_MAX / 2);
+    if (row_stride < n_columns) return fail(TT_EINVAL, "row_stride %lld < n_columns %lld", (long long)row_stride,
+                                            (long long)n_columns);
+    if (ambiguous < -1 || ambiguous > 255) return fail(TT_EINVAL, "ambiguous=%d: not a byte or -1", ambiguous);
+    if (flags & ~(TT_FILL_OVERHANGS | TT_COMPRESS_DEVICE)) return fail(TT_EINVAL, "unknown flags 0x%x", flags);
+    if (capacity < 0) return fail(TT_EINVAL, "capacity < 0");
+    const bool dev = flags & TT_COMPRESS_DEVICE;
+    const bool fill = (flags & TT_FILL_OVERHANGS) && ambiguous >= 0;
+    const int L = (int)n_columns;
+    *n_patterns = 0;
+    if (char_counts) std::fill(char_counts, char_counts + 256, 0);
+    if (L == 0) return TT_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> g(eng->cmu);
+    TT_HIP(hipSetDevice(eng->device));
+
+    const CompressLayout l = ttk::compress_layout(n_seqs, L);
+    int32_t rc = grow(eng, &eng->cscratch, &eng->cscratch_bytes, l.total);
+    if (rc) return rc;
+    const uint8_t *A = chars;
+    int64_t pitch = row_stride;
+    int *pat = pattern_of_column;
+    double *mult = multiplicity;
+    uint8_t *out = patterns;
+    size_t off_pat = 0, off_mult = 0, off_out = 0;
+    if (!dev) {
+        // staging: input rows at a 16-byte pitch, then pattern map, multiplicities, compressed rows
+        const int64_t pitch16 = (n_columns + 15) / 16 * 16;
+        size_t o = up256((size_t)n_seqs * pitch16);
+        off_pat = o;
+        o += up256((size_t)L * sizeof(int));
+        off_mult = o;
+        o += up256((size_t)L * sizeof(double));
+        off_out = o;
+        o += up256((size_t)n_seqs * L);
+        rc = grow(eng, &eng->cstage, &eng->cstage_bytes, o);
+        if (rc) return rc;
+        char *st = static_cast<char *>(eng->cstage);
+        TT_HIP(hipMemcpy2DAsync(st, pitch16, chars, row_stride, n_columns, n_seqs, hipMemcpyHostToDevice, s));
+        A = reinterpret_cast<const uint8_t *>(st);
+        pitch = pitch16;
+        pat = reinterpret_cast<int *>(st + off_pat);
+        mult = reinterpret_cast<double *>(st + off_mult);
+        out = reinterpret_cast<uint8_t *>(st + off_out);
+    }
+    TT_HIP(ttk::compress_columns(A, pitch, n_seqs, L, ambiguous, fill, char_counts != nullptr, eng->cscratch, l, pat,
+                                 s));
+    int P = 0;
+    TT_HIP(hipMemcpyAsync(&P, ttk::compress_total_ptr(eng->cscratch, l), sizeof(int), hipMemcpyDeviceToHost, s));
+    unsigned long long hist[256];
+    if (char_counts)
+        TT_HIP(hipMemcpyAsync(hist, ttk::compress_hist_ptr(eng->cscratch, l), sizeof hist, hipMemcpyDeviceToHost, s));
+    TT_HIP(hipStreamSynchronize(s));
+    *n_patterns = P;
+    if (P > capacity)
+        return fail(TT_ERANGE, "%d patterns exceed the output capacity %lld", P, (long long)capacity);
+    if (!patterns || !multiplicity) return fail(TT_EINVAL, "NULL patterns or multiplicity");
+    TT_HIP(ttk::compress_finish(A, pitch, n_seqs, L, ambiguous, fill, eng->cscratch, l, P, pat, mult, out, s));
+    if (!dev) {
+        TT_HIP(hipMemcpyAsync(pattern_of_column, pat, (size_t)L * sizeof(int), hipMemcpyDeviceToHost, s));
+        TT_HIP(hipMemcpyAsync(multiplicity, mult, (size_t)P * sizeof(double), hipMemcpyDeviceToHost, s));
+        TT_HIP(hipMemcpyAsync(patterns, out, (size_t)n_seqs * P, hipMemcpyDeviceToHost, s));
+    }
+    TT_HIP(hipStreamSynchronize(s));
+    if (char_counts)
+        for (int c = 0; c < 256; ++c) char_counts[c] = (int64_t)hist[c];
     return TT_OK;
 }
 

@@ -171,3 +171,23 @@ int batch_max_blocks_per_cu(int n, const BatchShape &shape);
 bool batch_all_shapes(int n);
 int device_cus();
 }  // namespace ttk
+
+// Device scratch of one pattern compression (tt_compress.hip): byte offsets
+// of its buffers in one allocation, and the row slicing of the column pass.
+struct CompressLayout {
+    int rows_per_slice, slices, part_pitch, n_blocks;
+    size_t off_ends, off_part, off_key, off_small, off_hist, off_blocks, off_col_of_pat, total;
+};
+
+namespace ttk {
+CompressLayout compress_layout(int n_rows, int L);
+// column statistics, column keys and the pattern number of every column that
+// opens a pattern; the pattern count is then at compress_total_ptr()
+hipError_t compress_columns(const uint8_t *A, int64_t pitch, int n_rows, int L, int amb, bool fill, bool hist,
+                            void *scratch, const CompressLayout &l, int *pat, hipStream_t s);
+// remaining pattern numbers, multiplicities and the [n_rows][P] compressed rows
+hipError_t compress_finish(const uint8_t *A, int64_t pitch, int n_rows, int L, int amb, bool fill, void *scratch,
+                           const CompressLayout &l, int P, int *pat, double *mult, uint8_t *out, hipStream_t s);
+const int *compress_total_ptr(void *scratch, const CompressLayout &l);
+const unsigned long long *compress_hist_ptr(void *scratch, const CompressLayout &l);
+}  // namespace ttk

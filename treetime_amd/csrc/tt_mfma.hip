@@ -146,15 +146,15 @@ __device__ __forceinline__ void store_post20(const TreeDev &T, int v, int p, boo
 
 // Postorder step of internal row v for the 32 patterns of this wave.
 template <class LP>
-__device__ __forceinline__ void post_node20(const TreeDev &T, int v, int p, bool act, int h, int i, LP lut,
-                                            LogAcc &acc) {
-    const int e0 = wave_uniform(T.child_ptr[v]), e1 = wave_uniform(T.child_ptr[v + 1]);
-    if (e1 - e0 == 2) {
+__device__ __forceinline__ void post_node20(const TreeDev &T, const PostDesc &dd, int p, bool act, int h, int i,
+                                            LP lut, LogAcc &acc) {
+    const int v = dd.v, e0 = dd.e0, e1 = dd.e0 + dd.nchild;
+    if (dd.nchild == 2) {
         float ro0[16], ro1[16];  // both children's rows fetched before any arithmetic
-        child_slots(T, wave_uniform(T.child_ref[e0]), p, act, h, lut, ro0);
-        child_slots(T, wave_uniform(T.child_ref[e0 + 1]), p, act, h, lut, ro1);
-        const v16f d0 = contract_in(ro0, h, i, T.D32 + (size_t)wave_uniform(T.child_node[e0]) * (NS * NS));
-        const v16f d1 = contract_in(ro1, h, i, T.D32 + (size_t)wave_uniform(T.child_node[e0 + 1]) * (NS * NS));
+        child_slots(T, dd.cref0, p, act, h, lut, ro0);
+        child_slots(T, dd.cref1, p, act, h, lut, ro1);
+        const v16f d0 = contract_in(ro0, h, i, T.D32 + (size_t)dd.cnode0 * (NS * NS));
+        const v16f d1 = contract_in(ro1, h, i, T.D32 + (size_t)dd.cnode1 * (NS * NS));
         float x[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) x[r] = (ro0[r] + d0[r]) * (ro1[r] + d1[r]);
@@ -218,7 +218,8 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
     if (chunk < L.n_chunks) {
         const int kb = chunk * L.chunk;
         const int ke = min(kb + L.chunk, L.n_nodes);
-        for (int k = kb; k < ke; ++k) post_node20(T, wave_uniform(L.nodes[k]), p, act, h, i, T.lut, acc);
+        for (int k = wave_uniform(kb); k < ke; ++k)
+            post_node20(T, load_post_desc(L.desc + kPostDesc * k), p, act, h, i, T.lut, acc);
     }
     s_acc[w][lane] = (act && h == 0) ? acc.log_value() : 0.0;
     __syncthreads();
@@ -231,12 +232,11 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
 }
 
 // Preorder step (treeanc.py:889-930) of node c for this wave's 32 patterns.
-__device__ __forceinline__ void pre_node20(const TreeDev &T, int c, int p, bool act, int h, int i) {
-    const int par = wave_uniform(T.parent[c]);
-    const int ir = wave_uniform(T.int_row[c]);
+__device__ __forceinline__ void pre_node20(const TreeDev &T, const PreDesc &qd, int p, bool act, int h, int i) {
+    const int c = qd.c, par = qd.par, ir = qd.ir;
     const auto P32 = T.P32 + (size_t)c * (NS * NS);
     float ro[16], pp[16];
-    child_slots(T, ir >= 0 ? ir : -(wave_uniform(T.tip_row[c]) + 1), p, act, h, T.lut, ro);
+    child_slots(T, ir >= 0 ? ir : -(qd.tr + 1), p, act, h, T.lut, ro);
     if (act) {
         load_groups(T.prof + ((size_t)par * T.P + p) * NS, h, pp);
     } else {
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(TT_BLOCK) void k_pre_marginal20(TreeDev T, LevelDev
     if (task >= L.n_nodes * n_ptiles32) return;  // whole wave exits together
     const int k = task / n_ptiles32, pt = task % n_ptiles32;
     const int p = pt * 32 + i;
-    pre_node20(T, wave_uniform(L.nodes[k]), p, p < T.P, h, i);
+    pre_node20(T, load_pre_desc(L.desc + kPreDesc * wave_uniform(k)), p, p < T.P, h, i);
 }
 
 // Runs of narrow levels in one launch (see k_post_run): a block of 4 waves
@@ -294,7 +294,8 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
     LogAcc acc;
     for (int l = R.l0; l < R.l1; ++l) {
         const int kb = R.off[l], ke = R.off[l + 1];
-        for (int k = kb + w; k < ke; k += kW) post_node20(T, wave_uniform(R.nodes[k]), p, act, h, i, T.lut, acc);
+        for (int k = wave_uniform(kb + w); k < ke; k += kW)
+            post_node20(T, load_post_desc(R.desc + kPostDesc * k), p, act, h, i, T.lut, acc);
         __syncthreads();
     }
     s_acc[w][lane] = (act && h == 0) ? acc.log_value() : 0.0;
@@ -317,7 +318,8 @@ __global__ __launch_bounds__(TT_BLOCK) void k_pre_run20(TreeDev T, RunDev R) {
     for (int l = R.l0; l < R.l1; ++l) {
         const int kb = R.off[l];
         const int ke = R.tips ? R.off[l + 1] : kb + R.nint[l];
-        for (int k = kb + w; k < ke; k += kW) pre_node20(T, wave_uniform(R.nodes[k]), p, act, h, i);
+        for (int k = wave_uniform(kb + w); k < ke; k += kW)
+            pre_node20(T, load_pre_desc(R.desc + kPreDesc * k), p, act, h, i);
         __syncthreads();
     }
 }

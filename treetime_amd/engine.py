@@ -270,11 +270,13 @@ def batch_marginal(engine, trees, reconstruct_tips=False, stream=None, want_logL
 
 def batch_info(engine, trees, reconstruct_tips=False):
     """Launch shape chosen by the autotuner for this list of trees:
-    {"wave_items", "tiles", "blocks_per_cu", "tuned_ms"}."""
+    {"wave_items", "tiles", "blocks_per_cu", "waves_per_simd", "tuned_ms"}
+    (waves_per_simd 0 = the compiler's register budget)."""
     k = len(trees)
     arr = (C.c_void_p * k)(*[t.handle.value for t in trees])
-    w, tl, b, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+    w, tl, b, wps, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
     flags = _native.TT_RECONSTRUCT_TIPS if reconstruct_tips else 0
     _native.check(engine.lib.tt_batch_info(engine.handle, arr, k, flags, C.byref(w), C.byref(tl), C.byref(b),
-                                           C.byref(ms)))
-    return {"wave_items": w.value, "tiles": tl.value, "blocks_per_cu": b.value, "tuned_ms": ms.value}
+                                           C.byref(wps), C.byref(ms)))
+    return {"wave_items": w.value, "tiles": tl.value, "blocks_per_cu": b.value, "waves_per_simd": wps.value,
+            "tuned_ms": ms.value}
