@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-compress", action="store_true", help="skip the pattern-compression measurement")
     ap.add_argument("--serve-rate", default="250,1000,4000,16000",
                     help="request arrival rates (per second) to sweep, comma-separated")
     ap.add_argument("--serve-requests", type=int, default=600)
@@ -235,6 +236,79 @@ def branch_opt_bench(trees, preps, torch):
         f"= {done / dt:.2f} branches/s (1 core)")
 
 
+def synthetic_alignment(N, L, n_variable, seed):
+    """(N, L) uint8 nucleotide alignment of the large single-tree shape: a
+    random root sequence, n_variable columns carrying mutations on a random
+    subset of rows, 2% N, and leading / trailing gaps on 1% of the rows."""
+    rng = np.random.default_rng(seed)
+    acgt = np.frombuffer(b"ACGT", dtype=np.uint8)
+    A = np.repeat(rng.choice(acgt, size=L)[None, :], N, axis=0)
+    cols = rng.choice(L, size=n_variable, replace=False)
+    for c in cols:
+        rows = rng.random(N) < rng.uniform(0.001, 0.3)
+        rows[rng.integers(N)] = True
+        A[rows, c] = rng.choice(acgt, size=int(rows.sum()))
+    k = N * L // 50
+    A[rng.integers(0, N, k), rng.integers(0, L, k)] = ord("N")
+    for r in rng.choice(N, size=max(1, N // 100), replace=False):
+        A[r, : rng.integers(1, 500)] = ord("-")
+        A[r, L - rng.integers(1, 500):] = ord("-")
+    return A
+
+
+def compress_bench(eng, torch, steps, N=5000, L=30000, n_variable=2600, cpu_seconds=5.0):
+    """Pattern compression (tt_compress_alignment) of a device-resident
+    5000 x 30000 alignment with the overhang fill, timed with HIP events per
+    call (each call waits for its pattern count once, mid-way); the FP64
+    oracle's column-by-column compression on a sample of columns beside it."""
+    from oracle import fp64 as O
+    from treetime_amd.prepare import fill_overhangs
+    from treetime_amd.sequence_data import compress_columns
+
+    A = synthetic_alignment(N, L, n_variable, 4242)
+    pitch = (L + 15) // 16 * 16
+    host = np.zeros((N, pitch), dtype=np.uint8)
+    host[:, :L] = A
+    d = torch.from_numpy(host).cuda()
+    pat = torch.empty(L, dtype=torch.int32, device="cuda")
+    comp = torch.empty(N * L, dtype=torch.uint8, device="cuda")
+    mult = torch.empty(L, dtype=torch.float64, device="cuda")
+    P = eng.compress_alignment_device(d, L, "N", pat, comp, mult, fill_overhangs=True)
+    # parity of this exact input against the numpy compression (itself pinned to the oracle)
+    rp, rc = compress_columns(fill_overhangs(A, "N"), "N")
+    ok = (P == rc.shape[1] and np.array_equal(pat.cpu().numpy(), rp)
+          and np.array_equal(comp[:N * P].cpu().numpy().reshape(N, P), rc))
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        eng.compress_alignment_device(d, L, "N", pat, comp, mult, fill_overhangs=True)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    n_var = int((mult[:P].cpu().numpy() == 1).sum())
+    alg = N * L + N * n_var + N * P + 4 * L + 8 * P
+    # CPU: the oracle's column loop on the first columns for about cpu_seconds
+    names = ["s%d" % i for i in range(N)]
+    F = fill_overhangs(A, "N").view("S1").astype("U1")
+    cols, t0 = 250, time.perf_counter()
+    while True:
+        O.compress_alignment(names, {nm: F[i, :cols] for i, nm in enumerate(names)}, "N")
+        dt = time.perf_counter() - t0
+        if dt > cpu_seconds or cols >= L:
+            break
+        cols, t0 = min(L, cols * 2), time.perf_counter()
+    out = {"workload": f"compress-{N}x{L}", "patterns": P, "variable_patterns": n_var, "ms_per_call": ms,
+           "columns_per_s": L / (ms / 1e3), "algorithmic_bytes": alg, "achieved_GBs": alg / (ms / 1e3) / 1e9,
+           "frac_of_hbm_peak": alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "parity_vs_numpy": bool(ok),
+           "cpu_oracle_columns_per_s": cols / dt, "cpu_sample": f"oracle column loop over the first {cols} columns"}
+    log(f"[bench] compress {N}x{L}: P={P} ({n_var} variable) {ms:.3f} ms/call, {out['achieved_GBs']:.0f} GB/s "
+        f"algorithmic, parity {ok}; oracle {cols / dt:.0f} columns/s")
+    del d, comp
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_ceiling(torch, nbytes=2 << 30, reps=5):
     """Streaming rates of plain device kernels (GB/s, bytes moved per second):
     copy (read + write) and write-only (fill) -- the measured ceilings beside
@@ -283,6 +357,15 @@ def main():
         return 2
     from treetime_amd import synth
 
+    if args.workload == "compress":
+        import torch
+
+        from treetime_amd.engine import Engine
+        eng = Engine(0, 0)
+        print(json.dumps({"compression": compress_bench(eng, torch, args.steps, cpu_seconds=0 if args.no_cpu else 5.0)}),
+              flush=True)
+        eng.close()
+        return 0
     if args.workload == "mix":
         specs = [synth.mix_job_spec(j) for j in range(args.jobs)]
         wname = f"mix{args.jobs}-nuc-marginal-batched"
@@ -359,6 +442,10 @@ def main():
     if args.branch_opt:
         branch_opt_bench(trees[: args.branch_opt], preps[: args.branch_opt], torch)
 
+    compression = None
+    if not args.no_compress:
+        compression = compress_bench(eng, torch, 10, cpu_seconds=0 if args.no_cpu else 5.0)
+
     serving = None
     if not args.no_serve and len(trees) > 1:
         serving = serve_bench(eng, trees[: args.serve_pool], args)
@@ -407,6 +494,7 @@ def main():
                      "algorithmic_bytes_per_launch": int(B)},
         "cpu_baseline": cpu,
         "serving": serving,
+        "compression": compression,
     }
     print(json.dumps(out), flush=True)
     for t in trees:

@@ -88,6 +88,22 @@ int32_t tt_tree_info(const tt_tree *tree, int32_t *n_levels_post, int32_t *n_lev
  * the device.  Replaces GTR.expQt calls inside the passes. */
 int32_t tt_tree_set_model(tt_tree *tree, const double *eigvals, const double *v, const double *v_inv, double mu);
 
+/* Site-specific GTR model (GTR_site_specific, gtr_site_specific.py:43-116,
+ * 312-371): pattern p has eigenvalues eigvals[p][n], eigenvectors v[p][n][n]
+ * and v_inv[p][n][n] (P_p(t) = v diag(exp(mu[p] t eigvals)) v_inv, the
+ * reference's v[:, :, p].T and v_inv[:, :, p].T), rate mu[p] and root prior
+ * pi[p][n] (treeanc.py:817-820, 1018).  With G >= 2 grid points t_grid[G],
+ * branches with t * rate_scale < 10 use P(t) interpolated linearly between
+ * the grid points around t (the approximate=True interpolator,
+ * gtr_site_specific.py:331-371); other branches, or G = 0, the exact P(t).
+ * Replaces tt_tree_set_model until that is called again; tt_marginal,
+ * tt_joint and tt_loglik then run per-pattern matrices.  Alphabets of up to
+ * 5 states.  tt_batch_marginal, tt_update_branch, the branch-length sweeps
+ * and tt_mutation_counts refuse such trees (TT_EINVAL). */
+int32_t tt_tree_set_site_model(tt_tree *tree, const double *eigvals, const double *v, const double *v_inv,
+                               const double *mu, const double *pi, const double *t_grid, int32_t G,
+                               double rate_scale);
+
 /* New effective lengths for k branches (node ids); rebuilds their P(t). */
 int32_t tt_tree_set_branch_lengths(tt_tree *tree, const int32_t *nodes, const double *t, int32_t k);
 

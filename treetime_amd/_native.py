@@ -25,6 +25,8 @@ TT_ESTATE = -5
 TT_RECONSTRUCT_TIPS = 1
 TT_PROFILE_MARGINAL = 0
 TT_PROFILE_SUBTREE = 1
+TT_FILL_OVERHANGS = 1
+TT_COMPRESS_DEVICE = 2
 
 
 class NativeLibraryError(RuntimeError):
@@ -64,6 +66,8 @@ _PF = C.POINTER(C.c_float)
 _PI32 = C.POINTER(C.c_int32)
 _PU8 = C.POINTER(C.c_uint8)
 _PI16 = C.POINTER(C.c_int16)
+_I64 = C.c_int64
+_PI64 = C.POINTER(C.c_int64)
 
 # name -> (restype, argtypes); every symbol the header declares
 SIGNATURES = {
@@ -97,6 +101,8 @@ SIGNATURES = {
     "tt_prob_t_profiles": (_I32, [_P, _P, _P, _I32, _I32, _PD, _I32, _I32, _PD, _P]),
     "tt_batch_marginal": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PD, _P]),
     "tt_batch_info": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PI32, _PI32, _PI32, _PI32, _PD]),
+    "tt_tree_set_site_model": (_I32, [_P, _PD, _PD, _PD, _PD, _PD, _PD, _I32, _D]),
+    "tt_compress_alignment": (_I32, [_P, _P, _I32, _I64, _I64, _I32, _U32, _I64, _PI64, _P, _P, _P, _PI64, _P]),
 }
 
 _lib = None

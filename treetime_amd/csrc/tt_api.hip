@@ -136,6 +136,12 @@ struct tt_tree {
     bool have_model = false;
     double mu = 1.0;
     std::vector<double> eig, v, vinv;
+    // site-specific model (tt_tree_set_site_model): interpolation grid (empty:
+    // exact P(t) for every branch), rate scale, per-branch (t_lo, t_hi, w)
+    bool site = false;
+    std::vector<double> site_grid, site_br;
+    double site_rate_scale = 0.0;
+    void *d_site = nullptr;  // one allocation holding every per-pattern site array
 
     TreeDev d{};
     int32_t *d_post_nodes = nullptr, *d_pre_nodes = nullptr, *d_tip_nodes = nullptr, *d_all_branches = nullptr;
@@ -407,7 +413,7 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
     if (mode == 1 || mode == 3) {
         size_t ri = 0;
         for (size_t h = 0; h < T->post_lv.size();) {
-            if (ri < T->post_runs.size() && T->post_runs[ri].l0 == (int)h) {
+            if (!T->site && ri < T->post_runs.size() && T->post_runs[ri].l0 == (int)h) {
                 const LvRun &r = T->post_runs[ri++];
                 const PostLv &last = T->post_lv[r.l1 - 1];
                 RunDev rd{(decltype(RunDev::nodes))T->d_post_nodes, (decltype(RunDev::desc))T->d_post_desc,
@@ -421,18 +427,18 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
             LevelDev ld{(decltype(LevelDev::nodes))(T->d_post_nodes + L.off),
                         (decltype(LevelDev::desc))(T->d_post_desc + kPostDesc * L.off), L.cnt, L.chunk, L.nchunks,
                         L.row_off};
-            TT_HIP(ttk::post_marginal(D, ld, s));
+            TT_HIP(T->site ? ttk::post_site(D, ld, s) : ttk::post_marginal(D, ld, s));
         }
         const int S = rowsum_slices(T->R_all);
         TT_HIP(ttk::rowsum(D, T->R_all, S, s));
-        TT_HIP(ttk::root_marginal(D, S, s));
+        TT_HIP(T->site ? ttk::root_site(D, S, s) : ttk::root_marginal(D, S, s));
         TT_HIP(ttk::final_sum(D, nb, s));
         if (mode == 1) {
             if (fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
             const std::vector<LvRun> &runs = T->pre_runs[tips ? 1 : 0];
             size_t ri = 0;
             for (size_t d = 0; d < T->pre_lv.size();) {
-                if (ri < runs.size() && runs[ri].l0 == (int)d) {
+                if (!T->site && ri < runs.size() && runs[ri].l0 == (int)d) {
                     const LvRun &r = runs[ri++];
                     RunDev rd{(decltype(RunDev::nodes))T->d_pre_nodes, (decltype(RunDev::desc))T->d_pre_desc,
                               (decltype(RunDev::off))T->d_pre_off,
@@ -445,7 +451,7 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
                 LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off),
                             (decltype(LevelDev::desc))(T->d_pre_desc + kPreDesc * L.off), tips ? L.n_all : L.n_int, 1,
                             0, 0};
-                TT_HIP(ttk::pre_marginal(D, ld, s));
+                TT_HIP(T->site ? ttk::pre_site(D, ld, s) : ttk::pre_marginal(D, ld, s));
             }
         }
     } else {
@@ -454,11 +460,11 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
             LevelDev ld{(decltype(LevelDev::nodes))(T->d_post_nodes + L.off),
                         (decltype(LevelDev::desc))(T->d_post_desc + kPostDesc * L.off), L.cnt, L.chunk, L.nchunks,
                         L.row_off};
-            TT_HIP(ttk::post_joint(D, ld, tips ? 1 : 0, s));
+            TT_HIP(T->site ? ttk::post_joint_site(D, ld, tips ? 1 : 0, s) : ttk::post_joint(D, ld, tips ? 1 : 0, s));
         }
         const int S = rowsum_slices(T->R_joint);
         TT_HIP(ttk::rowsum(D, T->R_joint, S, s));
-        TT_HIP(ttk::root_joint(D, S, tips ? 1 : 0, s));
+        TT_HIP(T->site ? ttk::root_joint_site(D, S, tips ? 1 : 0, s) : ttk::root_joint(D, S, tips ? 1 : 0, s));
         TT_HIP(ttk::final_sum(D, nb, s));
         if (fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
         for (auto &L : T->pre_lv) {
@@ -552,6 +558,43 @@ int32_t check_range(tt_tree *T, const double *t, int k) {
                         "large (mu*t*lambda = %g)",
                         T->mu * t[i] * lmax);
     }
+    return TT_OK;
+}
+
+int32_t refuse_site(const tt_tree *T, const char *what) {
+    return T->site ? fail(TT_EINVAL, "%s is not available for site-specific models", what) : TT_OK;
+}
+
+void destroy_graphs(tt_tree *T) {
+    for (auto &g : T->graphs) (void)hipGraphExecDestroy(g.second);
+    T->graphs.clear();
+}
+
+// (t_lo, t_hi, w) of every branch for the site-specific model: the
+// reference's interp1d over the grid (gtr_site_specific.py:331-371) when
+// t * rate_scale < 10 -- lo/hi around t as searchsorted (left side) clipped
+// to [1, G - 1] picks them -- else the exact P(t) as (t, t, 0).
+int32_t upload_site_branches(tt_tree *T) {
+    const int nn = T->n_nodes;
+    T->site_br.assign((size_t)3 * nn, 0.0);
+    const std::vector<double> &g = T->site_grid;
+    const int G = (int)g.size();
+    for (int c = 0; c < nn; ++c) {
+        const double t = T->t[c];
+        double *b = &T->site_br[3 * (size_t)c];
+        if (G >= 2 && t * T->site_rate_scale < 10.0) {
+            int hi = (int)(std::lower_bound(g.begin(), g.end(), t) - g.begin());
+            hi = std::min(std::max(hi, 1), G - 1);
+            b[0] = g[hi - 1];
+            b[1] = g[hi];
+            b[2] = (t - g[hi - 1]) / (g[hi] - g[hi - 1]);
+        } else {
+            b[0] = b[1] = t;
+            b[2] = 0.0;
+        }
+    }
+    TT_HIP(hipMemcpyAsync((void *)T->d.s_br, T->site_br.data(), T->site_br.size() * sizeof(double),
+                          hipMemcpyHostToDevice, T->stream));
     return TT_OK;
 }
 
@@ -853,11 +896,17 @@ int32_t tt_tree_set_model(tt_tree *T, const double *eigvals, const double *v, co
     T->eig.assign(eigvals, eigvals + n);
     T->v.assign(v, v + n * n);
     T->vinv.assign(v_inv, v_inv + n * n);
+    int32_t rc = check_range(T, T->t.data(), T->n_nodes);
+    if (rc) return rc;
     T->mu = mu;
     T->offs_valid = false;
     T->profiles_fresh = false;
-    int32_t rc = check_range(T, T->t.data(), T->n_nodes);
-    if (rc) return rc;
+    if (T->site) {  // back to one model for all patterns: other kernels
+        T->site = false;
+        T->d.s_V32 = T->d.s_Vi32 = T->d.s_rho32 = nullptr;
+        T->d.s_V64 = T->d.s_Vi64 = T->d.s_rho64 = T->d.s_pi = T->d.s_br = nullptr;
+        destroy_graphs(T);
+    }
     TT_HIP(hipSetDevice(T->eng->device));
     TT_HIP(hipMemcpyAsync(T->d_eig, eigvals, n * sizeof(double), hipMemcpyHostToDevice, T->stream));
     TT_HIP(hipMemcpyAsync(T->d_v, v, n * n * sizeof(double), hipMemcpyHostToDevice, T->stream));
@@ -870,6 +919,69 @@ int32_t tt_tree_set_model(tt_tree *T, const double *eigvals, const double *v, co
 
 namespace {
 int32_t set_lengths(tt_tree *T, const int32_t *nodes, const double *t, int32_t k);
+}
+
+int32_t tt_tree_set_site_model(tt_tree *T, const double *eigvals, const double *v, const double *v_inv,
+                               const double *mu, const double *pi, const double *t_grid, int32_t G,
+                               double rate_scale) {
+    if (!T || !eigvals || !v || !v_inv || !mu || !pi || (G > 0 && !t_grid)) return fail(TT_EINVAL, "NULL argument");
+    const int n = T->n, P = T->P;
+    if (n > 5) return fail(TT_EINVAL, "site-specific models are built for alphabets of up to 5 states (n = %d)", n);
+    if (G < 0 || G == 1) return fail(TT_EINVAL, "G = %d: the grid needs 0 or at least 2 points", G);
+    for (int i = 1; i < G; ++i)
+        if (!(t_grid[i] > t_grid[i - 1])) return fail(TT_EINVAL, "t_grid must increase strictly");
+    if (G > 0 && !(rate_scale > 0.0)) return fail(TT_EINVAL, "rate_scale must be > 0");
+    for (int p = 0; p < P; ++p) {
+        if (!(mu[p] >= 0.0)) return fail(TT_EINVAL, "mu[%d] must be >= 0", p);
+        for (int j = 0; j < n; ++j)
+            if (!(pi[(size_t)p * n + j] > 0.0)) return fail(TT_EINVAL, "pi[%d][%d] must be > 0", p, j);
+    }
+    TT_HIP(hipSetDevice(T->eng->device));
+    // one allocation: V32, Vi32, rho32 (float), then V64, Vi64, rho64, pi, branch params (double)
+    const size_t nn2 = (size_t)P * n * n, nn1 = (size_t)P * n;
+    const size_t f_bytes = ((2 * nn2 + nn1) * sizeof(float) + 255) & ~(size_t)255;
+    const size_t bytes = f_bytes + (2 * nn2 + 2 * nn1 + 3 * (size_t)T->n_nodes) * sizeof(double);
+    int32_t rc;
+    if (!T->d_site && (rc = T->alloc(&T->d_site, bytes))) return rc;
+    std::vector<float> f(2 * nn2 + nn1);
+    std::vector<double> dd(2 * nn2 + 2 * nn1);
+    for (size_t i = 0; i < nn2; ++i) {
+        f[i] = (float)v[i];
+        f[nn2 + i] = (float)v_inv[i];
+        dd[i] = v[i];
+        dd[nn2 + i] = v_inv[i];
+    }
+    for (int p = 0; p < P; ++p)
+        for (int k = 0; k < n; ++k) {
+            const size_t i = (size_t)p * n + k;
+            const double r = mu[p] * eigvals[i];
+            f[2 * nn2 + i] = (float)r;
+            dd[2 * nn2 + i] = r;
+            dd[2 * nn2 + nn1 + i] = pi[i];
+        }
+    char *base = static_cast<char *>(T->d_site);
+    TT_HIP(hipMemcpyAsync(base, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, T->stream));
+    TT_HIP(hipMemcpyAsync(base + f_bytes, dd.data(), dd.size() * sizeof(double), hipMemcpyHostToDevice, T->stream));
+    const float *fb = reinterpret_cast<const float *>(base);
+    const double *db = reinterpret_cast<const double *>(base + f_bytes);
+    T->d.s_V32 = (decltype(T->d.s_V32))fb;
+    T->d.s_Vi32 = (decltype(T->d.s_Vi32))(fb + nn2);
+    T->d.s_rho32 = (decltype(T->d.s_rho32))(fb + 2 * nn2);
+    T->d.s_V64 = (decltype(T->d.s_V64))db;
+    T->d.s_Vi64 = (decltype(T->d.s_Vi64))(db + nn2);
+    T->d.s_rho64 = (decltype(T->d.s_rho64))(db + 2 * nn2);
+    T->d.s_pi = (decltype(T->d.s_pi))(db + 2 * nn2 + nn1);
+    T->d.s_br = (decltype(T->d.s_br))(db + 2 * nn2 + 2 * nn1);
+    T->site_grid.assign(t_grid, t_grid + std::max(G, 0));
+    T->site_rate_scale = rate_scale;
+    if (!T->site) destroy_graphs(T);
+    T->site = true;
+    if ((rc = upload_site_branches(T))) return rc;
+    TT_HIP(hipStreamSynchronize(T->stream));
+    T->have_model = true;
+    T->offs_valid = false;
+    T->profiles_fresh = false;
+    return TT_OK;
 }
 
 int32_t tt_tree_set_branch_lengths(tt_tree *T, const int32_t *nodes, const double *t, int32_t k) {
@@ -887,14 +999,17 @@ int32_t set_lengths(tt_tree *T, const int32_t *nodes, const double *t, int32_t k
     if (k <= 0) return TT_OK;
     for (int i = 0; i < k; ++i)
         if (nodes[i] < 0 || nodes[i] >= T->n_nodes) return fail(TT_EINVAL, "node %d out of range", nodes[i]);
-    if (T->have_model) {
+    if (T->have_model && !T->site) {
         int32_t rc = check_range(T, t, k);
         if (rc) return rc;
     }
     TT_HIP(hipSetDevice(T->eng->device));
     for (int i = 0; i < k; ++i) T->t[nodes[i]] = t[i];
     TT_HIP(hipMemcpyAsync(T->d_t, T->t.data(), T->n_nodes * sizeof(double), hipMemcpyHostToDevice, T->stream));
-    if (T->have_model) {
+    if (T->site) {
+        int32_t rc = upload_site_branches(T);
+        if (rc) return rc;
+    } else if (T->have_model) {
         if (k > T->tmp_cap) {
             TT_HIP(hipStreamSynchronize(T->stream));  // earlier copies may still read the old buffers
             T->release(T->d_tmp_t);
@@ -932,6 +1047,7 @@ int32_t tt_update_branch(tt_tree *T, int32_t node, double t, double *logL, void 
     if (node < 0 || node >= T->n_nodes) return fail(TT_EINVAL, "node %d out of range", node);
     if (node == T->root) return fail(TT_EINVAL, "the root has no branch");
     if (!T->have_model) return fail(TT_ESTATE, "tt_tree_set_model has not been called");
+    if (int32_t r = refuse_site(T, "tt_update_branch")) return r;
     TT_HIP(hipSetDevice(T->eng->device));
     int32_t rc = enable_update_mode(T);
     if (rc) return rc;
@@ -961,6 +1077,7 @@ int32_t tt_optimize_branch_lengths(tt_tree *T, const int16_t *code_state, int32_
                                    double tol, double *t_opt, int32_t *status, double *pair_counts, void *stream) {
     if (!T || !code_state || !t_opt || !status) return fail(TT_EINVAL, "NULL argument");
     if (!T->have_model) return fail(TT_ESTATE, "tt_tree_set_model has not been called");
+    if (int32_t r = refuse_site(T, "tt_optimize_branch_lengths")) return r;
     if (!T->have_states) return fail(TT_ESTATE, "no marginal or joint reconstruction has run on this tree");
     if (gap_state < -1 || gap_state >= T->n) return fail(TT_EINVAL, "gap_state %d outside [-1, %d)", gap_state, T->n);
     if (!(max_t > 0.0) || !(tol >= 0.0)) return fail(TT_EINVAL, "need max_t > 0 and tol >= 0");
@@ -992,6 +1109,7 @@ int32_t tt_optimize_branch_lengths_marginal(tt_tree *T, int32_t gap_state, doubl
                                             int32_t *status, double *root_pair, void *stream) {
     if (!T || !t_opt || !status) return fail(TT_EINVAL, "NULL argument");
     if (!T->have_model) return fail(TT_ESTATE, "tt_tree_set_model has not been called");
+    if (int32_t r = refuse_site(T, "tt_optimize_branch_lengths_marginal")) return r;
     if (!T->profiles_fresh)
         return fail(TT_ESTATE, "marginal ancestral inference needs to be performed first (profiles are stale)");
     if (gap_state < -1 || gap_state >= T->n) return fail(TT_EINVAL, "gap_state %d outside [-1, %d)", gap_state, T->n);
@@ -1029,6 +1147,7 @@ int32_t tt_branch_profile_lh(tt_tree *T, int32_t gap_state, const double *t, int
     if (!T || (K > 0 && (!t || !out))) return fail(TT_EINVAL, "NULL argument");
     if (K <= 0) return TT_OK;
     if (!T->have_model) return fail(TT_ESTATE, "tt_tree_set_model has not been called");
+    if (int32_t r = refuse_site(T, "tt_branch_profile_lh")) return r;
     if (!T->profiles_fresh)
         return fail(TT_ESTATE, "marginal ancestral inference needs to be performed first (profiles are stale)");
     if (gap_state < -1 || gap_state >= T->n) return fail(TT_EINVAL, "gap_state %d outside [-1, %d)", gap_state, T->n);
@@ -1081,6 +1200,7 @@ int32_t tt_branch_state_counts(tt_tree *T, const int16_t *code_state, int32_t ga
 
 int32_t tt_mutation_counts(tt_tree *T, double *nij, double *Ti, double *per_branch, void *stream) {
     if (!T || !nij || !Ti) return fail(TT_EINVAL, "NULL argument");
+    if (int32_t r = refuse_site(T, "tt_mutation_counts")) return r;
     if (T->n > 5) return fail(TT_EINVAL, "mutation counts are built for alphabets of up to 5 states (n = %d)", T->n);
     if (!T->profiles_fresh)
         return fail(TT_ESTATE, "marginal ancestral inference needs to be performed first (profiles are stale)");
@@ -1277,6 +1397,7 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
         if (T->eng != eng) return fail(TT_EINVAL, "trees[%d] belongs to another engine", i);
         if (T->n != n) return fail(TT_EINVAL, "trees[%d] has %d states, trees[0] has %d", i, T->n, n);
         if (!T->have_model) return fail(TT_ESTATE, "trees[%d]: tt_tree_set_model has not been called", i);
+        if (T->site) return fail(TT_EINVAL, "trees[%d] has a site-specific model: run it with tt_marginal", i);
     }
     TT_HIP(hipSetDevice(eng->device));
     const uint32_t fl = flags & TT_RECONSTRUCT_TIPS;
@@ -1445,6 +1566,7 @@ int32_t tt_compress_alignment(tt_engine *eng, const uint8_t *chars, int32_t n_se
     if (ambiguous < -1 || ambiguous > 255) return fail(TT_EINVAL, "ambiguous=%d: not a byte or -1", ambiguous);
     if (flags & ~(TT_FILL_OVERHANGS | TT_COMPRESS_DEVICE)) return fail(TT_EINVAL, "unknown flags 0x%x", flags);
     if (capacity < 0) return fail(TT_EINVAL, "capacity < 0");
+    if (capacity > 0 && (!patterns || !multiplicity)) return fail(TT_EINVAL, "NULL patterns or multiplicity");
     const bool dev = flags & TT_COMPRESS_DEVICE;
     const bool fill = (flags & TT_FILL_OVERHANGS) && ambiguous >= 0;
     const int L = (int)n_columns;
@@ -1486,6 +1608,10 @@ int32_t tt_compress_alignment(tt_engine *eng, const uint8_t *chars, int32_t n_se
     }
     TT_HIP(ttk::compress_columns(A, pitch, n_seqs, L, ambiguous, fill, char_counts != nullptr, eng->cscratch, l, pat,
                                  s));
+    // device outputs: the caller's capacity bounds the writes; staged host
+    // outputs: the staging holds every possible pattern
+    TT_HIP(ttk::compress_finish(A, pitch, n_seqs, L, ambiguous, fill, eng->cscratch, l, dev ? capacity : (int64_t)L,
+                                pat, mult, out, s));
     int P = 0;
     TT_HIP(hipMemcpyAsync(&P, ttk::compress_total_ptr(eng->cscratch, l), sizeof(int), hipMemcpyDeviceToHost, s));
     unsigned long long hist[256];
@@ -1495,14 +1621,12 @@ int32_t tt_compress_alignment(tt_engine *eng, const uint8_t *chars, int32_t n_se
     *n_patterns = P;
     if (P > capacity)
         return fail(TT_ERANGE, "%d patterns exceed the output capacity %lld", P, (long long)capacity);
-    if (!patterns || !multiplicity) return fail(TT_EINVAL, "NULL patterns or multiplicity");
-    TT_HIP(ttk::compress_finish(A, pitch, n_seqs, L, ambiguous, fill, eng->cscratch, l, P, pat, mult, out, s));
     if (!dev) {
         TT_HIP(hipMemcpyAsync(pattern_of_column, pat, (size_t)L * sizeof(int), hipMemcpyDeviceToHost, s));
         TT_HIP(hipMemcpyAsync(multiplicity, mult, (size_t)P * sizeof(double), hipMemcpyDeviceToHost, s));
         TT_HIP(hipMemcpyAsync(patterns, out, (size_t)n_seqs * P, hipMemcpyDeviceToHost, s));
+        TT_HIP(hipStreamSynchronize(s));
     }
-    TT_HIP(hipStreamSynchronize(s));
     if (char_counts)
         for (int c = 0; c < 256; ++c) char_counts[c] = (int64_t)hist[c];
     return TT_OK;

@@ -58,6 +58,18 @@ struct TreeDev {
     double TT_G *bpart;              // [nblocks]       block partials of sum(mult * seqlh)
     double TT_G *total;              // [1]
     double TT_G *offs;               // [n_int][P] per-node log normaliser, written only in update mode
+    // site-specific model (all null for a model shared by every pattern):
+    // per pattern p, P_p(t) = V diag(exp(t rho)) V^-1 with V [p][i][k],
+    // V^-1 [p][k][j], rho [p][k] = mu_p * eigenvalue_k, root prior pi [p][n];
+    // per branch (node id c) s_br[3c..3c+2] = (t_lo, t_hi, w): P(t_lo) + w (P(t_hi) - P(t_lo))
+    const float TT_C *s_V32;
+    const float TT_C *s_Vi32;
+    const float TT_C *s_rho32;
+    const double TT_C *s_V64;
+    const double TT_C *s_Vi64;
+    const double TT_C *s_rho64;
+    const double TT_C *s_pi;
+    const double TT_C *s_br;
 };
 
 // One level of a level-scheduled pass.
@@ -106,6 +118,12 @@ hipError_t pre_run20(const TreeDev &T, const RunDev &R, hipStream_t s);   // MFM
 hipError_t post_marginal20(const TreeDev &T, const LevelDev &L, hipStream_t s);  // MFMA, n = 20
 hipError_t pre_marginal20(const TreeDev &T, const LevelDev &L, hipStream_t s);   // MFMA, n = 20
 hipError_t post_joint(const TreeDev &T, const LevelDev &L, int store_tip_cx, hipStream_t s);
+// site-specific models (tt_site.hip), alphabets of up to 5 states
+hipError_t post_site(const TreeDev &T, const LevelDev &L, hipStream_t s);
+hipError_t root_site(const TreeDev &T, int S, hipStream_t s);
+hipError_t pre_site(const TreeDev &T, const LevelDev &L, hipStream_t s);
+hipError_t post_joint_site(const TreeDev &T, const LevelDev &L, int store_tip_cx, hipStream_t s);
+hipError_t root_joint_site(const TreeDev &T, int S, int store_tip_cx, hipStream_t s);
 hipError_t root_joint(const TreeDev &T, int S, int store_tip_cx, hipStream_t s);
 hipError_t backtrack(const TreeDev &T, const LevelDev &L, hipStream_t s);
 hipError_t path_update(const TreeDev &T, int node, hipStream_t s);
@@ -186,8 +204,11 @@ CompressLayout compress_layout(int n_rows, int L);
 hipError_t compress_columns(const uint8_t *A, int64_t pitch, int n_rows, int L, int amb, bool fill, bool hist,
                             void *scratch, const CompressLayout &l, int *pat, hipStream_t s);
 // remaining pattern numbers, multiplicities and the [n_rows][P] compressed rows
+// (the pattern count is read on the device; nothing but the column map is
+// written when it exceeds capacity)
 hipError_t compress_finish(const uint8_t *A, int64_t pitch, int n_rows, int L, int amb, bool fill, void *scratch,
-                           const CompressLayout &l, int P, int *pat, double *mult, uint8_t *out, hipStream_t s);
+                           const CompressLayout &l, int64_t capacity, int *pat, double *mult, uint8_t *out,
+                           hipStream_t s);
 const int *compress_total_ptr(void *scratch, const CompressLayout &l);
 const unsigned long long *compress_hist_ptr(void *scratch, const CompressLayout &l);
 }  // namespace ttk

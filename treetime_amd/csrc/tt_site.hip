@@ -1,0 +1,508 @@
+// Site-specific GTR models (reference treetime/gtr_site_specific.py): every
+// pattern p has its own eigensystem, P_p(t) = V_p diag(exp(t rho_p)) V_p^-1
+// with rho_p = mu_p * eigenvalues_p, and its own root prior pi_p.
+//
+// A lane owns one pattern, so its model is per-lane data; nothing about the
+// matrix is wave-uniform any more.  Messages are formed in the eigenbasis,
+// as deficits from the identity (V V^-1 = I):
+//   postorder (propagate_profile, gtr_site_specific.py:376-406)
+//     m_j = r_j + sum_k (sum_i r_i V_ik) delta_k Vinv_kj
+//   preorder (evolve, gtr_site_specific.py:408-437)
+//     e_i = q_i + sum_k V_ik delta_k (sum_j Vinv_kj q_j)
+// with delta_k = exp(t rho_k) - 1, 2n^2 + n multiply-adds per (branch,
+// pattern) instead of n^3.  With the reference's approximate=True
+// interpolator (gtr_site_specific.py:331-371) P(t) is linear between the two
+// grid points around t, and so is delta: delta = d(t_lo) + w (d(t_hi) -
+// d(t_lo)).  The per-branch (t_lo, t_hi, w) come from the host (s_br).
+//
+// The row layout, the max-normalised FP32 rows, the FP64 normaliser product
+// and the partial-row sums are those of the standard level kernels; the
+// joint pass builds each lane's log P(t) in FP64.
+#include "tt_device.h"
+
+namespace {
+using namespace ttd;
+
+template <int N>
+struct SiteModel {
+    float V[N][N], Vi[N][N], rho[N];
+};
+
+template <int N>
+__device__ __forceinline__ void load_site(const TreeDev &T, int p, SiteModel<N> &m) {
+    const auto v = T.s_V32 + (size_t)p * (N * N);
+    const auto vi = T.s_Vi32 + (size_t)p * (N * N);
+    const auto r = T.s_rho32 + (size_t)p * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            m.V[i][k] = v[i * N + k];
+            m.Vi[i][k] = vi[i * N + k];
+        }
+        m.rho[i] = r[i];
+    }
+}
+
+// Branch parameters of node c: (t_lo, t_hi, w); w == 0 means P(t_lo) itself.
+struct SiteBranch {
+    float ta, tb, w;
+};
+__device__ __forceinline__ SiteBranch site_branch(const TreeDev &T, int c) {
+    const auto b = T.s_br + 3 * (size_t)c;
+    return SiteBranch{(float)b[0], (float)b[1], (float)b[2]};
+}
+
+template <int N>
+__device__ __forceinline__ void site_delta(const SiteModel<N> &m, const SiteBranch &b, float (&d)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) d[k] = expm1f(b.ta * m.rho[k]);
+    if (b.w != 0.0f) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) d[k] = fmaf(b.w, expm1f(b.tb * m.rho[k]) - d[k], d[k]);
+    }
+}
+
+// m = r + Delta (returned as Delta)
+template <int N>
+__device__ __forceinline__ void site_deficit(const SiteModel<N> &m, const float (&d)[N], const float (&r)[N],
+                                             float (&D)[N]) {
+    float y[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) y[k] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) y[k] = fmaf(r[i], m.V[i][k], y[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) y[k] *= d[k];
+#pragma unroll
+    for (int j = 0; j < N; ++j) D[j] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) D[j] = fmaf(y[k], m.Vi[k][j], D[j]);
+    }
+}
+
+// propagate_profile with return_log clamps at TINY (gtr_site_specific.py:404-406)
+__device__ __forceinline__ float site_msg32(float r, float D) { return fmaxf((float)TT_TINY, r + D); }
+__device__ __forceinline__ double site_msg64(float r, float D) { return fmax(TT_TINY, (double)r + (double)D); }
+
+template <int N>
+__device__ __forceinline__ void site_child(const TreeDev &T, const SiteModel<N> &m, int cref, int cnode, int p,
+                                           float (&r)[N], float (&D)[N]) {
+    child_row<N>(T, cref, p, T.lut, r);
+    float d[N];
+    site_delta<N>(m, site_branch(T, cnode), d);
+    site_deficit<N>(m, d, r, D);
+}
+
+// Postorder step for internal row v (treeanc.py:857-878).  Up to two
+// children keep their messages in registers; larger polytomies take the
+// log-sum path of the standard kernels and revisit the children for the
+// normaliser.
+template <int N>
+__device__ __forceinline__ void site_post_node(const TreeDev &T, const SiteModel<N> &m, const PostDesc &dd, int p,
+                                               LogAcc &acc) {
+    float x[N];
+    int jm = 0;
+    if (dd.nchild <= 2) {
+        float r0[N], D0[N], r1[N], D1[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) r1[j] = D1[j] = 0.0f;
+        site_child<N>(T, m, dd.cref0, dd.cnode0, p, r0, D0);
+        if (dd.nchild == 2) site_child<N>(T, m, dd.cref1, dd.cnode1, p, r1, D1);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            x[j] = site_msg32(r0[j], D0[j]);
+            if (dd.nchild == 2) x[j] *= site_msg32(r1[j], D1[j]);
+        }
+        jm = argmax_first<N>(x);
+        float a0 = r0[0], b0 = D0[0], a1 = r1[0], b1 = D1[0];
+#pragma unroll
+        for (int j = 1; j < N; ++j)
+            if (j == jm) {
+                a0 = r0[j];
+                b0 = D0[j];
+                a1 = r1[j];
+                b1 = D1[j];
+            }
+        double z = site_msg64(a0, b0);
+        if (dd.nchild == 2) z *= site_msg64(a1, b1);
+        acc.mul(z);
+        float xm = x[0];
+#pragma unroll
+        for (int j = 1; j < N; ++j)
+            if (j == jm) xm = x[j];
+        store_post<N>(T, dd.v, p, x, jm, xm);
+        return;
+    }
+    float Ls[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) Ls[j] = 0.0f;
+    for (int c = 0; c < dd.nchild; ++c) {
+        const int cref = wave_uniform(T.child_ref[dd.e0 + c]);
+        const int cnode = wave_uniform(T.child_node[dd.e0 + c]);
+        float r[N], D[N];
+        site_child<N>(T, m, cref, cnode, p, r, D);
+#pragma unroll
+        for (int j = 0; j < N; ++j) Ls[j] += logf(site_msg32(r[j], D[j]));
+    }
+    jm = argmax_first<N>(Ls);
+    float Lm = Ls[0];
+#pragma unroll
+    for (int j = 1; j < N; ++j)
+        if (j == jm) Lm = Ls[j];
+#pragma unroll
+    for (int j = 0; j < N; ++j) x[j] = (j == jm) ? 1.0f : expf(Ls[j] - Lm);
+    for (int c = 0; c < dd.nchild; ++c) {
+        const int cref = wave_uniform(T.child_ref[dd.e0 + c]);
+        const int cnode = wave_uniform(T.child_node[dd.e0 + c]);
+        float r[N], D[N];
+        site_child<N>(T, m, cref, cnode, p, r, D);
+        float a = r[0], b = D[0];
+#pragma unroll
+        for (int j = 1; j < N; ++j)
+            if (j == jm) {
+                a = r[j];
+                b = D[j];
+            }
+        acc.mul(site_msg64(a, b));
+    }
+    store_row<N>(T.sub + ((size_t)dd.v * T.P + p) * N, x);
+}
+
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_post_site(TreeDev T, LevelDev L, int n_ptiles) {
+    __shared__ double s_acc[TT_BLOCK / TT_WAVE][TT_WAVE];
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    const int pt = blockIdx.x % n_ptiles, cg = blockIdx.x / n_ptiles;
+    const int p = pt * TT_WAVE + lane;
+    const bool act = p < T.P;
+    const int chunk = cg * (TT_BLOCK / TT_WAVE) + w;
+    LogAcc acc;
+    if (chunk < L.n_chunks && act) {
+        SiteModel<N> m;
+        load_site<N>(T, p, m);
+        const int kb = chunk * L.chunk;
+        const int ke = min(kb + L.chunk, L.n_nodes);
+        for (int k = wave_uniform(kb); k < ke; ++k)
+            site_post_node<N>(T, m, load_post_desc(L.desc + kPostDesc * k), p, acc);
+    }
+    s_acc[w][lane] = act ? acc.log_value() : 0.0;
+    __syncthreads();
+    if (w == 0 && act) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < TT_BLOCK / TT_WAVE; ++q) s += s_acc[q][lane];
+        T.part[(size_t)(L.row_off + cg) * T.P + p] = s;
+    }
+}
+
+// Root with the pattern's own prior (treeanc.py:817-828 with
+// gtr.Pi of shape (n, L)).
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_root_site(TreeDev T, int S) {
+    const int p = blockIdx.x * TT_BLOCK + threadIdx.x;
+    double contrib = 0.0;
+    if (p < T.P) {
+        double off = 0.0;
+        for (int k = 0; k < S; ++k) off += T.part2[(size_t)k * T.P + p];
+        double r[N], x[N];
+        load_row<N>(T.sub + ((size_t)T.root_row * T.P + p) * N, r);
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            x[j] = T.s_pi[(size_t)p * N + j] * r[j];
+            s += x[j];
+        }
+        const double lp = log(s) + off;
+        T.seqlh[p] = lp;
+        const double inv = 1.0 / s;
+        float out[N];
+        int jm = 0;
+        double mx = x[0];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            out[j] = (float)(x[j] * inv);
+            if (x[j] > mx) {
+                mx = x[j];
+                jm = j;
+            }
+        }
+        store_row<N>(T.prof + ((size_t)T.root * T.P + p) * N, out);
+        T.states[(size_t)T.root * T.P + p] = (uint8_t)jm;
+        contrib = T.mult[p] * lp;
+    }
+    const double b = block_sum(contrib);
+    if (threadIdx.x == 0) T.bpart[blockIdx.x] = b;
+}
+
+// Preorder step for node q.c (treeanc.py:889-930): outgroup by the division
+// trick, evolved down the branch in the eigenbasis, times the subtree row.
+template <int N>
+__device__ __forceinline__ void site_pre_node(const TreeDev &T, const SiteModel<N> &m, const PreDesc &q, int p) {
+    float pp[N], sub[N];
+    load_rowf<N>(T.prof + ((size_t)q.par * T.P + p) * N, pp);
+    if (q.ir >= 0) {
+        load_rowf<N>(T.sub + ((size_t)q.ir * T.P + p) * N, sub);
+    } else {
+        const int code = T.tip_codes[(size_t)q.tr * T.P + p];
+#pragma unroll
+        for (int i = 0; i < N; ++i) sub[i] = T.lut[code * N + i];
+    }
+    float d[N], D[N];
+    site_delta<N>(m, site_branch(T, q.c), d);
+    site_deficit<N>(m, d, sub, D);
+    float qv[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) qv[j] = fmaxf((float)TT_TINY, pp[j]) / site_msg32(sub[j], D[j]);
+    // e = q + V (delta * (Vinv q))
+    float y[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        float a = 0.0f;
+#pragma unroll
+        for (int j = 0; j < N; ++j) a = fmaf(m.Vi[k][j], qv[j], a);
+        y[k] = a * d[k];
+    }
+    float msg[N];
+    float xs = 0.0f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        float a = qv[i];
+#pragma unroll
+        for (int k = 0; k < N; ++k) a = fmaf(m.V[i][k], y[k], a);
+        msg[i] = a * sub[i];
+        xs += msg[i];
+    }
+    const float xi = 1.0f / xs;
+    float out[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = msg[i] * xi;
+    store_row<N>(T.prof + ((size_t)q.c * T.P + p) * N, out);
+    T.states[(size_t)q.c * T.P + p] = (uint8_t)argmax_first<N>(msg);
+}
+
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_pre_site(TreeDev T, LevelDev L, int n_ptiles) {
+    const int lane = threadIdx.x & (TT_WAVE - 1);
+    const int task = blockIdx.x * (TT_BLOCK / TT_WAVE) + threadIdx.x / TT_WAVE;
+    const int n_quads = (L.n_nodes + 3) / 4;
+    if (task >= n_quads * n_ptiles) return;
+    const int k0 = 4 * (task / n_ptiles), pt = task % n_ptiles;
+    const int p = pt * TT_WAVE + lane;
+    if (p >= T.P) return;
+    SiteModel<N> m;
+    load_site<N>(T, p, m);
+    const int ke = min(k0 + 4, L.n_nodes);
+    for (int k = wave_uniform(k0); k < ke; ++k) site_pre_node<N>(T, m, load_pre_desc(L.desc + kPreDesc * k), p);
+}
+
+// ---- joint (FP64) -------------------------------------------------------------
+
+// log max(TINY, P_p(t)) of branch c for this lane's pattern (the reference's
+// log_transitions, treeanc.py:964-967, with P of shape (n, n, L)).
+template <int N>
+__device__ __forceinline__ void site_logP(const TreeDev &T, int c, int p, double (&lp)[N][N]) {
+    const auto b = T.s_br + 3 * (size_t)c;
+    const double ta = b[0], tb = b[1], w = b[2];
+    const auto rho = T.s_rho64 + (size_t)p * N;
+    const auto V = T.s_V64 + (size_t)p * (N * N);
+    const auto Vi = T.s_Vi64 + (size_t)p * (N * N);
+    double d[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        d[k] = expm1(ta * rho[k]);
+        if (w != 0.0) d[k] += w * (expm1(tb * rho[k]) - d[k]);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double vd[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) vd[k] = V[i * N + k] * d[k];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            double a = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) a = fma(vd[k], Vi[k * N + j], a);
+            lp[i][j] = log(fmax(TT_TINY, a));
+        }
+    }
+}
+
+// Lx[j] = max_i lp[i][j] + msg_in[i], first argmax in cx
+template <int N>
+__device__ __forceinline__ void site_joint_max(const double (&lp)[N][N], const double (&in)[N], double (&best)[N],
+                                               int (&cx)[N]) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        double b = lp[0][j] + in[0];
+        int bi = 0;
+#pragma unroll
+        for (int i = 1; i < N; ++i) {
+            const double val = lp[i][j] + in[i];
+            if (val > b) {
+                b = val;
+                bi = i;
+            }
+        }
+        best[j] = b;
+        cx[j] = bi;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void site_joint_children(const TreeDev &T, int e0, int e1, int p, double (&msg)[N],
+                                                    int store_tip_cx) {
+    for (int e = e0; e < e1; ++e) {
+        const int cref = wave_uniform(T.child_ref[e]);
+        if (cref < 0) {
+            const int cnode = wave_uniform(T.child_node[e]);
+            const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
+            double tl[N], lp[N][N], best[N];
+            int cx[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) tl[i] = T.loglut[code * N + i];
+            site_logP<N>(T, cnode, p, lp);
+            site_joint_max<N>(lp, tl, best, cx);
+#pragma unroll
+            for (int j = 0; j < N; ++j) msg[j] += best[j];
+            if (store_tip_cx) store_bytes<N>(T.Cx + ((size_t)cnode * T.P + p) * N, cx);
+        } else {
+            const auto src = T.Lx + ((size_t)cref * T.P + p) * N;
+#pragma unroll
+            for (int i = 0; i < N; ++i) msg[i] += src[i];
+        }
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_post_joint_site(TreeDev T, LevelDev L, int n_ptiles, int store_tip_cx) {
+    __shared__ double s_acc[TT_BLOCK / TT_WAVE][TT_WAVE];
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    const int pt = blockIdx.x % n_ptiles, cg = blockIdx.x / n_ptiles;
+    const int p = pt * TT_WAVE + lane;
+    const bool act = p < T.P;
+    const int chunk = cg * (TT_BLOCK / TT_WAVE) + w;
+    double acc = 0.0;
+    if (chunk < L.n_chunks && act) {
+        const int kb = chunk * L.chunk;
+        const int ke = min(kb + L.chunk, L.n_nodes);
+        for (int k = wave_uniform(kb); k < ke; ++k) {
+            const int v = wave_uniform(L.nodes[k]);
+            const int vn = wave_uniform(T.int_node[v]);
+            double msg[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) msg[i] = 0.0;
+            site_joint_children<N>(T, wave_uniform(T.child_ptr[v]), wave_uniform(T.child_ptr[v + 1]), p, msg,
+                                   store_tip_cx);
+            double lp[N][N], best[N];
+            int cx[N];
+            site_logP<N>(T, vn, p, lp);
+            site_joint_max<N>(lp, msg, best, cx);
+            double off = best[0];
+#pragma unroll
+            for (int j = 1; j < N; ++j) off = fmax(off, best[j]);
+            const auto dst = T.Lx + ((size_t)v * T.P + p) * N;
+#pragma unroll
+            for (int j = 0; j < N; ++j) dst[j] = best[j] - off;
+            store_bytes<N>(T.Cx + ((size_t)vn * T.P + p) * N, cx);
+            acc += off;
+        }
+    }
+    s_acc[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && act) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < TT_BLOCK / TT_WAVE; ++q) s += s_acc[q][lane];
+        T.part[(size_t)(L.row_off + cg) * T.P + p] = s;
+    }
+}
+
+// Joint root (treeanc.py:1015-1037) with the pattern's own log pi.
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_root_joint_site(TreeDev T, int S, int store_tip_cx) {
+    const int p = blockIdx.x * TT_BLOCK + threadIdx.x;
+    double contrib = 0.0;
+    if (p < T.P) {
+        double msg[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) msg[i] = 0.0;
+        site_joint_children<N>(T, T.child_ptr[T.root_row], T.child_ptr[T.root_row + 1], p, msg, store_tip_cx);
+        int jm = 0;
+        double mx = -DBL_MAX;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            msg[i] += log(T.s_pi[(size_t)p * N + i]);
+            if (msg[i] > mx) {
+                mx = msg[i];
+                jm = i;
+            }
+        }
+        double lp = mx;
+        for (int k = 0; k < S; ++k) lp += T.part2[(size_t)k * T.P + p];
+        T.jseqlh[p] = lp;
+        T.states[(size_t)T.root * T.P + p] = (uint8_t)jm;
+        contrib = T.mult[p] * lp;
+    }
+    const double b = block_sum(contrib);
+    if (threadIdx.x == 0) T.bpart[blockIdx.x] = b;
+}
+
+inline int n_ptiles(int P) { return (P + TT_WAVE - 1) / TT_WAVE; }
+inline unsigned grid_for_tasks(long tasks) { return (unsigned)((tasks + 3) / 4); }
+
+#define TT_SITE_DISPATCH(n, CALL)                       \
+    switch (n) {                                        \
+        case 2: { constexpr int N_ = 2; CALL; } break;  \
+        case 3: { constexpr int N_ = 3; CALL; } break;  \
+        case 4: { constexpr int N_ = 4; CALL; } break;  \
+        case 5: { constexpr int N_ = 5; CALL; } break;  \
+        default: return hipErrorInvalidValue;           \
+    }
+
+}  // namespace
+
+namespace ttk {
+
+hipError_t post_site(const TreeDev &T, const LevelDev &L, hipStream_t s) {
+    if (L.n_nodes <= 0) return hipSuccess;
+    const int npt = n_ptiles(T.P);
+    const int groups = (L.n_chunks + 3) / 4;
+    TT_SITE_DISPATCH(T.n, hipLaunchKernelGGL(k_post_site<N_>, dim3(groups * npt), dim3(TT_BLOCK), 0, s, T, L, npt));
+    return hipGetLastError();
+}
+
+hipError_t root_site(const TreeDev &T, int S, hipStream_t s) {
+    TT_SITE_DISPATCH(T.n, hipLaunchKernelGGL(k_root_site<N_>, dim3(root_blocks(T.P)), dim3(TT_BLOCK), 0, s, T, S));
+    return hipGetLastError();
+}
+
+hipError_t pre_site(const TreeDev &T, const LevelDev &L, hipStream_t s) {
+    if (L.n_nodes <= 0) return hipSuccess;
+    const int npt = n_ptiles(T.P);
+    TT_SITE_DISPATCH(T.n, hipLaunchKernelGGL(k_pre_site<N_>, dim3(grid_for_tasks((long)((L.n_nodes + 3) / 4) * npt)),
+                                             dim3(TT_BLOCK), 0, s, T, L, npt));
+    return hipGetLastError();
+}
+
+hipError_t post_joint_site(const TreeDev &T, const LevelDev &L, int store_tip_cx, hipStream_t s) {
+    if (L.n_nodes <= 0) return hipSuccess;
+    const int npt = n_ptiles(T.P);
+    const int groups = (L.n_chunks + 3) / 4;
+    TT_SITE_DISPATCH(T.n, hipLaunchKernelGGL(k_post_joint_site<N_>, dim3(groups * npt), dim3(TT_BLOCK), 0, s, T, L,
+                                             npt, store_tip_cx));
+    return hipGetLastError();
+}
+
+hipError_t root_joint_site(const TreeDev &T, int S, int store_tip_cx, hipStream_t s) {
+    TT_SITE_DISPATCH(T.n, hipLaunchKernelGGL(k_root_joint_site<N_>, dim3(root_blocks(T.P)), dim3(TT_BLOCK), 0, s, T,
+                                             S, store_tip_cx));
+    return hipGetLastError();
+}
+
+}  // namespace ttk

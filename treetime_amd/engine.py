@@ -36,6 +36,48 @@ class Engine:
         _native.check(self.lib.tt_engine_usage(self.handle, C.byref(used), C.byref(budget)))
         return used.value, budget.value
 
+    def compress_alignment(self, codes, ambiguous, fill_overhangs=False, char_counts=False):
+        """Pattern compression of a (n_seqs, L) uint8 alignment on the device
+        (tt_compress_alignment; sequence_data.py:325-464, with the overhang
+        fill of seq_utils.py:196-202 when fill_overhangs).
+
+        Returns (pattern_of_column (L,) int64, patterns (n_seqs, P) uint8,
+        multiplicity (P,) float64, byte counts (256,) int64 or None)."""
+        codes = np.ascontiguousarray(codes, dtype=np.uint8)
+        if codes.ndim != 2:
+            raise ValueError("alignment must be a (n_seqs, L) byte matrix")
+        N, L = codes.shape
+        amb = -1 if ambiguous is None else (ord(ambiguous) if isinstance(ambiguous, str) else int(ambiguous))
+        flags = _native.TT_FILL_OVERHANGS if fill_overhangs else 0
+        pat = np.empty(L, dtype=np.int32)
+        out = np.empty(N * max(L, 1), dtype=np.uint8)
+        mult = np.empty(max(L, 1), dtype=np.float64)
+        counts = np.zeros(256, dtype=np.int64) if char_counts else None
+        P = C.c_int64()
+        _native.check(self.lib.tt_compress_alignment(
+            self.handle, codes.ctypes.data, N, L, L, amb, flags, L, C.byref(P), pat.ctypes.data, out.ctypes.data,
+            mult.ctypes.data, counts.ctypes.data_as(_native._PI64) if counts is not None else None, None))
+        P = P.value
+        return pat.astype(np.int64), out[:N * P].reshape(N, P).copy(), mult[:P].copy(), counts
+
+    def compress_alignment_device(self, chars, n_columns, ambiguous, pat, patterns, mult, fill_overhangs=False,
+                                  stream=None):
+        """Device-pointer form: chars is a (n_seqs, pitch) uint8 CUDA tensor
+        holding n_columns columns per row; pat (L,) int32, patterns
+        (n_seqs * capacity,) uint8 and mult (capacity,) float64 CUDA tensors
+        receive the results.  Returns P."""
+        N, pitch = chars.shape
+        amb = -1 if ambiguous is None else ord(ambiguous)
+        flags = _native.TT_COMPRESS_DEVICE | (_native.TT_FILL_OVERHANGS if fill_overhangs else 0)
+        cap = mult.numel()
+        if patterns.numel() < N * cap or pat.numel() < n_columns:
+            raise ValueError("output tensors too small")
+        P = C.c_int64()
+        _native.check(self.lib.tt_compress_alignment(
+            self.handle, chars.data_ptr(), N, int(n_columns), int(chars.stride(0)), amb, flags, cap, C.byref(P),
+            pat.data_ptr(), patterns.data_ptr(), mult.data_ptr(), None, stream))
+        return P.value
+
     def close(self):
         """Release the engine; fails (MissingDataError) while trees created
         on it are still open."""

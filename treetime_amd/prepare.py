@@ -34,20 +34,26 @@ def effective_lengths(branch_length, full_length):
     return np.maximum(ttconf.MIN_BRANCH_LENGTH / full_length, np.asarray(branch_length, dtype=float))
 
 
-def prepare(parent, branch_length, tip_chars, gtr, ambiguous, do_fill_overhangs=True, compress=True):
+def prepare(parent, branch_length, tip_chars, gtr, ambiguous, do_fill_overhangs=True, compress=True, engine=None):
     """tip_chars: (n_tips, L) 'S1' or uint8, rows in ascending tip node index.
+    With an engine, the overhang fill and the compression run on the device
+    (Engine.compress_alignment), otherwise in numpy; the results are equal.
 
     Returns a dict of DeviceTree inputs plus the compression maps.
     """
     codes = np.ascontiguousarray(tip_chars).view(np.uint8).reshape(tip_chars.shape)
     L = codes.shape[1]
-    if do_fill_overhangs and ambiguous is not None:
-        codes = fill_overhangs(codes, ambiguous)
-    if compress:
-        pat, comp = compress_columns(codes, ambiguous)
+    fill = do_fill_overhangs and ambiguous is not None
+    if compress and engine is not None:
+        pat, comp, mult, _ = engine.compress_alignment(codes, ambiguous, fill_overhangs=fill)
     else:
-        pat, comp = np.arange(L), codes
-    mult = np.bincount(pat, minlength=comp.shape[1]).astype(np.float64)
+        if fill:
+            codes = fill_overhangs(codes, ambiguous)
+        if compress:
+            pat, comp = compress_columns(codes, ambiguous)
+        else:
+            pat, comp = np.arange(L), codes
+        mult = np.bincount(pat, minlength=comp.shape[1]).astype(np.float64)
     return {
         "parent": np.asarray(parent, dtype=np.int32),
         "t": effective_lengths(branch_length, L),

@@ -13,7 +13,9 @@ of a per-column Python loop:
 * additional constant sites are appended per character composition
   (sequence_data.py:415-442).
 
-Sparse (VCF) alignments are out of scope.
+``SequenceData(..., engine=e)`` compresses on the device instead
+(``Engine.compress_alignment``, the same rules in HIP kernels); the result is
+identical.  Sparse (VCF) alignments are out of scope.
 """
 import os
 
@@ -96,8 +98,9 @@ class SequenceData:
 
     def __init__(self, aln, ref=None, logger=None, convert_upper=True, sequence_length=None, compress=True,
                  word_length=1, sequence_type=None, fill_overhangs=True, seq_multiplicity=None, ambiguous=None,
-                 **kwargs):
+                 engine=None, **kwargs):
         self.logger = logger if logger else (lambda *a, **k: None)
+        self.engine = engine
         self.convert_upper = convert_upper
         self.compress = compress
         self.word_length = word_length
@@ -163,17 +166,23 @@ class SequenceData:
         if codes.max(initial=0) > 255:
             raise ValueError("non-ASCII characters in the alignment")
         codes = codes.astype(np.uint8)
+        extra = self.additional_constant_sites if self.compress else 0
+        byte_counts = None
         if not self.compress:
             pat = np.arange(L)
             comp = codes
+        elif self.engine is not None:
+            pat, comp, _, byte_counts = self.engine.compress_alignment(codes, self.ambiguous, char_counts=bool(extra))
         else:
             pat, comp = compress_columns(codes, self.ambiguous)
-        extra = self.additional_constant_sites if self.compress else 0
         cols = [comp]
         pat_full = [pat]
         if extra:
             chars = [c for c in alphabets[self.likely_alphabet + "_nogap"] if c not in (self.ambiguous, "-")]
-            counts = {c: int(np.sum(A == c)) for c in chars}
+            if byte_counts is not None:
+                counts = {c: int(byte_counts[ord(c)]) for c in chars}
+            else:
+                counts = {c: int(np.sum(A == c)) for c in chars}
             total = sum(counts.values())
             plan = [(c, int(np.round(extra * m / total))) for c, m in counts.items()]
             left = extra

@@ -90,7 +90,7 @@ class TreeAnc:
         self.set_gtr(gtr or "JC69", **kwargs)
         self.data = SequenceData(aln, ref=ref, logger=self.logger, compress=compress, convert_upper=convert_upper,
                                  fill_overhangs=fill_overhangs, ambiguous=self.gtr.ambiguous, sequence_length=seq_len,
-                                 seq_multiplicity=seq_multiplicity)
+                                 seq_multiplicity=seq_multiplicity, engine=engine)
         if self.data.aln and self.tree:
             self._check_alignment_tree_gtr_consistency()
 
