@@ -234,6 +234,14 @@ int32_t tt_get_device_buffers(tt_tree *tree, float **profiles, uint8_t **states,
 int32_t tt_propagate_profile(const double *in_dev, int32_t P, int32_t n, const double *Q, int32_t transpose,
                              int32_t return_log, double *out_dev, void *stream);
 
+/* GTR_site_specific.propagate_profile / evolve (gtr_site_specific.py:376-437)
+ * with one matrix per site: Q_dev[P][n][n] (device, Q[a][i][j] = P_a(t)[i][j]).
+ * transpose = 0: out[a][j] = sum_i in[a][i] Q[a][i][j], then log(max(TINY, x))
+ * or max(0, x); transpose = 1: out[a][j] = sum_i in[a][i] Q[a][j][i], then
+ * log(x) or x.  in/out are device [P][n] FP64. */
+int32_t tt_propagate_profile_site(const double *in_dev, int32_t P, int32_t n, const double *Q_dev,
+                                  int32_t transpose, int32_t return_log, double *out_dev, void *stream);
+
 /* GTR.prob_t_profiles (gtr.py:922-963) for K transition matrices at once:
  * out[k] = sum_a mult[a] * log(sum_ij pc[a,i] Q_k[i,j] pp[a,j] + 1e-24)
  * (times the non-gap weight when gap_index >= 0).  pp/pc/mult are device

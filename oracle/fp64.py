@@ -19,6 +19,10 @@ oracle function             reference
                             965-1025 (propagate/evolve), 922-963
                             (prob_t_profiles), 717-749 (prob_t_compressed),
                             631-712 (state_pair), 816-920 (optimal_t_compressed)
+``SiteGTRModel``            treetime/gtr_site_specific.py:43-116 (assign_rates),
+                            312-329 (_eig), 331-371 (expQt and its
+                            interpolator), 376-437 (propagate/evolve),
+                            491-495 (average_rate)
 ``compress_alignment``      treetime/sequence_data.py:325-464
 ``OracleTreeAnc.marginal``  treetime/treeanc.py:762-932
 ``OracleTreeAnc.joint``     treetime/treeanc.py:934-1080
@@ -42,6 +46,12 @@ Pinning (tests/test_oracle_*.py):
   sequence_data.py:387-402.
 * prob_t_profiles: one-hot profiles reduce to prob_t_compressed.
 * optimal_t_compressed: JC69 closed form t = -3/4 ln(1 - 4/3 p).
+* SiteGTRModel: exact P(t) of every site = scipy.linalg.expm of the site's
+  rate matrix; the interpolated P(t) equals the exact one on grid points and
+  the mean of the neighbours half-way; a model whose sites share pi and mu
+  gives the standard model's likelihoods; sum over all n^3 tip patterns of
+  exp(logL) == 1 for every site's model; joint = brute-force max.  The grid
+  points themselves (gtr_site_specific.py:336-343) are restated, not pinned.
 """
 import numpy as np
 
@@ -158,6 +168,18 @@ def avg_transition(W, pi, gap_index=None):
     return (r - np.sum(pi * W[:, gap_index]) * pi[gap_index]) / (1 - pi[gap_index])
 
 
+def eig_single_site(W, p):
+    """treetime/gtr.py:612-629: eigensystem of the symmetrised rate matrix;
+    returns (eigenvalues, v, v_inv) with P(t) = v diag(exp(mu t lam)) v_inv."""
+    sp = np.sqrt(p)
+    sym = W * np.outer(sp, sp)
+    np.fill_diagonal(sym, -np.sum(W * p, axis=1))
+    lam, U = np.linalg.eigh(sym)
+    tv = U.T * sp                       # rows: eigenvectors scaled by sqrt(pi)
+    norm1 = np.abs(tv).sum(axis=1)
+    return lam, tv.T / norm1, (U * norm1).T / sp
+
+
 class GTRModel:
     """Reversible substitution model: rates, eigensystem and P(t).
 
@@ -192,16 +214,7 @@ class GTRModel:
         self._eig()
 
     def _eig(self):
-        W, p = self.W, self.Pi
-        sp = np.sqrt(p)
-        sym = W * np.outer(sp, sp)
-        np.fill_diagonal(sym, -np.sum(W * p, axis=1))
-        lam, U = np.linalg.eigh(sym)
-        tv = U.T * sp                       # rows: eigenvectors scaled by sqrt(pi)
-        norm1 = np.abs(tv).sum(axis=1)
-        self.eigenvals = lam
-        self.v = tv.T / norm1
-        self.v_inv = (U * norm1).T / sp
+        self.eigenvals, self.v, self.v_inv = eig_single_site(self.W, self.Pi)
 
     @property
     def Q(self):
@@ -373,6 +386,75 @@ def gtr_from_text(text):
     return GTRModel(name, mu=mu, pi=np.array(pi), W=W)
 
 
+class SiteGTRModel:
+    """Site-specific GTR (treetime/gtr_site_specific.py): one W for all
+    sites, frequencies Pi (n, L) and rates mu (L,) per site.
+
+    assign_rates (43-116): Pi columns normalised, W symmetrised with a zero
+    diagonal and divided by the average of pi_a^T W pi_a over sites, mu
+    multiplied by it.  expQt(t) has shape (n, n, L), [i, j, a] =
+    Pr(child = i | parent = j) at site a; with approximate=True and
+    t * rate_scale < 10 it is interpolated linearly between the grid points
+    around t (331-371, scipy interp1d 'linear' rules), else exact.
+    """
+
+    def __init__(self, alphabet="nuc_nogap", mu=1.0, pi=None, W=None, approximate=True):
+        self.alphabet_name = alphabet
+        self.alphabet = ALPHABETS[alphabet]
+        self.profile_map = profile_map(alphabet)
+        self.ambiguous = ambiguous_char(alphabet)
+        self.n_states = n = len(self.alphabet)
+        self.approximate = approximate
+        pi = np.array(pi, dtype=float)
+        L = pi.shape[1]
+        self.L = L
+        self.Pi = pi / pi.sum(axis=0)
+        Wm = 0.5 * (np.array(W, dtype=float) + np.array(W, dtype=float).T)
+        np.fill_diagonal(Wm, 0.0)
+        avg = np.einsum("ia,ij,ja", self.Pi, Wm, self.Pi) / L
+        self.W = Wm / avg
+        self.mu = (mu * np.ones(L) if np.isscalar(mu) else np.array(mu, dtype=float)) * avg
+        self.lam = np.zeros((L, n))
+        self.V = np.zeros((L, n, n))
+        self.Vi = np.zeros((L, n, n))
+        for a in range(L):
+            self.lam[a], self.V[a], self.Vi[a] = eig_single_site(self.W, self.Pi[:, a])
+        self.rate_scale = self.average_rate().mean()
+        self.t_grid = (1.0 / self.rate_scale) * np.concatenate((
+            np.linspace(0, 0.1, 11)[:-1], np.linspace(0.1, 1, 21)[:-1],
+            np.linspace(1, 5, 21)[:-1], np.linspace(5, 10, 11)))
+        self._grid_P = np.stack([self._expQt(t) for t in self.t_grid], axis=0)
+
+    def average_rate(self):
+        """gtr_site_specific.py:491-495."""
+        return self.mu * np.einsum("ia,ij,ja->a", self.Pi, self.W, self.Pi)
+
+    def _expQt(self, t):
+        """gtr_site_specific.py:350-365, per site V diag(exp(mu t lam)) V^-1."""
+        e = np.exp(t * self.mu[:, None] * self.lam)                    # (L, n)
+        P = np.einsum("aik,ak,akj->aij", self.V, e, self.Vi)
+        return np.moveaxis(P, 0, -1)
+
+    def expQt(self, t):
+        if self.approximate and t * self.rate_scale < 10:
+            g = self.t_grid
+            hi = int(np.clip(np.searchsorted(g, t), 1, len(g) - 1))
+            lo = hi - 1
+            slope = (self._grid_P[hi] - self._grid_P[lo]) / (g[hi] - g[lo])
+            return slope * (t - g[lo]) + self._grid_P[lo]
+        return self._expQt(t)
+
+    def propagate_profile(self, profile, t, return_log=False):
+        """gtr_site_specific.py:376-406."""
+        res = np.einsum("ai,ija->aj", profile, self.expQt(t))
+        return np.log(np.maximum(TINY_NUMBER, res)) if return_log else np.maximum(0, res)
+
+    def evolve(self, profile, t, return_log=False):
+        """gtr_site_specific.py:408-437."""
+        res = np.einsum("ai,jia->aj", profile, self.expQt(t))
+        return np.log(res) if return_log else res
+
+
 # ---------------------------------------------------------------------------
 # alignment compression (treetime/sequence_data.py:325-464)
 # ---------------------------------------------------------------------------
@@ -498,6 +580,11 @@ class OracleTreeAnc:
         """treetime/treeanc.py:752-760."""
         return max(MIN_BRANCH_LENGTH * self.one_mutation, node.branch_length)
 
+    def _root_prior(self):
+        """treetime/treeanc.py:817-820: (L, n) root prior."""
+        pi = self.gtr.Pi
+        return np.repeat([pi], self.L, axis=0) if pi.ndim == 1 else np.copy(pi.T)
+
     def _tip_profile(self, leaf):
         if leaf.name in self.aln:
             return seq2prof(self.aln[leaf.name], self.gtr.profile_map)
@@ -525,7 +612,7 @@ class OracleTreeAnc:
     def total_LH(self):
         """treetime/treeanc.py:814-838."""
         r = self.root
-        r.marginal_outgroup_LH = np.repeat([self.gtr.Pi], self.L, axis=0)
+        r.marginal_outgroup_LH = self._root_prior()
         r.marginal_profile, pre = normalize_profile(r.marginal_outgroup_LH * r.marginal_subtree_LH)
         self.sequence_LH = r.marginal_subtree_LH_prefactor + pre
         self.total_sequence_LH = float((self.sequence_LH * self.mult).sum())
@@ -571,14 +658,15 @@ class OracleTreeAnc:
             node.joint_Cx = np.zeros((self.L, n), dtype=np.uint16)
             node.joint_margin = np.zeros((self.L, n))
             for j in range(n):
-                cand = logP[:, j][None, :] + msg
+                col = logP[:, j]
+                cand = (col.T if col.ndim == 2 else col[None, :]) + msg
                 node.joint_Cx[:, j] = cand.argmax(axis=1)
                 node.joint_Lx[:, j] = cand.max(axis=1)
                 srt = np.sort(cand, axis=1)
                 node.joint_margin[:, j] = srt[:, -1] - srt[:, -2]
         r = self.root
         msg = np.sum(np.stack([c.joint_Lx for c in r.clades], axis=0), axis=0)
-        r.joint_Lx = msg + np.log(self.gtr.Pi)[None, :]
+        r.joint_Lx = msg + np.log(self._root_prior())
         normed = r.joint_Lx - r.joint_Lx.max(axis=1)[:, None]
         _, _, idx = prof2seq(np.exp(normed), self.gtr.alphabet)
         srt = np.sort(r.joint_Lx, axis=1)

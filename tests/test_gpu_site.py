@@ -1,0 +1,169 @@
+"""Site-specific GTR models on the device (tt_tree_set_site_model and the
+tt_site.hip kernels) against the oracle's SiteGTRModel
+(gtr_site_specific.py), which is pinned to scipy expm, to the standard
+model and to brute force in tests/test_oracle_pins.py.
+
+Tolerances: marginal logL 1e-7 relative, per-pattern logL
+5e-7 * max(1, |logL_p|), profiles 2e-6 absolute (FP32 rows and eigenbasis
+arithmetic, FP64 normaliser products); states bit-exact where the oracle's
+margin exceeds 1e-5.  Joint (FP64 messages, per-lane FP64 log P): logL
+1e-9 relative, states bit-exact where the path margin exceeds 1e-5."""
+import sys
+
+import numpy as np
+import pytest
+
+import treetime_amd as tt
+from oracle import fp64 as O
+from treetime_amd import synth
+from treetime_amd.engine import DeviceTree, batch_marginal, code_lut
+from treetime_amd.prepare import prepare
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def site_case(n_tips, L, seed, approximate, depth=0.05, alphabet="nuc_nogap"):
+    job = synth.make_job(n_tips, L, depth, 4, seed, 0.01, 0.005)
+    prep = prepare(job["parent"], job["branch_length"], job["chars"], job["gtr"], job["ambiguous"], compress=False)
+    g = tt.GTR_site_specific.random(L=L, alphabet=alphabet, rng=np.random.default_rng(seed + 100))
+    g.approximate = approximate
+    og = O.SiteGTRModel(alphabet, mu=g.mu, pi=g.Pi, W=g.W, approximate=approximate)
+    parent = prep["parent"]
+    names = ["n%d" % i for i in range(len(parent))]
+    root, nodes = O.build_tree(parent, prep["t"], names)
+    is_tip = np.ones(len(parent), dtype=bool)
+    is_tip[parent[parent >= 0]] = False
+    tips = np.where(is_tip)[0]
+    chars = prep["tip_codes"].view("S1").astype("U1")
+    comp = {names[v]: chars[k] for k, v in enumerate(tips)}
+    ta = O.OracleTreeAnc(root, og, comp, prep["multiplicity"], full_length=1e300)
+    return prep, g, ta, nodes
+
+
+def device(engine, prep, g):
+    lut = code_lut(g.profile_map, g.n_states)
+    return DeviceTree(engine, prep["parent"], prep["t"], prep["tip_codes"], lut, prep["multiplicity"],
+                      g.Pi.mean(axis=1), None, None, None, None, site=g.device_model())
+
+
+@pytest.mark.parametrize("n_tips,L,seed,approximate", [(60, 400, 1, True), (60, 400, 2, False),
+                                                       (300, 1500, 3, True)])
+def test_site_marginal_matches_oracle(gpu_engine, n_tips, L, seed, approximate):
+    say(f"[site] marginal tips={n_tips} L={L} seed={seed} approximate={approximate}")
+    prep, g, ta, nodes = site_case(n_tips, L, seed, approximate)
+    dev = device(gpu_engine, prep, g)
+    lh = dev.marginal()
+    ref = ta.marginal()
+    rel = abs(lh - ref) / abs(ref)
+    say(f"  logL {lh:.10f} oracle {ref:.10f} rel {rel:.2e}")
+    assert rel < 1e-7
+    seq = dev.sequence_lh()
+    assert np.all(np.abs(seq - ta.sequence_LH) <= 5e-7 * np.maximum(1.0, np.abs(ta.sequence_LH)))
+    prof = dev.profiles()
+    states = dev.states()
+    worst = 0.0
+    for i, nd in enumerate(nodes):
+        if nd.is_terminal():
+            continue
+        worst = max(worst, float(np.max(np.abs(prof[i] - nd.marginal_profile))))
+        srt = np.sort(nd.marginal_profile, axis=1)
+        sure = srt[:, -1] - srt[:, -2] > 1e-5
+        assert np.array_equal(states[i][sure], nd.seq_idx[sure])
+    say(f"  profiles max abs diff {worst:.2e}")
+    assert worst < 2e-6
+    dev.close()
+
+
+@pytest.mark.parametrize("approximate", [True, False])
+def test_site_joint_matches_oracle(gpu_engine, approximate):
+    say(f"[site] joint approximate={approximate}")
+    prep, g, ta, nodes = site_case(80, 500, 4, approximate)
+    dev = device(gpu_engine, prep, g)
+    lh = dev.joint()
+    ref = ta.joint()
+    say(f"  joint logL {lh:.10f} oracle {ref:.10f}")
+    assert abs(lh - ref) <= 1e-9 * abs(ref)
+    states = dev.states()
+    for i, nd in enumerate(nodes):
+        if nd.is_terminal():
+            continue
+        sure = nd.path_margin > 1e-5
+        assert np.array_equal(states[i][sure], nd.seq_idx[sure])
+    dev.close()
+
+
+def test_site_model_switch_and_refusals(gpu_engine):
+    """tt_tree_set_model after a site model restores the standard kernels;
+    batched launches and root-path updates refuse site-specific trees."""
+    prep, g, ta, nodes = site_case(40, 300, 5, True)
+    dev = device(gpu_engine, prep, g)
+    dev.marginal()
+    with pytest.raises(ValueError):
+        batch_marginal(gpu_engine, [dev])
+    with pytest.raises(ValueError):
+        dev.update_branch(int(dev.nonroot_nodes()[0]), 0.1)
+    std = synth.make_job(40, 300, 0.05, 4, 5, 0.01, 0.005)["gtr"]
+    dev.set_model(std.eigenvals, std.v, std.v_inv, std.mu)
+    lh_switched = dev.marginal()
+    lut = code_lut(std.profile_map, std.n_states)
+    fresh = DeviceTree(gpu_engine, prep["parent"], prep["t"], prep["tip_codes"], lut, prep["multiplicity"], std.Pi,
+                       std.eigenvals, std.v, std.v_inv, std.mu)
+    lh_fresh = fresh.marginal()
+    say(f"  switched {lh_switched:.10f} fresh {lh_fresh:.10f}")
+    # the tree descriptor's pi was the sites' mean, the fresh one the model's
+    std_oracle = O.OracleTreeAnc(ta.root, O.GTRModel("nuc_nogap", mu=std.mu, pi=std.Pi, W=std.W), ta.aln,
+                                 prep["multiplicity"], full_length=1e300)
+    assert abs(lh_fresh - std_oracle.marginal()) < 1e-7 * abs(lh_fresh)
+    fresh.close()
+    dev.close()
+
+
+def test_site_propagate_profile_primitive():
+    rng = np.random.default_rng(6)
+    g = tt.GTR_site_specific.random(L=50, alphabet="nuc", rng=rng)
+    og = O.SiteGTRModel("nuc", mu=g.mu, pi=g.Pi, W=g.W)
+    prof = rng.random((50, 5))
+    for t in (0.01, 0.7, 30.0):
+        assert np.allclose(g.propagate_profile(prof, t), og.propagate_profile(prof, t), rtol=1e-12, atol=1e-14)
+        assert np.allclose(g.propagate_profile(prof, t, return_log=True),
+                           og.propagate_profile(prof, t, return_log=True), rtol=1e-12, atol=1e-12)
+        assert np.allclose(g.evolve(prof, t), og.evolve(prof, t), rtol=1e-12, atol=1e-14)
+
+
+def test_site_facade_treeanc(gpu_engine):
+    """TreeAnc with a GTR_site_specific model (compress=False, as the
+    reference requires, treeanc.py:186-187) equals the oracle."""
+    job = synth.make_job(50, 300, 0.05, 4, 7, 0.0, 0.0)
+    parent, bl = job["parent"], job["branch_length"]
+    clades = [tt.Clade(branch_length=float(bl[i])) for i in range(len(parent))]
+    for i, p in enumerate(parent):
+        if p >= 0:
+            clades[p].clades.append(clades[i])
+    for k, v in enumerate(job["tips"]):
+        clades[v].name = job["names"][k]
+    aln = {job["names"][k]: job["chars"][k].tobytes().decode() for k in range(len(job["tips"]))}
+    g = tt.GTR_site_specific.random(L=300, alphabet="nuc_nogap", rng=np.random.default_rng(8))
+    with pytest.raises(TypeError):
+        tt.TreeAnc(tree=tt.Tree(clades[0]), aln=aln, gtr=g, verbose=0)
+    ta = tt.TreeAnc(tree=tt.Tree(clades[0]), aln=aln, gtr=g, compress=False, verbose=0, engine=gpu_engine)
+    ta.infer_ancestral_sequences(marginal=True)
+    lh = ta.sequence_LH()
+    nodes = list(ta.tree.find_clades(order="preorder"))
+    idx = {id(x): i for i, x in enumerate(nodes)}
+    par = [-1] * len(nodes)
+    for i, x in enumerate(nodes):
+        for c in x.clades:
+            par[idx[id(c)]] = i
+    root, onodes = O.build_tree(par, [ta._branch_length_to_gtr(x) for x in nodes], [x.name for x in nodes])
+    og = O.SiteGTRModel("nuc_nogap", mu=g.mu, pi=g.Pi, W=g.W)
+    orc = O.OracleTreeAnc(root, og, ta.data.compressed_alignment, ta.data.multiplicity(), full_length=1e300)
+    ref = orc.marginal()
+    say(f"[site] facade marginal {lh:.10f} oracle {ref:.10f}")
+    assert abs(lh - ref) < 1e-7 * abs(ref)
+    ta.infer_ancestral_sequences(marginal=False)
+    assert abs(ta.tree.sequence_joint_LH - orc.joint()) < 1e-9 * abs(ref)
+    ta._drop_device()

@@ -389,3 +389,107 @@ def test_gtr_infer_recovers_model():
         assert np.max(np.abs(g.Pi - g0.Pi)) < 1e-4
         assert np.max(np.abs(g.W - g0.W) / g0.W.max()) < 1e-3
         assert abs(g.mu - g0.mu) < 1e-4 * g0.mu
+
+
+# ------------------------------------------------------- site-specific GTR --
+def rand_site_model(L, seed, approximate=True, alphabet="nuc_nogap", n=4):
+    rng = np.random.default_rng(seed)
+    pi = rng.gamma(1.0, size=(n, L))
+    tmp = np.tril(rng.gamma(3.0, size=(n, n)), k=-1)
+    return O.SiteGTRModel(alphabet, mu=rng.gamma(3.0, size=L) / 3.0, pi=pi, W=tmp + tmp.T, approximate=approximate)
+
+
+def site_Q(m, a):
+    """Rate matrix of site a (gtr_site_specific.py:31-41): Q[i, j] =
+    Pi[i, a] W[i, j] off the diagonal, columns summing to 0."""
+    q = m.Pi[:, a][:, None] * m.W
+    np.fill_diagonal(q, 0.0)
+    np.fill_diagonal(q, -q.sum(axis=0))
+    return q
+
+
+@pytest.mark.parametrize("alphabet,n", [("nuc_nogap", 4), ("nuc", 5)])
+def test_site_expqt_exact_is_expm(alphabet, n):
+    m = rand_site_model(6, 11, approximate=False, alphabet=alphabet, n=n)
+    for t in [0.0, 1e-5, 0.02, 0.4, 3.0]:
+        P = m.expQt(t)
+        assert P.shape == (n, n, 6)
+        for a in range(6):
+            assert np.allclose(P[:, :, a], expm(m.mu[a] * site_Q(m, a) * t), atol=1e-12, rtol=1e-10)
+
+
+def test_site_expqt_interpolation():
+    """Linear between the grid points: equal to the exact P(t) on a grid
+    point, the mean of the neighbours' exact P half-way (scipy interp1d
+    'linear'), exact beyond t * rate_scale >= 10."""
+    m = rand_site_model(5, 12)
+    g = m.t_grid
+    exact = lambda t: np.stack([expm(m.mu[a] * site_Q(m, a) * t) for a in range(5)], axis=-1)  # noqa: E731
+    for i in [1, 7, 15, 33, 59]:
+        assert np.allclose(m.expQt(g[i]), exact(g[i]), atol=1e-12)
+        mid = 0.5 * (g[i] + g[i + 1])
+        assert np.allclose(m.expQt(mid), 0.5 * (exact(g[i]) + exact(g[i + 1])), atol=1e-12)
+    t_far = 12.0 / m.rate_scale
+    assert np.allclose(m.expQt(t_far), exact(t_far), atol=1e-12)
+    assert len(g) == 61 and g[0] == 0.0 and abs(g[-1] * m.rate_scale - 10.0) < 1e-12
+
+
+def test_site_model_with_shared_sites_equals_standard():
+    """Every site with the same pi and mu: the marginal and joint
+    likelihoods equal the standard model's."""
+    root, nodes, _, aln = three_tip_reference_case()
+    std = O.GTRModel("nuc_nogap", mu=1.0, pi=np.array([0.9, 0.06, 0.02, 0.02]), W=np.ones((4, 4)))
+    L = 64
+    site = O.SiteGTRModel("nuc_nogap", mu=1.0, pi=np.repeat(std.Pi[:, None], L, axis=1), W=np.ones((4, 4)),
+                          approximate=False)
+    out = []
+    for model in (std, site):
+        ta = O.OracleTreeAnc(root, model, aln, np.ones(L), full_length=L)
+        lh = ta.marginal()
+        out.append((lh, ta.sequence_LH.copy(), ta.joint()))
+    assert abs(out[0][0] - out[1][0]) < 1e-10
+    assert np.allclose(out[0][1], out[1][1], atol=1e-12)
+    assert abs(out[0][2] - out[1][2]) < 1e-10
+
+
+@pytest.mark.parametrize("approximate", [False, True])
+def test_site_marginal_lh_sums_to_one(approximate):
+    """K different site models, each evaluated on all n^3 tip patterns: for
+    every model the pattern likelihoods sum to 1 (test_treetime.py:149-152
+    per site)."""
+    root, nodes, _, _ = three_tip_reference_case()
+    cols = list(itertools.product("ACGT", repeat=3))
+    K = 3
+    rng = np.random.default_rng(13)
+    pi = np.repeat(rng.gamma(1.0, size=(4, K)), 64, axis=1)
+    mu = np.repeat(rng.gamma(3.0, size=K) / 3.0, 64)
+    m = O.SiteGTRModel("nuc_nogap", mu=mu, pi=pi, W=np.array([[0, 1, 2, 1], [1, 0, 1, 3], [2, 1, 0, 1],
+                                                              [1, 3, 1, 0.]]), approximate=approximate)
+    aln = {k: np.array([c[i] for c in cols] * K) for i, k in enumerate("ABC")}
+    ta = O.OracleTreeAnc(root, m, aln, np.ones(64 * K), full_length=64 * K)
+    ta.marginal()
+    for k in range(K):
+        assert abs(np.exp(ta.sequence_LH[64 * k:64 * (k + 1)]).sum() - 1.0) < 1e-10
+
+
+def test_site_joint_is_bruteforce_max():
+    """Joint sequence_LH per site = max over the internal states of
+    log pi_a(root) + sum of log P_a(t) over branches, with P_a from expm."""
+    root, nodes = tiny_tree([("E", None, 0.004), ("C", "E", 0.030), ("A", "C", 0.0060), ("B", "C", 0.30),
+                             ("D", "E", 0.020)])
+    L = 12
+    m = rand_site_model(L, 14, approximate=False)
+    rng = np.random.default_rng(15)
+    comp = {k: rng.choice(list("ACGT"), size=L) for k in "ABD"}
+    ta = O.OracleTreeAnc(root, m, comp, np.ones(L), full_length=1e9)
+    ta.joint()
+    byname = {nd.name: nd for nd in nodes}
+    idx = {c: i for i, c in enumerate("ACGT")}
+    for a in range(L):
+        logP = {k: np.log(expm(m.mu[a] * site_Q(m, a) * byname[k].branch_length)) for k in "ABCD"}
+        best = -np.inf
+        for sc, se in itertools.product(range(4), repeat=2):
+            v = (np.log(m.Pi[se, a]) + logP["C"][sc, se] + logP["D"][idx[comp["D"][a]], se]
+                 + logP["A"][idx[comp["A"][a]], sc] + logP["B"][idx[comp["B"][a]], sc])
+            best = max(best, v)
+        assert abs(best - ta.joint_sequence_LH[a]) < 1e-10

@@ -1344,6 +1344,17 @@ int32_t tt_propagate_profile(const double *in_dev, int32_t P, int32_t n, const d
     return TT_OK;
 }
 
+int32_t tt_propagate_profile_site(const double *in_dev, int32_t P, int32_t n, const double *Q_dev,
+                                  int32_t transpose, int32_t return_log, double *out_dev, void *stream) {
+    if (!in_dev || !out_dev || !Q_dev) return fail(TT_EINVAL, "NULL argument");
+    if (P < 0 || n < 1 || n > TT_MAX_STATES) return fail(TT_EINVAL, "bad shape P=%d n=%d", P, n);
+    if (P == 0) return TT_OK;
+    hipStream_t s = (hipStream_t)stream;
+    TT_HIP(ttk::propagate_site(in_dev, P, n, Q_dev, transpose, return_log, out_dev, s));
+    TT_HIP(hipStreamSynchronize(s));
+    return TT_OK;
+}
+
 int32_t tt_prob_t_profiles(const double *pp_dev, const double *pc_dev, const double *mult_dev, int32_t P, int32_t n,
                            const double *Q, int32_t K, int32_t gap_index, double *out, void *stream) {
     if (!pp_dev || !pc_dev || !mult_dev || !Q || !out) return fail(TT_EINVAL, "NULL argument");

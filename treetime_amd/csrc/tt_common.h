@@ -124,6 +124,8 @@ hipError_t root_site(const TreeDev &T, int S, hipStream_t s);
 hipError_t pre_site(const TreeDev &T, const LevelDev &L, hipStream_t s);
 hipError_t post_joint_site(const TreeDev &T, const LevelDev &L, int store_tip_cx, hipStream_t s);
 hipError_t root_joint_site(const TreeDev &T, int S, int store_tip_cx, hipStream_t s);
+hipError_t propagate_site(const double *in, int P, int n, const double *Q, int transpose, int ret_log, double *out,
+                          hipStream_t s);
 hipError_t root_joint(const TreeDev &T, int S, int store_tip_cx, hipStream_t s);
 hipError_t backtrack(const TreeDev &T, const LevelDev &L, hipStream_t s);
 hipError_t path_update(const TreeDev &T, int node, hipStream_t s);

@@ -453,6 +453,26 @@ __global__ __launch_bounds__(TT_BLOCK) void k_root_joint_site(TreeDev T, int S, 
     if (threadIdx.x == 0) T.bpart[blockIdx.x] = b;
 }
 
+// GTR_site_specific.propagate_profile / evolve (gtr_site_specific.py:376-437)
+// with the matrices Q[a][i][j] = P_a(t)[i][j] of every site a:
+//   transpose = 0: out[a][j] = sum_i in[a][i] Q[a][i][j], then
+//                  log(max(TINY, x)) or max(0, x)
+//   transpose = 1: out[a][j] = sum_i in[a][i] Q[a][j][i], then log(x) or x
+__global__ void k_propagate_site(const double *in, int P, int n, const double *Q, int transpose, int ret_log,
+                                 double *out) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (size_t)P * n) return;
+    const size_t a = gid / n;
+    const int j = (int)(gid % n);
+    const double *q = Q + a * n * n;
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) acc += in[a * n + i] * (transpose ? q[j * n + i] : q[i * n + j]);
+    if (transpose)
+        out[gid] = ret_log ? log(acc) : acc;
+    else
+        out[gid] = ret_log ? log(fmax(TT_TINY, acc)) : fmax(0.0, acc);
+}
+
 inline int n_ptiles(int P) { return (P + TT_WAVE - 1) / TT_WAVE; }
 inline unsigned grid_for_tasks(long tasks) { return (unsigned)((tasks + 3) / 4); }
 
@@ -496,6 +516,15 @@ hipError_t post_joint_site(const TreeDev &T, const LevelDev &L, int store_tip_cx
     const int groups = (L.n_chunks + 3) / 4;
     TT_SITE_DISPATCH(T.n, hipLaunchKernelGGL(k_post_joint_site<N_>, dim3(groups * npt), dim3(TT_BLOCK), 0, s, T, L,
                                              npt, store_tip_cx));
+    return hipGetLastError();
+}
+
+hipError_t propagate_site(const double *in, int P, int n, const double *Q, int transpose, int ret_log, double *out,
+                          hipStream_t s) {
+    const size_t total = (size_t)P * n;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_propagate_site, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, P, n, Q,
+                       transpose, ret_log, out);
     return hipGetLastError();
 }
 

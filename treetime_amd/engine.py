@@ -112,7 +112,7 @@ class DeviceTree:
              index of the tips.
     """
 
-    def __init__(self, engine, parent, t, tip_codes, lut, multiplicity, pi, eigvals, v, v_inv, mu):
+    def __init__(self, engine, parent, t, tip_codes, lut, multiplicity, pi, eigvals, v, v_inv, mu, site=None):
         self.engine = engine
         self.lib = engine.lib
         self.parent = _native.as_c(parent, np.int32)
@@ -133,7 +133,10 @@ class DeviceTree:
         self.handle = h
         self.t_cur = t.copy()        # current effective branch lengths (host mirror)
         self.joint_ready = False     # joint buffers allocated on the device
-        self.set_model(eigvals, v, v_inv, mu)
+        if site is not None:         # (eig, V, Vi, mu, pi, grid, rate_scale) of a site-specific model
+            self.set_site_model(*site)
+        else:
+            self.set_model(eigvals, v, v_inv, mu)
 
     def set_model(self, eigvals, v, v_inv, mu):
         e = _native.as_c(eigvals, np.float64)
@@ -141,6 +144,16 @@ class DeviceTree:
         vi = _native.as_c(v_inv, np.float64)
         _native.check(self.lib.tt_tree_set_model(self.handle, _native.ptr(e, C.c_double), _native.ptr(vv, C.c_double),
                                                  _native.ptr(vi, C.c_double), float(mu)))
+
+    def set_site_model(self, eig, V, Vi, mu, pi, grid, rate_scale):
+        """Per-pattern model (tt_tree_set_site_model): eig (P, n), V and Vi
+        (P, n, n), mu (P,), pi (P, n); grid (G,) empty for exact P(t)."""
+        arrs = [_native.as_c(x, np.float64) for x in (eig, V, Vi, mu, pi, grid)]
+        self._site_arrays = arrs
+        g = arrs[5]
+        _native.check(self.lib.tt_tree_set_site_model(
+            self.handle, *[_native.ptr(x, C.c_double) for x in arrs[:5]],
+            _native.ptr(g, C.c_double) if len(g) else None, len(g), float(rate_scale)))
 
     def nonroot_nodes(self):
         return np.nonzero(self.parent >= 0)[0]

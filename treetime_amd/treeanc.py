@@ -91,6 +91,8 @@ class TreeAnc:
         self.data = SequenceData(aln, ref=ref, logger=self.logger, compress=compress, convert_upper=convert_upper,
                                  fill_overhangs=fill_overhangs, ambiguous=self.gtr.ambiguous, sequence_length=seq_len,
                                  seq_multiplicity=seq_multiplicity, engine=engine)
+        if self.gtr.is_site_specific and self.data.compress:
+            raise TypeError("TreeAnc: sequence compression and site specific gtr models are incompatible!")
         if self.data.aln and self.tree:
             self._check_alignment_tree_gtr_consistency()
 
@@ -282,16 +284,23 @@ class TreeAnc:
                 codes[k] = self.data.compressed_codes[row[leaf.name]]
         g = self.gtr
         eng = self.engine or default_engine()
-        self._device = DeviceTree(eng, parent, t, codes, code_lut(g.profile_map, g.n_states), self.data.multiplicity(),
-                                  g.Pi, g.eigenvals, g.v, g.v_inv, g.mu)
+        if g.is_site_specific:
+            if g.seq_len != P:
+                raise ValueError("site-specific model has {} sites, the alignment {} columns".format(g.seq_len, P))
+            self._device = DeviceTree(eng, parent, t, codes, code_lut(g.profile_map, g.n_states),
+                                      self.data.multiplicity(), g.Pi.mean(axis=1), None, None, None, None,
+                                      site=g.device_model())
+        else:
+            self._device = DeviceTree(eng, parent, t, codes, code_lut(g.profile_map, g.n_states),
+                                      self.data.multiplicity(), g.Pi, g.eigenvals, g.v, g.v_inv, g.mu)
         self._nodes = nodes
         self._t = t
-        self._gtr_state = (g.mu, g.Pi.copy(), g.W.copy())
+        self._gtr_state = (np.copy(g.mu), g.Pi.copy(), g.W.copy())
         return self._device
 
     def _sync_model(self, dev):
         g = self.gtr
-        if not (self._gtr_state[0] == g.mu and np.array_equal(self._gtr_state[1], g.Pi) and
+        if not (np.array_equal(self._gtr_state[0], g.mu) and np.array_equal(self._gtr_state[1], g.Pi) and
                 np.array_equal(self._gtr_state[2], g.W)):
             self._drop_device()
             return self._device_tree()
