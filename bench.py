@@ -12,7 +12,11 @@ Printed beside it: the plain path (one tt_marginal graph per tree, trees in
 sequence), the roofline of the batched kernel from the algorithmic bytes per
 update, and the FP64 numpy oracle timed on the host as the CPU baseline.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mix|large-5000|...]
+Also measured: pattern compression of a device-resident 5000 x 30000
+alignment (the `compression` object; alone with --workload compress), and
+marginal / joint passes with a site-specific model (--workload site).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mix|large-5000|compress|site|...]
 """
 import argparse
 import json
@@ -309,6 +313,64 @@ def compress_bench(eng, torch, steps, N=5000, L=30000, n_variable=2600, cpu_seco
     return out
 
 
+def site_bench(eng, torch, steps, n_tips=1000, L=5000, seed=77, cpu=True):
+    """Marginal and joint passes with a site-specific GTR model
+    (GTR_site_specific.random, approximate=True) on an uncompressed
+    1000-tip x 5000-site alignment; the FP64 oracle's SiteGTRModel pass on
+    the same input as the parity check and the CPU baseline."""
+    from oracle import fp64 as O
+    from treetime_amd import GTR_site_specific, synth
+    from treetime_amd.engine import DeviceTree, code_lut
+    from treetime_amd.prepare import prepare
+
+    job = synth.make_job(n_tips, L, 0.05, 4, seed, 0.01, 0.005)
+    prep = prepare(job["parent"], job["branch_length"], job["chars"], job["gtr"], job["ambiguous"], compress=False)
+    g = GTR_site_specific.random(L=L, alphabet="nuc_nogap", rng=np.random.default_rng(seed))
+    dev = DeviceTree(eng, prep["parent"], prep["t"], prep["tip_codes"], code_lut(g.profile_map, 4),
+                     prep["multiplicity"], g.Pi.mean(axis=1), None, None, None, None, site=g.device_model())
+    s = torch.cuda.Stream()
+    sp = s.cuda_stream
+    lh = dev.marginal()
+    ljt = dev.joint()
+    res = {}
+    for name, fn in (("marginal", dev.marginal), ("joint", dev.joint)):
+        fn(stream=sp, want_logL=False)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(steps):
+            fn(stream=sp, want_logL=False)
+        e1.record(s)
+        torch.cuda.synchronize()
+        res[name] = e0.elapsed_time(e1) / steps
+    U = L * (len(prep["parent"]) - 1)
+    out = {"workload": f"site-{n_tips}x{L}", "updates_per_pass": U, "marginal_ms": res["marginal"],
+           "marginal_updates_per_s": U / (res["marginal"] / 1e3), "joint_ms": res["joint"],
+           "joint_updates_per_s": U / (res["joint"] / 1e3)}
+    if cpu:
+        parent = prep["parent"]
+        names = ["n%d" % i for i in range(len(parent))]
+        root, nodes = O.build_tree(parent, prep["t"], names)
+        is_tip = np.ones(len(parent), dtype=bool)
+        is_tip[parent[parent >= 0]] = False
+        chars = prep["tip_codes"].view("S1").astype("U1")
+        comp = {names[v]: chars[k] for k, v in enumerate(np.where(is_tip)[0])}
+        og = O.SiteGTRModel("nuc_nogap", mu=g.mu, pi=g.Pi, W=g.W)
+        ta = O.OracleTreeAnc(root, og, comp, prep["multiplicity"], full_length=1e300)
+        t0 = time.perf_counter()
+        ref = ta.marginal()
+        dt = time.perf_counter() - t0
+        ref_j = ta.joint()
+        out.update({"logL_rel_err_vs_fp64_oracle": abs(lh - ref) / abs(ref),
+                    "joint_logL_rel_err_vs_fp64_oracle": abs(ljt - ref_j) / abs(ref_j),
+                    "cpu_oracle_marginal_s": dt, "cpu_oracle_updates_per_s": U / dt})
+    log(f"[bench] site-specific {n_tips}x{L}: marginal {res['marginal']:.3f} ms "
+        f"({out['marginal_updates_per_s']:.3e} U/s), joint {res['joint']:.3f} ms"
+        + (f"; logL rel err {out['logL_rel_err_vs_fp64_oracle']:.1e}, oracle {dt:.1f} s" if cpu else ""))
+    dev.close()
+    return out
+
+
 def copy_ceiling(torch, nbytes=2 << 30, reps=5):
     """Streaming rates of plain device kernels (GB/s, bytes moved per second):
     copy (read + write) and write-only (fill) -- the measured ceilings beside
@@ -357,6 +419,14 @@ def main():
         return 2
     from treetime_amd import synth
 
+    if args.workload == "site":
+        import torch
+
+        from treetime_amd.engine import Engine
+        eng = Engine(0, 0)
+        print(json.dumps({"site_specific": site_bench(eng, torch, args.steps, cpu=not args.no_cpu)}), flush=True)
+        eng.close()
+        return 0
     if args.workload == "compress":
         import torch
 
