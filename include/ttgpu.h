@@ -249,6 +249,19 @@ int32_t tt_propagate_profile_site(const double *in_dev, int32_t P, int32_t n, co
 int32_t tt_prob_t_profiles(const double *pp_dev, const double *pc_dev, const double *mult_dev, int32_t P,
                            int32_t n, const double *Q, int32_t K, int32_t gap_index, double *out, void *stream);
 
+/* Per-pattern GTR statistics summed over all branches, for the
+ * site-specific GTR.infer of TreeAnc.infer_gtr(site_specific=True)
+ * (treeanc.py:1556-1589, 1593-1602): n_out[P][n][n] = n_ija (derived state
+ * i, ancestral state j) and t_out[P][n] = T_ia, in node order.  marginal != 0:
+ * expected counts from the branch pair posteriors of the last marginal pass
+ * (get_branch_mutation_matrix, treeanc.py:1085-1120; per-pattern P(t) under
+ * a site-specific model); marginal == 0: state changes of the last
+ * reconstruction, tips without reconstructed states through code_state[256]
+ * (alphabet index of each character, -1 = skipped).  Multiplicity-weighted.
+ * Alphabets of up to 5 states.  n_out and t_out are host arrays. */
+int32_t tt_site_counts(tt_tree *tree, int32_t marginal, const int16_t *code_state, double *n_out, double *t_out,
+                       void *stream);
+
 /* ---- batching --------------------------------------------------------------
  * Marginal reconstruction of k resident trees in ONE persistent launch
  * driven by a device work queue of (tree, pattern-tile) items; each item

@@ -354,6 +354,41 @@ def gtr_infer(alphabet, nij, Ti, root_state, fixed_pi=None, pc=1.0, gap_limit=0.
     return GTRModel(alphabet, mu=mu, pi=pi, W=W)
 
 
+def site_gtr_infer(alphabet, sub_ija, T_ia, root_state, pc=1.0, gap_limit=0.01, Nit=30, dp=1e-5):
+    """treetime/gtr_site_specific.py:208-310 (GTR_site_specific.infer):
+    one W, per-site pi and mu; returns a SiteGTRModel (approximate=True)."""
+    q = len(ALPHABETS[alphabet])
+    L = sub_ija.shape[-1]
+    n_ija = np.array(sub_ija, dtype=float)
+    for i in range(q):
+        n_ija[i, i, :] = 0.0
+    n_ij = n_ija.sum(axis=-1)
+    m_ia = n_ija.sum(axis=1) + root_state + pc
+    n_a = n_ija.sum(axis=(0, 1)) + pc
+    Lam = root_state.sum(axis=0) + q * pc
+    p_old, p = np.zeros((q, L)), np.ones((q, L)) / q
+    mu = np.ones(L)
+    W = np.ones((q, q)) - np.eye(q)
+    it = 0
+    while np.sqrt(((p_old - p) ** 2).sum()) > dp and it < Nit:
+        it += 1
+        p_old = p.copy()
+        S = (mu[None, None, :] * p[:, None, :] * T_ia[None, :, :]).sum(axis=-1)
+        W = (n_ij + n_ij.T + pc) / (S + S.T + pc)
+        avg_pi = p.mean(axis=-1)
+        r = avg_pi @ W @ avg_pi
+        W = W / r
+        mu = mu * r
+        p = m_ia / (mu * (W @ T_ia) + Lam)
+        p = p / p.sum(axis=0)
+        mu = n_a / (pc + (p * (W @ T_ia)).sum(axis=0))
+    gap = list(ALPHABETS[alphabet]).index("-") if "-" in ALPHABETS[alphabet] else None
+    if gap is not None:
+        p[gap, :] = gap_limit          # every site, as the reference does
+        p = p / p.sum(axis=0)
+    return SiteGTRModel(alphabet, mu=mu, pi=p, W=W)
+
+
 def gtr_from_text(text):
     """Parse the reference's GTR text format (treetime/gtr.py:174-233)."""
     lines = text.splitlines()
@@ -702,7 +737,7 @@ class OracleTreeAnc:
         pattern, after a marginal pass."""
         pp, pc = node.marginal_outgroup_LH, node.marginal_subtree_LH
         E = self.gtr.expQt(self.branch_t(node)) + SUPERTINY_NUMBER
-        M = np.einsum("ai,aj,ij->aij", pc, pp, E)
+        M = np.einsum("ai,aj,ija->aij" if E.ndim == 3 else "ai,aj,ij->aij", pc, pp, E)
         return M / M.sum(axis=2).sum(axis=1)[:, None, None]
 
     def mutation_counts_marginal(self):
@@ -720,3 +755,44 @@ class OracleTreeAnc:
             Ti += 0.5 * t * (M.sum(axis=2) * w).sum(axis=0)
             nij += (M * self.mult[:, None, None]).sum(axis=0)
         return nij, Ti
+
+    def site_counts_marginal(self):
+        """treetime/treeanc.py:1556-1572 per pattern (site_specific=True):
+        n_ija (n, n, P) and T_ia (n, P) summed over branches."""
+        n = self.gtr.n_states
+        n_ija, T_ia = np.zeros((n, n, self.L)), np.zeros((n, self.L))
+        for node in preorder(self.root):
+            if node.up is None:
+                continue
+            st = np.transpose(self.branch_mutation_matrix(node), (1, 2, 0))   # (i, j, a)
+            t = self.branch_t(node)
+            T_ia += 0.5 * t * st.sum(axis=0) * self.mult
+            T_ia += 0.5 * t * st.sum(axis=1) * self.mult
+            n_ija += st * self.mult
+        return n_ija, T_ia
+
+    def site_counts_joint(self):
+        """treetime/treeanc.py:1573-1589 per pattern after a reconstruction
+        (node.seq_idx; tips without one use their characters, characters
+        outside the alphabet are skipped)."""
+        n = self.gtr.n_states
+        index = {c: i for i, c in enumerate(self.gtr.alphabet)}
+        n_ija, T_ia = np.zeros((n, n, self.L)), np.zeros((n, self.L))
+        cols = np.arange(self.L)
+        for node in preorder(self.root):
+            if node.up is None:
+                continue
+            if hasattr(node, "seq_idx"):
+                cs = np.asarray(node.seq_idx, dtype=int)
+            else:
+                chars = self.aln.get(node.name, np.array([""] * self.L))
+                cs = np.array([index.get(c, -1) for c in chars])
+            ps = np.asarray(node.up.seq_idx, dtype=int)
+            t = self.branch_t(node)
+            mut = (cs >= 0) & (cs != ps)
+            np.add.at(n_ija, (cs[mut], ps[mut], cols[mut]), self.mult[mut])
+            np.add.at(T_ia, (ps[mut], cols[mut]), 0.5 * t * self.mult[mut])
+            np.add.at(T_ia, (cs[mut], cols[mut]), -0.5 * t * self.mult[mut])
+            ok = cs >= 0
+            np.add.at(T_ia, (cs[ok], cols[ok]), t * self.mult[ok])
+        return n_ija, T_ia

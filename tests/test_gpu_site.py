@@ -167,3 +167,97 @@ def test_site_facade_treeanc(gpu_engine):
     ta.infer_ancestral_sequences(marginal=False)
     assert abs(ta.tree.sequence_joint_LH - orc.joint()) < 1e-9 * abs(ref)
     ta._drop_device()
+
+
+@pytest.mark.parametrize("site_model", [True, False])
+def test_site_counts_match_oracle(gpu_engine, site_model):
+    """tt_site_counts: per-pattern statistics summed over branches, marginal
+    (pair posteriors; the pattern's own P(t) under a site model, the branch's
+    P(t) otherwise) and joint (state changes), against the oracle."""
+    say(f"[site] counts site_model={site_model}")
+    prep, g, ta, nodes = site_case(70, 300, 9, True)
+    if site_model:
+        dev = device(gpu_engine, prep, g)
+    else:
+        std = synth.make_job(70, 300, 0.05, 4, 9, 0.01, 0.005)["gtr"]
+        dev = DeviceTree(gpu_engine, prep["parent"], prep["t"], prep["tip_codes"], code_lut(std.profile_map, 4),
+                         prep["multiplicity"], std.Pi, std.eigenvals, std.v, std.v_inv, std.mu)
+        ta = O.OracleTreeAnc(ta.root, O.GTRModel("nuc_nogap", mu=std.mu, pi=std.Pi, W=std.W), ta.aln,
+                             prep["multiplicity"], full_length=1e300)
+    dev.marginal()
+    ta.marginal()
+    n_d, t_d = dev.site_counts(True)
+    n_o, t_o = ta.site_counts_marginal()
+    en = np.max(np.abs(np.transpose(n_d, (1, 2, 0)) - n_o))
+    et = np.max(np.abs(t_d.T - t_o) / np.maximum(1e-300, np.abs(t_o).max()))
+    say(f"  marginal: n_ija max abs diff {en:.2e}, T_ia max rel diff {et:.2e}")
+    assert en < 2e-6 and et < 1e-6
+    dev.joint()
+    ta.joint()
+    cs = np.full(256, -1, dtype=np.int16)
+    for i, c in enumerate("ACGT"):
+        cs[ord(c)] = i
+    n_d, t_d = dev.site_counts(False, cs)
+    states = dev.states()
+    sure = np.ones(prep["tip_codes"].shape[1], dtype=bool)
+    for i, nd in enumerate(nodes):
+        if not nd.is_terminal():
+            sure &= nd.path_margin > 1e-5
+    n_o, t_o = ta.site_counts_joint()
+    assert np.array_equal(np.transpose(n_d, (1, 2, 0))[:, :, sure], n_o[:, :, sure])
+    assert np.allclose(t_d.T[:, sure], t_o[:, sure], rtol=1e-12, atol=1e-12)
+    assert states is not None
+    dev.close()
+
+
+@pytest.mark.parametrize("marginal", [True, False])
+def test_site_facade_infer_gtr(gpu_engine, marginal):
+    """TreeAnc.infer_gtr(site_specific=True) (treeanc.py:1500-1632) equals
+    GTR_site_specific.infer on the oracle's statistics."""
+    say(f"[site] facade infer_gtr site_specific marginal={marginal}")
+    job = synth.make_job(60, 250, 0.08, 4, 10, 0.0, 0.0)
+    parent, bl = job["parent"], job["branch_length"]
+    clades = [tt.Clade(branch_length=float(bl[i])) for i in range(len(parent))]
+    for i, p in enumerate(parent):
+        if p >= 0:
+            clades[p].clades.append(clades[i])
+    for k, v in enumerate(job["tips"]):
+        clades[v].name = job["names"][k]
+    aln = {job["names"][k]: job["chars"][k].tobytes().decode() for k in range(len(job["tips"]))}
+    ta = tt.TreeAnc(tree=tt.Tree(clades[0]), aln=aln, gtr=job["gtr"], compress=False, verbose=0, engine=gpu_engine)
+    with pytest.raises(TypeError):
+        tt.TreeAnc(tree=tt.Tree(clades[0]), aln=aln, gtr=job["gtr"], verbose=0,
+                   engine=gpu_engine).infer_gtr(site_specific=True)
+    ta.infer_ancestral_sequences(marginal=marginal)
+    nodes = list(ta.tree.find_clades(order="preorder"))
+    idx = {id(x): i for i, x in enumerate(nodes)}
+    par = [-1] * len(nodes)
+    for i, x in enumerate(nodes):
+        for c in x.clades:
+            par[idx[id(c)]] = i
+    root, onodes = O.build_tree(par, [ta._branch_length_to_gtr(x) for x in nodes], [x.name for x in nodes])
+    g0 = ta.gtr
+    orc = O.OracleTreeAnc(root, O.GTRModel("nuc_nogap", mu=g0.mu, pi=g0.Pi, W=g0.W), ta.data.compressed_alignment,
+                          ta.data.multiplicity(), full_length=1e300)
+    if marginal:
+        orc.marginal()
+        n_o, t_o = orc.site_counts_marginal()
+        root_state = orc.root.marginal_profile.T
+    else:
+        orc.joint()
+        n_o, t_o = orc.site_counts_joint()
+        root_state = np.eye(4)[:, orc.root.seq_idx]
+    g = ta.infer_gtr(marginal=marginal, site_specific=True, pc=1.0)
+    assert isinstance(g, tt.GTR_site_specific)
+    ref = O.site_gtr_infer("nuc_nogap", n_o, t_o, root_state, pc=1.0)
+    ref_mu = ref.mu / ref.average_rate().mean()
+    dpi = np.max(np.abs(g.Pi - ref.Pi))
+    dw = np.max(np.abs(g.W - ref.W)) / ref.W.max()
+    say(f"  pi max abs diff {dpi:.2e}, W max rel diff {dw:.2e}")
+    tol = 1e-5 if marginal else 1e-10
+    assert dpi < tol and dw < tol
+    assert np.allclose(g.mu, ref_mu, rtol=tol)
+    # the inferred site-specific model drives the next reconstruction
+    ta.infer_ancestral_sequences(marginal=True)
+    assert np.isfinite(ta.sequence_LH())
+    ta._drop_device()

@@ -493,3 +493,74 @@ def test_site_joint_is_bruteforce_max():
                  + logP["A"][idx[comp["A"][a]], sc] + logP["B"][idx[comp["B"][a]], sc])
             best = max(best, v)
         assert abs(best - ta.joint_sequence_LH[a]) < 1e-10
+
+
+def test_site_counts_marginal_reduce_to_standard():
+    """Per-site statistics at site a equal the standard model's statistics
+    (pinned to pair-posterior enumeration) for the single column a under a
+    standard model with site a's parameters."""
+    root, nodes = tiny_tree([("R", None, 0.01), ("X", "R", 0.1), ("A", "X", 0.2), ("B", "X", 0.05),
+                             ("C", "R", 0.3), ("D", "R", 0.15)])
+    L = 6
+    m = rand_site_model(L, 21, approximate=False)
+    rng = np.random.default_rng(22)
+    comp = {k: rng.choice(list("ACGTN"), size=L) for k in "ABCD"}
+    ta = O.OracleTreeAnc(root, m, comp, np.ones(L), full_length=1e9)
+    ta.marginal()
+    n_ija, T_ia = ta.site_counts_marginal()
+    for a in range(L):
+        std = O.GTRModel("nuc_nogap", mu=m.mu[a], pi=m.Pi[:, a], W=m.W)
+        one = O.OracleTreeAnc(root, std, {k: v[a:a + 1] for k, v in comp.items()}, np.ones(1), full_length=1e9)
+        one.marginal()
+        nij, Ti = one.mutation_counts_marginal()
+        assert np.allclose(n_ija[:, :, a], nij, atol=1e-12)
+        assert np.allclose(T_ia[:, a], Ti, atol=1e-12)
+
+
+def test_site_counts_joint_invariants():
+    """Joint per-site statistics: every branch puts its length into the
+    child's state (mutations move half of it to the parent's state), so
+    sum_i T_ia is the total branch length where every state is known, and
+    sum_ij n_ija counts the branches whose two ends differ."""
+    root, nodes = tiny_tree([("R", None, 0.01), ("X", "R", 0.1), ("A", "X", 0.2), ("B", "X", 0.05),
+                             ("C", "R", 0.3), ("D", "R", 0.15)])
+    L = 30
+    m = rand_site_model(L, 23)
+    rng = np.random.default_rng(24)
+    comp = {k: rng.choice(list("ACGT"), size=L) for k in "ABCD"}
+    ta = O.OracleTreeAnc(root, m, comp, np.ones(L), full_length=1e9)
+    ta.joint()
+    n_ija, T_ia = ta.site_counts_joint()
+    total_t = sum(ta.branch_t(nd) for nd in nodes if nd.up is not None)
+    assert np.allclose(T_ia.sum(axis=0), total_t, atol=1e-12)
+    idx = {c: i for i, c in enumerate("ACGT")}
+    byname = {nd.name: nd for nd in nodes}
+    state = {k: np.array([idx[c] for c in v]) for k, v in comp.items()}
+    state.update({k: byname[k].seq_idx for k in "RX"})
+    diff = sum((state[nd.name] != state[nd.up.name]).astype(float) for nd in nodes if nd.up is not None)
+    assert np.allclose(n_ija.sum(axis=(0, 1)), diff)
+    assert np.all(n_ija[range(4), range(4), :] == 0)
+
+
+def test_site_gtr_infer_recovers_model():
+    """Expected statistics of a known site-specific model, n_ija = mu_a
+    pi_ia W_ij T_ja with T_ja = pi_ja T, give back pi, W (up to scale) and
+    mu; the facade's GTR_site_specific.infer equals the oracle's."""
+    import treetime_amd as tt
+    rng = np.random.default_rng(25)
+    L = 40
+    W = np.array([[0, 1, 2, 1], [1, 0, 1, 3], [2, 1, 0, 1], [1, 3, 1, 0.]])
+    true = O.SiteGTRModel("nuc_nogap", mu=rng.gamma(3, size=L) / 3, pi=rng.gamma(2, size=(4, L)), W=W,
+                          approximate=False)
+    T_ia = true.Pi * 1e4
+    n_ija = true.mu[None, None, :] * true.Pi[:, None, :] * true.W[:, :, None] * T_ia[None, :, :]
+    for i in range(4):
+        n_ija[i, i, :] = 0
+    root = np.zeros((4, L))
+    m = O.site_gtr_infer("nuc_nogap", n_ija, T_ia, root, pc=1e-3)
+    assert np.max(np.abs(m.Pi - true.Pi)) < 1e-5
+    assert np.max(np.abs(m.W / m.W[0, 1] - true.W / true.W[0, 1])) < 1e-4
+    assert np.max(np.abs(m.mu / true.mu - 1)) < 1e-4
+    g = tt.GTR_site_specific.infer(n_ija, T_ia, root, pc=1e-3, alphabet="nuc_nogap")
+    assert np.allclose(g.Pi, m.Pi, atol=1e-14) and np.allclose(g.W, m.W, atol=1e-13)
+    assert np.allclose(g.mu, m.mu, rtol=1e-13)

@@ -1227,6 +1227,33 @@ int32_t tt_mutation_counts(tt_tree *T, double *nij, double *Ti, double *per_bran
     return TT_OK;
 }
 
+int32_t tt_site_counts(tt_tree *T, int32_t marginal, const int16_t *code_state, double *n_out, double *t_out,
+                       void *stream) {
+    if (!T || !n_out || !t_out || (!marginal && !code_state)) return fail(TT_EINVAL, "NULL argument");
+    if (T->n > 5) return fail(TT_EINVAL, "site statistics are built for alphabets of up to 5 states (n = %d)", T->n);
+    if (marginal && !T->profiles_fresh)
+        return fail(TT_ESTATE, "marginal ancestral inference needs to be performed first (profiles are stale)");
+    if (!marginal && !T->have_states)
+        return fail(TT_ESTATE, "no marginal or joint reconstruction has run on this tree");
+    TT_HIP(hipSetDevice(T->eng->device));
+    hipStream_t s = stream ? (hipStream_t)stream : T->stream;
+    int32_t rc;
+    if (!marginal) {
+        if (!T->d_code_state && (rc = T->alloc((void **)&T->d_code_state, 256 * sizeof(int16_t)))) return rc;
+        TT_HIP(hipMemcpyAsync(T->d_code_state, code_state, 256 * sizeof(int16_t), hipMemcpyHostToDevice, s));
+    }
+    const size_t nn = (size_t)T->P * T->n * T->n, n1 = (size_t)T->P * T->n;
+    double *d_out = nullptr;
+    if ((rc = T->alloc((void **)&d_out, (nn + n1) * sizeof(double)))) return rc;
+    hipError_t e = ttk::site_counts(T->d, T->d_t, marginal ? 0 : 1, T->d_code_state, d_out, d_out + nn, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(n_out, d_out, nn * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(t_out, d_out + nn, n1 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    T->release(d_out);
+    if (e != hipSuccess) return fail(TT_EDEVICE, "tt_site_counts: %s", hipGetErrorString(e));
+    return TT_OK;
+}
+
 int32_t tt_tree_trim(tt_tree *T) {
     if (!T) return fail(TT_EINVAL, "tree is NULL");
     if (!T->joint_ready) return TT_OK;

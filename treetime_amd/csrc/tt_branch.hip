@@ -254,10 +254,11 @@ __device__ inline double block_total(double x, double *s_red) {
     return t;
 }
 
-// Branch profiles of pattern p: pc (child side) and pp (parent side), FP64.
-template <int N>
-__device__ inline void branch_profiles(const TreeDev &T, int node, int pair_other, int p, double (&pc)[N],
-                                       double (&pp)[N]) {
+// Branch profiles of pattern p: pc (child side) and pp (parent side), FP64;
+// M is the branch's P(t) (row i = child state), indexed M[i * N + j].
+template <int N, class MP>
+__device__ inline void branch_profiles_m(const TreeDev &T, int node, int pair_other, int p, MP M, double (&pc)[N],
+                                         double (&pp)[N]) {
     const int ir = T.int_row[node];
     if (ir >= 0) {
         const float TT_G *r = T.sub + ((size_t)ir * T.P + p) * N;
@@ -288,7 +289,6 @@ __device__ inline void branch_profiles(const TreeDev &T, int node, int pair_othe
         }
     } else {  // outgroup: max(TINY, parent profile) / (child message at the current length)
         const float TT_G *pr = T.prof + ((size_t)T.parent[node] * T.P + p) * N;
-        const double TT_C *M = T.P64 + (size_t)node * (N * N);
         double r[N];
         if (ir >= 0) {
             const float TT_G *sr = T.sub + ((size_t)ir * T.P + p) * N;
@@ -303,13 +303,127 @@ __device__ inline void branch_profiles(const TreeDev &T, int node, int pair_othe
             double m = 0.0;
 #pragma unroll
             for (int i = 0; i < N; ++i) m = fma(r[i], M[i * N + j], m);
-            pp[j] = fmax(TT_TINY, (double)pr[j]) / m;
+            pp[j] = fmax(TT_TINY, (double)pr[j]) / fmax(TT_TINY, m);
         }
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) z += pp[j];
 #pragma unroll
     for (int j = 0; j < N; ++j) pp[j] /= z;
+}
+
+template <int N>
+__device__ inline void branch_profiles(const TreeDev &T, int node, int pair_other, int p, double (&pc)[N],
+                                       double (&pp)[N]) {
+    branch_profiles_m<N>(T, node, pair_other, p, T.P64 + (size_t)node * (N * N), pc, pp);
+}
+
+// P(t) of branch c for pattern p under a site-specific model, FP64, without
+// clipping (gtr_site_specific.py:350-371): I + V diag(delta) V^-1 with delta
+// interpolated between the branch's grid points.
+template <int N>
+__device__ inline void site_P64(const TreeDev &T, int c, int p, double (&E)[N * N]) {
+    const auto b = T.s_br + 3 * (size_t)c;
+    const double ta = b[0], tb = b[1], w = b[2];
+    const auto rho = T.s_rho64 + (size_t)p * N;
+    const auto V = T.s_V64 + (size_t)p * (N * N);
+    const auto Vi = T.s_Vi64 + (size_t)p * (N * N);
+    double d[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        d[k] = expm1(ta * rho[k]);
+        if (w != 0.0) d[k] += w * (expm1(tb * rho[k]) - d[k]);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            double a = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) a = fma(V[i * N + k] * d[k], Vi[k * N + j], a);
+            E[i * N + j] = a;
+        }
+}
+
+// Per-pattern statistics for GTR_site_specific.infer (treeanc.py:1556-1589
+// with site_specific=True), summed over the branches in node order for each
+// pattern p (one thread per pattern):
+//   marginal (mode 0): M = normalised pc_i pp_j (P_ij + 1e-24) of the branch
+//     (get_branch_mutation_matrix, treeanc.py:1085-1120; the pattern's own P
+//     under a site-specific model), n[p][i][j] += mult M_ij,
+//     T[p][i] += t/2 mult (sum_j M_ij + sum_j M_ji);
+//   joint (mode 1): child state i and parent state j of the reconstruction
+//     (tips without reconstructed states: code_state of their character, -1
+//     = not an alphabet state), T[p][i] += t mult; for i != j both valid:
+//     n[p][i][j] += mult, T[p][j] += t/2 mult, T[p][i] -= t/2 mult.
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_site_counts(TreeDev T, const double *t_eff, int mode,
+                                                         const int16_t *code_state, double *n_out, double *t_out) {
+    constexpr int NN = N * N;
+    const int p = blockIdx.x * TT_BLOCK + threadIdx.x;
+    if (p >= T.P) return;
+    double n_acc[NN], t_acc[N];
+#pragma unroll
+    for (int k = 0; k < NN; ++k) n_acc[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) t_acc[k] = 0.0;
+    const double mult = T.mult[p];
+    for (int c = 0; c < T.n_nodes; ++c) {
+        const int par = T.parent[c];
+        if (par < 0) continue;
+        const double tc = t_eff[c];
+        if (mode == 0) {
+            double E[NN], pc[N], pp[N];
+            if (T.s_V64) {
+                site_P64<N>(T, c, p, E);
+            } else {
+#pragma unroll
+                for (int k = 0; k < NN; ++k) E[k] = T.P64[(size_t)c * NN + k];
+            }
+            branch_profiles_m<N>(T, c, -1, p, E, pc, pp);
+            double m[NN], z = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    m[i * N + j] = pc[i] * pp[j] * (E[i * N + j] + kSuperTiny);
+                    z += m[i * N + j];
+                }
+            const double w = mult / z;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double row = 0.0, col = 0.0;
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    n_acc[i * N + j] = fma(m[i * N + j], w, n_acc[i * N + j]);
+                    row += m[i * N + j];
+                    col += m[j * N + i];
+                }
+                t_acc[i] = fma(0.5 * tc * w, row + col, t_acc[i]);
+            }
+        } else {
+            int cs = T.states[(size_t)c * T.P + p];
+            const int trow = T.tip_row[c];
+            if (cs == 255 && trow >= 0) cs = code_state[T.tip_codes[(size_t)trow * T.P + p]];
+            if (cs == 255) cs = -1;
+            const int ps = T.states[(size_t)par * T.P + p];
+            const bool mut = cs >= 0 && ps < N && cs != ps;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double dt = 0.0;
+                if (i == cs) dt += tc * mult - (mut ? 0.5 * tc * mult : 0.0);
+                if (mut && i == ps) dt += 0.5 * tc * mult;
+                t_acc[i] += dt;
+#pragma unroll
+                for (int j = 0; j < N; ++j)
+                    if (mut && i == cs && j == ps) n_acc[i * N + j] += mult;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NN; ++k) n_out[(size_t)p * NN + k] = n_acc[k];
+#pragma unroll
+    for (int k = 0; k < N; ++k) t_out[(size_t)p * N + k] = t_acc[k];
 }
 
 // P(t) = v diag(exp(mu t lambda)) v_inv clipped at 0, into LDS (gtr.py:1027-1067).
@@ -508,6 +622,19 @@ hipError_t mutation_counts(const TreeDev &T, const double *t_eff, double *out, h
         case 3: hipLaunchKernelGGL(k_mutation_counts<3>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
         case 4: hipLaunchKernelGGL(k_mutation_counts<4>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
         case 5: hipLaunchKernelGGL(k_mutation_counts<5>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t site_counts(const TreeDev &T, const double *t_eff, int mode, const int16_t *code_state, double *n_out,
+                       double *t_out, hipStream_t s) {
+    const unsigned blocks = (unsigned)((T.P + TT_BLOCK - 1) / TT_BLOCK);
+    switch (T.n) {
+        case 2: hipLaunchKernelGGL(k_site_counts<2>, dim3(blocks), dim3(TT_BLOCK), 0, s, T, t_eff, mode, code_state, n_out, t_out); break;
+        case 3: hipLaunchKernelGGL(k_site_counts<3>, dim3(blocks), dim3(TT_BLOCK), 0, s, T, t_eff, mode, code_state, n_out, t_out); break;
+        case 4: hipLaunchKernelGGL(k_site_counts<4>, dim3(blocks), dim3(TT_BLOCK), 0, s, T, t_eff, mode, code_state, n_out, t_out); break;
+        case 5: hipLaunchKernelGGL(k_site_counts<5>, dim3(blocks), dim3(TT_BLOCK), 0, s, T, t_eff, mode, code_state, n_out, t_out); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -146,6 +146,9 @@ hipError_t branch_opt_marginal(const TreeDev &T, int gap, const double *eig, con
                                int32_t *status, hipStream_t s);
 // GTR inference counts per branch, out [n_nodes][n*n + n] (n_ij then T_i), n <= 5
 hipError_t mutation_counts(const TreeDev &T, const double *t_eff, double *out, hipStream_t s);
+// per-pattern statistics summed over branches (mode 0 marginal, 1 joint), n <= 5
+hipError_t site_counts(const TreeDev &T, const double *t_eff, int mode, const int16_t *code_state, double *n_out,
+                       double *t_out, hipStream_t s);
 // prob_t_profiles of every branch at K lengths, out [n_nodes][K]
 hipError_t branch_profile_lh(const TreeDev &T, int gap, const double *eig, const double *v, const double *vinv,
                              double mu, const double *t, int K, double *out, hipStream_t s);

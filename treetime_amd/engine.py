@@ -263,6 +263,18 @@ class DeviceTree:
                                                   _native.ptr(pb, C.c_double) if per_branch else None, stream))
         return (nij, Ti, pb) if per_branch else (nij, Ti)
 
+    def site_counts(self, marginal, code_state=None, stream=None):
+        """Per-pattern statistics summed over branches (tt_site_counts):
+        (n_ija (P, n, n), T_ia (P, n))."""
+        n_out = np.empty((self.P, self.n, self.n))
+        t_out = np.empty((self.P, self.n))
+        cs = _native.as_c(code_state, np.int16) if code_state is not None else None
+        _native.check(self.lib.tt_site_counts(self.handle, int(bool(marginal)),
+                                              _native.ptr(cs, C.c_int16) if cs is not None else None,
+                                              _native.ptr(n_out, C.c_double), _native.ptr(t_out, C.c_double),
+                                              stream))
+        return n_out, t_out
+
     def trim(self):
         """Free the joint-pass buffers (returned to the engine budget)."""
         _native.check(self.lib.tt_tree_trim(self.handle))

@@ -146,8 +146,51 @@ class GTR_site_specific(GTR):
         return gtr
 
     @classmethod
-    def infer(cls, *args, **kwargs):
-        raise NotImplementedError("site-specific GTR inference is not part of the GPU pruning engine")
+    def infer(cls, sub_ija, T_ia, root_state, pc=1.0, gap_limit=0.01, Nit=30, dp=1e-5, **kwargs):
+        """Fixed point of n_ija + pc = pi_ia W_ij (T_ja + pc + root_state)
+        with one W and per-site pi and mu (gtr_site_specific.py:208-310):
+        pseudocounts off the diagonal, W symmetrised from the summed counts,
+        pi_ia and mu_a updated per site, stop when |p - p_old| <= dp or after
+        Nit rounds; with a gap state its frequency is set to gap_limit at
+        every site.  Host FP64 (n x n x L arrays)."""
+        gtr = cls(**kwargs)
+        gtr.logger("GTR: model inference ", 1)
+        q = len(gtr.alphabet)
+        L = sub_ija.shape[-1]
+        n_iter = 0
+        n_ija = np.copy(sub_ija)
+        n_ija[range(q), range(q), :] = 0
+        n_ij = n_ija.sum(axis=-1)
+        m_ia = np.sum(n_ija, axis=1) + root_state + pc
+        n_a = n_ija.sum(axis=1).sum(axis=0) + pc
+        Lambda = np.sum(root_state, axis=0) + q * pc
+        p_ia_old = np.zeros((q, L))
+        p_ia = np.ones((q, L)) / q
+        mu_a = np.ones(L)
+        W_ij = np.ones((q, q)) - np.eye(q)
+        while np.linalg.norm(p_ia_old - p_ia) > dp and n_iter < Nit:
+            n_iter += 1
+            p_ia_old = np.copy(p_ia)
+            S_ij = np.einsum("a,ia,ja", mu_a, p_ia, T_ia)
+            W_ij = (n_ij + n_ij.T + pc) / (S_ij + S_ij.T + pc)
+            avg_pi = p_ia.mean(axis=-1)
+            average_rate = W_ij.dot(avg_pi).dot(avg_pi)
+            W_ij = W_ij / average_rate
+            mu_a *= average_rate
+            p_ia = m_ia / (mu_a * np.dot(W_ij, T_ia) + Lambda)
+            p_ia = p_ia / p_ia.sum(axis=0)
+            mu_a = n_a / (pc + np.einsum("ia,ij,ja->a", p_ia, W_ij, T_ia))
+        if n_iter >= Nit:
+            gtr.logger("WARNING: maximum number of iterations has been reached in GTR inference", 3, warn=True)
+        if gtr.gap_index is not None:
+            for p in range(p_ia.shape[-1]):
+                if p_ia[gtr.gap_index, p] < gap_limit:
+                    gtr.logger("The model allows for gaps which are estimated to occur at a low fraction of %1.3e"
+                               % p_ia[gtr.gap_index, p], 4, warn=True)
+                p_ia[gtr.gap_index, p] = gap_limit
+                p_ia[:, p] /= p_ia[:, p].sum()
+        gtr.assign_rates(mu=mu_a, W=W_ij, pi=p_ia)
+        return gtr
 
     # --- device --------------------------------------------------------------
     def device_model(self):

@@ -27,6 +27,7 @@ from . import config as ttconf
 from .engine import MISSING_CODE, DeviceTree, code_lut, default_engine
 from .errors import MissingDataError, UnknownMethodError
 from .gtr import GTR
+from .gtr_site_specific import GTR_site_specific
 from .seq_utils import extend_profile, seq2prof
 from .sequence_data import SequenceData
 from .tree import Clade, Tree
@@ -520,9 +521,11 @@ class TreeAnc:
         branch at once), then GTR.infer on the host.  Marginal: expected
         counts from each branch's profile pair (tt_mutation_counts);
         joint: the (parent, child) state counts of every branch
-        (tt_branch_state_counts)."""
-        if site_specific:
-            raise NotImplementedError("site-specific GTR models are not part of the GPU pruning engine")
+        (tt_branch_state_counts).  site_specific: per-site statistics summed
+        over the branches (tt_site_counts), then GTR_site_specific.infer."""
+        if site_specific and self.data.compress:
+            raise TypeError("TreeAnc.infer_gtr(): sequence compression and site specific GTR models are "
+                            "incompatible!")
         if not self.ok:
             raise MissingDataError("TreeAnc.infer_gtr: ERROR, sequences or tree are missing")
         if marginal and self.sequence_reconstruction != "marginal":
@@ -534,6 +537,20 @@ class TreeAnc:
         dev = self._device
         mult = self.data.multiplicity()
         self.logger("TreeAnc.infer_gtr: counting mutations...", 2)
+        if site_specific:
+            root_idx = self._nodes.index(self.tree.root)
+            if marginal:
+                n_pij, t_pi = dev.site_counts(True)
+                root_state = dev.profiles()[root_idx].astype(float).T
+            else:
+                n_pij, t_pi = dev.site_counts(False, self._code_states(skip_unknown=True))
+                root_state = np.eye(n)[:, dev.states()[root_idx]]
+            self._gtr = GTR_site_specific.infer(np.transpose(n_pij, (1, 2, 0)), t_pi.T, pc=pc, root_state=root_state,
+                                                logger=self.logger, alphabet=g.alphabet, prof_map=g.profile_map)
+            if normalized_rate:
+                self.logger("TreeAnc.infer_gtr: setting overall rate to 1.0...", 2)
+                self._gtr.mu /= self._gtr.average_rate().mean()
+            return self._gtr
         if marginal:
             nij, Ti = dev.mutation_counts()
         else:
