@@ -173,3 +173,45 @@ def test_gtr_infer_matches_oracle():
         assert np.allclose(g.W, o.W, rtol=1e-12, atol=0)
         assert np.allclose(g.Pi, o.Pi, rtol=1e-12, atol=0)
         assert abs(g.mu - o.mu) <= 1e-12 * o.mu
+
+
+def test_gtr_site_specific_host_model_equals_oracle():
+    """GTR_site_specific set-up (gtr_site_specific.py:43-116, 312-371) on
+    the host: P(t) exact and interpolated equal the oracle's SiteGTRModel,
+    the rate matrices have zero column sums, device_model's per-site layout
+    rebuilds P(t)."""
+    rng = np.random.default_rng(31)
+    g = tt.GTR_site_specific.random(L=12, alphabet="nuc", rng=rng)
+    o = O.SiteGTRModel("nuc", mu=g.mu, pi=g.Pi, W=g.W)
+    for t in (0.0, 0.003, 0.05, 1.3, 40.0):
+        assert np.allclose(g.expQt(t), o.expQt(t), atol=1e-14)
+    assert np.abs(g.Q.sum(axis=0)).max() < 1e-14
+    assert np.allclose(g.average_rate().mean(), 1.0)       # random() scales mu to avg_mu = 1
+    eig, V, Vi, mu, pi, grid, rs = g.device_model()
+    t = 0.21
+    P = np.einsum("aik,ak,akj->aij", V, np.exp(t * mu[:, None] * eig), Vi)
+    assert np.allclose(np.moveaxis(P, 0, -1), g._expQt(t), atol=1e-14)
+    assert np.allclose(pi.T, g.Pi) and len(grid) == 61 and abs(grid[-1] * rs - 10) < 1e-12
+    g.approximate = False
+    assert len(g.device_model()[5]) == 0
+
+
+def test_gtr_site_specific_argument_errors():
+    g = tt.GTR_site_specific(seq_len=3, alphabet="nuc_nogap")
+    with pytest.raises(ValueError):
+        g.assign_rates(mu=np.ones(4), pi=np.ones((4, 3)))          # rate vector length != sites
+    with pytest.raises(ValueError):
+        g.assign_rates(pi=np.ones(5))                                # frequencies of the wrong length
+    with pytest.raises(ValueError):
+        g.assign_rates(W=np.ones((3, 3)))                            # W of the wrong size
+    g.assign_rates(mu=2.0, pi=np.array([1.0, 2, 3, 4]))             # one pi for all sites
+    assert g.Pi.shape == (4, 3) and np.allclose(g.Pi[:, 0], [0.1, 0.2, 0.3, 0.4])
+    c = tt.GTR_site_specific.custom(mu=np.ones(5), pi=np.ones((4, 5)), alphabet="nuc_nogap")
+    assert c.seq_len == 5 and c.is_site_specific
+
+
+def test_site_treeanc_requires_uncompressed_alignment():
+    g = tt.GTR_site_specific.random(L=3, alphabet="nuc_nogap", rng=np.random.default_rng(1))
+    with pytest.raises(TypeError):
+        tt.TreeAnc(tree=tt.Tree.from_newick("((A:0.1,B:0.2):0.1,C:0.3);"),
+                   aln={"A": "ACG", "B": "ACT", "C": "ACG"}, gtr=g, verbose=0)
