@@ -97,9 +97,11 @@ int32_t tt_tree_set_model(tt_tree *tree, const double *eigvals, const double *v,
  * the grid points around t (the approximate=True interpolator,
  * gtr_site_specific.py:331-371); other branches, or G = 0, the exact P(t).
  * Replaces tt_tree_set_model until that is called again; tt_marginal,
- * tt_joint and tt_loglik then run per-pattern matrices.  Alphabets of up to
- * 5 states.  tt_batch_marginal, tt_update_branch, the branch-length sweeps
- * and tt_mutation_counts refuse such trees (TT_EINVAL). */
+ * tt_joint, tt_loglik, tt_optimize_branch_lengths_marginal,
+ * tt_branch_profile_lh and tt_site_counts then use per-pattern matrices.
+ * Alphabets of up to 5 states.  tt_batch_marginal, tt_update_branch, the
+ * joint sweep tt_optimize_branch_lengths and tt_mutation_counts refuse such
+ * trees (TT_EINVAL). */
 int32_t tt_tree_set_site_model(tt_tree *tree, const double *eigvals, const double *v, const double *v_inv,
                                const double *mu, const double *pi, const double *t_grid, int32_t G,
                                double rate_scale);

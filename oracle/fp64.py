@@ -439,6 +439,8 @@ class SiteGTRModel:
         self.profile_map = profile_map(alphabet)
         self.ambiguous = ambiguous_char(alphabet)
         self.n_states = n = len(self.alphabet)
+        chars = list(self.alphabet)
+        self.gap_index = chars.index("-") if "-" in chars else None
         self.approximate = approximate
         pi = np.array(pi, dtype=float)
         L = pi.shape[1]
@@ -488,6 +490,22 @@ class SiteGTRModel:
         """gtr_site_specific.py:408-437."""
         res = np.einsum("ai,jia->aj", profile, self.expQt(t))
         return np.log(res) if return_log else res
+
+    def prob_t_profiles(self, profile_pair, multiplicity, t, return_log=False, ignore_gaps=True):
+        """treetime/gtr.py:922-963, site-specific branch (Qt of shape (n, n, L))."""
+        if t < 0:
+            logP = -BIG_NUMBER
+        else:
+            pp, pc = profile_pair
+            res = np.einsum("ai,ija,aj->a", pc, self.expQt(t), pp)
+            if ignore_gaps and self.gap_index is not None:
+                ng = (1 - pp[:, self.gap_index]) * (1 - pc[:, self.gap_index])
+                logP = np.sum(multiplicity * np.log(res + SUPERTINY_NUMBER) * ng)
+            else:
+                logP = np.sum(multiplicity * np.log(res + SUPERTINY_NUMBER))
+        return logP if return_log else np.exp(logP)
+
+    optimal_t_compressed = GTRModel.optimal_t_compressed
 
 
 # ---------------------------------------------------------------------------

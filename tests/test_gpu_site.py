@@ -261,3 +261,51 @@ def test_site_facade_infer_gtr(gpu_engine, marginal):
     ta.infer_ancestral_sequences(marginal=True)
     assert np.isfinite(ta.sequence_LH())
     ta._drop_device()
+
+
+def test_site_marginal_branch_sweep_matches_oracle(gpu_engine):
+    """tt_optimize_branch_lengths_marginal and tt_branch_profile_lh with a
+    site-specific model: prob_t_profiles with a (n, n, L) expQt (gtr.py:950-952)
+    and the interpolated P(t) at every Brent evaluation; tolerances as for
+    the standard model (optimum 1e-4 relative + 1e-8, objective no worse than
+    the oracle's optimum by 1e-9 relative, grid 1e-6)."""
+    from oracle.fp64 import normalize_profile
+
+    prep, g, ta, nodes = site_case(40, 300, 11, True)
+    dev = device(gpu_engine, prep, g)
+    dev.marginal()
+    t, st, pair = dev.optimize_branch_lengths_marginal(-1, 4.0, 1e-10)
+    ta.marginal()
+    og = ta.gtr
+    mult = prep["multiplicity"]
+
+    def objective(pp, pc, x):
+        return -og.prob_t_profiles((pp, pc), mult, x, return_log=True) + np.exp(x ** 2 / 10000)
+
+    worst = 0.0
+    for v, nd in enumerate(nodes):
+        if nd.up is None:
+            continue
+        pp, pc = nd.marginal_outgroup_LH, nd.marginal_subtree_LH
+        t_ref = og.optimal_t_compressed((pp, pc), mult, profiles=True, tol=1e-10)
+        assert st[v] == 0, (v, st[v])
+        assert objective(pp, pc, t[v]) - objective(pp, pc, t_ref) <= 1e-9 * abs(objective(pp, pc, t_ref))
+        assert abs(t[v] - t_ref) <= 1e-4 * t_ref + 1e-8, (v, t[v], t_ref)
+        if t_ref > 1e-6:
+            worst = max(worst, abs(t[v] - t_ref) / t_ref)
+    root = [nd for nd in nodes if nd.up is None][0]
+    if len(root.clades) == 2:
+        n1, n2 = root.clades
+        pp = normalize_profile(n2.marginal_subtree_LH * og.Pi.T)[0]
+        t_ref = og.optimal_t_compressed((pp, n1.marginal_subtree_LH), mult, profiles=True, tol=1e-10)
+        assert pair[1] == 0 and abs(pair[0] - t_ref) <= 1e-4 * t_ref + 1e-8, (pair, t_ref)
+    say(f"[site] marginal sweep: {len(nodes) - 1} branches, max rel diff of optimal t {worst:.2e}")
+    grid = np.array([-0.1, 1e-4, 0.003, 0.05, 0.4, 30.0])
+    lh = dev.branch_profile_lh(grid)
+    for v, nd in enumerate(nodes):
+        if nd.up is None:
+            continue
+        ref = np.array([og.prob_t_profiles((nd.marginal_outgroup_LH, nd.marginal_subtree_LH), mult, x,
+                                           return_log=True) for x in grid])
+        assert np.allclose(lh[v], ref, rtol=1e-6, atol=1e-6), (v, lh[v], ref)
+    dev.close()

@@ -904,7 +904,8 @@ int32_t tt_tree_set_model(tt_tree *T, const double *eigvals, const double *v, co
     if (T->site) {  // back to one model for all patterns: other kernels
         T->site = false;
         T->d.s_V32 = T->d.s_Vi32 = T->d.s_rho32 = nullptr;
-        T->d.s_V64 = T->d.s_Vi64 = T->d.s_rho64 = T->d.s_pi = T->d.s_br = nullptr;
+        T->d.s_V64 = T->d.s_Vi64 = T->d.s_rho64 = T->d.s_pi = T->d.s_br = T->d.s_grid = nullptr;
+        T->d.s_G = 0;
         destroy_graphs(T);
     }
     TT_HIP(hipSetDevice(T->eng->device));
@@ -940,7 +941,8 @@ int32_t tt_tree_set_site_model(tt_tree *T, const double *eigvals, const double *
     // one allocation: V32, Vi32, rho32 (float), then V64, Vi64, rho64, pi, branch params (double)
     const size_t nn2 = (size_t)P * n * n, nn1 = (size_t)P * n;
     const size_t f_bytes = ((2 * nn2 + nn1) * sizeof(float) + 255) & ~(size_t)255;
-    const size_t bytes = f_bytes + (2 * nn2 + 2 * nn1 + 3 * (size_t)T->n_nodes) * sizeof(double);
+    const size_t bytes = f_bytes + (2 * nn2 + 2 * nn1 + 3 * (size_t)T->n_nodes + 64) * sizeof(double);
+    if (G > 64) return fail(TT_EINVAL, "G = %d: at most 64 grid points", G);
     int32_t rc;
     if (!T->d_site && (rc = T->alloc(&T->d_site, bytes))) return rc;
     std::vector<float> f(2 * nn2 + nn1);
@@ -972,6 +974,11 @@ int32_t tt_tree_set_site_model(tt_tree *T, const double *eigvals, const double *
     T->d.s_rho64 = (decltype(T->d.s_rho64))(db + 2 * nn2);
     T->d.s_pi = (decltype(T->d.s_pi))(db + 2 * nn2 + nn1);
     T->d.s_br = (decltype(T->d.s_br))(db + 2 * nn2 + 2 * nn1);
+    double *d_grid = const_cast<double *>(db + 2 * nn2 + 2 * nn1 + 3 * (size_t)T->n_nodes);
+    if (G > 0) TT_HIP(hipMemcpyAsync(d_grid, t_grid, G * sizeof(double), hipMemcpyHostToDevice, T->stream));
+    T->d.s_grid = (decltype(T->d.s_grid))d_grid;
+    T->d.s_G = std::max(G, 0);
+    T->d.s_rate = rate_scale;
     T->site_grid.assign(t_grid, t_grid + std::max(G, 0));
     T->site_rate_scale = rate_scale;
     if (!T->site) destroy_graphs(T);
@@ -1109,7 +1116,6 @@ int32_t tt_optimize_branch_lengths_marginal(tt_tree *T, int32_t gap_state, doubl
                                             int32_t *status, double *root_pair, void *stream) {
     if (!T || !t_opt || !status) return fail(TT_EINVAL, "NULL argument");
     if (!T->have_model) return fail(TT_ESTATE, "tt_tree_set_model has not been called");
-    if (int32_t r = refuse_site(T, "tt_optimize_branch_lengths_marginal")) return r;
     if (!T->profiles_fresh)
         return fail(TT_ESTATE, "marginal ancestral inference needs to be performed first (profiles are stale)");
     if (gap_state < -1 || gap_state >= T->n) return fail(TT_EINVAL, "gap_state %d outside [-1, %d)", gap_state, T->n);
@@ -1147,7 +1153,6 @@ int32_t tt_branch_profile_lh(tt_tree *T, int32_t gap_state, const double *t, int
     if (!T || (K > 0 && (!t || !out))) return fail(TT_EINVAL, "NULL argument");
     if (K <= 0) return TT_OK;
     if (!T->have_model) return fail(TT_ESTATE, "tt_tree_set_model has not been called");
-    if (int32_t r = refuse_site(T, "tt_branch_profile_lh")) return r;
     if (!T->profiles_fresh)
         return fail(TT_ESTATE, "marginal ancestral inference needs to be performed first (profiles are stale)");
     if (gap_state < -1 || gap_state >= T->n) return fail(TT_EINVAL, "gap_state %d outside [-1, %d)", gap_state, T->n);

@@ -281,11 +281,12 @@ __device__ inline void branch_profiles_m(const TreeDev &T, int node, int pair_ot
         if (jr >= 0) {
             const float TT_G *r = T.sub + ((size_t)jr * T.P + p) * N;
 #pragma unroll
-            for (int j = 0; j < N; ++j) pp[j] = (double)r[j] * T.pi[j];
+            for (int j = 0; j < N; ++j) pp[j] = (double)r[j] * (T.s_pi ? T.s_pi[(size_t)p * N + j] : T.pi[j]);
         } else {
             const int code = T.tip_codes[(size_t)T.tip_row[pair_other] * T.P + p];
 #pragma unroll
-            for (int j = 0; j < N; ++j) pp[j] = (double)T.lut[code * N + j] * T.pi[j];
+            for (int j = 0; j < N; ++j)
+                pp[j] = (double)T.lut[code * N + j] * (T.s_pi ? T.s_pi[(size_t)p * N + j] : T.pi[j]);
         }
     } else {  // outgroup: max(TINY, parent profile) / (child message at the current length)
         const float TT_G *pr = T.prof + ((size_t)T.parent[node] * T.P + p) * N;
@@ -343,6 +344,72 @@ __device__ inline void site_P64(const TreeDev &T, int c, int p, double (&E)[N * 
             for (int k = 0; k < N; ++k) a = fma(V[i * N + k] * d[k], Vi[k * N + j], a);
             E[i * N + j] = a;
         }
+}
+
+// Branch profiles with the branch's own P(t) at its current length, also
+// under a site-specific model.
+template <int N>
+__device__ inline void branch_profiles_any(const TreeDev &T, int node, int pair_other, int p, double (&pc)[N],
+                                           double (&pp)[N]) {
+    if constexpr (N <= 5) {  // site-specific models exist for up to 5 states
+        if (T.s_V64) {
+            double E[N * N];
+            if (pair_other < 0) site_P64<N>(T, node, p, E);
+            branch_profiles_m<N>(T, node, pair_other, p, E, pc, pp);
+            return;
+        }
+    }
+    branch_profiles<N>(T, node, pair_other, p, pc, pp);
+}
+
+// (t_lo, t_hi, w) of a length t under the site-specific model's
+// interpolator (scipy interp1d 'linear': searchsorted left, clipped to
+// [1, G - 1]) when t * rate_scale < 10, else (t, t, 0): the exact P(t).
+__device__ inline void site_interp(const TreeDev &T, double t, double &ta, double &tb, double &w) {
+    const int G = T.s_G;
+    if (G >= 2 && t * T.s_rate < 10.0) {
+        int lo = 0, hi = G;  // first index with grid >= t
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (T.s_grid[mid] < t)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        const int h = min(max(lo, 1), G - 1);
+        ta = T.s_grid[h - 1];
+        tb = T.s_grid[h];
+        w = (t - ta) / (tb - ta);
+    } else {
+        ta = tb = t;
+        w = 0.0;
+    }
+}
+
+// pc . P_p(t) pp for one pattern in the eigenbasis:
+// sum_i pc_i pp_i + sum_k (pc V)_k delta_k (V^-1 pp)_k.
+template <int N>
+__device__ inline double site_pair_lh(const TreeDev &T, int p, const double (&pc)[N], const double (&pp)[N],
+                                      double ta, double tb, double w) {
+    const auto rho = T.s_rho64 + (size_t)p * N;
+    const auto V = T.s_V64 + (size_t)p * (N * N);
+    const auto Vi = T.s_Vi64 + (size_t)p * (N * N);
+    double res = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) res = fma(pc[i], pp[i], res);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            a = fma(pc[i], V[i * N + k], a);
+            b = fma(Vi[k * N + i], pp[i], b);
+        }
+        double d = expm1(ta * rho[k]);
+        if (w != 0.0) d += w * (expm1(tb * rho[k]) - d);
+        res = fma(a * d, b, res);
+    }
+    return res;
 }
 
 // Per-pattern statistics for GTR_site_specific.infer (treeanc.py:1556-1589
@@ -463,6 +530,25 @@ __device__ inline double profile_loglik(const TreeDev &T, int node, int other, i
     return block_total(acc, s_red);
 }
 
+// Site-specific model: the same sum with P_p(t) per pattern
+// (prob_t_profiles with a (n, n, L) expQt, gtr.py:950-952).
+template <int N>
+__device__ inline double profile_loglik_site(const TreeDev &T, int node, int other, int gap, double t,
+                                             double *s_red) {
+    double ta, tb, w;
+    site_interp(T, t, ta, tb, w);
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < T.P; p += blockDim.x) {
+        double pc[N], pp[N];
+        branch_profiles_any<N>(T, node, other, p, pc, pp);
+        const double res = site_pair_lh<N>(T, p, pc, pp, ta, tb, w);
+        double wt = T.mult[p];
+        if (gap >= 0) wt *= (1.0 - pp[gap]) * (1.0 - pc[gap]);
+        acc = fma(wt, log(res + kSuperTiny), acc);
+    }
+    return block_total(acc, s_red);
+}
+
 template <int N>
 __global__ __launch_bounds__(kOptThreads) void k_branch_opt_marginal(TreeDev T, int gap, const double *eig,
                                                                      const double *v, const double *vinv, double mu,
@@ -487,7 +573,7 @@ __global__ __launch_bounds__(kOptThreads) void k_branch_opt_marginal(TreeDev T, 
     double same = 0.0, all = 0.0;
     for (int p = threadIdx.x; p < T.P; p += blockDim.x) {
         double pc[N], pp[N];
-        branch_profiles<N>(T, node, other, p, pc, pp);
+        branch_profiles_any<N>(T, node, other, p, pc, pp);
         double d = 0.0;
 #pragma unroll
         for (int j = 0; j < N; ++j) d = fma(pp[j], pc[j], d);
@@ -499,8 +585,12 @@ __global__ __launch_bounds__(kOptThreads) void k_branch_opt_marginal(TreeDev T, 
     const double hamming = 1.0 - same / all;
 
     auto f = [&](double s) {
+        double lp;
+        if constexpr (N <= 5) {
+            if (T.s_V64) return -profile_loglik_site<N>(T, node, other, gap, s * s, s_red) + exp(s * s * s * s / 10000.0);
+        }
         build_expqt<N>(s * s, eig, v, vinv, mu, s_P);
-        const double lp = profile_loglik<N>(T, node, other, gap, s_P, s_red);
+        lp = profile_loglik<N>(T, node, other, gap, s_P, s_red);
         return -lp + exp(s * s * s * s / 10000.0);
     };
     int st = 0;
@@ -530,8 +620,17 @@ __global__ __launch_bounds__(kOptThreads) void k_branch_profile_lh(TreeDev T, in
         if (tk < 0.0) {
             lp = -1e10;  // config.py BIG_NUMBER (gtr.py:946-947)
         } else {
-            build_expqt<N>(tk, eig, v, vinv, mu, s_P);
-            lp = profile_loglik<N>(T, node, -1, gap, s_P, s_red);
+            bool done = false;
+            if constexpr (N <= 5) {
+                if (T.s_V64) {
+                    lp = profile_loglik_site<N>(T, node, -1, gap, tk, s_red);
+                    done = true;
+                }
+            }
+            if (!done) {
+                build_expqt<N>(tk, eig, v, vinv, mu, s_P);
+                lp = profile_loglik<N>(T, node, -1, gap, s_P, s_red);
+            }
         }
         if (threadIdx.x == 0) out[(size_t)node * K + k] = lp;
     }

@@ -70,6 +70,9 @@ struct TreeDev {
     const double TT_C *s_rho64;
     const double TT_C *s_pi;
     const double TT_C *s_br;
+    const double TT_C *s_grid;       // [s_G] interpolation grid (s_G = 0: exact P(t) for every length)
+    int32_t s_G;
+    double s_rate;                   // rate scale: lengths with t * s_rate < 10 are interpolated
 };
 
 // One level of a level-scheduled pass.
