@@ -449,7 +449,7 @@ def main():
 
     import torch
 
-    from treetime_amd.engine import Engine, batch_info, batch_marginal
+    from treetime_amd.engine import Engine, batch_info, batch_marginal, batch_run
     from treetime_amd.prepare import device_tree
 
     eng = Engine(0, 0)
@@ -492,6 +492,20 @@ def main():
     wall = time.perf_counter() - w0
     ms = ev0.elapsed_time(ev1) / args.steps
     log(f"[bench] batched: {ms:.3f} ms/step (events), wall {wall / args.steps * 1e3:.3f} ms/step")
+
+    # batched log-likelihood only (postorder + root, the re-evaluation pass)
+    ms_ll = float("nan")
+    if args.plain_steps > 0:
+        batch_run(eng, trees, "loglik", stream=sp, want_logL=False)
+        torch.cuda.synchronize()
+        l0, l1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        l0.record(stream)
+        for _ in range(args.plain_steps):
+            batch_run(eng, trees, "loglik", stream=sp, want_logL=False)
+        l1.record(stream)
+        torch.cuda.synchronize()
+        ms_ll = l0.elapsed_time(l1) / args.plain_steps
+        log(f"[bench] batched loglik: {ms_ll:.3f} ms/step")
 
     # plain: one tree per launch sequence, trees in sequence
     ms_plain = float("nan")
@@ -550,6 +564,8 @@ def main():
             "updates_per_step": int(U),
             "precision": "f32 rows, messages and preorder; f64 normaliser products, root and final sum",
             "plain_ms_per_step": ms_plain,
+            "batched_loglik_ms_per_step": ms_ll,
+            "batched_loglik_value": U / (ms_ll / 1e3),
             "plain_value": U / (ms_plain / 1e3),
             "batched_vs_plain": ms_plain / ms,
             "batched_vs_plain_logL_max_rel_diff": rel,

@@ -276,6 +276,21 @@ int32_t tt_site_counts(tt_tree *tree, int32_t marginal, const int16_t *code_stat
 int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, double *logL,
                           void *stream);
 
+/* One batched pass over k resident trees (the batching entry of the build
+ * plan; no reference counterpart, the reference runs one tree at a time):
+ * mode = pass kind | TT_RECONSTRUCT_TIPS.
+ *   TT_BATCH_MARGINAL: tt_batch_marginal.
+ *   TT_BATCH_LOGLIK:   postorder and root only, the persistent launch with
+ *                      its preorder skipped (per tree as tt_loglik: logL and
+ *                      sequence_LH; profiles and states are left stale).
+ *   TT_BATCH_JOINT:    each tree's own captured joint pass (tt_joint), all
+ *                      enqueued on `stream` (no persistent joint kernel).
+ * logL is host [k] (NULL: no read-back). */
+#define TT_BATCH_MARGINAL 0x00u
+#define TT_BATCH_JOINT 0x10u
+#define TT_BATCH_LOGLIK 0x20u
+int32_t tt_batch_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t mode, double *logL, void *stream);
+
 /* Launch shape the autotuner chose for the plan of (trees, flags): one
  * 4-wave block (0) or one wave (1) per item of the persistent kernel, or each
  * tree's own level-kernel pass in turn (2); 64-pattern tiles per item;
@@ -283,6 +298,7 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
  * compiled for (0 = the compiler's own budget); the measured time per launch
  * while tuning (0 if not timed).  Any output pointer may be NULL.  TT_ESTATE
  * when no plan exists yet.  No reference counterpart. */
+/* flags: as the mode of tt_batch_run (TT_BATCH_LOGLIK selects the loglik plan). */
 int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, int32_t *wave_items,
                       int32_t *tiles, int32_t *blocks_per_cu, int32_t *waves_per_simd, double *tuned_ms);
 

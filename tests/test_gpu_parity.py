@@ -15,6 +15,7 @@ import pytest
 
 from helpers import build_oracle, polytomy_job, small_job
 from treetime_amd.engine import Engine
+from treetime_amd.errors import MissingDataError
 from treetime_amd.prepare import device_tree
 
 pytestmark = pytest.mark.gpu
@@ -209,6 +210,41 @@ def test_batch_matches_plain(gpu_engine, tips_flag):
     ta, nodes, tips = build_oracle(prep, alphabet, job["gtr"])
     ref = ta.marginal()
     assert abs(got[3] - ref) <= LOGL_RTOL * abs(ref)
+    for t in trees:
+        t.close()
+
+
+def test_batch_run_modes(gpu_engine):
+    """tt_batch_run: 'loglik' (persistent launch without the preorder) equals
+    tt_loglik per tree and leaves profiles stale; 'joint' equals tt_joint per
+    tree; 'marginal' equals tt_batch_marginal."""
+    from treetime_amd.engine import batch_info, batch_marginal, batch_run
+
+    say("[batch] tt_batch_run loglik / joint / marginal on 6 trees")
+    specs = [(50, 1000, 11), (300, 200, 12), (7, 65, 13), (120, 3000, 14), (3, 10, 15)]
+    jobs = [small_job(n, L, 4, seed) for n, L, seed in specs] + [polytomy_job(150, 500, 4, 16, 0.3)]
+    trees = [device_tree(gpu_engine, prep) for _, prep, _ in jobs]
+    plain_ll = [t.loglik() for t in trees]
+    plain_slh = [t.sequence_lh() for t in trees]
+    for rep in range(2):
+        got = batch_run(gpu_engine, trees, "loglik")
+        for i, t in enumerate(trees):
+            assert abs(got[i] - plain_ll[i]) <= 1e-12 * abs(plain_ll[i]), (i, got[i], plain_ll[i])
+            assert np.allclose(t.sequence_lh(), plain_slh[i], rtol=0, atol=1e-12)
+    assert batch_info(gpu_engine, trees, kind="loglik")["blocks_per_cu"] >= 0
+    with pytest.raises(MissingDataError):  # profiles are stale after a loglik pass
+        trees[0].optimize_branch_lengths_marginal(-1, 4.0, 1e-10)
+    joint_plain = [t.joint() for t in trees]
+    joint_states = [t.states() for t in trees]
+    got = batch_run(gpu_engine, trees, "joint")
+    for i, t in enumerate(trees):
+        assert got[i] == joint_plain[i]
+        assert np.array_equal(t.states(), joint_states[i])
+    got_m = batch_run(gpu_engine, trees, "marginal")
+    ref_m = batch_marginal(gpu_engine, trees)
+    assert np.array_equal(got_m, ref_m)
+    for i in range(len(trees)):
+        assert abs(got_m[i] - plain_ll[i]) <= 1e-12 * abs(plain_ll[i])
     for t in trees:
         t.close()
 

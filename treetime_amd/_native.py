@@ -25,6 +25,9 @@ TT_ESTATE = -5
 TT_RECONSTRUCT_TIPS = 1
 TT_PROFILE_MARGINAL = 0
 TT_PROFILE_SUBTREE = 1
+TT_BATCH_MARGINAL = 0x00
+TT_BATCH_JOINT = 0x10
+TT_BATCH_LOGLIK = 0x20
 TT_FILL_OVERHANGS = 1
 TT_COMPRESS_DEVICE = 2
 
@@ -103,6 +106,7 @@ SIGNATURES = {
     "tt_batch_info": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PI32, _PI32, _PI32, _PI32, _PD]),
     "tt_tree_set_site_model": (_I32, [_P, _PD, _PD, _PD, _PD, _PD, _PD, _I32, _D]),
     "tt_propagate_profile_site": (_I32, [_P, _I32, _I32, _P, _I32, _I32, _P, _P]),
+    "tt_batch_run": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PD, _P]),
     "tt_site_counts": (_I32, [_P, _I32, _PI16, _PD, _PD, _P]),
     "tt_compress_alignment": (_I32, [_P, _P, _I32, _I64, _I64, _I32, _U32, _I64, _PI64, _P, _P, _P, _PI64, _P]),
 }

@@ -404,6 +404,8 @@ int32_t ensure_joint(tt_tree *T) {
 // flag of enqueue_pass (not part of the ABI): reset the tip rows of the
 // states to 255 because an earlier pass reconstructed them
 constexpr uint32_t kFillTips = 0x100u;
+// plan flag of a loglik-only (postorder + root) batched pass
+constexpr uint32_t kPlanLoglik = 2u;
 
 int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
     const TreeDev &D = T->d;
@@ -613,13 +615,15 @@ int32_t ensure_opt_out(tt_tree *T) {
 // large trees, whose items would each walk thousands of levels alone)
 constexpr int kPerTreeGraphs = 2;
 
-int32_t launch_plan(BatchPlan *P, int n, int tips, const BatchShape &c, hipStream_t s) {
+// mode: bit 0 reconstruct tips, bit 1 loglik only (postorder and root)
+int32_t launch_plan(BatchPlan *P, int n, int mode, const BatchShape &c, hipStream_t s) {
     if (c.wave_items == kPerTreeGraphs) {
         for (tt_tree *T : P->trees)
-            if (int32_t rc = run_pass(T, 1, tips ? TT_RECONSTRUCT_TIPS : 0u, nullptr, s)) return rc;
+            if (int32_t rc = run_pass(T, (mode & 2) ? 3 : 1, (mode & 1) ? TT_RECONSTRUCT_TIPS : 0u, nullptr, s))
+                return rc;
         return TT_OK;
     }
-    TT_HIP(P->launch(n, tips, c, s));
+    TT_HIP(P->launch(n, mode, c, s));
     return TT_OK;
 }
 
@@ -1418,7 +1422,8 @@ int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t
     std::vector<tt_tree *> key(trees, trees + std::max(k, 0));
     std::lock_guard<std::mutex> g(eng->mu);
     for (BatchPlan *p : eng->plans)
-        if (p->flags == (flags & TT_RECONSTRUCT_TIPS) && p->trees == key) {
+        if (p->flags == (((flags & 0x30u) == TT_BATCH_LOGLIK) ? kPlanLoglik : (flags & TT_RECONSTRUCT_TIPS)) &&
+            p->trees == key) {
             if (wave_items) *wave_items = p->shape.wave_items;
             if (tiles) *tiles = p->shape.tiles;
             if (blocks_per_cu) *blocks_per_cu = p->shape.blocks_per_cu;
@@ -1429,8 +1434,12 @@ int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t
     return fail(TT_ESTATE, "no batch plan for these trees and flags");
 }
 
-int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, double *logL,
-                          void *stream) {
+}  // extern "C"
+
+namespace {
+
+int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, bool loglik, double *logL,
+                       void *stream) {
     if (!eng || (k > 0 && !trees)) return fail(TT_EINVAL, "NULL argument");
     if (k <= 0) return TT_OK;
     const int n = trees[0] ? trees[0]->n : 0;
@@ -1443,24 +1452,26 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
         if (T->site) return fail(TT_EINVAL, "trees[%d] has a site-specific model: run it with tt_marginal", i);
     }
     TT_HIP(hipSetDevice(eng->device));
-    const uint32_t fl = flags & TT_RECONSTRUCT_TIPS;
+    const uint32_t fl = loglik ? 0u : (flags & TT_RECONSTRUCT_TIPS);
+    const uint32_t pflags = fl | (loglik ? kPlanLoglik : 0u);
+    const int mode = (fl ? 1 : 0) | (loglik ? 2 : 0);
     std::vector<tt_tree *> key(trees, trees + k);
     BatchPlan *plan = nullptr;
     {
         std::lock_guard<std::mutex> g(eng->mu);
         for (BatchPlan *p : eng->plans)
-            if (p->flags == fl && p->trees == key) plan = p;
+            if (p->flags == pflags && p->trees == key) plan = p;
     }
     if (!plan) {
         auto *P = new BatchPlan;
         P->eng = eng;
         P->trees = key;
-        P->flags = fl;
+        P->flags = pflags;
         // items, longest tree first (LPT), tiles in ascending order within a tree
         std::vector<std::pair<long, int>> order;
         for (int i = 0; i < k; ++i) {
-            long pre = (long)trees[i]->pre_nodes.size();
-            if (!fl) pre -= trees[i]->n_tips;
+            long pre = loglik ? 0L : (long)trees[i]->pre_nodes.size();
+            if (!loglik && !fl) pre -= trees[i]->n_tips;
             order.push_back({(long)trees[i]->n_int + pre, i});
         }
         std::stable_sort(order.begin(), order.end(), [](auto &a, auto &b) { return a.first > b.first; });
@@ -1508,7 +1519,7 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
             delete P;
             return fail(TT_EDEVICE, "batch plan set-up: %s", hipGetErrorString(e));
         }
-        if (int32_t rc = tune_batch(P, n, fl ? 1 : 0, P->stream)) {
+        if (int32_t rc = tune_batch(P, n, mode, P->stream)) {
             delete P;
             return rc;
         }
@@ -1517,7 +1528,7 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
             e = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal);
             hipError_t e2 = hipSuccess;
             if (e == hipSuccess)
-                e2 = P->launch(n, fl ? 1 : 0, P->shape, P->stream);
+                e2 = P->launch(n, mode, P->shape, P->stream);
             if (e == hipSuccess) e = hipStreamEndCapture(P->stream, &g);
             if (e == hipSuccess) e = e2;
             if (e == hipSuccess) e = hipGraphInstantiate(&P->graph, g, nullptr, nullptr, 0);
@@ -1534,7 +1545,7 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
     hipStream_t s = stream ? (hipStream_t)stream : plan->stream;
     // tip rows that an earlier pass reconstructed are reset to 255 (read by the kernel at run time)
     bool changed = false;
-    for (int i = 0; i < k; ++i) {
+    for (int i = 0; i < k && !loglik; ++i) {
         const int32_t want = (!fl && trees[i]->tips_dirty) ? 1 : 0;
         if (plan->host_trees[i].fill_tips != want) {
             plan->host_trees[i].fill_tips = want;
@@ -1546,10 +1557,15 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
                               hipMemcpyHostToDevice, s));
     if (plan->graph) {
         TT_HIP(hipGraphLaunch(plan->graph, s));
-    } else if (int32_t rc = launch_plan(plan, n, fl ? 1 : 0, plan->shape, s)) {
+    } else if (int32_t rc = launch_plan(plan, n, mode, plan->shape, s)) {
         return rc;
     }
     for (int i = 0; i < k; ++i) {
+        if (loglik) {  // subtree rows rewritten, profiles and states not
+            trees[i]->last_pass = 3;
+            trees[i]->profiles_fresh = false;
+            continue;
+        }
         trees[i]->last_pass = 1;
         trees[i]->have_states = true;
         trees[i]->tips_dirty = fl != 0;
@@ -1566,6 +1582,42 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
         TT_HIP(hipStreamSynchronize(s));
         for (int i = 0; i < k; ++i) logL[i] = 0.0;
         for (size_t j = 0; j < out.size(); ++j) logL[items[j].tree] += out[j];
+    }
+    return TT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, double *logL,
+                          void *stream) {
+    return batch_plan_run(eng, trees, k, flags, false, logL, stream);
+}
+
+int32_t tt_batch_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t mode, double *logL, void *stream) {
+    const uint32_t kind = mode & 0x30u;
+    if (mode & ~(0x30u | TT_RECONSTRUCT_TIPS)) return fail(TT_EINVAL, "unknown mode bits 0x%x", mode);
+    if (kind == TT_BATCH_MARGINAL) return batch_plan_run(eng, trees, k, mode, false, logL, stream);
+    if (kind == TT_BATCH_LOGLIK) return batch_plan_run(eng, trees, k, 0u, true, logL, stream);
+    if (kind != TT_BATCH_JOINT) return fail(TT_EINVAL, "unknown pass kind 0x%x", kind);
+    // joint: each tree's own captured joint pass, all on one stream
+    if (!eng || (k > 0 && !trees)) return fail(TT_EINVAL, "NULL argument");
+    for (int i = 0; i < k; ++i) {
+        if (!trees[i]) return fail(TT_EINVAL, "trees[%d] is NULL", i);
+        if (trees[i]->eng != eng) return fail(TT_EINVAL, "trees[%d] belongs to another engine", i);
+    }
+    for (int i = 0; i < k; ++i)
+        if (int32_t rc = run_pass(trees[i], 2, mode & TT_RECONSTRUCT_TIPS, nullptr, stream)) return rc;
+    if (logL) {
+        hipStream_t s = (hipStream_t)stream;
+        for (int i = 0; i < k; ++i)
+            TT_HIP(hipMemcpyAsync(&logL[i], trees[i]->d.total, sizeof(double), hipMemcpyDeviceToHost,
+                                  s ? s : trees[i]->stream));
+        if (s)
+            TT_HIP(hipStreamSynchronize(s));
+        else
+            for (int i = 0; i < k; ++i) TT_HIP(hipStreamSynchronize(trees[i]->stream));
     }
     return TT_OK;
 }

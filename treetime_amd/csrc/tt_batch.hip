@@ -91,7 +91,9 @@ __device__ __forceinline__ void pre_pair_t(const TreeDev &T, const PreDesc &q0, 
 template <int N, int TL, int WPE>
 __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_batch_marginal(const BatchTreeDev *__restrict__ trees_arg,
                                                              const BatchItem *__restrict__ items_arg, int n_items,
-                                                             int *queue, double *item_out, int tips) {
+                                                             int *queue, double *item_out, int mode) {
+    // mode bit 0: reconstruct tip states; bit 1: postorder and root only (loglik)
+    const bool tips = mode & 1, pre = !(mode & 2);
     // tree records and items are read-only: scalar loads through the constant address space
     const BatchTreeDev TT_C *trees = (const BatchTreeDev TT_C *)trees_arg;
     const BatchItem TT_C *items = (const BatchItem TT_C *)items_arg;
@@ -145,6 +147,7 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
         }
         const double tot = block_sum(contrib);
         if (threadIdx.x == 0) item_out[it] = tot;
+        if (!pre) continue;  // loglik: the next item (its first barrier orders the LDS reuse)
         if (!tips && B.fill_tips)
             for (int k = w; k < T.n_tips; k += kWaves) {
                 const size_t row = (size_t)B.tip_nodes[k] * T.P;
@@ -179,7 +182,8 @@ __device__ __forceinline__ double wave_sum(double x) {
 template <int N>
 __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal_w(const BatchTreeDev *__restrict__ trees_arg,
                                                                const BatchItem *__restrict__ items_arg,
-                                                               int n_items, int *queue, double *item_out, int tips) {
+                                                               int n_items, int *queue, double *item_out, int mode) {
+    const bool tips = mode & 1, pre = !(mode & 2);
     const BatchTreeDev TT_C *trees = (const BatchTreeDev TT_C *)trees_arg;
     const BatchItem TT_C *items = (const BatchItem TT_C *)items_arg;
     __shared__ float s_lut[kWaves][256 * N];
@@ -217,6 +221,7 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal_w(const BatchTreeDe
         if (act) contrib = root_step<N>(T, p, acc.log_value());
         const double tot = wave_sum(contrib);
         if (lane == 0) item_out[it] = tot;
+        if (!pre) continue;
         if (!tips && B.fill_tips && act)
             for (int k = 0; k < T.n_tips; ++k) T.states[(size_t)B.tip_nodes[k] * T.P + p] = 255;
         for (int d = 0; d < B.D; ++d) {  // depth order: parents before children

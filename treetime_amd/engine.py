@@ -335,7 +335,23 @@ def batch_marginal(engine, trees, reconstruct_tips=False, stream=None, want_logL
     return out
 
 
-def batch_info(engine, trees, reconstruct_tips=False):
+def batch_run(engine, trees, kind="marginal", reconstruct_tips=False, stream=None, want_logL=True):
+    """tt_batch_run: kind 'marginal' (one persistent launch), 'loglik'
+    (persistent launch without the preorder) or 'joint' (each tree's joint
+    pass on one stream).  Returns logL per tree."""
+    k = len(trees)
+    arr = (C.c_void_p * k)(*[t.handle.value for t in trees])
+    out = np.full(k, np.nan)
+    mode = {"marginal": _native.TT_BATCH_MARGINAL, "joint": _native.TT_BATCH_JOINT,
+            "loglik": _native.TT_BATCH_LOGLIK}[kind]
+    if reconstruct_tips:
+        mode |= _native.TT_RECONSTRUCT_TIPS
+    _native.check(engine.lib.tt_batch_run(engine.handle, arr, k, mode,
+                                          _native.ptr(out, C.c_double) if want_logL else None, stream))
+    return out
+
+
+def batch_info(engine, trees, reconstruct_tips=False, kind="marginal"):
     """Launch shape chosen by the autotuner for this list of trees:
     {"wave_items", "tiles", "blocks_per_cu", "waves_per_simd", "tuned_ms"}
     (waves_per_simd 0 = the compiler's register budget)."""
@@ -343,6 +359,8 @@ def batch_info(engine, trees, reconstruct_tips=False):
     arr = (C.c_void_p * k)(*[t.handle.value for t in trees])
     w, tl, b, wps, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
     flags = _native.TT_RECONSTRUCT_TIPS if reconstruct_tips else 0
+    if kind == "loglik":
+        flags = _native.TT_BATCH_LOGLIK
     _native.check(engine.lib.tt_batch_info(engine.handle, arr, k, flags, C.byref(w), C.byref(tl), C.byref(b),
                                            C.byref(wps), C.byref(ms)))
     return {"wave_items": w.value, "tiles": tl.value, "blocks_per_cu": b.value, "waves_per_simd": wps.value,
