@@ -17,7 +17,8 @@
 //
 // The row layout, the max-normalised FP32 rows, the FP64 normaliser product
 // and the partial-row sums are those of the standard level kernels; the
-// joint pass builds each lane's log P(t) in FP64.
+// joint pass builds each lane's P(t) in FP64 and takes its maxima in product
+// space.
 #include "tt_device.h"
 
 namespace {
@@ -303,10 +304,11 @@ __global__ __launch_bounds__(TT_BLOCK) void k_pre_site(TreeDev T, LevelDev L, in
 
 // ---- joint (FP64) -------------------------------------------------------------
 
-// log max(TINY, P_p(t)) of branch c for this lane's pattern (the reference's
-// log_transitions, treeanc.py:964-967, with P of shape (n, n, L)).
+// max(TINY, P_p(t)) of branch c for this lane's pattern, FP64 (the
+// reference's log_transitions before the log, treeanc.py:964-967, with P of
+// shape (n, n, L)).
 template <int N>
-__device__ __forceinline__ void site_logP(const TreeDev &T, int c, int p, double (&lp)[N][N]) {
+__device__ __forceinline__ void site_Pc(const TreeDev &T, int c, int p, double (&P)[N][N]) {
     const auto b = T.s_br + 3 * (size_t)c;
     const double ta = b[0], tb = b[1], w = b[2];
     const auto rho = T.s_rho64 + (size_t)p * N;
@@ -328,28 +330,30 @@ __device__ __forceinline__ void site_logP(const TreeDev &T, int c, int p, double
             double a = (i == j) ? 1.0 : 0.0;
 #pragma unroll
             for (int k = 0; k < N; ++k) a = fma(vd[k], Vi[k * N + j], a);
-            lp[i][j] = log(fmax(TT_TINY, a));
+            P[i][j] = fmax(TT_TINY, a);
         }
     }
 }
 
-// Lx[j] = max_i lp[i][j] + msg_in[i], first argmax in cx
+// Lx[j] = max_i log max(TINY, P_ij) + in_i, first argmax in cx, formed in
+// product space: with E_i = exp(in_i - m) (m the shift passed in), Lx[j] =
+// m + log(max_i P_ij E_i) -- one log per state instead of one per (i, j).
 template <int N>
-__device__ __forceinline__ void site_joint_max(const double (&lp)[N][N], const double (&in)[N], double (&best)[N],
-                                               int (&cx)[N]) {
+__device__ __forceinline__ void site_joint_max(const double (&P)[N][N], const double (&E)[N], double m,
+                                               double (&best)[N], int (&cx)[N]) {
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        double b = lp[0][j] + in[0];
+        double b = P[0][j] * E[0];
         int bi = 0;
 #pragma unroll
         for (int i = 1; i < N; ++i) {
-            const double val = lp[i][j] + in[i];
+            const double val = P[i][j] * E[i];
             if (val > b) {
                 b = val;
                 bi = i;
             }
         }
-        best[j] = b;
+        best[j] = m + log(b);
         cx[j] = bi;
     }
 }
@@ -360,14 +364,15 @@ __device__ __forceinline__ void site_joint_children(const TreeDev &T, int e0, in
     for (int e = e0; e < e1; ++e) {
         const int cref = wave_uniform(T.child_ref[e]);
         if (cref < 0) {
+            // tip: exp(log max(profile, TINY)) = max(profile, TINY), no shift
             const int cnode = wave_uniform(T.child_node[e]);
             const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
-            double tl[N], lp[N][N], best[N];
+            double E[N], P[N][N], best[N];
             int cx[N];
 #pragma unroll
-            for (int i = 0; i < N; ++i) tl[i] = T.loglut[code * N + i];
-            site_logP<N>(T, cnode, p, lp);
-            site_joint_max<N>(lp, tl, best, cx);
+            for (int i = 0; i < N; ++i) E[i] = fmax(TT_TINY, (double)T.lut[code * N + i]);
+            site_Pc<N>(T, cnode, p, P);
+            site_joint_max<N>(P, E, 0.0, best, cx);
 #pragma unroll
             for (int j = 0; j < N; ++j) msg[j] += best[j];
             if (store_tip_cx) store_bytes<N>(T.Cx + ((size_t)cnode * T.P + p) * N, cx);
@@ -399,10 +404,15 @@ __global__ __launch_bounds__(TT_BLOCK) void k_post_joint_site(TreeDev T, LevelDe
             for (int i = 0; i < N; ++i) msg[i] = 0.0;
             site_joint_children<N>(T, wave_uniform(T.child_ptr[v]), wave_uniform(T.child_ptr[v + 1]), p, msg,
                                    store_tip_cx);
-            double lp[N][N], best[N];
+            double P[N][N], E[N], best[N];
             int cx[N];
-            site_logP<N>(T, vn, p, lp);
-            site_joint_max<N>(lp, msg, best, cx);
+            double m = msg[0];
+#pragma unroll
+            for (int i = 1; i < N; ++i) m = fmax(m, msg[i]);
+#pragma unroll
+            for (int i = 0; i < N; ++i) E[i] = exp(msg[i] - m);
+            site_Pc<N>(T, vn, p, P);
+            site_joint_max<N>(P, E, m, best, cx);
             double off = best[0];
 #pragma unroll
             for (int j = 1; j < N; ++j) off = fmax(off, best[j]);
