@@ -309,3 +309,37 @@ def test_site_marginal_branch_sweep_matches_oracle(gpu_engine):
                                            return_log=True) for x in grid])
         assert np.allclose(lh[v], ref, rtol=1e-6, atol=1e-6), (v, lh[v], ref)
     dev.close()
+
+
+def test_site_five_states_with_gaps(gpu_engine):
+    """Alphabet 'nuc' (gap as a fifth state): marginal, joint and the
+    gap-weighted branch objective (gtr.py:953-959) against the oracle."""
+    say("[site] 5 states")
+    job = synth.make_job(50, 300, 0.05, 4, 12, 0.01, 0.05)
+    prep = prepare(job["parent"], job["branch_length"], job["chars"], job["gtr"], job["ambiguous"], compress=False)
+    g = tt.GTR_site_specific.random(L=300, alphabet="nuc", rng=np.random.default_rng(112))
+    og = O.SiteGTRModel("nuc", mu=g.mu, pi=g.Pi, W=g.W)
+    parent = prep["parent"]
+    names = ["n%d" % i for i in range(len(parent))]
+    root, nodes = O.build_tree(parent, prep["t"], names)
+    is_tip = np.ones(len(parent), dtype=bool)
+    is_tip[parent[parent >= 0]] = False
+    chars = prep["tip_codes"].view("S1").astype("U1")
+    comp = {names[v]: chars[k] for k, v in enumerate(np.where(is_tip)[0])}
+    ta = O.OracleTreeAnc(root, og, comp, prep["multiplicity"], full_length=1e300)
+    dev = device(gpu_engine, prep, g)
+    lh, ref = dev.marginal(), ta.marginal()
+    say(f"  marginal {lh:.10f} oracle {ref:.10f}")
+    assert abs(lh - ref) < 1e-7 * abs(ref)
+    lj, rj = dev.joint(), ta.joint()
+    assert abs(lj - rj) < 1e-9 * abs(rj)
+    dev.marginal()
+    t, st, _ = dev.optimize_branch_lengths_marginal(4, 4.0, 1e-10)
+    mult = prep["multiplicity"]
+    for v, nd in enumerate(nodes[:40]):
+        if nd.up is None:
+            continue
+        pp, pc = nd.marginal_outgroup_LH, nd.marginal_subtree_LH
+        t_ref = og.optimal_t_compressed((pp, pc), mult, profiles=True, tol=1e-10)
+        assert abs(t[v] - t_ref) <= 1e-4 * t_ref + 1e-8, (v, t[v], t_ref)
+    dev.close()
