@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-compress", action="store_true", help="skip the pattern-compression measurement")
+    ap.add_argument("--no-site", action="store_true", help="skip the site-specific model measurement")
     ap.add_argument("--serve-rate", default="250,1000,4000,16000",
                     help="request arrival rates (per second) to sweep, comma-separated")
     ap.add_argument("--serve-requests", type=int, default=600)
@@ -507,6 +508,22 @@ def main():
         ms_ll = l0.elapsed_time(l1) / args.plain_steps
         log(f"[bench] batched loglik: {ms_ll:.3f} ms/step")
 
+    # batched joint: every tree's joint pass on one stream
+    ms_joint = float("nan")
+    if args.plain_steps > 0:
+        batch_run(eng, trees, "joint", stream=sp, want_logL=False)
+        torch.cuda.synchronize()
+        j0, j1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        j0.record(stream)
+        for _ in range(args.plain_steps):
+            batch_run(eng, trees, "joint", stream=sp, want_logL=False)
+        j1.record(stream)
+        torch.cuda.synchronize()
+        ms_joint = j0.elapsed_time(j1) / args.plain_steps
+        log(f"[bench] joint, all trees: {ms_joint:.3f} ms/step")
+        for t in trees:  # return the joint buffers (tens of GB on the mix) before the serving leg
+            t.trim()
+
     # plain: one tree per launch sequence, trees in sequence
     ms_plain = float("nan")
     if args.plain_steps > 0:
@@ -529,6 +546,9 @@ def main():
     compression = None
     if not args.no_compress:
         compression = compress_bench(eng, torch, 10, cpu_seconds=0 if args.no_cpu else 5.0)
+    site = None
+    if not args.no_site and args.workload == "mix":
+        site = site_bench(eng, torch, 10, cpu=not args.no_cpu)
 
     serving = None
     if not args.no_serve and len(trees) > 1:
@@ -566,6 +586,8 @@ def main():
             "plain_ms_per_step": ms_plain,
             "batched_loglik_ms_per_step": ms_ll,
             "batched_loglik_value": U / (ms_ll / 1e3),
+            "joint_ms_per_step": ms_joint,
+            "joint_value": U / (ms_joint / 1e3),
             "plain_value": U / (ms_plain / 1e3),
             "batched_vs_plain": ms_plain / ms,
             "batched_vs_plain_logL_max_rel_diff": rel,
@@ -581,6 +603,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "compression": compression,
+        "site_specific": site,
     }
     print(json.dumps(out), flush=True)
     for t in trees:
