@@ -457,7 +457,8 @@ def main():
     t0 = time.perf_counter()
     trees = [device_tree(eng, p) for p in preps]
     torch.cuda.synchronize()
-    log(f"[bench] {len(trees)} trees resident ({eng.usage()[0] / 2**30:.1f} GiB) in {time.perf_counter() - t0:.1f}s")
+    upload_s = time.perf_counter() - t0
+    log(f"[bench] {len(trees)} trees resident ({eng.usage()[0] / 2**30:.1f} GiB) in {upload_s:.1f}s")
     U = sum(updates(p) for p in preps)
     B = sum(algorithmic_bytes(p) for p in preps)
     P_total = sum(p["tip_codes"].shape[1] for p in preps)
@@ -492,6 +493,14 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
     ms = ev0.elapsed_time(ev1) / args.steps
+    # per-step distribution: the same steps again, one event pair each
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    for a, b in evs:
+        a.record(stream)
+        batch_marginal(eng, trees, stream=sp, want_logL=False)
+        b.record(stream)
+    torch.cuda.synchronize()
+    step_ms = np.array([a.elapsed_time(b) for a, b in evs])
     log(f"[bench] batched: {ms:.3f} ms/step (events), wall {wall / args.steps * 1e3:.3f} ms/step")
 
     # batched log-likelihood only (postorder + root, the re-evaluation pass)
@@ -593,6 +602,9 @@ def main():
             "batched_vs_plain_logL_max_rel_diff": rel,
             "logL_rel_err_vs_fp64_oracle": err_fp64,
             "wall_ms_per_step": wall / args.steps * 1e3,
+            "step_ms_median": float(np.median(step_ms)),
+            "step_ms_p99": float(np.percentile(step_ms, 99)),
+            "resident_upload_s": upload_s,
             "launch_shape": shape,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
