@@ -283,8 +283,11 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
  *   TT_BATCH_LOGLIK:   postorder and root only, the persistent launch with
  *                      its preorder skipped (per tree as tt_loglik: logL and
  *                      sequence_LH; profiles and states are left stale).
- *   TT_BATCH_JOINT:    each tree's own captured joint pass (tt_joint), all
- *                      enqueued on `stream` (no persistent joint kernel).
+ *   TT_BATCH_JOINT:    joint reconstruction (tt_joint per tree) in one
+ *                      persistent launch over (tree, tile) items: max-plus
+ *                      postorder, root and backtrack; states identical to
+ *                      tt_joint, logL to rounding.  tt_tree_trim on a member
+ *                      drops the plan (it holds the joint buffers).
  * logL is host [k] (NULL: no read-back). */
 #define TT_BATCH_MARGINAL 0x00u
 #define TT_BATCH_JOINT 0x10u

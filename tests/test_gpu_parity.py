@@ -214,7 +214,14 @@ def test_batch_matches_plain(gpu_engine, tips_flag):
         t.close()
 
 
-def test_batch_run_modes(gpu_engine):
+@pytest.fixture
+def block_batches(monkeypatch):
+    """Plans built while this fixture is active use the persistent kernel
+    (TT_BATCH_MODE=block), whatever the tuner would pick."""
+    monkeypatch.setenv("TT_BATCH_MODE", "block")
+
+
+def test_batch_run_modes(gpu_engine, block_batches):
     """tt_batch_run: 'loglik' (persistent launch without the preorder) equals
     tt_loglik per tree and leaves profiles stale; 'joint' equals tt_joint per
     tree; 'marginal' equals tt_batch_marginal."""
@@ -234,12 +241,19 @@ def test_batch_run_modes(gpu_engine):
     assert batch_info(gpu_engine, trees, kind="loglik")["blocks_per_cu"] >= 0
     with pytest.raises(MissingDataError):  # profiles are stale after a loglik pass
         trees[0].optimize_branch_lengths_marginal(-1, 4.0, 1e-10)
-    joint_plain = [t.joint() for t in trees]
-    joint_states = [t.states() for t in trees]
+    for tips in (False, True):
+        joint_plain = [t.joint(reconstruct_tips=tips) for t in trees]
+        joint_states = [t.states() for t in trees]
+        for rep in range(2):
+            got = batch_run(gpu_engine, trees, "joint", reconstruct_tips=tips)
+            for i, t in enumerate(trees):
+                assert abs(got[i] - joint_plain[i]) <= 1e-12 * abs(joint_plain[i]), (tips, i, got[i], joint_plain[i])
+                assert np.array_equal(t.states(), joint_states[i]), (tips, i)
+    # freeing the joint buffers drops the joint plan; the next batched joint rebuilds it
+    for t in trees:
+        t.trim()
     got = batch_run(gpu_engine, trees, "joint")
-    for i, t in enumerate(trees):
-        assert got[i] == joint_plain[i]
-        assert np.array_equal(t.states(), joint_states[i])
+    assert np.allclose(got, [t.joint() for t in trees], rtol=1e-12, atol=0)
     got_m = batch_run(gpu_engine, trees, "marginal")
     ref_m = batch_marginal(gpu_engine, trees)
     assert np.array_equal(got_m, ref_m)

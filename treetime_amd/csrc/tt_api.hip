@@ -97,10 +97,14 @@ struct BatchPlan {
     hipGraphExec_t graph = nullptr;
     BatchShape shape{0, 1, 1, 0};
     double tuned_ms = 0.0;  // time per launch of the chosen shape while tuning (0: not timed)
+    bool joint = false;     // joint reconstruction plan (k_batch_joint)
     const std::vector<BatchItem> &items_for(const BatchShape &c) const { return c.tiles == 2 ? items2 : items1; }
     const BatchItem *d_items_for(const BatchShape &c) const { return c.tiles == 2 ? d_items2 : d_items1; }
-    hipError_t launch(int n, int tips, const BatchShape &c, hipStream_t s) const {
-        return ttk::batch_marginal(n, d_trees, d_items_for(c), (int)items_for(c).size(), d_queue, d_item_out, tips,
+    hipError_t launch(int n, int mode, const BatchShape &c, hipStream_t s) const {
+        if (joint)
+            return ttk::batch_joint(n, d_trees, d_items_for(c), (int)items_for(c).size(), d_queue, d_item_out, mode,
+                                    c, s);
+        return ttk::batch_marginal(n, d_trees, d_items_for(c), (int)items_for(c).size(), d_queue, d_item_out, mode,
                                    c, s);
     }
     ~BatchPlan() {
@@ -406,6 +410,8 @@ int32_t ensure_joint(tt_tree *T) {
 constexpr uint32_t kFillTips = 0x100u;
 // plan flag of a loglik-only (postorder + root) batched pass
 constexpr uint32_t kPlanLoglik = 2u;
+// plan flag of a joint batched pass
+constexpr uint32_t kPlanJoint = 4u;
 
 int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
     const TreeDev &D = T->d;
@@ -618,9 +624,9 @@ constexpr int kPerTreeGraphs = 2;
 // mode: bit 0 reconstruct tips, bit 1 loglik only (postorder and root)
 int32_t launch_plan(BatchPlan *P, int n, int mode, const BatchShape &c, hipStream_t s) {
     if (c.wave_items == kPerTreeGraphs) {
+        const int pass = P->joint ? 2 : ((mode & 2) ? 3 : 1);
         for (tt_tree *T : P->trees)
-            if (int32_t rc = run_pass(T, (mode & 2) ? 3 : 1, (mode & 1) ? TT_RECONSTRUCT_TIPS : 0u, nullptr, s))
-                return rc;
+            if (int32_t rc = run_pass(T, pass, (mode & 1) ? TT_RECONSTRUCT_TIPS : 0u, nullptr, s)) return rc;
         return TT_OK;
     }
     TT_HIP(P->launch(n, mode, c, s));
@@ -654,7 +660,8 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     for (BatchShape kd : kinds) {
         if (!allowed(kd.wave_items)) continue;
         if (!ttk::batch_all_shapes(n) && !(kd.wave_items == 0 && kd.tiles == 1)) continue;
-        const int occ = ttk::batch_max_blocks_per_cu(n, kd);
+        if (P->joint && !(kd.wave_items == 0 && kd.tiles == 1)) continue;  // the joint kernel: one-tile blocks
+        const int occ = P->joint ? ttk::batch_joint_max_blocks_per_cu(n) : ttk::batch_max_blocks_per_cu(n, kd);
         std::vector<int> per_cu;
         if (bpc && atoi(bpc) > 0) {
             per_cu.push_back(atoi(bpc));
@@ -1269,6 +1276,18 @@ int32_t tt_tree_trim(tt_tree *T) {
     TT_HIP(hipSetDevice(T->eng->device));
     TT_HIP(hipStreamSynchronize(T->stream));
     TT_HIP(hipDeviceSynchronize());
+    {  // joint batch plans hold the freed pointers too
+        std::lock_guard<std::mutex> g(T->eng->mu);
+        auto &pl = T->eng->plans;
+        for (size_t i = 0; i < pl.size();) {
+            if ((pl[i]->flags & kPlanJoint) && std::find(pl[i]->trees.begin(), pl[i]->trees.end(), T) != pl[i]->trees.end()) {
+                delete pl[i];
+                pl.erase(pl.begin() + i);
+            } else {
+                ++i;
+            }
+        }
+    }
     for (auto it = T->graphs.begin(); it != T->graphs.end();) {  // joint graphs hold the freed pointers
         if (it->first / 8 == 2) {
             (void)hipGraphExecDestroy(it->second);
@@ -1420,10 +1439,13 @@ int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t
                       int32_t *tiles, int32_t *blocks_per_cu, int32_t *waves_per_simd, double *tuned_ms) {
     if (!eng || (k > 0 && !trees)) return fail(TT_EINVAL, "NULL argument");
     std::vector<tt_tree *> key(trees, trees + std::max(k, 0));
+    const uint32_t kind = flags & 0x30u;
+    const uint32_t want = kind == TT_BATCH_LOGLIK ? kPlanLoglik
+                          : kind == TT_BATCH_JOINT ? (kPlanJoint | (flags & TT_RECONSTRUCT_TIPS))
+                                                   : (flags & TT_RECONSTRUCT_TIPS);
     std::lock_guard<std::mutex> g(eng->mu);
     for (BatchPlan *p : eng->plans)
-        if (p->flags == (((flags & 0x30u) == TT_BATCH_LOGLIK) ? kPlanLoglik : (flags & TT_RECONSTRUCT_TIPS)) &&
-            p->trees == key) {
+        if (p->flags == want && p->trees == key) {
             if (wave_items) *wave_items = p->shape.wave_items;
             if (tiles) *tiles = p->shape.tiles;
             if (blocks_per_cu) *blocks_per_cu = p->shape.blocks_per_cu;
@@ -1438,8 +1460,10 @@ int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t
 
 namespace {
 
-int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, bool loglik, double *logL,
+// kind: 0 marginal, 1 loglik (postorder + root), 2 joint
+int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, int kind, double *logL,
                        void *stream) {
+    const bool loglik = kind == 1, joint = kind == 2;
     if (!eng || (k > 0 && !trees)) return fail(TT_EINVAL, "NULL argument");
     if (k <= 0) return TT_OK;
     const int n = trees[0] ? trees[0]->n : 0;
@@ -1452,8 +1476,11 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
         if (T->site) return fail(TT_EINVAL, "trees[%d] has a site-specific model: run it with tt_marginal", i);
     }
     TT_HIP(hipSetDevice(eng->device));
+    if (joint)
+        for (int i = 0; i < k; ++i)
+            if (int32_t rc = ensure_joint(trees[i])) return rc;
     const uint32_t fl = loglik ? 0u : (flags & TT_RECONSTRUCT_TIPS);
-    const uint32_t pflags = fl | (loglik ? kPlanLoglik : 0u);
+    const uint32_t pflags = fl | (loglik ? kPlanLoglik : 0u) | (joint ? kPlanJoint : 0u);
     const int mode = (fl ? 1 : 0) | (loglik ? 2 : 0);
     std::vector<tt_tree *> key(trees, trees + k);
     BatchPlan *plan = nullptr;
@@ -1467,6 +1494,7 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
         P->eng = eng;
         P->trees = key;
         P->flags = pflags;
+        P->joint = joint;
         // items, longest tree first (LPT), tiles in ascending order within a tree
         std::vector<std::pair<long, int>> order;
         for (int i = 0; i < k; ++i) {
@@ -1566,10 +1594,10 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
             trees[i]->profiles_fresh = false;
             continue;
         }
-        trees[i]->last_pass = 1;
+        trees[i]->last_pass = joint ? 2 : 1;
         trees[i]->have_states = true;
         trees[i]->tips_dirty = fl != 0;
-        trees[i]->profiles_fresh = true;
+        if (!joint) trees[i]->profiles_fresh = true;
     }
     if (logL && plan->shape.wave_items == kPerTreeGraphs) {
         for (int i = 0; i < k; ++i)
@@ -1592,34 +1620,16 @@ extern "C" {
 
 int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, double *logL,
                           void *stream) {
-    return batch_plan_run(eng, trees, k, flags, false, logL, stream);
+    return batch_plan_run(eng, trees, k, flags, 0, logL, stream);
 }
 
 int32_t tt_batch_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t mode, double *logL, void *stream) {
     const uint32_t kind = mode & 0x30u;
     if (mode & ~(0x30u | TT_RECONSTRUCT_TIPS)) return fail(TT_EINVAL, "unknown mode bits 0x%x", mode);
-    if (kind == TT_BATCH_MARGINAL) return batch_plan_run(eng, trees, k, mode, false, logL, stream);
-    if (kind == TT_BATCH_LOGLIK) return batch_plan_run(eng, trees, k, 0u, true, logL, stream);
-    if (kind != TT_BATCH_JOINT) return fail(TT_EINVAL, "unknown pass kind 0x%x", kind);
-    // joint: each tree's own captured joint pass, all on one stream
-    if (!eng || (k > 0 && !trees)) return fail(TT_EINVAL, "NULL argument");
-    for (int i = 0; i < k; ++i) {
-        if (!trees[i]) return fail(TT_EINVAL, "trees[%d] is NULL", i);
-        if (trees[i]->eng != eng) return fail(TT_EINVAL, "trees[%d] belongs to another engine", i);
-    }
-    for (int i = 0; i < k; ++i)
-        if (int32_t rc = run_pass(trees[i], 2, mode & TT_RECONSTRUCT_TIPS, nullptr, stream)) return rc;
-    if (logL) {
-        hipStream_t s = (hipStream_t)stream;
-        for (int i = 0; i < k; ++i)
-            TT_HIP(hipMemcpyAsync(&logL[i], trees[i]->d.total, sizeof(double), hipMemcpyDeviceToHost,
-                                  s ? s : trees[i]->stream));
-        if (s)
-            TT_HIP(hipStreamSynchronize(s));
-        else
-            for (int i = 0; i < k; ++i) TT_HIP(hipStreamSynchronize(trees[i]->stream));
-    }
-    return TT_OK;
+    if (kind == TT_BATCH_MARGINAL) return batch_plan_run(eng, trees, k, mode, 0, logL, stream);
+    if (kind == TT_BATCH_LOGLIK) return batch_plan_run(eng, trees, k, 0u, 1, logL, stream);
+    if (kind == TT_BATCH_JOINT) return batch_plan_run(eng, trees, k, mode, 2, logL, stream);
+    return fail(TT_EINVAL, "unknown pass kind 0x%x", kind);
 }
 
 namespace {

@@ -237,6 +237,75 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal_w(const BatchTreeDe
     }
 }
 
+// Joint reconstruction of many trees in one persistent launch
+// (treeanc.py:934-1080 per tree): a block of 4 waves pops a (tree, 64-pattern
+// tile) item and walks its max-plus postorder levels (FP64 messages, argmax
+// table), the root and the integer backtrack down the depths, the level's
+// nodes split over the waves with a block barrier between levels.  The
+// per-(node, pattern) arithmetic is the level kernels' (post_joint_node,
+// root_joint_step), so states are identical to tt_joint and logL agrees to
+// the order of the per-pattern normaliser sum.  mode bit 0: reconstruct tips.
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_batch_joint(const BatchTreeDev *__restrict__ trees_arg,
+                                                         const BatchItem *__restrict__ items_arg, int n_items,
+                                                         int *queue, double *item_out, int mode) {
+    const BatchTreeDev TT_C *trees = (const BatchTreeDev TT_C *)trees_arg;
+    const BatchItem TT_C *items = (const BatchItem TT_C *)items_arg;
+    const int tips = mode & 1;
+    __shared__ int s_item;
+    __shared__ double s_off[kWaves][TT_WAVE];
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= n_items) break;
+        const BatchItem item = items[it];
+        const BatchTreeDev TT_C &B = trees[item.tree];
+        const TreeDev TT_C &T = B.T;
+        const int p = item.tile * TT_WAVE + lane;
+        const bool act = p < T.P;
+        double off = 0.0;
+        // postorder: heights 1..H-1 (the last level is the root)
+        for (int h = 0; h + 1 < B.H; ++h) {
+            const int kb = B.post_off[h], ke = B.post_off[h + 1];
+            for (int k = wave_uniform(kb + w); k < ke; k += kWaves) {
+                const int v = wave_uniform(B.post_nodes[k]);
+                const int vn = wave_uniform(T.int_node[v]);
+                if (act) off += post_joint_node<N>(T, v, vn, p, tips);
+            }
+            __syncthreads();
+        }
+        s_off[w][lane] = off;
+        __syncthreads();
+        double contrib = 0.0;
+        if (w == 0 && act) {
+            double o = 0.0;
+#pragma unroll
+            for (int q = 0; q < kWaves; ++q) o += s_off[q][lane];
+            contrib = root_joint_step<N>(T, p, o, tips);
+        }
+        const double tot = block_sum(contrib);
+        if (threadIdx.x == 0) item_out[it] = tot;
+        if (!tips && B.fill_tips && act)
+            for (int k = w; k < T.n_tips; k += kWaves) T.states[(size_t)B.tip_nodes[k] * T.P + p] = 255;
+        __syncthreads();
+        // backtrack (treeanc.py:1048-1063): depths 1..D, parents before children
+        for (int d = 0; d < B.D; ++d) {
+            const int kb = B.pre_off[d];
+            const int ke = tips ? B.pre_off[d + 1] : kb + B.pre_nint[d];
+            for (int k = wave_uniform(kb + w); k < ke; k += kWaves) {
+                const int c = wave_uniform(B.pre_nodes[k]);
+                if (act) {
+                    const int ps = T.states[(size_t)T.parent[c] * T.P + p];
+                    T.states[(size_t)c * T.P + p] = T.Cx[((size_t)c * T.P + p) * N + ps];
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Wide alphabets (n = 20: 256 VGPRs with spills already at one tile) get only
 // the one-tile block kernel; the other shapes are instantiated for n <= 8.
 constexpr bool all_shapes(int n) { return n <= 8; }
@@ -284,6 +353,29 @@ int ttk::device_cus() {
 }
 
 bool ttk::batch_all_shapes(int n) { return all_shapes(n); }
+
+hipError_t ttk::batch_joint(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
+                            double *item_out, int mode, const BatchShape &shape, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    const int resident = device_cus() * std::max(1, shape.blocks_per_cu);
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, queue);
+    TT_BATCH_DISPATCH(n, {
+        const int grid = std::min(resident, n_items);
+        hipLaunchKernelGGL(k_batch_joint<N_>, dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items, queue,
+                           item_out, mode);
+    });
+    return hipGetLastError();
+}
+
+int ttk::batch_joint_max_blocks_per_cu(int n) {
+    auto f = [&]() -> int {
+        int per_cu = 0;
+        TT_BATCH_DISPATCH(n, (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_joint<N_>,
+                                                                                  TT_BLOCK, 0));
+        return std::max(1, per_cu);
+    };
+    return f();
+}
 
 int ttk::batch_max_blocks_per_cu(int n, const BatchShape &c) {
     auto f = [&]() -> int {

@@ -171,121 +171,6 @@ __global__ __launch_bounds__(TT_BLOCK) void k_pre_run(TreeDev T, RunDev R) {
     }
 }
 
-// Joint message of a tip child (treeanc.py:970-978, 997-1011):
-// Lx[j] = max_i log P[i][j] + log max(profile_i, TINY); Cx[j] = first argmax.
-template <int N>
-__device__ __forceinline__ void tip_joint(const TreeDev &T, int cref, int cnode, int p, double (&msg)[N],
-                                          int store_cx) {
-    const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
-    const auto ll = T.loglut + code * N;
-    const auto LP = T.logP64 + (size_t)cnode * (N * N);
-    const auto cxo = T.Cx + ((size_t)cnode * T.P + p) * N;
-    double tl[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) tl[i] = ll[i];
-    int cx[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        double best = LP[j] + tl[0];
-        int bi = 0;
-#pragma unroll
-        for (int i = 1; i < N; ++i) {
-            const double val = LP[i * N + j] + tl[i];
-            if (val > best) {
-                best = val;
-                bi = i;
-            }
-        }
-        msg[j] += best;
-        if constexpr (N > 8) {
-            if (store_cx) cxo[j] = (uint8_t)bi;  // large alphabets: no packed argmax row in registers
-        } else {
-            cx[j] = bi;
-        }
-    }
-    if constexpr (N <= 8) {
-        if (store_cx) store_bytes<N>(cxo, cx);
-    }
-}
-
-template <int N>
-__device__ __forceinline__ void joint_children(const TreeDev &T, int v, int p, double (&msg)[N], int store_tip_cx) {
-    const int e0 = T.child_ptr[v], e1 = T.child_ptr[v + 1];
-    for (int e = e0; e < e1; ++e) {
-        const int cref = T.child_ref[e];
-        if (cref < 0) {
-            tip_joint<N>(T, cref, T.child_node[e], p, msg, store_tip_cx);
-        } else {
-            const auto src = T.Lx + ((size_t)cref * T.P + p) * N;
-            double r[N];
-#pragma unroll
-            for (int i = 0; i < N; ++i) r[i] = src[i];
-#pragma unroll
-            for (int i = 0; i < N; ++i) msg[i] += r[i];
-        }
-    }
-}
-
-// Joint postorder step for internal non-root row v with node id vn.  The
-// stored message is shifted by an offset (its maximum for small alphabets,
-// its first entry for large ones, where rows are written out state by state
-// to keep register pressure low); the offset goes to the FP64 normaliser sum.
-template <int N>
-__device__ __forceinline__ double post_joint_node(const TreeDev &T, int v, int vn, int p, int store_tip_cx) {
-    double msg[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) msg[i] = 0.0;
-    joint_children<N>(T, v, p, msg, store_tip_cx);
-    const auto LP = T.logP64 + (size_t)vn * (N * N);
-    const auto dst = T.Lx + ((size_t)v * T.P + p) * N;
-    const auto cxo = T.Cx + ((size_t)vn * T.P + p) * N;
-    if constexpr (N > 8) {
-        double off = 0.0;
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            double b = LP[j] + msg[0];
-            int bi = 0;
-#pragma unroll
-            for (int i = 1; i < N; ++i) {
-                const double val = LP[i * N + j] + msg[i];
-                if (val > b) {
-                    b = val;
-                    bi = i;
-                }
-            }
-            if (j == 0) off = b;
-            dst[j] = b - off;
-            cxo[j] = (uint8_t)bi;
-        }
-        return off;
-    } else {
-        double best[N];
-        int cx[N];
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            double b = LP[j] + msg[0];
-            int bi = 0;
-#pragma unroll
-            for (int i = 1; i < N; ++i) {
-                const double val = LP[i * N + j] + msg[i];
-                if (val > b) {
-                    b = val;
-                    bi = i;
-                }
-            }
-            best[j] = b;
-            cx[j] = bi;
-        }
-        double off = best[0];
-#pragma unroll
-        for (int j = 1; j < N; ++j) off = fmax(off, best[j]);
-#pragma unroll
-        for (int j = 0; j < N; ++j) dst[j] = best[j] - off;
-        store_bytes<N>(cxo, cx);
-        return off;
-    }
-}
-
 template <int N>
 __global__ __launch_bounds__(TT_BLOCK) void k_post_joint(TreeDev T, LevelDev L, int n_ptiles, int store_tip_cx) {
     __shared__ double s_acc[TT_BLOCK / TT_WAVE][TT_WAVE];
@@ -314,32 +199,15 @@ __global__ __launch_bounds__(TT_BLOCK) void k_post_joint(TreeDev T, LevelDev L, 
     }
 }
 
-// Joint root (treeanc.py:1015-1037): Lx_root = sum of children + log pi,
-// state = first argmax, sequence_LH = Lx_root[state] + normalisers.
+// Joint root (treeanc.py:1015-1037), one thread per pattern.
 template <int N>
 __global__ __launch_bounds__(TT_BLOCK) void k_root_joint(TreeDev T, int S, int store_tip_cx) {
     const int p = blockIdx.x * TT_BLOCK + threadIdx.x;
     double contrib = 0.0;
     if (p < T.P) {
-        double msg[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) msg[i] = 0.0;
-        joint_children<N>(T, T.root_row, p, msg, store_tip_cx);
-        int jm = 0;
-        double mx = -DBL_MAX;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            msg[i] += log(T.pi[i]);
-            if (msg[i] > mx) {
-                mx = msg[i];
-                jm = i;
-            }
-        }
-        double lp = mx;
-        for (int k = 0; k < S; ++k) lp += T.part2[(size_t)k * T.P + p];
-        T.jseqlh[p] = lp;
-        T.states[(size_t)T.root * T.P + p] = (uint8_t)jm;
-        contrib = T.mult[p] * lp;
+        double off = 0.0;
+        for (int k = 0; k < S; ++k) off += T.part2[(size_t)k * T.P + p];
+        contrib = root_joint_step<N>(T, p, off, store_tip_cx);
     }
     const double b = block_sum(contrib);
     if (threadIdx.x == 0) T.bpart[blockIdx.x] = b;
