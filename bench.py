@@ -116,6 +116,35 @@ def algorithmic_bytes(prep):
     return P * (n_int_br * (18 * n + 7) + n_tip_br * (2 * n + 3) + 4 + 4 * n)
 
 
+def joint_algorithmic_bytes(prep):
+    """Minimal compulsory HBM bytes of one joint pass with FP64 messages: per
+    pattern, 17n+3 per internal-child branch (the child's message row written
+    and read back, 8n each; its argmax row written, n; the backtrack reads the
+    parent state and one argmax byte and writes the state, 3), 1 per tip
+    branch (the character code), plus 8 (weights) and 9 (root: logL, state)."""
+    n = prep["n_states"]
+    parent, is_tip = tree_arrays(prep)
+    nonroot = parent >= 0
+    n_tip_br = int((is_tip & nonroot).sum())
+    n_int_br = int((~is_tip & nonroot).sum())
+    P = prep["tip_codes"].shape[1]
+    return P * (n_int_br * (17 * n + 3) + n_tip_br + 8 + 9)
+
+
+def loglik_algorithmic_bytes(prep):
+    """Minimal compulsory HBM bytes of one log-likelihood pass (postorder +
+    root): per pattern, 6n+2 per internal-child branch (the child's row read,
+    4n, half of the parent's row write, 2n+2), 2n+3 per tip branch, plus 8
+    (weights), 4n (root row) and 8 (per-pattern logL)."""
+    n = prep["n_states"]
+    parent, is_tip = tree_arrays(prep)
+    nonroot = parent >= 0
+    n_tip_br = int((is_tip & nonroot).sum())
+    n_int_br = int((~is_tip & nonroot).sum())
+    P = prep["tip_codes"].shape[1]
+    return P * (n_int_br * (6 * n + 2) + n_tip_br * (2 * n + 3) + 16 + 4 * n)
+
+
 def oracle_marginal(prep):
     """FP64 oracle (CPU baseline / checker) on one prepared job."""
     return oracle_tree(prep).marginal()
@@ -461,6 +490,8 @@ def main():
     log(f"[bench] {len(trees)} trees resident ({eng.usage()[0] / 2**30:.1f} GiB) in {upload_s:.1f}s")
     U = sum(updates(p) for p in preps)
     B = sum(algorithmic_bytes(p) for p in preps)
+    BJ = sum(joint_algorithmic_bytes(p) for p in preps)
+    BL = sum(loglik_algorithmic_bytes(p) for p in preps)
     P_total = sum(p["tip_codes"].shape[1] for p in preps)
 
     ceilings = copy_ceiling(torch)
@@ -597,6 +628,10 @@ def main():
             "batched_loglik_value": U / (ms_ll / 1e3),
             "joint_ms_per_step": ms_joint,
             "joint_value": U / (ms_joint / 1e3),
+            "joint_algorithmic_bytes_per_step": int(BJ),
+            "joint_achieved_GBs": BJ / (ms_joint / 1e3) / 1e9,
+            "loglik_algorithmic_bytes_per_step": int(BL),
+            "loglik_achieved_GBs": BL / (ms_ll / 1e3) / 1e9,
             "plain_value": U / (ms_plain / 1e3),
             "batched_vs_plain": ms_plain / ms,
             "batched_vs_plain_logL_max_rel_diff": rel,
