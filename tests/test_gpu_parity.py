@@ -1,7 +1,7 @@
 """Parity of the HIP passes (through the C ABI) with the FP64 oracle.
 
 Tolerances (FP32 storage, FP64 arithmetic and normaliser sums):
-  total logL            relative error <= 1e-7   (north-star bound; measured ~1e-10)
+  total logL            relative error <= 1e-7   (measured ~1e-10)
   per-pattern logL      error <= 5e-7 * max(1, |logL_p|)  (FP32 arithmetic)
   marginal profiles     max absolute error <= 2e-6
   argmax states         bit-exact wherever the FP64 margin between the two
