@@ -343,3 +343,40 @@ def test_site_five_states_with_gaps(gpu_engine):
         t_ref = og.optimal_t_compressed((pp, pc), mult, profiles=True, tol=1e-10)
         assert abs(t[v] - t_ref) <= 1e-4 * t_ref + 1e-8, (v, t[v], t_ref)
     dev.close()
+
+
+def test_site_model_argument_errors(gpu_engine):
+    """tt_tree_set_site_model refuses malformed models (ValueError) and
+    leaves the tree usable."""
+    prep, g, ta, nodes = site_case(20, 60, 13, True)
+    dev = device(gpu_engine, prep, g)
+    eig, V, Vi, mu, pi, grid, rs = g.device_model()
+    with pytest.raises(ValueError):
+        dev.set_site_model(eig, V, Vi, mu, pi, grid[:1], rs)          # one grid point
+    with pytest.raises(ValueError):
+        dev.set_site_model(eig, V, Vi, mu, pi, grid[::-1].copy(), rs)  # grid not increasing
+    bad_pi = pi.copy()
+    bad_pi[3, 1] = 0.0
+    with pytest.raises(ValueError):
+        dev.set_site_model(eig, V, Vi, mu, bad_pi, grid, rs)          # zero frequency (log pi at the root)
+    with pytest.raises(ValueError):
+        dev.set_site_model(eig, V, Vi, -mu, pi, grid, rs)             # negative rate
+    dev.set_site_model(eig, V, Vi, mu, pi, grid, rs)
+    assert abs(dev.marginal() - ta.marginal()) < 1e-7 * abs(ta.marginal())
+    dev.close()
+
+
+def test_batch_run_refuses_bad_mode(gpu_engine):
+    import ctypes as C
+
+    from treetime_amd import _native
+
+    prep, g, ta, nodes = site_case(10, 40, 14, True)
+    dev = device(gpu_engine, prep, g)
+    arr = (C.c_void_p * 1)(dev.handle.value)
+    assert gpu_engine.lib.tt_batch_run(gpu_engine.handle, arr, 1, 0x40, None, None) == _native.TT_EINVAL
+    assert gpu_engine.lib.tt_batch_run(gpu_engine.handle, arr, 1, 0x30, None, None) == _native.TT_EINVAL
+    with pytest.raises(ValueError):   # site-specific trees are not batched
+        from treetime_amd.engine import batch_run
+        batch_run(gpu_engine, [dev], "loglik")
+    dev.close()
