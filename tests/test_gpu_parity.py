@@ -325,3 +325,46 @@ def test_engine_lifetime_and_growing_updates():
     dt.close()
     assert eng.usage()[0] < used
     eng.close()
+
+
+
+def test_batch_run_twenty_states(gpu_engine, block_batches):
+    """tt_batch_run on 20-state trees against the FP64 oracle. The batched
+    kernel multiplies by P row by row while the plain 20-state passes use
+    matrix-core tiles, so the two GPU paths agree only to FP32 rounding; each
+    is held to the oracle's tolerances."""
+    from treetime_amd.engine import batch_run
+
+    say("[batch] 20-state trees: marginal / loglik / joint against the oracle")
+    specs = [(40, 300, 31), (9, 70, 32), (120, 500, 33)]
+    jobs = [small_job(n, L, 20, seed, 0.3, 0.0, 0.0) for n, L, seed in specs]
+    trees = [device_tree(gpu_engine, prep) for _, prep, _ in jobs]
+    oracles = [build_oracle(prep, alphabet, job["gtr"]) for job, prep, alphabet in jobs]
+    ref_m = [ta.marginal() for ta, _, _ in oracles]
+    got = batch_run(gpu_engine, trees, "marginal")
+    for i, t in enumerate(trees):
+        assert abs(got[i] - ref_m[i]) <= LOGL_RTOL * abs(ref_m[i]), (i, got[i], ref_m[i])
+        prof, states = t.profiles(), t.states()
+        for j, nd in enumerate(oracles[i][1]):
+            if nd.is_terminal():
+                continue
+            assert np.max(np.abs(prof[j] - nd.marginal_profile)) <= PROF_ATOL, (i, j)
+            ok = marginal_margin(nd.marginal_profile) > MARGIN
+            assert np.array_equal(states[j][ok], nd.seq_idx[ok]), (i, j)
+    got = batch_run(gpu_engine, trees, "loglik")
+    for i, t in enumerate(trees):
+        assert abs(got[i] - ref_m[i]) <= LOGL_RTOL * abs(ref_m[i]), (i, got[i], ref_m[i])
+        assert site_err(t.sequence_lh(), oracles[i][0].sequence_LH) <= SITE_RTOL
+    got = batch_run(gpu_engine, trees, "joint")
+    for i, t in enumerate(trees):
+        ta, nodes, _ = oracles[i]
+        ref = ta.joint()
+        assert abs(got[i] - ref) <= LOGL_RTOL * abs(ref), (i, got[i], ref)
+        states = t.states()
+        for j, nd in enumerate(nodes):
+            if nd.is_terminal():
+                continue
+            ok = nd.path_margin > MARGIN
+            assert np.array_equal(states[j][ok], nd.seq_idx[ok]), (i, j)
+    for t in trees:
+        t.close()
