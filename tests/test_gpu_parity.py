@@ -368,3 +368,74 @@ def test_batch_run_twenty_states(gpu_engine, block_batches):
             assert np.array_equal(states[j][ok], nd.seq_idx[ok]), (i, j)
     for t in trees:
         t.close()
+
+
+def _edge_jobs():
+    """Degenerate shapes: two tips, one site, all-ambiguous and all-gap
+    columns, identical sequences (constant columns only), zero and very long
+    branches, a 20-state pair."""
+    from treetime_amd.prepare import prepare
+
+    def reprep(job):
+        prep = prepare(job["parent"], job["branch_length"], job["chars"], job["gtr"], job["ambiguous"])
+        return job, prep, "nuc_nogap" if job["gtr"].n_states == 4 else "aa_nogap"
+
+    out = [("two-tips-one-site", small_job(2, 1, 4, 71, 0.05, 0.0, 0.0)),
+           ("two-tips-aa", small_job(2, 50, 20, 72, 0.3, 0.0, 0.0))]
+    job = small_job(20, 100, 4, 73, 0.05)[0]
+    job["chars"][:, 0] = b"N"
+    job["chars"][:, 1] = b"-"
+    job["chars"][:5, 2] = b"N"
+    job["chars"][3, :] = b"N"
+    out.append(("ambiguous-columns-and-row", reprep(job)))
+    job = small_job(30, 200, 4, 74, 0.05, 0.0, 0.0)[0]
+    job["chars"][:] = job["chars"][0]
+    out.append(("identical-sequences", reprep(job)))
+    job = small_job(40, 300, 4, 75, 0.05, 0.0, 0.0)[0]
+    bl = job["branch_length"].copy()
+    bl[1::7] = 0.0
+    bl[2::11] = 5.0
+    job["branch_length"] = bl
+    out.append(("zero-and-long-branches", reprep(job)))
+    return out
+
+
+def test_edge_shapes(gpu_engine):
+    """Degenerate inputs through the plain and the batched passes, against
+    the oracle at the stated tolerances."""
+    from treetime_amd.engine import batch_run
+
+    jobs = _edge_jobs()
+    trees, refs = [], []
+    for name, (job, prep, alphabet) in jobs:
+        say(f"[edge] {name}: P={prep['tip_codes'].shape[1]} nodes={len(prep['parent'])}")
+        ta, nodes, tips = build_oracle(prep, alphabet, job["gtr"])
+        dt = device_tree(gpu_engine, prep)
+        ref = ta.marginal()
+        got = dt.marginal()
+        assert abs(got - ref) <= LOGL_RTOL * abs(ref), (name, got, ref)
+        assert site_err(dt.sequence_lh(), ta.sequence_LH) <= SITE_RTOL, name
+        prof, states = dt.profiles(), dt.states()
+        for i, nd in enumerate(nodes):
+            if nd.is_terminal():
+                continue
+            assert np.max(np.abs(prof[i] - nd.marginal_profile)) <= PROF_ATOL, (name, i)
+            ok = marginal_margin(nd.marginal_profile) > MARGIN
+            assert np.array_equal(states[i][ok], nd.seq_idx[ok]), (name, i)
+        ref_j = ta.joint()
+        got_j = dt.joint()
+        assert abs(got_j - ref_j) <= LOGL_RTOL * abs(ref_j), (name, got_j, ref_j)
+        states = dt.states()
+        for i, nd in enumerate(nodes):
+            if not nd.is_terminal():
+                ok = nd.path_margin > MARGIN
+                assert np.array_equal(states[i][ok], nd.seq_idx[ok]), (name, i)
+        trees.append(dt)
+        refs.append((ref, ref_j))
+    nuc = [i for i, (_, (job, _, _)) in enumerate(jobs) if job["gtr"].n_states == 4]
+    for kind, col in (("marginal", 0), ("loglik", 0), ("joint", 1)):
+        got = batch_run(gpu_engine, [trees[i] for i in nuc], kind)
+        for g, i in zip(got, nuc):
+            assert abs(g - refs[i][col]) <= LOGL_RTOL * abs(refs[i][col]), (kind, jobs[i][0], g, refs[i][col])
+    for t in trees:
+        t.close()
