@@ -284,8 +284,9 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
  *                      its preorder skipped (per tree as tt_loglik: logL and
  *                      sequence_LH; profiles and states are left stale).
  *   TT_BATCH_JOINT:    joint reconstruction (tt_joint per tree) in one
- *                      persistent launch over (tree, tile) items: max-plus
- *                      postorder, root and backtrack; states identical to
+ *                      persistent launch over (tree, tile) items:
+ *                      max-product postorder, root, backtrack and the
+ *                      log-likelihood of the decoded states; states identical to
  *                      tt_joint, logL to rounding.  tt_tree_trim on a member
  *                      drops the plan (it holds the joint buffers).
  * logL is host [k] (NULL: no read-back). */

@@ -156,6 +156,7 @@ struct tt_tree {
     int32_t *d_tmp_br = nullptr;
     int tmp_cap = 0;
     bool joint_ready = false;
+    bool joint_site = false;  // the joint buffers hold site-specific (FP64 log) messages
     bool update_mode = false;  // postorder also stores per-node normalisers (d.offs)
     bool offs_valid = false;   // d.offs and the subtree rows match the current lengths
     hipStream_t stream = nullptr;
@@ -394,12 +395,20 @@ int rowsum_slices(int R) {
     return std::min(R, s);
 }
 
+// Joint buffers: FP32 product-space messages for a shared model, FP64 log
+// messages for a site-specific one (reallocated when the model kind changed).
 int32_t ensure_joint(tt_tree *T) {
-    if (T->joint_ready) return TT_OK;
     int32_t rc;
+    if (T->joint_ready && T->joint_site == T->site) return TT_OK;
+    if (T->joint_ready && (rc = tt_tree_trim(T))) return rc;
     size_t rows = (size_t)T->n_int * T->P * T->n;
-    if ((rc = T->alloc((void **)&T->d.Lx, rows * sizeof(double)))) return rc;
+    if (T->site)
+        rc = T->alloc((void **)&T->d.Lx64, rows * sizeof(double));
+    else
+        rc = T->alloc((void **)&T->d.Lx, rows * sizeof(float));
+    if (rc) return rc;
     if ((rc = T->alloc((void **)&T->d.Cx, (size_t)T->n_nodes * T->P * T->n))) return rc;
+    T->joint_site = T->site;
     T->joint_ready = true;
     return TT_OK;
 }
@@ -470,16 +479,28 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
                         L.row_off};
             TT_HIP(T->site ? ttk::post_joint_site(D, ld, tips ? 1 : 0, s) : ttk::post_joint(D, ld, tips ? 1 : 0, s));
         }
-        const int S = rowsum_slices(T->R_joint);
-        TT_HIP(ttk::rowsum(D, T->R_joint, S, s));
-        TT_HIP(T->site ? ttk::root_joint_site(D, S, tips ? 1 : 0, s) : ttk::root_joint(D, S, tips ? 1 : 0, s));
-        TT_HIP(ttk::final_sum(D, nb, s));
+        if (T->site) {  // logL from the root and the node normalisers
+            const int S = rowsum_slices(T->R_joint);
+            TT_HIP(ttk::rowsum(D, T->R_joint, S, s));
+            TT_HIP(ttk::root_joint_site(D, S, tips ? 1 : 0, s));
+            TT_HIP(ttk::final_sum(D, nb, s));
+        } else {
+            TT_HIP(ttk::root_joint(D, tips ? 1 : 0, s));
+        }
         if (fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
         for (auto &L : T->pre_lv) {
             LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off),
                             (decltype(LevelDev::desc))(T->d_pre_desc + kPreDesc * L.off), tips ? L.n_all : L.n_int, 1,
                             0, 0};
             TT_HIP(ttk::backtrack(D, ld, s));
+        }
+        if (!T->site) {  // logL: FP64 score of the decoded states, slices of G node ids (S <= R_all rows)
+            const int G = std::max(16, (T->n_nodes + std::max(T->R_all, 1) - 1) / std::max(T->R_all, 1));
+            const int S = (T->n_nodes + G - 1) / G, S2 = rowsum_slices(S);
+            TT_HIP(ttk::joint_score(D, S, G, s));
+            TT_HIP(ttk::rowsum(D, S, S2, s));
+            TT_HIP(ttk::joint_final(D, S2, s));
+            TT_HIP(ttk::final_sum(D, nb, s));
         }
     }
     return TT_OK;
@@ -551,7 +572,7 @@ int32_t enable_update_mode(tt_tree *T) {
 
 int32_t refresh_expqt(tt_tree *T, const int32_t *d_branches, const double *d_t, int k, hipStream_t s) {
     TT_HIP(ttk::expqt(T->n, d_t, d_branches, k, T->d_eig, T->d_v, T->d_vinv, T->mu, (double *)T->d.P64,
-                      (double *)T->d.logP64, (float *)T->d.P32, (float *)T->d.D32, s));
+                      (double *)T->d.Pj64, (double *)T->d.logP64, (float *)T->d.P32, (float *)T->d.D32, s));
     return TT_OK;
 }
 
@@ -656,12 +677,17 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
         const int want = mode[0] == 'b' ? 0 : mode[0] == 'w' ? 1 : mode[0] == 'p' ? kPerTreeGraphs : -1;
         return wave_items == want;
     };
-    const BatchShape kinds[] = {{0, 0, 2, 0}, {0, 0, 2, 5}, {0, 0, 1, 0}, {1, 0, 1, 0}};
+    // the joint kernel has one-tile blocks, with the compiler's register
+    // budget or one that fits 6 or 8 waves per SIMD
+    const std::vector<BatchShape> kinds =
+        P->joint ? std::vector<BatchShape>{{0, 0, 1, 0}, {0, 0, 1, 6}, {0, 0, 1, 8}}
+                 : std::vector<BatchShape>{{0, 0, 2, 0}, {0, 0, 2, 5}, {0, 0, 1, 0}, {1, 0, 1, 0}};
     for (BatchShape kd : kinds) {
         if (!allowed(kd.wave_items)) continue;
-        if (!ttk::batch_all_shapes(n) && !(kd.wave_items == 0 && kd.tiles == 1)) continue;
-        if (P->joint && !(kd.wave_items == 0 && kd.tiles == 1)) continue;  // the joint kernel: one-tile blocks
-        const int occ = P->joint ? ttk::batch_joint_max_blocks_per_cu(n) : ttk::batch_max_blocks_per_cu(n, kd);
+        if (!ttk::batch_all_shapes(n) && !(kd.wave_items == 0 && kd.tiles == 1 && !(P->joint && kd.waves_per_simd)))
+            continue;
+        const int occ = P->joint ? ttk::batch_joint_max_blocks_per_cu(n, kd.waves_per_simd)
+                                 : ttk::batch_max_blocks_per_cu(n, kd);
         std::vector<int> per_cu;
         if (bpc && atoi(bpc) > 0) {
             per_cu.push_back(atoi(bpc));
@@ -751,8 +777,11 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
         return fail(TT_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(he));
     }
     const int n = T->n, P = T->P, nn = T->n_nodes;
-    std::vector<double> loglut((size_t)D->n_codes * n);
-    for (size_t i = 0; i < loglut.size(); ++i) loglut[i] = std::log(std::max((double)D->code_lut[i], TT_TINY));
+    std::vector<double> lutj((size_t)D->n_codes * n), loglut(lutj.size());
+    for (size_t i = 0; i < lutj.size(); ++i) {
+        lutj[i] = std::max((double)D->code_lut[i], TT_TINY);
+        loglut[i] = std::log(lutj[i]);
+    }
     std::vector<int32_t> all(nn);
     for (int i = 0; i < nn; ++i) all[i] = i;
     TreeDev &d = T->d;
@@ -764,7 +793,7 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     int32_t *p_child_ptr, *p_child_ref, *p_child_node, *p_parent, *p_int_row, *p_tip_row, *p_int_node;
     uint8_t *p_codes;
     float *p_lut;
-    double *p_loglut, *p_mult, *p_pi;
+    double *p_lutj, *p_loglut, *p_mult, *p_pi;
     UP(p_child_ptr, T->child_ptr.data(), T->child_ptr.size());
     UP(p_child_ref, T->child_ref.data(), T->child_ref.size());
     UP(p_child_node, T->child_node.data(), T->child_node.size());
@@ -774,6 +803,7 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     UP(p_int_node, T->int_node.data(), T->n_int);
     UP(p_codes, D->tip_codes, ncodes_total);
     UP(p_lut, D->code_lut, (size_t)D->n_codes * n);
+    UP(p_lutj, lutj.data(), lutj.size());
     UP(p_loglut, loglut.data(), loglut.size());
     UP(p_mult, D->multiplicity, P);
     UP(p_pi, D->pi, n);
@@ -798,7 +828,8 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
             const int e0 = T->child_ptr[r], nc = T->child_ptr[r + 1] - e0;
             const bool bin = nc == 2;
             post_desc.insert(post_desc.end(), {r, e0, nc, bin ? T->child_ref[e0] : 0, bin ? T->child_ref[e0 + 1] : 0,
-                                               bin ? T->child_node[e0] : 0, bin ? T->child_node[e0 + 1] : 0, 0});
+                                               bin ? T->child_node[e0] : 0, bin ? T->child_node[e0 + 1] : 0,
+                                               T->int_node[r]});
         }
         for (int c : T->pre_nodes) pre_desc.insert(pre_desc.end(), {c, T->parent[c], T->int_row[c], T->tip_row[c]});
         if (post_desc.empty()) post_desc.assign(kPostDesc, 0);
@@ -827,6 +858,7 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     d.int_node = (decltype(d.int_node))p_int_node;
     d.tip_codes = (decltype(d.tip_codes))p_codes;
     d.lut = (decltype(d.lut))p_lut;
+    d.lutj = (decltype(d.lutj))p_lutj;
     d.loglut = (decltype(d.loglut))p_loglut;
     d.mult = (decltype(d.mult))p_mult;
     d.pi = (decltype(d.pi))p_pi;
@@ -835,6 +867,7 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
         size_t bytes;
     } bufs[] = {
         {(void **)&d.P64, (size_t)nn * n * n * sizeof(double)},
+        {(void **)&d.Pj64, (size_t)nn * n * n * sizeof(double)},
         {(void **)&d.logP64, (size_t)nn * n * n * sizeof(double)},
         {(void **)&d.P32, (size_t)nn * n * n * sizeof(float)},
         {(void **)&d.D32, (size_t)nn * n * n * sizeof(float)},
@@ -1079,7 +1112,8 @@ int32_t tt_update_branch(tt_tree *T, int32_t node, double t, double *logL, void 
     T->t[node] = t;
     TT_HIP(hipMemcpyAsync(T->d_t + node, &T->t[node], sizeof(double), hipMemcpyHostToDevice, s));
     TT_HIP(ttk::expqt(T->n, T->d_t + node, T->d_all_branches + node, 1, T->d_eig, T->d_v, T->d_vinv, T->mu,
-                      (double *)T->d.P64, (double *)T->d.logP64, (float *)T->d.P32, (float *)T->d.D32, s));
+                      (double *)T->d.P64, (double *)T->d.Pj64, (double *)T->d.logP64, (float *)T->d.P32,
+                      (float *)T->d.D32, s));
     TT_HIP(ttk::path_update(T->d, node, s));
     TT_HIP(ttk::final_sum(T->d, ttk::root_blocks(T->P), s));
     T->last_pass = 3;
@@ -1296,7 +1330,8 @@ int32_t tt_tree_trim(tt_tree *T) {
             ++it;
         }
     }
-    for (void *q : {(void *)T->d.Lx, (void *)T->d.Cx}) {
+    for (void *q : {(void *)T->d.Lx, (void *)T->d.Lx64, (void *)T->d.Cx}) {
+        if (!q) continue;
         for (size_t i = 0; i < T->allocs.size(); ++i)
             if (T->allocs[i].first == q) {
                 (void)hipFree(q);
@@ -1307,6 +1342,7 @@ int32_t tt_tree_trim(tt_tree *T) {
             }
     }
     T->d.Lx = nullptr;
+    T->d.Lx64 = nullptr;
     T->d.Cx = nullptr;
     T->joint_ready = false;
     if (T->last_pass == 2) T->last_pass = 0;

@@ -239,22 +239,30 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal_w(const BatchTreeDe
 
 // Joint reconstruction of many trees in one persistent launch
 // (treeanc.py:934-1080 per tree): a block of 4 waves pops a (tree, 64-pattern
-// tile) item and walks its max-plus postorder levels (FP64 messages, argmax
-// table), the root and the integer backtrack down the depths, the level's
-// nodes split over the waves with a block barrier between levels.  The
-// per-(node, pattern) arithmetic is the level kernels' (post_joint_node,
-// root_joint_step), so states are identical to tt_joint and logL agrees to
-// the order of the per-pattern normaliser sum.  mode bit 0: reconstruct tips.
-template <int N>
-__global__ __launch_bounds__(TT_BLOCK) void k_batch_joint(const BatchTreeDev *__restrict__ trees_arg,
+// tile) item and walks its max-product postorder levels (FP32 messages, FP64
+// arithmetic, argmax table), the root and the integer backtrack down the
+// depths, the level's nodes split over the waves with a block barrier between
+// levels.  The backtrack also sums the FP64 log-space score of the decoded
+// states (every branch once, tips included); the per-(node, pattern)
+// arithmetic is the level kernels' (post_joint_node, root_joint_state,
+// joint_int_term, joint_tip_term), so states are identical to tt_joint and
+// logL agrees to the order of the per-pattern sum.  mode bit 0: reconstruct
+// tips.
+// WPE: waves per SIMD the register allocator must allow (1 = no constraint).
+template <int N, int WPE>
+__global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_batch_joint(const BatchTreeDev *__restrict__ trees_arg,
                                                          const BatchItem *__restrict__ items_arg, int n_items,
                                                          int *queue, double *item_out, int mode) {
     const BatchTreeDev TT_C *trees = (const BatchTreeDev TT_C *)trees_arg;
     const BatchItem TT_C *items = (const BatchItem TT_C *)items_arg;
     const int tips = mode & 1;
+    constexpr bool kStage = N <= 8;  // profile maps staged in LDS (n_codes <= 256 rows of N doubles each)
     __shared__ int s_item;
-    __shared__ double s_off[kWaves][TT_WAVE];
+    __shared__ double s_sc[kWaves][TT_WAVE];
+    __shared__ double s_lj[kStage ? 256 * N : 1];
+    __shared__ double s_ll[kStage ? 256 * N : 1];
     const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    int cur_tree = -1;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
         __syncthreads();
@@ -263,46 +271,115 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_joint(const BatchTreeDev *__
         const BatchItem item = items[it];
         const BatchTreeDev TT_C &B = trees[item.tree];
         const TreeDev TT_C &T = B.T;
-        const int p = item.tile * TT_WAVE + lane;
-        const bool act = p < T.P;
-        double off = 0.0;
-        // postorder: heights 1..H-1 (the last level is the root)
-        for (int h = 0; h + 1 < B.H; ++h) {
-            const int kb = B.post_off[h], ke = B.post_off[h + 1];
-            for (int k = wave_uniform(kb + w); k < ke; k += kWaves) {
-                const int v = wave_uniform(B.post_nodes[k]);
-                const int vn = wave_uniform(T.int_node[v]);
-                if (act) off += post_joint_node<N>(T, v, vn, p, tips);
+        if constexpr (kStage) {
+            if (item.tree != cur_tree) {
+                for (int i = threadIdx.x; i < T.n_codes * N; i += TT_BLOCK) {
+                    s_lj[i] = T.lutj[i];
+                    s_ll[i] = T.loglut[i];
+                }
+                cur_tree = item.tree;
             }
             __syncthreads();
         }
-        s_off[w][lane] = off;
-        __syncthreads();
-        double contrib = 0.0;
-        if (w == 0 && act) {
-            double o = 0.0;
-#pragma unroll
-            for (int q = 0; q < kWaves; ++q) o += s_off[q][lane];
-            contrib = root_joint_step<N>(T, p, o, tips);
-        }
-        const double tot = block_sum(contrib);
-        if (threadIdx.x == 0) item_out[it] = tot;
-        if (!tips && B.fill_tips && act)
-            for (int k = w; k < T.n_tips; k += kWaves) T.states[(size_t)B.tip_nodes[k] * T.P + p] = 255;
-        __syncthreads();
-        // backtrack (treeanc.py:1048-1063): depths 1..D, parents before children
-        for (int d = 0; d < B.D; ++d) {
-            const int kb = B.pre_off[d];
-            const int ke = tips ? B.pre_off[d + 1] : kb + B.pre_nint[d];
-            for (int k = wave_uniform(kb + w); k < ke; k += kWaves) {
-                const int c = wave_uniform(B.pre_nodes[k]);
-                if (act) {
-                    const int ps = T.states[(size_t)T.parent[c] * T.P + p];
-                    T.states[(size_t)c * T.P + p] = T.Cx[((size_t)c * T.P + p) * N + ps];
+        auto tip_term = [&](int c, int code, int sp) {
+            if constexpr (kStage)
+                return joint_tip_term_sel<N>(T, s_ll, c, code, sp);
+            else
+                return joint_tip_term<N>(T, T.loglut, c, code, sp);
+        };
+        const int p = item.tile * TT_WAVE + lane;
+        const bool act = p < T.P;
+        // postorder: heights 1..H-1 (the last level is the root), two nodes per step
+        for (int h = 0; h + 1 < B.H; ++h) {
+            const int kb = B.post_off[h], ke = B.post_off[h + 1];
+            for (int k = wave_uniform(kb + w); k < ke; k += 2 * kWaves) {
+                const bool has1 = k + kWaves < ke;
+                const PostDesc d0 = load_post_desc(B.post_desc + kPostDesc * k);
+                const PostDesc d1 = has1 ? load_post_desc(B.post_desc + kPostDesc * (k + kWaves)) : d0;
+                if (!act) continue;
+                bool pair = false;
+                if constexpr (kStage) pair = d0.nchild == 2 && d1.nchild == 2;
+                if (pair) {
+                    if constexpr (kStage) {
+                        JointRaw<N> RA, RB;  // both fetched unconditionally so they stay in registers
+                        joint_fetch_d<N>(T, d0, p, RA);
+                        joint_fetch_d<N>(T, d1, p, RB);
+                        joint_finish_d<N>(T, d0, p, RA, s_lj, tips);
+                        if (has1) joint_finish_d<N>(T, d1, p, RB, s_lj, tips);
+                    }
+                } else {
+                    post_joint_node<N>(T, d0.v, d0.vn, p, tips);
+                    if (has1) post_joint_node<N>(T, d1.v, d1.vn, p, tips);
                 }
             }
             __syncthreads();
         }
+        if (w == 0 && act) (void)root_joint_state<N>(T, p, tips);
+        if (!tips && B.fill_tips && act)
+            for (int k = w; k < T.n_tips; k += kWaves) T.states[(size_t)B.tip_nodes[k] * T.P + p] = 255;
+        __syncthreads();
+        // backtrack (treeanc.py:1048-1063): depths 1..D, parents before
+        // children, kStep nodes per wave step; the score of the decoded states
+        // (every non-root branch once) rides along for internal children
+        constexpr int kStep = 4;
+        double sc = 0.0;
+        for (int d = 0; d < B.D; ++d) {
+            const int kb = B.pre_off[d], kint = kb + B.pre_nint[d];
+            const int ke = tips ? B.pre_off[d + 1] : kint;
+            for (int k = wave_uniform(kb + w); k < ke; k += kStep * kWaves) {
+                PreDesc q[kStep];
+#pragma unroll
+                for (int m = 0; m < kStep; ++m)
+                    q[m] = load_pre_desc(B.pre_desc + kPreDesc * (k + m * kWaves < ke ? k + m * kWaves : k));
+                if (!act) continue;
+                int ps[kStep], cs[kStep];
+#pragma unroll
+                for (int m = 0; m < kStep; ++m) ps[m] = T.states[(size_t)q[m].par * T.P + p];
+#pragma unroll
+                for (int m = 0; m < kStep; ++m) cs[m] = T.Cx[((size_t)q[m].c * T.P + p) * N + ps[m]];
+#pragma unroll
+                for (int m = 0; m < kStep; ++m)
+                    if (k + m * kWaves < ke) T.states[(size_t)q[m].c * T.P + p] = cs[m];
+#pragma unroll
+                for (int m = 0; m < kStep; ++m)
+                    if (k + m * kWaves < kint) sc += joint_int_term<N>(T, q[m].c, cs[m], ps[m]);
+            }
+            __syncthreads();
+        }
+        // tip branches (states final: no barriers)
+        if (act) {
+            for (int d = 0; d < B.D; ++d) {
+                const int kt = B.pre_off[d] + B.pre_nint[d], ke = B.pre_off[d + 1];
+                for (int k = wave_uniform(kt + w); k < ke; k += kStep * kWaves) {
+                    PreDesc q[kStep];
+#pragma unroll
+                    for (int m = 0; m < kStep; ++m)
+                        q[m] = load_pre_desc(B.pre_desc + kPreDesc * (k + m * kWaves < ke ? k + m * kWaves : k));
+                    int ps[kStep], code[kStep];
+#pragma unroll
+                    for (int m = 0; m < kStep; ++m) {
+                        ps[m] = T.states[(size_t)q[m].par * T.P + p];
+                        code[m] = T.tip_codes[(size_t)q[m].tr * T.P + p];
+                    }
+#pragma unroll
+                    for (int m = 0; m < kStep; ++m)
+                        if (k + m * kWaves < ke) sc += tip_term(q[m].c, code[m], ps[m]);
+                }
+            }
+        }
+        s_sc[w][lane] = sc;
+        __syncthreads();
+        double contrib = 0.0;
+        if (w == 0 && act) {
+            double lp = log(T.pi[T.states[(size_t)T.root * T.P + p]]);
+#pragma unroll
+            for (int q = 0; q < kWaves; ++q) lp += s_sc[q][lane];
+            T.jseqlh[p] = lp;
+            contrib = T.mult[p] * lp;
+        }
+        const double tot = block_sum(contrib);
+        if (threadIdx.x == 0) item_out[it] = tot;
+        __syncthreads();
     }
 }
 
@@ -361,17 +438,40 @@ hipError_t ttk::batch_joint(int n, const BatchTreeDev *trees, const BatchItem *i
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, queue);
     TT_BATCH_DISPATCH(n, {
         const int grid = std::min(resident, n_items);
-        hipLaunchKernelGGL(k_batch_joint<N_>, dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items, queue,
+        if constexpr (all_shapes(N_)) {
+            if (shape.waves_per_simd == 6) {
+                hipLaunchKernelGGL((k_batch_joint<N_, 6>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items,
+                                   queue, item_out, mode);
+                return hipGetLastError();
+            }
+            if (shape.waves_per_simd == 8) {
+                hipLaunchKernelGGL((k_batch_joint<N_, 8>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items,
+                                   queue, item_out, mode);
+                return hipGetLastError();
+            }
+        }
+        hipLaunchKernelGGL((k_batch_joint<N_, 1>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items, queue,
                            item_out, mode);
     });
     return hipGetLastError();
 }
 
-int ttk::batch_joint_max_blocks_per_cu(int n) {
+int ttk::batch_joint_max_blocks_per_cu(int n, int waves_per_simd) {
     auto f = [&]() -> int {
         int per_cu = 0;
-        TT_BATCH_DISPATCH(n, (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_joint<N_>,
-                                                                                  TT_BLOCK, 0));
+        TT_BATCH_DISPATCH(n, {
+            if constexpr (all_shapes(N_)) {
+                if (waves_per_simd == 6) {
+                    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_joint<N_, 6>, TT_BLOCK, 0);
+                    return std::max(1, per_cu);
+                }
+                if (waves_per_simd == 8) {
+                    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_joint<N_, 8>, TT_BLOCK, 0);
+                    return std::max(1, per_cu);
+                }
+            }
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_joint<N_, 1>, TT_BLOCK, 0);
+        });
         return std::max(1, per_cu);
     };
     return f();

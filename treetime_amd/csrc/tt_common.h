@@ -39,19 +39,22 @@ struct TreeDev {
     const int32_t TT_C *int_node;    // [n_int] node id of each internal row
     const uint8_t TT_C *tip_codes;   // [n_tips][P]
     const float TT_C *lut;           // [n_codes][n]   profile of each character code
-    const double TT_C *loglut;       // [n_codes][n]   log(max(profile, TINY)) for the joint pass
+    const double TT_C *lutj;         // [n_codes][n]   max(profile, TINY): joint messages
+    const double TT_C *loglut;       // [n_codes][n]   log(max(profile, TINY)): joint path scores
     const double TT_C *mult;         // [P]
     const double TT_C *pi;           // [n]
     const double TT_C *P64;          // [n_nodes][n][n]  expQt per branch, FP64
-    const double TT_C *logP64;       // [n_nodes][n][n]  log(max(TINY, expQt))
+    const double TT_C *Pj64;         // [n_nodes][n][n]  max(TINY, expQt): joint messages
+    const double TT_C *logP64;       // [n_nodes][n][n]  log(max(TINY, expQt)): joint path scores
     const float TT_C *P32;           // [n_nodes][n][n]  expQt, FP32
     const float TT_C *D32;           // [n_nodes][n][n]  expQt - I (computed in FP64), FP32
     float TT_G *sub;                 // [n_int][P][n]   subtree partials, max entry = 1
     float TT_G *prof;                // [n_nodes][P][n] marginal profiles (sum = 1)
     uint8_t TT_G *states;            // [n_nodes][P]
-    double TT_G *Lx;                 // [n_int][P][n]   joint messages, max entry = 0 (FP64)
+    float TT_G *Lx;                  // [n_int][P][n]   joint messages (max-product), max entry = 1
+    double TT_G *Lx64;               // [n_int][P][n]   site-specific joint messages (log), max entry = 0
     uint8_t TT_G *Cx;                // [n_nodes][P][n] joint argmax table
-    double TT_G *part;               // [R][P]          per-chunk-group FP64 log-normaliser sums
+    double TT_G *part;               // [R][P]          per-chunk-group FP64 log-normaliser (or joint score) sums
     double TT_G *part2;              // [S][P]          row-slice sums
     double TT_G *seqlh;              // [P]  marginal per-pattern logL (root-path updates start here)
     double TT_G *jseqlh;             // [P]  joint per-pattern logL
@@ -108,7 +111,8 @@ struct RunDev {
 namespace ttk {
 // launchers (tt_kernels.hip); n is the alphabet size
 hipError_t expqt(int n, const double *t, const int32_t *branches, int k, const double *eig, const double *v,
-                 const double *vinv, double mu, double *P64, double *logP64, float *P32, float *D32, hipStream_t s);
+                 const double *vinv, double mu, double *P64, double *Pj64, double *logP64, float *P32, float *D32,
+                 hipStream_t s);
 hipError_t post_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s);
 hipError_t rowsum(const TreeDev &T, int R, int S, hipStream_t s);
 hipError_t root_marginal(const TreeDev &T, int S, hipStream_t s);
@@ -129,7 +133,11 @@ hipError_t post_joint_site(const TreeDev &T, const LevelDev &L, int store_tip_cx
 hipError_t root_joint_site(const TreeDev &T, int S, int store_tip_cx, hipStream_t s);
 hipError_t propagate_site(const double *in, int P, int n, const double *Q, int transpose, int ret_log, double *out,
                           hipStream_t s);
-hipError_t root_joint(const TreeDev &T, int S, int store_tip_cx, hipStream_t s);
+hipError_t root_joint(const TreeDev &T, int store_tip_cx, hipStream_t s);
+// log-likelihood of the decoded joint states: slices of G node ids per row
+// of part (S rows), then jseqlh and the block partials from S2 row-slice sums
+hipError_t joint_score(const TreeDev &T, int S, int G, hipStream_t s);
+hipError_t joint_final(const TreeDev &T, int S2, hipStream_t s);
 hipError_t backtrack(const TreeDev &T, const LevelDev &L, hipStream_t s);
 hipError_t path_update(const TreeDev &T, int node, hipStream_t s);
 hipError_t fill_states(const TreeDev &T, const int32_t *nodes, int n_nodes, uint8_t value, hipStream_t s);
@@ -194,7 +202,7 @@ hipError_t batch_marginal(int n, const BatchTreeDev *trees, const BatchItem *ite
 // joint reconstruction of many trees in one persistent launch (one-tile block items)
 hipError_t batch_joint(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
                        double *item_out, int mode, const BatchShape &shape, hipStream_t s);
-int batch_joint_max_blocks_per_cu(int n);
+int batch_joint_max_blocks_per_cu(int n, int waves_per_simd);
 // occupancy limit (blocks per CU) of the batched kernel for n states and item shape; 0 if n unsupported
 int batch_max_blocks_per_cu(int n, const BatchShape &shape);
 // false when only the one-tile block kernel exists for n states

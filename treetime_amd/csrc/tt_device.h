@@ -10,6 +10,8 @@
 
 #include <float.h>
 
+#include <type_traits>
+
 namespace ttd {
 
 __device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -285,10 +287,10 @@ __device__ __forceinline__ void bin_finish(const TreeDev &T, int v, int e0, int 
 
 // A postorder descriptor (kPostDesc ints) read with one scalar load.
 struct PostDesc {
-    int v, e0, nchild, cref0, cref1, cnode0, cnode1;
+    int v, e0, nchild, cref0, cref1, cnode0, cnode1, vn;  // vn: node id of row v
 };
 __device__ __forceinline__ PostDesc load_post_desc(const int32_t TT_C *d) {
-    return PostDesc{d[0], d[1], d[2], d[3], d[4], d[5], d[6]};
+    return PostDesc{d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7]};
 }
 
 template <int N>
@@ -498,32 +500,47 @@ __device__ __forceinline__ void pre_pair(const TreeDev &T, int c0, int c1, bool 
     if (has1) pre_finish<N>(T, c1, p, RB, lut);
 }
 
-// Joint message of a tip child (treeanc.py:970-978, 997-1011):
-// Lx[j] = max_i log P[i][j] + log max(profile_i, TINY); Cx[j] = first argmax.
-template <int N>
-__device__ __forceinline__ void tip_joint(const TreeDev &T, int cref, int cnode, int p, double (&msg)[N],
-                                          int store_cx) {
-    const int code = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
-    const auto ll = T.loglut + code * N;
-    const auto LP = T.logP64 + (size_t)cnode * (N * N);
+// ---- joint (max-product) reconstruction -------------------------------------
+//
+// The reference works on logs (treeanc.py:960-1037): a message is
+// Lx_j = max_i log max(TINY, P_ij) + (sum of the children's log messages)_i.
+// Max and product commute with the log, so the same decisions are taken in
+// product space: messages are stored in FP32, divided by their largest entry
+// (which is therefore exactly 1), and every product and comparison is formed
+// in FP64 from the FP64 tables max(TINY, P) and max(profile, TINY).  A stored
+// entry is >= TINY (a row whose largest entry is 1, times max(TINY, P)), so
+// the product of two children stays far inside the FP64 range; from a
+// polytomy's third child on, the running product is rescaled by a power of
+// two after each child.  The messages only decide the states: the joint
+// log-likelihood is the FP64 log-space score of the decoded states
+// (joint_tip_term / joint_int_term), summed after the backtrack, so no FP32
+// rounding reaches it.
+
+// Tip child (treeanc.py:970-978, 997-1011): msg_j *= max_i max(TINY, P_ij)
+// max(profile_i, TINY); Cx[j] = the first argmax.
+template <int N, class LJ>
+__device__ __forceinline__ void tip_joint_code(const TreeDev &T, LJ lutj, int code, int cnode, int p,
+                                               double (&msg)[N], int store_cx) {
+    const auto lj = lutj + code * N;
+    const auto PJ = T.Pj64 + (size_t)cnode * (N * N);
     const auto cxo = T.Cx + ((size_t)cnode * T.P + p) * N;
     double tl[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) tl[i] = ll[i];
+    for (int i = 0; i < N; ++i) tl[i] = lj[i];
     int cx[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        double best = LP[j] + tl[0];
+        double best = PJ[j] * tl[0];
         int bi = 0;
 #pragma unroll
         for (int i = 1; i < N; ++i) {
-            const double val = LP[i * N + j] + tl[i];
+            const double val = PJ[i * N + j] * tl[i];
             if (val > best) {
                 best = val;
                 bi = i;
             }
         }
-        msg[j] += best;
+        msg[j] *= best;
         if constexpr (N > 8) {
             if (store_cx) cxo[j] = (uint8_t)bi;  // large alphabets: no packed argmax row in registers
         } else {
@@ -536,107 +553,192 @@ __device__ __forceinline__ void tip_joint(const TreeDev &T, int cref, int cnode,
 }
 
 template <int N>
+__device__ __forceinline__ void tip_joint(const TreeDev &T, int cref, int cnode, int p, double (&msg)[N],
+                                          int store_cx) {
+    tip_joint_code<N>(T, T.lutj, T.tip_codes[(size_t)(-cref - 1) * T.P + p], cnode, p, msg, store_cx);
+}
+
+// msg = product over the children of row v of their messages (FP64); a
+// polytomy's running product is rescaled by 2^-e (largest entry into
+// [0.5, 1)) after each child from the third on.
+template <int N>
 __device__ __forceinline__ void joint_children(const TreeDev &T, int v, int p, double (&msg)[N], int store_tip_cx) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) msg[i] = 1.0;
     const int e0 = T.child_ptr[v], e1 = T.child_ptr[v + 1];
     for (int e = e0; e < e1; ++e) {
         const int cref = T.child_ref[e];
         if (cref < 0) {
             tip_joint<N>(T, cref, T.child_node[e], p, msg, store_tip_cx);
         } else {
-            const auto src = T.Lx + ((size_t)cref * T.P + p) * N;
-            double r[N];
+            float r[N];
+            load_rowf<N>(T.Lx + ((size_t)cref * T.P + p) * N, r);
 #pragma unroll
-            for (int i = 0; i < N; ++i) r[i] = src[i];
+            for (int i = 0; i < N; ++i) msg[i] *= (double)r[i];
+        }
+        if (e - e0 >= 2) {
+            double mx = msg[0];
 #pragma unroll
-            for (int i = 0; i < N; ++i) msg[i] += r[i];
+            for (int i = 1; i < N; ++i) mx = fmax(mx, msg[i]);
+            int ex;
+            (void)frexp(mx, &ex);
+#pragma unroll
+            for (int i = 0; i < N; ++i) msg[i] = ldexp(msg[i], -ex);
         }
     }
 }
 
-// Joint postorder step for internal non-root row v with node id vn.  The
-// stored message is shifted by an offset (its maximum for small alphabets,
-// its first entry for large ones, where rows are written out state by state
-// to keep register pressure low); the offset goes to the FP64 normaliser sum.
+// Joint postorder step for internal non-root row v with node id vn
+// (treeanc.py:980-1011): Lx_j = max_i max(TINY, P_ij) msg_i, Cx[j] = the
+// first argmax; the row is stored divided by its largest entry.
 template <int N>
-__device__ __forceinline__ double post_joint_node(const TreeDev &T, int v, int vn, int p, int store_tip_cx) {
-    double msg[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) msg[i] = 0.0;
-    joint_children<N>(T, v, p, msg, store_tip_cx);
-    const auto LP = T.logP64 + (size_t)vn * (N * N);
+__device__ __forceinline__ void joint_store(const TreeDev &T, int v, int vn, int p, const double (&msg)[N]) {
+    const auto PJ = T.Pj64 + (size_t)vn * (N * N);
     const auto dst = T.Lx + ((size_t)v * T.P + p) * N;
     const auto cxo = T.Cx + ((size_t)vn * T.P + p) * N;
-    if constexpr (N > 8) {
-        double off = 0.0;
+    // large alphabets keep the maxima in FP32 registers (the largest one also
+    // in FP64, as the divisor) and write the argmax bytes one by one
+    using Best = std::conditional_t<(N > 8), float, double>;
+    Best best[N];
+    int cx[N];
+    double off = 0.0;
+    int jm = 0;
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            double b = LP[j] + msg[0];
-            int bi = 0;
+    for (int j = 0; j < N; ++j) {
+        double b = PJ[j] * msg[0];
+        int bi = 0;
 #pragma unroll
-            for (int i = 1; i < N; ++i) {
-                const double val = LP[i * N + j] + msg[i];
-                if (val > b) {
-                    b = val;
-                    bi = i;
-                }
+        for (int i = 1; i < N; ++i) {
+            const double val = PJ[i * N + j] * msg[i];
+            if (val > b) {
+                b = val;
+                bi = i;
             }
-            if (j == 0) off = b;
-            dst[j] = b - off;
-            cxo[j] = (uint8_t)bi;
         }
-        return off;
-    } else {
-        double best[N];
-        int cx[N];
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            double b = LP[j] + msg[0];
-            int bi = 0;
-#pragma unroll
-            for (int i = 1; i < N; ++i) {
-                const double val = LP[i * N + j] + msg[i];
-                if (val > b) {
-                    b = val;
-                    bi = i;
-                }
-            }
-            best[j] = b;
+        best[j] = (Best)b;
+        if (b > off) {
+            off = b;
+            jm = j;
+        }
+        if constexpr (N > 8) {
+            cxo[j] = (uint8_t)bi;
+        } else {
             cx[j] = bi;
         }
-        double off = best[0];
+    }
+    const double inv = 1.0 / off;
+    float out[N];
 #pragma unroll
-        for (int j = 1; j < N; ++j) off = fmax(off, best[j]);
+    for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : (float)((double)best[j] * inv);
+    store_row<N>(dst, out);
+    if constexpr (N <= 8) store_bytes<N>(cxo, cx);
+}
+
+template <int N>
+__device__ __forceinline__ void post_joint_node(const TreeDev &T, int v, int vn, int p, int store_tip_cx) {
+    double msg[N];
+    joint_children<N>(T, v, p, msg, store_tip_cx);
+    joint_store<N>(T, v, vn, p, msg);
+}
+
+// Binary joint steps split into fetch (the two children's rows or tip codes)
+// and finish, so that a wave issues the loads of two nodes before any
+// arithmetic (the batched kernel); same arithmetic as post_joint_node.
+template <int N>
+struct JointRaw {
+    int code[2];
+    float row[2][N];
+};
+
+template <int N>
+__device__ __forceinline__ void joint_fetch_d(const TreeDev &T, const PostDesc &d, int p, JointRaw<N> &R) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) dst[j] = best[j] - off;
-        store_bytes<N>(cxo, cx);
-        return off;
+    for (int c = 0; c < 2; ++c) {
+        const int cref = c ? d.cref1 : d.cref0;
+        if (cref < 0) {
+            R.code[c] = T.tip_codes[(size_t)(-cref - 1) * T.P + p];
+        } else {
+            R.code[c] = 0;
+            load_rowf<N>(T.Lx + ((size_t)cref * T.P + p) * N, R.row[c]);
+        }
     }
 }
 
-// Joint root for pattern p (treeanc.py:1015-1037): Lx_root = sum of the
-// children's messages + log pi, state = first argmax, sequence_LH =
-// Lx_root[state] + ``offsets`` (the node normalisers of this pattern).
-// Writes jseqlh and the root state; returns mult[p] * logL.
-template <int N>
-__device__ __forceinline__ double root_joint_step(const TreeDev &T, int p, double offsets, int store_tip_cx) {
+template <int N, class LJ>
+__device__ __forceinline__ void joint_finish_d(const TreeDev &T, const PostDesc &d, int p, const JointRaw<N> &R,
+                                               LJ lutj, int store_tip_cx) {
     double msg[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) msg[i] = 0.0;
+    for (int i = 0; i < N; ++i) msg[i] = 1.0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        if ((c ? d.cref1 : d.cref0) < 0) {
+            tip_joint_code<N>(T, lutj, R.code[c], c ? d.cnode1 : d.cnode0, p, msg, store_tip_cx);
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) msg[i] *= (double)R.row[c][i];
+        }
+    }
+    joint_store<N>(T, d.v, d.vn, p, msg);
+}
+
+// Joint root state for pattern p (treeanc.py:1015-1037): the first argmax of
+// pi times the product of the children's messages.
+template <int N>
+__device__ __forceinline__ int root_joint_state(const TreeDev &T, int p, int store_tip_cx) {
+    double msg[N];
     joint_children<N>(T, T.root_row, p, msg, store_tip_cx);
     int jm = 0;
-    double mx = -DBL_MAX;
+    double mx = -1.0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        msg[i] += log(T.pi[i]);
-        if (msg[i] > mx) {
-            mx = msg[i];
+        const double x = msg[i] * T.pi[i];
+        if (x > mx) {
+            mx = x;
             jm = i;
         }
     }
-    const double lp = mx + offsets;
-    T.jseqlh[p] = lp;
     T.states[(size_t)T.root * T.P + p] = (uint8_t)jm;
-    return T.mult[p] * lp;
+    return jm;
+}
+
+// Log-space score terms of the decoded states (the reference's joint
+// log-likelihood, treeanc.py:970-1037, evaluated on its own argmax path):
+// an internal child c in state sc under parent state sp contributes
+// log max(TINY, P_c)[sc][sp]; a tip contributes its message
+// max_i log max(TINY, P_c)[i][sp] + log max(profile_i, TINY), which is the
+// same number whether or not the tip's state is reconstructed.
+template <int N>
+__device__ __forceinline__ double joint_int_term(const TreeDev &T, int c, int sc, int sp) {
+    return T.logP64[(size_t)c * (N * N) + sc * N + sp];
+}
+
+template <int N, class LL>
+__device__ __forceinline__ double joint_tip_term(const TreeDev &T, LL loglut, int c, int code, int sp) {
+    const auto LP = T.logP64 + (size_t)c * (N * N) + sp;
+    const auto ll = loglut + code * N;
+    double b = -DBL_MAX;
+#pragma unroll
+    for (int i = 0; i < N; ++i) b = fmax(b, LP[i * N] + ll[i]);
+    return b;
+}
+
+// joint_tip_term for small alphabets with wave-uniform c: the branch's
+// log P block is read through the scalar unit and each lane selects its
+// column sp (no per-lane gathers); the same sums and maxima, so the same value.
+template <int N, class LL>
+__device__ __forceinline__ double joint_tip_term_sel(const TreeDev &T, LL loglut, int c, int code, int sp) {
+    const auto LP = T.logP64 + (size_t)c * (N * N);
+    const auto ll = loglut + code * N;
+    double b = -DBL_MAX;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double x = LP[i * N];
+#pragma unroll
+        for (int j = 1; j < N; ++j) x = (sp == j) ? LP[i * N + j] : x;
+        b = fmax(b, x + ll[i]);
+    }
+    return b;
 }
 
 // Deterministic block reduction of one double per thread (wave shuffles,

@@ -5,12 +5,12 @@
 //                               entry is exactly 1.0 (reference:
 //                               node.marginal_subtree_LH, treeanc.py:877)
 //   prof [n_nodes][P][n] float  marginal profiles, sum = 1 (treeanc.py:909-917)
-//   Lx   [n_int][P][n]   double joint messages, largest entry exactly 0
+//   Lx   [n_int][P][n]   float  joint (max-product) messages, largest entry exactly 1
 //   Cx   [n_nodes][P][n] uint8  joint argmax table (treeanc.py:996-1011)
 //   part [R][P]          double per-chunk-group sums of log normalisers
 //
 // Precision scheme: rows are stored in FP32, arithmetic is FP64.  Because a
-// stored row's dominant entry is exactly 1 (0 for the joint), the only FP32
+// stored row's dominant entry is exactly 1, the only FP32
 // rounding that reaches the likelihood is on entries weighted by their
 // (small) posterior, and the per-node log normalisers are accumulated in
 // FP64.  This keeps the total log-likelihood within ~1e-9 relative of the
@@ -173,41 +173,62 @@ __global__ __launch_bounds__(TT_BLOCK) void k_pre_run(TreeDev T, RunDev R) {
 
 template <int N>
 __global__ __launch_bounds__(TT_BLOCK) void k_post_joint(TreeDev T, LevelDev L, int n_ptiles, int store_tip_cx) {
-    __shared__ double s_acc[TT_BLOCK / TT_WAVE][TT_WAVE];
     const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
     const int pt = blockIdx.x % n_ptiles, cg = blockIdx.x / n_ptiles;
     const int p = pt * TT_WAVE + lane;
-    const bool act = p < T.P;
     const int chunk = cg * (TT_BLOCK / TT_WAVE) + w;
-    double acc = 0.0;
-    if (chunk < L.n_chunks) {
-        const int kb = chunk * L.chunk;
-        const int ke = min(kb + L.chunk, L.n_nodes);
-        for (int k = kb; k < ke; ++k) {
-            const int v = wave_uniform(L.nodes[k]);
-            const int vn = wave_uniform(T.int_node[v]);
-            if (act) acc += post_joint_node<N>(T, v, vn, p, store_tip_cx);
-        }
-    }
-    s_acc[w][lane] = acc;
-    __syncthreads();
-    if (w == 0 && act) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < TT_BLOCK / TT_WAVE; ++q) s += s_acc[q][lane];
-        T.part[(size_t)(L.row_off + cg) * T.P + p] = s;
+    if (chunk >= L.n_chunks || p >= T.P) return;
+    const int kb = chunk * L.chunk;
+    const int ke = min(kb + L.chunk, L.n_nodes);
+    for (int k = kb; k < ke; ++k) {
+        const int v = wave_uniform(L.nodes[k]);
+        post_joint_node<N>(T, v, wave_uniform(T.int_node[v]), p, store_tip_cx);
     }
 }
 
-// Joint root (treeanc.py:1015-1037), one thread per pattern.
+// Joint root state (treeanc.py:1015-1037), one thread per pattern.
 template <int N>
-__global__ __launch_bounds__(TT_BLOCK) void k_root_joint(TreeDev T, int S, int store_tip_cx) {
+__global__ __launch_bounds__(TT_BLOCK) void k_root_joint(TreeDev T, int store_tip_cx) {
+    const int p = blockIdx.x * TT_BLOCK + threadIdx.x;
+    if (p < T.P) (void)root_joint_state<N>(T, p, store_tip_cx);
+}
+
+// Score of the decoded joint states, part 1: the wave of (slice sl, pattern
+// tile) sums the terms of node ids [sl G, sl G + G) (every non-root branch
+// once) into part row sl.
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_joint_score(TreeDev T, int S, int G, int n_ptiles) {
+    const int lane = threadIdx.x & (TT_WAVE - 1);
+    const int task = blockIdx.x * (TT_BLOCK / TT_WAVE) + threadIdx.x / TT_WAVE;
+    const int pt = task % n_ptiles, sl = task / n_ptiles;
+    const int p = pt * TT_WAVE + lane;
+    if (sl >= S || p >= T.P) return;
+    const int v0 = sl * G, v1 = min(T.n_nodes, v0 + G);
+    double a = 0.0;
+    for (int v = wave_uniform(v0); v < v1; ++v) {
+        const int par = wave_uniform(T.parent[v]);
+        if (par < 0) continue;
+        const int sp = T.states[(size_t)par * T.P + p];
+        const int tr = wave_uniform(T.tip_row[v]);
+        if (tr >= 0)
+            a += joint_tip_term<N>(T, T.loglut, v, T.tip_codes[(size_t)tr * T.P + p], sp);
+        else
+            a += joint_int_term<N>(T, v, T.states[(size_t)v * T.P + p], sp);
+    }
+    T.part[(size_t)sl * T.P + p] = a;
+}
+
+// Part 2, one thread per pattern: jseqlh = log pi[root state] + the S2
+// row-slice sums; block partials of sum(mult * jseqlh).
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_joint_final(TreeDev T, int S2) {
     const int p = blockIdx.x * TT_BLOCK + threadIdx.x;
     double contrib = 0.0;
     if (p < T.P) {
-        double off = 0.0;
-        for (int k = 0; k < S; ++k) off += T.part2[(size_t)k * T.P + p];
-        contrib = root_joint_step<N>(T, p, off, store_tip_cx);
+        double lp = log(T.pi[T.states[(size_t)T.root * T.P + p]]);
+        for (int k = 0; k < S2; ++k) lp += T.part2[(size_t)k * T.P + p];
+        T.jseqlh[p] = lp;
+        contrib = T.mult[p] * lp;
     }
     const double b = block_sum(contrib);
     if (threadIdx.x == 0) T.bpart[blockIdx.x] = b;
@@ -271,11 +292,11 @@ __global__ void k_fill_states(TreeDev T, const int32_t *nodes, int n_nodes, uint
 }
 
 // P(t) for a list of branches (gtr.py:1027-1067): v diag(exp(mu t lambda))
-// v_inv, clipped at 0, plus log(max(TINY, P)) for the joint pass
+// v_inv, clipped at 0, plus max(TINY, P) and its log for the joint pass
 // (treeanc.py:970).
 __global__ void k_expqt(int n, const double *t, const int32_t *branches, int k, const double *eig,
-                        const double *v, const double *vinv, double mu, double *P64, double *logP64, float *P32,
-                        float *D32) {
+                        const double *v, const double *vinv, double mu, double *P64, double *Pj64, double *logP64,
+                        float *P32, float *D32) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int nn = n * n;
     if (gid >= k * nn) return;
@@ -286,6 +307,7 @@ __global__ void k_expqt(int n, const double *t, const int32_t *branches, int k, 
     const double P = fmax(0.0, acc);
     const size_t o = (size_t)branches[b] * nn + ij;
     P64[o] = P;
+    Pj64[o] = fmax(TT_TINY, P);
     logP64[o] = log(fmax(TT_TINY, P));
     P32[o] = (float)P;
     D32[o] = (float)(P - (i == j ? 1.0 : 0.0));  // deficit form keeps the diagonal's small part exact
@@ -360,11 +382,12 @@ bool mfma_enabled() {
 }
 
 hipError_t expqt(int n, const double *t, const int32_t *branches, int k, const double *eig, const double *v,
-                 const double *vinv, double mu, double *P64, double *logP64, float *P32, float *D32, hipStream_t s) {
+                 const double *vinv, double mu, double *P64, double *Pj64, double *logP64, float *P32, float *D32,
+                 hipStream_t s) {
     if (k <= 0) return hipSuccess;
     const int total = k * n * n;
     hipLaunchKernelGGL(k_expqt, dim3((total + 255) / 256), dim3(256), 0, s, n, t, branches, k, eig, v, vinv, mu,
-                       P64, logP64, P32, D32);
+                       P64, Pj64, logP64, P32, D32);
     return hipGetLastError();
 }
 
@@ -415,9 +438,22 @@ hipError_t pre_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t root_joint(const TreeDev &T, int S, int store_tip_cx, hipStream_t s) {
-    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_root_joint<N_>, dim3(root_blocks(T.P)), dim3(TT_BLOCK), 0, s, T, S,
+hipError_t root_joint(const TreeDev &T, int store_tip_cx, hipStream_t s) {
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_root_joint<N_>, dim3(root_blocks(T.P)), dim3(TT_BLOCK), 0, s, T,
                                         store_tip_cx));
+    return hipGetLastError();
+}
+
+hipError_t joint_score(const TreeDev &T, int S, int G, hipStream_t s) {
+    if (S <= 0) return hipSuccess;
+    const int npt = n_ptiles(T.P);
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_joint_score<N_>, dim3(grid_for_tasks((long)S * npt)), dim3(TT_BLOCK), 0,
+                                        s, T, S, G, npt));
+    return hipGetLastError();
+}
+
+hipError_t joint_final(const TreeDev &T, int S2, hipStream_t s) {
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_joint_final<N_>, dim3(root_blocks(T.P)), dim3(TT_BLOCK), 0, s, T, S2));
     return hipGetLastError();
 }
 

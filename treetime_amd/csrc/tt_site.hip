@@ -377,7 +377,7 @@ __device__ __forceinline__ void site_joint_children(const TreeDev &T, int e0, in
             for (int j = 0; j < N; ++j) msg[j] += best[j];
             if (store_tip_cx) store_bytes<N>(T.Cx + ((size_t)cnode * T.P + p) * N, cx);
         } else {
-            const auto src = T.Lx + ((size_t)cref * T.P + p) * N;
+            const auto src = T.Lx64 + ((size_t)cref * T.P + p) * N;
 #pragma unroll
             for (int i = 0; i < N; ++i) msg[i] += src[i];
         }
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(TT_BLOCK) void k_post_joint_site(TreeDev T, LevelDe
             double off = best[0];
 #pragma unroll
             for (int j = 1; j < N; ++j) off = fmax(off, best[j]);
-            const auto dst = T.Lx + ((size_t)v * T.P + p) * N;
+            const auto dst = T.Lx64 + ((size_t)v * T.P + p) * N;
 #pragma unroll
             for (int j = 0; j < N; ++j) dst[j] = best[j] - off;
             store_bytes<N>(T.Cx + ((size_t)vn * T.P + p) * N, cx);
