@@ -671,6 +671,7 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     const char *mode = getenv("TT_BATCH_MODE");
     const char *bpc = getenv("TT_BATCH_BLOCKS_PER_CU");
     const char *tune = getenv("TT_AUTOTUNE");
+    const char *jw = getenv("TT_JOINT_WAVES_PER_SIMD");  // joint plans: keep only this register budget
     const bool log = getenv("TT_AUTOTUNE_LOG") != nullptr;  // print every candidate's time to stderr
     auto allowed = [&](int wave_items) {
         if (!mode || !mode[0]) return true;
@@ -684,6 +685,7 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
                  : std::vector<BatchShape>{{0, 0, 2, 0}, {0, 0, 2, 5}, {0, 0, 1, 0}, {1, 0, 1, 0}};
     for (BatchShape kd : kinds) {
         if (!allowed(kd.wave_items)) continue;
+        if (P->joint && jw && jw[0] && kd.waves_per_simd != atoi(jw)) continue;
         if (!ttk::batch_all_shapes(n) && !(kd.wave_items == 0 && kd.tiles == 1 && !(P->joint && kd.waves_per_simd)))
             continue;
         const int occ = P->joint ? ttk::batch_joint_max_blocks_per_cu(n, kd.waves_per_simd)
