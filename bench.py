@@ -551,7 +551,7 @@ def main():
         log(f"[bench] batched loglik: {ms_ll:.3f} ms/step")
 
     # batched joint: every tree's joint pass on one stream
-    ms_joint = float("nan")
+    ms_joint = ms_joint_plain = float("nan")
     if args.plain_steps > 0:
         batch_run(eng, trees, "joint", stream=sp, want_logL=False)
         torch.cuda.synchronize()
@@ -563,6 +563,18 @@ def main():
         torch.cuda.synchronize()
         ms_joint = j0.elapsed_time(j1) / args.plain_steps
         log(f"[bench] joint, all trees: {ms_joint:.3f} ms/step")
+        # the same joint passes one tree at a time (each tree's captured graph)
+        for t in trees:
+            t.joint(stream=sp, want_logL=False)
+        torch.cuda.synchronize()
+        j0.record(stream)
+        for _ in range(args.plain_steps):
+            for t in trees:
+                t.joint(stream=sp, want_logL=False)
+        j1.record(stream)
+        torch.cuda.synchronize()
+        ms_joint_plain = j0.elapsed_time(j1) / args.plain_steps
+        log(f"[bench] joint, plain: {ms_joint_plain:.3f} ms/step")
         for t in trees:  # return the joint buffers (tens of GB on the mix) before the serving leg
             t.trim()
 
@@ -629,6 +641,7 @@ def main():
             "batched_loglik_ms_per_step": ms_ll,
             "batched_loglik_value": U / (ms_ll / 1e3),
             "joint_ms_per_step": ms_joint,
+            "plain_joint_ms_per_step": ms_joint_plain,
             "joint_value": U / (ms_joint / 1e3),
             "joint_algorithmic_bytes_per_step": int(BJ),
             "joint_achieved_GBs": BJ / (ms_joint / 1e3) / 1e9,

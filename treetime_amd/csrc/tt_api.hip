@@ -572,7 +572,7 @@ int32_t enable_update_mode(tt_tree *T) {
 
 int32_t refresh_expqt(tt_tree *T, const int32_t *d_branches, const double *d_t, int k, hipStream_t s) {
     TT_HIP(ttk::expqt(T->n, d_t, d_branches, k, T->d_eig, T->d_v, T->d_vinv, T->mu, (double *)T->d.P64,
-                      (double *)T->d.Pj64, (double *)T->d.logP64, (float *)T->d.P32, (float *)T->d.D32, s));
+                      (double *)T->d.Pj64, (double *)T->d.logP64, (float *)T->d.Pj32, (float *)T->d.P32, (float *)T->d.D32, s));
     return TT_OK;
 }
 
@@ -780,9 +780,11 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     }
     const int n = T->n, P = T->P, nn = T->n_nodes;
     std::vector<double> lutj((size_t)D->n_codes * n), loglut(lutj.size());
+    std::vector<float> lutj32(lutj.size());
     for (size_t i = 0; i < lutj.size(); ++i) {
         lutj[i] = std::max((double)D->code_lut[i], TT_TINY);
         loglut[i] = std::log(lutj[i]);
+        lutj32[i] = (float)lutj[i];
     }
     std::vector<int32_t> all(nn);
     for (int i = 0; i < nn; ++i) all[i] = i;
@@ -796,6 +798,7 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     uint8_t *p_codes;
     float *p_lut;
     double *p_lutj, *p_loglut, *p_mult, *p_pi;
+    float *p_lutj32;
     UP(p_child_ptr, T->child_ptr.data(), T->child_ptr.size());
     UP(p_child_ref, T->child_ref.data(), T->child_ref.size());
     UP(p_child_node, T->child_node.data(), T->child_node.size());
@@ -807,6 +810,7 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     UP(p_lut, D->code_lut, (size_t)D->n_codes * n);
     UP(p_lutj, lutj.data(), lutj.size());
     UP(p_loglut, loglut.data(), loglut.size());
+    UP(p_lutj32, lutj32.data(), lutj32.size());
     UP(p_mult, D->multiplicity, P);
     UP(p_pi, D->pi, n);
     UP(T->d_post_nodes, T->post_nodes.data(), T->post_nodes.size());
@@ -862,6 +866,7 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     d.lut = (decltype(d.lut))p_lut;
     d.lutj = (decltype(d.lutj))p_lutj;
     d.loglut = (decltype(d.loglut))p_loglut;
+    d.lutj32 = (decltype(d.lutj32))p_lutj32;
     d.mult = (decltype(d.mult))p_mult;
     d.pi = (decltype(d.pi))p_pi;
     struct {
@@ -871,6 +876,7 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
         {(void **)&d.P64, (size_t)nn * n * n * sizeof(double)},
         {(void **)&d.Pj64, (size_t)nn * n * n * sizeof(double)},
         {(void **)&d.logP64, (size_t)nn * n * n * sizeof(double)},
+        {(void **)&d.Pj32, (size_t)nn * n * n * sizeof(float)},
         {(void **)&d.P32, (size_t)nn * n * n * sizeof(float)},
         {(void **)&d.D32, (size_t)nn * n * n * sizeof(float)},
         {(void **)&d.sub, (size_t)T->n_int * P * n * sizeof(float)},
@@ -1114,7 +1120,7 @@ int32_t tt_update_branch(tt_tree *T, int32_t node, double t, double *logL, void 
     T->t[node] = t;
     TT_HIP(hipMemcpyAsync(T->d_t + node, &T->t[node], sizeof(double), hipMemcpyHostToDevice, s));
     TT_HIP(ttk::expqt(T->n, T->d_t + node, T->d_all_branches + node, 1, T->d_eig, T->d_v, T->d_vinv, T->mu,
-                      (double *)T->d.P64, (double *)T->d.Pj64, (double *)T->d.logP64, (float *)T->d.P32,
+                      (double *)T->d.P64, (double *)T->d.Pj64, (double *)T->d.logP64, (float *)T->d.Pj32, (float *)T->d.P32,
                       (float *)T->d.D32, s));
     TT_HIP(ttk::path_update(T->d, node, s));
     TT_HIP(ttk::final_sum(T->d, ttk::root_blocks(T->P), s));

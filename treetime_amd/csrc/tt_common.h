@@ -46,6 +46,8 @@ struct TreeDev {
     const double TT_C *P64;          // [n_nodes][n][n]  expQt per branch, FP64
     const double TT_C *Pj64;         // [n_nodes][n][n]  max(TINY, expQt): joint messages
     const double TT_C *logP64;       // [n_nodes][n][n]  log(max(TINY, expQt)): joint path scores
+    const float TT_C *Pj32;          // [n_nodes][n][n]  max(TINY, expQt) in FP32: joint messages, n > 8
+    const float TT_C *lutj32;        // [n_codes][n]   max(profile, TINY) in FP32: joint messages, n > 8
     const float TT_C *P32;           // [n_nodes][n][n]  expQt, FP32
     const float TT_C *D32;           // [n_nodes][n][n]  expQt - I (computed in FP64), FP32
     float TT_G *sub;                 // [n_int][P][n]   subtree partials, max entry = 1
@@ -111,8 +113,8 @@ struct RunDev {
 namespace ttk {
 // launchers (tt_kernels.hip); n is the alphabet size
 hipError_t expqt(int n, const double *t, const int32_t *branches, int k, const double *eig, const double *v,
-                 const double *vinv, double mu, double *P64, double *Pj64, double *logP64, float *P32, float *D32,
-                 hipStream_t s);
+                 const double *vinv, double mu, double *P64, double *Pj64, double *logP64, float *Pj32, float *P32,
+                 float *D32, hipStream_t s);
 hipError_t post_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s);
 hipError_t rowsum(const TreeDev &T, int R, int S, hipStream_t s);
 hipError_t root_marginal(const TreeDev &T, int S, hipStream_t s);

@@ -507,7 +507,8 @@ __device__ __forceinline__ void pre_pair(const TreeDev &T, int c0, int c1, bool 
 // Max and product commute with the log, so the same decisions are taken in
 // product space: messages are stored in FP32, divided by their largest entry
 // (which is therefore exactly 1), and every product and comparison is formed
-// in FP64 from the FP64 tables max(TINY, P) and max(profile, TINY).  A stored
+// from the tables max(TINY, P) and max(profile, TINY) (FP64, or FP32 for
+// n > 8, see JR).  A stored
 // entry is >= TINY (a row whose largest entry is 1, times max(TINY, P)), so
 // the product of two children stays far inside the FP64 range; from a
 // polytomy's third child on, the running product is rescaled by a power of
@@ -516,25 +517,50 @@ __device__ __forceinline__ void pre_pair(const TreeDev &T, int c0, int c1, bool 
 // (joint_tip_term / joint_int_term), summed after the backtrack, so no FP32
 // rounding reaches it.
 
+// Arithmetic of the joint messages: FP64 for small alphabets; FP32 for
+// large ones (n > 8), where the n^2 products and comparisons per child make
+// the pass VALU-bound.  The messages only decide the argmax states (the
+// log-likelihood is the FP64 score of those states), and FP32 keeps every
+// decision whose FP64 margin exceeds ~1e-6.
+template <int N>
+using JR = std::conditional_t<(N > 8), float, double>;
+
+template <int N>
+__device__ __forceinline__ const JR<N> TT_C *joint_P(const TreeDev &T) {
+    if constexpr (N > 8)
+        return T.Pj32;
+    else
+        return T.Pj64;
+}
+
+template <int N>
+__device__ __forceinline__ const JR<N> TT_C *joint_lut(const TreeDev &T) {
+    if constexpr (N > 8)
+        return T.lutj32;
+    else
+        return T.lutj;
+}
+
 // Tip child (treeanc.py:970-978, 997-1011): msg_j *= max_i max(TINY, P_ij)
 // max(profile_i, TINY); Cx[j] = the first argmax.
 template <int N, class LJ>
 __device__ __forceinline__ void tip_joint_code(const TreeDev &T, LJ lutj, int code, int cnode, int p,
-                                               double (&msg)[N], int store_cx) {
+                                               JR<N> (&msg)[N], int store_cx) {
+    using R = JR<N>;
     const auto lj = lutj + code * N;
-    const auto PJ = T.Pj64 + (size_t)cnode * (N * N);
+    const auto PJ = joint_P<N>(T) + (size_t)cnode * (N * N);
     const auto cxo = T.Cx + ((size_t)cnode * T.P + p) * N;
-    double tl[N];
+    R tl[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) tl[i] = lj[i];
     int cx[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        double best = PJ[j] * tl[0];
+        R best = PJ[j] * tl[0];
         int bi = 0;
 #pragma unroll
         for (int i = 1; i < N; ++i) {
-            const double val = PJ[i * N + j] * tl[i];
+            const R val = PJ[i * N + j] * tl[i];
             if (val > best) {
                 best = val;
                 bi = i;
@@ -553,18 +579,19 @@ __device__ __forceinline__ void tip_joint_code(const TreeDev &T, LJ lutj, int co
 }
 
 template <int N>
-__device__ __forceinline__ void tip_joint(const TreeDev &T, int cref, int cnode, int p, double (&msg)[N],
+__device__ __forceinline__ void tip_joint(const TreeDev &T, int cref, int cnode, int p, JR<N> (&msg)[N],
                                           int store_cx) {
-    tip_joint_code<N>(T, T.lutj, T.tip_codes[(size_t)(-cref - 1) * T.P + p], cnode, p, msg, store_cx);
+    tip_joint_code<N>(T, joint_lut<N>(T), T.tip_codes[(size_t)(-cref - 1) * T.P + p], cnode, p, msg, store_cx);
 }
 
 // msg = product over the children of row v of their messages (FP64); a
 // polytomy's running product is rescaled by 2^-e (largest entry into
 // [0.5, 1)) after each child from the third on.
 template <int N>
-__device__ __forceinline__ void joint_children(const TreeDev &T, int v, int p, double (&msg)[N], int store_tip_cx) {
+__device__ __forceinline__ void joint_children(const TreeDev &T, int v, int p, JR<N> (&msg)[N], int store_tip_cx) {
+    using R = JR<N>;
 #pragma unroll
-    for (int i = 0; i < N; ++i) msg[i] = 1.0;
+    for (int i = 0; i < N; ++i) msg[i] = R(1);
     const int e0 = T.child_ptr[v], e1 = T.child_ptr[v + 1];
     for (int e = e0; e < e1; ++e) {
         const int cref = T.child_ref[e];
@@ -574,10 +601,10 @@ __device__ __forceinline__ void joint_children(const TreeDev &T, int v, int p, d
             float r[N];
             load_rowf<N>(T.Lx + ((size_t)cref * T.P + p) * N, r);
 #pragma unroll
-            for (int i = 0; i < N; ++i) msg[i] *= (double)r[i];
+            for (int i = 0; i < N; ++i) msg[i] *= (R)r[i];
         }
         if (e - e0 >= 2) {
-            double mx = msg[0];
+            R mx = msg[0];
 #pragma unroll
             for (int i = 1; i < N; ++i) mx = fmax(mx, msg[i]);
             int ex;
@@ -592,30 +619,29 @@ __device__ __forceinline__ void joint_children(const TreeDev &T, int v, int p, d
 // (treeanc.py:980-1011): Lx_j = max_i max(TINY, P_ij) msg_i, Cx[j] = the
 // first argmax; the row is stored divided by its largest entry.
 template <int N>
-__device__ __forceinline__ void joint_store(const TreeDev &T, int v, int vn, int p, const double (&msg)[N]) {
-    const auto PJ = T.Pj64 + (size_t)vn * (N * N);
+__device__ __forceinline__ void joint_store(const TreeDev &T, int v, int vn, int p, const JR<N> (&msg)[N]) {
+    using R = JR<N>;
+    const auto PJ = joint_P<N>(T) + (size_t)vn * (N * N);
     const auto dst = T.Lx + ((size_t)v * T.P + p) * N;
     const auto cxo = T.Cx + ((size_t)vn * T.P + p) * N;
-    // large alphabets keep the maxima in FP32 registers (the largest one also
-    // in FP64, as the divisor) and write the argmax bytes one by one
-    using Best = std::conditional_t<(N > 8), float, double>;
-    Best best[N];
+    // large alphabets write the argmax bytes one by one
+    R best[N];
     int cx[N];
-    double off = 0.0;
+    R off = R(0);
     int jm = 0;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        double b = PJ[j] * msg[0];
+        R b = PJ[j] * msg[0];
         int bi = 0;
 #pragma unroll
         for (int i = 1; i < N; ++i) {
-            const double val = PJ[i * N + j] * msg[i];
+            const R val = PJ[i * N + j] * msg[i];
             if (val > b) {
                 b = val;
                 bi = i;
             }
         }
-        best[j] = (Best)b;
+        best[j] = b;
         if (b > off) {
             off = b;
             jm = j;
@@ -626,17 +652,17 @@ __device__ __forceinline__ void joint_store(const TreeDev &T, int v, int vn, int
             cx[j] = bi;
         }
     }
-    const double inv = 1.0 / off;
+    const R inv = R(1) / off;
     float out[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : (float)((double)best[j] * inv);
+    for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : (float)(best[j] * inv);
     store_row<N>(dst, out);
     if constexpr (N <= 8) store_bytes<N>(cxo, cx);
 }
 
 template <int N>
 __device__ __forceinline__ void post_joint_node(const TreeDev &T, int v, int vn, int p, int store_tip_cx) {
-    double msg[N];
+    JR<N> msg[N];
     joint_children<N>(T, v, p, msg, store_tip_cx);
     joint_store<N>(T, v, vn, p, msg);
 }
@@ -665,18 +691,19 @@ __device__ __forceinline__ void joint_fetch_d(const TreeDev &T, const PostDesc &
 }
 
 template <int N, class LJ>
-__device__ __forceinline__ void joint_finish_d(const TreeDev &T, const PostDesc &d, int p, const JointRaw<N> &R,
+__device__ __forceinline__ void joint_finish_d(const TreeDev &T, const PostDesc &d, int p, const JointRaw<N> &Raw,
                                                LJ lutj, int store_tip_cx) {
-    double msg[N];
+    using R = JR<N>;
+    R msg[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) msg[i] = 1.0;
+    for (int i = 0; i < N; ++i) msg[i] = R(1);
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         if ((c ? d.cref1 : d.cref0) < 0) {
-            tip_joint_code<N>(T, lutj, R.code[c], c ? d.cnode1 : d.cnode0, p, msg, store_tip_cx);
+            tip_joint_code<N>(T, lutj, Raw.code[c], c ? d.cnode1 : d.cnode0, p, msg, store_tip_cx);
         } else {
 #pragma unroll
-            for (int i = 0; i < N; ++i) msg[i] *= (double)R.row[c][i];
+            for (int i = 0; i < N; ++i) msg[i] *= (R)Raw.row[c][i];
         }
     }
     joint_store<N>(T, d.v, d.vn, p, msg);
@@ -686,13 +713,14 @@ __device__ __forceinline__ void joint_finish_d(const TreeDev &T, const PostDesc 
 // pi times the product of the children's messages.
 template <int N>
 __device__ __forceinline__ int root_joint_state(const TreeDev &T, int p, int store_tip_cx) {
-    double msg[N];
+    using R = JR<N>;
+    R msg[N];
     joint_children<N>(T, T.root_row, p, msg, store_tip_cx);
     int jm = 0;
-    double mx = -1.0;
+    R mx = R(-1);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        const double x = msg[i] * T.pi[i];
+        const R x = msg[i] * (R)T.pi[i];
         if (x > mx) {
             mx = x;
             jm = i;

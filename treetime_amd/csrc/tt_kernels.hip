@@ -296,7 +296,7 @@ __global__ void k_fill_states(TreeDev T, const int32_t *nodes, int n_nodes, uint
 // (treeanc.py:970).
 __global__ void k_expqt(int n, const double *t, const int32_t *branches, int k, const double *eig,
                         const double *v, const double *vinv, double mu, double *P64, double *Pj64, double *logP64,
-                        float *P32, float *D32) {
+                        float *Pj32, float *P32, float *D32) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int nn = n * n;
     if (gid >= k * nn) return;
@@ -309,6 +309,7 @@ __global__ void k_expqt(int n, const double *t, const int32_t *branches, int k, 
     P64[o] = P;
     Pj64[o] = fmax(TT_TINY, P);
     logP64[o] = log(fmax(TT_TINY, P));
+    Pj32[o] = (float)fmax(TT_TINY, P);
     P32[o] = (float)P;
     D32[o] = (float)(P - (i == j ? 1.0 : 0.0));  // deficit form keeps the diagonal's small part exact
 }
@@ -382,12 +383,12 @@ bool mfma_enabled() {
 }
 
 hipError_t expqt(int n, const double *t, const int32_t *branches, int k, const double *eig, const double *v,
-                 const double *vinv, double mu, double *P64, double *Pj64, double *logP64, float *P32, float *D32,
-                 hipStream_t s) {
+                 const double *vinv, double mu, double *P64, double *Pj64, double *logP64, float *Pj32, float *P32,
+                 float *D32, hipStream_t s) {
     if (k <= 0) return hipSuccess;
     const int total = k * n * n;
     hipLaunchKernelGGL(k_expqt, dim3((total + 255) / 256), dim3(256), 0, s, n, t, branches, k, eig, v, vinv, mu,
-                       P64, Pj64, logP64, P32, D32);
+                       P64, Pj64, logP64, Pj32, P32, D32);
     return hipGetLastError();
 }
 
