@@ -439,3 +439,32 @@ def test_edge_shapes(gpu_engine):
             assert abs(g - refs[i][col]) <= LOGL_RTOL * abs(refs[i][col]), (kind, jobs[i][0], g, refs[i][col])
     for t in trees:
         t.close()
+
+
+def test_joint_logL_is_score_of_decoded_states(gpu_engine):
+    """Size-independent property at the 5000-tip x 30000-site size: the
+    joint per-pattern logL is the FP64 log-likelihood of the decoded states
+    themselves (the oracle's ancestral_likelihood, treeanc.py:718-750, on the
+    device's states with tips reconstructed), to 1e-9 relative per pattern:
+    the two sides build P(t) in different operation orders, and log P_jj of
+    a short branch (P_jj ~ 1 - 1e-6) turns P's ~1e-16 absolute rounding into
+    ~1e-10 relative.  A missing or misindexed term is off by >= 1e-6."""
+    from treetime_amd import synth
+    from treetime_amd.prepare import prepare
+
+    say("[joint] logL = score of the decoded states, 5000 tips x 30000 sites")
+    job = synth.make_job(5000, 30000, 1e-3, 4, 1, 0.02, 0.0)
+    prep = prepare(job["parent"], job["branch_length"], job["chars"], job["gtr"], job["ambiguous"])
+    ta, nodes, tips = build_oracle(prep, "nuc_nogap", job["gtr"])
+    dt = device_tree(gpu_engine, prep)
+    total = dt.joint(reconstruct_tips=True)
+    slh = dt.sequence_lh()
+    states = dt.states()
+    assert (states < 4).all()
+    seq = {nd.name: states[i].astype(int) for i, nd in enumerate(nodes)}
+    ref = ta.ancestral_likelihood(seq)
+    err = float(np.max(np.abs(slh - ref) / np.abs(ref)))
+    say(f"  P={len(slh)} max per-pattern rel diff {err:.2e}")
+    assert err <= 1e-9
+    assert abs(total - float(np.sum(prep["multiplicity"] * ref))) <= 1e-9 * abs(total)
+    dt.close()
