@@ -111,7 +111,7 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
         const BatchTreeDev TT_C &B = trees[item.tree];
         const TreeDev TT_C &T = B.T;
         if (item.tree != cur_tree) {  // stage the tree's profile map in LDS
-            for (int i = threadIdx.x; i < 256 * N; i += TT_BLOCK) s_lut[i] = T.lut[i];
+            for (int i = threadIdx.x; i < T.n_codes * N; i += TT_BLOCK) s_lut[i] = T.lut[i];
             cur_tree = item.tree;
         }
         __syncthreads();
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal_w(const BatchTreeDe
         const BatchTreeDev TT_C &B = trees[item.tree];
         const TreeDev TT_C &T = B.T;
         if (item.tree != cur_tree) {  // stage the tree's profile map in this wave's LDS
-            for (int i = lane; i < 256 * N; i += TT_WAVE) lut[i] = T.lut[i];
+            for (int i = lane; i < T.n_codes * N; i += TT_WAVE) lut[i] = T.lut[i];
             cur_tree = item.tree;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
