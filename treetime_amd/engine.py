@@ -133,6 +133,7 @@ class DeviceTree:
         self.handle = h
         self.t_cur = t.copy()        # current effective branch lengths (host mirror)
         self.joint_ready = False     # joint buffers allocated on the device
+        self.site = False            # a site-specific model is set
         if site is not None:         # (eig, V, Vi, mu, pi, grid, rate_scale) of a site-specific model
             self.set_site_model(*site)
         else:
@@ -144,6 +145,7 @@ class DeviceTree:
         vi = _native.as_c(v_inv, np.float64)
         _native.check(self.lib.tt_tree_set_model(self.handle, _native.ptr(e, C.c_double), _native.ptr(vv, C.c_double),
                                                  _native.ptr(vi, C.c_double), float(mu)))
+        self.site = False
 
     def set_site_model(self, eig, V, Vi, mu, pi, grid, rate_scale):
         """Per-pattern model (tt_tree_set_site_model): eig (P, n), V and Vi
@@ -154,6 +156,7 @@ class DeviceTree:
         _native.check(self.lib.tt_tree_set_site_model(
             self.handle, *[_native.ptr(x, C.c_double) for x in arrs[:5]],
             _native.ptr(g, C.c_double) if len(g) else None, len(g), float(rate_scale)))
+        self.site = True
 
     def nonroot_nodes(self):
         return np.nonzero(self.parent >= 0)[0]
@@ -281,9 +284,10 @@ class DeviceTree:
         self.joint_ready = False
 
     def joint_bytes(self):
-        """Device bytes a joint pass allocates on first use."""
+        """Device bytes a joint pass allocates on first use: the messages
+        (FP32, FP64 for a site-specific model) and the argmax table."""
         n_int = int(np.count_nonzero(np.bincount(self.parent[self.parent >= 0], minlength=self.n_nodes)))
-        return n_int * self.P * self.n * 8 + self.n_nodes * self.P * self.n
+        return n_int * self.P * self.n * (8 if self.site else 4) + self.n_nodes * self.P * self.n
 
     def query(self):
         """True when the work queued on the tree's own stream has finished."""
