@@ -263,6 +263,31 @@ def test_batch_run_modes(gpu_engine, block_batches):
         t.close()
 
 
+
+@pytest.mark.parametrize("wps", ["0", "6", "8"])
+def test_batch_joint_register_budgets(gpu_engine, block_batches, monkeypatch, wps):
+    """Every register budget of the batched joint kernel (the tuner's
+    candidates, pinned one at a time): states identical to tt_joint, logL to
+    1e-12, with and without tip reconstruction, on binary and multifurcating
+    trees."""
+    from treetime_amd.engine import batch_info, batch_run
+
+    say(f"[batch] joint kernel, waves per SIMD {wps}")
+    monkeypatch.setenv("TT_JOINT_WAVES_PER_SIMD", wps)
+    jobs = [small_job(n, L, 4, seed) for n, L, seed in [(90, 700, 81), (12, 130, 82)]]
+    jobs.append(polytomy_job(120, 400, 4, 83, 0.3))
+    trees = [device_tree(gpu_engine, prep) for _, prep, _ in jobs]
+    for tips in (False, True):
+        plain = [t.joint(reconstruct_tips=tips) for t in trees]
+        plain_states = [t.states() for t in trees]
+        got = batch_run(gpu_engine, trees, "joint", reconstruct_tips=tips)
+        assert batch_info(gpu_engine, trees, kind="joint", reconstruct_tips=tips)["waves_per_simd"] == int(wps)
+        for i, t in enumerate(trees):
+            assert abs(got[i] - plain[i]) <= 1e-12 * abs(plain[i]), (wps, tips, i, got[i], plain[i])
+            assert np.array_equal(t.states(), plain_states[i]), (wps, tips, i)
+    for t in trees:
+        t.close()
+
 def test_tip_states_reset_after_reconstruction(gpu_engine):
     """Tip rows of the states hold 255 after a pass without tip
     reconstruction, also when an earlier pass (plain or batched) filled them."""

@@ -365,6 +365,10 @@ def batch_info(engine, trees, reconstruct_tips=False, kind="marginal"):
     flags = _native.TT_RECONSTRUCT_TIPS if reconstruct_tips else 0
     if kind == "loglik":
         flags = _native.TT_BATCH_LOGLIK
+    elif kind == "joint":
+        flags |= _native.TT_BATCH_JOINT
+    elif kind != "marginal":
+        raise ValueError(f"kind must be 'marginal', 'loglik' or 'joint', not {kind!r}")
     _native.check(engine.lib.tt_batch_info(engine.handle, arr, k, flags, C.byref(w), C.byref(tl), C.byref(b),
                                            C.byref(wps), C.byref(ms)))
     return {"wave_items": w.value, "tiles": tl.value, "blocks_per_cu": b.value, "waves_per_simd": wps.value,
