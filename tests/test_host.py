@@ -215,3 +215,14 @@ def test_site_treeanc_requires_uncompressed_alignment():
     with pytest.raises(TypeError):
         tt.TreeAnc(tree=tt.Tree.from_newick("((A:0.1,B:0.2):0.1,C:0.3);"),
                    aln={"A": "ACG", "B": "ACT", "C": "ACG"}, gtr=g, verbose=0)
+
+
+def test_batch_entry_points_reject_unknown_kind():
+    """batch_info / batch_run name the pass kind; anything else is refused
+    before any device call."""
+    from treetime_amd.engine import batch_info, batch_run
+
+    with pytest.raises(ValueError):
+        batch_info(None, [], kind="posterior")
+    with pytest.raises(ValueError):
+        batch_run(None, [], "posterior")

@@ -341,13 +341,16 @@ def batch_marginal(engine, trees, reconstruct_tips=False, stream=None, want_logL
 
 def batch_run(engine, trees, kind="marginal", reconstruct_tips=False, stream=None, want_logL=True):
     """tt_batch_run: kind 'marginal' (one persistent launch), 'loglik'
-    (persistent launch without the preorder) or 'joint' (each tree's joint
-    pass on one stream).  Returns logL per tree."""
+    (the persistent launch without the preorder) or 'joint' (one persistent
+    launch of the joint kernel).  Returns logL per tree."""
+    modes = {"marginal": _native.TT_BATCH_MARGINAL, "joint": _native.TT_BATCH_JOINT,
+             "loglik": _native.TT_BATCH_LOGLIK}
+    if kind not in modes:
+        raise ValueError(f"kind must be 'marginal', 'loglik' or 'joint', not {kind!r}")
     k = len(trees)
     arr = (C.c_void_p * k)(*[t.handle.value for t in trees])
     out = np.full(k, np.nan)
-    mode = {"marginal": _native.TT_BATCH_MARGINAL, "joint": _native.TT_BATCH_JOINT,
-            "loglik": _native.TT_BATCH_LOGLIK}[kind]
+    mode = modes[kind]
     if reconstruct_tips:
         mode |= _native.TT_RECONSTRUCT_TIPS
     _native.check(engine.lib.tt_batch_run(engine.handle, arr, k, mode,
@@ -359,6 +362,8 @@ def batch_info(engine, trees, reconstruct_tips=False, kind="marginal"):
     """Launch shape chosen by the autotuner for this list of trees:
     {"wave_items", "tiles", "blocks_per_cu", "waves_per_simd", "tuned_ms"}
     (waves_per_simd 0 = the compiler's register budget)."""
+    if kind not in ("marginal", "loglik", "joint"):
+        raise ValueError(f"kind must be 'marginal', 'loglik' or 'joint', not {kind!r}")
     k = len(trees)
     arr = (C.c_void_p * k)(*[t.handle.value for t in trees])
     w, tl, b, wps, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
@@ -367,8 +372,6 @@ def batch_info(engine, trees, reconstruct_tips=False, kind="marginal"):
         flags = _native.TT_BATCH_LOGLIK
     elif kind == "joint":
         flags |= _native.TT_BATCH_JOINT
-    elif kind != "marginal":
-        raise ValueError(f"kind must be 'marginal', 'loglik' or 'joint', not {kind!r}")
     _native.check(engine.lib.tt_batch_info(engine.handle, arr, k, flags, C.byref(w), C.byref(tl), C.byref(b),
                                            C.byref(wps), C.byref(ms)))
     return {"wave_items": w.value, "tiles": tl.value, "blocks_per_cu": b.value, "waves_per_simd": wps.value,
