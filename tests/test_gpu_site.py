@@ -122,6 +122,32 @@ def test_site_model_switch_and_refusals(gpu_engine):
     dev.close()
 
 
+
+def test_joint_buffers_follow_the_model_kind(gpu_engine):
+    """Joint passes across model switches: the joint buffers are reallocated
+    between the site-specific (FP64 log) and the shared-model (FP32
+    product-space) layouts, and each joint equals a tree built directly in
+    that mode."""
+    say("[site] joint across site -> shared -> site model switches")
+    prep, g, ta, nodes = site_case(40, 300, 8, True)
+    dev = device(gpu_engine, prep, g)
+    lj_site = dev.joint()
+    st_site = dev.states()
+    std = synth.make_job(40, 300, 0.05, 4, 9, 0.01, 0.005)["gtr"]
+    dev.set_model(std.eigenvals, std.v, std.v_inv, std.mu)
+    lj_std = dev.joint(reconstruct_tips=True)
+    st_std = dev.states()
+    lut = code_lut(g.profile_map, g.n_states)
+    fresh = DeviceTree(gpu_engine, prep["parent"], prep["t"], prep["tip_codes"], lut, prep["multiplicity"],
+                       g.Pi.mean(axis=1), std.eigenvals, std.v, std.v_inv, std.mu)
+    assert abs(lj_std - fresh.joint(reconstruct_tips=True)) <= 1e-12 * abs(lj_std)
+    assert np.array_equal(st_std, fresh.states())
+    fresh.close()
+    dev.set_site_model(*g.device_model())
+    assert abs(dev.joint() - lj_site) <= 1e-12 * abs(lj_site)
+    assert np.array_equal(dev.states(), st_site)
+    dev.close()
+
 def test_site_propagate_profile_primitive():
     rng = np.random.default_rng(6)
     g = tt.GTR_site_specific.random(L=50, alphabet="nuc", rng=rng)
