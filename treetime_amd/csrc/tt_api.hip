@@ -686,7 +686,8 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     for (BatchShape kd : kinds) {
         if (!allowed(kd.wave_items)) continue;
         if (P->joint && jw && jw[0] && kd.waves_per_simd != atoi(jw)) continue;
-        if (!ttk::batch_all_shapes(n) && !(kd.wave_items == 0 && kd.tiles == 1 && !(P->joint && kd.waves_per_simd)))
+        if (P->joint ? !ttk::batch_joint_budget(n, kd.waves_per_simd)
+                     : !ttk::batch_all_shapes(n) && !(kd.wave_items == 0 && kd.tiles == 1))
             continue;
         const int occ = P->joint ? ttk::batch_joint_max_blocks_per_cu(n, kd.waves_per_simd)
                                  : ttk::batch_max_blocks_per_cu(n, kd);

@@ -386,6 +386,8 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
 // Wide alphabets (n = 20: 256 VGPRs with spills already at one tile) get only
 // the one-tile block kernel; the other shapes are instantiated for n <= 8.
 constexpr bool all_shapes(int n) { return n <= 8; }
+// the joint kernel fits 8 waves per SIMD only up to this alphabet size
+constexpr int kJoint8MaxN = 4;
 
 template <int N>
 int max_blocks_per_cu(const BatchShape &c) {
@@ -431,6 +433,10 @@ int ttk::device_cus() {
 
 bool ttk::batch_all_shapes(int n) { return all_shapes(n); }
 
+bool ttk::batch_joint_budget(int n, int waves_per_simd) {
+    return waves_per_simd == 0 || (waves_per_simd == 6 && all_shapes(n)) || (waves_per_simd == 8 && n <= kJoint8MaxN);
+}
+
 hipError_t ttk::batch_joint(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
                             double *item_out, int mode, const BatchShape &shape, hipStream_t s) {
     if (n_items <= 0) return hipSuccess;
@@ -444,9 +450,9 @@ hipError_t ttk::batch_joint(int n, const BatchTreeDev *trees, const BatchItem *i
                                    queue, item_out, mode);
                 return hipGetLastError();
             }
-            if (shape.waves_per_simd == 8) {
-                hipLaunchKernelGGL((k_batch_joint<N_, 8>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items,
-                                   queue, item_out, mode);
+            if (N_ <= kJoint8MaxN && shape.waves_per_simd == 8) {
+                hipLaunchKernelGGL((k_batch_joint<N_, N_ <= kJoint8MaxN ? 8 : 1>), dim3(grid), dim3(TT_BLOCK), 0, s,
+                                   trees, items, n_items, queue, item_out, mode);
                 return hipGetLastError();
             }
         }
@@ -465,8 +471,9 @@ int ttk::batch_joint_max_blocks_per_cu(int n, int waves_per_simd) {
                     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_joint<N_, 6>, TT_BLOCK, 0);
                     return std::max(1, per_cu);
                 }
-                if (waves_per_simd == 8) {
-                    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_joint<N_, 8>, TT_BLOCK, 0);
+                if (N_ <= kJoint8MaxN && waves_per_simd == 8) {
+                    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &per_cu, k_batch_joint<N_, N_ <= kJoint8MaxN ? 8 : 1>, TT_BLOCK, 0);
                     return std::max(1, per_cu);
                 }
             }

@@ -209,6 +209,8 @@ int batch_joint_max_blocks_per_cu(int n, int waves_per_simd);
 int batch_max_blocks_per_cu(int n, const BatchShape &shape);
 // false when only the one-tile block kernel exists for n states
 bool batch_all_shapes(int n);
+// a register budget (waves per SIMD: 0, 6 or 8) the joint kernel is built with for alphabet size n
+bool batch_joint_budget(int n, int waves_per_simd);
 int device_cus();
 }  // namespace ttk
 
