@@ -348,7 +348,7 @@ def gtr_infer(alphabet, nij, Ti, root_state, fixed_pi=None, pc=1.0, gap_limit=0.
             mu = (nij.sum() + pc) / (np.sum(pi * W.dot(Ti)) + pc)
         else:
             mu = (nij.sum() + pc) / (np.sum(pi * W.dot(pi)) * Ti.sum() + pc)
-    if gap is not None and pi[gap] < gap_limit:
+    if gap is not None:  # set unconditionally (gtr.py:585-596); only the warning is conditional
         pi[gap] = gap_limit
         pi /= pi.sum()
     return GTRModel(alphabet, mu=mu, pi=pi, W=W)

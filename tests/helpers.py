@@ -30,11 +30,10 @@ def build_oracle(prep, alphabet, gtr, names=None):
     return ta, nodes, tips
 
 
-def small_job(n_tips, L, n_states=4, seed=1, depth=0.02, p_ambig=0.01, p_gap=0.005):
-    job = synth.make_job(n_tips, L, depth, n_states, seed, p_ambig, p_gap)
+def small_job(n_tips, L, n_states=4, seed=1, depth=0.02, p_ambig=0.01, p_gap=0.005, p_iupac=0.0):
+    job = synth.make_job(n_tips, L, depth, n_states, seed, p_ambig, p_gap, p_iupac)
     prep = prepare(job["parent"], job["branch_length"], job["chars"], job["gtr"], job["ambiguous"])
-    alphabet = "nuc_nogap" if n_states == 4 else "aa_nogap"
-    return job, prep, alphabet
+    return job, prep, synth.ALPHABET_OF[n_states]
 
 
 def collapse_job(job, frac, seed, root_extra=True):
@@ -73,8 +72,7 @@ def collapse_job(job, frac, seed, root_extra=True):
     return out
 
 
-def polytomy_job(n_tips, L, n_states=4, seed=1, frac=0.3, depth=0.05, p_ambig=0.01, p_gap=0.005):
-    job = collapse_job(synth.make_job(n_tips, L, depth, n_states, seed, p_ambig, p_gap), frac, seed + 1)
+def polytomy_job(n_tips, L, n_states=4, seed=1, frac=0.3, depth=0.05, p_ambig=0.01, p_gap=0.005, p_iupac=0.0):
+    job = collapse_job(synth.make_job(n_tips, L, depth, n_states, seed, p_ambig, p_gap, p_iupac), frac, seed + 1)
     prep = prepare(job["parent"], job["branch_length"], job["chars"], job["gtr"], job["ambiguous"])
-    alphabet = "nuc_nogap" if n_states == 4 else "aa_nogap"
-    return job, prep, alphabet
+    return job, prep, synth.ALPHABET_OF[n_states]

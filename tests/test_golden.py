@@ -71,3 +71,56 @@ def test_engine_matches_golden(gpu_engine, tag, alphabet):
     jsure = g["joint_margin"] > 1e-5
     assert np.array_equal(dev.states()[internal][jsure], g["joint_states"][jsure])
     dev.close()
+
+
+# ----------------------------------------------------------- profile tables --
+PROFILE_MAPS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "profile_maps.json")
+
+
+def _reference_tables():
+    import json
+
+    with open(PROFILE_MAPS) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("alphabet", ["nuc", "nuc_nogap", "aa", "aa_nogap"])
+def test_profile_maps_equal_reference_tables(alphabet):
+    """The oracle's and the facade's alphabets and character profiles equal
+    the literal tables of seq_utils.py:19-122 (tests/golden/profile_maps.json,
+    read from the reference source with ast by make_profile_maps.py): the
+    same characters, the same multi-hot vectors."""
+    from treetime_amd import seq_utils as S
+
+    ref = _reference_tables()
+    alpha = ref["alphabets"][alphabet]
+    assert list(O.ALPHABETS[alphabet]) == alpha
+    assert list(S.alphabets[alphabet]) == alpha
+    table = {c: np.array(v, dtype=float) for c, v in ref["profile_maps"][alphabet].items()}
+    for name, pm in [("oracle", O.profile_map(alphabet)), ("facade", S.profile_maps[alphabet])]:
+        assert sorted(pm) == sorted(table), name
+        for c, v in table.items():
+            assert np.array_equal(np.asarray(pm[c], dtype=float), v), (name, c)
+
+
+def test_alphabet_synonyms_equal_reference():
+    from treetime_amd import seq_utils as S
+
+    assert S.alphabet_synonyms == _reference_tables()["alphabet_synonyms"]
+
+
+@pytest.mark.parametrize("alphabet", ["nuc", "nuc_nogap", "aa", "aa_nogap"])
+def test_device_code_lut_follows_reference_tables(alphabet):
+    """The 256-entry code table the device reads (engine.code_lut) holds the
+    reference's profile of every character of the tables, and all ones for
+    every other byte (extend_profile, seq_utils.py:126-136)."""
+    from treetime_amd.engine import code_lut
+    from treetime_amd import seq_utils as S
+
+    table = _reference_tables()["profile_maps"][alphabet]
+    n = len(_reference_tables()["alphabets"][alphabet])
+    lut = code_lut(S.profile_maps[alphabet], n)
+    for b in range(256):
+        c = chr(b)
+        want = np.array(table[c], dtype=np.float32) if c in table else np.ones(n, dtype=np.float32)
+        assert np.array_equal(lut[b], want), (alphabet, c)

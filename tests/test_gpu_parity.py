@@ -51,18 +51,28 @@ CASES = [
     ("poly-nuc", 150, 900, 4, 31, 0.05, 0.01, 0.005),
     ("poly-aa", 50, 300, 20, 32, 0.3, 0.0, 0.0),
     ("poly-big", 1500, 1200, 4, 33, 0.02, 0.01, 0.005),
+    # the reference's default alphabet 'nuc': five states, the gap is a state
+    ("nuc5", 150, 900, 5, 41, 0.05, 0.01, 0.02),
+    ("poly-nuc5", 120, 700, 5, 42, 0.05, 0.01, 0.02),
+    ("nuc5-deep", 64, 500, 5, 43, 0.5, 0.02, 0.03),
+    # tips carrying every partial ambiguity code (multi-hot tip rows):
+    # IUPAC R Y S W K M D H B V for nucleotides, B Z (and X) for amino acids
+    ("iupac-nuc", 200, 1200, 4, 44, 0.05, 0.01, 0.005, 0.05),
+    ("iupac-nuc5", 100, 800, 5, 45, 0.05, 0.01, 0.01, 0.05),
+    ("iupac-aa", 60, 400, 20, 46, 0.3, 0.02, 0.0, 0.05),
+    ("poly-iupac-aa", 50, 300, 20, 47, 0.3, 0.02, 0.0, 0.05),
 ]
 
 
-def make_case(name, n_tips, L, n_states, seed, depth, pa, pg):
+def make_case(name, n_tips, L, n_states, seed, depth, pa, pg, pi=0.0):
     if name.startswith("poly"):
-        return polytomy_job(n_tips, L, n_states, seed, 0.3, depth, pa, pg)
-    return small_job(n_tips, L, n_states, seed, depth, pa, pg)
+        return polytomy_job(n_tips, L, n_states, seed, 0.3, depth, pa, pg, pi)
+    return small_job(n_tips, L, n_states, seed, depth, pa, pg, pi)
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_marginal_parity(gpu_engine, case):
-    name, n_tips, L, n_states, seed, depth, pa, pg = case
+    name, n_tips, L, n_states, seed, depth, pa, pg = case[:8]
     say(f"[marginal] case={name} n_tips={n_tips} L={L} n={n_states} seed={seed}")
     job, prep, alphabet = make_case(*case)
     ta, nodes, tips = build_oracle(prep, alphabet, job["gtr"])
@@ -91,7 +101,7 @@ def test_marginal_parity(gpu_engine, case):
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_joint_parity(gpu_engine, case):
-    name, n_tips, L, n_states, seed, depth, pa, pg = case
+    name, n_tips, L, n_states, seed, depth, pa, pg = case[:8]
     say(f"[joint] case={name} n_tips={n_tips} L={L} n={n_states} seed={seed}")
     job, prep, alphabet = make_case(*case)
     ta, nodes, tips = build_oracle(prep, alphabet, job["gtr"])
@@ -264,18 +274,27 @@ def test_batch_run_modes(gpu_engine, block_batches):
 
 
 
+@pytest.mark.parametrize("n_states", [4, 5])
 @pytest.mark.parametrize("wps", ["0", "6", "8"])
-def test_batch_joint_register_budgets(gpu_engine, block_batches, monkeypatch, wps):
+def test_batch_joint_register_budgets(gpu_engine, block_batches, monkeypatch, wps, n_states):
     """Every register budget of the batched joint kernel (the tuner's
     candidates, pinned one at a time): states identical to tt_joint, logL to
     1e-12, with and without tip reconstruction, on binary and multifurcating
-    trees."""
+    trees.  The 8-wave budget exists for alphabets of up to 4 states only;
+    pinning it for 5 states is refused with an error naming the budget."""
     from treetime_amd.engine import batch_info, batch_run
 
-    say(f"[batch] joint kernel, waves per SIMD {wps}")
+    say(f"[batch] joint kernel, waves per SIMD {wps}, n={n_states}")
     monkeypatch.setenv("TT_JOINT_WAVES_PER_SIMD", wps)
-    jobs = [small_job(n, L, 4, seed) for n, L, seed in [(90, 700, 81), (12, 130, 82)]]
-    jobs.append(polytomy_job(120, 400, 4, 83, 0.3))
+    jobs = [small_job(n, L, n_states, seed) for n, L, seed in [(90, 700, 81), (12, 130, 82)]]
+    jobs.append(polytomy_job(120, 400, n_states, 83, 0.3))
+    if n_states == 5 and wps == "8":
+        trees = [device_tree(gpu_engine, prep) for _, prep, _ in jobs]
+        with pytest.raises(ValueError, match="waves per SIMD"):
+            batch_run(gpu_engine, trees, "joint")
+        for t in trees:
+            t.close()
+        return
     trees = [device_tree(gpu_engine, prep) for _, prep, _ in jobs]
     for tips in (False, True):
         plain = [t.joint(reconstruct_tips=tips) for t in trees]

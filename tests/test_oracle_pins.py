@@ -564,3 +564,29 @@ def test_site_gtr_infer_recovers_model():
     g = tt.GTR_site_specific.infer(n_ija, T_ia, root, pc=1e-3, alphabet="nuc_nogap")
     assert np.allclose(g.Pi, m.Pi, atol=1e-14) and np.allclose(g.W, m.W, atol=1e-13)
     assert np.allclose(g.mu, m.mu, rtol=1e-13)
+
+
+def test_gtr_infer_gap_floor_is_unconditional():
+    """gtr.py:585-596: with a gap state, GTR.infer always sets pi[gap] to
+    gap_limit and renormalises (only the warning depends on the estimate).
+    Exact counts of a model whose gap frequency is 0.2 give a fixed point
+    with pi[gap] ~ 0.2; the returned model must hold 0.01 / (1 - 0.2 + 0.01)
+    there and keep the ratios of the other frequencies.  A floor applied only
+    below the limit would return ~0.2."""
+    from treetime_amd.gtr import GTR
+
+    rng = np.random.default_rng(11)
+    n = 5
+    pi0 = np.array([0.3, 0.2, 0.1, 0.2, 0.2])
+    W = rng.lognormal(0, 0.4, (n, n))
+    W = (W + W.T) / 2
+    g0 = O.GTRModel("nuc", mu=1.0, pi=pi0, W=W)
+    T = 1e5 * g0.Pi
+    nij = g0.mu * g0.Pi[:, None] * g0.W * T[None, :]
+    for g in (O.gtr_infer("nuc", nij, T, np.zeros(n), pc=0.0), GTR.infer(nij, T, np.zeros(n), pc=0.0,
+                                                                          alphabet="nuc")):
+        pi = g.Pi
+        want_gap = 0.01 / (1.0 - 0.2 + 0.01)
+        assert abs(pi[4] - want_gap) < 2e-4, pi
+        assert np.allclose(pi[:4] / pi[:4].sum(), pi0[:4] / pi0[:4].sum(), atol=2e-4)
+        assert abs(pi.sum() - 1.0) < 1e-12

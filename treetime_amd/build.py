@@ -1,24 +1,28 @@
 """Build recipe for the in-tree HIP library ``treetime_amd/libttgpu.so``.
 
 ``python -m treetime_amd.build`` compiles every HIP source under ``csrc/``
-for gfx950 with hipcc into one shared library next to this file.
+for gfx950 with hipcc (one object per source, compiled in parallel) and links
+them into one shared library next to this file.
 """
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libttgpu.so")
-SOURCES = ["tt_kernels.hip", "tt_api.hip", "tt_batch.hip", "tt_mfma.hip", "tt_branch.hip", "tt_compress.hip", "tt_site.hip"]
+OBJ = os.path.join(HERE, "_build")
+SOURCES = ["tt_kernels.hip", "tt_api.hip", "tt_batch.hip", "tt_mfma.hip", "tt_branch.hip", "tt_compress.hip",
+           "tt_site.hip", "tt_noalloc.hip", "tt_stream.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TT_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions are disabled for all device code.
 FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
@@ -28,10 +32,24 @@ def sources():
     return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
-def build(verbose=False, extra_flags=()):
+def _compile(src, extra_flags, verbose):
+    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    cmd = [HIPCC, *FLAGS, *extra_flags, "-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return obj
+
+
+def build(verbose=False, extra_flags=(), jobs=None):
     """Compile the library; raises CalledProcessError on failure."""
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = sources()
+    jobs = jobs or min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda s: _compile(s, tuple(extra_flags), verbose), srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, *extra_flags, "-I", os.path.join(ROOT, "include"), *sources(), "-o", tmp]
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -407,7 +407,15 @@ int32_t ensure_joint(tt_tree *T) {
     else
         rc = T->alloc((void **)&T->d.Lx, rows * sizeof(float));
     if (rc) return rc;
-    if ((rc = T->alloc((void **)&T->d.Cx, (size_t)T->n_nodes * T->P * T->n))) return rc;
+    if ((rc = T->alloc((void **)&T->d.Cx, (size_t)T->n_nodes * T->P * T->n))) {
+        // give the message buffer back, so a retry under budget pressure
+        // starts from nothing instead of stacking message buffers
+        T->release(T->site ? (void *)T->d.Lx64 : (void *)T->d.Lx);
+        T->d.Lx = nullptr;
+        T->d.Lx64 = nullptr;
+        T->d.Cx = nullptr;
+        return rc;
+    }
     T->joint_site = T->site;
     T->joint_ready = true;
     return TT_OK;
@@ -683,6 +691,13 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     const std::vector<BatchShape> kinds =
         P->joint ? std::vector<BatchShape>{{0, 0, 1, 0}, {0, 0, 1, 6}, {0, 0, 1, 8}}
                  : std::vector<BatchShape>{{0, 0, 2, 0}, {0, 0, 2, 5}, {0, 0, 1, 0}, {1, 0, 1, 0}};
+    // a pinned joint register budget must exist for this alphabet: refuse it
+    // rather than silently falling back to another launch path
+    if (P->joint && jw && jw[0] && !ttk::batch_joint_budget(n, atoi(jw)))
+        return fail(TT_EINVAL,
+                    "TT_JOINT_WAVES_PER_SIMD=%s: the batched joint kernel has no register budget of %s waves per "
+                    "SIMD for an alphabet of %d states",
+                    jw, jw, n);
     for (BatchShape kd : kinds) {
         if (!allowed(kd.wave_items)) continue;
         if (P->joint && jw && jw[0] && kd.waves_per_simd != atoi(jw)) continue;

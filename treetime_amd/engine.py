@@ -145,6 +145,10 @@ class DeviceTree:
         vi = _native.as_c(v_inv, np.float64)
         _native.check(self.lib.tt_tree_set_model(self.handle, _native.ptr(e, C.c_double), _native.ptr(vv, C.c_double),
                                                  _native.ptr(vi, C.c_double), float(mu)))
+        if self.site:
+            # the next joint pass reallocates the joint buffers at the FP32
+            # message size (the library frees the FP64 ones first)
+            self.joint_ready = False
         self.site = False
 
     def set_site_model(self, eig, V, Vi, mu, pi, grid, rate_scale):
@@ -156,6 +160,8 @@ class DeviceTree:
         _native.check(self.lib.tt_tree_set_site_model(
             self.handle, *[_native.ptr(x, C.c_double) for x in arrs[:5]],
             _native.ptr(g, C.c_double) if len(g) else None, len(g), float(rate_scale)))
+        if not self.site:
+            self.joint_ready = False  # joint buffers are reallocated at the FP64 message size
         self.site = True
 
     def nonroot_nodes(self):
@@ -355,6 +361,9 @@ def batch_run(engine, trees, kind="marginal", reconstruct_tips=False, stream=Non
         mode |= _native.TT_RECONSTRUCT_TIPS
     _native.check(engine.lib.tt_batch_run(engine.handle, arr, k, mode,
                                           _native.ptr(out, C.c_double) if want_logL else None, stream))
+    if kind == "joint":
+        for t in trees:  # the batched joint pass allocated every member's joint buffers
+            t.joint_ready = True
     return out
 
 

@@ -211,9 +211,12 @@ class GTR:
                 mu = (nij.sum() + pc) / (np.sum(pi * (W.dot(pi))) * Ti.sum() + pc)
         if rounds >= 40:
             gtr.logger("WARNING: maximum number of iterations has been reached in GTR inference", 3, warn=True)
-        if gtr.gap_index is not None and pi[gtr.gap_index] < gap_limit:
-            gtr.logger("The model allows for gaps which are estimated to occur at a low fraction of %1.3e; gap "
-                       "fraction will be set to %1.4f" % (pi[gtr.gap_index], gap_limit), 2, warn=True)
+        if gtr.gap_index is not None:
+            # the gap frequency is always set to gap_limit (gtr.py:585-596);
+            # only the warning depends on the estimate
+            if pi[gtr.gap_index] < gap_limit:
+                gtr.logger("The model allows for gaps which are estimated to occur at a low fraction of %1.3e; gap "
+                           "fraction will be set to %1.4f" % (pi[gtr.gap_index], gap_limit), 2, warn=True)
             pi[gtr.gap_index] = gap_limit
             pi /= pi.sum()
         gtr.assign_rates(mu=mu, W=W, pi=pi)

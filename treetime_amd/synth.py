@@ -5,7 +5,8 @@ Tree: with k lineages the waiting time is Exp(k(k-1)/2) and a uniformly
 random pair merges; the ultrametric tree is scaled to the requested
 root-to-tip depth.  Sequences: the root is drawn from pi and every child
 site from column P(t)[:, parent state] (the sampling of SeqGen.evolve,
-treetime/seqgen.py:38-67).  Tip noise replaces characters by 'N' / '-'.
+treetime/seqgen.py:38-67).  Tip noise replaces characters by the ambiguous
+character, a gap or a partial ambiguity code.
 """
 import numpy as np
 
@@ -111,34 +112,46 @@ def simulate(parent, bl, gtr, L, rng):
     return states
 
 
-def make_job(n_tips, L, depth, n_states, seed, p_ambig=0.0, p_gap=0.0):
+# partial ambiguity codes of each alphabet (seq_utils.py:28-122): nucleotide
+# IUPAC sets, amino-acid B (D/N) and Z (E/Q)
+IUPAC_CODES = {4: b"RYSWKMDHBV", 5: b"RYSWKMDHBV", 20: b"BZ"}
+ALPHABET_OF = {4: "nuc_nogap", 5: "nuc", 20: "aa_nogap"}
+
+
+def make_job(n_tips, L, depth, n_states, seed, p_ambig=0.0, p_gap=0.0, p_iupac=0.0):
     """One synthetic job.  Returns a dict with the tree (preorder numbering),
     the GTR model, and the tip alignment as a (n_tips, L) char matrix whose
-    rows follow ascending tip node index."""
+    rows follow ascending tip node index.  n_states 4 ('nuc_nogap'), 5
+    ('nuc', the gap is a state) or 20 ('aa_nogap').  Tip noise: a fraction
+    p_ambig of the characters becomes the ambiguous character, p_gap a gap
+    and p_iupac a random partial ambiguity code of the alphabet."""
     rng = np.random.default_rng(seed)
     parent, bl = kingman_tree(n_tips, depth, rng)
     parent, bl, _ = preorder_renumber(parent, bl)
-    if n_states == 4:
-        pi, W = random_gtr(4, rng, 5.0, 0.5)
-        gtr = GTR.custom(mu=1.0, pi=pi, W=W, alphabet="nuc_nogap")
-        alpha = alphabets["nuc_nogap"]
+    if n_states in (4, 5):
+        pi, W = random_gtr(n_states, rng, 5.0, 0.5)
         amb = "N"
     elif n_states == 20:
         pi, W = random_gtr(20, rng, 2.0, 1.0)
-        gtr = GTR.custom(mu=1.0, pi=pi, W=W, alphabet="aa_nogap")
-        alpha = alphabets["aa_nogap"]
         amb = "X"
     else:
-        raise ValueError("n_states must be 4 or 20")
+        raise ValueError("n_states must be 4, 5 or 20")
+    alphabet = ALPHABET_OF[n_states]
+    gtr = GTR.custom(mu=1.0, pi=pi, W=W, alphabet=alphabet)
+    alpha = alphabets[alphabet]
     states = simulate(parent, bl, gtr, L, rng)
     is_tip = np.ones(len(parent), dtype=bool)
     is_tip[parent[parent >= 0]] = False
     tips = np.where(is_tip)[0]
     chars = alpha.astype("S1")[states[tips]]
-    if p_ambig or p_gap:
+    if p_ambig or p_gap or p_iupac:
         u = rng.random(chars.shape)
         chars[u < p_ambig] = amb.encode()
         chars[(u >= p_ambig) & (u < p_ambig + p_gap)] = b"-"
+        if p_iupac:
+            sel = (u >= p_ambig + p_gap) & (u < p_ambig + p_gap + p_iupac)
+            codes = np.frombuffer(IUPAC_CODES[n_states], dtype="S1")
+            chars[sel] = codes[rng.integers(0, len(codes), size=int(sel.sum()))]
     names = ["t%05d" % i for i in range(len(tips))]
     return {"parent": parent, "branch_length": bl, "tips": tips, "names": names, "gtr": gtr,
             "chars": chars, "ambiguous": amb, "L": L, "seed": seed, "true_states": states}
