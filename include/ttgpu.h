@@ -251,6 +251,18 @@ int32_t tt_propagate_profile_site(const double *in_dev, int32_t P, int32_t n, co
 int32_t tt_prob_t_profiles(const double *pp_dev, const double *pc_dev, const double *mult_dev, int32_t P,
                            int32_t n, const double *Q, int32_t K, int32_t gap_index, double *out, void *stream);
 
+/* tt_prob_t_profiles without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as tt_prob_t_profiles (identical values); the K
+ * matrices and the partial sums live in a fixed device array of 4 MiB per
+ * GPU, so K * (n*n + ceil(P/256) + 1) must not exceed 524288 and K 65535
+ * (TT_EINVAL otherwise).  Calls that overlap on one GPU race on that array;
+ * calls ordered on one stream do not. */
+int32_t tt_prob_t_profiles_noalloc(const double *pp_dev, const double *pc_dev, const double *mult_dev, int32_t P,
+                                   int32_t n, const double *Q, int32_t K, int32_t gap_index, double *out,
+                                   void *stream);
+
 /* Per-pattern GTR statistics summed over all branches, for the
  * site-specific GTR.infer of TreeAnc.infer_gtr(site_specific=True)
  * (treeanc.py:1556-1589, 1593-1602): n_out[P][n][n] = n_ija (derived state

@@ -102,6 +102,7 @@ SIGNATURES = {
     "tt_get_device_buffers": (_I32, [_P, C.POINTER(_PF), C.POINTER(_PU8), C.POINTER(_PD)]),
     "tt_propagate_profile": (_I32, [_P, _I32, _I32, _PD, _I32, _I32, _P, _P]),
     "tt_prob_t_profiles": (_I32, [_P, _P, _P, _I32, _I32, _PD, _I32, _I32, _PD, _P]),
+    "tt_prob_t_profiles_noalloc": (_I32, [_P, _P, _P, _I32, _I32, _PD, _I32, _I32, _PD, _P]),
     "tt_batch_marginal": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PD, _P]),
     "tt_batch_info": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PI32, _PI32, _PI32, _PI32, _PD]),
     "tt_tree_set_site_model": (_I32, [_P, _PD, _PD, _PD, _PD, _PD, _PD, _I32, _D]),

@@ -65,6 +65,9 @@ bool g_use_graphs() {
 
 }  // namespace
 
+// Error text for entry points defined in other translation units.
+int32_t ttk::set_error(int32_t code, const char *msg) { return fail(code, "%s", msg); }
+
 struct BatchPlan;
 
 struct tt_engine {

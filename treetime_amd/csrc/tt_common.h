@@ -149,6 +149,8 @@ hipError_t prob_t_profiles(const double *pp, const double *pc, const double *mul
                            int K, int gap, double *block_out, int nblk, hipStream_t s);
 int root_blocks(int P);
 bool mfma_enabled();
+// sets tt_last_error()'s text on the calling thread and returns code (tt_api.hip)
+int32_t set_error(int32_t code, const char *msg);
 // joint branch-length optimisation (tt_branch.hip)
 hipError_t branch_opt(const TreeDev &T, const int16_t *code_state, int gap, const double *eig, const double *v,
                       const double *vinv, double mu, double max_t, double tol, double *counts, double *t_out,
