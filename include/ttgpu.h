@@ -38,10 +38,17 @@ enum {
 
 /* flags for tt_marginal / tt_joint */
 #define TT_RECONSTRUCT_TIPS 1u  /* reconstruct_tip_states=True (treeanc.py:900-903, 1043-1045) */
+#define TT_KEEP_NORMALISERS 2u  /* tt_marginal: from this pass on, the postorder also keeps every node's log
+                                   normaliser (debug=True, treeanc.py:803-810; see tt_get_subtree_prefactor);
+                                   shared models only */
 
 /* which= values for tt_get_profiles */
 #define TT_PROFILE_MARGINAL 0   /* node.marginal_profile (treeanc.py:909-917, 822-824) */
-#define TT_PROFILE_SUBTREE 1    /* node.marginal_subtree_LH, row-normalised (treeanc.py:877) */
+#define TT_PROFILE_SUBTREE 1    /* node.marginal_subtree_LH, row-normalised (treeanc.py:877); tips: the
+                                   character's profile (treeanc.py:847-853) */
+#define TT_PROFILE_OUTGROUP 2   /* node.marginal_outgroup_LH (treeanc.py:895-899); root: pi (:817-820) */
+#define TT_PROFILE_LOG_MSG 3    /* node.marginal_log_Lx, the log message to the parent (treeanc.py:862-866,
+                                   kept with debug=True, :803-810); root: zeros */
 
 typedef struct tt_engine tt_engine;
 typedef struct tt_tree tt_tree;
@@ -123,6 +130,29 @@ int32_t tt_loglik(tt_tree *tree, double *logL, void *stream);
  * tree.sequence_joint_LH (treeanc.py:1037). */
 int32_t tt_joint(tt_tree *tree, uint32_t flags, double *logL, void *stream);
 
+/* tt_get_joint_root: after tt_joint, the root's joint log message less its
+ * per-pattern maximum, out[P][n] (host) -- normalized_profile of
+ * treeanc.py:1017-1019, from which the reference samples the root state
+ * (sample_from_profile='root' or True, treeanc.py:1022-1033). */
+int32_t tt_get_joint_root(tt_tree *tree, double *out);
+
+/* tt_get_joint_debug: the debug attributes of the last joint pass
+ * (treeanc.py:1066-1077): lx[n_nodes][P][n] (host) = node.joint_Lx, the log
+ * message of every non-root node to its parent and, at the root, log pi plus
+ * the children's messages (treeanc.py:1017); cx (nullable, host
+ * [n_nodes][P][n]) = node.joint_Cx, the argmax state of each node per parent
+ * state (the root's row is unused).  Recomputed on the device from the
+ * stored messages. */
+int32_t tt_get_joint_debug(tt_tree *tree, double *lx, uint8_t *cx);
+
+/* tt_joint_from_root: finish the last tt_joint pass from given root states
+ * root_states[P] (host, state indices < n): backtrack every node from them
+ * and set sequence_LH to the root's joint log message at those states
+ * (treeanc.py:1034-1063, np.choose at :1036); *logL receives
+ * tree.sequence_joint_LH (:1037).  flags must equal the tt_joint call's.
+ * TT_ESTATE unless the last pass on the tree was a joint pass. */
+int32_t tt_joint_from_root(tt_tree *tree, uint32_t flags, const uint8_t *root_states, double *logL, void *stream);
+
 /* tt_update_branch: change one branch length and recompute the marginal logL
  * along the path from that branch to the root only.  Equivalent to setting
  * the length and re-running tt_loglik (treeanc.py:1685-1689 pattern).  The
@@ -198,9 +228,18 @@ int32_t tt_branch_state_counts(tt_tree *tree, const int16_t *code_state, int32_t
  * multiplicities into nij[i][j] (host, [n][n]) and the time in each state
  * Ti[i] (host, [n]: half the branch length for the parent's and half for the
  * child's state); per_branch (nullable, host [n_nodes][n*n + n]) gets each
- * branch's share.  Alphabets of up to 5 states; TT_ESTATE unless the profiles
- * are those of the current lengths. */
+ * branch's share.  Shared (not site-specific) models, every alphabet size;
+ * TT_ESTATE unless the profiles are those of the current lengths. */
 int32_t tt_mutation_counts(tt_tree *tree, double *nij, double *Ti, double *per_branch, void *stream);
+
+/* tt_branch_mutation_matrix: get_branch_mutation_matrix (treeanc.py:1085-1120,
+ * compressed) of the branch above `node` from the last marginal pass:
+ * out[P][n][n] (host) = the normalised joint distribution of (child state i,
+ * parent state j) per pattern, pc_i pp_j (P(t)_ij + 1e-24) / Z with
+ * pc = marginal_subtree_LH and pp = marginal_outgroup_LH of the node
+ * (marginal_branch_profile, :1122-1146).  TT_EINVAL for the root, TT_ESTATE
+ * unless the profiles are those of the current lengths. */
+int32_t tt_branch_mutation_matrix(tt_tree *tree, int32_t node, double *out);
 
 /* Free the joint-pass buffers (Lx, Cx) and return them to the engine's
  * budget; the next tt_joint allocates them again.  No reference counterpart:
@@ -222,9 +261,18 @@ int32_t tt_get_sequence_lh(tt_tree *tree, double *out);
 /* Reconstructed state index per node and pattern, [n_nodes][P]; 255 where a
  * node has no reconstruction (tips without TT_RECONSTRUCT_TIPS). */
 int32_t tt_get_states(tt_tree *tree, uint8_t *out);
-/* Profiles, [n_nodes][P][n_states] (float32): which = TT_PROFILE_*.  Rows the
- * last pass did not produce hold an earlier pass's values (or zero). */
+/* Profiles, [n_nodes][P][n_states] (float32): which = TT_PROFILE_*.
+ * TT_PROFILE_MARGINAL: rows the last pass did not produce hold an earlier
+ * pass's values (or zero).  The other kinds are formed on the device from
+ * the last marginal pass (TT_ESTATE if the profiles are stale). */
 int32_t tt_get_profiles(tt_tree *tree, int32_t which, float *out);
+/* node.marginal_subtree_LH_prefactor (treeanc.py:857-878, kept with
+ * debug=True): out[n_nodes][P] (host), the log factor between a node's
+ * row-normalised subtree likelihood and the unnormalised one; tips 0.
+ * TT_ESTATE unless the last marginal pass ran with TT_KEEP_NORMALISERS (or
+ * after tt_update_branch) at the current lengths. */
+int32_t tt_get_subtree_prefactor(tt_tree *tree, double *out);
+
 /* Device pointers of the resident buffers (for zero-copy consumers). */
 int32_t tt_get_device_buffers(tt_tree *tree, float **profiles, uint8_t **states, double **sequence_lh);
 

@@ -25,7 +25,8 @@ oracle function             reference
                             491-495 (average_rate)
 ``compress_alignment``      treetime/sequence_data.py:325-464
 ``OracleTreeAnc.marginal``  treetime/treeanc.py:762-932
-``OracleTreeAnc.joint``     treetime/treeanc.py:934-1080
+``OracleTreeAnc.joint``     treetime/treeanc.py:934-1080 (root_rng: the
+                            root draw of 1022-1037, seq_utils.py:257-269)
 ``ancestral_likelihood``    treetime/treeanc.py:718-750
 ``branch_t``                treetime/treeanc.py:752-760, config.py:7
 ==========================  =================================================
@@ -42,6 +43,11 @@ Pinning (tests/test_oracle_*.py):
 * marginal profiles: exact posteriors by enumeration on <=5-tip trees.
 * joint: brute-force max over internal states (test_treetime.py:241-307);
   ancestral_likelihood == sequence_LH (test_treetime.py:160-238).
+* node attributes by enumeration on a 5-tip tree: marginal_outgroup_LH,
+  marginal_log_Lx + marginal_subtree_LH_prefactor, subtree likelihood
+  (prefactor x marginal_subtree_LH); joint_Lx / joint_Cx of every node; the
+  root draw of joint(root_rng=...) with fixed draws, and the argmax states
+  and sequence_LH given the drawn root.
 * compression: hand-built columns hitting every rule of
   sequence_data.py:387-402.
 * prob_t_profiles: one-hot profiles reduce to prob_t_compressed.
@@ -693,9 +699,14 @@ class OracleTreeAnc:
         return self.total_sequence_LH
 
     # -- joint ---------------------------------------------------------------
-    def joint(self, reconstruct_tip_states=False):
+    def joint(self, reconstruct_tip_states=False, root_rng=None):
         """treetime/treeanc.py:934-1080.  Also records, per node, the decision
-        margin (best minus second-best log value) behind each chosen state."""
+        margin (best minus second-best log value) behind each chosen state.
+        root_rng: draw the root states from the root's joint profile with
+        root_rng.random(L) (sample_from_profile='root' or True,
+        treeanc.py:1022-1037, prof2seq sampling at seq_utils.py:257-269) and
+        record, per pattern, the distance of the draw from the nearest
+        cumulative boundary (root.sample_margin)."""
         n = self.gtr.n_states
         for node in postorder(self.root):
             if node.up is None:
@@ -721,9 +732,17 @@ class OracleTreeAnc:
         msg = np.sum(np.stack([c.joint_Lx for c in r.clades], axis=0), axis=0)
         r.joint_Lx = msg + np.log(self._root_prior())
         normed = r.joint_Lx - r.joint_Lx.max(axis=1)[:, None]
-        _, _, idx = prof2seq(np.exp(normed), self.gtr.alphabet)
-        srt = np.sort(r.joint_Lx, axis=1)
-        r.path_margin = srt[:, -1] - srt[:, -2]
+        if root_rng is None:
+            _, _, idx = prof2seq(np.exp(normed), self.gtr.alphabet)
+            srt = np.sort(r.joint_Lx, axis=1)
+            r.path_margin = srt[:, -1] - srt[:, -2]
+        else:
+            prof, _ = normalize_profile(np.exp(normed), return_offset=False)
+            cum = prof.cumsum(axis=1).T
+            u = root_rng.random(size=cum.shape[1])
+            idx = np.argmax(cum >= u, axis=0)
+            r.sample_margin = np.min(np.abs(cum - u[None, :]), axis=0)
+            r.path_margin = np.full(self.L, np.inf)  # the root state is drawn, not decided
         self.joint_sequence_LH = r.joint_Lx[np.arange(self.L), idx]
         self.total_joint_LH = float((self.joint_sequence_LH * self.mult).sum())
         r.seq_idx = idx

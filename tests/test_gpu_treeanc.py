@@ -214,3 +214,87 @@ def test_sample_from_profile():
         idx = np.argmax(cum >= rng.random(size=cum.shape[1]), axis=0)
         assert np.array_equal(np.array(alpha)[idx], s1[id(c)])
     say(f"  {n_sampled} nodes sampled")
+
+
+def _oracle_of(t):
+    nodes = list(t.tree.find_clades(order="preorder"))
+    idx = {id(x): i for i, x in enumerate(nodes)}
+    par = [-1] * len(nodes)
+    for i, x in enumerate(nodes):
+        for c in x.clades:
+            par[idx[id(c)]] = i
+    root, onodes = O.build_tree(par, [t._branch_length_to_gtr(x) for x in nodes], [x.name for x in nodes])
+    g = t.gtr
+    ta = O.OracleTreeAnc(root, O.GTRModel("nuc_nogap", mu=g.mu, pi=g.Pi, W=g.W), t.data.compressed_alignment,
+                         t.data.multiplicity(), full_length=1e300)
+    return ta, nodes, onodes
+
+
+@pytest.mark.parametrize("mode", ["root", True])
+def test_joint_sample_from_profile(mode):
+    """infer_ancestral_sequences(marginal=False, sample_from_profile='root' or
+    True) (treeanc.py:1022-1037, the call of ClockTree.init_date_constraints,
+    clock_tree.py:335-338): the root states are drawn with the TreeAnc's
+    generator, one rng.random(P), and the rest follow the joint argmax;
+    sequence_LH is the root's joint log message at the drawn states."""
+    say(f"[treeanc] joint with sample_from_profile={mode!r}")
+    t = _sim_treeanc(52, 11)
+    t.infer_ancestral_sequences("probabilistic", marginal=False, sample_from_profile=mode)
+    ta, nodes, onodes = _oracle_of(t)
+    ref = ta.joint(root_rng=np.random.default_rng(11))
+    alpha = np.array(list(t.gtr.alphabet))
+    sure = ta.root.sample_margin > 1e-5
+    assert np.array_equal(t.tree.root.cseq[sure], alpha[ta.root.seq_idx][sure])
+    if sure.all():
+        assert abs(t.tree.sequence_joint_LH - ref) <= 1e-7 * abs(ref)
+        assert np.all(np.abs(t.tree.sequence_LH - ta.joint_sequence_LH)
+                      <= 5e-7 * np.maximum(1.0, np.abs(ta.joint_sequence_LH)))
+        for x, o in zip(nodes, onodes):
+            if not x.is_terminal():
+                ok = o.path_margin > 1e-5
+                assert np.array_equal(x.cseq[ok], alpha[o.seq_idx][ok])
+    # the generator moved by exactly one draw of P numbers
+    t2 = _sim_treeanc(52, 11)
+    t2.rng.random(size=t2.data.compressed_length)
+    assert t.rng.random() == t2.rng.random()
+
+
+def test_debug_attributes_and_branch_profiles():
+    """debug=True keeps marginal_log_Lx / marginal_subtree_LH_prefactor
+    (treeanc.py:803-810) and joint_Lx / joint_Cx / seq_idx (1066-1077);
+    marginal_branch_profile and get_branch_mutation_matrix (1085-1146)."""
+    say("[treeanc] debug attributes and branch profiles")
+    t = _sim_treeanc(53, 3)
+    t.infer_ancestral_sequences(marginal=True, debug=True)
+    ta, nodes, onodes = _oracle_of(t)
+    ta.marginal()
+    for x, o in zip(nodes, onodes):
+        assert np.max(np.abs(x.marginal_outgroup_LH - o.marginal_outgroup_LH)) <= 2e-6
+        assert np.max(np.abs(x.marginal_subtree_LH_prefactor - o.marginal_subtree_LH_prefactor)
+                      - 5e-7 * np.maximum(1, np.abs(o.marginal_subtree_LH_prefactor))) <= 2e-6
+        if x.up is not None:
+            assert np.max(np.abs(x.marginal_log_Lx - o.marginal_log_Lx)
+                          - 5e-7 * np.maximum(1, np.abs(o.marginal_log_Lx))) <= 2e-6
+            pp, pc = t.marginal_branch_profile(x)
+            assert pp is x.marginal_outgroup_LH and pc is x.marginal_subtree_LH
+    x, o = nodes[3], onodes[3]
+    m = t.get_branch_mutation_matrix(x)
+    assert np.max(np.abs(m - ta.branch_mutation_matrix(o))) <= 2e-6
+    assert t.get_branch_mutation_matrix(x, full_sequence=True).shape[0] == t.data.full_length
+    with pytest.raises(Exception):
+        t.marginal_branch_profile(t.tree.root)
+    t2 = _sim_treeanc(53, 3)
+    t2.infer_ancestral_sequences(marginal=True)
+    assert not hasattr(t2.tree.root.clades[0], "marginal_log_Lx")
+    t.infer_ancestral_sequences(marginal=False, debug=True)
+    ta.joint()
+    for x, o in zip(nodes, onodes):
+        assert np.max(np.abs(x.joint_Lx - o.joint_Lx) - 5e-7 * np.maximum(1, np.abs(o.joint_Lx))) <= 2e-6
+        if x.up is None:
+            assert x.joint_Cx is None
+        else:
+            ok = o.joint_margin > 1e-5
+            assert np.array_equal(x.joint_Cx[ok], o.joint_Cx[ok])
+        if not x.is_terminal():
+            ok = o.path_margin > 1e-5
+            assert np.array_equal(x.seq_idx[ok], o.seq_idx[ok])

@@ -590,3 +590,151 @@ def test_gtr_infer_gap_floor_is_unconditional():
         assert abs(pi[4] - want_gap) < 2e-4, pi
         assert np.allclose(pi[:4] / pi[:4].sum(), pi0[:4] / pi0[:4].sum(), atol=2e-4)
         assert abs(pi.sum() - 1.0) < 1e-12
+
+
+# ---------------------------------------- per-node attributes by enumeration --
+def _subtree(nd):
+    out = [nd]
+    for c in nd.clades:
+        out += _subtree(c)
+    return out
+
+
+def _branch_P(model, nd, full_length):
+    return np.maximum(expm(model.mu * model.Q * max(O.MIN_BRANCH_LENGTH / full_length, nd.branch_length)), 0.0)
+
+
+def _below_given_parent(model, c, tips, full_length, reduce):
+    """(L, n): for every state j of c's parent, reduce (sum or max) over the
+    states of the internal nodes of c's subtree (c included) of the
+    probability of those states and of the subtree's tip data."""
+    sub = _subtree(c)
+    internal = [x for x in sub if not x.is_terminal()]
+    n = model.n_states
+    L = next(iter(tips.values())).shape[0]
+    P = {x.name: _branch_P(model, x, full_length) for x in sub}
+    out = np.zeros((L, n)) if reduce == "sum" else np.full((L, n), -np.inf)
+    for j in range(n):
+        for combo in itertools.product(range(n), repeat=len(internal)):
+            st = {x.name: s for x, s in zip(internal, combo)}
+            w = np.ones(L)
+            for x in sub:
+                ps = j if x is c else st[x.up.name]
+                w = w * (tips[x.name] @ P[x.name][:, ps] if x.is_terminal() else P[x.name][st[x.name], ps])
+            out[:, j] = out[:, j] + w if reduce == "sum" else np.maximum(out[:, j], np.log(w))
+    return out
+
+
+def _above(model, root, nodes, c, tips, full_length):
+    """(L, n): probability of all data outside c's subtree (branch above c
+    excluded) with c's parent in state j, by enumeration."""
+    inside = set(id(x) for x in _subtree(c))
+    outside = [x for x in nodes if id(x) not in inside]
+    internal = [x for x in outside if not x.is_terminal()]
+    n = model.n_states
+    L = next(iter(tips.values())).shape[0]
+    P = {x.name: _branch_P(model, x, full_length) for x in outside if x.up is not None}
+    out = np.zeros((L, n))
+    for combo in itertools.product(range(n), repeat=len(internal)):
+        st = {x.name: s for x, s in zip(internal, combo)}
+        w = np.full(L, model.Pi[st[root.name]])
+        for x in outside:
+            if x.up is None:
+                continue
+            ps = st[x.up.name]
+            w = w * (tips[x.name] @ P[x.name][:, ps] if x.is_terminal() else P[x.name][st[x.name], ps])
+        out[:, st[c.up.name]] += w
+    return out
+
+
+@pytest.mark.parametrize("seed", [6, 7])
+def test_outgroup_prefactor_and_log_messages_match_enumeration(seed):
+    """marginal_outgroup_LH (treeanc.py:895-899) = normalised probability of
+    the data outside a node's subtree per parent state; exp(prefactor) times
+    marginal_subtree_LH (treeanc.py:857-878) = probability of the subtree's
+    tips per node state; exp(marginal_log_Lx + prefactor) = probability of
+    the subtree's tips per parent state."""
+    root, nodes, model, aln = five_tip_case(seed, amb=True)
+    comp, mult, _, _ = O.compress_alignment(list("ABCDE"), aln, "N")
+    ta = O.OracleTreeAnc(root, model, comp, mult, full_length=40)
+    ta.marginal()
+    tips = {k: O.seq2prof(comp[k], model.profile_map) for k in "ABCDE"}
+    for nd in nodes:
+        if nd.up is None:
+            assert np.allclose(nd.marginal_outgroup_LH, model.Pi[None, :])
+            continue
+        above = _above(model, root, nodes, nd, tips, 40)
+        assert np.allclose(nd.marginal_outgroup_LH, above / above.sum(axis=1)[:, None], atol=1e-10), nd.name
+        below = _below_given_parent(model, nd, tips, 40, "sum")
+        assert np.allclose(np.exp(nd.marginal_log_Lx + nd.marginal_subtree_LH_prefactor[:, None]), below,
+                           rtol=1e-10, atol=1e-300), nd.name
+        if not nd.is_terminal():
+            own = np.zeros((len(mult), model.n_states))  # tips of the subtree given the node's own state
+            for j in range(model.n_states):
+                w = np.ones(len(mult))
+                for ch in nd.clades:
+                    w = w * _below_given_parent(model, ch, tips, 40, "sum")[:, j]
+                own[:, j] = w
+            assert np.allclose(np.exp(nd.marginal_subtree_LH_prefactor)[:, None] * nd.marginal_subtree_LH, own,
+                               rtol=1e-10), nd.name
+
+
+def test_joint_messages_and_root_sampling_match_enumeration():
+    """joint_Lx / joint_Cx (treeanc.py:960-1011) of every non-root node =
+    max over its subtree's internal states per parent state, and the argmax
+    state of the node; root joint_Lx = log pi + children's messages; with a
+    sampled root (treeanc.py:1022-1037) the root state is the first
+    cumulative bin of the normalised root profile at or above the draw, the
+    other states the argmax given it, and sequence_LH the brute-force max
+    with the root fixed to the drawn state."""
+    root, nodes, model, aln = five_tip_case(8, amb=False)
+    comp, mult, _, _ = O.compress_alignment(list("ABCDE"), aln, "N")
+    tips = {k: O.seq2prof(comp[k], model.profile_map) for k in "ABCDE"}
+    L, n = len(mult), model.n_states
+
+    class Draws:  # stand-in generator: fixed draws
+        def __init__(self, u):
+            self.u = u
+
+        def random(self, size):
+            assert size == len(self.u)
+            return self.u
+
+    u = np.random.default_rng(3).random(L)
+    ta = O.OracleTreeAnc(root, model, comp, mult, full_length=1e9)
+    ta.joint(root_rng=Draws(u))
+    for nd in nodes:
+        if nd.up is None:
+            continue
+        ref = _below_given_parent(model, nd, tips, 1e9, "max")
+        assert np.allclose(nd.joint_Lx, ref, atol=1e-10), nd.name
+        if not nd.is_terminal():  # argmax of the node's own state per parent state
+            down = sum(_below_given_parent(model, ch, tips, 1e9, "max") for ch in nd.clades)
+            logP = np.log(_branch_P(model, nd, 1e9))
+            cand = np.stack([down[:, i][:, None] + logP[i][None, :] for i in range(n)], axis=2)
+            assert np.array_equal(nd.joint_Cx, cand.argmax(axis=2)), nd.name
+    rootlx = np.log(model.Pi)[None, :] + sum(_below_given_parent(model, ch, tips, 1e9, "max") for ch in root.clades)
+    assert np.allclose(root.joint_Lx, rootlx, atol=1e-10)
+    prof = np.exp(rootlx - rootlx.max(axis=1)[:, None])
+    prof /= prof.sum(axis=1)[:, None]
+    want = np.array([np.searchsorted(np.cumsum(prof[p]), u[p]) for p in range(L)])
+    assert np.array_equal(root.seq_idx, want)
+    assert np.allclose(ta.joint_sequence_LH, rootlx[np.arange(L), want], atol=1e-10)
+    # states below the root: argmax given the drawn root state (brute force)
+    internal = [x for x in nodes if not x.is_terminal() and x.up is not None]
+    tipidx = {k: np.array([{"A": 0, "C": 1, "G": 2, "T": 3}[c] for c in comp[k]]) for k in "ABCDE"}
+    best = np.full(L, -np.inf)
+    arg = {x.name: np.zeros(L, dtype=int) for x in internal}
+    for combo in itertools.product(range(n), repeat=len(internal)):
+        seqs = dict(tipidx)
+        seqs[root.name] = want
+        for x, s in zip(internal, combo):
+            seqs[x.name] = np.full(L, s)
+        lh = ta.ancestral_likelihood(seqs)
+        better = lh > best
+        best = np.where(better, lh, best)
+        for x, s in zip(internal, combo):
+            arg[x.name][better] = s
+    assert np.allclose(best, ta.joint_sequence_LH, atol=1e-10)
+    for x in internal:
+        assert np.array_equal(x.seq_idx, arg[x.name]), x.name

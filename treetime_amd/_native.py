@@ -23,8 +23,11 @@ TT_ERANGE = -4
 TT_ESTATE = -5
 
 TT_RECONSTRUCT_TIPS = 1
+TT_KEEP_NORMALISERS = 2
 TT_PROFILE_MARGINAL = 0
 TT_PROFILE_SUBTREE = 1
+TT_PROFILE_OUTGROUP = 2
+TT_PROFILE_LOG_MSG = 3
 TT_BATCH_MARGINAL = 0x00
 TT_BATCH_JOINT = 0x10
 TT_BATCH_LOGLIK = 0x20
@@ -93,6 +96,11 @@ SIGNATURES = {
     "tt_branch_profile_lh": (_I32, [_P, _I32, _PD, _I32, _PD, _P]),
     "tt_branch_state_counts": (_I32, [_P, _PI16, _I32, _PD, _P]),
     "tt_mutation_counts": (_I32, [_P, _PD, _PD, _PD, _P]),
+    "tt_branch_mutation_matrix": (_I32, [_P, _I32, _PD]),
+    "tt_get_joint_root": (_I32, [_P, _PD]),
+    "tt_joint_from_root": (_I32, [_P, _U32, _PU8, _PD, _P]),
+    "tt_get_joint_debug": (_I32, [_P, _PD, _PU8]),
+    "tt_get_subtree_prefactor": (_I32, [_P, _PD]),
     "tt_tree_trim": (_I32, [_P]),
     "tt_tree_query": (_I32, [_P]),
     "tt_get_total": (_I32, [_P, _PD]),

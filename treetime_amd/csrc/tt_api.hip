@@ -166,6 +166,7 @@ struct tt_tree {
     std::map<int, hipGraphExec_t> graphs;
     std::vector<std::pair<void *, size_t>> allocs;
     int last_pass = 0;  // 1 marginal, 2 joint, 3 loglik
+    hipStream_t last_stream = nullptr;  // stream of the last pass (result reads wait for it)
     bool have_states = false;  // d.states holds a marginal or joint reconstruction
     bool tips_dirty = false;   // tip rows of d.states hold reconstructed states (else 255)
     bool profiles_fresh = false;  // d.prof and d.sub belong to the current lengths (last marginal pass)
@@ -424,7 +425,8 @@ int32_t ensure_joint(tt_tree *T) {
     return TT_OK;
 }
 
-// Enqueue one pass on stream s.  mode: 1 marginal, 2 joint, 3 loglik.
+// Enqueue one pass on stream s.  mode: 1 marginal, 2 joint, 3 loglik,
+// 4 the joint pass's backtrack and score from caller-set root states.
 // flag of enqueue_pass (not part of the ABI): reset the tip rows of the
 // states to 255 because an earlier pass reconstructed them
 constexpr uint32_t kFillTips = 0x100u;
@@ -432,6 +434,33 @@ constexpr uint32_t kFillTips = 0x100u;
 constexpr uint32_t kPlanLoglik = 2u;
 // plan flag of a joint batched pass
 constexpr uint32_t kPlanJoint = 4u;
+
+// Joint pass after the root states are set: backtrack (treeanc.py:1048-1063)
+// and the log-likelihood of the decoded states.  given_root: the root states
+// came from the caller (sampled, treeanc.py:1022-1037) rather than from the
+// root kernel; a site-specific model then takes sequence_LH from the root's
+// stored log message at those states.
+int32_t enqueue_joint_tail(tt_tree *T, bool tips, bool given_root, hipStream_t s) {
+    const TreeDev &D = T->d;
+    const int nb = ttk::root_blocks(T->P);
+    for (auto &L : T->pre_lv) {
+        LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off),
+                    (decltype(LevelDev::desc))(T->d_pre_desc + kPreDesc * L.off), tips ? L.n_all : L.n_int, 1, 0, 0};
+        TT_HIP(ttk::backtrack(D, ld, s));
+    }
+    if (!T->site) {  // logL: FP64 score of the decoded states, slices of G node ids (S <= R_all rows)
+        const int G = std::max(16, (T->n_nodes + std::max(T->R_all, 1) - 1) / std::max(T->R_all, 1));
+        const int S = (T->n_nodes + G - 1) / G, S2 = rowsum_slices(S);
+        TT_HIP(ttk::joint_score(D, S, G, s));
+        TT_HIP(ttk::rowsum(D, S, S2, s));
+        TT_HIP(ttk::joint_final(D, S2, s));
+        TT_HIP(ttk::final_sum(D, nb, s));
+    } else if (given_root) {  // part2 still holds the node offsets of the joint pass
+        TT_HIP(ttk::root_given_site(D, rowsum_slices(T->R_joint), s));
+        TT_HIP(ttk::final_sum(D, nb, s));
+    }
+    return TT_OK;
+}
 
 int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
     const TreeDev &D = T->d;
@@ -482,6 +511,8 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
                 TT_HIP(T->site ? ttk::pre_site(D, ld, s) : ttk::pre_marginal(D, ld, s));
             }
         }
+    } else if (mode == 4) {
+        if (int32_t rc = enqueue_joint_tail(T, tips, true, s)) return rc;
     } else {
         for (size_t h = 0; h + 1 < T->post_lv.size(); ++h) {
             auto &L = T->post_lv[h];
@@ -499,21 +530,17 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
             TT_HIP(ttk::root_joint(D, tips ? 1 : 0, s));
         }
         if (fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
-        for (auto &L : T->pre_lv) {
-            LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off),
-                            (decltype(LevelDev::desc))(T->d_pre_desc + kPreDesc * L.off), tips ? L.n_all : L.n_int, 1,
-                            0, 0};
-            TT_HIP(ttk::backtrack(D, ld, s));
-        }
-        if (!T->site) {  // logL: FP64 score of the decoded states, slices of G node ids (S <= R_all rows)
-            const int G = std::max(16, (T->n_nodes + std::max(T->R_all, 1) - 1) / std::max(T->R_all, 1));
-            const int S = (T->n_nodes + G - 1) / G, S2 = rowsum_slices(S);
-            TT_HIP(ttk::joint_score(D, S, G, s));
-            TT_HIP(ttk::rowsum(D, S, S2, s));
-            TT_HIP(ttk::joint_final(D, S2, s));
-            TT_HIP(ttk::final_sum(D, nb, s));
-        }
+        if (int32_t rc = enqueue_joint_tail(T, tips, false, s)) return rc;
     }
+    return TT_OK;
+}
+
+// Result reads wait for the tree's own stream and the stream of its last
+// pass, not for the whole device.
+int32_t wait_results(tt_tree *T) {
+    TT_HIP(hipSetDevice(T->eng->device));
+    if (T->last_stream && T->last_stream != T->stream) TT_HIP(hipStreamSynchronize(T->last_stream));
+    TT_HIP(hipStreamSynchronize(T->stream));
     return TT_OK;
 }
 
@@ -553,7 +580,8 @@ int32_t run_pass(tt_tree *T, int mode, uint32_t flags, double *logL, void *strea
         int32_t rc = enqueue_pass(T, mode, flags, s);
         if (rc) return rc;
     }
-    T->last_pass = mode;
+    T->last_pass = mode == 4 ? 2 : mode;
+    T->last_stream = s;
     if (mode == 1 || mode == 2) {
         T->have_states = true;
         T->tips_dirty = tips;
@@ -1111,7 +1139,12 @@ int32_t set_lengths(tt_tree *T, const int32_t *nodes, const double *t, int32_t k
 }  // namespace
 
 int32_t tt_marginal(tt_tree *tree, uint32_t flags, double *logL, void *stream) {
-    return run_pass(tree, 1, flags, logL, stream);
+    if (tree && (flags & TT_KEEP_NORMALISERS)) {
+        if (tree->site) return fail(TT_EINVAL, "TT_KEEP_NORMALISERS: not available for site-specific models");
+        TT_HIP(hipSetDevice(tree->eng->device));
+        if (int32_t rc = enable_update_mode(tree)) return rc;
+    }
+    return run_pass(tree, 1, flags & TT_RECONSTRUCT_TIPS, logL, stream);
 }
 
 int32_t tt_loglik(tt_tree *tree, double *logL, void *stream) { return run_pass(tree, 3, 0, logL, stream); }
@@ -1144,6 +1177,7 @@ int32_t tt_update_branch(tt_tree *T, int32_t node, double t, double *logL, void 
     TT_HIP(ttk::path_update(T->d, node, s));
     TT_HIP(ttk::final_sum(T->d, ttk::root_blocks(T->P), s));
     T->last_pass = 3;
+    T->last_stream = s;
     T->profiles_fresh = false;
     if (logL) {
         TT_HIP(hipMemcpyAsync(logL, T->d.total, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1278,7 +1312,6 @@ int32_t tt_branch_state_counts(tt_tree *T, const int16_t *code_state, int32_t ga
 int32_t tt_mutation_counts(tt_tree *T, double *nij, double *Ti, double *per_branch, void *stream) {
     if (!T || !nij || !Ti) return fail(TT_EINVAL, "NULL argument");
     if (int32_t r = refuse_site(T, "tt_mutation_counts")) return r;
-    if (T->n > 5) return fail(TT_EINVAL, "mutation counts are built for alphabets of up to 5 states (n = %d)", T->n);
     if (!T->profiles_fresh)
         return fail(TT_ESTATE, "marginal ancestral inference needs to be performed first (profiles are stale)");
     TT_HIP(hipSetDevice(T->eng->device));
@@ -1387,8 +1420,7 @@ int32_t tt_tree_query(tt_tree *T) {
 int32_t tt_get_total(tt_tree *T, double *logL) {
     if (!T || !logL) return fail(TT_EINVAL, "NULL argument");
     if (!T->last_pass) return fail(TT_ESTATE, "no pass has run on this tree");
-    TT_HIP(hipSetDevice(T->eng->device));
-    TT_HIP(hipStreamSynchronize(T->stream));
+    if (int32_t rc = wait_results(T)) return rc;
     TT_HIP(hipMemcpy(logL, T->d.total, sizeof(double), hipMemcpyDeviceToHost));
     return TT_OK;
 }
@@ -1396,8 +1428,7 @@ int32_t tt_get_total(tt_tree *T, double *logL) {
 int32_t tt_get_sequence_lh(tt_tree *T, double *out) {
     if (!T || !out) return fail(TT_EINVAL, "NULL argument");
     if (!T->last_pass) return fail(TT_ESTATE, "no pass has run on this tree");
-    TT_HIP(hipSetDevice(T->eng->device));
-    TT_HIP(hipDeviceSynchronize());
+    if (int32_t rc = wait_results(T)) return rc;
     TT_HIP(hipMemcpy(out, T->last_pass == 2 ? T->d.jseqlh : T->d.seqlh, (size_t)T->P * sizeof(double),
                      hipMemcpyDeviceToHost));
     return TT_OK;
@@ -1406,8 +1437,7 @@ int32_t tt_get_sequence_lh(tt_tree *T, double *out) {
 int32_t tt_get_states(tt_tree *T, uint8_t *out) {
     if (!T || !out) return fail(TT_EINVAL, "NULL argument");
     if (!T->last_pass) return fail(TT_ESTATE, "no pass has run on this tree");
-    TT_HIP(hipSetDevice(T->eng->device));
-    TT_HIP(hipDeviceSynchronize());
+    if (int32_t rc = wait_results(T)) return rc;
     TT_HIP(hipMemcpy(out, T->d.states, (size_t)T->n_nodes * T->P, hipMemcpyDeviceToHost));
     return TT_OK;
 }
@@ -1415,28 +1445,160 @@ int32_t tt_get_states(tt_tree *T, uint8_t *out) {
 int32_t tt_get_profiles(tt_tree *T, int32_t which, float *out) {
     if (!T || !out) return fail(TT_EINVAL, "NULL argument");
     if (!T->last_pass) return fail(TT_ESTATE, "no pass has run on this tree");
-    TT_HIP(hipSetDevice(T->eng->device));
-    TT_HIP(hipDeviceSynchronize());
-    const int n = T->n, P = T->P;
-    const size_t row = (size_t)P * n;
+    if (which < TT_PROFILE_MARGINAL || which > TT_PROFILE_LOG_MSG)
+        return fail(TT_EINVAL, "unknown profile kind %d", which);
+    if (which != TT_PROFILE_MARGINAL && !T->profiles_fresh)
+        return fail(TT_ESTATE, "marginal ancestral inference needs to be performed first (profiles are stale)");
+    if (int32_t rc = wait_results(T)) return rc;
+    const size_t bytes = (size_t)T->n_nodes * T->P * T->n * sizeof(float);
     if (which == TT_PROFILE_MARGINAL) {
-        TT_HIP(hipMemcpy(out, T->d.prof, (size_t)T->n_nodes * row * sizeof(float), hipMemcpyDeviceToHost));
+        TT_HIP(hipMemcpy(out, T->d.prof, bytes, hipMemcpyDeviceToHost));
         return TT_OK;
     }
-    if (which != TT_PROFILE_SUBTREE) return fail(TT_EINVAL, "unknown profile kind %d", which);
-    std::vector<float> sub((size_t)T->n_int * row);
+    // derived rows are formed on the device into a staging buffer
+    hipStream_t s = T->stream;
+    float *d_rows = nullptr;
+    TT_HIP(hipMallocAsync((void **)&d_rows, std::max<size_t>(bytes, 16), s));
+    hipError_t e = ttk::node_rows(T->d, which, d_rows, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_rows, bytes, hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = hipFreeAsync(d_rows, s);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(TT_EDEVICE, "tt_get_profiles: %s", hipGetErrorString(e));
+    return TT_OK;
+}
+
+int32_t tt_branch_mutation_matrix(tt_tree *T, int32_t node, double *out) {
+    if (!T || !out) return fail(TT_EINVAL, "NULL argument");
+    if (node < 0 || node >= T->n_nodes) return fail(TT_EINVAL, "node %d out of range", node);
+    if (T->parent[node] < 0) return fail(TT_EINVAL, "Branch profiles can't be calculated for the root!");
+    if (!T->profiles_fresh)
+        return fail(TT_ESTATE, "marginal ancestral inference needs to be performed first!");
+    if (int32_t rc = wait_results(T)) return rc;
+    hipStream_t s = T->stream;
+    const size_t bytes = (size_t)T->P * T->n * T->n * sizeof(double);
+    double *d_m = nullptr;
+    TT_HIP(hipMallocAsync((void **)&d_m, std::max<size_t>(bytes, 16), s));
+    hipError_t e = ttk::branch_mutation_matrix(T->d, node, d_m, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_m, bytes, hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = hipFreeAsync(d_m, s);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(TT_EDEVICE, "tt_branch_mutation_matrix: %s", hipGetErrorString(e));
+    return TT_OK;
+}
+
+int32_t tt_get_joint_root(tt_tree *T, double *out) {
+    if (!T || !out) return fail(TT_EINVAL, "NULL argument");
+    if (T->last_pass != 2 || !T->joint_ready) return fail(TT_ESTATE, "the last pass on this tree is not a joint pass");
+    if (int32_t rc = wait_results(T)) return rc;
+    const int n = T->n, P = T->P;
+    const size_t cnt = (size_t)P * n, off = (size_t)T->root_row * cnt;
+    if (T->site) {  // log message: subtract the per-pattern maximum
+        TT_HIP(hipMemcpy(out, T->d.Lx64 + off, cnt * sizeof(double), hipMemcpyDeviceToHost));
+        for (int p = 0; p < P; ++p) {
+            double *r = out + (size_t)p * n, mx = r[0];
+            for (int j = 1; j < n; ++j) mx = std::max(mx, r[j]);
+            for (int j = 0; j < n; ++j) r[j] -= mx;
+        }
+    } else {  // product space, largest entry 1
+        std::vector<float> h(cnt);
+        TT_HIP(hipMemcpy(h.data(), T->d.Lx + off, cnt * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < cnt; ++i) out[i] = std::log((double)h[i]);
+    }
+    return TT_OK;
+}
+
+int32_t tt_get_joint_debug(tt_tree *T, double *lx, uint8_t *cx) {
+    if (!T || !lx) return fail(TT_EINVAL, "NULL argument");
+    if (T->last_pass != 2 || !T->joint_ready || T->joint_site != T->site)
+        return fail(TT_ESTATE, "the last pass on this tree is not a joint pass");
+    if (int32_t rc = wait_results(T)) return rc;
+    hipStream_t s = T->stream;
+    const size_t rows = (size_t)T->n_nodes * T->P, nrow = rows * T->n;
+    double *d_rel = nullptr;
+    TT_HIP(hipMallocAsync((void **)&d_rel, (rows + nrow) * sizeof(double), s));
+    double *d_own = d_rel + nrow;
+    std::vector<double> own(rows);
+    hipError_t e = T->site ? ttk::joint_rows_site(T->d, d_rel, d_own, s) : ttk::joint_rows(T->d, d_rel, d_own, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(lx, d_rel, nrow * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(own.data(), d_own, rows * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && cx) e = hipMemcpyAsync(cx, T->d.Cx, nrow, hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = hipFreeAsync(d_rel, s);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(TT_EDEVICE, "tt_get_joint_debug: %s", hipGetErrorString(e));
+    // absolute log messages: add, per internal node, the sum of the log
+    // maxima over its subtree's internal nodes (children before parents)
+    const int P = T->P, n = T->n;
+    for (int idx : T->post_nodes) {
+        const int v = T->int_node[idx];
+        double *ov = own.data() + (size_t)v * P;
+        for (int e0 = T->child_ptr[idx]; e0 < T->child_ptr[idx + 1]; ++e0) {
+            if (T->child_ref[e0] < 0) continue;
+            const double *oc = own.data() + (size_t)T->child_node[e0] * P;
+            for (int p = 0; p < P; ++p) ov[p] += oc[p];
+        }
+        double *row = lx + (size_t)v * P * n;
+        for (int p = 0; p < P; ++p)
+            for (int j = 0; j < n; ++j) row[(size_t)p * n + j] += ov[p];
+    }
+    return TT_OK;
+}
+
+int32_t tt_get_subtree_prefactor(tt_tree *T, double *out) {
+    if (!T || !out) return fail(TT_EINVAL, "NULL argument");
+    if (!T->update_mode || !T->offs_valid || !T->profiles_fresh)
+        return fail(TT_ESTATE, "the last marginal pass did not keep its node normalisers (TT_KEEP_NORMALISERS)");
+    if (int32_t rc = wait_results(T)) return rc;
+    const int P = T->P, n = T->n;
+    std::vector<double> offs((size_t)T->n_int * P);
+    std::vector<float> sub((size_t)T->n_int * P * n);
+    TT_HIP(hipMemcpy(offs.data(), T->d.offs, offs.size() * sizeof(double), hipMemcpyDeviceToHost));
     TT_HIP(hipMemcpy(sub.data(), T->d.sub, sub.size() * sizeof(float), hipMemcpyDeviceToHost));
-    std::memset(out, 0, (size_t)T->n_nodes * row * sizeof(float));
-    for (int r = 0; r < T->n_int; ++r) {
-        const float *s = sub.data() + (size_t)r * row;
-        float *o = out + (size_t)T->int_node[r] * row;
+    // prefactor = sum of the node normalisers over the subtree's internal
+    // nodes (children before parents) + log of the row sum of the stored row
+    std::memset(out, 0, (size_t)T->n_nodes * P * sizeof(double));
+    for (int idx : T->post_nodes) {
+        const int v = T->int_node[idx];
+        double *o = out + (size_t)v * P;
+        const double *own = offs.data() + (size_t)idx * P;
+        for (int p = 0; p < P; ++p) o[p] = own[p];
+        for (int e0 = T->child_ptr[idx]; e0 < T->child_ptr[idx + 1]; ++e0) {
+            const int cr = T->child_ref[e0];
+            if (cr < 0) continue;
+            const double *oc = offs.data() + (size_t)cr * P;  // child's own subtree sum, kept in offs below
+            for (int p = 0; p < P; ++p) o[p] += oc[p];
+        }
+        // keep the subtree sum for the parent
+        std::memcpy(offs.data() + (size_t)idx * P, o, (size_t)P * sizeof(double));
+    }
+    for (int idx = 0; idx < T->n_int; ++idx) {
+        double *o = out + (size_t)T->int_node[idx] * P;
+        const float *r = sub.data() + (size_t)idx * P * n;
         for (int p = 0; p < P; ++p) {
             double z = 0.0;
-            for (int j = 0; j < n; ++j) z += s[(size_t)p * n + j];
-            for (int j = 0; j < n; ++j) o[(size_t)p * n + j] = (float)(s[(size_t)p * n + j] / z);
+            for (int j = 0; j < n; ++j) z += r[(size_t)p * n + j];
+            o[p] += std::log(z);
         }
     }
     return TT_OK;
+}
+
+int32_t tt_joint_from_root(tt_tree *T, uint32_t flags, const uint8_t *root_states, double *logL, void *stream) {
+    if (!T || !root_states) return fail(TT_EINVAL, "NULL argument");
+    if (T->last_pass != 2 || !T->joint_ready || T->joint_site != T->site)
+        return fail(TT_ESTATE, "tt_joint_from_root needs the messages of a joint pass (the last pass on this tree)");
+    if (((flags & TT_RECONSTRUCT_TIPS) != 0) != T->tips_dirty)
+        return fail(TT_EINVAL, "reconstruct-tips flag differs from the last joint pass");
+    for (int p = 0; p < T->P; ++p)
+        if (root_states[p] >= T->n) return fail(TT_EINVAL, "root state %d at pattern %d >= n %d", root_states[p], p, T->n);
+    TT_HIP(hipSetDevice(T->eng->device));
+    hipStream_t s = stream ? (hipStream_t)stream : T->stream;
+    if (T->last_stream && T->last_stream != s) TT_HIP(hipStreamSynchronize(T->last_stream));
+    TT_HIP(hipMemcpyAsync(T->d.states + (size_t)T->root * T->P, root_states, T->P, hipMemcpyHostToDevice, s));
+    TT_HIP(hipStreamSynchronize(s));  // the caller's array may go away after the call
+    return run_pass(T, 4, flags & TT_RECONSTRUCT_TIPS, logL, stream);
 }
 
 int32_t tt_get_device_buffers(tt_tree *T, float **profiles, uint8_t **states, double **sequence_lh) {
@@ -1652,6 +1814,7 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
         return rc;
     }
     for (int i = 0; i < k; ++i) {
+        trees[i]->last_stream = s;
         if (loglik) {  // subtree rows rewritten, profiles and states not
             trees[i]->last_pass = 3;
             trees[i]->profiles_fresh = false;

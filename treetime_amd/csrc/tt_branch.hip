@@ -700,6 +700,198 @@ __global__ __launch_bounds__(kOptThreads) void k_mutation_counts(TreeDev T, cons
     }
 }
 
+// Mutation counts for wide alphabets (n > 5), the same sums as
+// k_mutation_counts: one workgroup per branch walks the patterns in chunks
+// of kOptThreads.  Phase 1 (thread = pattern) forms the branch profiles, the
+// weight w_p = mult_p / Z_p and the state-time vector
+//   tau_i = half_t w_p (pc_i (E pp)_i + pp_i (pc E)_i)
+// into LDS; phase 2 (thread = matrix entry (i, j), or state i for tau) adds
+// w_p pc_i pp_j E_ij over the chunk's patterns in pattern order, so every
+// entry is summed in a fixed order.
+template <int N>
+__global__ __launch_bounds__(kOptThreads) void k_mutation_counts_wide(TreeDev T, const double *t_eff, double *out) {
+    constexpr int NN = N * N;
+    __shared__ double s_pc[kOptThreads][N];
+    __shared__ double s_pp[kOptThreads][N];
+    __shared__ double s_w[kOptThreads];
+    __shared__ double s_tau[kOptThreads][N];
+    const int c = blockIdx.x;
+    double *o = out + (size_t)c * (NN + N);
+    if (T.parent[c] < 0) {
+        for (int k = threadIdx.x; k < NN + N; k += blockDim.x) o[k] = 0.0;
+        return;
+    }
+    const double TT_C *E = T.P64 + (size_t)c * NN;
+    const double half_t = 0.5 * t_eff[c];
+    constexpr int kPer = (NN + N + kOptThreads - 1) / kOptThreads;  // outputs per thread
+    double acc[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) acc[k] = 0.0;
+    for (int p0 = 0; p0 < T.P; p0 += kOptThreads) {
+        const int p = p0 + threadIdx.x;
+        if (p < T.P) {
+            double pc[N], pp[N];
+            branch_profiles<N>(T, c, -1, p, pc, pp);
+            double z = 0.0;
+            double ep[N], pe[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) ep[i] = pe[i] = 0.0;
+            for (int i = 0; i < N; ++i) {
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    const double e = E[i * N + j] + kSuperTiny;
+                    ep[i] = fma(e, pp[j], ep[i]);
+                    pe[j] = fma(pc[i], e, pe[j]);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i) z = fma(pc[i], ep[i], z);
+            const double w = T.mult[p] / z;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                s_pc[threadIdx.x][i] = pc[i];
+                s_pp[threadIdx.x][i] = pp[i];
+                s_tau[threadIdx.x][i] = half_t * w * (pc[i] * ep[i] + pp[i] * pe[i]);
+            }
+            s_w[threadIdx.x] = w;
+        }
+        __syncthreads();
+        const int np = min(kOptThreads, T.P - p0);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int q = threadIdx.x + k * kOptThreads;
+            if (q < NN) {
+                const int i = q / N, j = q % N;
+                const double e = E[q] + kSuperTiny;
+                double a = acc[k];
+                for (int r = 0; r < np; ++r) a = fma(s_w[r] * s_pc[r][i] * s_pp[r][j], e, a);
+                acc[k] = a;
+            } else if (q < NN + N) {
+                const int i = q - NN;
+                double a = acc[k];
+                for (int r = 0; r < np; ++r) a += s_tau[r][i];
+                acc[k] = a;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int q = threadIdx.x + k * kOptThreads;
+        if (q < NN + N) o[q] = acc[k];
+    }
+}
+
+// Rows derived from the last marginal pass, one thread per (node, pattern),
+// for the reference's node attributes:
+//   kind 1: marginal_subtree_LH (treeanc.py:857-877), internal rows
+//           normalised to sum 1, tips the character's profile (seq2prof,
+//           treeanc.py:847-853);
+//   kind 2: marginal_outgroup_LH (treeanc.py:895-899), normalise(max(TINY,
+//           parent profile) / child message); the root's is pi
+//           (treeanc.py:817-820);
+//   kind 3: marginal_log_Lx = log(marginal_subtree_LH @ P(t))
+//           (treeanc.py:862-866, GTR.propagate_profile with return_log; a
+//           site-specific model clamps at TINY, gtr_site_specific.py:404-406);
+//           the root has none (zeros).
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_node_rows(TreeDev T, int kind, float *out) {
+    const size_t gid = (size_t)blockIdx.x * TT_BLOCK + threadIdx.x;
+    if (gid >= (size_t)T.n_nodes * T.P) return;
+    const int c = (int)(gid / T.P), p = (int)(gid % T.P);
+    float *o = out + gid * N;
+    double pc[N], pp[N];
+    if (T.parent[c] < 0) {
+        if (kind == 1) {
+            const float TT_G *r = T.sub + ((size_t)T.root_row * T.P + p) * N;
+            double z = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) z += (double)r[i];
+#pragma unroll
+            for (int i = 0; i < N; ++i) o[i] = (float)((double)r[i] / z);
+        } else if (kind == 2) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) o[i] = (float)(T.s_pi ? T.s_pi[(size_t)p * N + i] : T.pi[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) o[i] = 0.0f;
+        }
+        return;
+    }
+    branch_profiles_any<N>(T, c, -1, p, pc, pp);
+    if (kind == 1) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) o[i] = (float)pc[i];
+    } else if (kind == 2) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) o[i] = (float)pp[i];
+    } else {
+        bool site = false;
+        if constexpr (N <= 5) site = T.s_V64 != nullptr;
+        if (site) {
+            if constexpr (N <= 5) {
+                double E[N * N];
+                site_P64<N>(T, c, p, E);
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    double m = 0.0;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) m = fma(pc[i], E[i * N + j], m);
+                    o[j] = (float)log(fmax(TT_TINY, m));
+                }
+            }
+        } else {
+            const double TT_C *M = T.P64 + (size_t)c * (N * N);
+            for (int j = 0; j < N; ++j) {
+                double m = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) m = fma(pc[i], M[i * N + j], m);
+                o[j] = (float)log(m);
+            }
+        }
+    }
+}
+
+// get_branch_mutation_matrix (treeanc.py:1085-1120) of the branch above
+// node c: out[p][i][j] = pc_i pp_j (P(t)_ij + 1e-24) / Z_p, i the child's
+// state, j the parent's; one thread per pattern.
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_branch_mutation_matrix(TreeDev T, int c, double *out) {
+    const int p = blockIdx.x * TT_BLOCK + threadIdx.x;
+    if (p >= T.P) return;
+    double pc[N], pp[N];
+    branch_profiles_any<N>(T, c, -1, p, pc, pp);
+    double *o = out + (size_t)p * (N * N);
+    bool site = false;
+    if constexpr (N <= 5) site = T.s_V64 != nullptr;
+    double z = 0.0;
+    if (site) {
+        if constexpr (N <= 5) {
+            double E[N * N];
+            site_P64<N>(T, c, p, E);
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    const double m = pc[i] * pp[j] * (E[i * N + j] + kSuperTiny);
+                    o[i * N + j] = m;
+                    z += m;
+                }
+        }
+    } else {
+        const double TT_C *E = T.P64 + (size_t)c * (N * N);
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const double m = pc[i] * pp[j] * (E[i * N + j] + kSuperTiny);
+                o[i * N + j] = m;
+                z += m;
+            }
+    }
+    const double inv = 1.0 / z;
+    for (int k = 0; k < N * N; ++k) o[k] *= inv;
+}
+
 }  // namespace
 
 #define TT_DISPATCH_ALL(n, CALL)                          \
@@ -721,8 +913,26 @@ hipError_t mutation_counts(const TreeDev &T, const double *t_eff, double *out, h
         case 3: hipLaunchKernelGGL(k_mutation_counts<3>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
         case 4: hipLaunchKernelGGL(k_mutation_counts<4>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
         case 5: hipLaunchKernelGGL(k_mutation_counts<5>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
+        case 20: hipLaunchKernelGGL(k_mutation_counts_wide<20>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
+        case 21: hipLaunchKernelGGL(k_mutation_counts_wide<21>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
+        case 22: hipLaunchKernelGGL(k_mutation_counts_wide<22>, dim3(T.n_nodes), dim3(kOptThreads), 0, s, T, t_eff, out); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t node_rows(const TreeDev &T, int kind, float *out, hipStream_t s) {
+    const size_t total = (size_t)T.n_nodes * T.P;
+    if (!total) return hipSuccess;
+    const unsigned blocks = (unsigned)((total + TT_BLOCK - 1) / TT_BLOCK);
+    TT_DISPATCH_ALL(T.n, hipLaunchKernelGGL(k_node_rows<N_>, dim3(blocks), dim3(TT_BLOCK), 0, s, T, kind, out));
+    return hipGetLastError();
+}
+
+hipError_t branch_mutation_matrix(const TreeDev &T, int c, double *out, hipStream_t s) {
+    const unsigned blocks = (unsigned)((T.P + TT_BLOCK - 1) / TT_BLOCK);
+    TT_DISPATCH_ALL(T.n, hipLaunchKernelGGL(k_branch_mutation_matrix<N_>, dim3(blocks), dim3(TT_BLOCK), 0, s, T, c,
+                                            out));
     return hipGetLastError();
 }
 

@@ -133,6 +133,13 @@ hipError_t root_site(const TreeDev &T, int S, hipStream_t s);
 hipError_t pre_site(const TreeDev &T, const LevelDev &L, hipStream_t s);
 hipError_t post_joint_site(const TreeDev &T, const LevelDev &L, int store_tip_cx, hipStream_t s);
 hipError_t root_joint_site(const TreeDev &T, int S, int store_tip_cx, hipStream_t s);
+// joint logL of the root states already in the root's state row (site-specific
+// model, after a joint pass): root log message at that state + node offsets
+hipError_t root_given_site(const TreeDev &T, int S, hipStream_t s);
+// debug rows of the last joint pass: rel [n_nodes][P][n] and own [n_nodes][P]
+// (k_joint_rows), shared model (tt_kernels.hip) or site-specific (tt_site.hip)
+hipError_t joint_rows(const TreeDev &T, double *rel, double *own, hipStream_t s);
+hipError_t joint_rows_site(const TreeDev &T, double *rel, double *own, hipStream_t s);
 hipError_t propagate_site(const double *in, int P, int n, const double *Q, int transpose, int ret_log, double *out,
                           hipStream_t s);
 hipError_t root_joint(const TreeDev &T, int store_tip_cx, hipStream_t s);
@@ -161,6 +168,11 @@ hipError_t branch_opt_marginal(const TreeDev &T, int gap, const double *eig, con
                                int32_t *status, hipStream_t s);
 // GTR inference counts per branch, out [n_nodes][n*n + n] (n_ij then T_i), n <= 5
 hipError_t mutation_counts(const TreeDev &T, const double *t_eff, double *out, hipStream_t s);
+// rows derived from the last marginal pass for export, [n_nodes][P][n] FP32:
+// kind 1 subtree (sum 1), 2 outgroup, 3 log message to the parent (tt_branch.hip)
+hipError_t node_rows(const TreeDev &T, int kind, float *out, hipStream_t s);
+// per-pattern (child, parent) state distribution of the branch above node c, [P][n][n] FP64
+hipError_t branch_mutation_matrix(const TreeDev &T, int c, double *out, hipStream_t s);
 // per-pattern statistics summed over branches (mode 0 marginal, 1 joint), n <= 5
 hipError_t site_counts(const TreeDev &T, const double *t_eff, int mode, const int16_t *code_state, double *n_out,
                        double *t_out, hipStream_t s);

@@ -586,9 +586,11 @@ __device__ __forceinline__ void tip_joint(const TreeDev &T, int cref, int cnode,
 
 // msg = product over the children of row v of their messages (FP64); a
 // polytomy's running product is rescaled by 2^-e (largest entry into
-// [0.5, 1)) after each child from the third on.
+// [0.5, 1)) after each child from the third on (ex_sum, when given, adds up
+// the exponents e).
 template <int N>
-__device__ __forceinline__ void joint_children(const TreeDev &T, int v, int p, JR<N> (&msg)[N], int store_tip_cx) {
+__device__ __forceinline__ void joint_children(const TreeDev &T, int v, int p, JR<N> (&msg)[N], int store_tip_cx,
+                                               int *ex_sum = nullptr) {
     using R = JR<N>;
 #pragma unroll
     for (int i = 0; i < N; ++i) msg[i] = R(1);
@@ -609,6 +611,7 @@ __device__ __forceinline__ void joint_children(const TreeDev &T, int v, int p, J
             for (int i = 1; i < N; ++i) mx = fmax(mx, msg[i]);
             int ex;
             (void)frexp(mx, &ex);
+            if (ex_sum) *ex_sum += ex;
 #pragma unroll
             for (int i = 0; i < N; ++i) msg[i] = ldexp(msg[i], -ex);
         }
@@ -710,7 +713,10 @@ __device__ __forceinline__ void joint_finish_d(const TreeDev &T, const PostDesc 
 }
 
 // Joint root state for pattern p (treeanc.py:1015-1037): the first argmax of
-// pi times the product of the children's messages.
+// pi times the product of the children's messages.  The root's row of Lx
+// (unused by the messages: the root sends none) receives that product
+// divided by its largest entry, exp(normalized_profile) of treeanc.py:1019,
+// from which a caller samples the root state.
 template <int N>
 __device__ __forceinline__ int root_joint_state(const TreeDev &T, int p, int store_tip_cx) {
     using R = JR<N>;
@@ -720,12 +726,17 @@ __device__ __forceinline__ int root_joint_state(const TreeDev &T, int p, int sto
     R mx = R(-1);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        const R x = msg[i] * (R)T.pi[i];
-        if (x > mx) {
-            mx = x;
+        msg[i] *= (R)T.pi[i];
+        if (msg[i] > mx) {
+            mx = msg[i];
             jm = i;
         }
     }
+    const R inv = R(1) / mx;
+    float out[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = (i == jm) ? 1.0f : (float)(msg[i] * inv);
+    store_row<N>(T.Lx + ((size_t)T.root_row * T.P + p) * N, out);
     T.states[(size_t)T.root * T.P + p] = (uint8_t)jm;
     return jm;
 }

@@ -351,6 +351,63 @@ __global__ __launch_bounds__(TT_BLOCK) void k_prob_t_profiles(const double *pp, 
     if (threadIdx.x == 0) block_out[(size_t)kq * nblk + blockIdx.x] = b;
 }
 
+// Rows of a joint pass for the debug attributes (treeanc.py:1066-1077 keeps
+// joint_Lx and joint_Cx with debug=True), recomputed from the stored
+// messages with the joint kernels' arithmetic, one thread per (node c,
+// pattern p):
+//   internal c (the root too): rel[c][p][j] = log of c's message (the root:
+//     of pi times the children's product) less its log maximum, and
+//     own[c][p] = that log maximum plus ln 2 per polytomy rescaling step, so
+//     that the absolute log message is rel plus the sum of own over the
+//     internal nodes of c's subtree;
+//   tip c: rel = its absolute log message max_i log max(TINY, P_ij) +
+//     log max(profile_i, TINY), own = 0, and its argmax row goes to Cx (the
+//     joint pass stores tips' rows only when it reconstructs tips).
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_joint_rows(TreeDev T, double *rel, double *own) {
+    using R = JR<N>;
+    const size_t gid = (size_t)blockIdx.x * TT_BLOCK + threadIdx.x;
+    if (gid >= (size_t)T.n_nodes * T.P) return;
+    const int c = (int)(gid / T.P), p = (int)(gid % T.P);
+    double *o = rel + gid * N;
+    const int ir = T.int_row[c];
+    if (ir < 0) {
+        const int code = T.tip_codes[(size_t)T.tip_row[c] * T.P + p];
+        for (int j = 0; j < N; ++j) o[j] = joint_tip_term<N>(T, T.loglut, c, code, j);
+        R msg[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) msg[i] = R(1);
+        tip_joint_code<N>(T, joint_lut<N>(T), code, c, p, msg, 1);
+        own[gid] = 0.0;
+        return;
+    }
+    R msg[N], x[N];
+    int ex = 0;
+    joint_children<N>(T, ir, p, msg, 0, &ex);
+    R mx = R(-1);
+    if (T.parent[c] < 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            x[i] = msg[i] * (R)T.pi[i];
+            mx = x[i] > mx ? x[i] : mx;
+        }
+    } else {
+        const auto PJ = joint_P<N>(T) + (size_t)c * (N * N);
+        for (int j = 0; j < N; ++j) {
+            R b = PJ[j] * msg[0];
+            for (int i = 1; i < N; ++i) {
+                const R val = PJ[i * N + j] * msg[i];
+                if (val > b) b = val;
+            }
+            x[j] = b;
+            mx = b > mx ? b : mx;
+        }
+    }
+    const double lm = log((double)mx);
+    for (int j = 0; j < N; ++j) o[j] = log((double)x[j]) - lm;
+    own[gid] = lm + ex * 0.6931471805599453094;
+}
+
 inline int n_ptiles(int P) { return (P + TT_WAVE - 1) / TT_WAVE; }
 inline unsigned grid_for_tasks(long tasks) { return (unsigned)((tasks + 3) / 4); }
 
@@ -436,6 +493,14 @@ hipError_t pre_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s) {
     const int npt = n_ptiles(T.P);
     TT_DISPATCH(T.n, hipLaunchKernelGGL(k_pre_marginal<N_>, dim3(grid_for_tasks((long)((L.n_nodes + 1) / 2) * npt)),
                                         dim3(TT_BLOCK), 0, s, T, L, npt));
+    return hipGetLastError();
+}
+
+hipError_t joint_rows(const TreeDev &T, double *rel, double *own, hipStream_t s) {
+    const size_t total = (size_t)T.n_nodes * T.P;
+    if (!total) return hipSuccess;
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_joint_rows<N_>, dim3((unsigned)((total + TT_BLOCK - 1) / TT_BLOCK)),
+                                        dim3(TT_BLOCK), 0, s, T, rel, own));
     return hipGetLastError();
 }
 

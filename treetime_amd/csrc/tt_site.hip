@@ -453,10 +453,94 @@ __global__ __launch_bounds__(TT_BLOCK) void k_root_joint_site(TreeDev T, int S, 
                 jm = i;
             }
         }
+        // the root's row of Lx64 (the root sends no message) keeps its
+        // log message, root.joint_Lx (treeanc.py:1017-1018) less the node
+        // offsets, for root-state sampling
+        const auto dst = T.Lx64 + ((size_t)T.root_row * T.P + p) * N;
+#pragma unroll
+        for (int i = 0; i < N; ++i) dst[i] = msg[i];
         double lp = mx;
         for (int k = 0; k < S; ++k) lp += T.part2[(size_t)k * T.P + p];
         T.jseqlh[p] = lp;
         T.states[(size_t)T.root * T.P + p] = (uint8_t)jm;
+        contrib = T.mult[p] * lp;
+    }
+    const double b = block_sum(contrib);
+    if (threadIdx.x == 0) T.bpart[blockIdx.x] = b;
+}
+
+// Debug rows of a joint pass under a site-specific model, as k_joint_rows
+// (tt_kernels.hip) for the shared model: internal c (the root too): rel =
+// c's log message less its maximum, own = that maximum (the root: of
+// log pi_p + the children's log messages); tip c: rel = its log message,
+// own = 0, its argmax row stored in Cx.
+// A wave handles 64 patterns of one node (the child walk reads the tree
+// structure as wave-uniform values).
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_joint_rows_site(TreeDev T, double *rel, double *own) {
+    const int ntiles = (T.P + TT_WAVE - 1) / TT_WAVE;
+    const long wave = ((long)blockIdx.x * TT_BLOCK + threadIdx.x) / TT_WAVE;
+    const int c = (int)(wave / ntiles), p = (int)(wave % ntiles) * TT_WAVE + (int)(threadIdx.x & (TT_WAVE - 1));
+    if (c >= T.n_nodes || p >= T.P) return;
+    const size_t gid = (size_t)c * T.P + p;
+    double *o = rel + gid * N;
+    const int ir = T.int_row[c];
+    double best[N];
+    int cx[N];
+    if (ir < 0) {
+        const int code = T.tip_codes[(size_t)T.tip_row[c] * T.P + p];
+        double E[N], P[N][N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) E[i] = fmax(TT_TINY, (double)T.lut[code * N + i]);
+        site_Pc<N>(T, c, p, P);
+        site_joint_max<N>(P, E, 0.0, best, cx);
+#pragma unroll
+        for (int j = 0; j < N; ++j) o[j] = best[j];
+        store_bytes<N>(T.Cx + ((size_t)c * T.P + p) * N, cx);
+        own[gid] = 0.0;
+        return;
+    }
+    double msg[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) msg[i] = 0.0;
+    site_joint_children<N>(T, T.child_ptr[ir], T.child_ptr[ir + 1], p, msg, 0);
+    double m = -DBL_MAX;
+    if (T.parent[c] < 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            best[i] = msg[i] + log(T.s_pi[(size_t)p * N + i]);
+            m = fmax(m, best[i]);
+        }
+    } else {
+        double P[N][N], E[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) m = fmax(m, msg[i]);
+#pragma unroll
+        for (int i = 0; i < N; ++i) E[i] = exp(msg[i] - m);
+        site_Pc<N>(T, c, p, P);
+        site_joint_max<N>(P, E, m, best, cx);
+        m = best[0];
+#pragma unroll
+        for (int j = 1; j < N; ++j) m = fmax(m, best[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) o[j] = best[j] - m;
+    own[gid] = m;
+}
+
+// Joint log-likelihood of given root states (written into the root's state
+// row by the caller), site-specific model: sequence_LH = root.joint_Lx at the
+// chosen state (treeanc.py:1036), the root's stored log message at that
+// state plus the node offsets of the last joint pass.
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_root_given_site(TreeDev T, int S) {
+    const int p = blockIdx.x * TT_BLOCK + threadIdx.x;
+    double contrib = 0.0;
+    if (p < T.P) {
+        const int s = T.states[(size_t)T.root * T.P + p];
+        double lp = T.Lx64[((size_t)T.root_row * T.P + p) * N + s];
+        for (int k = 0; k < S; ++k) lp += T.part2[(size_t)k * T.P + p];
+        T.jseqlh[p] = lp;
         contrib = T.mult[p] * lp;
     }
     const double b = block_sum(contrib);
@@ -541,6 +625,21 @@ hipError_t propagate_site(const double *in, int P, int n, const double *Q, int t
 hipError_t root_joint_site(const TreeDev &T, int S, int store_tip_cx, hipStream_t s) {
     TT_SITE_DISPATCH(T.n, hipLaunchKernelGGL(k_root_joint_site<N_>, dim3(root_blocks(T.P)), dim3(TT_BLOCK), 0, s, T,
                                              S, store_tip_cx));
+    return hipGetLastError();
+}
+
+hipError_t joint_rows_site(const TreeDev &T, double *rel, double *own, hipStream_t s) {
+    const size_t threads = (size_t)T.n_nodes * n_ptiles(T.P) * TT_WAVE;
+    if (!threads) return hipSuccess;
+    TT_SITE_DISPATCH(T.n, hipLaunchKernelGGL(k_joint_rows_site<N_>,
+                                             dim3((unsigned)((threads + TT_BLOCK - 1) / TT_BLOCK)), dim3(TT_BLOCK), 0,
+                                             s, T, rel, own));
+    return hipGetLastError();
+}
+
+hipError_t root_given_site(const TreeDev &T, int S, hipStream_t s) {
+    TT_SITE_DISPATCH(T.n, hipLaunchKernelGGL(k_root_given_site<N_>, dim3(root_blocks(T.P)), dim3(TT_BLOCK), 0, s, T,
+                                             S));
     return hipGetLastError();
 }
 

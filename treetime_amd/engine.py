@@ -187,9 +187,11 @@ class DeviceTree:
         _native.check(self.lib.tt_tree_info(self.handle, C.byref(a), C.byref(b), C.byref(c)))
         return {"levels_post": a.value, "levels_pre": b.value, "device_bytes": c.value}
 
-    def marginal(self, reconstruct_tips=False, stream=None, want_logL=True):
+    def marginal(self, reconstruct_tips=False, stream=None, want_logL=True, keep_normalisers=False):
         out = C.c_double(np.nan)
         flags = _native.TT_RECONSTRUCT_TIPS if reconstruct_tips else 0
+        if keep_normalisers:
+            flags |= _native.TT_KEEP_NORMALISERS
         _native.check(self.lib.tt_marginal(self.handle, flags, C.byref(out) if want_logL else None, stream))
         return out.value
 
@@ -199,6 +201,48 @@ class DeviceTree:
         _native.check(self.lib.tt_joint(self.handle, flags, C.byref(out) if want_logL else None, stream))
         self.joint_ready = True
         return out.value
+
+    def joint_root(self):
+        """normalized_profile of the root after a joint pass (treeanc.py:1019):
+        the root's joint log message less its per-pattern maximum, (P, n)."""
+        out = np.empty((self.P, self.n), dtype=np.float64)
+        _native.check(self.lib.tt_get_joint_root(self.handle, _native.ptr(out, C.c_double)))
+        return out
+
+    def joint_from_root(self, root_states, reconstruct_tips=False, stream=None, want_logL=True):
+        """Finish the last joint pass from given root states (backtrack and
+        sequence_LH at those states); returns sequence_joint_LH."""
+        rs = _native.as_c(root_states, np.uint8)
+        if rs.shape != (self.P,):
+            raise ValueError("root_states must hold one state per pattern")
+        out = C.c_double(np.nan)
+        flags = _native.TT_RECONSTRUCT_TIPS if reconstruct_tips else 0
+        _native.check(self.lib.tt_joint_from_root(self.handle, flags, _native.ptr(rs, C.c_uint8),
+                                                  C.byref(out) if want_logL else None, stream))
+        return out.value
+
+    def joint_debug(self):
+        """joint_Lx (n_nodes, P, n) FP64 and joint_Cx (n_nodes, P, n) uint8 of
+        the last joint pass (tt_get_joint_debug)."""
+        lx = np.empty((self.n_nodes, self.P, self.n), dtype=np.float64)
+        cx = np.empty((self.n_nodes, self.P, self.n), dtype=np.uint8)
+        _native.check(self.lib.tt_get_joint_debug(self.handle, _native.ptr(lx, C.c_double),
+                                                  _native.ptr(cx, C.c_uint8)))
+        return lx, cx
+
+    def subtree_prefactor(self):
+        """marginal_subtree_LH_prefactor (n_nodes, P) of the last marginal pass
+        run with keep_normalisers (tt_get_subtree_prefactor)."""
+        out = np.empty((self.n_nodes, self.P), dtype=np.float64)
+        _native.check(self.lib.tt_get_subtree_prefactor(self.handle, _native.ptr(out, C.c_double)))
+        return out
+
+    def branch_mutation_matrix(self, node):
+        """get_branch_mutation_matrix of the branch above ``node`` from the
+        last marginal pass: (P, n, n), [p, child state, parent state]."""
+        out = np.empty((self.P, self.n, self.n), dtype=np.float64)
+        _native.check(self.lib.tt_branch_mutation_matrix(self.handle, int(node), _native.ptr(out, C.c_double)))
+        return out
 
     def loglik(self, stream=None, want_logL=True):
         out = C.c_double(np.nan)

@@ -15,8 +15,11 @@ follow treetime/treeanc.py (TreeTime 0.12.1):
 * ``infer_gtr`` (1500-1632) with the counts from the device; marginal
   branch lengths ``optimize_tree_marginal`` (1297-1358).
 
-Fitch parsimony and site-specific models are outside this engine
-(``NotImplementedError``).
+* ``marginal_branch_profile`` / ``get_branch_mutation_matrix``
+  (1085-1146) from device rows; ``debug=True`` keeps the reference's debug
+  attributes (803-810, 1066-1077).
+
+Fitch parsimony is outside this engine (``NotImplementedError``).
 """
 import sys
 import time
@@ -366,14 +369,17 @@ class TreeAnc:
         else:
             raise ValueError("sample_from_profile must be True, False or 'root'")
         dev = self._sync_model(self._device_tree())
-        total = dev.marginal(reconstruct_tips=reconstruct_tip_states)
+        debug = debug and not dev.site  # site-specific passes keep no node normalisers
+        total = dev.marginal(reconstruct_tips=reconstruct_tip_states, keep_normalisers=debug)
         prof = dev.profiles()
         sub = dev.profiles(which=1)
+        outg = dev.profiles(which=2)
         tree = self.tree
         tree.sequence_LH = dev.sequence_lh()
         tree.total_sequence_LH = total
         tree.sequence_marginal_LH = total
         for i, node in enumerate(self._nodes):
+            node.marginal_outgroup_LH = outg[i].astype(float)  # treeanc.py:819, 895-899
             if node.is_terminal():
                 if node.name in self.data.compressed_alignment:
                     node.marginal_subtree_LH = seq2prof(self.data.compressed_alignment[node.name],
@@ -385,19 +391,49 @@ class TreeAnc:
             else:
                 node.marginal_subtree_LH = sub[i].astype(float)
                 node.marginal_profile = prof[i].astype(float)
+        if debug:  # treeanc.py:803-810 deletes these unless debug
+            logm = dev.profiles(which=3)
+            pref = dev.subtree_prefactor()
+            for i, node in enumerate(self._nodes):
+                node.marginal_subtree_LH_prefactor = pref[i]
+                if node.up is not None:
+                    node.marginal_log_Lx = logm[i].astype(float)
         n_diff = self._assign_sequences(dev, reconstruct_tip_states, sample_root, sample_other, prof)
         self.reconstructed_tip_sequences = reconstruct_tip_states
         self.sequence_reconstruction = "marginal"
         return n_diff
 
     def _ml_anc_joint(self, sample_from_profile=False, reconstruct_tip_states=False, debug=False, **kwargs):
-        """treeanc.py:934-1080."""
-        if sample_from_profile:
-            raise NotImplementedError("sampling from profiles is not part of the GPU pruning engine")
+        """treeanc.py:934-1080.  With sample_from_profile ('root' or True) the
+        root state is drawn from the root's joint profile with this TreeAnc's
+        generator (one rng.random(P), treeanc.py:1022-1033) and every other
+        node follows the joint argmax given it."""
+        if sample_from_profile == "root":
+            sample_root = True
+        elif isinstance(sample_from_profile, bool):
+            sample_root = sample_from_profile
+        else:
+            raise ValueError("sample_from_profile must be True, False or 'root'")
         dev = self._sync_model(self._device_tree())
         total = dev.joint(reconstruct_tips=reconstruct_tip_states)
+        if sample_root:
+            # prof2seq(np.exp(normalized_profile), sample_from_prof=True): rows
+            # normalised to sum 1, cumulative sums against rng.random(P)
+            prof = np.exp(dev.joint_root())
+            prof = np.einsum("ai,a->ai", prof, 1.0 / prof.sum(axis=1))
+            cum = prof.cumsum(axis=1).T
+            idx = np.argmax(cum >= self.rng.random(size=cum.shape[1]), axis=0)
+            total = dev.joint_from_root(idx.astype(np.uint8), reconstruct_tips=reconstruct_tip_states)
         self.tree.sequence_LH = dev.sequence_lh()
         self.tree.sequence_joint_LH = total
+        if debug:  # treeanc.py:1066-1077 deletes these unless debug
+            lx, cx = dev.joint_debug()
+            states = dev.states()
+            for i, node in enumerate(self._nodes):
+                node.joint_Lx = lx[i]
+                node.joint_Cx = None if node.up is None else cx[i].astype(np.uint16)
+                if node.up is None or not node.is_terminal() or reconstruct_tip_states:
+                    node.seq_idx = states[i].astype(int)
         n_diff = self._assign_sequences(dev, reconstruct_tip_states)
         self.reconstructed_tip_sequences = reconstruct_tip_states
         self.sequence_reconstruction = "joint"
@@ -612,13 +648,14 @@ class TreeAnc:
         """treeanc.py:1297-1358: iterate (all branch optima from the current
         marginal profiles, damped update, marginal reconstruction) until the
         likelihood gain drops below LHtol."""
-        if infer_gtr:
-            raise NotImplementedError("GTR inference is not part of the GPU pruning engine")
         self.infer_ancestral_sequences(marginal=True, **kwargs)
         oldLH = self.sequence_LH()
         self.logger("TreeAnc.optimize_tree_marginal: initial, LH=%1.2f, total branch_length %1.4f"
                     % (oldLH, self.tree.total_branch_length()), 2)
         for i in range(max_iter):
+            if infer_gtr:  # treeanc.py:1308-1310
+                self.infer_gtr(site_specific=site_specific_gtr, marginal=True, normalized_rate=True, pc=pc)
+                self.infer_ancestral_sequences(marginal=True, **kwargs)
             old_bl = self.tree.total_branch_length()
             tol = 1e-8 + 0.01 ** (i + 1)
             res, pair = self._marginal_optima(tol)
@@ -652,6 +689,28 @@ class TreeAnc:
                 self.logger("TreeAnc.optimize_tree_marginal: deltaLH=%f, stopping iteration." % deltaLH, 1)
                 break
         return ttconf.SUCCESS
+
+    def marginal_branch_profile(self, node):
+        """treeanc.py:1122-1146: (marginal_outgroup_LH, marginal_subtree_LH)
+        of the branch above ``node``."""
+        if node.up is None:
+            raise Exception("Branch profiles can't be calculated for the root!")
+        if not hasattr(node, "marginal_outgroup_LH"):
+            raise Exception("marginal ancestral inference needs to be performed first!")
+        return node.marginal_outgroup_LH, node.marginal_subtree_LH
+
+    def get_branch_mutation_matrix(self, node, full_sequence=False):
+        """treeanc.py:1085-1120: per pattern, the joint distribution of the
+        states at both ends of the branch above ``node`` (child state i,
+        parent state j), computed on the device from the last marginal pass."""
+        self.marginal_branch_profile(node)
+        if self._device is None or self.sequence_reconstruction != "marginal":
+            raise Exception("marginal ancestral inference needs to be performed first!")
+        idx = next(i for i, x in enumerate(self._nodes) if x is node)
+        m = self._device.branch_mutation_matrix(idx)
+        if full_sequence:
+            return m[self.data.full_to_compressed_sequence_map]
+        return m
 
     def branch_profile_lh(self, grid):
         """prob_t_profiles(return_log=True) of every branch on the lengths
