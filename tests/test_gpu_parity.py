@@ -512,3 +512,31 @@ def test_joint_logL_is_score_of_decoded_states(gpu_engine):
     assert err <= 1e-9
     assert abs(total - float(np.sum(prep["multiplicity"] * ref))) <= 1e-9 * abs(total)
     dt.close()
+
+
+ONE_LAUNCH_CASES = [c for c in CASES if c[3] in (4, 5)]
+
+
+@pytest.mark.parametrize("case", ONE_LAUNCH_CASES, ids=[c[0] for c in ONE_LAUNCH_CASES])
+def test_one_launch_pass_equals_level_kernels(gpu_engine, case, monkeypatch):
+    """The one-launch pass with LDS-resident rows (TT_TILE=1) gives the level
+    kernels' profiles, subtree rows and states bit for bit, and their logL
+    to rounding, with and without reconstructed tips and for the
+    log-likelihood pass."""
+    say(f"[one-launch] case={case[0]}")
+    job, prep, alphabet = make_case(*case)
+    monkeypatch.setenv("TT_TILE", "0")
+    lv = device_tree(gpu_engine, prep)
+    monkeypatch.setenv("TT_TILE", "1")
+    one = device_tree(gpu_engine, prep)
+    for tips in (False, True):
+        a, b = lv.marginal(reconstruct_tips=tips), one.marginal(reconstruct_tips=tips)
+        assert abs(a - b) <= 1e-12 * abs(a), (tips, a, b)
+        assert site_err(one.sequence_lh(), lv.sequence_lh()) <= 1e-12
+        assert np.array_equal(one.profiles(), lv.profiles())
+        assert np.array_equal(one.profiles(1), lv.profiles(1))
+        assert np.array_equal(one.states(), lv.states())
+    a, b = lv.loglik(), one.loglik()
+    assert abs(a - b) <= 1e-12 * abs(a)
+    lv.close()
+    one.close()

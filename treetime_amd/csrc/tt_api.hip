@@ -167,6 +167,11 @@ struct tt_tree {
     std::vector<std::pair<void *, size_t>> allocs;
     int last_pass = 0;  // 1 marginal, 2 joint, 3 loglik
     hipStream_t last_stream = nullptr;  // stream of the last pass (result reads wait for it)
+    // one-launch marginal / loglik pass with LDS-resident rows (tt_tile.hip):
+    // patterns per workgroup, 0 when the level kernels run instead
+    int tile_tp = 0;
+    double *d_tile_partial = nullptr;
+    unsigned *d_tile_counter = nullptr;
     bool have_states = false;  // d.states holds a marginal or joint reconstruction
     bool tips_dirty = false;   // tip rows of d.states hold reconstructed states (else 255)
     bool profiles_fresh = false;  // d.prof and d.sub belong to the current lengths (last marginal pass)
@@ -467,7 +472,12 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
     const bool tips = (flags & TT_RECONSTRUCT_TIPS) != 0;
     const bool fill = (flags & kFillTips) != 0;
     const int nb = ttk::root_blocks(T->P);
-    if (mode == 1 || mode == 3) {
+    if ((mode == 1 || mode == 3) && T->tile_tp && !T->site) {
+        if (mode == 1 && fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
+        TT_HIP(ttk::tile_marginal(D, T->d_post_desc, T->d_post_off, (int)T->post_lv.size(), T->d_pre_desc,
+                                  T->d_pre_off, T->d_pre_nint, (int)T->pre_lv.size(), T->tile_tp,
+                                  T->d_tile_partial, T->d_tile_counter, tips ? 1 : 0, mode == 1 ? 1 : 0, s));
+    } else if (mode == 1 || mode == 3) {
         size_t ri = 0;
         for (size_t h = 0; h < T->post_lv.size();) {
             if (!T->site && ri < T->post_runs.size() && T->post_runs[ri].l0 == (int)h) {
@@ -791,6 +801,39 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
 
 extern "C" {
 
+// Choose the one-launch LDS-resident pass for a tree (nucleotide-sized
+// alphabets, shared model): the largest power-of-two pattern tile whose rows
+// fit a workgroup's LDS, no larger than needed for ~2 workgroups per CU, and
+// only while the tree needs at most kTileMaxBlocks workgroups (beyond that
+// the level kernels stream the tree at full bandwidth).  TT_TILE=0 turns the
+// path off, TT_TILE=1 uses it whenever the rows fit.
+int32_t setup_tile(tt_tree *T) {
+    T->tile_tp = 0;
+    const char *e = getenv("TT_TILE");
+    if (!(e && e[0] == '1')) return TT_OK;  // opt-in while the per-round loads are exposed
+    if (T->n < 2 || T->n > 5 || T->n_int < 1) return TT_OK;
+    int dev = 0, lds_max = 0, n_cu = 0;
+    TT_HIP(hipGetDevice(&dev));
+    TT_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    TT_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    // dynamic LDS up to the device's per-workgroup limit needs no opt-in here
+    const size_t budget = (size_t)std::max(0, lds_max - 2048);  // the kernel's static LDS
+    int tp_fit = 0;
+    for (int tp = 1; tp <= 64; tp *= 2)
+        if (ttk::tile_lds_bytes(T->d, tp) <= budget) tp_fit = tp;
+    if (!tp_fit) return TT_OK;
+    int tp = 1;
+    while (tp * 2 <= tp_fit && (long)T->P / (tp * 2) >= 2L * n_cu) tp *= 2;
+    const int blocks = (T->P + tp - 1) / tp;
+    constexpr int kTileMaxBlocks = 2048;
+    if (blocks > kTileMaxBlocks && !(e && e[0] == '1')) return TT_OK;
+    int32_t rc;
+    if ((rc = T->alloc((void **)&T->d_tile_partial, (size_t)blocks * sizeof(double)))) return rc;
+    if ((rc = T->alloc((void **)&T->d_tile_counter, sizeof(unsigned)))) return rc;
+    T->tile_tp = tp;
+    return TT_OK;
+}
+
 int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     if (!eng || !D || !out) return fail(TT_EINVAL, "NULL argument");
     if (D->n_nodes < 2) return fail(TT_EINVAL, "n_nodes=%d: need at least 2 nodes", D->n_nodes);
@@ -945,8 +988,13 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
             return rc;
         }
     }
+    if ((rc = setup_tile(T))) {
+        delete T;
+        return rc;
+    }
     he = hipMemset(d.states, 255, (size_t)nn * P);
     if (he == hipSuccess) he = hipMemset(d.prof, 0, (size_t)nn * P * n * sizeof(float));
+    if (he == hipSuccess && T->d_tile_counter) he = hipMemset(T->d_tile_counter, 0, sizeof(unsigned));
     // the passes run on a non-blocking stream that does not order after the
     // null-stream memsets
     if (he == hipSuccess) he = hipDeviceSynchronize();

@@ -156,6 +156,11 @@ hipError_t prob_t_profiles(const double *pp, const double *pc, const double *mul
                            int K, int gap, double *block_out, int nblk, hipStream_t s);
 int root_blocks(int P);
 bool mfma_enabled();
+// one-launch marginal / loglik pass with LDS-resident rows (tt_tile.hip)
+size_t tile_lds_bytes(const TreeDev &T, int tp);
+hipError_t tile_marginal(const TreeDev &T, const int32_t *post_desc, const int32_t *post_off, int H,
+                         const int32_t *pre_desc, const int32_t *pre_off, const int32_t *pre_nint, int D, int tp,
+                         double *partial, unsigned *counter, int tips, int pre, hipStream_t s);
 // sets tt_last_error()'s text on the calling thread and returns code (tt_api.hip)
 int32_t set_error(int32_t code, const char *msg);
 // joint branch-length optimisation (tt_branch.hip)

@@ -139,14 +139,20 @@ __device__ __forceinline__ int argmax_first(const float (&x)[N]) {
 
 // Store row v normalised by its entry jm (value xm), which is stored as
 // exactly 1.
+// lds_out (nullable): the same row also goes to this LDS slot.
 template <int N>
-__device__ __forceinline__ void store_post(const TreeDev &T, int v, int p, const float (&x)[N], int jm, float xm) {
+__device__ __forceinline__ void store_post(const TreeDev &T, int v, int p, const float (&x)[N], int jm, float xm,
+                                           float *lds_out = nullptr) {
     // correctly rounded quotient: these entries feed the parents' messages,
     // and a biased approximate reciprocal would bias variable patterns' logL
     float out[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : x[j] / xm;
     store_row<N>(T.sub + ((size_t)v * T.P + p) * N, out);
+    if (lds_out) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) lds_out[j] = out[j];
+    }
 }
 
 // Subtree row of child edge e: stored row of an internal child or the
@@ -164,17 +170,22 @@ __device__ __forceinline__ void child_row(const TreeDev &T, int cref, int p, LP 
 
 // Postorder step for internal row v with any number of children
 // (treeanc.py:857-878), in log space per state as the reference does.
-template <int N, class LP>
-__device__ __forceinline__ void post_node(const TreeDev &T, int v, int p, LP lut, LogAcc &acc) {
+// post_node with the child rows read through rowf(cref, r) (tips and
+// internal children), uni() applied to the tree-structure values (wave
+// uniform in the level kernels, per lane elsewhere) and the stored row also
+// copied to lds_out when given.
+template <int N, class LP, class RowF, class Uni>
+__device__ __forceinline__ void post_node_f(const TreeDev &T, int v, int p, LP lut, LogAcc &acc, RowF rowf, Uni uni,
+                                            float *lds_out = nullptr) {
     float L[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) L[j] = 0.0f;
-    const int e0 = wave_uniform(T.child_ptr[v]), e1 = wave_uniform(T.child_ptr[v + 1]);
+    const int e0 = uni(T.child_ptr[v]), e1 = uni(T.child_ptr[v + 1]);
     for (int e = e0; e < e1; ++e) {
-        const int cref = wave_uniform(T.child_ref[e]);
-        const int cnode = wave_uniform(T.child_node[e]);
+        const int cref = uni(T.child_ref[e]);
+        const int cnode = uni(T.child_node[e]);
         float r[N], d[N];
-        child_row<N>(T, cref, p, lut, r);
+        rowf(cref, r);
         deficit<N>(r, T.D32 + (size_t)cnode * (N * N), d);
 #pragma unroll
         for (int j = 0; j < N; ++j) L[j] += log_msg(r[j], d[j]);
@@ -194,10 +205,10 @@ __device__ __forceinline__ void post_node(const TreeDev &T, int v, int p, LP lut
     // normaliser at the selected state, children revisited (cache hits)
     double lnode = 0.0;
     for (int e = e0; e < e1; ++e) {
-        const int cref = wave_uniform(T.child_ref[e]);
-        const int cnode = wave_uniform(T.child_node[e]);
+        const int cref = uni(T.child_ref[e]);
+        const int cnode = uni(T.child_node[e]);
         float r[N], d[N];
-        child_row<N>(T, cref, p, lut, r);
+        rowf(cref, r);
         deficit<N>(r, T.D32 + (size_t)cnode * (N * N), d);
         float rj = r[0], dj = d[0];
 #pragma unroll
@@ -211,6 +222,16 @@ __device__ __forceinline__ void post_node(const TreeDev &T, int v, int p, LP lut
     }
     if (T.offs) T.offs[(size_t)v * T.P + p] = lnode;
     store_row<N>(T.sub + ((size_t)v * T.P + p) * N, out);
+    if (lds_out) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) lds_out[j] = out[j];
+    }
+}
+
+template <int N, class LP>
+__device__ __forceinline__ void post_node(const TreeDev &T, int v, int p, LP lut, LogAcc &acc) {
+    post_node_f<N>(T, v, p, lut, acc, [&](int cref, float (&r)[N]) { child_row<N>(T, cref, p, lut, r); },
+                   [](int e) { return wave_uniform(e); });
 }
 
 // Binary fast path, split in a fetch phase (global loads only) and a finish
@@ -242,7 +263,8 @@ __device__ __forceinline__ void bin_fetch(const TreeDev &T, int e0, int p, BinRa
 
 template <int N, class LP>
 __device__ __forceinline__ void bin_finish_refs(const TreeDev &T, int v, int cref0, int cref1, int cnode0, int cnode1,
-                                                int p, const BinRaw<N> &R, LP lut, LogAcc &acc) {
+                                                int p, const BinRaw<N> &R, LP lut, LogAcc &acc,
+                                                float *lds_out = nullptr) {
     float r[2][N], d[2][N], x[N];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
@@ -275,7 +297,7 @@ __device__ __forceinline__ void bin_finish_refs(const TreeDev &T, int v, int cre
     const double f = msg64(r0, d0) * msg64(r1, d1);
     acc.mul(f);
     if (T.offs) T.offs[(size_t)v * T.P + p] = log(f);  // update mode: keep the node's own normaliser
-    store_post<N>(T, v, p, x, jm, xm);
+    store_post<N>(T, v, p, x, jm, xm, lds_out);
 }
 
 template <int N, class LP>
@@ -432,7 +454,8 @@ __device__ __forceinline__ void pre_fetch(const TreeDev &T, int c, int p, PreRaw
 }
 
 template <int N, class LP>
-__device__ __forceinline__ void pre_finish_ir(const TreeDev &T, int c, int ir, int p, const PreRaw<N> &R, LP lut) {
+__device__ __forceinline__ void pre_finish_ir(const TreeDev &T, int c, int ir, int p, const PreRaw<N> &R, LP lut,
+                                              float *lds_out = nullptr) {
     const auto M = T.P32 + (size_t)c * (N * N);
     float sub[N], m[N], q[N], msg[N];
     if (ir >= 0) {
@@ -467,6 +490,10 @@ __device__ __forceinline__ void pre_finish_ir(const TreeDev &T, int c, int ir, i
     const int jm = argmax_first<N>(msg);
     store_row<N>(T.prof + ((size_t)c * T.P + p) * N, out);
     T.states[(size_t)c * T.P + p] = (uint8_t)jm;
+    if (lds_out) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) lds_out[i] = out[i];
+    }
 }
 
 template <int N, class LP>
