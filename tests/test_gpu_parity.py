@@ -538,5 +538,19 @@ def test_one_launch_pass_equals_level_kernels(gpu_engine, case, monkeypatch):
         assert np.array_equal(one.states(), lv.states())
     a, b = lv.loglik(), one.loglik()
     assert abs(a - b) <= 1e-12 * abs(a)
+    # new branch lengths (set, and changed on the root path) reach the one-launch pass
+    nodes = lv.nonroot_nodes()[:3]
+    for dt in (lv, one):
+        dt.set_branch_lengths(nodes, dt.t_cur[nodes] * 1.7)
+    a, b = lv.marginal(), one.marginal()
+    assert abs(a - b) <= 1e-12 * abs(a)
+    assert np.array_equal(one.profiles(), lv.profiles())
+    node = int(lv.nonroot_nodes()[-1])
+    ua, ub = lv.update_branch(node, 0.031), one.update_branch(node, 0.031)
+    assert abs(ua - ub) <= 1e-12 * abs(ua)
+    a, b = lv.marginal(), one.marginal()
+    assert abs(a - b) <= 1e-12 * abs(a) and abs(a - ua) <= 1e-11 * abs(a)
+    assert np.array_equal(one.profiles(), lv.profiles())
+    assert np.array_equal(one.states(), lv.states())
     lv.close()
     one.close()

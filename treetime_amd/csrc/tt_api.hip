@@ -170,8 +170,11 @@ struct tt_tree {
     // one-launch marginal / loglik pass with LDS-resident rows (tt_tile.hip):
     // patterns per workgroup, 0 when the level kernels run instead
     int tile_tp = 0;
+    int tile_Rp = 0, tile_Ri = 0, tile_Ra = 0;  // rounds: postorder, preorder internal nodes, preorder all
+    int32_t *d_tile_rounds = nullptr, *d_tile_post_rec = nullptr, *d_tile_pre_rec = nullptr;
     double *d_tile_partial = nullptr;
     unsigned *d_tile_counter = nullptr;
+    bool tile_stale = true;  // the records do not hold the current P(t) yet
     bool have_states = false;  // d.states holds a marginal or joint reconstruction
     bool tips_dirty = false;   // tip rows of d.states hold reconstructed states (else 255)
     bool profiles_fresh = false;  // d.prof and d.sub belong to the current lengths (last marginal pass)
@@ -474,9 +477,9 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
     const int nb = ttk::root_blocks(T->P);
     if ((mode == 1 || mode == 3) && T->tile_tp && !T->site) {
         if (mode == 1 && fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
-        TT_HIP(ttk::tile_marginal(D, T->d_post_desc, T->d_post_off, (int)T->post_lv.size(), T->d_pre_desc,
-                                  T->d_pre_off, T->d_pre_nint, (int)T->pre_lv.size(), T->tile_tp,
-                                  T->d_tile_partial, T->d_tile_counter, tips ? 1 : 0, mode == 1 ? 1 : 0, s));
+        TT_HIP(ttk::tile_marginal(D, T->d_tile_post_rec, T->d_tile_pre_rec, T->d_tile_rounds, T->tile_Rp,
+                                  T->tile_Ri, T->tile_Ra, T->tile_tp, T->d_tile_partial, T->d_tile_counter,
+                                  tips ? 1 : 0, mode == 1 ? 1 : 0, s));
     } else if (mode == 1 || mode == 3) {
         size_t ri = 0;
         for (size_t h = 0; h < T->post_lv.size();) {
@@ -567,6 +570,11 @@ int32_t run_pass(tt_tree *T, int mode, uint32_t flags, double *logL, void *strea
     const bool fill = !tips && T->tips_dirty && (mode == 1 || mode == 2);
     flags = (flags & TT_RECONSTRUCT_TIPS) | (fill ? kFillTips : 0u);
     const int key = mode * 8 + (tips ? 1 : 0) + (fill ? 2 : 0);
+    if (T->tile_tp && !T->site && (mode == 1 || mode == 3) && T->tile_stale) {
+        TT_HIP(ttk::tile_pack(T->d, T->d_post_desc, (int)T->post_nodes.size(), T->d_pre_desc,
+                              (int)T->pre_nodes.size(), T->d_tile_post_rec, T->d_tile_pre_rec, s));
+        T->tile_stale = false;
+    }
     if (g_use_graphs()) {
         auto it = T->graphs.find(key);
         if (it == T->graphs.end()) {
@@ -622,6 +630,7 @@ int32_t enable_update_mode(tt_tree *T) {
 int32_t refresh_expqt(tt_tree *T, const int32_t *d_branches, const double *d_t, int k, hipStream_t s) {
     TT_HIP(ttk::expqt(T->n, d_t, d_branches, k, T->d_eig, T->d_v, T->d_vinv, T->mu, (double *)T->d.P64,
                       (double *)T->d.Pj64, (double *)T->d.logP64, (float *)T->d.Pj32, (float *)T->d.P32, (float *)T->d.D32, s));
+    T->tile_stale = true;
     return TT_OK;
 }
 
@@ -801,16 +810,33 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
 
 extern "C" {
 
+// Rounds of the one-launch pass: each level's entries in chunks of at most
+// cap, as [first, end) pairs.
+void tile_rounds(const tt_tree *T, int cap, std::vector<int32_t> &out, int &Rp, int &Ri, int &Ra) {
+    out.clear();
+    auto add = [&](int first, int end) {
+        for (int b = first; b < end; b += cap) {
+            out.push_back(b);
+            out.push_back(std::min(end, b + cap));
+        }
+    };
+    for (const PostLv &L : T->post_lv) add(L.off, L.off + L.cnt);
+    Rp = (int)out.size() / 2;
+    for (const PreLv &L : T->pre_lv) add(L.off, L.off + L.n_int);
+    Ri = (int)out.size() / 2 - Rp;
+    for (const PreLv &L : T->pre_lv) add(L.off, L.off + L.n_all);
+    Ra = (int)out.size() / 2 - Rp - Ri;
+}
+
 // Choose the one-launch LDS-resident pass for a tree (nucleotide-sized
-// alphabets, shared model): the largest power-of-two pattern tile whose rows
-// fit a workgroup's LDS, no larger than needed for ~2 workgroups per CU, and
-// only while the tree needs at most kTileMaxBlocks workgroups (beyond that
-// the level kernels stream the tree at full bandwidth).  TT_TILE=0 turns the
-// path off, TT_TILE=1 uses it whenever the rows fit.
+// alphabets, shared model) and its power-of-two pattern tile: rows of the
+// tile in a workgroup's LDS, and every workgroup resident at once (beyond
+// that the level kernels stream the tree at full bandwidth).
 int32_t setup_tile(tt_tree *T) {
     T->tile_tp = 0;
     const char *e = getenv("TT_TILE");
-    if (!(e && e[0] == '1')) return TT_OK;  // opt-in while the per-round loads are exposed
+    if (e && e[0] == '0') return TT_OK;
+    const bool forced = e && e[0] == '1';
     if (T->n < 2 || T->n > 5 || T->n_int < 1) return TT_OK;
     int dev = 0, lds_max = 0, n_cu = 0;
     TT_HIP(hipGetDevice(&dev));
@@ -818,19 +844,47 @@ int32_t setup_tile(tt_tree *T) {
     TT_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     // dynamic LDS up to the device's per-workgroup limit needs no opt-in here
     const size_t budget = (size_t)std::max(0, lds_max - 2048);  // the kernel's static LDS
-    int tp_fit = 0;
-    for (int tp = 1; tp <= 64; tp *= 2)
-        if (ttk::tile_lds_bytes(T->d, tp) <= budget) tp_fit = tp;
-    if (!tp_fit) return TT_OK;
-    int tp = 1;
-    while (tp * 2 <= tp_fit && (long)T->P / (tp * 2) >= 2L * n_cu) tp *= 2;
+    // the smallest tile whose workgroups are all resident at once: a round
+    // costs about one wave's arithmetic for one (node, pattern) step, the
+    // level kernels about a launch per level, so one wave of workgroups wins
+    // and a second one (the chain of rounds run twice) no longer does.
+    // TT_TILE=1 also takes two waves of workgroups; TT_TILE=0 turns the path off.
+    const int kTileMaxWaves = forced ? 2 : 1;
+    std::vector<int32_t> rounds;
+    int Rp = 0, Ri = 0, Ra = 0, tp = 0;
+    long best_waves = kTileMaxWaves + 1;
+    for (int c = 1; c <= 64; c *= 2) {
+        tile_rounds(T, ttk::tile_threads() / c, rounds, Rp, Ri, Ra);
+        const size_t lds = ttk::tile_lds_bytes(T->d, c, (int)rounds.size());
+        if (lds > budget) break;
+        const int bpc = ttk::tile_blocks_per_cu(T->n, lds);
+        if (bpc < 1) break;
+        const long blocks = (T->P + c - 1) / c;
+        const long waves = (blocks + (long)bpc * n_cu - 1) / ((long)bpc * n_cu);
+        if (waves < best_waves) {
+            best_waves = waves;
+            tp = c;
+        }
+        if (waves == 1) break;
+    }
+    if (!tp) return TT_OK;
     const int blocks = (T->P + tp - 1) / tp;
-    constexpr int kTileMaxBlocks = 2048;
-    if (blocks > kTileMaxBlocks && !(e && e[0] == '1')) return TT_OK;
+    tile_rounds(T, ttk::tile_threads() / tp, rounds, Rp, Ri, Ra);
     int32_t rc;
+    if ((rc = T->upload(&T->d_tile_rounds, rounds.data(), rounds.size()))) return rc;
+    if ((rc = T->alloc((void **)&T->d_tile_post_rec,
+                       (size_t)T->post_nodes.size() * ttk::tile_post_rec_words(T->n) * sizeof(int32_t))))
+        return rc;
+    if ((rc = T->alloc((void **)&T->d_tile_pre_rec,
+                       (size_t)T->pre_nodes.size() * ttk::tile_pre_rec_words(T->n) * sizeof(int32_t))))
+        return rc;
     if ((rc = T->alloc((void **)&T->d_tile_partial, (size_t)blocks * sizeof(double)))) return rc;
     if ((rc = T->alloc((void **)&T->d_tile_counter, sizeof(unsigned)))) return rc;
+    T->tile_Rp = Rp;
+    T->tile_Ri = Ri;
+    T->tile_Ra = Ra;
     T->tile_tp = tp;
+    T->tile_stale = true;
     return TT_OK;
 }
 
@@ -1224,6 +1278,7 @@ int32_t tt_update_branch(tt_tree *T, int32_t node, double t, double *logL, void 
                       (float *)T->d.D32, s));
     TT_HIP(ttk::path_update(T->d, node, s));
     TT_HIP(ttk::final_sum(T->d, ttk::root_blocks(T->P), s));
+    T->tile_stale = true;  // P(t) of the branch changed
     T->last_pass = 3;
     T->last_stream = s;
     T->profiles_fresh = false;

@@ -157,10 +157,16 @@ hipError_t prob_t_profiles(const double *pp, const double *pc, const double *mul
 int root_blocks(int P);
 bool mfma_enabled();
 // one-launch marginal / loglik pass with LDS-resident rows (tt_tile.hip)
-size_t tile_lds_bytes(const TreeDev &T, int tp);
-hipError_t tile_marginal(const TreeDev &T, const int32_t *post_desc, const int32_t *post_off, int H,
-                         const int32_t *pre_desc, const int32_t *pre_off, const int32_t *pre_nint, int D, int tp,
-                         double *partial, unsigned *counter, int tips, int pre, hipStream_t s);
+int tile_threads();
+int tile_post_rec_words(int n);
+int tile_pre_rec_words(int n);
+size_t tile_lds_bytes(const TreeDev &T, int tp, int n_rounds);
+int tile_blocks_per_cu(int n, size_t lds);
+hipError_t tile_pack(const TreeDev &T, const int32_t *post_desc, int n_post, const int32_t *pre_desc, int n_pre,
+                     int32_t *post_rec, int32_t *pre_rec, hipStream_t s);
+hipError_t tile_marginal(const TreeDev &T, const int32_t *post_rec, const int32_t *pre_rec, const int32_t *rounds,
+                         int Rp, int Ri, int Ra, int tp, double *partial, unsigned *counter, int tips, int pre,
+                         hipStream_t s);
 // sets tt_last_error()'s text on the calling thread and returns code (tt_api.hip)
 int32_t set_error(int32_t code, const char *msg);
 // joint branch-length optimisation (tt_branch.hip)

@@ -261,15 +261,15 @@ __device__ __forceinline__ void bin_fetch(const TreeDev &T, int e0, int p, BinRa
     bin_fetch_refs<N>(T, wave_uniform(T.child_ref[e0]), wave_uniform(T.child_ref[e0 + 1]), p, R);
 }
 
-template <int N, class LP>
-__device__ __forceinline__ void bin_finish_refs(const TreeDev &T, int v, int cref0, int cref1, int cnode0, int cnode1,
-                                                int p, const BinRaw<N> &R, LP lut, LogAcc &acc,
-                                                float *lds_out = nullptr) {
+// Binary postorder step with the children's P - I given as D0 and D1
+// (matrices in memory or in registers).
+template <int N, class LP, class MP0, class MP1>
+__device__ __forceinline__ void bin_finish_m(const TreeDev &T, int v, int cref0, int cref1, MP0 D0, MP1 D1, int p,
+                                             const BinRaw<N> &R, LP lut, LogAcc &acc, float *lds_out = nullptr) {
     float r[2][N], d[2][N], x[N];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         const int cref = c ? cref1 : cref0;
-        const int cnode = c ? cnode1 : cnode0;
         if (cref < 0) {
 #pragma unroll
             for (int i = 0; i < N; ++i) r[c][i] = lut[R.code[c] * N + i];
@@ -277,7 +277,10 @@ __device__ __forceinline__ void bin_finish_refs(const TreeDev &T, int v, int cre
 #pragma unroll
             for (int i = 0; i < N; ++i) r[c][i] = R.row[c][i];
         }
-        deficit<N>(r[c], T.D32 + (size_t)cnode * (N * N), d[c]);
+        if (c)
+            deficit<N>(r[c], D1, d[c]);
+        else
+            deficit<N>(r[c], D0, d[c]);
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) x[j] = (r[0][j] + d[0][j]) * (r[1][j] + d[1][j]);
@@ -298,6 +301,14 @@ __device__ __forceinline__ void bin_finish_refs(const TreeDev &T, int v, int cre
     acc.mul(f);
     if (T.offs) T.offs[(size_t)v * T.P + p] = log(f);  // update mode: keep the node's own normaliser
     store_post<N>(T, v, p, x, jm, xm, lds_out);
+}
+
+template <int N, class LP>
+__device__ __forceinline__ void bin_finish_refs(const TreeDev &T, int v, int cref0, int cref1, int cnode0, int cnode1,
+                                                int p, const BinRaw<N> &R, LP lut, LogAcc &acc,
+                                                float *lds_out = nullptr) {
+    bin_finish_m<N>(T, v, cref0, cref1, T.D32 + (size_t)cnode0 * (N * N), T.D32 + (size_t)cnode1 * (N * N), p, R, lut,
+                    acc, lds_out);
 }
 
 template <int N, class LP>
@@ -453,10 +464,10 @@ __device__ __forceinline__ void pre_fetch(const TreeDev &T, int c, int p, PreRaw
                    p, R);
 }
 
-template <int N, class LP>
-__device__ __forceinline__ void pre_finish_ir(const TreeDev &T, int c, int ir, int p, const PreRaw<N> &R, LP lut,
-                                              float *lds_out = nullptr) {
-    const auto M = T.P32 + (size_t)c * (N * N);
+// Preorder step with the branch's P(t) given as M (memory or registers).
+template <int N, class LP, class MP>
+__device__ __forceinline__ void pre_finish_m(const TreeDev &T, int c, int ir, int p, const PreRaw<N> &R, LP lut, MP M,
+                                             float *lds_out = nullptr) {
     float sub[N], m[N], q[N], msg[N];
     if (ir >= 0) {
 #pragma unroll
@@ -494,6 +505,12 @@ __device__ __forceinline__ void pre_finish_ir(const TreeDev &T, int c, int ir, i
 #pragma unroll
         for (int i = 0; i < N; ++i) lds_out[i] = out[i];
     }
+}
+
+template <int N, class LP>
+__device__ __forceinline__ void pre_finish_ir(const TreeDev &T, int c, int ir, int p, const PreRaw<N> &R, LP lut,
+                                              float *lds_out = nullptr) {
+    pre_finish_m<N>(T, c, ir, p, R, lut, T.P32 + (size_t)c * (N * N), lds_out);
 }
 
 template <int N, class LP>

@@ -5,18 +5,24 @@
 // medium tree is a chain of ~2 x depth dependent levels whose work is a few
 // microseconds in all.  Here a workgroup takes TP patterns through the whole
 // tree: the internal rows of those patterns live in LDS (n_int x TP x n
-// floats), every level is one round of the workgroup's threads over
-// (node, pattern) pairs, and levels are separated by a workgroup barrier
+// floats), a level is one or more rounds of the workgroup's threads over
+// (node, pattern) pairs, and rounds are separated by a workgroup barrier
 // instead of a launch and a round trip through HBM.  Patterns are
 // independent, so workgroups never wait for each other; the last one to
 // finish sums the per-pattern log-likelihoods (fixed order) into the total.
 //
+// What a thread needs from memory for one (node, pattern) step -- the
+// node's descriptor and its branches' P(t) or P - I -- is packed into one
+// record per level entry (k_tile_pack, refreshed whenever P(t) changes), and
+// the record of the thread's next round is loaded while the current round
+// computes, so a round costs LDS traffic, arithmetic and a barrier.
+//
 // Rows are written to HBM as well (subtree rows, profiles, states: the
 // results), but never read back.  The per-(node, pattern) arithmetic is the
-// level kernels' own device code (tt_device.h) on values read from LDS, so
-// profiles and states are identical to theirs; the per-pattern log
-// normalisers are summed per thread and then across the threads of a
-// pattern, so logL agrees to rounding.
+// level kernels' own device code (tt_device.h) on values read from LDS and
+// registers, so profiles and states are identical to theirs; the
+// per-pattern log normalisers are summed per thread and then across the
+// threads of a pattern, so logL agrees to rounding.
 //
 // Postorder levels are node heights (treeanc.py:840-878), the root step
 // treeanc.py:814-829, preorder levels node depths (treeanc.py:880-932): a
@@ -31,50 +37,86 @@ constexpr int kTileThreads = 512;
 
 __host__ __device__ inline size_t align4(size_t x) { return (x + 3) & ~(size_t)3; }
 
-// level-list descriptors read per lane (a wave covers several nodes)
-__device__ __forceinline__ PostDesc post_desc_at(const int32_t *d) {
-    const int4 a = *reinterpret_cast<const int4 *>(d), b = *reinterpret_cast<const int4 *>(d + 4);
-    return PostDesc{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-}
-__device__ __forceinline__ PreDesc pre_desc_at(const int32_t *d) {
-    const int4 a = *reinterpret_cast<const int4 *>(d);
-    return PreDesc{a.x, a.y, a.z, a.w};
-}
+// record sizes in 32-bit words, multiples of 4 (16-byte loads)
+__host__ __device__ constexpr int post_rec_words(int n) { return (8 + 2 * n * n + 3) & ~3; }
+__host__ __device__ constexpr int pre_rec_words(int n) { return (4 + n * n + 3) & ~3; }
 
 struct TileDev {
-    const int32_t *post_desc;  // kPostDesc ints per entry of the height-ordered internal rows
-    const int32_t *post_off;   // [H + 1] level offsets (the last level is the root)
-    const int32_t *pre_desc;   // kPreDesc ints per entry of the depth-ordered nodes
-    const int32_t *pre_off;    // [D + 1]
-    const int32_t *pre_nint;   // [D] internal nodes per depth (listed first)
-    int32_t H, D;
-    int32_t TP;                     // patterns per workgroup (power of two)
+    const int32_t *post_rec;    // per height-ordered internal row: PostDesc, P - I of child 0 and child 1
+    const int32_t *pre_rec;     // per depth-ordered node: {node, parent's row, row, tip row}, P(t)
+    const int32_t *rounds;      // rounds as [first, end) entry pairs: post [Rp], preorder internal [Ri], all [Ra]
+    int32_t Rp, Ri, Ra;
+    int32_t TP;                 // patterns per workgroup (power of two)
     int32_t n_blocks;
-    double *partial;                // [n_blocks] sum over the workgroup's patterns of mult * logL
-    unsigned *counter;              // workgroups finished; reset to 0 by the last one
+    double *partial;            // [n_blocks] sum over the workgroup's patterns of mult * logL
+    unsigned *counter;          // workgroups finished; reset to 0 by the last one
 };
+
+// Workgroup barrier that orders LDS only: rounds talk through LDS, and their
+// HBM writes (results nobody in the launch reads back) need not drain first.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int N>
+struct PostRec {
+    int4 a, b;  // v, e0, nchild, cref0 | cref1, cnode0, cnode1, vn
+    float D0[N * N], D1[N * N];
+};
+
+template <int N>
+struct PreRec {
+    int4 a;  // node, parent's row, row (or -1), tip row (or -1)
+    float M[N * N];
+};
+
+template <int N>
+__device__ __forceinline__ void load_rec(const int32_t *base, PostRec<N> &r) {
+    const int4 *s = reinterpret_cast<const int4 *>(base);
+    r.a = s[0];
+    r.b = s[1];
+    const float *f = reinterpret_cast<const float *>(base + 8);
+#pragma unroll
+    for (int i = 0; i < N * N; ++i) {
+        r.D0[i] = f[i];
+        r.D1[i] = f[N * N + i];
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void load_rec(const int32_t *base, PreRec<N> &r) {
+    r.a = *reinterpret_cast<const int4 *>(base);
+    const float *f = reinterpret_cast<const float *>(base + 4);
+#pragma unroll
+    for (int i = 0; i < N * N; ++i) r.M[i] = f[i];
+}
 
 template <int N>
 __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileDev S, int tips, int pre) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     const int TP = S.TP;
-    // LDS layout (tile_lds_bytes): rows, profile map, reduction, tip codes;
-    // each part starts on a 16-byte boundary
-    float *rows = reinterpret_cast<float *>(s_dyn);                               // [n_int][TP][N]
-    float *s_lut = rows + align4((size_t)T.n_int * TP * N);                       // [n_codes][N]
+    // LDS layout (tile_lds_bytes): rows, profile map, reduction, round
+    // offsets, tip codes; each part starts on a 16-byte boundary
+    float *rows = reinterpret_cast<float *>(s_dyn);                                      // [n_int][TP][N]
+    float *s_lut = rows + align4((size_t)T.n_int * TP * N);                              // [n_codes][N]
     double *s_red = reinterpret_cast<double *>(s_lut + align4((size_t)T.n_codes * N));  // [kTileThreads]
-    uint8_t *s_tip = reinterpret_cast<uint8_t *>(s_red + kTileThreads);             // [n_tips][TP]
+    int *s_rounds = reinterpret_cast<int *>(s_red + kTileThreads);
+    const int n_rounds = 2 * (S.Rp + S.Ri + S.Ra);
+    uint8_t *s_tip = reinterpret_cast<uint8_t *>(s_rounds + align4(n_rounds));  // [n_tips][TP]
     __shared__ double s_sum[TT_WAVE];
     __shared__ bool s_last;
 
     const int t = threadIdx.x;
-    const int q = t & (TP - 1);            // this thread's pattern within the tile
-    const int k0 = t / TP, kstep = kTileThreads / TP;
+    const int q = t & (TP - 1);  // this thread's pattern within the tile
+    const int k0 = t / TP;       // this thread's entry within a round
     const int p = blockIdx.x * TP + q;
     const bool act = p < T.P;
 
-    // stage the profile map and the tile's tip characters
+    // stage the profile map, the round offsets and the tile's tip characters
     for (int i = t; i < T.n_codes * N; i += kTileThreads) s_lut[i] = T.lut[i];
+    for (int i = t; i < n_rounds; i += kTileThreads) s_rounds[i] = S.rounds[i];
     for (int i = t; i < T.n_tips * TP; i += kTileThreads) {
         const int r = i / TP, pp = blockIdx.x * TP + (i & (TP - 1));
         s_tip[i] = pp < T.P ? T.tip_codes[(size_t)r * T.P + pp] : 0;
@@ -82,45 +124,56 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
     __syncthreads();
 
     auto slot = [&](int row) { return rows + ((size_t)row * TP + q) * N; };
-    auto rowf = [&](int cref, float (&r)[N]) {
-        if (cref < 0) {
-            const int code = s_tip[(-cref - 1) * TP + q];
-#pragma unroll
-            for (int i = 0; i < N; ++i) r[i] = s_lut[code * N + i];
-        } else {
-            const float *src = slot(cref);
-#pragma unroll
-            for (int i = 0; i < N; ++i) r[i] = src[i];
-        }
-    };
 
-    // ---- postorder: one round per height -----------------------------------
+    // ---- postorder: rounds over height-ordered internal rows ---------------------
     LogAcc acc;
-    for (int h = 0; h < S.H; ++h) {
-        const int kb = S.post_off[h], ke = S.post_off[h + 1];
-        for (int k = kb + k0; k < ke; k += kstep) {
-            const PostDesc d = post_desc_at(S.post_desc + kPostDesc * k);
-            if (!act) continue;
-            if (d.nchild == 2) {
-                BinRaw<N> R;
+    {
+        constexpr int W = post_rec_words(N);
+        const int *rd = s_rounds;
+        PostRec<N> cur, nxt;
+        int k = S.Rp > 0 ? rd[0] + k0 : 0;
+        if (S.Rp > 0 && k < rd[1]) load_rec<N>(S.post_rec + (size_t)W * k, cur);
+        for (int r = 0; r < S.Rp; ++r) {
+            const bool valid = k < rd[2 * r + 1];
+            const int kn = r + 1 < S.Rp ? rd[2 * r + 2] + k0 : 0;
+            const bool vnext = r + 1 < S.Rp && kn < rd[2 * r + 3];
+            if (vnext) load_rec<N>(S.post_rec + (size_t)W * kn, nxt);  // the next round's record, in flight
+            if (valid && act) {
+                const int v = cur.a.x, nchild = cur.a.z, cref0 = cur.a.w, cref1 = cur.b.x;
+                if (nchild == 2) {
+                    BinRaw<N> R;
 #pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    const int cref = c ? d.cref1 : d.cref0;
-                    if (cref < 0) {
-                        R.code[c] = s_tip[(-cref - 1) * TP + q];
-                    } else {
-                        R.code[c] = 0;
-                        const float *src = slot(cref);
+                    for (int c = 0; c < 2; ++c) {
+                        const int cref = c ? cref1 : cref0;
+                        if (cref < 0) {
+                            R.code[c] = s_tip[(-cref - 1) * TP + q];
+                        } else {
+                            R.code[c] = 0;
+                            const float *src = slot(cref);
 #pragma unroll
-                        for (int i = 0; i < N; ++i) R.row[c][i] = src[i];
+                            for (int i = 0; i < N; ++i) R.row[c][i] = src[i];
+                        }
                     }
+                    bin_finish_m<N>(T, v, cref0, cref1, cur.D0, cur.D1, p, R, s_lut, acc, slot(v));
+                } else {  // polytomies and unary nodes read their child lists from memory
+                    auto rowf = [&](int cref, float (&row)[N]) {
+                        if (cref < 0) {
+                            const int code = s_tip[(-cref - 1) * TP + q];
+#pragma unroll
+                            for (int i = 0; i < N; ++i) row[i] = s_lut[code * N + i];
+                        } else {
+                            const float *src = slot(cref);
+#pragma unroll
+                            for (int i = 0; i < N; ++i) row[i] = src[i];
+                        }
+                    };
+                    post_node_f<N>(T, v, p, s_lut, acc, rowf, [](int e) { return e; }, slot(v));
                 }
-                bin_finish_refs<N>(T, d.v, d.cref0, d.cref1, d.cnode0, d.cnode1, p, R, s_lut, acc, slot(d.v));
-            } else {
-                post_node_f<N>(T, d.v, p, s_lut, acc, rowf, [](int e) { return e; }, slot(d.v));
             }
+            lds_barrier();
+            if (vnext) cur = nxt;
+            k = kn;
         }
-        __syncthreads();
     }
 
     // ---- root (treeanc.py:814-829) ------------------------------------------
@@ -160,36 +213,45 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
     }
     __syncthreads();
 
-    // ---- preorder: one round per depth ----------------------------------------
+    // ---- preorder: rounds over depth-ordered nodes -------------------------------
     if (pre) {
-        for (int dd = 0; dd < S.D; ++dd) {
-            const int kb = S.pre_off[dd];
-            const int ke = tips ? S.pre_off[dd + 1] : kb + S.pre_nint[dd];
-            for (int k = kb + k0; k < ke; k += kstep) {
-                const PreDesc c = pre_desc_at(S.pre_desc + kPreDesc * k);
-                if (!act) continue;
-                PreRaw<N> R;
-                const float *pp = slot(T.int_row[c.par]);
+        constexpr int W = pre_rec_words(N);
+        const int *rd = s_rounds + 2 * (tips ? S.Rp + S.Ri : S.Rp);
+        const int R = tips ? S.Ra : S.Ri;
+        PreRec<N> cur, nxt;
+        int k = R > 0 ? rd[0] + k0 : 0;
+        if (R > 0 && k < rd[1]) load_rec<N>(S.pre_rec + (size_t)W * k, cur);
+        for (int r = 0; r < R; ++r) {
+            const bool valid = k < rd[2 * r + 1];
+            const int kn = r + 1 < R ? rd[2 * r + 2] + k0 : 0;
+            const bool vnext = r + 1 < R && kn < rd[2 * r + 3];
+            if (vnext) load_rec<N>(S.pre_rec + (size_t)W * kn, nxt);
+            if (valid && act) {
+                const int c = cur.a.x, prow = cur.a.y, ir = cur.a.z, tr = cur.a.w;
+                PreRaw<N> Rw;
+                const float *pp = slot(prow);
 #pragma unroll
-                for (int i = 0; i < N; ++i) R.pp[i] = pp[i];
-                if (c.ir >= 0) {
-                    R.code = 0;
-                    const float *src = slot(c.ir);
+                for (int i = 0; i < N; ++i) Rw.pp[i] = pp[i];
+                if (ir >= 0) {
+                    Rw.code = 0;
+                    const float *src = slot(ir);
 #pragma unroll
-                    for (int i = 0; i < N; ++i) R.sub[i] = src[i];
+                    for (int i = 0; i < N; ++i) Rw.sub[i] = src[i];
                 } else {
-                    R.code = s_tip[c.tr * TP + q];
+                    Rw.code = s_tip[tr * TP + q];
                 }
-                pre_finish_ir<N>(T, c.c, c.ir, p, R, s_lut, c.ir >= 0 ? slot(c.ir) : nullptr);
+                pre_finish_m<N>(T, c, ir, p, Rw, s_lut, cur.M, ir >= 0 ? slot(ir) : nullptr);
             }
-            __syncthreads();
+            lds_barrier();
+            if (vnext) cur = nxt;
+            k = kn;
         }
     }
 
     // ---- workgroup sum of mult * logL, then the total over workgroups -------
     if (t < TT_WAVE) {
         double a = 0.0;
-        for (int k = t; k < TP; k += TT_WAVE) a += (k < TP && blockIdx.x * TP + k < T.P) ? s_red[k] : 0.0;
+        for (int k = t; k < TP; k += TT_WAVE) a += (blockIdx.x * TP + k < T.P) ? s_red[k] : 0.0;
         s_sum[t] = a;
     }
     __syncthreads();
@@ -202,13 +264,50 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
         s_last = done == (unsigned)S.n_blocks - 1;
     }
     __syncthreads();
-    if (s_last && t == 0) {  // the last workgroup: sum the partials in workgroup order
+    if (s_last) {  // the last workgroup: sum the partials, every thread a strided share, then a fixed tree
         __threadfence();
         double a = 0.0;
-        for (int b = 0; b < S.n_blocks; ++b)
+        for (int b = t; b < S.n_blocks; b += kTileThreads)
             a += __hip_atomic_load(&S.partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        T.total[0] = a;
-        __hip_atomic_store(S.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_red[t] = a;
+        __syncthreads();
+        for (int w = kTileThreads / 2; w > 0; w >>= 1) {
+            if (t < w) s_red[t] += s_red[t + w];
+            __syncthreads();
+        }
+        if (t == 0) {
+            T.total[0] = s_red[0];
+            __hip_atomic_store(S.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Records of the tile pass from the level lists and the current P(t): one
+// thread per post entry (g < n_post) or pre entry.
+template <int N>
+__global__ void k_tile_pack(TreeDev T, const int32_t *post_desc, int n_post, const int32_t *pre_desc, int n_pre,
+                            int32_t *post_rec, int32_t *pre_rec) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n_post) {
+        const int32_t *d = post_desc + kPostDesc * g;
+        int32_t *o = post_rec + (size_t)post_rec_words(N) * g;
+        for (int i = 0; i < kPostDesc; ++i) o[i] = d[i];
+        float *f = reinterpret_cast<float *>(o + 8);
+        const bool bin = d[2] == 2;
+        for (int i = 0; i < N * N; ++i) {
+            f[i] = bin ? T.D32[(size_t)d[5] * (N * N) + i] : 0.0f;
+            f[N * N + i] = bin ? T.D32[(size_t)d[6] * (N * N) + i] : 0.0f;
+        }
+    } else if (g < n_post + n_pre) {
+        const int k = g - n_post;
+        const int32_t *d = pre_desc + kPreDesc * k;
+        int32_t *o = pre_rec + (size_t)pre_rec_words(N) * k;
+        o[0] = d[0];
+        o[1] = T.int_row[d[1]];
+        o[2] = d[2];
+        o[3] = d[3];
+        float *f = reinterpret_cast<float *>(o + 4);
+        for (int i = 0; i < N * N; ++i) f[i] = T.P32[(size_t)d[0] * (N * N) + i];
     }
 }
 
@@ -216,36 +315,64 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
 
 namespace ttk {
 
-// LDS bytes of k_tile_marginal for tp patterns per workgroup.
-size_t tile_lds_bytes(const TreeDev &T, int tp) {
+int tile_threads() { return kTileThreads; }
+int tile_post_rec_words(int n) { return post_rec_words(n); }
+int tile_pre_rec_words(int n) { return pre_rec_words(n); }
+
+// LDS bytes of k_tile_marginal for tp patterns per workgroup and n_rounds
+// round bounds (two per round) in all.
+size_t tile_lds_bytes(const TreeDev &T, int tp, int n_rounds) {
     const size_t rows = align4((size_t)T.n_int * tp * T.n) * sizeof(float);
     const size_t lut = align4((size_t)T.n_codes * T.n) * sizeof(float);
     const size_t red = (size_t)kTileThreads * sizeof(double);
+    const size_t rounds = align4((size_t)n_rounds) * sizeof(int);
     const size_t tips = ((size_t)T.n_tips * tp + 15) & ~(size_t)15;
-    return rows + lut + red + tips;
+    return rows + lut + red + rounds + tips;
 }
 
-int tile_threads() { return kTileThreads; }
-
-hipError_t tile_marginal(const TreeDev &T, const int32_t *post_desc, const int32_t *post_off, int H,
-                         const int32_t *pre_desc, const int32_t *pre_off, const int32_t *pre_nint, int D, int tp,
-                         double *partial, unsigned *counter, int tips, int pre, hipStream_t s) {
-    TileDev S{post_desc, post_off, pre_desc, pre_off, pre_nint, H, D, tp, (T.P + tp - 1) / tp, partial, counter};
-    const size_t lds = tile_lds_bytes(T, tp);
-    switch (T.n) {
-#define TT_TILE_CASE(NN)                                                                                          \
-    case NN:                                                                                                      \
-        hipLaunchKernelGGL(k_tile_marginal<NN>, dim3((unsigned)S.n_blocks), dim3(kTileThreads), lds, s, T, S, tips, \
-                           pre);                                                                                  \
-        break;
-        TT_TILE_CASE(2)
-        TT_TILE_CASE(3)
-        TT_TILE_CASE(4)
-        TT_TILE_CASE(5)
-#undef TT_TILE_CASE
-        default:
-            return hipErrorInvalidValue;
+#define TT_TILE_DISPATCH(n, CALL)                        \
+    switch (n) {                                         \
+        case 2: { constexpr int N_ = 2; CALL; } break;   \
+        case 3: { constexpr int N_ = 3; CALL; } break;   \
+        case 4: { constexpr int N_ = 4; CALL; } break;   \
+        case 5: { constexpr int N_ = 5; CALL; } break;   \
+        default: return hipErrorInvalidValue;            \
     }
+
+// Workgroups of k_tile_marginal one CU holds at once (registers and LDS).
+int tile_blocks_per_cu(int n, size_t lds) {
+    int b = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (n) {
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_tile_marginal<2>, kTileThreads, lds); break;
+        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_tile_marginal<3>, kTileThreads, lds); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_tile_marginal<4>, kTileThreads, lds); break;
+        case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_tile_marginal<5>, kTileThreads, lds); break;
+        default: break;
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return b;
+}
+
+hipError_t tile_pack(const TreeDev &T, const int32_t *post_desc, int n_post, const int32_t *pre_desc, int n_pre,
+                     int32_t *post_rec, int32_t *pre_rec, hipStream_t s) {
+    const int total = n_post + n_pre;
+    if (!total) return hipSuccess;
+    TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL(k_tile_pack<N_>, dim3((total + 255) / 256), dim3(256), 0, s, T,
+                                             post_desc, n_post, pre_desc, n_pre, post_rec, pre_rec));
+    return hipGetLastError();
+}
+
+hipError_t tile_marginal(const TreeDev &T, const int32_t *post_rec, const int32_t *pre_rec, const int32_t *rounds,
+                         int Rp, int Ri, int Ra, int tp, double *partial, unsigned *counter, int tips, int pre,
+                         hipStream_t s) {
+    TileDev S{post_rec, pre_rec, rounds, Rp, Ri, Ra, tp, (T.P + tp - 1) / tp, partial, counter};
+    const size_t lds = tile_lds_bytes(T, tp, 2 * (Rp + Ri + Ra));
+    TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL(k_tile_marginal<N_>, dim3((unsigned)S.n_blocks), dim3(kTileThreads),
+                                             lds, s, T, S, tips, pre));
     return hipGetLastError();
 }
 
