@@ -26,7 +26,11 @@ import platform
 import sys
 import time
 
-import numpy as np
+# the CPU baseline times one BLAS thread per process: set before numpy loads
+for _v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+    os.environ.setdefault(_v, "1")
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -46,8 +50,8 @@ def parse():
     ap.add_argument("--workload", default="mix")
     ap.add_argument("--jobs", type=int, default=256)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-tree (drop-in) passes")
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-compress", action="store_true", help="skip the pattern-compression measurement")
@@ -167,17 +171,53 @@ def oracle_tree(prep):
     return O.OracleTreeAnc(root, g, comp, prep["multiplicity"], full_length=1e300)
 
 
-def cpu_baseline(preps, seconds):
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
-    done_u, t0, used = 0, time.perf_counter(), []
-    for i in sorted(range(len(preps)), key=lambda i: updates(preps[i])):
-        oracle_marginal(preps[i])
-        done_u += updates(preps[i])
-        used.append(i)
-        if time.perf_counter() - t0 > seconds:
-            break
-    dt = time.perf_counter() - t0
-    return done_u / dt, dt, used
+def cpu_info():
+    """The host the CPU baseline ran on."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    env = {v: os.environ.get(v) for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    return {"cpu_model": model, "affinity_cores": cores, "blas_threads_env": env}
+
+
+def _oracle_job(prep):
+    t0 = time.perf_counter()
+    oracle_marginal(prep)
+    return updates(prep), time.perf_counter() - t0
+
+
+def cpu_baseline(preps, workers):
+    """FP64 numpy oracle marginal passes on the host, jobs across the size
+    range: (a) one process, one BLAS thread, the jobs at the centres of the
+    four quartiles of the updates per job; (b) `workers` processes, one BLAS
+    thread each, over every job below the 90th percentile (largest first),
+    run to completion: aggregate updates/s over the wall time."""
+    order = sorted(range(len(preps)), key=lambda i: updates(preps[i]))
+    n = len(order)
+    picks = sorted({order[min(n - 1, int(q * n))] for q in (0.125, 0.375, 0.625, 0.875)})
+    t0 = time.perf_counter()
+    done_u = sum(_oracle_job(preps[i])[0] for i in picks)
+    single = (done_u / (time.perf_counter() - t0), time.perf_counter() - t0, picks)
+    par = None
+    if workers > 1:
+        jobs = [order[j] for j in range(int(0.9 * n) - 1, -1, -1)]
+        ctx = mp.get_context("fork")  # no GPU has been touched yet in this process
+        t0 = time.perf_counter()
+        with ctx.Pool(workers) as pool:
+            done_u = sum(u for u, _ in pool.imap_unordered(_oracle_job, [preps[i] for i in jobs], chunksize=1))
+        wall = time.perf_counter() - t0
+        par = (done_u / wall, wall, len(jobs))
+    return single, par
 
 
 def serve_bench(eng, pool, args):
@@ -403,6 +443,28 @@ def site_bench(eng, torch, steps, n_tips=1000, L=5000, seed=77, cpu=True):
     return out
 
 
+def hbm_ceilings(torch, nbytes=4 << 30, reps=5):
+    """Streaming rates of the library's own kernels (tt_hbm_probe: 16-byte
+    accesses, every CU busy), GB/s of bytes moved: read-only sweep, copy
+    (read + write) and write-only sweep."""
+    import ctypes as C
+
+    from treetime_amd import _native
+
+    lib = _native.load()
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    ms = C.c_double()
+    out = {}
+    for kind, name, moved in ((0, "read", nbytes), (1, "copy", 2 * nbytes), (2, "write", nbytes)):
+        _native.check(lib.tt_hbm_probe(kind, b.data_ptr(), a.data_ptr(), nbytes, reps, C.byref(ms),
+                                       torch.cuda.current_stream().cuda_stream))
+        out[name] = moved / (ms.value / 1e3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_ceiling(torch, nbytes=2 << 30, reps=5):
     """Streaming rates of plain device kernels (GB/s, bytes moved per second):
     copy (read + write) and write-only (fill) -- the measured ceilings beside
@@ -440,6 +502,80 @@ def measured_traffic(path, workload):
     return None
 
 
+SINGLE_TREES = ("plain-200", "large-5000", "large-5000-deep", "aa-1000")
+
+
+def single_spec(name):
+    from treetime_amd import synth
+
+    n_tips, L, depth, n_states, seed, pa, pg = synth.WORKLOADS[name]
+    return dict(n_tips=n_tips, L=L, depth=depth, n_states=n_states, seed=seed, p_ambig=pa, p_gap=pg, name=name)
+
+
+def oracle_logl_of(name, prep):
+    """The FP64 oracle's marginal and joint logL of a single-tree workload:
+    computed here for the 200-tip tree, taken from the committed oracle
+    digests (tests/golden/configs_golden.npz, checked against the inputs'
+    hash) for the large ones, whose oracle runs take minutes."""
+    if name == "plain-200":
+        return oracle_marginal(prep), oracle_tree(prep).joint()
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import config_digest as CD
+
+    path = os.path.join(ROOT, "tests", "golden", "configs_golden.npz")
+    if not os.path.exists(path):
+        return None, None
+    with np.load(path) as z:
+        if f"{name}/hash" not in z.files or str(z[f"{name}/hash"]) != CD.input_hash(prep):
+            return None, None
+        return float(z[f"{name}/m_logL"]), float(z[f"{name}/j_logL"])
+
+
+def single_tree_bench(eng, torch, named, reps=50):
+    """The plain (drop-in) passes on one tree at a time: ms per marginal,
+    log-likelihood and joint pass (HIP events over `reps` passes on one
+    stream), updates/s, the algorithmic bytes rate against the HBM peak, and
+    logL against the FP64 oracle."""
+    from treetime_amd.prepare import device_tree
+
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    out = {}
+    for name, prep in named.items():
+        dt = device_tree(eng, prep)
+        U = updates(prep)
+        bytes_of = {"marginal": algorithmic_bytes(prep), "loglik": loglik_algorithmic_bytes(prep),
+                    "joint": joint_algorithmic_bytes(prep)}
+        res = {"tips": int((len(prep["parent"]) + 1) // 2), "patterns": int(prep["tip_codes"].shape[1]),
+               "states": int(prep["n_states"]), "updates_per_pass": int(U)}
+        got = {}
+        for kind in ("marginal", "loglik", "joint"):
+            f = getattr(dt, kind)
+            got[kind] = f(stream=sp)
+            for _ in range(3):
+                f(stream=sp, want_logL=False)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                f(stream=sp, want_logL=False)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            res[kind] = {"ms": ms, "updates_per_s": U / (ms / 1e3), "algorithmic_bytes": int(bytes_of[kind]),
+                         "achieved_GBs": bytes_of[kind] / (ms / 1e3) / 1e9,
+                         "frac_of_hbm_peak": bytes_of[kind] / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+        dt.close()
+        ref_m, ref_j = oracle_logl_of(name, prep)
+        res["logL_rel_err_vs_fp64_oracle"] = {
+            "marginal": abs(got["marginal"] - ref_m) / abs(ref_m) if ref_m is not None else None,
+            "joint": abs(got["joint"] - ref_j) / abs(ref_j) if ref_j is not None else None}
+        out[name] = res
+        log(f"[bench] single tree {name}: marginal {res['marginal']['ms']:.4f} ms, loglik {res['loglik']['ms']:.4f} ms, "
+            f"joint {res['joint']['ms']:.4f} ms; logL rel err {res['logL_rel_err_vs_fp64_oracle']}")
+    return out
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -475,9 +611,29 @@ def main():
         n_tips, L, depth, n_states, seed, pa, pg = synth.WORKLOADS[args.workload]
         specs = [dict(n_tips=n_tips, L=L, depth=depth, n_states=n_states, seed=seed, p_ambig=pa, p_gap=pg)]
         wname = args.workload
+    singles = list(SINGLE_TREES) if (args.workload == "mix" and not args.no_single) else []
     t0 = time.perf_counter()
-    preps = gen_preps(specs, n_workers(args.workers))
-    log(f"[bench] generated {len(preps)} jobs in {time.perf_counter() - t0:.1f}s")
+    all_preps = gen_preps(specs + [single_spec(nm) for nm in singles], n_workers(args.workers))
+    preps, single_preps = all_preps[:len(specs)], dict(zip(singles, all_preps[len(specs):]))
+    log(f"[bench] generated {len(all_preps)} jobs in {time.perf_counter() - t0:.1f}s")
+
+    # CPU baseline first: the worker processes fork before anything touches the GPU
+    cpu = None
+    if not args.no_cpu:
+        info = cpu_info()
+        workers = n_workers(args.workers)
+        (rate1, secs1, picks), par = cpu_baseline(preps, workers)
+        log(f"[bench] cpu baseline: 1 core {rate1:.3e} U/s over {secs1:.1f}s ({len(picks)} jobs)"
+            + (f"; {workers} cores {par[0]:.3e} U/s over {par[1]:.1f}s ({par[2]} jobs)" if par else ""))
+        cpu = {"value": par[0] if par else rate1, "unit": "pattern*branch updates/s",
+               "cores": workers if par else 1, "kind": "port",
+               "sample": (f"FP64 numpy oracle marginal passes, one BLAS thread per process: {workers} processes "
+                          f"over the {par[2]} jobs of the mix below its 90th percentile of updates per job, "
+                          f"{par[1]:.1f}s wall" if par else
+                          f"FP64 numpy oracle marginal passes on {len(picks)} jobs"),
+               "single_core_value": rate1,
+               "single_core_sample": f"jobs {picks} (centres of the quartiles of updates per job), {secs1:.1f}s",
+               **info}
 
     import torch
 
@@ -496,9 +652,11 @@ def main():
     BL = sum(loglik_algorithmic_bytes(p) for p in preps)
     P_total = sum(p["tip_codes"].shape[1] for p in preps)
 
-    ceilings = copy_ceiling(torch)
+    ceilings = hbm_ceilings(torch)
+    torch_ceilings = copy_ceiling(torch)
     ceiling = ceilings["copy"]
-    log(f"[bench] streaming ceilings (GB/s): copy {ceilings['copy']:.0f}, write {ceilings['write']:.0f}")
+    log(f"[bench] streaming ceilings (GB/s), own kernels: read {ceilings['read']:.0f}, copy {ceilings['copy']:.0f}, "
+        f"write {ceilings['write']:.0f}; torch: copy {torch_ceilings['copy']:.0f}, fill {torch_ceilings['write']:.0f}")
     stream = torch.cuda.Stream()
     sp = stream.cuda_stream
     # correctness: batched logL equals the plain pass per tree
@@ -608,13 +766,7 @@ def main():
     if not args.no_serve and len(trees) > 1:
         serving = serve_bench(eng, trees[: args.serve_pool], args)
 
-    cpu = None
-    if not args.no_cpu:
-        rate, secs, used = cpu_baseline(preps, args.cpu_seconds)
-        cpu = {"value": rate, "unit": "pattern*branch updates/s", "cores": 1, "kind": "port",
-               "sample": f"FP64 numpy oracle marginal pass on {len(used)} of {len(preps)} jobs (smallest first), "
-                         f"{secs:.1f}s on {platform.processor() or platform.machine()}"}
-        log(f"[bench] cpu baseline {rate:.3e} U/s over {secs:.1f}s")
+    single = single_tree_bench(eng, torch, single_preps) if single_preps else None
 
     value = U / (ms / 1e3)
     achieved = B / (ms / 1e3) / 1e9
@@ -660,9 +812,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.traffic_file, wname),
                      "copy_ceiling": ceiling, "frac_of_copy_ceiling": achieved / ceiling,
-                     "streaming_ceilings": ceilings,
+                     "streaming_ceilings": {**ceilings, "torch_copy": torch_ceilings["copy"],
+                                            "torch_fill": torch_ceilings["write"]},
                      "algorithmic_bytes_per_launch": int(B)},
         "cpu_baseline": cpu,
+        "single_tree": single,
         "serving": serving,
         "compression": compression,
         "site_specific": site,

@@ -391,6 +391,17 @@ int32_t tt_compress_alignment(tt_engine *eng, const uint8_t *chars, int32_t n_se
                               int64_t *n_patterns, int32_t *pattern_of_column, uint8_t *patterns,
                               double *multiplicity, int64_t *char_counts, void *stream);
 
+/* ---- measurement -------------------------------------------------------------
+ * Streaming HBM ceiling of the device with hand-written kernels (no
+ * reference counterpart; the bench reports its rooflines against it): kind 0
+ * reads `bytes` of src (dst: a sink of at least as many bytes, not written
+ * in practice), 1 copies src to dst, 2 writes dst; 16-byte accesses, every
+ * CU busy.  *ms = mean time of one sweep over reps sweeps (after one warm-up),
+ * timed with events on `stream`.  Buffers are device memory, 16-byte aligned,
+ * bytes a multiple of 16. */
+int32_t tt_hbm_probe(int32_t kind, void *dst, const void *src, uint64_t bytes, int32_t reps, double *ms,
+                     void *stream);
+
 #ifdef __cplusplus
 }
 #endif

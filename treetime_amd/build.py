@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libttgpu.so")
 OBJ = os.path.join(HERE, "_build")
 SOURCES = ["tt_kernels.hip", "tt_api.hip", "tt_batch.hip", "tt_mfma.hip", "tt_branch.hip", "tt_compress.hip",
-           "tt_site.hip", "tt_noalloc.hip", "tt_tile.hip"]
+           "tt_site.hip", "tt_noalloc.hip", "tt_tile.hip",
+           "tt_probe.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TT_OFFLOAD_ARCH", "gfx950")
