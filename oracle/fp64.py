@@ -24,6 +24,8 @@ oracle function             reference
                             interpolator), 376-437 (propagate/evolve),
                             491-495 (average_rate)
 ``compress_alignment``      treetime/sequence_data.py:325-464
+``compress_sparse_alignment`` treetime/vcf_utils.py:657-749,
+                            treetime/sequence_data.py:362-413 (sparse)
 ``OracleTreeAnc.marginal``  treetime/treeanc.py:762-932
 ``OracleTreeAnc.joint``     treetime/treeanc.py:934-1080 (root_rng: the
                             root draw of 1022-1037, seq_utils.py:257-269)
@@ -50,6 +52,12 @@ Pinning (tests/test_oracle_*.py):
   and sequence_LH given the drawn root.
 * compression: hand-built columns hitting every rule of
   sequence_data.py:387-402.
+* sparse compression: hand-built reference + differences hitting every
+  class of vcf_utils.py:694-718; on random inputs, every position's column
+  equals the dense compression's column of the expanded alignment and the
+  pattern multiplicities agree (dense compression pinned above), apart from
+  the positions whose constant state is the ambiguous character, which the
+  sparse path leaves out of every pattern.
 * prob_t_profiles: one-hot profiles reduce to prob_t_compressed.
 * optimal_t_compressed: JC69 closed form t = -3/4 ln(1 - 4/3 p).
 * SiteGTRModel: exact P(t) of every site = scipy.linalg.expm of the site's
@@ -571,6 +579,76 @@ def compress_alignment(names, seqs, ambiguous, additional_constant_sites=0, noga
         c2f[p] = np.array(plist, dtype=int)
     comp = np.array(columns).T
     return {k: comp[i] for i, k in enumerate(names)}, mult, f2c, c2f
+
+
+def compress_sparse_alignment(names, diffs, ref, ambiguous):
+    """Restatement of the sparse (reference + differences) compression:
+    process_alignment_dictionary (treetime/vcf_utils.py:657-749) followed by
+    the is_sparse branch of make_compressed_alignment
+    (treetime/sequence_data.py:362-413, 444-464).
+
+    diffs: dict name -> {position: character}; ref: (L,) char array.
+    Returns (compressed dict name -> (P,) char array, multiplicity (P,),
+    full_to_compressed (L,), compressed_to_full dict p -> positions,
+    nonref_positions (sorted), inferred constant sites (list)).
+    """
+    nseq = len(names)
+    inv_map = {}
+    for k in names:
+        for pos, b in diffs[k].items():
+            inv_map.setdefault(pos, []).append(b)
+    nonref_positions = np.sort(list(inv_map.keys()))
+    const_up_to_amb, nonref_const, nonref_alleles, variable = [], [], [], []
+    for pos, bs in inv_map.items():
+        bases = list(np.unique(bs))
+        if len(bs) == nseq:
+            if (len(bases) <= 2 and ambiguous in bases) or len(bases) == 1:
+                nonref_const.append(pos)
+                nonref_alleles.append(bases[0] if len(bases) == 1 else [x for x in bases if x != ambiguous][0])
+                if ambiguous in bases:
+                    const_up_to_amb.append(pos)
+            else:
+                variable.append(pos)
+        elif len(bases) == 1 and bases[0] == ambiguous:
+            const_up_to_amb.append(pos)
+        else:
+            variable.append(pos)
+    ref_mod = np.copy(ref)
+    ref_mod[nonref_const] = nonref_alleles
+    states = np.unique(ref_mod)          # taken before the variable sites are masked
+    ref_mod[variable] = "."
+    columns, patterns = [], {}
+    for base in states:
+        if base == ambiguous:
+            continue
+        where = list(np.where(ref_mod == base)[0])
+        if where:
+            patterns[base * nseq] = [len(columns), where]
+            columns.append(np.repeat(base, nseq))
+    for pos in variable:
+        col = np.array([diffs[k][pos] if pos in diffs[k] else ref[pos] for k in names])
+        uniq = list(np.unique(col))
+        if len(uniq) == 2 and ambiguous in uniq:
+            other = [c for c in uniq if c != ambiguous][0]
+            col[col == ambiguous] = other
+            uniq = [other]
+        key = "".join(col.astype("U"))
+        if len(uniq) > 1:
+            key += "_%d" % pos
+        if key not in patterns:
+            patterns[key] = [len(columns), [pos]]
+            columns.append(col)
+        else:
+            patterns[key][1].append(pos)
+    mult = np.zeros(len(patterns))
+    f2c = np.zeros(len(ref), dtype=int)
+    c2f = {}
+    for p, plist in patterns.values():
+        mult[p] = len(plist)
+        f2c[np.array(plist)] = p
+        c2f[p] = np.array(plist, dtype=int)
+    comp = np.array(columns).T
+    return ({k: comp[i] for i, k in enumerate(names)}, mult, f2c, c2f, nonref_positions, const_up_to_amb)
 
 
 # ---------------------------------------------------------------------------

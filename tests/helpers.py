@@ -76,3 +76,40 @@ def polytomy_job(n_tips, L, n_states=4, seed=1, frac=0.3, depth=0.05, p_ambig=0.
     job = collapse_job(synth.make_job(n_tips, L, depth, n_states, seed, p_ambig, p_gap, p_iupac), frac, seed + 1)
     prep = prepare(job["parent"], job["branch_length"], job["chars"], job["gtr"], job["ambiguous"])
     return job, prep, synth.ALPHABET_OF[n_states]
+
+
+def sparse_case(seed, nseq, L, amb="N", restate=0.05):
+    """A reference + differences alignment with every class of position of
+    vcf_utils.py:694-718: private and shared variants, positions where every
+    sequence differs (with and without the ambiguous character), positions
+    ambiguous in a few sequences, ambiguous reference positions, and a few
+    entries that restate the reference character.  Each sequence's dict is
+    in shuffled position order.
+
+    Returns (names, diffs, ref (L,) chars, dense (nseq, L) chars)."""
+    rng = np.random.default_rng(seed)
+    ref = rng.choice(np.array(list("ACGT" + amb + "-")), size=L, p=[0.24, 0.24, 0.24, 0.24, 0.02, 0.02])
+    A = np.repeat(ref[None, :], nseq, axis=0)
+    mut = rng.random((nseq, L)) < 0.03
+    A[mut] = rng.choice(np.array(list("ACGT-")), size=int(mut.sum()))
+    cols = rng.choice(L, size=max(1, L // 10), replace=False)
+    third = len(cols) // 3
+    A[:, cols[:third]] = rng.choice(np.array(list("ACGT")), size=third)[None, :]       # all differ, one allele
+    sub = cols[third:2 * third]
+    A[:, sub] = rng.choice(np.array(list("ACGT")), size=len(sub))[None, :]
+    hit = rng.random((nseq, len(sub))) < 0.3
+    blk = A[:, sub]
+    blk[hit] = amb                                                                   # allele + ambiguous
+    A[:, sub] = blk
+    A[:, cols[2 * third:2 * third + 2]] = amb                                        # all ambiguous
+    amb_hit = rng.random((nseq, L)) < 0.01
+    A[amb_hit] = amb
+    names = ["s%d" % i for i in range(nseq)]
+    diffs = {}
+    for i, nm in enumerate(names):
+        pos = np.flatnonzero(A[i] != ref)
+        extra = np.flatnonzero((A[i] == ref) & (rng.random(L) < restate / 10))
+        pos = np.concatenate((pos, extra))
+        rng.shuffle(pos)
+        diffs[nm] = {int(p): str(A[i, p]) for p in pos}
+    return names, diffs, ref, A

@@ -178,3 +178,99 @@ def test_sequence_data_on_device_equals_host(gpu_engine):
         assert np.array_equal(h.compressed_codes, d.compressed_codes)
         for k in aln:
             assert np.array_equal(h.compressed_alignment[k], d.compressed_alignment[k])
+
+
+# ------------------------------------------------------ sparse alignments --
+def test_sparse_on_device_equals_host(gpu_engine):
+    """Sparse compression with the variable columns compressed on the device
+    equals the vectorised host path (itself equal to the oracle's position
+    loop on the CPU): bit-exact patterns, multiplicities and maps."""
+    from helpers import sparse_case
+
+    for seed, nseq, L in [(11, 7, 400), (12, 400, 40000)]:
+        say(f"[compress] sparse device vs host nseq={nseq} L={L}")
+        names, diffs, ref, A = sparse_case(seed, nseq, L)
+        if L <= 400:
+            comp, mult, f2c, c2f, nonref, inferred = O.compress_sparse_alignment(names, diffs, ref, "N")
+        host = SequenceData(diffs, ref="".join(ref), ambiguous="N")
+        dev = SequenceData(diffs, ref="".join(ref), ambiguous="N", engine=gpu_engine)
+        assert np.array_equal(dev.compressed_codes, host.compressed_codes)
+        assert np.array_equal(dev.multiplicity(), host.multiplicity())
+        assert np.array_equal(dev.full_to_compressed_sequence_map, host.full_to_compressed_sequence_map)
+        for p, v in host.compressed_to_full_sequence_map.items():
+            assert np.array_equal(dev.compressed_to_full_sequence_map[p], v)
+        assert np.array_equal(dev.nonref_positions, host.nonref_positions)
+        assert dev.inferred_const_sites == host.inferred_const_sites
+        if L <= 400:
+            assert np.array_equal(dev.multiplicity(), mult)
+            assert np.array_equal(dev.full_to_compressed_sequence_map, f2c)
+
+
+def _named_tree(parent, bl, tip_names):
+    from treetime_amd.tree import Clade, Tree
+
+    n = len(parent)
+    nodes = [Clade(branch_length=float(bl[i])) for i in range(n)]
+    k = 0
+    is_tip = np.ones(n, dtype=bool)
+    is_tip[parent[parent >= 0]] = False
+    for i in range(n):
+        if is_tip[i]:
+            nodes[i].name = tip_names[k]
+            k += 1
+        else:
+            nodes[i].name = "node_%d" % i
+        if parent[i] >= 0:
+            nodes[parent[i]].clades.append(nodes[i])
+    return Tree(root=nodes[int(np.flatnonzero(parent < 0)[0])])
+
+
+def test_sparse_treeanc_equals_dense(gpu_engine):
+    """TreeAnc on reference + differences against TreeAnc on the expanded
+    dense alignment: same log-likelihood (1e-7 relative), the same
+    reconstructed states at every position that belongs to a pattern, and
+    the sparse output format of get_reconstructed_alignment
+    (treeanc.py:1739-1746)."""
+    import treetime_amd as tt
+    from treetime_amd import synth
+
+    say("[compress] sparse TreeAnc vs dense")
+    job = synth.make_job(300, 3000, 0.05, 4, 21, p_ambig=0.01)
+    chars = job["chars"].astype("U1")
+    L = chars.shape[1]
+    ref = np.array([max("ACGT", key=lambda c: int((chars[:, j] == c).sum())) for j in range(L)])
+    ref[::97] = "N"                      # a few ambiguous reference positions
+    chars[:, 5::211] = "N"               # and all-ambiguous columns
+    names = ["t%d" % i for i in range(chars.shape[0])]
+    diffs = {nm: {int(p): str(chars[i, p]) for p in np.flatnonzero(chars[i] != ref)} for i, nm in enumerate(names)}
+    dense = {nm: "".join(chars[i]) for i, nm in enumerate(names)}
+    g = job["gtr"]
+    res = {}
+    for kind, aln, ref_arg in [("sparse", diffs, "".join(ref)), ("dense", dense, None)]:
+        tree = _named_tree(job["parent"], job["branch_length"], names)
+        gtr = tt.GTR.custom(alphabet="nuc_nogap", pi=g.Pi, W=g.W, mu=g.mu)
+        t = tt.TreeAnc(tree=tree, aln=aln, ref=ref_arg, gtr=gtr, engine=gpu_engine, verbose=0, fill_overhangs=False)
+        t.infer_ancestral_sequences(marginal=True)
+        res[kind] = t
+    ts, td = res["sparse"], res["dense"]
+    assert ts.data.is_sparse and not td.data.is_sparse
+    lh_s, lh_d = ts.sequence_LH(), td.sequence_LH()
+    say(f"  logL sparse {lh_s:.10f} dense {lh_d:.10f}")
+    assert abs(lh_s - lh_d) <= 1e-7 * abs(lh_d)
+    out = ts.get_reconstructed_alignment()
+    assert set(out) == {"sequences", "reference", "positions", "inferred_const_sites"}
+    assert np.array_equal(out["positions"], ts.data.nonref_positions)
+    keep = ~(chars == "N").all(axis=0)     # positions in some pattern
+    dn = {n.name: n for n in td.tree.find_clades()}
+    for n in ts.tree.find_clades():
+        if n.is_terminal():
+            continue
+        full = ref.copy()
+        for p, c in out["sequences"][n.name].items():
+            full[p] = c
+        want = td.sequence(dn[n.name], as_string=False)
+        assert np.array_equal(full[keep], want[keep]), n.name
+    # tip mutations name the tip's own character
+    for n in ts.tree.get_terminals()[:20]:
+        for a, p, b in n.mutations:
+            assert b == diffs[n.name].get(p, ref[p])

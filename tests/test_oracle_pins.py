@@ -300,6 +300,73 @@ def test_compression_additional_constant_sites():
     assert len(f2c) == 10
 
 
+SPARSE_REF = "ACGTNACGTANNN"
+SPARSE_DIFFS = {
+    "s1": {8: "A", 5: "G", 3: "N", 2: "T", 1: "N", 7: "N", 9: "N", 10: "A", 12: "N"},
+    "s2": {2: "T", 3: "A", 6: "N", 7: "N", 8: "C", 9: "G", 11: "-"},
+    "s3": {2: "T", 3: "A", 6: "T", 7: "N", 9: "N"},
+    "s4": {2: "T", 3: "A", 7: "N", 9: "G"},
+}
+
+
+def test_sparse_compression_rules():
+    """Hand-built reference + differences for each class of
+    vcf_utils.py:694-718 and each merge rule of sequence_data.py:378-413;
+    the expected values are worked out by hand:
+
+    0 no differences -> constant A      7 all N -> allele N, no pattern
+    1 N in one sequence -> constant C   8 A, C in two -> private ACTT
+    2 T everywhere -> constant T        9 G/N everywhere -> constant G
+    3 N,A,A,A -> constant A             10 ref N, A in one -> AAAA, merges
+    4 ref N, no differences -> none     11 ref N, '-' in one -> new ----
+    5 G in one -> private GAAA          12 N restated -> no pattern
+    6 N, T -> private CNTC (N kept)
+
+    Constant patterns come first in character order (A C G T), then the
+    variable positions in order of first mention (8, 5, 10, 6, 11)."""
+    names = list(SPARSE_DIFFS)
+    comp, mult, f2c, c2f, nonref, inferred = O.compress_sparse_alignment(
+        names, SPARSE_DIFFS, np.array(list(SPARSE_REF)), "N")
+    cols = ["".join(comp[nm][p] for nm in names) for p in range(len(mult))]
+    assert cols == ["AAAA", "CCCC", "GGGG", "TTTT", "ACTT", "GAAA", "CNTC", "----"]
+    assert mult.tolist() == [3, 1, 1, 1, 1, 1, 1, 1]
+    assert f2c.tolist() == [0, 1, 3, 0, 0, 5, 6, 0, 4, 2, 0, 7, 0]
+    assert {p: v.tolist() for p, v in c2f.items()} == {0: [0, 3, 10], 1: [1], 2: [9], 3: [2], 4: [8], 5: [5],
+                                                       6: [6], 7: [11]}
+    assert nonref.tolist() == [1, 2, 3, 5, 6, 7, 8, 9, 10, 11, 12]
+    assert inferred == [3, 1, 7, 9, 12]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_sparse_compression_equals_dense_of_expanded(seed):
+    """The sparse path and the (pinned) dense compression of the expanded
+    alignment put every position in a pattern with the same column and the
+    same set of positions, except the positions whose expanded column is
+    all ambiguous, which the sparse path leaves out of every pattern."""
+    from helpers import sparse_case
+
+    names, diffs, ref, A = sparse_case(seed, 7, 400)
+    comp, mult, f2c, c2f, nonref, inferred = O.compress_sparse_alignment(names, diffs, ref, "N")
+    dcomp, dmult, df2c, dc2f = O.compress_alignment(names, {nm: A[i] for i, nm in enumerate(names)}, "N")
+    dropped = (A == "N").all(axis=0)
+    assert dropped.sum() >= 2
+    covered = np.zeros(len(ref), dtype=int)
+    for p, plist in c2f.items():
+        covered[plist] += 1
+        q = df2c[plist[0]]
+        assert sorted(plist.tolist()) == sorted(dc2f[q].tolist())
+        assert "".join(comp[nm][p] for nm in names) == "".join(dcomp[nm][q] for nm in names)
+        assert mult[p] == dmult[q]
+    assert np.array_equal(covered, (~dropped).astype(int))
+    assert np.array_equal(f2c[dropped], np.zeros(dropped.sum(), dtype=int))
+    assert nonref.tolist() == sorted({p for d in diffs.values() for p in d})
+    # positions made constant by the ambiguous character: mentioned, not
+    # variable in the expanded alignment, and holding the ambiguous character
+    for pos in inferred:
+        col = A[:, pos]
+        assert (col == "N").any() and len(set(col) - {"N"}) <= 1
+
+
 # ----------------------------------------------------- branch statistics --
 def test_prob_t_profiles_one_hot_equals_compressed():
     m = rand_model(4, 21, "nuc_nogap")

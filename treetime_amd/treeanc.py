@@ -245,7 +245,8 @@ class TreeAnc:
                 if not self.ignore_missing_alns and failed > self.tree.count_terminals() / 3:
                     raise MissingDataError("TreeAnc._check_alignment_tree_gtr_consistency: At least 30\\% terminal "
                                            "nodes cannot be assigned a sequence!")
-        extend_profile(self.gtr, self.data.aln.values(), logger=self.logger)
+        extend_profile(self.gtr, [self.data.ref] if self.data.is_sparse else self.data.aln.values(),
+                       logger=self.logger)
         self.ok = True
         self._drop_device()
 
@@ -732,8 +733,16 @@ class TreeAnc:
         return self.data.compressed_to_full_sequence(seq, as_string=as_string)
 
     def get_reconstructed_alignment(self, reconstruct_tip_states=False):
-        """Full-length sequences of every node as a dict name -> str."""
+        """Full-length sequences of every node as a dict name -> str; for a
+        sparse alignment the reference's dict of 'sequences' (differences
+        from the reference per node), 'reference', 'positions' and
+        'inferred_const_sites' (treeanc.py:1716-1760)."""
         if not self.sequence_reconstruction:
             self.infer_ancestral_sequences("probabilistic", marginal=True, reconstruct_tip_states=reconstruct_tip_states)
+        if self.data.is_sparse:
+            return {"sequences": {n.name: self.data.compressed_to_sparse_sequence(n.cseq)
+                                  for n in self.tree.find_clades() if n.cseq is not None},
+                    "reference": self.data.ref, "positions": self.data.nonref_positions,
+                    "inferred_const_sites": self.data.inferred_const_sites}
         return {n.name: self.sequence(n, as_string=True) for n in self.tree.find_clades()
                 if n.cseq is not None}
