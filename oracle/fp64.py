@@ -30,6 +30,9 @@ oracle function             reference
 ``OracleTreeAnc.joint``     treetime/treeanc.py:934-1080 (root_rng: the
                             root draw of 1022-1037, seq_utils.py:257-269)
 ``ancestral_likelihood``    treetime/treeanc.py:718-750
+``OracleTreeAnc.optimize_tree`` treetime/treeanc.py:1384-1473 (joint mode),
+                            1176-1270 (optimize_joint_lengths), 1475-1495
+                            (prune_short_branches)
 ``branch_t``                treetime/treeanc.py:752-760, config.py:7
 ==========================  =================================================
 
@@ -59,6 +62,11 @@ Pinning (tests/test_oracle_*.py):
   the positions whose constant state is the ambiguous character, which the
   sparse path leaves out of every pattern.
 * prob_t_profiles: one-hot profiles reduce to prob_t_compressed.
+* optimize_tree: at its end the joint states are a fixed point (one more
+  reconstruction changes nothing), every branch sits at the optimum of its
+  end states (checked against a dense grid), no prunable branch is left,
+  each pruned node carried its parent's sequence, the tips are untouched
+  and the joint log-likelihood never decreased across the iterations.
 * optimal_t_compressed: JC69 closed form t = -3/4 ln(1 - 4/3 p).
 * SiteGTRModel: exact P(t) of every site = scipy.linalg.expm of the site's
   rate matrix; the interpolated P(t) equals the exact one on grid points and
@@ -830,6 +838,73 @@ class OracleTreeAnc:
             pidx = node.up.seq_idx
             node.seq_idx = node.joint_Cx[np.arange(self.L), pidx].astype(int)
             node.path_margin = np.minimum(node.up.path_margin, node.joint_margin[np.arange(self.L), pidx])
+        return self.total_joint_LH
+
+    # -- iterative joint reconstruction + branch lengths ------------------------
+    def _seq_chars(self, node):
+        if node.is_terminal():
+            return np.asarray(self.aln[node.name])
+        return self.gtr.alphabet[node.seq_idx]
+
+    def optimize_joint_lengths(self):
+        """treetime/treeanc.py:1176-1270: every branch set to the optimum of
+        the state pairs at its ends (gaps ignored), never below zero."""
+        for node in postorder(self.root):
+            if node.up is None:
+                continue
+            pairs, m = self.gtr.state_pair(self._seq_chars(node.up), self._seq_chars(node),
+                                           pattern_multiplicity=self.mult, ignore_gaps=True)
+            node.branch_length = max(0.0, self.gtr.optimal_t_compressed(pairs, m))
+
+    def prune_short_branches(self):
+        """treetime/treeanc.py:1475-1495: an internal branch shorter than a
+        tenth of one mutation with probability above 0.1 at length zero
+        (identical ends, gaps ignored) is removed, its children moved up."""
+        pruned = []
+        for node in preorder(self.root):
+            if node.up is None or node.is_terminal():
+                continue
+            if node.branch_length < 0.1 * self.one_mutation:
+                pairs, m = self.gtr.state_pair(self._seq_chars(node.up), self._seq_chars(node),
+                                               pattern_multiplicity=self.mult, ignore_gaps=True)
+                if self.gtr.prob_t_compressed(pairs, m, 0.0) > 0.1:
+                    node.up.clades = [k for k in node.up.clades if k is not node] + node.clades
+                    for c in node.clades:
+                        c.up = node.up
+                    pruned.append(node.name)
+        return pruned
+
+    def optimize_tree(self, max_iter=5, prune_short=True):
+        """treetime/treeanc.py:1384-1473, joint branch-length mode with
+        joint ML sequences: reconstruct, optimise the lengths, then repeat
+        (prune, reconstruct, stop when no internal character changed,
+        optimise) at most max_iter times.  Records the smallest joint
+        decision margin over all reconstructions (min_path_margin), the
+        number of iterations and the pruned nodes; returns the final joint
+        log-likelihood."""
+        def internal():
+            return [nd for nd in preorder(self.root) if nd.up is not None and not nd.is_terminal()]
+
+        def margin():
+            return min(float(np.min(nd.path_margin)) for nd in preorder(self.root) if not nd.is_terminal())
+
+        self.joint()
+        self.min_path_margin = margin()
+        self.optimize_joint_lengths()
+        self.pruned = []
+        n = 0
+        while n < max_iter:
+            n += 1
+            if prune_short:
+                self.pruned += self.prune_short_branches()
+            prev = {id(nd): nd.seq_idx.copy() for nd in internal()}
+            self.joint()
+            self.min_path_margin = min(self.min_path_margin, margin())
+            n_diff = sum(int((nd.seq_idx != prev[id(nd)]).sum()) for nd in internal())
+            if n_diff < 1:
+                break
+            self.optimize_joint_lengths()
+        self.n_iter = n
         return self.total_joint_LH
 
     # -- likelihood of one realisation ----------------------------------------

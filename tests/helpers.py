@@ -113,3 +113,19 @@ def sparse_case(seed, nseq, L, amb="N", restate=0.05):
         rng.shuffle(pos)
         diffs[nm] = {int(p): str(A[i, p]) for p in pos}
     return names, diffs, ref, A
+
+
+def optimize_tree_case(seed, n_tips=40, L=300, depth=0.02, scale=1.7):
+    """Oracle tree + compressed alignment of a synthetic job, branch lengths
+    scaled away from their optimum."""
+    from treetime_amd import synth
+
+    job = synth.make_job(n_tips, L, depth, 4, seed, 0.01, 0.0)
+    names = ["n%d" % i for i in range(len(job["parent"]))]
+    root, nodes = O.build_tree(job["parent"], job["branch_length"] * scale, names)
+    tip_names = [names[v] for v in job["tips"]]
+    chars = job["chars"].astype("U1")
+    comp, mult, _, _ = O.compress_alignment(tip_names, {nm: chars[k] for k, nm in enumerate(tip_names)}, "N")
+    g = job["gtr"]
+    og = O.GTRModel("nuc_nogap", mu=g.mu, pi=g.Pi, W=g.W)
+    return O.OracleTreeAnc(root, og, comp, mult, full_length=L), job, names

@@ -287,3 +287,51 @@ def test_facade_optimize_tree_marginal():
     lh1 = ta.sequence_LH()
     say(f"[branch-opt marginal] facade: logL {lh0:.3f} -> {lh1:.3f}")
     assert lh1 >= lh0 - 1e-6
+
+
+@pytest.mark.parametrize("seed,n_tips,L,depth,scale", [(6, 60, 400, 0.2, 0.3), (3, 60, 400, 0.1, 3.0),
+                                                       (3, 40, 300, 0.02, 1.7)])
+def test_facade_optimize_tree_matches_oracle(gpu_engine, seed, n_tips, L, depth, scale):
+    """TreeAnc.optimize_tree (treeanc.py:1384-1495: joint reconstruction,
+    joint branch lengths, pruning of zero-length internal branches, until no
+    character changes) against the oracle's iteration: the same pruned
+    nodes, the same topology, every branch length to the optimum tolerance
+    and the final joint log-likelihood to 1e-7 relative."""
+    import treetime_amd as tt
+    from helpers import optimize_tree_case
+    from oracle import fp64 as O
+
+    say(f"[optimize_tree] seed={seed} tips={n_tips} L={L} depth={depth} scale={scale}")
+    orc, job, names = optimize_tree_case(seed, n_tips, L, depth, scale)
+    parent, bl = job["parent"], job["branch_length"] * scale
+    clades = [tt.Clade(name=names[i], branch_length=float(bl[i])) for i in range(len(parent))]
+    for i, p in enumerate(parent):
+        if p >= 0:
+            clades[p].clades.append(clades[i])
+    chars = job["chars"].astype("U1")
+    aln = {names[v]: "".join(chars[k]) for k, v in enumerate(job["tips"])}
+    g = job["gtr"]
+    ta = tt.TreeAnc(tree=tt.Tree(clades[int(np.flatnonzero(parent < 0)[0])]), aln=aln, verbose=0,
+                    gtr=tt.GTR.custom(alphabet="nuc_nogap", pi=g.Pi, W=g.W, mu=g.mu), engine=gpu_engine,
+                    keep_node_order=True)
+    assert ta.optimize_tree(max_iter=5) == "success"
+    total = orc.optimize_tree(max_iter=5)
+    assert orc.min_path_margin > 1e-5        # no joint decision of the oracle's run is a near tie
+    say(f"  oracle: {orc.n_iter} iterations, {len(orc.pruned)} pruned, logL {total:.6f}; "
+        f"device logL {ta.tree.unconstrained_sequence_LH:.6f}")
+    o_nodes = {nd.name: nd for nd in O.preorder(orc.root)}
+    f_nodes = {nd.name: nd for nd in ta.tree.find_clades()}
+    assert set(o_nodes) == set(f_nodes)
+    worst = 0.0
+    for name, nd in f_nodes.items():
+        on = o_nodes[name]
+        assert (nd.up.name if nd.up is not None else None) == (on.up.name if on.up is not None else None), name
+        assert [c.name for c in nd.clades] == [c.name for c in on.clades], name
+        if nd.up is None:
+            continue
+        t_ref = on.branch_length
+        assert abs(nd.branch_length - t_ref) <= T_RTOL * t_ref + T_ATOL, (name, nd.branch_length, t_ref)
+        if t_ref > 1e-6:
+            worst = max(worst, abs(nd.branch_length - t_ref) / t_ref)
+    say(f"  max rel diff of branch lengths {worst:.2e} (t > 1e-6)")
+    assert abs(ta.tree.unconstrained_sequence_LH - total) <= 1e-7 * abs(total)

@@ -367,6 +367,46 @@ def test_sparse_compression_equals_dense_of_expanded(seed):
         assert (col == "N").any() and len(set(col) - {"N"}) <= 1
 
 
+@pytest.mark.parametrize("seed", [3, 4])
+def test_optimize_tree_fixed_point(seed):
+    """The iteration of treeanc.py:1384-1495 ends at a fixed point: joint
+    states unchanged by one more reconstruction, every branch at the
+    optimum of its end states (no better length on a grid around it), no
+    prunable branch left, pruned nodes carried their parent's states, the
+    tips untouched, and the joint log-likelihood never decreased."""
+    from helpers import optimize_tree_case
+
+    ta, job, names = optimize_tree_case(seed)
+    tips0 = sorted(nd.name for nd in O.preorder(ta.root) if nd.is_terminal())
+    n_int0 = sum(1 for nd in O.preorder(ta.root) if not nd.is_terminal())
+    history = []
+    orig = ta.joint
+    ta.joint = lambda *a, **k: history.append(orig(*a, **k)) or history[-1]
+    total = ta.optimize_tree(max_iter=10)
+    ta.joint = orig
+    assert ta.n_iter < 10 and len(ta.pruned) > 0
+    assert all(b >= a - 1e-9 * abs(a) for a, b in zip(history, history[1:])), history
+    assert sorted(nd.name for nd in O.preorder(ta.root) if nd.is_terminal()) == tips0
+    assert sum(1 for nd in O.preorder(ta.root) if not nd.is_terminal()) == n_int0 - len(ta.pruned)
+    states = {nd.name: nd.seq_idx.copy() for nd in O.preorder(ta.root) if not nd.is_terminal()}
+    assert ta.joint() == total
+    for nd in O.preorder(ta.root):
+        if not nd.is_terminal():
+            assert np.array_equal(nd.seq_idx, states[nd.name])
+    og = ta.gtr
+    for nd in O.preorder(ta.root):
+        if nd.up is None:
+            continue
+        pairs, m = og.state_pair(ta._seq_chars(nd.up), ta._seq_chars(nd), pattern_multiplicity=ta.mult,
+                                 ignore_gaps=True)
+        f0 = og.prob_t_compressed(pairs, m, nd.branch_length, return_log=True)
+        for t in [0.0, 1e-6] + [nd.branch_length * r for r in (0.5, 0.9, 0.99, 1.01, 1.1, 2.0)]:
+            assert og.prob_t_compressed(pairs, m, t, return_log=True) <= f0 + 1e-9 * abs(f0), (nd.name, t)
+        if not nd.is_terminal() and nd.branch_length < 0.1 * ta.one_mutation:
+            assert og.prob_t_compressed(pairs, m, 0.0) <= 0.1, nd.name
+    assert ta.prune_short_branches() == []
+
+
 # ----------------------------------------------------- branch statistics --
 def test_prob_t_profiles_one_hot_equals_compressed():
     m = rand_model(4, 21, "nuc_nogap")

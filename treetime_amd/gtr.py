@@ -359,6 +359,19 @@ class GTR:
         return res if return_log else np.exp(res)
 
     # --- fixed-sequence branch statistics (host) -----------------------------
+    def _state_indices(self, seq, default):
+        """Alphabet index of every character (``default`` for the others)."""
+        if seq.dtype.kind == "U" and seq.dtype.itemsize == 4 and all(len(str(a)) == 1 for a in self.alphabet):
+            codes = seq.view(np.uint32).reshape(seq.shape)
+            if codes.size == 0 or codes.max() < 256:
+                table = np.full(256, default, dtype=int)
+                for i, a in enumerate(self.alphabet):
+                    if ord(str(a)) < 256:
+                        table[ord(str(a))] = i
+                return table[codes]
+        lookup = {c: i for i, c in enumerate(self.alphabet)}
+        return np.array([lookup.get(c, default) for c in seq.tolist()], dtype=int)
+
     def state_pair(self, seq_p, seq_ch, pattern_multiplicity=None, ignore_gaps=False):
         """Multiplicity-weighted (parent, child) state-pair counts, cast to int
         as the reference does (gtr.py:631-712)."""
@@ -368,10 +381,8 @@ class GTR:
             raise ValueError("GTR.state_pair: Sequence lengths do not match!")
         w = np.ones(len(seq_p)) if pattern_multiplicity is None else np.asarray(pattern_multiplicity, dtype=float)
         n = self.n_states
-        lookup = {c: i for i, c in enumerate(self.alphabet)}
         default = 1 if n >= 10 else -1
-        ip = np.array([lookup.get(c, default) for c in seq_p.tolist()], dtype=int)
-        ic = np.array([lookup.get(c, default) for c in seq_ch.tolist()], dtype=int)
+        ip, ic = self._state_indices(seq_p, default), self._state_indices(seq_ch, default)
         keep = (ip >= 0) & (ic >= 0)
         if ignore_gaps and self.gap_index is not None:
             keep &= (ip != self.gap_index) & (ic != self.gap_index)

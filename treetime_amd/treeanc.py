@@ -691,6 +691,75 @@ class TreeAnc:
                 break
         return ttconf.SUCCESS
 
+    def optimize_sequences_and_branch_length(self, *args, **kwargs):
+        """Deprecated alias of optimize_tree (treeanc.py:1362-1371)."""
+        self.logger("Deprecation warning: 'optimize_sequences_and_branch_length' will be removed and replaced by "
+                    "'optimize_tree'!", 1, warn=True)
+        return self.optimize_tree(*args, **kwargs)
+
+    def optimize_seq_and_branch_len(self, *args, **kwargs):
+        """Deprecated alias of optimize_tree (treeanc.py:1373-1382)."""
+        self.logger("Deprecation warning: 'optimize_seq_and_branch_len' will be removed and replaced by "
+                    "'optimize_tree'!", 1, warn=True)
+        return self.optimize_tree(*args, **kwargs)
+
+    def optimize_tree(self, prune_short=True, marginal_sequences=False, branch_length_mode="joint", max_iter=5,
+                      infer_gtr=False, pc=1.0, method_anc="probabilistic", **kwargs):
+        """treeanc.py:1384-1473: alternate ancestral reconstruction and
+        joint branch-length optimisation (each a device pass) until no
+        character changes, pruning zero-length internal branches between
+        iterations; 'marginal' runs optimize_tree_marginal, 'input' only
+        reconstructs."""
+        if branch_length_mode == "marginal":
+            self.optimize_tree_marginal(max_iter=max_iter, infer_gtr=infer_gtr, pc=pc, **kwargs)
+            if prune_short:
+                self.prune_short_branches()
+            return ttconf.SUCCESS
+        if branch_length_mode == "input":
+            self.reconstruct_anc(method=method_anc, infer_gtr=infer_gtr, pc=pc, marginal=marginal_sequences,
+                                 **kwargs)
+            if prune_short:
+                self.prune_short_branches()
+            return ttconf.SUCCESS
+        if branch_length_mode != "joint":
+            raise UnknownMethodError("TreeAnc.optimize_tree: `branch_length_mode` should be in ['marginal', "
+                                     "'joint', 'input']")
+        self.logger("TreeAnc.optimize_tree: sequences...", 1)
+        self.reconstruct_anc(method=method_anc, infer_gtr=infer_gtr, pc=pc, marginal=marginal_sequences, **kwargs)
+        self.optimize_branch_lengths_joint(store_old=False)
+        n = 0
+        while n < max_iter:
+            n += 1
+            if prune_short:
+                self.prune_short_branches()
+            n_diff = self.reconstruct_anc(method=method_anc, infer_gtr=False, marginal=marginal_sequences,
+                                          **kwargs)
+            self.logger("TreeAnc.optimize_tree: Iteration %d. #Nuc changed since prev reconstructions: %d"
+                        % (n, n_diff), 2)
+            if n_diff < 1:
+                break
+            self.optimize_branch_lengths_joint(store_old=False)
+        self.tree.unconstrained_sequence_LH = float((self.tree.sequence_LH * self.data.multiplicity()).sum())
+        self._prepare_nodes()
+        self.logger("TreeAnc.optimize_tree: Unconstrained sequence LH:%f" % self.tree.unconstrained_sequence_LH, 2)
+        return ttconf.SUCCESS
+
+    def prune_short_branches(self):
+        """treeanc.py:1475-1495: an internal branch shorter than a tenth of
+        one mutation whose two ends carry the same sequence (probability
+        above 0.1 at length zero, gaps ignored) is removed; its children
+        move to its parent."""
+        self.logger("TreeAnc.prune_short_branches: pruning short branches (max prob at zero)...", 1)
+        mult = self.data.multiplicity()
+        for node in self.tree.find_clades():
+            if node.up is None or node.is_terminal():
+                continue
+            if node.branch_length < 0.1 * self.one_mutation and \
+                    self.gtr.prob_t(node.up._cseq, node._cseq, 0.0, pattern_multiplicity=mult) > 0.1:
+                node.up.clades = [k for k in node.up.clades if k is not node] + node.clades
+                for clade in node.clades:
+                    clade.up = node.up
+
     def marginal_branch_profile(self, node):
         """treeanc.py:1122-1146: (marginal_outgroup_LH, marginal_subtree_LH)
         of the branch above ``node``."""
