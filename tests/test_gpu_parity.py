@@ -514,21 +514,28 @@ def test_joint_logL_is_score_of_decoded_states(gpu_engine):
     dt.close()
 
 
-ONE_LAUNCH_CASES = [c for c in CASES if c[3] in (4, 5)]
+# trees the one-launch pass takes (large trees stay on the level kernels)
+ONE_LAUNCH_CASES = [c for c in CASES if c[3] in (4, 5) and c[1] <= 1000]
 
 
 @pytest.mark.parametrize("case", ONE_LAUNCH_CASES, ids=[c[0] for c in ONE_LAUNCH_CASES])
-def test_one_launch_pass_equals_level_kernels(gpu_engine, case, monkeypatch):
-    """The one-launch pass with LDS-resident rows (TT_TILE=1) gives the level
-    kernels' profiles, subtree rows and states bit for bit, and their logL
-    to rounding, with and without reconstructed tips and for the
+@pytest.mark.parametrize("stage", ["1", "0"], ids=["staged", "streamed"])
+def test_one_launch_pass_equals_level_kernels(gpu_engine, case, stage, monkeypatch):
+    """The one-launch pass with LDS-resident rows (TT_TILE=1), with its
+    records staged in LDS or streamed from memory, gives the level kernels'
+    profiles, subtree rows and states bit for bit, and their logL to
+    rounding, with and without reconstructed tips and for the
     log-likelihood pass."""
-    say(f"[one-launch] case={case[0]}")
+    say(f"[one-launch] case={case[0]} stage={stage}")
     job, prep, alphabet = make_case(*case)
     monkeypatch.setenv("TT_TILE", "0")
     lv = device_tree(gpu_engine, prep)
     monkeypatch.setenv("TT_TILE", "1")
+    monkeypatch.setenv("TT_TILE_STAGE", stage)
     one = device_tree(gpu_engine, prep)
+    assert lv.pass_shape() == (0, False)
+    tp, staged = one.pass_shape()
+    assert tp > 0 and staged == (stage == "1"), (tp, staged)
     for tips in (False, True):
         a, b = lv.marginal(reconstruct_tips=tips), one.marginal(reconstruct_tips=tips)
         assert abs(a - b) <= 1e-12 * abs(a), (tips, a, b)

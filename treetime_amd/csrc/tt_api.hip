@@ -175,6 +175,7 @@ struct tt_tree {
     double *d_tile_partial = nullptr;
     unsigned *d_tile_counter = nullptr;
     bool tile_stale = true;  // the records do not hold the current P(t) yet
+    bool tile_staged = false;  // the kernel stages each phase's records in LDS
     bool have_states = false;  // d.states holds a marginal or joint reconstruction
     bool tips_dirty = false;   // tip rows of d.states hold reconstructed states (else 255)
     bool profiles_fresh = false;  // d.prof and d.sub belong to the current lengths (last marginal pass)
@@ -478,7 +479,8 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
     if ((mode == 1 || mode == 3) && T->tile_tp && !T->site) {
         if (mode == 1 && fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
         TT_HIP(ttk::tile_marginal(D, T->d_tile_post_rec, T->d_tile_pre_rec, T->d_tile_rounds, T->tile_Rp,
-                                  T->tile_Ri, T->tile_Ra, T->tile_tp, T->d_tile_partial, T->d_tile_counter,
+                                  T->tile_Ri, T->tile_Ra, T->tile_tp, (int)T->post_nodes.size(),
+                                  (int)T->pre_nodes.size(), T->tile_staged, T->d_tile_partial, T->d_tile_counter,
                                   tips ? 1 : 0, mode == 1 ? 1 : 0, s));
     } else if (mode == 1 || mode == 3) {
         size_t ri = 0;
@@ -844,28 +846,47 @@ int32_t setup_tile(tt_tree *T) {
     TT_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     // dynamic LDS up to the device's per-workgroup limit needs no opt-in here
     const size_t budget = (size_t)std::max(0, lds_max - 2048);  // the kernel's static LDS
-    // the smallest tile whose workgroups are all resident at once: a round
-    // costs about one wave's arithmetic for one (node, pattern) step, the
-    // level kernels about a launch per level, so one wave of workgroups wins
-    // and a second one (the chain of rounds run twice) no longer does.
-    // TT_TILE=1 also takes two waves of workgroups; TT_TILE=0 turns the path off.
+    // Tile choice by an estimate of the pass time: every round (a level, or
+    // a chunk of a wide one) costs about a microsecond of one wave's latency
+    // chain, a quarter more when its records stream from memory; staging the
+    // records costs every workgroup a copy of them through L2 (about 2 TB/s
+    // for the whole device); a second wave of workgroups runs the rounds
+    // twice.  Measured on the 200-tip tree: 8 patterns per workgroup with
+    // staged records beat 2 (staging traffic) and 32 (rounds).
+    // TT_TILE=1 also allows two waves of workgroups; TT_TILE=0 turns the path
+    // off; TT_TILE_STAGE=0 never stages the records; TT_TILE_TP fixes the tile.
     const int kTileMaxWaves = forced ? 2 : 1;
+    const char *se = getenv("TT_TILE_STAGE");
+    const bool may_stage = !(se && se[0] == '0');
+    const char *te = getenv("TT_TILE_TP");
+    const int only_c = te ? atoi(te) : 0;
+    const size_t stage_words = ttk::tile_stage_words(T->n, (int)T->post_nodes.size(), (int)T->pre_nodes.size());
     std::vector<int32_t> rounds;
     int Rp = 0, Ri = 0, Ra = 0, tp = 0;
-    long best_waves = kTileMaxWaves + 1;
+    bool staged = false;
+    double best_us = 1e300;
     for (int c = 1; c <= 64; c *= 2) {
+        if (only_c > 0 && c != only_c) continue;
         tile_rounds(T, ttk::tile_threads() / c, rounds, Rp, Ri, Ra);
-        const size_t lds = ttk::tile_lds_bytes(T->d, c, (int)rounds.size());
-        if (lds > budget) break;
-        const int bpc = ttk::tile_blocks_per_cu(T->n, lds);
-        if (bpc < 1) break;
-        const long blocks = (T->P + c - 1) / c;
-        const long waves = (blocks + (long)bpc * n_cu - 1) / ((long)bpc * n_cu);
-        if (waves < best_waves) {
-            best_waves = waves;
-            tp = c;
+        bool fits = false;
+        for (int st = may_stage ? 1 : 0; st >= 0; --st) {
+            const size_t lds = ttk::tile_lds_bytes(T->d, c, (int)rounds.size(), st ? stage_words : 0);
+            if (lds > budget) continue;
+            const int bpc = ttk::tile_blocks_per_cu(T->n, lds, st != 0);
+            if (bpc < 1) continue;
+            fits = true;
+            const long blocks = (T->P + c - 1) / c;
+            const long waves = (blocks + (long)bpc * n_cu - 1) / ((long)bpc * n_cu);
+            if (waves > kTileMaxWaves) continue;
+            const double us = (double)waves * (Rp + Ri) * (st ? 1.0 : 1.25) +
+                              (st ? (double)blocks * stage_words * 4.0 / 2.0e6 : 0.0);
+            if (us < best_us) {
+                best_us = us;
+                tp = c;
+                staged = st != 0;
+            }
         }
-        if (waves == 1) break;
+        if (!fits) break;  // no larger tile fits either
     }
     if (!tp) return TT_OK;
     const int blocks = (T->P + tp - 1) / tp;
@@ -884,6 +905,7 @@ int32_t setup_tile(tt_tree *T) {
     T->tile_Ri = Ri;
     T->tile_Ra = Ra;
     T->tile_tp = tp;
+    T->tile_staged = staged;
     T->tile_stale = true;
     return TT_OK;
 }
@@ -1087,6 +1109,14 @@ int32_t tt_tree_info(const tt_tree *T, int32_t *n_levels_post, int32_t *n_levels
     if (n_levels_post) *n_levels_post = (int32_t)T->post_lv.size();
     if (n_levels_pre) *n_levels_pre = (int32_t)T->pre_lv.size();
     if (device_bytes) *device_bytes = T->bytes;
+    return TT_OK;
+}
+
+int32_t tt_tree_pass_shape(const tt_tree *T, int32_t *tile_patterns, int32_t *records_staged) {
+    if (!T) return fail(TT_EINVAL, "tree is NULL");
+    const bool tile = T->tile_tp && !T->site;
+    if (tile_patterns) *tile_patterns = tile ? T->tile_tp : 0;
+    if (records_staged) *records_staged = tile && T->tile_staged ? 1 : 0;
     return TT_OK;
 }
 

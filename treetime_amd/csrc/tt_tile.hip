@@ -13,9 +13,15 @@
 //
 // What a thread needs from memory for one (node, pattern) step -- the
 // node's descriptor and its branches' P(t) or P - I -- is packed into one
-// record per level entry (k_tile_pack, refreshed whenever P(t) changes), and
-// the record of the thread's next round is loaded while the current round
-// computes, so a round costs LDS traffic, arithmetic and a barrier.
+// record per level entry (k_tile_pack, refreshed whenever P(t) changes).
+// When they fit, a phase's records are staged in LDS before its first round
+// (the postorder's at the start, the preorder's during the root step), so
+// the rounds issue no global load at all: on this architecture loads and
+// stores share one counter that completes in order, and a load waited on in
+// every round would also wait for the previous round's result stores to be
+// acknowledged.  Otherwise the record of the thread's next round is loaded
+// while the current round computes.  Either way a round costs LDS traffic,
+// arithmetic and a barrier.
 //
 // Rows are written to HBM as well (subtree rows, profiles, states: the
 // results), but never read back.  The per-(node, pattern) arithmetic is the
@@ -48,6 +54,7 @@ struct TileDev {
     int32_t Rp, Ri, Ra;
     int32_t TP;                 // patterns per workgroup (power of two)
     int32_t n_blocks;
+    int32_t n_post, n_pre;      // records of each phase (staged into LDS when the kernel is built STAGED)
     double *partial;            // [n_blocks] sum over the workgroup's patterns of mult * logL
     unsigned *counter;          // workgroups finished; reset to 0 by the last one
 };
@@ -93,18 +100,27 @@ __device__ __forceinline__ void load_rec(const int32_t *base, PreRec<N> &r) {
     for (int i = 0; i < N * N; ++i) r.M[i] = f[i];
 }
 
-template <int N>
+// Copy words [0, n_words) of a record array into LDS (16-byte pieces).
+__device__ __forceinline__ void stage_records(int32_t *dst, const int32_t *src, size_t n_words) {
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+    for (size_t i = threadIdx.x; i < n_words / 4; i += kTileThreads) d4[i] = s4[i];
+}
+
+template <int N, bool STAGED>
 __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileDev S, int tips, int pre) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     const int TP = S.TP;
     // LDS layout (tile_lds_bytes): rows, profile map, reduction, round
-    // offsets, tip codes; each part starts on a 16-byte boundary
+    // offsets, tip codes, staged records; each part starts on a 16-byte
+    // boundary
     float *rows = reinterpret_cast<float *>(s_dyn);                                      // [n_int][TP][N]
     float *s_lut = rows + align4((size_t)T.n_int * TP * N);                              // [n_codes][N]
     double *s_red = reinterpret_cast<double *>(s_lut + align4((size_t)T.n_codes * N));  // [kTileThreads]
     int *s_rounds = reinterpret_cast<int *>(s_red + kTileThreads);
     const int n_rounds = 2 * (S.Rp + S.Ri + S.Ra);
     uint8_t *s_tip = reinterpret_cast<uint8_t *>(s_rounds + align4(n_rounds));  // [n_tips][TP]
+    int32_t *s_rec = reinterpret_cast<int32_t *>(s_tip + (((size_t)T.n_tips * TP + 15) & ~(size_t)15));
     __shared__ double s_sum[TT_WAVE];
     __shared__ bool s_last;
 
@@ -121,64 +137,85 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
         const int r = i / TP, pp = blockIdx.x * TP + (i & (TP - 1));
         s_tip[i] = pp < T.P ? T.tip_codes[(size_t)r * T.P + pp] : 0;
     }
-    __syncthreads();
+    if constexpr (STAGED) stage_records(s_rec, S.post_rec, (size_t)S.n_post * post_rec_words(N));
+    lds_barrier();  // the staging loads have landed in LDS (each thread waited for its own)
 
     auto slot = [&](int row) { return rows + ((size_t)row * TP + q) * N; };
 
     // ---- postorder: rounds over height-ordered internal rows ---------------------
     LogAcc acc;
+    // a, b: the record's descriptor words; D0, D1: the children's P - I
+    // (registers, or the staged record in LDS)
+    auto post_entry = [&](const int4 &a, const int4 &b, auto D0, auto D1) {
+        const int v = a.x, nchild = a.z, cref0 = a.w, cref1 = b.x;
+        if (nchild == 2) {
+            BinRaw<N> R;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int cref = c ? cref1 : cref0;
+                if (cref < 0) {
+                    R.code[c] = s_tip[(-cref - 1) * TP + q];
+                } else {
+                    R.code[c] = 0;
+                    const float *src = slot(cref);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) R.row[c][i] = src[i];
+                }
+            }
+            bin_finish_m<N>(T, v, cref0, cref1, D0, D1, p, R, s_lut, acc, slot(v));
+        } else {  // polytomies and unary nodes read their child lists from memory
+            auto rowf = [&](int cref, float (&row)[N]) {
+                if (cref < 0) {
+                    const int code = s_tip[(-cref - 1) * TP + q];
+#pragma unroll
+                    for (int i = 0; i < N; ++i) row[i] = s_lut[code * N + i];
+                } else {
+                    const float *src = slot(cref);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) row[i] = src[i];
+                }
+            };
+            post_node_f<N>(T, v, p, s_lut, acc, rowf, [](int e) { return e; }, slot(v));
+        }
+    };
     {
         constexpr int W = post_rec_words(N);
         const int *rd = s_rounds;
-        PostRec<N> cur, nxt;
-        int k = S.Rp > 0 ? rd[0] + k0 : 0;
-        if (S.Rp > 0 && k < rd[1]) load_rec<N>(S.post_rec + (size_t)W * k, cur);
-        for (int r = 0; r < S.Rp; ++r) {
-            const bool valid = k < rd[2 * r + 1];
-            const int kn = r + 1 < S.Rp ? rd[2 * r + 2] + k0 : 0;
-            const bool vnext = r + 1 < S.Rp && kn < rd[2 * r + 3];
-            if (vnext) load_rec<N>(S.post_rec + (size_t)W * kn, nxt);  // the next round's record, in flight
-            if (valid && act) {
-                const int v = cur.a.x, nchild = cur.a.z, cref0 = cur.a.w, cref1 = cur.b.x;
-                if (nchild == 2) {
-                    BinRaw<N> R;
-#pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        const int cref = c ? cref1 : cref0;
-                        if (cref < 0) {
-                            R.code[c] = s_tip[(-cref - 1) * TP + q];
-                        } else {
-                            R.code[c] = 0;
-                            const float *src = slot(cref);
-#pragma unroll
-                            for (int i = 0; i < N; ++i) R.row[c][i] = src[i];
-                        }
-                    }
-                    bin_finish_m<N>(T, v, cref0, cref1, cur.D0, cur.D1, p, R, s_lut, acc, slot(v));
-                } else {  // polytomies and unary nodes read their child lists from memory
-                    auto rowf = [&](int cref, float (&row)[N]) {
-                        if (cref < 0) {
-                            const int code = s_tip[(-cref - 1) * TP + q];
-#pragma unroll
-                            for (int i = 0; i < N; ++i) row[i] = s_lut[code * N + i];
-                        } else {
-                            const float *src = slot(cref);
-#pragma unroll
-                            for (int i = 0; i < N; ++i) row[i] = src[i];
-                        }
-                    };
-                    post_node_f<N>(T, v, p, s_lut, acc, rowf, [](int e) { return e; }, slot(v));
+        if constexpr (STAGED) {
+            for (int r = 0; r < S.Rp; ++r) {
+                const int k = rd[2 * r] + k0;
+                if (k < rd[2 * r + 1] && act) {
+                    const int32_t *rec = s_rec + (size_t)W * k;
+                    const float *D = reinterpret_cast<const float *>(rec + 8);
+                    post_entry(reinterpret_cast<const int4 *>(rec)[0], reinterpret_cast<const int4 *>(rec)[1], D,
+                               D + N * N);
                 }
+                lds_barrier();
             }
-            lds_barrier();
-            if (vnext) cur = nxt;
-            k = kn;
+        } else {
+            PostRec<N> cur, nxt;
+            int k = S.Rp > 0 ? rd[0] + k0 : 0;
+            if (S.Rp > 0 && k < rd[1]) load_rec<N>(S.post_rec + (size_t)W * k, cur);
+            for (int r = 0; r < S.Rp; ++r) {
+                const bool valid = k < rd[2 * r + 1];
+                const int kn = r + 1 < S.Rp ? rd[2 * r + 2] + k0 : 0;
+                const bool vnext = r + 1 < S.Rp && kn < rd[2 * r + 3];
+                if (vnext) load_rec<N>(S.post_rec + (size_t)W * kn, nxt);  // the next round's record, in flight
+                if (valid && act) post_entry(cur.a, cur.b, cur.D0, cur.D1);
+                lds_barrier();
+                if (vnext) cur = nxt;
+                k = kn;
+            }
         }
+    }
+    // the preorder's records replace the postorder's while the root step runs
+    if constexpr (STAGED) {
+        if (pre) stage_records(s_rec, S.pre_rec, (size_t)S.n_pre * pre_rec_words(N));
     }
 
     // ---- root (treeanc.py:814-829) ------------------------------------------
     s_red[t] = act ? acc.log_value() : 0.0;
-    __syncthreads();
+    lds_barrier();
     if (t < TP && act) {
         double off = 0.0;
         for (int k = q; k < kTileThreads; k += TP) off += s_red[k];
@@ -211,40 +248,52 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
         }
         s_red[t] = T.mult[p] * lp;
     }
-    __syncthreads();
+    lds_barrier();
 
     // ---- preorder: rounds over depth-ordered nodes -------------------------------
     if (pre) {
         constexpr int W = pre_rec_words(N);
         const int *rd = s_rounds + 2 * (tips ? S.Rp + S.Ri : S.Rp);
         const int R = tips ? S.Ra : S.Ri;
-        PreRec<N> cur, nxt;
-        int k = R > 0 ? rd[0] + k0 : 0;
-        if (R > 0 && k < rd[1]) load_rec<N>(S.pre_rec + (size_t)W * k, cur);
-        for (int r = 0; r < R; ++r) {
-            const bool valid = k < rd[2 * r + 1];
-            const int kn = r + 1 < R ? rd[2 * r + 2] + k0 : 0;
-            const bool vnext = r + 1 < R && kn < rd[2 * r + 3];
-            if (vnext) load_rec<N>(S.pre_rec + (size_t)W * kn, nxt);
-            if (valid && act) {
-                const int c = cur.a.x, prow = cur.a.y, ir = cur.a.z, tr = cur.a.w;
-                PreRaw<N> Rw;
-                const float *pp = slot(prow);
+        auto pre_entry = [&](const int4 &a, auto M) {
+            const int c = a.x, prow = a.y, ir = a.z, tr = a.w;
+            PreRaw<N> Rw;
+            const float *pp = slot(prow);
 #pragma unroll
-                for (int i = 0; i < N; ++i) Rw.pp[i] = pp[i];
-                if (ir >= 0) {
-                    Rw.code = 0;
-                    const float *src = slot(ir);
+            for (int i = 0; i < N; ++i) Rw.pp[i] = pp[i];
+            if (ir >= 0) {
+                Rw.code = 0;
+                const float *src = slot(ir);
 #pragma unroll
-                    for (int i = 0; i < N; ++i) Rw.sub[i] = src[i];
-                } else {
-                    Rw.code = s_tip[tr * TP + q];
-                }
-                pre_finish_m<N>(T, c, ir, p, Rw, s_lut, cur.M, ir >= 0 ? slot(ir) : nullptr);
+                for (int i = 0; i < N; ++i) Rw.sub[i] = src[i];
+            } else {
+                Rw.code = s_tip[tr * TP + q];
             }
-            lds_barrier();
-            if (vnext) cur = nxt;
-            k = kn;
+            pre_finish_m<N>(T, c, ir, p, Rw, s_lut, M, ir >= 0 ? slot(ir) : nullptr);
+        };
+        if constexpr (STAGED) {
+            for (int r = 0; r < R; ++r) {
+                const int k = rd[2 * r] + k0;
+                if (k < rd[2 * r + 1] && act) {
+                    const int32_t *rec = s_rec + (size_t)W * k;
+                    pre_entry(reinterpret_cast<const int4 *>(rec)[0], reinterpret_cast<const float *>(rec + 4));
+                }
+                lds_barrier();
+            }
+        } else {
+            PreRec<N> cur, nxt;
+            int k = R > 0 ? rd[0] + k0 : 0;
+            if (R > 0 && k < rd[1]) load_rec<N>(S.pre_rec + (size_t)W * k, cur);
+            for (int r = 0; r < R; ++r) {
+                const bool valid = k < rd[2 * r + 1];
+                const int kn = r + 1 < R ? rd[2 * r + 2] + k0 : 0;
+                const bool vnext = r + 1 < R && kn < rd[2 * r + 3];
+                if (vnext) load_rec<N>(S.pre_rec + (size_t)W * kn, nxt);
+                if (valid && act) pre_entry(cur.a, cur.M);
+                lds_barrier();
+                if (vnext) cur = nxt;
+                k = kn;
+            }
         }
     }
 
@@ -254,7 +303,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
         for (int k = t; k < TP; k += TT_WAVE) a += (blockIdx.x * TP + k < T.P) ? s_red[k] : 0.0;
         s_sum[t] = a;
     }
-    __syncthreads();
+    lds_barrier();
     if (t == 0) {
         double a = 0.0;
         for (int k = 0; k < TT_WAVE; ++k) a += s_sum[k];
@@ -263,17 +312,17 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
         const unsigned done = atomicAdd(S.counter, 1u);
         s_last = done == (unsigned)S.n_blocks - 1;
     }
-    __syncthreads();
+    lds_barrier();
     if (s_last) {  // the last workgroup: sum the partials, every thread a strided share, then a fixed tree
         __threadfence();
         double a = 0.0;
         for (int b = t; b < S.n_blocks; b += kTileThreads)
             a += __hip_atomic_load(&S.partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_red[t] = a;
-        __syncthreads();
+        lds_barrier();
         for (int w = kTileThreads / 2; w > 0; w >>= 1) {
             if (t < w) s_red[t] += s_red[t + w];
-            __syncthreads();
+            lds_barrier();
         }
         if (t == 0) {
             T.total[0] = s_red[0];
@@ -319,15 +368,20 @@ int tile_threads() { return kTileThreads; }
 int tile_post_rec_words(int n) { return post_rec_words(n); }
 int tile_pre_rec_words(int n) { return pre_rec_words(n); }
 
+// Words of the staged-record region: the larger phase's records.
+size_t tile_stage_words(int n, int n_post, int n_pre) {
+    return std::max((size_t)n_post * post_rec_words(n), (size_t)n_pre * pre_rec_words(n));
+}
+
 // LDS bytes of k_tile_marginal for tp patterns per workgroup and n_rounds
-// round bounds (two per round) in all.
-size_t tile_lds_bytes(const TreeDev &T, int tp, int n_rounds) {
+// round bounds (two per round) in all, with stage_words of staged records.
+size_t tile_lds_bytes(const TreeDev &T, int tp, int n_rounds, size_t stage_words) {
     const size_t rows = align4((size_t)T.n_int * tp * T.n) * sizeof(float);
     const size_t lut = align4((size_t)T.n_codes * T.n) * sizeof(float);
     const size_t red = (size_t)kTileThreads * sizeof(double);
     const size_t rounds = align4((size_t)n_rounds) * sizeof(int);
     const size_t tips = ((size_t)T.n_tips * tp + 15) & ~(size_t)15;
-    return rows + lut + red + rounds + tips;
+    return rows + lut + red + rounds + tips + stage_words * sizeof(int32_t);
 }
 
 #define TT_TILE_DISPATCH(n, CALL)                        \
@@ -339,18 +393,18 @@ size_t tile_lds_bytes(const TreeDev &T, int tp, int n_rounds) {
         default: return hipErrorInvalidValue;            \
     }
 
+template <int N>
+const void *tile_kernel(bool staged) {
+    return staged ? (const void *)k_tile_marginal<N, true> : (const void *)k_tile_marginal<N, false>;
+}
+
 // Workgroups of k_tile_marginal one CU holds at once (registers and LDS).
-int tile_blocks_per_cu(int n, size_t lds) {
+int tile_blocks_per_cu(int n, size_t lds, bool staged) {
+    const void *fn = n == 2 ? tile_kernel<2>(staged) : n == 3 ? tile_kernel<3>(staged)
+                   : n == 4 ? tile_kernel<4>(staged) : n == 5 ? tile_kernel<5>(staged) : nullptr;
+    if (!fn) return 0;
     int b = 0;
-    hipError_t e = hipErrorInvalidValue;
-    switch (n) {
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_tile_marginal<2>, kTileThreads, lds); break;
-        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_tile_marginal<3>, kTileThreads, lds); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_tile_marginal<4>, kTileThreads, lds); break;
-        case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_tile_marginal<5>, kTileThreads, lds); break;
-        default: break;
-    }
-    if (e != hipSuccess) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, kTileThreads, lds) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
@@ -367,12 +421,17 @@ hipError_t tile_pack(const TreeDev &T, const int32_t *post_desc, int n_post, con
 }
 
 hipError_t tile_marginal(const TreeDev &T, const int32_t *post_rec, const int32_t *pre_rec, const int32_t *rounds,
-                         int Rp, int Ri, int Ra, int tp, double *partial, unsigned *counter, int tips, int pre,
-                         hipStream_t s) {
-    TileDev S{post_rec, pre_rec, rounds, Rp, Ri, Ra, tp, (T.P + tp - 1) / tp, partial, counter};
-    const size_t lds = tile_lds_bytes(T, tp, 2 * (Rp + Ri + Ra));
-    TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL(k_tile_marginal<N_>, dim3((unsigned)S.n_blocks), dim3(kTileThreads),
-                                             lds, s, T, S, tips, pre));
+                         int Rp, int Ri, int Ra, int tp, int n_post, int n_pre, bool staged, double *partial,
+                         unsigned *counter, int tips, int pre, hipStream_t s) {
+    TileDev S{post_rec, pre_rec, rounds, Rp, Ri, Ra, tp, (T.P + tp - 1) / tp, n_post, n_pre, partial, counter};
+    const size_t lds = tile_lds_bytes(T, tp, 2 * (Rp + Ri + Ra), staged ? tile_stage_words(T.n, n_post, n_pre) : 0);
+    if (staged) {
+        TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL((k_tile_marginal<N_, true>), dim3((unsigned)S.n_blocks),
+                                                 dim3(kTileThreads), lds, s, T, S, tips, pre));
+    } else {
+        TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL((k_tile_marginal<N_, false>), dim3((unsigned)S.n_blocks),
+                                                 dim3(kTileThreads), lds, s, T, S, tips, pre));
+    }
     return hipGetLastError();
 }
 

@@ -187,6 +187,13 @@ class DeviceTree:
         _native.check(self.lib.tt_tree_info(self.handle, C.byref(a), C.byref(b), C.byref(c)))
         return {"levels_post": a.value, "levels_pre": b.value, "device_bytes": c.value}
 
+    def pass_shape(self):
+        """(pattern tile of the one-launch pass or 0 for the level kernels,
+        whether its records are staged in LDS)."""
+        a, b = C.c_int32(), C.c_int32()
+        _native.check(self.lib.tt_tree_pass_shape(self.handle, C.byref(a), C.byref(b)))
+        return a.value, bool(b.value)
+
     def marginal(self, reconstruct_tips=False, stream=None, want_logL=True, keep_normalisers=False):
         out = C.c_double(np.nan)
         flags = _native.TT_RECONSTRUCT_TIPS if reconstruct_tips else 0
