@@ -122,19 +122,21 @@ def algorithmic_bytes(prep):
 
 def joint_algorithmic_bytes(prep):
     """Minimal compulsory HBM bytes of one joint pass with FP32 messages: per
-    pattern, 9n+3 per internal-child branch (the child's message row written
-    and read back, 4n each; its argmax row written, n; the backtrack reads the
-    parent state and one argmax byte and writes the state, 3), 3 per tip
-    branch (the character code read by the postorder and again by the
-    log-likelihood score of the decoded states, plus the parent state), plus
-    8 (weights) and 9 (root: logL, state)."""
+    pattern, 8n+c+3 per internal-child branch (the child's message row written
+    and read back, 4n each; its argmax row written, c bytes: 1 for n <= 4 at 2
+    bits per parent state, else n; the backtrack reads the parent state and
+    one argmax byte and writes the state, 3), 3 per tip branch (the character
+    code read by the postorder and again by the log-likelihood score of the
+    decoded states, plus the parent state), plus 8 (weights) and 9 (root:
+    logL, state)."""
     n = prep["n_states"]
     parent, is_tip = tree_arrays(prep)
     nonroot = parent >= 0
     n_tip_br = int((is_tip & nonroot).sum())
     n_int_br = int((~is_tip & nonroot).sum())
     P = prep["tip_codes"].shape[1]
-    return P * (n_int_br * (9 * n + 3) + 3 * n_tip_br + 8 + 9)
+    cx = 1 if n <= 4 else n
+    return P * (n_int_br * (8 * n + cx + 3) + 3 * n_tip_br + 8 + 9)
 
 
 def loglik_algorithmic_bytes(prep):

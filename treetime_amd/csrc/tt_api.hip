@@ -103,12 +103,13 @@ struct BatchPlan {
     bool joint = false;     // joint reconstruction plan (k_batch_joint)
     const std::vector<BatchItem> &items_for(const BatchShape &c) const { return c.tiles == 2 ? items2 : items1; }
     const BatchItem *d_items_for(const BatchShape &c) const { return c.tiles == 2 ? d_items2 : d_items1; }
+    const int *d_xoff_for(const BatchShape &c) const { return d_queue + (c.tiles == 2 ? kQueueXoff2 : kQueueXoff1); }
     hipError_t launch(int n, int mode, const BatchShape &c, hipStream_t s) const {
         if (joint)
-            return ttk::batch_joint(n, d_trees, d_items_for(c), (int)items_for(c).size(), d_queue, d_item_out, mode,
-                                    c, s);
-        return ttk::batch_marginal(n, d_trees, d_items_for(c), (int)items_for(c).size(), d_queue, d_item_out, mode,
-                                   c, s);
+            return ttk::batch_joint(n, d_trees, d_items_for(c), (int)items_for(c).size(), d_queue, d_xoff_for(c),
+                                    d_item_out, mode, c, s);
+        return ttk::batch_marginal(n, d_trees, d_items_for(c), (int)items_for(c).size(), d_queue, d_xoff_for(c),
+                                   d_item_out, mode, c, s);
     }
     ~BatchPlan() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -420,7 +421,7 @@ int32_t ensure_joint(tt_tree *T) {
     else
         rc = T->alloc((void **)&T->d.Lx, rows * sizeof(float));
     if (rc) return rc;
-    if ((rc = T->alloc((void **)&T->d.Cx, (size_t)T->n_nodes * T->P * T->n))) {
+    if ((rc = T->alloc((void **)&T->d.Cx, (size_t)T->n_nodes * T->P * cx_row_bytes(T->n)))) {
         // give the message buffer back, so a retry under budget pressure
         // starts from nothing instead of stacking message buffers
         T->release(T->site ? (void *)T->d.Lx64 : (void *)T->d.Lx);
@@ -1656,11 +1657,19 @@ int32_t tt_get_joint_debug(tt_tree *T, double *lx, uint8_t *cx) {
     hipError_t e = T->site ? ttk::joint_rows_site(T->d, d_rel, d_own, s) : ttk::joint_rows(T->d, d_rel, d_own, s);
     if (e == hipSuccess) e = hipMemcpyAsync(lx, d_rel, nrow * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(own.data(), d_own, rows * sizeof(double), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess && cx) e = hipMemcpyAsync(cx, T->d.Cx, nrow, hipMemcpyDeviceToHost, s);
+    // argmax rows: packed rows are unpacked below, in place from the end
+    if (e == hipSuccess && cx) e = hipMemcpyAsync(cx, T->d.Cx, rows * cx_row_bytes(T->n), hipMemcpyDeviceToHost, s);
     const hipError_t e2 = hipFreeAsync(d_rel, s);
     if (e == hipSuccess) e = e2;
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return fail(TT_EDEVICE, "tt_get_joint_debug: %s", hipGetErrorString(e));
+    if (cx && cx_packed(T->n)) {
+        const int nn = T->n;
+        for (size_t r = rows; r-- > 0;) {
+            const uint8_t b = cx[r];
+            for (int j = nn - 1; j >= 0; --j) cx[r * nn + j] = (uint8_t)((b >> (2 * j)) & 3);
+        }
+    }
     // absolute log messages: add, per internal node, the sum of the log
     // maxima over its subtree's internal nodes (children before parents)
     const int P = T->P, n = T->n;
@@ -1861,11 +1870,32 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
             order.push_back({(long)trees[i]->n_int + pre, i});
         }
         std::stable_sort(order.begin(), order.end(), [](auto &a, auto &b) { return a.first > b.first; });
-        for (auto &o : order) {
-            const int tiles = (trees[o.second]->P + TT_WAVE - 1) / TT_WAVE;
-            for (int t = 0; t < tiles; ++t) P->items1.push_back({o.second, t});
-            for (int t = 0; t < tiles; t += 2) P->items2.push_back({o.second, t});
+        // trees are dealt to the dies longest first, each to the least loaded
+        // die (work: tiles x node steps), so that a tree's P(t) tables and
+        // rows stay in one die's L2; each die's items keep the global order
+        std::vector<int> die_of(k, 0);
+        {
+            std::vector<double> load(kXcds, 0.0);
+            for (auto &o : order) {
+                const int tiles = (trees[o.second]->P + TT_WAVE - 1) / TT_WAVE;
+                const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                die_of[o.second] = x;
+                load[x] += (double)tiles * (double)std::max(1L, o.first);
+            }
         }
+        std::vector<int32_t> xoff(2 * 16, 0);
+        for (int x = 0; x < kXcds; ++x) {
+            xoff[x] = (int32_t)P->items1.size();
+            xoff[16 + x] = (int32_t)P->items2.size();
+            for (auto &o : order) {
+                if (die_of[o.second] != x) continue;
+                const int tiles = (trees[o.second]->P + TT_WAVE - 1) / TT_WAVE;
+                for (int t = 0; t < tiles; ++t) P->items1.push_back({o.second, t});
+                for (int t = 0; t < tiles; t += 2) P->items2.push_back({o.second, t});
+            }
+        }
+        xoff[kXcds] = (int32_t)P->items1.size();
+        xoff[16 + kXcds] = (int32_t)P->items2.size();
         std::vector<BatchTreeDev> bt(k);
         for (int i = 0; i < k; ++i) {
             tt_tree *T = trees[i];
@@ -1884,7 +1914,7 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
         }
         const size_t nb_trees = bt.size() * sizeof(BatchTreeDev), nb_items1 = P->items1.size() * sizeof(BatchItem),
                      nb_items2 = P->items2.size() * sizeof(BatchItem), nb_out = P->items1.size() * sizeof(double);
-        P->bytes = nb_trees + nb_items1 + nb_items2 + nb_out + 256;
+        P->bytes = nb_trees + nb_items1 + nb_items2 + nb_out + kQueueInts * sizeof(int);
         if (eng->budget && eng->used.load() + P->bytes > eng->budget) {
             P->bytes = 0;
             delete P;
@@ -1894,7 +1924,10 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
         hipError_t e = hipMalloc((void **)&P->d_trees, nb_trees);
         if (e == hipSuccess) e = hipMalloc((void **)&P->d_items1, nb_items1);
         if (e == hipSuccess) e = hipMalloc((void **)&P->d_items2, nb_items2);
-        if (e == hipSuccess) e = hipMalloc((void **)&P->d_queue, 256);
+        if (e == hipSuccess) e = hipMalloc((void **)&P->d_queue, kQueueInts * sizeof(int));
+        if (e == hipSuccess) e = hipMemset(P->d_queue, 0, kQueueInts * sizeof(int));
+        if (e == hipSuccess)
+            e = hipMemcpy(P->d_queue + kQueueXoff1, xoff.data(), xoff.size() * sizeof(int32_t), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc((void **)&P->d_item_out, nb_out);
         if (e == hipSuccess) e = hipMemcpy(P->d_trees, bt.data(), nb_trees, hipMemcpyHostToDevice);
         P->host_trees = bt;

@@ -22,7 +22,25 @@ using namespace ttd;
 
 constexpr int kWaves = TT_BLOCK / TT_WAVE;
 
-__global__ void k_reset_queue(int *queue) { *queue = 0; }
+__global__ void k_reset_queue(int *queue) {
+    if (threadIdx.x < kXcds) queue[threadIdx.x * kQueueStride] = 0;
+}
+
+// Next item for the calling thread's workgroup (-1: all queues empty): its
+// die's queue first, then the others; *start is the first queue (relative
+// to the own die) still worth trying.
+__device__ __forceinline__ int pop_item(int *queue, const int *xoff, int &start) {
+    const int x0 = blockIdx.x % kXcds;
+    for (; start < kXcds; ++start) {
+        const int x = (x0 + start) % kXcds;
+        const int lo = xoff[x], n = xoff[x + 1] - lo;
+        if (n > 0) {
+            const int i = atomicAdd(queue + x * kQueueStride, 1);
+            if (i < n) return lo + i;
+        }
+    }
+    return -1;
+}
 
 // TILES 64-pattern tiles per item: every lane carries TILES patterns of the
 // same nodes, so each node step keeps 2 * TILES row fetches in flight and the
@@ -91,7 +109,8 @@ __device__ __forceinline__ void pre_pair_t(const TreeDev &T, const PreDesc &q0, 
 template <int N, int TL, int WPE>
 __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_batch_marginal(const BatchTreeDev *__restrict__ trees_arg,
                                                              const BatchItem *__restrict__ items_arg, int n_items,
-                                                             int *queue, double *item_out, int mode) {
+                                                             int *queue, const int *xoff, double *item_out,
+                                                             int mode) {
     // mode bit 0: reconstruct tip states; bit 1: postorder and root only (loglik)
     const bool tips = mode & 1, pre = !(mode & 2);
     // tree records and items are read-only: scalar loads through the constant address space
@@ -101,12 +120,12 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
     __shared__ double s_acc[kWaves][TL][TT_WAVE];
     __shared__ float s_lut[256 * N];
     const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
-    int cur_tree = -1;
+    int cur_tree = -1, qstart = 0;
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+        if (threadIdx.x == 0) s_item = pop_item(queue, xoff, qstart);
         __syncthreads();
         const int it = s_item;
-        if (it >= n_items) break;
+        if (it < 0) break;
         const BatchItem item = items[it];
         const BatchTreeDev TT_C &B = trees[item.tree];
         const TreeDev TT_C &T = B.T;
@@ -182,19 +201,20 @@ __device__ __forceinline__ double wave_sum(double x) {
 template <int N>
 __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal_w(const BatchTreeDev *__restrict__ trees_arg,
                                                                const BatchItem *__restrict__ items_arg,
-                                                               int n_items, int *queue, double *item_out, int mode) {
+                                                               int n_items, int *queue, const int *xoff,
+                                                               double *item_out, int mode) {
     const bool tips = mode & 1, pre = !(mode & 2);
     const BatchTreeDev TT_C *trees = (const BatchTreeDev TT_C *)trees_arg;
     const BatchItem TT_C *items = (const BatchItem TT_C *)items_arg;
     __shared__ float s_lut[kWaves][256 * N];
     const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
     float *lut = s_lut[w];
-    int cur_tree = -1;
+    int cur_tree = -1, qstart = 0;
     for (;;) {
         int it = 0;
-        if (lane == 0) it = atomicAdd(queue, 1);
+        if (lane == 0) it = pop_item(queue, xoff, qstart);
         it = __shfl(it, 0, TT_WAVE);
-        if (it >= n_items) break;
+        if (it < 0) break;
         const BatchItem item = items[it];
         const BatchTreeDev TT_C &B = trees[item.tree];
         const TreeDev TT_C &T = B.T;
@@ -252,7 +272,8 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal_w(const BatchTreeDe
 template <int N, int WPE>
 __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_batch_joint(const BatchTreeDev *__restrict__ trees_arg,
                                                          const BatchItem *__restrict__ items_arg, int n_items,
-                                                         int *queue, double *item_out, int mode) {
+                                                         int *queue, const int *xoff, double *item_out,
+                                                         int mode) {
     const BatchTreeDev TT_C *trees = (const BatchTreeDev TT_C *)trees_arg;
     const BatchItem TT_C *items = (const BatchItem TT_C *)items_arg;
     const int tips = mode & 1;
@@ -262,12 +283,12 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
     __shared__ double s_lj[kStage ? 256 * N : 1];
     __shared__ double s_ll[kStage ? 256 * N : 1];
     const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
-    int cur_tree = -1;
+    int cur_tree = -1, qstart = 0;
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+        if (threadIdx.x == 0) s_item = pop_item(queue, xoff, qstart);
         __syncthreads();
         const int it = s_item;
-        if (it >= n_items) break;
+        if (it < 0) break;
         const BatchItem item = items[it];
         const BatchTreeDev TT_C &B = trees[item.tree];
         const TreeDev TT_C &T = B.T;
@@ -336,7 +357,12 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
 #pragma unroll
                 for (int m = 0; m < kStep; ++m) ps[m] = T.states[(size_t)q[m].par * T.P + p];
 #pragma unroll
-                for (int m = 0; m < kStep; ++m) cs[m] = T.Cx[((size_t)q[m].c * T.P + p) * N + ps[m]];
+                for (int m = 0; m < kStep; ++m) {
+                    if constexpr (cx_packed(N))
+                        cs[m] = (T.Cx[(size_t)q[m].c * T.P + p] >> (2 * ps[m])) & 3;
+                    else
+                        cs[m] = T.Cx[((size_t)q[m].c * T.P + p) * N + ps[m]];
+                }
 #pragma unroll
                 for (int m = 0; m < kStep; ++m)
                     if (k + m * kWaves < ke) T.states[(size_t)q[m].c * T.P + p] = cs[m];
@@ -438,26 +464,26 @@ bool ttk::batch_joint_budget(int n, int waves_per_simd) {
 }
 
 hipError_t ttk::batch_joint(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
-                            double *item_out, int mode, const BatchShape &shape, hipStream_t s) {
+                            const int *xoff, double *item_out, int mode, const BatchShape &shape, hipStream_t s) {
     if (n_items <= 0) return hipSuccess;
     const int resident = device_cus() * std::max(1, shape.blocks_per_cu);
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, queue);
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(kXcds), 0, s, queue);
     TT_BATCH_DISPATCH(n, {
         const int grid = std::min(resident, n_items);
         if constexpr (all_shapes(N_)) {
             if (shape.waves_per_simd == 6) {
                 hipLaunchKernelGGL((k_batch_joint<N_, 6>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items,
-                                   queue, item_out, mode);
+                                   queue, xoff, item_out, mode);
                 return hipGetLastError();
             }
             if (N_ <= kJoint8MaxN && shape.waves_per_simd == 8) {
                 hipLaunchKernelGGL((k_batch_joint<N_, N_ <= kJoint8MaxN ? 8 : 1>), dim3(grid), dim3(TT_BLOCK), 0, s,
-                                   trees, items, n_items, queue, item_out, mode);
+                                   trees, items, n_items, queue, xoff, item_out, mode);
                 return hipGetLastError();
             }
         }
         hipLaunchKernelGGL((k_batch_joint<N_, 1>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items, queue,
-                           item_out, mode);
+                           xoff, item_out, mode);
     });
     return hipGetLastError();
 }
@@ -493,31 +519,31 @@ int ttk::batch_max_blocks_per_cu(int n, const BatchShape &c) {
 }
 
 hipError_t ttk::batch_marginal(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
-                               double *item_out, int tips, const BatchShape &shape, hipStream_t s) {
+                               const int *xoff, double *item_out, int tips, const BatchShape &shape, hipStream_t s) {
     if (n_items <= 0) return hipSuccess;
     const int resident = device_cus() * std::max(1, shape.blocks_per_cu);
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, queue);
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(kXcds), 0, s, queue);
     TT_BATCH_DISPATCH(n, {
         if constexpr (!all_shapes(N_)) {
             const int grid = std::min(resident, n_items);
             hipLaunchKernelGGL((k_batch_marginal<N_, 1, 1>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items,
-                               n_items, queue, item_out, tips);
+                               n_items, queue, xoff, item_out, tips);
         } else if (shape.wave_items) {
             const int grid = std::min(resident, (n_items + kWaves - 1) / kWaves);
             hipLaunchKernelGGL(k_batch_marginal_w<N_>, dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items,
-                               queue, item_out, tips);
+                               queue, xoff, item_out, tips);
         } else if (shape.tiles == 2 && shape.waves_per_simd == 5) {
             const int grid = std::min(resident, n_items);
             hipLaunchKernelGGL((k_batch_marginal<N_, 2, 5>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items,
-                               n_items, queue, item_out, tips);
+                               n_items, queue, xoff, item_out, tips);
         } else if (shape.tiles == 2) {
             const int grid = std::min(resident, n_items);
             hipLaunchKernelGGL((k_batch_marginal<N_, 2, 1>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items,
-                               n_items, queue, item_out, tips);
+                               n_items, queue, xoff, item_out, tips);
         } else {
             const int grid = std::min(resident, n_items);
             hipLaunchKernelGGL((k_batch_marginal<N_, 1, 1>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items,
-                               n_items, queue, item_out, tips);
+                               n_items, queue, xoff, item_out, tips);
         }
     });
     return hipGetLastError();

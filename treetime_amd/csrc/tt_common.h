@@ -55,7 +55,7 @@ struct TreeDev {
     uint8_t TT_G *states;            // [n_nodes][P]
     float TT_G *Lx;                  // [n_int][P][n]   joint messages (max-product), max entry = 1
     double TT_G *Lx64;               // [n_int][P][n]   site-specific joint messages (log), max entry = 0
-    uint8_t TT_G *Cx;                // [n_nodes][P][n] joint argmax table
+    uint8_t TT_G *Cx;                // joint argmax table: [n_nodes][P] packed (n <= 4) or [n_nodes][P][n]
     double TT_G *part;               // [R][P]          per-chunk-group FP64 log-normaliser (or joint score) sums
     double TT_G *part2;              // [S][P]          row-slice sums
     double TT_G *seqlh;              // [P]  marginal per-pattern logL (root-path updates start here)
@@ -193,6 +193,13 @@ hipError_t branch_profile_lh(const TreeDev &T, int gap, const double *eig, const
                              double mu, const double *t, int K, double *out, hipStream_t s);
 }  // namespace ttk
 
+// Joint argmax rows (Cx): for alphabets of up to 4 states one byte per
+// (node, pattern), 2 bits per parent state j at bits 2j..2j+1; wider
+// alphabets one byte per parent state.
+constexpr int kCxPackedMaxN = 4;
+__host__ __device__ constexpr bool cx_packed(int n) { return n <= kCxPackedMaxN; }
+__host__ __device__ constexpr size_t cx_row_bytes(int n) { return cx_packed(n) ? 1 : (size_t)n; }
+
 // A tree as seen by the persistent batched kernel: the tree buffers plus its
 // level schedule.
 struct BatchTreeDev {
@@ -208,6 +215,19 @@ struct BatchTreeDev {
     int32_t H, D;
     int32_t fill_tips;               // reset tip states to 255 (an earlier pass reconstructed them)
 };
+
+// Work queues of the batched launch, one per XCD (accelerator die; each has
+// its own L2): the items are split by tree over the dies, a workgroup pops
+// from its own die's queue (workgroups are dispatched round-robin over the
+// dies, so workgroup b runs on die b % kXcds) and from the others once its
+// own is empty.  Layout of the queue buffer (ints): counter of die x at
+// x * kQueueStride; item offsets per die ([kXcds + 1]) of the one-tile and
+// of the two-tile item lists at kQueueXoff1 and kQueueXoff2.
+constexpr int kXcds = 8;
+constexpr int kQueueStride = 32;
+constexpr int kQueueXoff1 = kXcds * kQueueStride;
+constexpr int kQueueXoff2 = kQueueXoff1 + 16;
+constexpr int kQueueInts = kQueueXoff2 + 16;
 
 // One work item of the batched launch: patterns [64 tile, 64 (tile + tiles))
 // of one tree, tiles per item given by the launch shape.
@@ -226,10 +246,10 @@ struct BatchShape {
 
 namespace ttk {
 hipError_t batch_marginal(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
-                          double *item_out, int tips, const BatchShape &shape, hipStream_t s);
+                          const int *xoff, double *item_out, int tips, const BatchShape &shape, hipStream_t s);
 // joint reconstruction of many trees in one persistent launch (one-tile block items)
 hipError_t batch_joint(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
-                       double *item_out, int mode, const BatchShape &shape, hipStream_t s);
+                       const int *xoff, double *item_out, int mode, const BatchShape &shape, hipStream_t s);
 int batch_joint_max_blocks_per_cu(int n, int waves_per_simd);
 // occupancy limit (blocks per CU) of the batched kernel for n states and item shape; 0 if n unsupported
 int batch_max_blocks_per_cu(int n, const BatchShape &shape);

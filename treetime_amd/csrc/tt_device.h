@@ -65,6 +65,25 @@ __device__ __forceinline__ void store_bytes(Ptr dst, const int (&c)[N]) {
     }
 }
 
+// Joint argmax rows (Cx), layout in tt_common.h (cx_packed).
+template <int N>
+__device__ __forceinline__ void cx_store(const TreeDev &T, int node, int p, const int (&c)[N]) {
+    if constexpr (cx_packed(N)) {
+        uint32_t b = 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) b |= (uint32_t)c[j] << (2 * j);
+        T.Cx[(size_t)node * T.P + p] = (uint8_t)b;
+    } else {
+        store_bytes<N>(T.Cx + ((size_t)node * T.P + p) * N, c);
+    }
+}
+
+// State of node under parent state ps from its argmax row.
+__device__ __forceinline__ int cx_state(const TreeDev &T, int node, int p, int ps) {
+    if (cx_packed(T.n)) return (T.Cx[(size_t)node * T.P + p] >> (2 * ps)) & 3;
+    return T.Cx[((size_t)node * T.P + p) * T.n + ps];
+}
+
 // Child message m_j = sum_i r_i P[i][j] (GTR.propagate_profile, gtr.py:992-995).
 template <int N, class Ptr>
 __device__ __forceinline__ void child_message(const double (&r)[N], Ptr M, double (&m)[N]) {
@@ -593,7 +612,7 @@ __device__ __forceinline__ void tip_joint_code(const TreeDev &T, LJ lutj, int co
     using R = JR<N>;
     const auto lj = lutj + code * N;
     const auto PJ = joint_P<N>(T) + (size_t)cnode * (N * N);
-    const auto cxo = T.Cx + ((size_t)cnode * T.P + p) * N;
+    const auto cxo = T.Cx + ((size_t)cnode * T.P + p) * N;  // byte rows (N > 8)
     R tl[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) tl[i] = lj[i];
@@ -618,7 +637,7 @@ __device__ __forceinline__ void tip_joint_code(const TreeDev &T, LJ lutj, int co
         }
     }
     if constexpr (N <= 8) {
-        if (store_cx) store_bytes<N>(cxo, cx);
+        if (store_cx) cx_store<N>(T, cnode, p, cx);
     }
 }
 
@@ -670,7 +689,7 @@ __device__ __forceinline__ void joint_store(const TreeDev &T, int v, int vn, int
     using R = JR<N>;
     const auto PJ = joint_P<N>(T) + (size_t)vn * (N * N);
     const auto dst = T.Lx + ((size_t)v * T.P + p) * N;
-    const auto cxo = T.Cx + ((size_t)vn * T.P + p) * N;
+    const auto cxo = T.Cx + ((size_t)vn * T.P + p) * N;  // byte rows (N > 8)
     // large alphabets write the argmax bytes one by one
     R best[N];
     int cx[N];
@@ -704,7 +723,7 @@ __device__ __forceinline__ void joint_store(const TreeDev &T, int v, int vn, int
 #pragma unroll
     for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : (float)(best[j] * inv);
     store_row<N>(dst, out);
-    if constexpr (N <= 8) store_bytes<N>(cxo, cx);
+    if constexpr (N <= 8) cx_store<N>(T, vn, p, cx);
 }
 
 template <int N>

@@ -6,7 +6,8 @@
 //                               node.marginal_subtree_LH, treeanc.py:877)
 //   prof [n_nodes][P][n] float  marginal profiles, sum = 1 (treeanc.py:909-917)
 //   Lx   [n_int][P][n]   float  joint (max-product) messages, largest entry exactly 1
-//   Cx   [n_nodes][P][n] uint8  joint argmax table (treeanc.py:996-1011)
+//   Cx   [n_nodes][P] (n <= 4, 2 bits per parent state) or [n_nodes][P][n]
+//        uint8  joint argmax table (treeanc.py:996-1011)
 //   part [R][P]          double per-chunk-group sums of log normalisers
 //
 // Precision scheme: rows are stored in FP32, arithmetic is FP64.  Because a
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(TT_BLOCK) void k_backtrack(TreeDev T, LevelDev L, i
     const int c = wave_uniform(L.nodes[k]);
     const int par = T.parent[c];
     const int ps = T.states[(size_t)par * T.P + p];
-    T.states[(size_t)c * T.P + p] = T.Cx[((size_t)c * T.P + p) * T.n + ps];
+    T.states[(size_t)c * T.P + p] = cx_state(T, c, p, ps);
 }
 
 // Branch re-evaluation along one root path (update mode).  Starting at the

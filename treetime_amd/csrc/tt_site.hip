@@ -375,7 +375,7 @@ __device__ __forceinline__ void site_joint_children(const TreeDev &T, int e0, in
             site_joint_max<N>(P, E, 0.0, best, cx);
 #pragma unroll
             for (int j = 0; j < N; ++j) msg[j] += best[j];
-            if (store_tip_cx) store_bytes<N>(T.Cx + ((size_t)cnode * T.P + p) * N, cx);
+            if (store_tip_cx) cx_store<N>(T, cnode, p, cx);
         } else {
             const auto src = T.Lx64 + ((size_t)cref * T.P + p) * N;
 #pragma unroll
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(TT_BLOCK) void k_post_joint_site(TreeDev T, LevelDe
             const auto dst = T.Lx64 + ((size_t)v * T.P + p) * N;
 #pragma unroll
             for (int j = 0; j < N; ++j) dst[j] = best[j] - off;
-            store_bytes<N>(T.Cx + ((size_t)vn * T.P + p) * N, cx);
+            cx_store<N>(T, vn, p, cx);
             acc += off;
         }
     }
@@ -496,7 +496,7 @@ __global__ __launch_bounds__(TT_BLOCK) void k_joint_rows_site(TreeDev T, double 
         site_joint_max<N>(P, E, 0.0, best, cx);
 #pragma unroll
         for (int j = 0; j < N; ++j) o[j] = best[j];
-        store_bytes<N>(T.Cx + ((size_t)c * T.P + p) * N, cx);
+        cx_store<N>(T, c, p, cx);
         own[gid] = 0.0;
         return;
     }
