@@ -274,18 +274,21 @@ def test_batch_run_modes(gpu_engine, block_batches):
 
 
 
+@pytest.mark.parametrize("tiles", ["1", "2"])
 @pytest.mark.parametrize("n_states", [4, 5])
 @pytest.mark.parametrize("wps", ["0", "6", "8"])
-def test_batch_joint_register_budgets(gpu_engine, block_batches, monkeypatch, wps, n_states):
-    """Every register budget of the batched joint kernel (the tuner's
-    candidates, pinned one at a time): states identical to tt_joint, logL to
-    1e-12, with and without tip reconstruction, on binary and multifurcating
-    trees.  The 8-wave budget exists for alphabets of up to 4 states only;
-    pinning it for 5 states is refused with an error naming the budget."""
+def test_batch_joint_register_budgets(gpu_engine, block_batches, monkeypatch, wps, n_states, tiles):
+    """Every register budget and item size of the batched joint kernel (the
+    tuner's candidates, pinned one at a time): states identical to tt_joint,
+    logL to 1e-12, with and without tip reconstruction, on binary and
+    multifurcating trees.  The 8-wave budget exists for alphabets of up to 4
+    states only; pinning it for 5 states is refused with an error naming the
+    budget."""
     from treetime_amd.engine import batch_info, batch_run
 
-    say(f"[batch] joint kernel, waves per SIMD {wps}, n={n_states}")
+    say(f"[batch] joint kernel, waves per SIMD {wps}, n={n_states}, tiles per item {tiles}")
     monkeypatch.setenv("TT_JOINT_WAVES_PER_SIMD", wps)
+    monkeypatch.setenv("TT_BATCH_TILES", tiles)
     jobs = [small_job(n, L, n_states, seed) for n, L, seed in [(90, 700, 81), (12, 130, 82)]]
     jobs.append(polytomy_job(120, 400, n_states, 83, 0.3))
     if n_states == 5 and wps == "8":
@@ -300,7 +303,8 @@ def test_batch_joint_register_budgets(gpu_engine, block_batches, monkeypatch, wp
         plain = [t.joint(reconstruct_tips=tips) for t in trees]
         plain_states = [t.states() for t in trees]
         got = batch_run(gpu_engine, trees, "joint", reconstruct_tips=tips)
-        assert batch_info(gpu_engine, trees, kind="joint", reconstruct_tips=tips)["waves_per_simd"] == int(wps)
+        info = batch_info(gpu_engine, trees, kind="joint", reconstruct_tips=tips)
+        assert info["waves_per_simd"] == int(wps) and info["tiles"] == int(tiles)
         for i, t in enumerate(trees):
             assert abs(got[i] - plain[i]) <= 1e-12 * abs(plain[i]), (wps, tips, i, got[i], plain[i])
             assert np.array_equal(t.states(), plain_states[i]), (wps, tips, i)

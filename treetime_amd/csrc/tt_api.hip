@@ -733,16 +733,18 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     const char *bpc = getenv("TT_BATCH_BLOCKS_PER_CU");
     const char *tune = getenv("TT_AUTOTUNE");
     const char *jw = getenv("TT_JOINT_WAVES_PER_SIMD");  // joint plans: keep only this register budget
+    const char *tl = getenv("TT_BATCH_TILES");           // keep only block items of this many tiles
     const bool log = getenv("TT_AUTOTUNE_LOG") != nullptr;  // print every candidate's time to stderr
     auto allowed = [&](int wave_items) {
         if (!mode || !mode[0]) return true;
         const int want = mode[0] == 'b' ? 0 : mode[0] == 'w' ? 1 : mode[0] == 'p' ? kPerTreeGraphs : -1;
         return wave_items == want;
     };
-    // the joint kernel has one-tile blocks, with the compiler's register
-    // budget or one that fits 6 or 8 waves per SIMD
+    // the joint kernel has block items of one or two tiles, with the
+    // compiler's register budget or one that fits 6 or 8 waves per SIMD
     const std::vector<BatchShape> kinds =
-        P->joint ? std::vector<BatchShape>{{0, 0, 1, 0}, {0, 0, 1, 6}, {0, 0, 1, 8}}
+        P->joint ? std::vector<BatchShape>{{0, 0, 2, 8}, {0, 0, 1, 8}, {0, 0, 2, 6}, {0, 0, 1, 6}, {0, 0, 2, 0},
+                                           {0, 0, 1, 0}}
                  : std::vector<BatchShape>{{0, 0, 2, 0}, {0, 0, 2, 5}, {0, 0, 1, 0}, {1, 0, 1, 0}};
     // a pinned joint register budget must exist for this alphabet: refuse it
     // rather than silently falling back to another launch path
@@ -754,10 +756,11 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     for (BatchShape kd : kinds) {
         if (!allowed(kd.wave_items)) continue;
         if (P->joint && jw && jw[0] && kd.waves_per_simd != atoi(jw)) continue;
+        if (tl && tl[0] && kd.tiles != atoi(tl)) continue;
         if (P->joint ? !ttk::batch_joint_budget(n, kd.waves_per_simd)
                      : !ttk::batch_all_shapes(n) && !(kd.wave_items == 0 && kd.tiles == 1))
             continue;
-        const int occ = P->joint ? ttk::batch_joint_max_blocks_per_cu(n, kd.waves_per_simd)
+        const int occ = P->joint ? ttk::batch_joint_max_blocks_per_cu(n, kd.waves_per_simd, kd.tiles)
                                  : ttk::batch_max_blocks_per_cu(n, kd);
         std::vector<int> per_cu;
         if (bpc && atoi(bpc) > 0) {

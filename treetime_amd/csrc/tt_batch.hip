@@ -269,7 +269,11 @@ __global__ __launch_bounds__(TT_BLOCK) void k_batch_marginal_w(const BatchTreeDe
 // logL agrees to the order of the per-pattern sum.  mode bit 0: reconstruct
 // tips.
 // WPE: waves per SIMD the register allocator must allow (1 = no constraint).
-template <int N, int WPE>
+// TL: 64-pattern tiles per item; with two, each node step is taken by two
+// waves, one per tile, so the node's tables and the byte rows (states,
+// argmax rows, tip codes: 64 bytes per tile) are fetched for both tiles at
+// once, and 4 / TL nodes are in flight per block step.
+template <int N, int WPE, int TL>
 __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void k_batch_joint(const BatchTreeDev *__restrict__ trees_arg,
                                                          const BatchItem *__restrict__ items_arg, int n_items,
                                                          int *queue, const int *xoff, double *item_out,
@@ -282,7 +286,9 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
     __shared__ double s_sc[kWaves][TT_WAVE];
     __shared__ double s_lj[kStage ? 256 * N : 1];
     __shared__ double s_ll[kStage ? 256 * N : 1];
+    constexpr int NW = kWaves / TL;  // nodes per block step
     const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    const int wn = w % NW, wt = w / NW;  // this wave's node slot and tile
     int cur_tree = -1, qstart = 0;
     for (;;) {
         if (threadIdx.x == 0) s_item = pop_item(queue, xoff, qstart);
@@ -308,15 +314,15 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
             else
                 return joint_tip_term<N>(T, T.loglut, c, code, sp);
         };
-        const int p = item.tile * TT_WAVE + lane;
+        const int p = (item.tile + wt) * TT_WAVE + lane;
         const bool act = p < T.P;
         // postorder: heights 1..H-1 (the last level is the root), two nodes per step
         for (int h = 0; h + 1 < B.H; ++h) {
             const int kb = B.post_off[h], ke = B.post_off[h + 1];
-            for (int k = wave_uniform(kb + w); k < ke; k += 2 * kWaves) {
-                const bool has1 = k + kWaves < ke;
+            for (int k = wave_uniform(kb + wn); k < ke; k += 2 * NW) {
+                const bool has1 = k + NW < ke;
                 const PostDesc d0 = load_post_desc(B.post_desc + kPostDesc * k);
-                const PostDesc d1 = has1 ? load_post_desc(B.post_desc + kPostDesc * (k + kWaves)) : d0;
+                const PostDesc d1 = has1 ? load_post_desc(B.post_desc + kPostDesc * (k + NW)) : d0;
                 if (!act) continue;
                 bool pair = false;
                 if constexpr (kStage) pair = d0.nchild == 2 && d1.nchild == 2;
@@ -335,9 +341,9 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
             }
             __syncthreads();
         }
-        if (w == 0 && act) (void)root_joint_state<N>(T, p, tips);
+        if (wn == 0 && act) (void)root_joint_state<N>(T, p, tips);
         if (!tips && B.fill_tips && act)
-            for (int k = w; k < T.n_tips; k += kWaves) T.states[(size_t)B.tip_nodes[k] * T.P + p] = 255;
+            for (int k = wn; k < T.n_tips; k += NW) T.states[(size_t)B.tip_nodes[k] * T.P + p] = 255;
         __syncthreads();
         // backtrack (treeanc.py:1048-1063): depths 1..D, parents before
         // children, kStep nodes per wave step; the score of the decoded states
@@ -347,11 +353,11 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
         for (int d = 0; d < B.D; ++d) {
             const int kb = B.pre_off[d], kint = kb + B.pre_nint[d];
             const int ke = tips ? B.pre_off[d + 1] : kint;
-            for (int k = wave_uniform(kb + w); k < ke; k += kStep * kWaves) {
+            for (int k = wave_uniform(kb + wn); k < ke; k += kStep * NW) {
                 PreDesc q[kStep];
 #pragma unroll
                 for (int m = 0; m < kStep; ++m)
-                    q[m] = load_pre_desc(B.pre_desc + kPreDesc * (k + m * kWaves < ke ? k + m * kWaves : k));
+                    q[m] = load_pre_desc(B.pre_desc + kPreDesc * (k + m * NW < ke ? k + m * NW : k));
                 if (!act) continue;
                 int ps[kStep], cs[kStep];
 #pragma unroll
@@ -365,10 +371,10 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
                 }
 #pragma unroll
                 for (int m = 0; m < kStep; ++m)
-                    if (k + m * kWaves < ke) T.states[(size_t)q[m].c * T.P + p] = cs[m];
+                    if (k + m * NW < ke) T.states[(size_t)q[m].c * T.P + p] = cs[m];
 #pragma unroll
                 for (int m = 0; m < kStep; ++m)
-                    if (k + m * kWaves < kint) sc += joint_int_term<N>(T, q[m].c, cs[m], ps[m]);
+                    if (k + m * NW < kint) sc += joint_int_term<N>(T, q[m].c, cs[m], ps[m]);
             }
             __syncthreads();
         }
@@ -376,11 +382,11 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
         if (act) {
             for (int d = 0; d < B.D; ++d) {
                 const int kt = B.pre_off[d] + B.pre_nint[d], ke = B.pre_off[d + 1];
-                for (int k = wave_uniform(kt + w); k < ke; k += kStep * kWaves) {
+                for (int k = wave_uniform(kt + wn); k < ke; k += kStep * NW) {
                     PreDesc q[kStep];
 #pragma unroll
                     for (int m = 0; m < kStep; ++m)
-                        q[m] = load_pre_desc(B.pre_desc + kPreDesc * (k + m * kWaves < ke ? k + m * kWaves : k));
+                        q[m] = load_pre_desc(B.pre_desc + kPreDesc * (k + m * NW < ke ? k + m * NW : k));
                     int ps[kStep], code[kStep];
 #pragma unroll
                     for (int m = 0; m < kStep; ++m) {
@@ -389,17 +395,17 @@ __global__ __launch_bounds__(TT_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
                     }
 #pragma unroll
                     for (int m = 0; m < kStep; ++m)
-                        if (k + m * kWaves < ke) sc += tip_term(q[m].c, code[m], ps[m]);
+                        if (k + m * NW < ke) sc += tip_term(q[m].c, code[m], ps[m]);
                 }
             }
         }
         s_sc[w][lane] = sc;
         __syncthreads();
         double contrib = 0.0;
-        if (w == 0 && act) {
+        if (wn == 0 && act) {
             double lp = log(T.pi[T.states[(size_t)T.root * T.P + p]]);
 #pragma unroll
-            for (int q = 0; q < kWaves; ++q) lp += s_sc[q][lane];
+            for (int q = 0; q < NW; ++q) lp += s_sc[wt * NW + q][lane];
             T.jseqlh[p] = lp;
             contrib = T.mult[p] * lp;
         }
@@ -468,44 +474,47 @@ hipError_t ttk::batch_joint(int n, const BatchTreeDev *trees, const BatchItem *i
     if (n_items <= 0) return hipSuccess;
     const int resident = device_cus() * std::max(1, shape.blocks_per_cu);
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(kXcds), 0, s, queue);
+    const int grid = std::min(resident, n_items);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items, queue, xoff, item_out, mode);
+    };
     TT_BATCH_DISPATCH(n, {
-        const int grid = std::min(resident, n_items);
+        const bool two = shape.tiles == 2;
         if constexpr (all_shapes(N_)) {
             if (shape.waves_per_simd == 6) {
-                hipLaunchKernelGGL((k_batch_joint<N_, 6>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items,
-                                   queue, xoff, item_out, mode);
+                two ? go(k_batch_joint<N_, 6, 2>) : go(k_batch_joint<N_, 6, 1>);
                 return hipGetLastError();
             }
             if (N_ <= kJoint8MaxN && shape.waves_per_simd == 8) {
-                hipLaunchKernelGGL((k_batch_joint<N_, N_ <= kJoint8MaxN ? 8 : 1>), dim3(grid), dim3(TT_BLOCK), 0, s,
-                                   trees, items, n_items, queue, xoff, item_out, mode);
+                two ? go(k_batch_joint<N_, N_ <= kJoint8MaxN ? 8 : 1, 2>)
+                    : go(k_batch_joint<N_, N_ <= kJoint8MaxN ? 8 : 1, 1>);
                 return hipGetLastError();
             }
         }
-        hipLaunchKernelGGL((k_batch_joint<N_, 1>), dim3(grid), dim3(TT_BLOCK), 0, s, trees, items, n_items, queue,
-                           xoff, item_out, mode);
+        two ? go(k_batch_joint<N_, 1, 2>) : go(k_batch_joint<N_, 1, 1>);
     });
     return hipGetLastError();
 }
 
-int ttk::batch_joint_max_blocks_per_cu(int n, int waves_per_simd) {
-    auto f = [&]() -> int {
+int ttk::batch_joint_max_blocks_per_cu(int n, int waves_per_simd, int tiles) {
+    auto occ = [](const void *fn) {
         int per_cu = 0;
-        TT_BATCH_DISPATCH(n, {
-            if constexpr (all_shapes(N_)) {
-                if (waves_per_simd == 6) {
-                    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_joint<N_, 6>, TT_BLOCK, 0);
-                    return std::max(1, per_cu);
-                }
-                if (N_ <= kJoint8MaxN && waves_per_simd == 8) {
-                    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &per_cu, k_batch_joint<N_, N_ <= kJoint8MaxN ? 8 : 1>, TT_BLOCK, 0);
-                    return std::max(1, per_cu);
-                }
-            }
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_joint<N_, 1>, TT_BLOCK, 0);
-        });
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TT_BLOCK, 0);
         return std::max(1, per_cu);
+    };
+    auto f = [&]() -> int {
+        TT_BATCH_DISPATCH(n, {
+            const bool two = tiles == 2;
+            if constexpr (all_shapes(N_)) {
+                if (waves_per_simd == 6)
+                    return occ(two ? (const void *)k_batch_joint<N_, 6, 2> : (const void *)k_batch_joint<N_, 6, 1>);
+                if (N_ <= kJoint8MaxN && waves_per_simd == 8)
+                    return occ(two ? (const void *)k_batch_joint<N_, N_ <= kJoint8MaxN ? 8 : 1, 2>
+                                   : (const void *)k_batch_joint<N_, N_ <= kJoint8MaxN ? 8 : 1, 1>);
+            }
+            return occ(two ? (const void *)k_batch_joint<N_, 1, 2> : (const void *)k_batch_joint<N_, 1, 1>);
+        });
+        return 1;
     };
     return f();
 }

@@ -247,10 +247,10 @@ struct BatchShape {
 namespace ttk {
 hipError_t batch_marginal(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
                           const int *xoff, double *item_out, int tips, const BatchShape &shape, hipStream_t s);
-// joint reconstruction of many trees in one persistent launch (one-tile block items)
+// joint reconstruction of many trees in one persistent launch (block items of one or two tiles)
 hipError_t batch_joint(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
                        const int *xoff, double *item_out, int mode, const BatchShape &shape, hipStream_t s);
-int batch_joint_max_blocks_per_cu(int n, int waves_per_simd);
+int batch_joint_max_blocks_per_cu(int n, int waves_per_simd, int tiles);
 // occupancy limit (blocks per CU) of the batched kernel for n states and item shape; 0 if n unsupported
 int batch_max_blocks_per_cu(int n, const BatchShape &shape);
 // false when only the one-tile block kernel exists for n states
