@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--no-single", action="store_true", help="skip the single-tree (drop-in) passes")
     ap.add_argument("--workers", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="time the batched launches only (no per-tree plain passes; for counter runs)")
     ap.add_argument("--no-compress", action="store_true", help="skip the pattern-compression measurement")
     ap.add_argument("--no-site", action="store_true", help="skip the site-specific model measurement")
     ap.add_argument("--serve-rate", default="250,1000,4000,16000",
@@ -723,6 +725,7 @@ def main():
         torch.cuda.synchronize()
         ms_joint = j0.elapsed_time(j1) / args.plain_steps
         log(f"[bench] joint, all trees: {ms_joint:.3f} ms/step")
+    if args.plain_steps > 0 and not args.batched_only:
         # the same joint passes one tree at a time (each tree's captured graph)
         for t in trees:
             t.joint(stream=sp, want_logL=False)
@@ -735,12 +738,12 @@ def main():
         torch.cuda.synchronize()
         ms_joint_plain = j0.elapsed_time(j1) / args.plain_steps
         log(f"[bench] joint, plain: {ms_joint_plain:.3f} ms/step")
-        for t in trees:  # return the joint buffers (tens of GB on the mix) before the serving leg
-            t.trim()
+    for t in trees:  # return the joint buffers (tens of GB on the mix) before the serving leg
+        t.trim()
 
     # plain: one tree per launch sequence, trees in sequence
     ms_plain = float("nan")
-    if args.plain_steps > 0:
+    if args.plain_steps > 0 and not args.batched_only:
         for t in trees:
             t.marginal(stream=sp, want_logL=False)
         torch.cuda.synchronize()

@@ -743,8 +743,8 @@ int32_t tune_batch(BatchPlan *P, int n, int tips, hipStream_t s) {
     // the joint kernel has block items of one or two tiles, with the
     // compiler's register budget or one that fits 6 or 8 waves per SIMD
     const std::vector<BatchShape> kinds =
-        P->joint ? std::vector<BatchShape>{{0, 0, 2, 8}, {0, 0, 1, 8}, {0, 0, 2, 6}, {0, 0, 1, 6}, {0, 0, 2, 0},
-                                           {0, 0, 1, 0}}
+        P->joint ? std::vector<BatchShape>{{0, 0, 1, 8}, {0, 0, 2, 8}, {0, 0, 1, 6}, {0, 0, 2, 6}, {0, 0, 1, 0},
+                                           {0, 0, 2, 0}}
                  : std::vector<BatchShape>{{0, 0, 2, 0}, {0, 0, 2, 5}, {0, 0, 1, 0}, {1, 0, 1, 0}};
     // a pinned joint register budget must exist for this alphabet: refuse it
     // rather than silently falling back to another launch path
