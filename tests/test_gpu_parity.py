@@ -523,23 +523,26 @@ ONE_LAUNCH_CASES = [c for c in CASES if c[3] in (4, 5) and c[1] <= 1000]
 
 
 @pytest.mark.parametrize("case", ONE_LAUNCH_CASES, ids=[c[0] for c in ONE_LAUNCH_CASES])
+@pytest.mark.parametrize("reuse", ["0", "1"], ids=["rows", "reused"])
 @pytest.mark.parametrize("stage", ["1", "0"], ids=["staged", "streamed"])
-def test_one_launch_pass_equals_level_kernels(gpu_engine, case, stage, monkeypatch):
+def test_one_launch_pass_equals_level_kernels(gpu_engine, case, stage, reuse, monkeypatch):
     """The one-launch pass with LDS-resident rows (TT_TILE=1), with its
-    records staged in LDS or streamed from memory, gives the level kernels'
-    profiles, subtree rows and states bit for bit, and their logL to
-    rounding, with and without reconstructed tips and for the
-    log-likelihood pass."""
-    say(f"[one-launch] case={case[0]} stage={stage}")
+    records staged in LDS or streamed from memory, and with one LDS slot per
+    internal row or slots reused by lifetime (subtree rows re-read from
+    memory in the preorder), gives the level kernels' profiles, subtree rows
+    and states bit for bit, and their logL to rounding, with and without
+    reconstructed tips and for the log-likelihood pass."""
+    say(f"[one-launch] case={case[0]} stage={stage} reuse={reuse}")
     job, prep, alphabet = make_case(*case)
     monkeypatch.setenv("TT_TILE", "0")
     lv = device_tree(gpu_engine, prep)
     monkeypatch.setenv("TT_TILE", "1")
     monkeypatch.setenv("TT_TILE_STAGE", stage)
+    monkeypatch.setenv("TT_TILE_REUSE", reuse)
     one = device_tree(gpu_engine, prep)
-    assert lv.pass_shape() == (0, False)
-    tp, staged = one.pass_shape()
-    assert tp > 0 and staged == (stage == "1"), (tp, staged)
+    assert lv.pass_shape() == (0, False, False)
+    tp, staged, reused = one.pass_shape()
+    assert tp > 0 and staged == (stage == "1") and reused == (reuse == "1"), (tp, staged, reused)
     for tips in (False, True):
         a, b = lv.marginal(reconstruct_tips=tips), one.marginal(reconstruct_tips=tips)
         assert abs(a - b) <= 1e-12 * abs(a), (tips, a, b)

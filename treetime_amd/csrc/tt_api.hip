@@ -177,6 +177,9 @@ struct tt_tree {
     unsigned *d_tile_counter = nullptr;
     bool tile_stale = true;  // the records do not hold the current P(t) yet
     bool tile_staged = false;  // the kernel stages each phase's records in LDS
+    bool tile_subg = false;    // LDS slots are reused; the preorder re-reads subtree rows from memory
+    int tile_slots = 0, tile_root_post = 0, tile_root_pre = 0;
+    int32_t *d_tile_post_slot = nullptr, *d_tile_pre_slot = nullptr;  // slot of every internal row / node
     bool have_states = false;  // d.states holds a marginal or joint reconstruction
     bool tips_dirty = false;   // tip rows of d.states hold reconstructed states (else 255)
     bool profiles_fresh = false;  // d.prof and d.sub belong to the current lengths (last marginal pass)
@@ -479,10 +482,11 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
     const int nb = ttk::root_blocks(T->P);
     if ((mode == 1 || mode == 3) && T->tile_tp && !T->site) {
         if (mode == 1 && fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
-        TT_HIP(ttk::tile_marginal(D, T->d_tile_post_rec, T->d_tile_pre_rec, T->d_tile_rounds, T->tile_Rp,
-                                  T->tile_Ri, T->tile_Ra, T->tile_tp, (int)T->post_nodes.size(),
-                                  (int)T->pre_nodes.size(), T->tile_staged, T->d_tile_partial, T->d_tile_counter,
-                                  tips ? 1 : 0, mode == 1 ? 1 : 0, s));
+        ttk::TileArgs A{T->d_tile_post_rec, T->d_tile_pre_rec, T->d_tile_rounds, T->tile_Rp, T->tile_Ri,
+                        T->tile_Ra, T->tile_tp, (int)T->post_nodes.size(), (int)T->pre_nodes.size(),
+                        T->tile_slots, T->tile_root_post, T->tile_root_pre, T->d_tile_post_slot, T->tile_staged,
+                        T->tile_subg, T->d_tile_partial, T->d_tile_counter};
+        TT_HIP(ttk::tile_marginal(D, A, tips ? 1 : 0, mode == 1 ? 1 : 0, s));
     } else if (mode == 1 || mode == 3) {
         size_t ri = 0;
         for (size_t h = 0; h < T->post_lv.size();) {
@@ -575,7 +579,8 @@ int32_t run_pass(tt_tree *T, int mode, uint32_t flags, double *logL, void *strea
     const int key = mode * 8 + (tips ? 1 : 0) + (fill ? 2 : 0);
     if (T->tile_tp && !T->site && (mode == 1 || mode == 3) && T->tile_stale) {
         TT_HIP(ttk::tile_pack(T->d, T->d_post_desc, (int)T->post_nodes.size(), T->d_pre_desc,
-                              (int)T->pre_nodes.size(), T->d_tile_post_rec, T->d_tile_pre_rec, s));
+                              (int)T->pre_nodes.size(), T->d_tile_post_slot, T->d_tile_pre_slot, T->tile_subg,
+                              T->d_tile_post_rec, T->d_tile_pre_rec, s));
         T->tile_stale = false;
     }
     if (g_use_graphs()) {
@@ -834,6 +839,69 @@ void tile_rounds(const tt_tree *T, int cap, std::vector<int32_t> &out, int &Rp, 
     Ra = (int)out.size() / 2 - Rp - Ri;
 }
 
+// LDS slots of the one-launch pass with reuse (level-based lifetimes): an
+// internal row occupies its slot from its postorder level to its parent's
+// (the root's to the end); an internal node's profile from its preorder
+// depth to the next depth (its children read it).  A slot is handed to a new
+// row only at a level after the last one that reads the old row.  Returns
+// the number of slots of each phase; post_slot is indexed by row, pre_slot
+// by node (-1 for tips).
+void tile_slots(const tt_tree *T, std::vector<int32_t> &post_slot, std::vector<int32_t> &pre_slot, int &n_post,
+                int &n_pre) {
+    auto assign = [](int n_levels, const std::vector<std::vector<int>> &by_level, const std::vector<int> &last,
+                     std::vector<int32_t> &slot) {
+        // active: (last level that reads it, slot)
+        std::vector<std::pair<int, int>> active;
+        std::vector<int> free_slots;
+        int count = 0;
+        for (int l = 0; l < n_levels; ++l) {
+            std::vector<std::pair<int, int>> keep;
+            for (auto &a : active) (a.first < l ? free_slots.push_back(a.second) : keep.push_back(a));
+            active.swap(keep);
+            for (int x : by_level[l]) {
+                int sl;
+                if (!free_slots.empty()) {
+                    sl = free_slots.back();
+                    free_slots.pop_back();
+                } else {
+                    sl = count++;
+                }
+                slot[x] = sl;
+                active.push_back({last[x], sl});
+            }
+        }
+        return count;
+    };
+    const int H = (int)T->post_lv.size();
+    std::vector<int> lvl(T->n_int, 0), last(T->n_int, H);
+    std::vector<std::vector<int>> by_level(H);
+    for (int h = 0; h < H; ++h)
+        for (int k = T->post_lv[h].off; k < T->post_lv[h].off + T->post_lv[h].cnt; ++k) {
+            lvl[T->post_nodes[k]] = h;
+            by_level[h].push_back(T->post_nodes[k]);
+        }
+    for (int r = 0; r < T->n_int; ++r) {
+        const int par = T->parent[T->int_node[r]];
+        last[r] = par < 0 ? H : lvl[T->int_row[par]];
+    }
+    post_slot.assign(T->n_int, -1);
+    n_post = assign(H, by_level, last, post_slot);
+    // preorder: depth 0 is the root, depth d + 1 the nodes of pre_lv[d]
+    const int D = (int)T->pre_lv.size();
+    std::vector<std::vector<int>> by_depth(D + 1);
+    std::vector<int> lastp(T->n_nodes, 0);
+    by_depth[0].push_back(T->root);
+    lastp[T->root] = 1;
+    for (int d = 0; d < D; ++d)
+        for (int k = T->pre_lv[d].off; k < T->pre_lv[d].off + T->pre_lv[d].n_int; ++k) {
+            const int c = T->pre_nodes[k];
+            by_depth[d + 1].push_back(c);
+            lastp[c] = d + 2;
+        }
+    pre_slot.assign(T->n_nodes, -1);
+    n_pre = assign(D + 1, by_depth, lastp, pre_slot);
+}
+
 // Choose the one-launch LDS-resident pass for a tree (nucleotide-sized
 // alphabets, shared model) and its power-of-two pattern tile: rows of the
 // tile in a workgroup's LDS, and every workgroup resident at once (beyond
@@ -854,40 +922,56 @@ int32_t setup_tile(tt_tree *T) {
     // a chunk of a wide one) costs about a microsecond of one wave's latency
     // chain, a quarter more when its records stream from memory; staging the
     // records costs every workgroup a copy of them through L2 (about 2 TB/s
-    // for the whole device); a second wave of workgroups runs the rounds
-    // twice.  Measured on the 200-tip tree: 8 patterns per workgroup with
-    // staged records beat 2 (staging traffic) and 32 (rounds).
+    // for the whole device); re-reading subtree rows from memory (slots
+    // reused) adds about half a microsecond per preorder round; a second wave
+    // of workgroups runs the rounds twice.  Measured on the 200-tip tree: 8
+    // patterns per workgroup with staged records beat 2 (staging traffic) and
+    // 32 (rounds).  Slots: one per internal row (the preorder reads subtree
+    // rows from LDS) or reused by lifetime (fewer rows per pattern, so larger
+    // trees fit; tile_slots).
     // TT_TILE=1 also allows two waves of workgroups; TT_TILE=0 turns the path
-    // off; TT_TILE_STAGE=0 never stages the records; TT_TILE_TP fixes the tile.
+    // off; TT_TILE_STAGE=0 never stages the records; TT_TILE_TP fixes the
+    // tile; TT_TILE_REUSE=0 / 1 never / always reuses slots.
     const int kTileMaxWaves = forced ? 2 : 1;
     const char *se = getenv("TT_TILE_STAGE");
     const bool may_stage = !(se && se[0] == '0');
     const char *te = getenv("TT_TILE_TP");
     const int only_c = te ? atoi(te) : 0;
+    const char *re = getenv("TT_TILE_REUSE");
+    const int reuse_pin = re && re[0] ? (re[0] == '1' ? 1 : 0) : -1;
     const size_t stage_words = ttk::tile_stage_words(T->n, (int)T->post_nodes.size(), (int)T->pre_nodes.size());
+    std::vector<int32_t> post_live, pre_live;
+    int n_post_live = 0, n_pre_live = 0;
+    tile_slots(T, post_live, pre_live, n_post_live, n_pre_live);
+    const int live_slots = std::max(n_post_live, n_pre_live);
     std::vector<int32_t> rounds;
     int Rp = 0, Ri = 0, Ra = 0, tp = 0;
-    bool staged = false;
+    bool staged = false, subg = false;
     double best_us = 1e300;
     for (int c = 1; c <= 64; c *= 2) {
         if (only_c > 0 && c != only_c) continue;
         tile_rounds(T, ttk::tile_threads() / c, rounds, Rp, Ri, Ra);
         bool fits = false;
-        for (int st = may_stage ? 1 : 0; st >= 0; --st) {
-            const size_t lds = ttk::tile_lds_bytes(T->d, c, (int)rounds.size(), st ? stage_words : 0);
-            if (lds > budget) continue;
-            const int bpc = ttk::tile_blocks_per_cu(T->n, lds, st != 0);
-            if (bpc < 1) continue;
-            fits = true;
-            const long blocks = (T->P + c - 1) / c;
-            const long waves = (blocks + (long)bpc * n_cu - 1) / ((long)bpc * n_cu);
-            if (waves > kTileMaxWaves) continue;
-            const double us = (double)waves * (Rp + Ri) * (st ? 1.0 : 1.25) +
-                              (st ? (double)blocks * stage_words * 4.0 / 2.0e6 : 0.0);
-            if (us < best_us) {
-                best_us = us;
-                tp = c;
-                staged = st != 0;
+        for (int rs = 0; rs < 2; ++rs) {
+            if (reuse_pin >= 0 && rs != reuse_pin) continue;
+            const int n_slots = rs ? live_slots : T->n_int;
+            for (int st = may_stage ? 1 : 0; st >= 0; --st) {
+                const size_t lds = ttk::tile_lds_bytes(T->d, c, n_slots, (int)rounds.size(), st ? stage_words : 0);
+                if (lds > budget) continue;
+                const int bpc = ttk::tile_blocks_per_cu(T->n, lds, st != 0, rs != 0);
+                if (bpc < 1) continue;
+                fits = true;
+                const long blocks = (T->P + c - 1) / c;
+                const long waves = (blocks + (long)bpc * n_cu - 1) / ((long)bpc * n_cu);
+                if (waves > kTileMaxWaves) continue;
+                const double us = (double)waves * ((Rp + Ri) * (st ? 1.0 : 1.25) + (rs ? 0.5 * Ri : 0.0)) +
+                                  (st ? (double)blocks * stage_words * 4.0 / 2.0e6 : 0.0);
+                if (us < best_us) {
+                    best_us = us;
+                    tp = c;
+                    staged = st != 0;
+                    subg = rs != 0;
+                }
             }
         }
         if (!fits) break;  // no larger tile fits either
@@ -895,8 +979,25 @@ int32_t setup_tile(tt_tree *T) {
     if (!tp) return TT_OK;
     const int blocks = (T->P + tp - 1) / tp;
     tile_rounds(T, ttk::tile_threads() / tp, rounds, Rp, Ri, Ra);
+    // slot maps: reused by lifetime, or one slot per internal row (the
+    // preorder then finds each node's subtree row in its own slot)
+    std::vector<int32_t> post_slot, pre_slot;
+    if (subg) {
+        post_slot = post_live;
+        pre_slot = pre_live;
+        T->tile_slots = live_slots;
+    } else {
+        post_slot.resize(T->n_int);
+        for (int r = 0; r < T->n_int; ++r) post_slot[r] = r;
+        pre_slot.assign(T->int_row.begin(), T->int_row.end());
+        T->tile_slots = T->n_int;
+    }
+    T->tile_root_post = post_slot[T->root_row];
+    T->tile_root_pre = pre_slot[T->root];
     int32_t rc;
     if ((rc = T->upload(&T->d_tile_rounds, rounds.data(), rounds.size()))) return rc;
+    if ((rc = T->upload(&T->d_tile_post_slot, post_slot.data(), post_slot.size()))) return rc;
+    if ((rc = T->upload(&T->d_tile_pre_slot, pre_slot.data(), pre_slot.size()))) return rc;
     if ((rc = T->alloc((void **)&T->d_tile_post_rec,
                        (size_t)T->post_nodes.size() * ttk::tile_post_rec_words(T->n) * sizeof(int32_t))))
         return rc;
@@ -910,6 +1011,7 @@ int32_t setup_tile(tt_tree *T) {
     T->tile_Ra = Ra;
     T->tile_tp = tp;
     T->tile_staged = staged;
+    T->tile_subg = subg;
     T->tile_stale = true;
     return TT_OK;
 }
@@ -1116,11 +1218,13 @@ int32_t tt_tree_info(const tt_tree *T, int32_t *n_levels_post, int32_t *n_levels
     return TT_OK;
 }
 
-int32_t tt_tree_pass_shape(const tt_tree *T, int32_t *tile_patterns, int32_t *records_staged) {
+int32_t tt_tree_pass_shape(const tt_tree *T, int32_t *tile_patterns, int32_t *records_staged,
+                           int32_t *slots_reused) {
     if (!T) return fail(TT_EINVAL, "tree is NULL");
     const bool tile = T->tile_tp && !T->site;
     if (tile_patterns) *tile_patterns = tile ? T->tile_tp : 0;
     if (records_staged) *records_staged = tile && T->tile_staged ? 1 : 0;
+    if (slots_reused) *slots_reused = tile && T->tile_subg ? 1 : 0;
     return TT_OK;
 }
 

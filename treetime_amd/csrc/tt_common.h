@@ -161,13 +161,22 @@ int tile_threads();
 int tile_post_rec_words(int n);
 int tile_pre_rec_words(int n);
 size_t tile_stage_words(int n, int n_post, int n_pre);
-size_t tile_lds_bytes(const TreeDev &T, int tp, int n_rounds, size_t stage_words);
-int tile_blocks_per_cu(int n, size_t lds, bool staged);
+size_t tile_lds_bytes(const TreeDev &T, int tp, int n_slots, int n_rounds, size_t stage_words);
+int tile_blocks_per_cu(int n, size_t lds, bool staged, bool subg);
 hipError_t tile_pack(const TreeDev &T, const int32_t *post_desc, int n_post, const int32_t *pre_desc, int n_pre,
-                     int32_t *post_rec, int32_t *pre_rec, hipStream_t s);
-hipError_t tile_marginal(const TreeDev &T, const int32_t *post_rec, const int32_t *pre_rec, const int32_t *rounds,
-                         int Rp, int Ri, int Ra, int tp, int n_post, int n_pre, bool staged, double *partial,
-                         unsigned *counter, int tips, int pre, hipStream_t s);
+                     const int32_t *post_slot, const int32_t *pre_slot, bool subg, int32_t *post_rec,
+                     int32_t *pre_rec, hipStream_t s);
+// arguments of one one-launch pass
+struct TileArgs {
+    const int32_t *post_rec, *pre_rec, *rounds;
+    int Rp, Ri, Ra, tp, n_post, n_pre;
+    int n_slots, root_post, root_pre;  // LDS rows per pattern; the root's slot in each phase
+    const int32_t *post_slot;          // slot of every internal row
+    bool staged, subg;                 // records staged in LDS; slots reused (subtree rows re-read from memory)
+    double *partial;
+    unsigned *counter;
+};
+hipError_t tile_marginal(const TreeDev &T, const TileArgs &A, int tips, int pre, hipStream_t s);
 // sets tt_last_error()'s text on the calling thread and returns code (tt_api.hip)
 int32_t set_error(int32_t code, const char *msg);
 // joint branch-length optimisation (tt_branch.hip)

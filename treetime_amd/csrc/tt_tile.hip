@@ -45,16 +45,19 @@ __host__ __device__ inline size_t align4(size_t x) { return (x + 3) & ~(size_t)3
 
 // record sizes in 32-bit words, multiples of 4 (16-byte loads)
 __host__ __device__ constexpr int post_rec_words(int n) { return (8 + 2 * n * n + 3) & ~3; }
-__host__ __device__ constexpr int pre_rec_words(int n) { return (4 + n * n + 3) & ~3; }
+__host__ __device__ constexpr int pre_rec_words(int n) { return (8 + n * n + 3) & ~3; }
 
 struct TileDev {
-    const int32_t *post_rec;    // per height-ordered internal row: PostDesc, P - I of child 0 and child 1
-    const int32_t *pre_rec;     // per depth-ordered node: {node, parent's row, row, tip row}, P(t)
+    const int32_t *post_rec;    // per height-ordered internal row: {row, slot, children, child 0}, {child 1, ..}, P - I of both children
+    const int32_t *pre_rec;     // per depth-ordered node: {node, parent's slot, own slot, tip row}, {row, row's slot}, P(t)
     const int32_t *rounds;      // rounds as [first, end) entry pairs: post [Rp], preorder internal [Ri], all [Ra]
     int32_t Rp, Ri, Ra;
     int32_t TP;                 // patterns per workgroup (power of two)
     int32_t n_blocks;
     int32_t n_post, n_pre;      // records of each phase (staged into LDS when the kernel is built STAGED)
+    int32_t n_slots;            // LDS rows per pattern
+    int32_t root_post, root_pre;  // the root's slot in each phase
+    const int32_t *post_slot;   // slot of every internal row (polytomies' child lists), or null: slot = row
     double *partial;            // [n_blocks] sum over the workgroup's patterns of mult * logL
     unsigned *counter;          // workgroups finished; reset to 0 by the last one
 };
@@ -75,7 +78,8 @@ struct PostRec {
 
 template <int N>
 struct PreRec {
-    int4 a;  // node, parent's row, row (or -1), tip row (or -1)
+    int4 a;  // node, parent's slot, own slot (or -1), tip row (or -1)
+    int4 b;  // internal row (or -1), the slot holding its subtree row (when not read from memory)
     float M[N * N];
 };
 
@@ -94,8 +98,9 @@ __device__ __forceinline__ void load_rec(const int32_t *base, PostRec<N> &r) {
 
 template <int N>
 __device__ __forceinline__ void load_rec(const int32_t *base, PreRec<N> &r) {
-    r.a = *reinterpret_cast<const int4 *>(base);
-    const float *f = reinterpret_cast<const float *>(base + 4);
+    r.a = reinterpret_cast<const int4 *>(base)[0];
+    r.b = reinterpret_cast<const int4 *>(base)[1];
+    const float *f = reinterpret_cast<const float *>(base + 8);
 #pragma unroll
     for (int i = 0; i < N * N; ++i) r.M[i] = f[i];
 }
@@ -107,15 +112,18 @@ __device__ __forceinline__ void stage_records(int32_t *dst, const int32_t *src, 
     for (size_t i = threadIdx.x; i < n_words / 4; i += kTileThreads) d4[i] = s4[i];
 }
 
-template <int N, bool STAGED>
+// STAGED: each phase's records copied into LDS; SUBG: slots are reused once
+// their rows are dead (fewer rows per pattern in LDS), so the preorder reads
+// each node's subtree row back from memory instead of from its slot.
+template <int N, bool STAGED, bool SUBG>
 __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileDev S, int tips, int pre) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     const int TP = S.TP;
     // LDS layout (tile_lds_bytes): rows, profile map, reduction, round
     // offsets, tip codes, staged records; each part starts on a 16-byte
     // boundary
-    float *rows = reinterpret_cast<float *>(s_dyn);                                      // [n_int][TP][N]
-    float *s_lut = rows + align4((size_t)T.n_int * TP * N);                              // [n_codes][N]
+    float *rows = reinterpret_cast<float *>(s_dyn);                                      // [n_slots][TP][N]
+    float *s_lut = rows + align4((size_t)S.n_slots * TP * N);                            // [n_codes][N]
     double *s_red = reinterpret_cast<double *>(s_lut + align4((size_t)T.n_codes * N));  // [kTileThreads]
     int *s_rounds = reinterpret_cast<int *>(s_red + kTileThreads);
     const int n_rounds = 2 * (S.Rp + S.Ri + S.Ra);
@@ -147,7 +155,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
     // a, b: the record's descriptor words; D0, D1: the children's P - I
     // (registers, or the staged record in LDS)
     auto post_entry = [&](const int4 &a, const int4 &b, auto D0, auto D1) {
-        const int v = a.x, nchild = a.z, cref0 = a.w, cref1 = b.x;
+        // row v, its slot vs; children as slots (>= 0) or tips (< 0)
+        const int v = a.x, vs = a.y, nchild = a.z, cref0 = a.w, cref1 = b.x;
         if (nchild == 2) {
             BinRaw<N> R;
 #pragma unroll
@@ -162,7 +171,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
                     for (int i = 0; i < N; ++i) R.row[c][i] = src[i];
                 }
             }
-            bin_finish_m<N>(T, v, cref0, cref1, D0, D1, p, R, s_lut, acc, slot(v));
+            bin_finish_m<N>(T, v, cref0, cref1, D0, D1, p, R, s_lut, acc, slot(vs));
         } else {  // polytomies and unary nodes read their child lists from memory
             auto rowf = [&](int cref, float (&row)[N]) {
                 if (cref < 0) {
@@ -170,12 +179,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
 #pragma unroll
                     for (int i = 0; i < N; ++i) row[i] = s_lut[code * N + i];
                 } else {
-                    const float *src = slot(cref);
+                    const float *src = slot(S.post_slot ? S.post_slot[cref] : cref);
 #pragma unroll
                     for (int i = 0; i < N; ++i) row[i] = src[i];
                 }
             };
-            post_node_f<N>(T, v, p, s_lut, acc, rowf, [](int e) { return e; }, slot(v));
+            post_node_f<N>(T, v, p, s_lut, acc, rowf, [](int e) { return e; }, slot(vs));
         }
     };
     {
@@ -219,7 +228,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
     if (t < TP && act) {
         double off = 0.0;
         for (int k = q; k < kTileThreads; k += TP) off += s_red[k];
-        float *rr = slot(T.root_row);
+        const float *rr = slot(S.root_post);
         double x[N], s = 0.0;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
@@ -243,40 +252,51 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
         if (pre) {
             store_row<N>(T.prof + ((size_t)T.root * T.P + p) * N, out);
             T.states[(size_t)T.root * T.P + p] = (uint8_t)jm;
+            float *rp = slot(S.root_pre);
 #pragma unroll
-            for (int j = 0; j < N; ++j) rr[j] = out[j];
+            for (int j = 0; j < N; ++j) rp[j] = out[j];
         }
         s_red[t] = T.mult[p] * lp;
     }
-    lds_barrier();
+    // the preorder reads the subtree rows back from memory when slots were
+    // reused: a full barrier makes the postorder's row stores visible
+    if constexpr (SUBG)
+        __syncthreads();
+    else
+        lds_barrier();
 
     // ---- preorder: rounds over depth-ordered nodes -------------------------------
     if (pre) {
         constexpr int W = pre_rec_words(N);
         const int *rd = s_rounds + 2 * (tips ? S.Rp + S.Ri : S.Rp);
         const int R = tips ? S.Ra : S.Ri;
-        auto pre_entry = [&](const int4 &a, auto M) {
-            const int c = a.x, prow = a.y, ir = a.z, tr = a.w;
+        auto pre_entry = [&](const int4 &a, const int4 &b, auto M) {
+            const int c = a.x, pslot = a.y, oslot = a.z, tr = a.w, ir = b.x, sslot = b.y;
             PreRaw<N> Rw;
-            const float *pp = slot(prow);
-#pragma unroll
-            for (int i = 0; i < N; ++i) Rw.pp[i] = pp[i];
             if (ir >= 0) {
                 Rw.code = 0;
-                const float *src = slot(ir);
+                if constexpr (SUBG) {
+                    load_rowf<N>(T.sub + ((size_t)ir * T.P + p) * N, Rw.sub);
+                } else {
+                    const float *src = slot(sslot);
 #pragma unroll
-                for (int i = 0; i < N; ++i) Rw.sub[i] = src[i];
+                    for (int i = 0; i < N; ++i) Rw.sub[i] = src[i];
+                }
             } else {
                 Rw.code = s_tip[tr * TP + q];
             }
-            pre_finish_m<N>(T, c, ir, p, Rw, s_lut, M, ir >= 0 ? slot(ir) : nullptr);
+            const float *pp = slot(pslot);
+#pragma unroll
+            for (int i = 0; i < N; ++i) Rw.pp[i] = pp[i];
+            pre_finish_m<N>(T, c, ir, p, Rw, s_lut, M, oslot >= 0 ? slot(oslot) : nullptr);
         };
         if constexpr (STAGED) {
             for (int r = 0; r < R; ++r) {
                 const int k = rd[2 * r] + k0;
                 if (k < rd[2 * r + 1] && act) {
                     const int32_t *rec = s_rec + (size_t)W * k;
-                    pre_entry(reinterpret_cast<const int4 *>(rec)[0], reinterpret_cast<const float *>(rec + 4));
+                    pre_entry(reinterpret_cast<const int4 *>(rec)[0], reinterpret_cast<const int4 *>(rec)[1],
+                              reinterpret_cast<const float *>(rec + 8));
                 }
                 lds_barrier();
             }
@@ -289,7 +309,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
                 const int kn = r + 1 < R ? rd[2 * r + 2] + k0 : 0;
                 const bool vnext = r + 1 < R && kn < rd[2 * r + 3];
                 if (vnext) load_rec<N>(S.pre_rec + (size_t)W * kn, nxt);
-                if (valid && act) pre_entry(cur.a, cur.M);
+                if (valid && act) pre_entry(cur.a, cur.b, cur.M);
                 lds_barrier();
                 if (vnext) cur = nxt;
                 k = kn;
@@ -333,14 +353,20 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
 
 // Records of the tile pass from the level lists and the current P(t): one
 // thread per post entry (g < n_post) or pre entry.
+// post_slot: slot of every internal row; pre_slot: slot of every node's
+// profile (-1 for tips).
 template <int N>
 __global__ void k_tile_pack(TreeDev T, const int32_t *post_desc, int n_post, const int32_t *pre_desc, int n_pre,
-                            int32_t *post_rec, int32_t *pre_rec) {
+                            const int32_t *post_slot, const int32_t *pre_slot, int subg, int32_t *post_rec,
+                            int32_t *pre_rec) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n_post) {
-        const int32_t *d = post_desc + kPostDesc * g;
+        const int32_t *d = post_desc + kPostDesc * g;  // {row, first edge, children, child 0, child 1, ...}
         int32_t *o = post_rec + (size_t)post_rec_words(N) * g;
         for (int i = 0; i < kPostDesc; ++i) o[i] = d[i];
+        o[1] = post_slot[d[0]];
+        o[3] = d[3] >= 0 ? post_slot[d[3]] : d[3];
+        o[4] = d[4] >= 0 ? post_slot[d[4]] : d[4];
         float *f = reinterpret_cast<float *>(o + 8);
         const bool bin = d[2] == 2;
         for (int i = 0; i < N * N; ++i) {
@@ -349,13 +375,17 @@ __global__ void k_tile_pack(TreeDev T, const int32_t *post_desc, int n_post, con
         }
     } else if (g < n_post + n_pre) {
         const int k = g - n_post;
-        const int32_t *d = pre_desc + kPreDesc * k;
+        const int32_t *d = pre_desc + kPreDesc * k;  // {node, parent node, row or -1, tip row or -1}
         int32_t *o = pre_rec + (size_t)pre_rec_words(N) * k;
         o[0] = d[0];
-        o[1] = T.int_row[d[1]];
-        o[2] = d[2];
+        o[1] = pre_slot[d[1]];
+        o[2] = d[2] >= 0 ? pre_slot[d[0]] : -1;
         o[3] = d[3];
-        float *f = reinterpret_cast<float *>(o + 4);
+        o[4] = d[2];
+        o[5] = d[2] >= 0 && !subg ? post_slot[d[2]] : -1;
+        o[6] = 0;
+        o[7] = 0;
+        float *f = reinterpret_cast<float *>(o + 8);
         for (int i = 0; i < N * N; ++i) f[i] = T.P32[(size_t)d[0] * (N * N) + i];
     }
 }
@@ -373,10 +403,11 @@ size_t tile_stage_words(int n, int n_post, int n_pre) {
     return std::max((size_t)n_post * post_rec_words(n), (size_t)n_pre * pre_rec_words(n));
 }
 
-// LDS bytes of k_tile_marginal for tp patterns per workgroup and n_rounds
-// round bounds (two per round) in all, with stage_words of staged records.
-size_t tile_lds_bytes(const TreeDev &T, int tp, int n_rounds, size_t stage_words) {
-    const size_t rows = align4((size_t)T.n_int * tp * T.n) * sizeof(float);
+// LDS bytes of k_tile_marginal for tp patterns per workgroup, n_slots rows
+// per pattern and n_rounds round bounds (two per round) in all, with
+// stage_words of staged records.
+size_t tile_lds_bytes(const TreeDev &T, int tp, int n_slots, int n_rounds, size_t stage_words) {
+    const size_t rows = align4((size_t)n_slots * tp * T.n) * sizeof(float);
     const size_t lut = align4((size_t)T.n_codes * T.n) * sizeof(float);
     const size_t red = (size_t)kTileThreads * sizeof(double);
     const size_t rounds = align4((size_t)n_rounds) * sizeof(int);
@@ -394,14 +425,16 @@ size_t tile_lds_bytes(const TreeDev &T, int tp, int n_rounds, size_t stage_words
     }
 
 template <int N>
-const void *tile_kernel(bool staged) {
-    return staged ? (const void *)k_tile_marginal<N, true> : (const void *)k_tile_marginal<N, false>;
+const void *tile_kernel(bool staged, bool subg) {
+    if (subg)
+        return staged ? (const void *)k_tile_marginal<N, true, true> : (const void *)k_tile_marginal<N, false, true>;
+    return staged ? (const void *)k_tile_marginal<N, true, false> : (const void *)k_tile_marginal<N, false, false>;
 }
 
 // Workgroups of k_tile_marginal one CU holds at once (registers and LDS).
-int tile_blocks_per_cu(int n, size_t lds, bool staged) {
-    const void *fn = n == 2 ? tile_kernel<2>(staged) : n == 3 ? tile_kernel<3>(staged)
-                   : n == 4 ? tile_kernel<4>(staged) : n == 5 ? tile_kernel<5>(staged) : nullptr;
+int tile_blocks_per_cu(int n, size_t lds, bool staged, bool subg) {
+    const void *fn = n == 2 ? tile_kernel<2>(staged, subg) : n == 3 ? tile_kernel<3>(staged, subg)
+                   : n == 4 ? tile_kernel<4>(staged, subg) : n == 5 ? tile_kernel<5>(staged, subg) : nullptr;
     if (!fn) return 0;
     int b = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, kTileThreads, lds) != hipSuccess) {
@@ -412,25 +445,30 @@ int tile_blocks_per_cu(int n, size_t lds, bool staged) {
 }
 
 hipError_t tile_pack(const TreeDev &T, const int32_t *post_desc, int n_post, const int32_t *pre_desc, int n_pre,
-                     int32_t *post_rec, int32_t *pre_rec, hipStream_t s) {
+                     const int32_t *post_slot, const int32_t *pre_slot, bool subg, int32_t *post_rec,
+                     int32_t *pre_rec, hipStream_t s) {
     const int total = n_post + n_pre;
     if (!total) return hipSuccess;
     TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL(k_tile_pack<N_>, dim3((total + 255) / 256), dim3(256), 0, s, T,
-                                             post_desc, n_post, pre_desc, n_pre, post_rec, pre_rec));
+                                             post_desc, n_post, pre_desc, n_pre, post_slot, pre_slot, subg ? 1 : 0,
+                                             post_rec, pre_rec));
     return hipGetLastError();
 }
 
-hipError_t tile_marginal(const TreeDev &T, const int32_t *post_rec, const int32_t *pre_rec, const int32_t *rounds,
-                         int Rp, int Ri, int Ra, int tp, int n_post, int n_pre, bool staged, double *partial,
-                         unsigned *counter, int tips, int pre, hipStream_t s) {
-    TileDev S{post_rec, pre_rec, rounds, Rp, Ri, Ra, tp, (T.P + tp - 1) / tp, n_post, n_pre, partial, counter};
-    const size_t lds = tile_lds_bytes(T, tp, 2 * (Rp + Ri + Ra), staged ? tile_stage_words(T.n, n_post, n_pre) : 0);
-    if (staged) {
-        TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL((k_tile_marginal<N_, true>), dim3((unsigned)S.n_blocks),
-                                                 dim3(kTileThreads), lds, s, T, S, tips, pre));
+hipError_t tile_marginal(const TreeDev &T, const TileArgs &A, int tips, int pre, hipStream_t s) {
+    TileDev S{A.post_rec, A.pre_rec, A.rounds, A.Rp, A.Ri, A.Ra, A.tp, (T.P + A.tp - 1) / A.tp, A.n_post, A.n_pre,
+              A.n_slots, A.root_post, A.root_pre, A.subg ? A.post_slot : nullptr, A.partial, A.counter};
+    const size_t lds = tile_lds_bytes(T, A.tp, A.n_slots, 2 * (A.Rp + A.Ri + A.Ra),
+                                      A.staged ? tile_stage_words(T.n, A.n_post, A.n_pre) : 0);
+    const dim3 grid((unsigned)S.n_blocks), block(kTileThreads);
+    if (A.staged && A.subg) {
+        TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL((k_tile_marginal<N_, true, true>), grid, block, lds, s, T, S, tips, pre));
+    } else if (A.staged) {
+        TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL((k_tile_marginal<N_, true, false>), grid, block, lds, s, T, S, tips, pre));
+    } else if (A.subg) {
+        TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL((k_tile_marginal<N_, false, true>), grid, block, lds, s, T, S, tips, pre));
     } else {
-        TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL((k_tile_marginal<N_, false>), dim3((unsigned)S.n_blocks),
-                                                 dim3(kTileThreads), lds, s, T, S, tips, pre));
+        TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL((k_tile_marginal<N_, false, false>), grid, block, lds, s, T, S, tips, pre));
     }
     return hipGetLastError();
 }

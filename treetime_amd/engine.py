@@ -189,10 +189,11 @@ class DeviceTree:
 
     def pass_shape(self):
         """(pattern tile of the one-launch pass or 0 for the level kernels,
-        whether its records are staged in LDS)."""
-        a, b = C.c_int32(), C.c_int32()
-        _native.check(self.lib.tt_tree_pass_shape(self.handle, C.byref(a), C.byref(b)))
-        return a.value, bool(b.value)
+        whether its records are staged in LDS, whether its LDS rows are
+        reused by lifetime)."""
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        _native.check(self.lib.tt_tree_pass_shape(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, bool(b.value), bool(c.value)
 
     def marginal(self, reconstruct_tips=False, stream=None, want_logL=True, keep_normalisers=False):
         out = C.c_double(np.nan)
