@@ -922,7 +922,8 @@ int32_t setup_tile(tt_tree *T) {
     // a chunk of a wide one) costs about a microsecond of one wave's latency
     // chain, a quarter more when its records stream from memory; staging the
     // records costs every workgroup a copy of them through L2 (about 2 TB/s
-    // for the whole device); re-reading subtree rows from memory (slots
+    // for the whole device), streaming them costs every workgroup one read
+    // of them (about 4 TB/s); re-reading subtree rows from memory (slots
     // reused) adds about half a microsecond per preorder round; a second wave
     // of workgroups runs the rounds twice.  Measured on the 200-tip tree: 8
     // patterns per workgroup with staged records beat 2 (staging traffic) and
@@ -940,6 +941,9 @@ int32_t setup_tile(tt_tree *T) {
     const char *re = getenv("TT_TILE_REUSE");
     const int reuse_pin = re && re[0] ? (re[0] == '1' ? 1 : 0) : -1;
     const size_t stage_words = ttk::tile_stage_words(T->n, (int)T->post_nodes.size(), (int)T->pre_nodes.size());
+    // records every workgroup streams through L2 when they are not staged
+    const double stream_bytes = 4.0 * ((double)T->post_nodes.size() * ttk::tile_post_rec_words(T->n) +
+                                       (double)T->pre_nodes.size() * ttk::tile_pre_rec_words(T->n));
     std::vector<int32_t> post_live, pre_live;
     int n_post_live = 0, n_pre_live = 0;
     tile_slots(T, post_live, pre_live, n_post_live, n_pre_live);
@@ -965,7 +969,7 @@ int32_t setup_tile(tt_tree *T) {
                 const long waves = (blocks + (long)bpc * n_cu - 1) / ((long)bpc * n_cu);
                 if (waves > kTileMaxWaves) continue;
                 const double us = (double)waves * ((Rp + Ri) * (st ? 1.0 : 1.25) + (rs ? 0.5 * Ri : 0.0)) +
-                                  (st ? (double)blocks * stage_words * 4.0 / 2.0e6 : 0.0);
+                                  (double)blocks * (st ? stage_words * 4.0 / 2.0e6 : stream_bytes / 4.0e6);
                 if (us < best_us) {
                     best_us = us;
                     tp = c;

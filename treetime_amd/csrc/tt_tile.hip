@@ -202,18 +202,25 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
                 lds_barrier();
             }
         } else {
-            PostRec<N> cur, nxt;
-            int k = S.Rp > 0 ? rd[0] + k0 : 0;
-            if (S.Rp > 0 && k < rd[1]) load_rec<N>(S.post_rec + (size_t)W * k, cur);
-            for (int r = 0; r < S.Rp; ++r) {
-                const bool valid = k < rd[2 * r + 1];
-                const int kn = r + 1 < S.Rp ? rd[2 * r + 2] + k0 : 0;
-                const bool vnext = r + 1 < S.Rp && kn < rd[2 * r + 3];
-                if (vnext) load_rec<N>(S.post_rec + (size_t)W * kn, nxt);  // the next round's record, in flight
-                if (valid && act) post_entry(cur.a, cur.b, cur.D0, cur.D1);
+            // two record buffers in turn, loaded unconditionally (clamped
+            // index) one round ahead, so a round waits only for the record it
+            // uses and never for the loads just issued for the next round
+            PostRec<N> ra, rb;
+            auto entry = [&](int r, bool &valid) {
+                const int k = r < S.Rp ? rd[2 * r] + k0 : 0;
+                valid = r < S.Rp && k < rd[2 * r + 1];
+                return (size_t)W * min(k, S.n_post - 1);
+            };
+            bool va, vb;
+            if (S.Rp > 0) load_rec<N>(S.post_rec + entry(0, va), ra);
+            for (int r = 0; r < S.Rp; r += 2) {
+                load_rec<N>(S.post_rec + entry(r + 1, vb), rb);
+                if (va && act) post_entry(ra.a, ra.b, ra.D0, ra.D1);
                 lds_barrier();
-                if (vnext) cur = nxt;
-                k = kn;
+                if (r + 1 >= S.Rp) break;
+                load_rec<N>(S.post_rec + entry(r + 2, va), ra);
+                if (vb && act) post_entry(rb.a, rb.b, rb.D0, rb.D1);
+                lds_barrier();
             }
         }
     }
@@ -301,18 +308,22 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
                 lds_barrier();
             }
         } else {
-            PreRec<N> cur, nxt;
-            int k = R > 0 ? rd[0] + k0 : 0;
-            if (R > 0 && k < rd[1]) load_rec<N>(S.pre_rec + (size_t)W * k, cur);
-            for (int r = 0; r < R; ++r) {
-                const bool valid = k < rd[2 * r + 1];
-                const int kn = r + 1 < R ? rd[2 * r + 2] + k0 : 0;
-                const bool vnext = r + 1 < R && kn < rd[2 * r + 3];
-                if (vnext) load_rec<N>(S.pre_rec + (size_t)W * kn, nxt);
-                if (valid && act) pre_entry(cur.a, cur.b, cur.M);
+            PreRec<N> ra, rb;
+            auto entry = [&](int r, bool &valid) {
+                const int k = r < R ? rd[2 * r] + k0 : 0;
+                valid = r < R && k < rd[2 * r + 1];
+                return (size_t)W * min(k, S.n_pre - 1);
+            };
+            bool va, vb;
+            if (R > 0) load_rec<N>(S.pre_rec + entry(0, va), ra);
+            for (int r = 0; r < R; r += 2) {
+                load_rec<N>(S.pre_rec + entry(r + 1, vb), rb);
+                if (va && act) pre_entry(ra.a, ra.b, ra.M);
                 lds_barrier();
-                if (vnext) cur = nxt;
-                k = kn;
+                if (r + 1 >= R) break;
+                load_rec<N>(S.pre_rec + entry(r + 2, va), ra);
+                if (vb && act) pre_entry(rb.a, rb.b, rb.M);
+                lds_barrier();
             }
         }
     }
