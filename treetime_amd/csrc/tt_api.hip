@@ -355,6 +355,7 @@ int32_t build_schedule(tt_tree *T, const tt_tree_desc *D) {
         L.cnt = (int)byh[h].size();
         long waves = (long)L.cnt * npt;
         L.chunk = (int)std::max<long>(1, (waves + g_target_waves() - 1) / g_target_waves());
+        if (D->n_states == 20 && ttk::mfma_enabled()) L.chunk = 1;  // the MFMA kernel takes one node per wave
         L.nchunks = (L.cnt + L.chunk - 1) / L.chunk;
         L.rows = (L.nchunks + 3) / 4;
         L.row_off = row;
