@@ -62,6 +62,8 @@ def parse():
                     help="request arrival rates (per second) to sweep, comma-separated")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-pool", type=int, default=64)
+    ap.add_argument("--serve-batch", default="",
+                    help="minimum group sizes for device-batched serving replays (comma list; empty: none)")
     ap.add_argument("--branch-opt", type=int, default=0, metavar="K",
                     help="also time the device branch-length sweeps (joint and marginal) on the first K trees "
                          "against the FP64 oracle's per-branch Brent")
@@ -227,7 +229,9 @@ def cpu_baseline(preps, workers):
 def serve_bench(eng, pool, args):
     """Poisson request stream over a resident pool: 70% branch re-evaluation
     (root path only), 20% marginal, 10% joint; HBM budget 200 GB.  The same
-    stream is replayed with every branch request recomputed from scratch."""
+    stream is replayed with every branch request recomputed from scratch,
+    and with same-kind pass requests fused into batched launches once at
+    least B of them are admitted together (``--serve-batch``)."""
     from treetime_amd.serve import Request, Server, make_stream, summarize
 
     for t in pool:  # switch to update mode (keeps per-node normalisers)
@@ -240,19 +244,31 @@ def serve_bench(eng, pool, args):
         rng = np.random.default_rng(4000)
         reqs = make_stream(pool, args.serve_requests, rate, rng)
 
-        def replay(scratch):
+        def replay(scratch, device_batch=0):
             rr = [Request(r.kind, r.tree, r.arrival, r.node, r.scale, r.rid) for r in reqs]
-            srv = Server(pool, engine=eng, slots=16, hbm_budget=200 * 10**9, scratch=scratch)
+            srv = Server(pool, engine=eng, slots=16, hbm_budget=200 * 10**9, scratch=scratch,
+                         device_batch=device_batch)
             w0 = time.perf_counter()
             done = srv.run(rr, realtime=True)
-            return summarize(done, time.perf_counter() - w0)
+            out = summarize(done, time.perf_counter() - w0)
+            if device_batch:
+                out["batched_launches"] = srv.batches
+                out["requests_in_batches"] = sum(1 for r in done if r.meta.get("batched"))
+            return out
 
         rp = replay(False)
         sc = replay(True)
         log(f"[bench] serving @ {rate:.0f} req/s offered: p50 {rp['p50_ms']:.2f} ms p99 {rp['p99_ms']:.2f} ms "
             f"({rp['throughput_req_s']:.0f} req/s served); from scratch p50 {sc['p50_ms']:.2f} p99 {sc['p99_ms']:.2f} "
             f"({sc['throughput_req_s']:.0f} req/s)")
-        out["sweep"].append({"rate_req_s": rate, "root_path": rp, "from_scratch": sc})
+        entry = {"rate_req_s": rate, "root_path": rp, "from_scratch": sc}
+        for b in [int(x) for x in str(args.serve_batch).split(",") if x.strip()]:
+            db = replay(False, device_batch=b)
+            log(f"[bench] serving @ {rate:.0f} req/s, device batches of >= {b}: p50 {db['p50_ms']:.2f} ms "
+                f"p99 {db['p99_ms']:.2f} ms ({db['throughput_req_s']:.0f} req/s served, "
+                f"{db['requests_in_batches']} requests in {db['batched_launches']} batched launches)")
+            entry[f"device_batched_min{b}"] = db
+        out["sweep"].append(entry)
     return out
 
 

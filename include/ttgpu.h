@@ -363,6 +363,22 @@ int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint
 #define TT_BATCH_LOGLIK 0x20u
 int32_t tt_batch_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t mode, double *logL, void *stream);
 
+/* The batched pass of tt_batch_run over the trees pool[i], i in members[0..m)
+ * only, through the plan of the whole pool (items and launch shape, built and
+ * tuned by the first call for (pool, mode), which runs the tuning passes over
+ * every pool tree): the members' items are staged to the device and run in
+ * one persistent launch.  For the request server, which fuses the pending
+ * requests of several trees into one launch (no reference counterpart: the
+ * reference runs one tree at a time, treeanc.py:762-1080).  The run waits for
+ * the work queued on each member's stream and each member's stream waits for
+ * the run, so tt_tree_query, tt_get_total and later per-tree calls see it.
+ * Every tree's total (tt_get_total) is written on the device, by this call
+ * and by tt_batch_run.  Members must be distinct; logL is host [m] (NULL: no
+ * read-back, the call does not wait).  Calls on one plan run in order on the
+ * plan's stream (stream NULL) or on the given stream. */
+int32_t tt_batch_run_members(tt_engine *eng, tt_tree *const *pool, int32_t k, uint32_t mode, const int32_t *members,
+                             int32_t m, double *logL, void *stream);
+
 /* Launch shape the autotuner chose for the plan of (trees, flags): one
  * 4-wave block (0) or one wave (1) per item of the persistent kernel, or each
  * tree's own level-kernel pass in turn (2); 64-pattern tiles per item;

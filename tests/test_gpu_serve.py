@@ -95,3 +95,72 @@ def test_serve_budget_admission(gpu_engine):
     for t in trees:
         t.close()
     eng.close()
+
+
+def test_batch_members_equal_own_passes(gpu_engine):
+    """tt_batch_run_members over subsets of a pool gives each member what its
+    own pass gives (states and profiles identical, logL to 1e-12), leaves
+    non-members untouched and is ordered after work already queued on a
+    member's stream; tt_get_total after tt_batch_run equals the logL that
+    call returned."""
+    from treetime_amd.engine import batch_run, batch_run_members
+
+    say("[serve] batched member runs vs each tree's own pass")
+    jobs = [small_job(n, L, 4, seed) for n, L, seed in SPECS]
+    pool = fresh(gpu_engine, jobs)
+    ref = fresh(gpu_engine, jobs)
+    for kind in ("marginal", "loglik", "joint"):
+        logL = batch_run(gpu_engine, pool, kind=kind)  # builds and tunes the pool's plan
+        assert [t.total() for t in pool] == list(logL), kind
+        for members in ([0, 2, 4], [3], [1, 3]):
+            others = [i for i in range(len(pool)) if i not in members]
+            before = {i: (pool[i].states(), pool[i].total()) for i in others}
+            batch_run_members(gpu_engine, pool, members, kind=kind)
+            for i in members:
+                want = getattr(ref[i], kind)()
+                got = pool[i].total()
+                assert abs(got - want) <= 1e-12 * abs(want), (kind, members, i, got, want)
+                if kind != "loglik":
+                    assert np.array_equal(pool[i].states(), ref[i].states()), (kind, i)
+                if kind == "marginal":
+                    assert np.array_equal(pool[i].profiles(), ref[i].profiles()), (kind, i)
+                if kind == "loglik":
+                    np.testing.assert_allclose(pool[i].sequence_lh(), ref[i].sequence_lh(), rtol=1e-12)
+            for i in others:
+                assert np.array_equal(pool[i].states(), before[i][0]) and pool[i].total() == before[i][1], (kind, i)
+    # ordering: a marginal pass queued on tree 0's own stream, then a joint member run
+    pool[0].marginal(want_logL=False)
+    batch_run_members(gpu_engine, pool, [0], kind="joint")
+    ref[0].joint()
+    assert np.array_equal(pool[0].states(), ref[0].states())
+    with pytest.raises(ValueError):
+        batch_run_members(gpu_engine, pool, [0, 0], kind="marginal")
+    with pytest.raises(ValueError):
+        batch_run_members(gpu_engine, pool, [len(pool)], kind="marginal")
+    for t in pool + ref:
+        t.close()
+
+
+def test_serve_device_batching_same_results(gpu_engine):
+    """With device batching the served results equal the one-at-a-time run
+    (to 1e-12: batched and per-tree passes sum logL in different orders), and
+    batched launches were made."""
+    say("[serve] device-batched stream vs alone")
+    jobs = [small_job(n, L, 4, seed) for n, L, seed in SPECS]
+    rng = np.random.default_rng(12)
+    trees = fresh(gpu_engine, jobs)
+    reqs = make_stream(trees, 150, 2000.0, rng, mix=(0.3, 0.4, 0.3))
+    srv = Server(trees, engine=gpu_engine, slots=8, device_batch=2)
+    batched = srv.run(copy_reqs(reqs), realtime=False)
+    for t in trees:
+        t.close()
+    trees = fresh(gpu_engine, jobs)
+    alone = Server(trees, engine=gpu_engine, slots=1).run(copy_reqs(reqs), realtime=False)
+    for t in trees:
+        t.close()
+    assert all(not r.error for r in batched + alone), [r.error for r in batched + alone if r.error]
+    worst = max(abs(a.result - b.result) / abs(b.result) for a, b in zip(batched, alone))
+    n_fused = sum(1 for r in batched if r.meta.get("batched"))
+    say(f"  {srv.batches} batched launches carried {n_fused} requests; max rel diff {worst:.1e}")
+    assert srv.batches >= 1 and n_fused >= 2
+    assert worst <= 1e-12

@@ -117,6 +117,7 @@ SIGNATURES = {
     "tt_tree_set_site_model": (_I32, [_P, _PD, _PD, _PD, _PD, _PD, _PD, _I32, _D]),
     "tt_propagate_profile_site": (_I32, [_P, _I32, _I32, _P, _I32, _I32, _P, _P]),
     "tt_batch_run": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PD, _P]),
+    "tt_batch_run_members": (_I32, [_P, C.POINTER(_P), _I32, _U32, _PI32, _I32, _PD, _P]),
     "tt_site_counts": (_I32, [_P, _I32, _PI16, _PD, _PD, _P]),
     "tt_compress_alignment": (_I32, [_P, _P, _I32, _I64, _I64, _I32, _U32, _I64, _PI64, _P, _P, _P, _PI64, _P]),
     "tt_hbm_probe": (_I32, [_I32, _P, _P, _U64, _I32, _PD, _P]),

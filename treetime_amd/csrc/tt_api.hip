@@ -101,20 +101,48 @@ struct BatchPlan {
     BatchShape shape{0, 1, 1, 0};
     double tuned_ms = 0.0;  // time per launch of the chosen shape while tuning (0: not timed)
     bool joint = false;     // joint reconstruction plan (k_batch_joint)
+    // trees longest first, the die each tree is dealt to, and each tree's
+    // items (contiguous) in items1 / items2
+    std::vector<int> order, die_of, first[2], count[2];
+    int32_t *d_tot = nullptr;  // [2][k][3]: (tree, first item, item count) for items1, items2
+    // member runs (tt_batch_run_members): the members' items, queues and
+    // outputs on the device, staged through a ring of pinned host slots
+    BatchItem *d_sub_items = nullptr;
+    int *d_sub_queue = nullptr;
+    double *d_sub_out = nullptr;
+    int32_t *d_sub_tot = nullptr;
+    static constexpr int kSubSlots = 4;
+    char *h_sub[kSubSlots] = {};
+    hipEvent_t ev_sub[kSubSlots] = {};
+    size_t sub_slot_bytes = 0;
+    int sub_next = 0;
+    hipEvent_t ev_done = nullptr;
     const std::vector<BatchItem> &items_for(const BatchShape &c) const { return c.tiles == 2 ? items2 : items1; }
     const BatchItem *d_items_for(const BatchShape &c) const { return c.tiles == 2 ? d_items2 : d_items1; }
     const int *d_xoff_for(const BatchShape &c) const { return d_queue + (c.tiles == 2 ? kQueueXoff2 : kQueueXoff1); }
     hipError_t launch(int n, int mode, const BatchShape &c, hipStream_t s) const {
+        hipError_t e;
         if (joint)
-            return ttk::batch_joint(n, d_trees, d_items_for(c), (int)items_for(c).size(), d_queue, d_xoff_for(c),
+            e = ttk::batch_joint(n, d_trees, d_items_for(c), (int)items_for(c).size(), d_queue, d_xoff_for(c),
+                                 d_item_out, mode, c, s);
+        else
+            e = ttk::batch_marginal(n, d_trees, d_items_for(c), (int)items_for(c).size(), d_queue, d_xoff_for(c),
                                     d_item_out, mode, c, s);
-        return ttk::batch_marginal(n, d_trees, d_items_for(c), (int)items_for(c).size(), d_queue, d_xoff_for(c),
-                                   d_item_out, mode, c, s);
+        const int k = (int)trees.size();
+        if (e == hipSuccess) e = ttk::batch_totals(d_trees, d_tot + (c.tiles == 2 ? 3 * k : 0), k, d_item_out, s);
+        return e;
     }
     ~BatchPlan() {
         if (graph) (void)hipGraphExecDestroy(graph);
-        for (void *q : {(void *)d_trees, (void *)d_items1, (void *)d_items2, (void *)d_queue, (void *)d_item_out})
+        // d_sub_queue is the base of the member-run block (items, totals and outputs live inside it)
+        for (void *q : {(void *)d_trees, (void *)d_items1, (void *)d_items2, (void *)d_queue, (void *)d_item_out,
+                        (void *)d_tot, (void *)d_sub_queue})
             if (q) (void)hipFree(q);
+        for (int i = 0; i < kSubSlots; ++i) {
+            if (ev_sub[i]) (void)hipEventDestroy(ev_sub[i]);
+            if (h_sub[i]) (void)hipHostFree(h_sub[i]);
+        }
+        if (ev_done) (void)hipEventDestroy(ev_done);
         if (stream) (void)hipStreamDestroy(stream);
         eng->used -= bytes;
     }
@@ -168,6 +196,7 @@ struct tt_tree {
     std::vector<std::pair<void *, size_t>> allocs;
     int last_pass = 0;  // 1 marginal, 2 joint, 3 loglik
     hipStream_t last_stream = nullptr;  // stream of the last pass (result reads wait for it)
+    hipEvent_t ev_order = nullptr;      // orders a member run of a batch plan after the tree's own work
     // one-launch marginal / loglik pass with LDS-resident rows (tt_tile.hip):
     // patterns per workgroup, 0 when the level kernels run instead
     int tile_tp = 0;
@@ -229,6 +258,7 @@ struct tt_tree {
             (void)hipFree(a.first);
             eng->used -= a.second;
         }
+        if (ev_order) (void)hipEventDestroy(ev_order);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -1940,8 +1970,123 @@ int32_t tt_batch_info(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t
 namespace {
 
 // kind: 0 marginal, 1 loglik (postorder + root), 2 joint
+// A batched pass over the members of a plan's trees: the members' items
+// (per die, in the plan's order) go through the plan's tuned launch shape,
+// followed by the members' totals.  The run waits for work already queued
+// on each member's own stream, and each member's stream waits for the run,
+// so per-tree calls before and after it stay ordered.
+int32_t plan_run_members(BatchPlan *plan, tt_tree *const *trees, int k, int n, int mode, uint32_t fl, int pass,
+                         const int32_t *members, int m, double *logL, hipStream_t s) {
+    std::vector<char> is(k, 0);
+    for (int j = 0; j < m; ++j) {
+        if (members[j] < 0 || members[j] >= k)
+            return fail(TT_EINVAL, "members[%d]=%d out of range [0, %d)", j, members[j], k);
+        if (is[members[j]]) return fail(TT_EINVAL, "members[%d]=%d listed twice", j, members[j]);
+        is[members[j]] = 1;
+    }
+    if (m == 0) return TT_OK;
+    const BatchShape &c = plan->shape;
+    if (c.wave_items == kPerTreeGraphs) {
+        for (int j = 0; j < m; ++j)
+            if (int32_t rc = run_pass(trees[members[j]], pass, fl ? TT_RECONSTRUCT_TIPS : 0u,
+                                      logL ? &logL[j] : nullptr, nullptr))
+                return rc;
+        return TT_OK;
+    }
+    const int ci = c.tiles == 2 ? 1 : 0;
+    const std::vector<BatchItem> &all = plan->items_for(c);
+    const size_t ni_max = std::max(plan->items1.size(), plan->items2.size());
+    // device block: [queue counters | xoff (16) | items | totals (3 per tree) | item outputs]
+    const size_t off_items = (size_t)(kQueueXoff1 + 16) * sizeof(int32_t);
+    const size_t off_tot = off_items + ni_max * sizeof(BatchItem);
+    const size_t off_out = (off_tot + (size_t)3 * k * sizeof(int32_t) + 255) & ~(size_t)255;
+    const size_t dev_bytes = off_out + ni_max * sizeof(double);
+    if (!plan->d_sub_queue) {
+        tt_engine *eng = plan->eng;
+        if (eng->budget && eng->used.load() + dev_bytes > eng->budget)
+            return fail(TT_ENOMEM_BUDGET, "HBM budget exceeded by the member-run buffers of a batch plan");
+        char *blk = nullptr;
+        TT_HIP(hipMalloc((void **)&blk, dev_bytes));
+        eng->used += dev_bytes;
+        plan->bytes += dev_bytes;
+        plan->d_sub_queue = reinterpret_cast<int *>(blk);
+        plan->d_sub_items = reinterpret_cast<BatchItem *>(blk + off_items);
+        plan->d_sub_tot = reinterpret_cast<int32_t *>(blk + off_tot);
+        plan->d_sub_out = reinterpret_cast<double *>(blk + off_out);
+        TT_HIP(hipMemset(blk, 0, (size_t)kQueueXoff1 * sizeof(int32_t)));
+        plan->sub_slot_bytes = off_tot - (size_t)kQueueXoff1 * sizeof(int32_t) + (size_t)3 * k * sizeof(int32_t);
+        for (int i = 0; i < BatchPlan::kSubSlots; ++i) {
+            TT_HIP(hipHostMalloc((void **)&plan->h_sub[i], plan->sub_slot_bytes, hipHostMallocDefault));
+            TT_HIP(hipEventCreateWithFlags(&plan->ev_sub[i], hipEventDisableTiming));
+        }
+        TT_HIP(hipEventCreateWithFlags(&plan->ev_done, hipEventDisableTiming));
+    }
+    // host slot: [xoff (16) | items | totals], the same layout as the device block after the counters
+    const int slot = plan->sub_next;
+    plan->sub_next = (slot + 1) % BatchPlan::kSubSlots;
+    TT_HIP(hipEventSynchronize(plan->ev_sub[slot]));  // its previous copy has been read
+    int32_t *hx = reinterpret_cast<int32_t *>(plan->h_sub[slot]);
+    BatchItem *hi = reinterpret_cast<BatchItem *>(plan->h_sub[slot] + 16 * sizeof(int32_t));
+    int32_t *ht = reinterpret_cast<int32_t *>(plan->h_sub[slot] + (off_tot - (size_t)kQueueXoff1 * sizeof(int32_t)));
+    std::fill(hx, hx + 16, 0);
+    int ni = 0, nt = 0;
+    for (int x = 0; x < kXcds; ++x) {
+        hx[x] = ni;
+        for (int i : plan->order) {
+            if (!is[i] || plan->die_of[i] != x) continue;
+            const int f = plan->first[ci][i], cnt = plan->count[ci][i];
+            ht[3 * nt] = i;
+            ht[3 * nt + 1] = ni;
+            ht[3 * nt + 2] = cnt;
+            ++nt;
+            std::copy(all.begin() + f, all.begin() + f + cnt, hi + ni);
+            ni += cnt;
+        }
+    }
+    hx[kXcds] = ni;
+    for (int j = 0; j < m; ++j) {  // the run follows each member's own queued work
+        tt_tree *T = trees[members[j]];
+        if (!T->ev_order) TT_HIP(hipEventCreateWithFlags(&T->ev_order, hipEventDisableTiming));
+        TT_HIP(hipEventRecord(T->ev_order, T->stream));
+        TT_HIP(hipStreamWaitEvent(s, T->ev_order, 0));
+    }
+    TT_HIP(hipMemcpyAsync(plan->d_sub_queue + kQueueXoff1, plan->h_sub[slot], plan->sub_slot_bytes,
+                          hipMemcpyHostToDevice, s));
+    TT_HIP(hipEventRecord(plan->ev_sub[slot], s));
+    const int *xoff = plan->d_sub_queue + kQueueXoff1;
+    if (plan->joint)
+        TT_HIP(ttk::batch_joint(n, plan->d_trees, plan->d_sub_items, ni, plan->d_sub_queue, xoff, plan->d_sub_out, mode,
+                                c, s));
+    else
+        TT_HIP(ttk::batch_marginal(n, plan->d_trees, plan->d_sub_items, ni, plan->d_sub_queue, xoff,
+                                   plan->d_sub_out, mode, c, s));
+    TT_HIP(ttk::batch_totals(plan->d_trees, plan->d_sub_tot, nt, plan->d_sub_out, s));
+    TT_HIP(hipEventRecord(plan->ev_done, s));
+    for (int j = 0; j < m; ++j) {
+        tt_tree *T = trees[members[j]];
+        TT_HIP(hipStreamWaitEvent(T->stream, plan->ev_done, 0));
+        T->last_stream = T->stream;
+        if (T->update_mode && pass != 2) T->offs_valid = true;  // the plan's rows carry d.offs
+        if (pass == 3) {
+            T->last_pass = 3;
+            T->profiles_fresh = false;
+            continue;
+        }
+        T->last_pass = pass == 2 ? 2 : 1;
+        T->have_states = true;
+        T->tips_dirty = fl != 0;
+        if (pass != 2) T->profiles_fresh = true;
+    }
+    if (logL) {
+        TT_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < m; ++j)
+            TT_HIP(hipMemcpy(&logL[j], trees[members[j]]->d.total, sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return TT_OK;
+}
+
 int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, int kind, double *logL,
-                       void *stream) {
+                       void *stream, const int32_t *members = nullptr, int32_t m = -1) {
     const bool loglik = kind == 1, joint = kind == 2;
     if (!eng || (k > 0 && !trees)) return fail(TT_EINVAL, "NULL argument");
     if (k <= 0) return TT_OK;
@@ -1963,6 +2108,7 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
     const int mode = (fl ? 1 : 0) | (loglik ? 2 : 0);
     std::vector<tt_tree *> key(trees, trees + k);
     BatchPlan *plan = nullptr;
+    bool created = false;
     {
         std::lock_guard<std::mutex> g(eng->mu);
         for (BatchPlan *p : eng->plans)
@@ -1996,16 +2142,29 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
             }
         }
         std::vector<int32_t> xoff(2 * 16, 0);
+        for (int c = 0; c < 2; ++c) {
+            P->first[c].assign(k, 0);
+            P->count[c].assign(k, 0);
+        }
         for (int x = 0; x < kXcds; ++x) {
             xoff[x] = (int32_t)P->items1.size();
             xoff[16 + x] = (int32_t)P->items2.size();
             for (auto &o : order) {
                 if (die_of[o.second] != x) continue;
                 const int tiles = (trees[o.second]->P + TT_WAVE - 1) / TT_WAVE;
+                P->first[0][o.second] = (int)P->items1.size();
+                P->first[1][o.second] = (int)P->items2.size();
                 for (int t = 0; t < tiles; ++t) P->items1.push_back({o.second, t});
                 for (int t = 0; t < tiles; t += 2) P->items2.push_back({o.second, t});
+                P->count[0][o.second] = (int)P->items1.size() - P->first[0][o.second];
+                P->count[1][o.second] = (int)P->items2.size() - P->first[1][o.second];
             }
         }
+        for (auto &o : order) P->order.push_back(o.second);
+        P->die_of = die_of;
+        std::vector<int32_t> tot;
+        for (int c = 0; c < 2; ++c)
+            for (int i = 0; i < k; ++i) tot.insert(tot.end(), {i, P->first[c][i], P->count[c][i]});
         xoff[kXcds] = (int32_t)P->items1.size();
         xoff[16 + kXcds] = (int32_t)P->items2.size();
         std::vector<BatchTreeDev> bt(k);
@@ -2026,7 +2185,7 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
         }
         const size_t nb_trees = bt.size() * sizeof(BatchTreeDev), nb_items1 = P->items1.size() * sizeof(BatchItem),
                      nb_items2 = P->items2.size() * sizeof(BatchItem), nb_out = P->items1.size() * sizeof(double);
-        P->bytes = nb_trees + nb_items1 + nb_items2 + nb_out + kQueueInts * sizeof(int);
+        P->bytes = nb_trees + nb_items1 + nb_items2 + nb_out + kQueueInts * sizeof(int) + tot.size() * sizeof(int32_t);
         if (eng->budget && eng->used.load() + P->bytes > eng->budget) {
             P->bytes = 0;
             delete P;
@@ -2041,6 +2200,9 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
         if (e == hipSuccess)
             e = hipMemcpy(P->d_queue + kQueueXoff1, xoff.data(), xoff.size() * sizeof(int32_t), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc((void **)&P->d_item_out, nb_out);
+        if (e == hipSuccess) e = hipMalloc((void **)&P->d_tot, tot.size() * sizeof(int32_t));
+        if (e == hipSuccess)
+            e = hipMemcpy(P->d_tot, tot.data(), tot.size() * sizeof(int32_t), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(P->d_trees, bt.data(), nb_trees, hipMemcpyHostToDevice);
         P->host_trees = bt;
         if (e == hipSuccess) e = hipMemcpy(P->d_items1, P->items1.data(), nb_items1, hipMemcpyHostToDevice);
@@ -2072,20 +2234,41 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
         std::lock_guard<std::mutex> g(eng->mu);
         eng->plans.push_back(P);
         plan = P;
+        created = true;
     }
     hipStream_t s = stream ? (hipStream_t)stream : plan->stream;
-    // tip rows that an earlier pass reconstructed are reset to 255 (read by the kernel at run time)
+    // tip rows that an earlier pass reconstructed are reset to 255 (read by the
+    // kernel at run time); a tree switched to update mode after the plan was
+    // built gets its normaliser rows into the plan's copy
     bool changed = false;
-    for (int i = 0; i < k && !loglik; ++i) {
+    for (int i = 0; i < k; ++i) {
+        BatchTreeDev &b = plan->host_trees[i];
+        if (b.T.offs != trees[i]->d.offs) {
+            b.T.offs = trees[i]->d.offs;
+            changed = true;
+        }
+        if (loglik) continue;
         const int32_t want = (!fl && trees[i]->tips_dirty) ? 1 : 0;
-        if (plan->host_trees[i].fill_tips != want) {
-            plan->host_trees[i].fill_tips = want;
+        if (b.fill_tips != want) {
+            b.fill_tips = want;
             changed = true;
         }
     }
     if (changed)
         TT_HIP(hipMemcpyAsync(plan->d_trees, plan->host_trees.data(), plan->host_trees.size() * sizeof(BatchTreeDev),
                               hipMemcpyHostToDevice, s));
+    if (m >= 0) {
+        if (created)  // the autotuner ran full passes over every tree of the plan
+            for (int i = 0; i < k; ++i) {
+                trees[i]->last_stream = plan->stream;
+                trees[i]->last_pass = joint ? 2 : (loglik ? 3 : 1);
+                if (!loglik) trees[i]->have_states = true;
+                if (!loglik) trees[i]->tips_dirty = fl != 0;
+                trees[i]->profiles_fresh = !joint && !loglik;
+                if (trees[i]->update_mode && !joint) trees[i]->offs_valid = true;
+            }
+        return plan_run_members(plan, trees, k, n, mode, fl, joint ? 2 : (loglik ? 3 : 1), members, m, logL, s);
+    }
     if (plan->graph) {
         TT_HIP(hipGraphLaunch(plan->graph, s));
     } else if (int32_t rc = launch_plan(plan, n, mode, plan->shape, s)) {
@@ -2093,6 +2276,7 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
     }
     for (int i = 0; i < k; ++i) {
         trees[i]->last_stream = s;
+        if (trees[i]->update_mode && !joint) trees[i]->offs_valid = true;  // the plan's rows carry d.offs
         if (loglik) {  // subtree rows rewritten, profiles and states not
             trees[i]->last_pass = 3;
             trees[i]->profiles_fresh = false;
@@ -2125,6 +2309,17 @@ extern "C" {
 int32_t tt_batch_marginal(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t flags, double *logL,
                           void *stream) {
     return batch_plan_run(eng, trees, k, flags, 0, logL, stream);
+}
+
+int32_t tt_batch_run_members(tt_engine *eng, tt_tree *const *pool, int32_t k, uint32_t mode, const int32_t *members,
+                             int32_t m, double *logL, void *stream) {
+    const uint32_t kind = mode & 0x30u;
+    if (mode & ~(0x30u | TT_RECONSTRUCT_TIPS)) return fail(TT_EINVAL, "unknown mode bits 0x%x", mode);
+    if (m < 0 || (m > 0 && !members)) return fail(TT_EINVAL, "members: NULL or m=%d < 0", m);
+    if (kind == TT_BATCH_MARGINAL) return batch_plan_run(eng, pool, k, mode, 0, logL, stream, members, m);
+    if (kind == TT_BATCH_LOGLIK) return batch_plan_run(eng, pool, k, 0u, 1, logL, stream, members, m);
+    if (kind == TT_BATCH_JOINT) return batch_plan_run(eng, pool, k, mode, 2, logL, stream, members, m);
+    return fail(TT_EINVAL, "unknown pass kind 0x%x", kind);
 }
 
 int32_t tt_batch_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_t mode, double *logL, void *stream) {

@@ -453,6 +453,24 @@ int max_blocks_per_cu(const BatchShape &c) {
         default: return hipErrorInvalidValue;            \
     }
 
+namespace {
+__global__ void k_batch_totals(const BatchTreeDev *trees, const int32_t *tot, int m, const double *item_out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    const int t = tot[3 * e], first = tot[3 * e + 1], cnt = tot[3 * e + 2];
+    double sum = 0.0;
+    for (int j = first; j < first + cnt; ++j) sum += item_out[j];
+    trees[t].T.total[0] = sum;
+}
+}  // namespace
+
+hipError_t ttk::batch_totals(const BatchTreeDev *trees, const int32_t *tot, int m, const double *item_out,
+                             hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_totals, dim3((m + 255) / 256), dim3(256), 0, s, trees, tot, m, item_out);
+    return hipGetLastError();
+}
+
 int ttk::device_cus() {
     static int cus = [] {
         int dev = 0, c = 0;

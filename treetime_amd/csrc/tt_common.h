@@ -259,6 +259,10 @@ hipError_t batch_marginal(int n, const BatchTreeDev *trees, const BatchItem *ite
 // joint reconstruction of many trees in one persistent launch (block items of one or two tiles)
 hipError_t batch_joint(int n, const BatchTreeDev *trees, const BatchItem *items, int n_items, int *queue,
                        const int *xoff, double *item_out, int mode, const BatchShape &shape, hipStream_t s);
+// Per-tree log-likelihood of a batched launch: for each of the m entries
+// (tree index, first item, item count) of tot, the tree's total = the sum of
+// its items' outputs in item order (the order tt_batch_run sums them in).
+hipError_t batch_totals(const BatchTreeDev *trees, const int32_t *tot, int m, const double *item_out, hipStream_t s);
 int batch_joint_max_blocks_per_cu(int n, int waves_per_simd, int tiles);
 // occupancy limit (blocks per CU) of the batched kernel for n states and item shape; 0 if n unsupported
 int batch_max_blocks_per_cu(int n, const BatchShape &shape);

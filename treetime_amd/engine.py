@@ -419,6 +419,33 @@ def batch_run(engine, trees, kind="marginal", reconstruct_tips=False, stream=Non
     return out
 
 
+def batch_run_members(engine, pool, members, kind="marginal", reconstruct_tips=False, stream=None, want_logL=False):
+    """tt_batch_run_members: the batched pass of ``kind`` over the trees
+    ``pool[i]`` for i in ``members`` only, through the plan (items, tuned
+    launch shape) of the whole pool, which the first call builds.  The run is
+    ordered after work already queued on each member's stream and before
+    later work on it; ``tree.query()`` / ``tree.total()`` of a member see it.
+    Returns logL per member (host sync) when ``want_logL``."""
+    modes = {"marginal": _native.TT_BATCH_MARGINAL, "joint": _native.TT_BATCH_JOINT,
+             "loglik": _native.TT_BATCH_LOGLIK}
+    if kind not in modes:
+        raise ValueError(f"kind must be 'marginal', 'loglik' or 'joint', not {kind!r}")
+    k = len(pool)
+    arr = (C.c_void_p * k)(*[t.handle.value for t in pool])
+    mem = np.ascontiguousarray(members, dtype=np.int32)
+    out = np.full(len(mem), np.nan)
+    mode = modes[kind]
+    if reconstruct_tips:
+        mode |= _native.TT_RECONSTRUCT_TIPS
+    _native.check(engine.lib.tt_batch_run_members(engine.handle, arr, k, mode, _native.ptr(mem, C.c_int32),
+                                                  len(mem), _native.ptr(out, C.c_double) if want_logL else None,
+                                                  stream))
+    if kind == "joint":
+        for t in pool:  # a joint plan holds every pool tree's joint buffers
+            t.joint_ready = True
+    return out
+
+
 def batch_info(engine, trees, reconstruct_tips=False, kind="marginal"):
     """Launch shape chosen by the autotuner for this list of trees:
     {"wave_items", "tiles", "blocks_per_cu", "waves_per_simd", "tuned_ms"}

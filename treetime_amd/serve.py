@@ -21,6 +21,13 @@ Scheduling:
 
 ``scratch=True`` recomputes a branch request with a full postorder instead
 of the root path (the baseline the root-path update is measured against).
+
+``device_batch=B`` (B >= 2) fuses, at each scheduling step, the admitted
+``marginal`` / ``joint`` / ``loglik`` requests of one kind into one batched
+launch over their trees (``tt_batch_run_members`` through the pool's tuned
+plan) when there are at least B of them; fewer run as per-tree passes.
+Results are the same either way (the batched kernels equal the per-tree
+passes).  The pool's plans are built and tuned when the server starts.
 """
 import time
 from collections import deque
@@ -52,7 +59,8 @@ class Request:
 class Server:
     """Continuous-batching loop over ``trees`` (DeviceTree-like objects)."""
 
-    def __init__(self, trees, engine=None, slots=16, hbm_budget=None, scratch=False, clock=time.perf_counter):
+    def __init__(self, trees, engine=None, slots=16, hbm_budget=None, scratch=False, clock=time.perf_counter,
+                 device_batch=0, batch_kinds=("marginal", "joint")):
         self.trees = list(trees)
         self.engine = engine
         self.slots = slots
@@ -60,6 +68,13 @@ class Server:
         self.scratch = scratch
         self.clock = clock
         self.last_use = [0.0] * len(self.trees)
+        self.device_batch = device_batch if device_batch and device_batch >= 2 else 0
+        self.batches = 0          # batched launches made
+        if self.device_batch:
+            from .engine import batch_run_members
+            self._members = batch_run_members
+            for kind in batch_kinds:  # build and tune the pool's plan of each kind now
+                batch_run_members(engine, self.trees, [], kind=kind)
 
     # -- memory ------------------------------------------------------------
     def _used(self):
@@ -112,6 +127,15 @@ class Server:
         else:
             raise ValueError("unknown request kind %r" % req.kind)
 
+    def _launch_batch(self, kind, reqs):
+        """One batched launch for same-kind requests on distinct trees."""
+        self._members(self.engine, self.trees, [r.tree for r in reqs], kind=kind)
+        self.batches += 1
+        for r in reqs:
+            tree = self.trees[r.tree]
+            r.updates = tree.P * (tree.n_nodes - 1)
+            r.meta["batched"] = len(reqs)
+
     def _complete(self, req):
         req.result = self.trees[req.tree].total()
 
@@ -137,8 +161,9 @@ class Server:
                 r.t_done = self.clock() - t0
             blocked = set(inflight)
             need_inflight = sum(self._need(r) for r in inflight.values())
+            admitted = []
             for r in list(pending):
-                if len(inflight) >= self.slots:
+                if len(inflight) + len(admitted) >= self.slots:
                     break
                 if r.tree in blocked:
                     continue
@@ -153,14 +178,34 @@ class Server:
                 pending.remove(r)
                 r.t_admit = self.clock() - t0
                 self.last_use[r.tree] = r.t_admit
+                admitted.append(r)
+                need_inflight += self._need(r)
+                blocked.add(r.tree)
+            groups = {}
+            if self.device_batch:
+                for r in admitted:
+                    if r.kind in ("marginal", "joint", "loglik"):
+                        groups.setdefault(r.kind, []).append(r)
+                groups = {k: g for k, g in groups.items() if len(g) >= self.device_batch}
+            fused = {id(r) for g in groups.values() for r in g}
+            for kind, g in groups.items():
+                try:
+                    self._launch_batch(kind, g)
+                    for r in g:
+                        inflight[r.tree] = r
+                except Exception as e:  # the batch fails alone
+                    for r in g:
+                        r.error = repr(e)
+                        r.t_done = self.clock() - t0
+            for r in admitted:
+                if id(r) in fused:
+                    continue
                 try:
                     self._launch(r)
                     inflight[r.tree] = r
-                    need_inflight += self._need(r)
                 except Exception as e:
                     r.error = repr(e)
                     r.t_done = self.clock() - t0
-                blocked.add(r.tree)
             if not inflight and pending and i >= len(reqs):
                 # everything left is blocked by memory with nothing in flight
                 for r in list(pending):
