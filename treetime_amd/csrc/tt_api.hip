@@ -451,11 +451,11 @@ int32_t ensure_joint(tt_tree *T) {
     if (T->joint_ready && (rc = tt_tree_trim(T))) return rc;
     size_t rows = (size_t)T->n_int * T->P * T->n;
     if (T->site)
-        rc = T->alloc((void **)&T->d.Lx64, rows * sizeof(double));
+        rc = T->alloc((void **)(void *)&T->d.Lx64, rows * sizeof(double));
     else
-        rc = T->alloc((void **)&T->d.Lx, rows * sizeof(float));
+        rc = T->alloc((void **)(void *)&T->d.Lx, rows * sizeof(float));
     if (rc) return rc;
-    if ((rc = T->alloc((void **)&T->d.Cx, (size_t)T->n_nodes * T->P * cx_row_bytes(T->n)))) {
+    if ((rc = T->alloc((void **)(void *)&T->d.Cx, (size_t)T->n_nodes * T->P * cx_row_bytes(T->n)))) {
         // give the message buffer back, so a retry under budget pressure
         // starts from nothing instead of stacking message buffers
         T->release(T->site ? (void *)T->d.Lx64 : (void *)T->d.Lx);
@@ -658,7 +658,7 @@ int32_t run_pass(tt_tree *T, int mode, uint32_t flags, double *logL, void *strea
 int32_t enable_update_mode(tt_tree *T) {
     if (T->update_mode) return TT_OK;
     int32_t rc;
-    if ((rc = T->alloc((void **)&T->d.offs, (size_t)T->n_int * T->P * sizeof(double)))) return rc;
+    if ((rc = T->alloc((void **)(void *)&T->d.offs, (size_t)T->n_int * T->P * sizeof(double)))) return rc;
     for (auto &g : T->graphs) (void)hipGraphExecDestroy(g.second);  // recorded without d.offs
     T->graphs.clear();
     T->update_mode = true;
@@ -1180,21 +1180,21 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
         void **p;
         size_t bytes;
     } bufs[] = {
-        {(void **)&d.P64, (size_t)nn * n * n * sizeof(double)},
-        {(void **)&d.Pj64, (size_t)nn * n * n * sizeof(double)},
-        {(void **)&d.logP64, (size_t)nn * n * n * sizeof(double)},
-        {(void **)&d.Pj32, (size_t)nn * n * n * sizeof(float)},
-        {(void **)&d.P32, (size_t)nn * n * n * sizeof(float)},
-        {(void **)&d.D32, (size_t)nn * n * n * sizeof(float)},
-        {(void **)&d.sub, (size_t)T->n_int * P * n * sizeof(float)},
-        {(void **)&d.prof, (size_t)nn * P * n * sizeof(float)},
-        {(void **)&d.states, (size_t)nn * P},
-        {(void **)&d.part, (size_t)std::max(T->R_all, 1) * P * sizeof(double)},
-        {(void **)&d.part2, (size_t)std::max(rowsum_slices(T->R_all), 1) * P * sizeof(double)},
-        {(void **)&d.seqlh, (size_t)P * sizeof(double)},
-        {(void **)&d.jseqlh, (size_t)P * sizeof(double)},
-        {(void **)&d.bpart, (size_t)ttk::root_blocks(P) * sizeof(double)},
-        {(void **)&d.total, sizeof(double)},
+        {(void **)(void *)&d.P64, (size_t)nn * n * n * sizeof(double)},
+        {(void **)(void *)&d.Pj64, (size_t)nn * n * n * sizeof(double)},
+        {(void **)(void *)&d.logP64, (size_t)nn * n * n * sizeof(double)},
+        {(void **)(void *)&d.Pj32, (size_t)nn * n * n * sizeof(float)},
+        {(void **)(void *)&d.P32, (size_t)nn * n * n * sizeof(float)},
+        {(void **)(void *)&d.D32, (size_t)nn * n * n * sizeof(float)},
+        {(void **)(void *)&d.sub, (size_t)T->n_int * P * n * sizeof(float)},
+        {(void **)(void *)&d.prof, (size_t)nn * P * n * sizeof(float)},
+        {(void **)(void *)&d.states, (size_t)nn * P},
+        {(void **)(void *)&d.part, (size_t)std::max(T->R_all, 1) * P * sizeof(double)},
+        {(void **)(void *)&d.part2, (size_t)std::max(rowsum_slices(T->R_all), 1) * P * sizeof(double)},
+        {(void **)(void *)&d.seqlh, (size_t)P * sizeof(double)},
+        {(void **)(void *)&d.jseqlh, (size_t)P * sizeof(double)},
+        {(void **)(void *)&d.bpart, (size_t)ttk::root_blocks(P) * sizeof(double)},
+        {(void **)(void *)&d.total, sizeof(double)},
         {(void **)&T->d_eig, TT_MAX_STATES * sizeof(double)},
         {(void **)&T->d_v, TT_MAX_STATES * TT_MAX_STATES * sizeof(double)},
         {(void **)&T->d_vinv, TT_MAX_STATES * TT_MAX_STATES * sizeof(double)},
