@@ -464,6 +464,19 @@ int32_t ensure_joint(tt_tree *T) {
         T->d.Cx = nullptr;
         return rc;
     }
+    if (!T->site && T->d.n_cc > 0) {  // the tip tables of wide alphabets
+        const size_t cells = (size_t)T->n_tips * T->d.n_cc * T->n;
+        if ((rc = T->alloc((void **)(void *)&T->d.jtab, cells * sizeof(float))) ||
+            (rc = T->alloc((void **)(void *)&T->d.jcx, cells))) {
+            for (void *q : {(void *)T->d.jtab, (void *)T->d.Lx, (void *)T->d.Cx})
+                if (q) T->release(q);
+            T->d.jtab = nullptr;
+            T->d.jcx = nullptr;
+            T->d.Lx = nullptr;
+            T->d.Cx = nullptr;
+            return rc;
+        }
+    }
     T->joint_site = T->site;
     T->joint_ready = true;
     return TT_OK;
@@ -565,6 +578,7 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
     } else if (mode == 4) {
         if (int32_t rc = enqueue_joint_tail(T, tips, true, s)) return rc;
     } else {
+        if (!T->site) TT_HIP(ttk::joint_tiptab(D, T->d_tip_nodes, s));
         for (size_t h = 0; h + 1 < T->post_lv.size(); ++h) {
             auto &L = T->post_lv[h];
             LevelDev ld{(decltype(LevelDev::nodes))(T->d_post_nodes + L.off),
@@ -1176,6 +1190,35 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     d.lutj32 = (decltype(d.lutj32))p_lutj32;
     d.mult = (decltype(d.mult))p_mult;
     d.pi = (decltype(d.pi))p_pi;
+    d.tip_cc = nullptr;
+    d.lutj_c = nullptr;
+    d.jtab = nullptr;
+    d.jcx = nullptr;
+    d.n_cc = 0;
+    if (n > 8 && T->n_tips > 0) {
+        // the codes the tips use, numbered in code order: the joint pass takes
+        // a tip child's message from its branch's table of these codes
+        std::vector<int> idx(256, -1);
+        for (size_t i = 0; i < ncodes_total; ++i) idx[D->tip_codes[i]] = 0;
+        int m = 0;
+        std::vector<float> lutj_c;
+        for (int c = 0; c < 256; ++c)
+            if (idx[c] == 0) {
+                idx[c] = m++;
+                lutj_c.insert(lutj_c.end(), lutj32.begin() + (size_t)c * n, lutj32.begin() + (size_t)(c + 1) * n);
+            }
+        std::vector<uint8_t> cc(ncodes_total);
+        for (size_t i = 0; i < ncodes_total; ++i) cc[i] = (uint8_t)idx[D->tip_codes[i]];
+        uint8_t *p_cc;
+        float *p_lutj_c;
+        if ((rc = T->upload(&p_cc, cc.data(), cc.size())) || (rc = T->upload(&p_lutj_c, lutj_c.data(), lutj_c.size()))) {
+            delete T;
+            return rc;
+        }
+        d.tip_cc = (decltype(d.tip_cc))p_cc;
+        d.lutj_c = (decltype(d.lutj_c))p_lutj_c;
+        d.n_cc = m;
+    }
     struct {
         void **p;
         size_t bytes;
@@ -1666,7 +1709,7 @@ int32_t tt_tree_trim(tt_tree *T) {
             ++it;
         }
     }
-    for (void *q : {(void *)T->d.Lx, (void *)T->d.Lx64, (void *)T->d.Cx}) {
+    for (void *q : {(void *)T->d.Lx, (void *)T->d.Lx64, (void *)T->d.Cx, (void *)T->d.jtab, (void *)T->d.jcx}) {
         if (!q) continue;
         for (size_t i = 0; i < T->allocs.size(); ++i)
             if (T->allocs[i].first == q) {
@@ -1680,6 +1723,8 @@ int32_t tt_tree_trim(tt_tree *T) {
     T->d.Lx = nullptr;
     T->d.Lx64 = nullptr;
     T->d.Cx = nullptr;
+    T->d.jtab = nullptr;
+    T->d.jcx = nullptr;
     T->joint_ready = false;
     if (T->last_pass == 2) T->last_pass = 0;
     return TT_OK;
@@ -2171,6 +2216,7 @@ int32_t batch_plan_run(tt_engine *eng, tt_tree *const *trees, int32_t k, uint32_
         for (int i = 0; i < k; ++i) {
             tt_tree *T = trees[i];
             bt[i].T = T->d;
+            bt[i].T.jtab = nullptr;  // the tip tables belong to the level-kernel joint pass
             bt[i].post_nodes = (decltype(bt[i].post_nodes))T->d_post_nodes;
             bt[i].post_desc = (decltype(bt[i].post_desc))T->d_post_desc;
             bt[i].pre_desc = (decltype(bt[i].pre_desc))T->d_pre_desc;

@@ -63,6 +63,16 @@ struct TreeDev {
     double TT_G *bpart;              // [nblocks]       block partials of sum(mult * seqlh)
     double TT_G *total;              // [1]
     double TT_G *offs;               // [n_int][P] per-node log normaliser, written only in update mode
+    // joint pass of alphabets of more than 8 states: the tips' character
+    // codes numbered 0..n_cc-1 (the codes in use, in code order), and per tip
+    // branch and code the tip's message max_i max(TINY, P_ij) max(profile_i,
+    // TINY) with its first argmax i, formed at the start of each joint pass
+    // (jtab null: not allocated)
+    const uint8_t TT_C *tip_cc;      // [n_tips][P] compact code of each tip character
+    const float TT_C *lutj_c;        // [n_cc][n]   max(profile, TINY) in FP32 of each compact code
+    float TT_G *jtab;                // [n_tips][n_cc][n]
+    uint8_t TT_G *jcx;               // [n_tips][n_cc][n]
+    int32_t n_cc;
     // site-specific model (all null for a model shared by every pattern):
     // per pattern p, P_p(t) = V diag(exp(t rho)) V^-1 with V [p][i][k],
     // V^-1 [p][k][j], rho [p][k] = mu_p * eigenvalue_k, root prior pi [p][n];
@@ -126,6 +136,8 @@ hipError_t post_run20(const TreeDev &T, const RunDev &R, hipStream_t s);  // MFM
 hipError_t pre_run20(const TreeDev &T, const RunDev &R, hipStream_t s);   // MFMA, n = 20
 hipError_t post_marginal20(const TreeDev &T, const LevelDev &L, hipStream_t s);  // MFMA, n = 20
 hipError_t pre_marginal20(const TreeDev &T, const LevelDev &L, hipStream_t s);   // MFMA, n = 20
+// jtab / jcx of every tip branch (n > 8, jtab allocated)
+hipError_t joint_tiptab(const TreeDev &T, const int32_t *tip_nodes, hipStream_t s);
 hipError_t post_joint(const TreeDev &T, const LevelDev &L, int store_tip_cx, hipStream_t s);
 // site-specific models (tt_site.hip), alphabets of up to 5 states
 hipError_t post_site(const TreeDev &T, const LevelDev &L, hipStream_t s);

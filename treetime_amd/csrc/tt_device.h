@@ -644,6 +644,22 @@ __device__ __forceinline__ void tip_joint_code(const TreeDev &T, LJ lutj, int co
 template <int N>
 __device__ __forceinline__ void tip_joint(const TreeDev &T, int cref, int cnode, int p, JR<N> (&msg)[N],
                                           int store_cx) {
+    if constexpr (N > 8) {
+        if (T.jtab) {  // the tip branch's table row of the pattern's code (the same numbers as below)
+            const int tr = -cref - 1;
+            const size_t row = ((size_t)tr * T.n_cc + T.tip_cc[(size_t)tr * T.P + p]) * N;
+            float m[N];
+            load_rowf<N>(T.jtab + row, m);
+#pragma unroll
+            for (int j = 0; j < N; ++j) msg[j] *= m[j];
+            if (store_cx) {
+                uint8_t TT_G *cxo = T.Cx + ((size_t)cnode * T.P + p) * N;
+#pragma unroll
+                for (int j = 0; j < N; ++j) cxo[j] = T.jcx[row + j];
+            }
+            return;
+        }
+    }
     tip_joint_code<N>(T, joint_lut<N>(T), T.tip_codes[(size_t)(-cref - 1) * T.P + p], cnode, p, msg, store_cx);
 }
 

@@ -426,7 +426,49 @@ inline unsigned grid_for_tasks(long tasks) { return (unsigned)((tasks + 3) / 4);
         default: return hipErrorInvalidValue;            \
     }
 
+namespace {
+
+// Tip tables of the joint pass (n > 8): for tip row t, compact code c and
+// parent state j, the best child state's message max_i max(TINY, P_ij)
+// max(profile_i, TINY) and the first argmax, with the arithmetic of
+// tip_joint_code (so the table holds the same numbers per pattern).
+template <int N>
+__global__ void k_joint_tiptab(TreeDev T, const int32_t *tip_nodes) {
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int mN = T.n_cc * N;
+    if (gid >= (long)T.n_tips * mN) return;
+    const int t = (int)(gid / mN), r = (int)(gid % mN), c = r / N, j = r % N;
+    const float TT_C *PJ = T.Pj32 + (size_t)tip_nodes[t] * (N * N);
+    const float TT_C *lj = T.lutj_c + c * N;
+    float best = PJ[j] * lj[0];
+    int bi = 0;
+    for (int i = 1; i < N; ++i) {
+        const float val = PJ[i * N + j] * lj[i];
+        if (val > best) {
+            best = val;
+            bi = i;
+        }
+    }
+    T.jtab[(size_t)t * mN + r] = best;
+    T.jcx[(size_t)t * mN + r] = (uint8_t)bi;
+}
+
+}  // namespace
+
 namespace ttk {
+
+hipError_t joint_tiptab(const TreeDev &T, const int32_t *tip_nodes, hipStream_t s) {
+    const long total = (long)T.n_tips * T.n_cc * T.n;
+    if (!T.jtab || total <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    switch (T.n) {
+        case 20: hipLaunchKernelGGL(k_joint_tiptab<20>, grid, block, 0, s, T, tip_nodes); break;
+        case 21: hipLaunchKernelGGL(k_joint_tiptab<21>, grid, block, 0, s, T, tip_nodes); break;
+        case 22: hipLaunchKernelGGL(k_joint_tiptab<22>, grid, block, 0, s, T, tip_nodes); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 int root_blocks(int P) { return (P + TT_BLOCK - 1) / TT_BLOCK; }
 
