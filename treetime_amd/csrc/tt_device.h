@@ -742,6 +742,45 @@ __device__ __forceinline__ void joint_store(const TreeDev &T, int v, int vn, int
     if constexpr (N <= 8) cx_store<N>(T, vn, p, cx);
 }
 
+// joint_store for alphabets of more than 8 states with max(TINY, P) of the
+// branch read from LDS, transposed (sp[j N + i] = max(TINY, P_ij)): every
+// read is one address for the whole wave, and the 400 entries never have to
+// live in scalar registers.  Same arithmetic, in the same order.
+template <int N>
+__device__ __forceinline__ void joint_store_lds(const TreeDev &T, int v, int vn, int p, const float (&msg)[N],
+                                                const float *sp) {
+    const auto dst = T.Lx + ((size_t)v * T.P + p) * N;
+    const auto cxo = T.Cx + ((size_t)vn * T.P + p) * N;
+    float best[N];
+    float off = 0.0f;
+    int jm = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const float *col = sp + j * N;
+        float b = col[0] * msg[0];
+        int bi = 0;
+#pragma unroll
+        for (int i = 1; i < N; ++i) {
+            const float val = col[i] * msg[i];
+            if (val > b) {
+                b = val;
+                bi = i;
+            }
+        }
+        best[j] = b;
+        if (b > off) {
+            off = b;
+            jm = j;
+        }
+        cxo[j] = (uint8_t)bi;
+    }
+    const float inv = 1.0f / off;
+    float out[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : best[j] * inv;
+    store_row<N>(dst, out);
+}
+
 template <int N>
 __device__ __forceinline__ void post_joint_node(const TreeDev &T, int v, int vn, int p, int store_tip_cx) {
     JR<N> msg[N];

@@ -187,6 +187,38 @@ __global__ __launch_bounds__(TT_BLOCK) void k_post_joint(TreeDev T, LevelDev L, 
     }
 }
 
+// Joint postorder level for alphabets of more than 8 states: as
+// k_post_joint, with each node's max(TINY, P) staged transposed in the
+// wave's own LDS slice (joint_store_lds).
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_post_joint_wide(TreeDev T, LevelDev L, int n_ptiles,
+                                                              int store_tip_cx) {
+    __shared__ float s_p[TT_BLOCK / TT_WAVE][N * N];
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    const int pt = blockIdx.x % n_ptiles, cg = blockIdx.x / n_ptiles;
+    const int p = pt * TT_WAVE + lane;
+    const int chunk = cg * (TT_BLOCK / TT_WAVE) + w;
+    if (chunk >= L.n_chunks) return;  // whole wave
+    const int kb = chunk * L.chunk;
+    const int ke = min(kb + L.chunk, L.n_nodes);
+    float *sp = s_p[w];
+    for (int k = kb; k < ke; ++k) {
+        const int v = wave_uniform(L.nodes[k]);
+        const int vn = wave_uniform(T.int_node[v]);
+        const float TT_C *PJ = T.Pj32 + (size_t)vn * (N * N);
+        __builtin_amdgcn_wave_barrier();  // the previous node's reads of the slice come first
+        for (int q = lane; q < N * N; q += TT_WAVE) sp[(q % N) * N + q / N] = PJ[q];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (p < T.P) {
+            float msg[N];
+            joint_children<N>(T, v, p, msg, store_tip_cx);
+            joint_store_lds<N>(T, v, vn, p, msg, sp);
+        }
+    }
+}
+
 // Joint root state (treeanc.py:1015-1037), one thread per pattern.
 template <int N>
 __global__ __launch_bounds__(TT_BLOCK) void k_root_joint(TreeDev T, int store_tip_cx) {
@@ -608,7 +640,13 @@ hipError_t ttk::post_joint(const TreeDev &T, const LevelDev &L, int store_tip_cx
     if (L.n_nodes <= 0) return hipSuccess;
     const int npt = n_ptiles(T.P);
     const int groups = (L.n_chunks + 3) / 4;
-    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_post_joint<N_>, dim3(groups * npt), dim3(TT_BLOCK), 0, s, T, L, npt,
-                                        store_tip_cx));
+    const dim3 grid(groups * npt), block(TT_BLOCK);
+    switch (T.n) {  // wide alphabets: the branch matrix from LDS
+        case 20: hipLaunchKernelGGL(k_post_joint_wide<20>, grid, block, 0, s, T, L, npt, store_tip_cx); break;
+        case 21: hipLaunchKernelGGL(k_post_joint_wide<21>, grid, block, 0, s, T, L, npt, store_tip_cx); break;
+        case 22: hipLaunchKernelGGL(k_post_joint_wide<22>, grid, block, 0, s, T, L, npt, store_tip_cx); break;
+        default:
+            TT_DISPATCH(T.n, hipLaunchKernelGGL(k_post_joint<N_>, grid, block, 0, s, T, L, npt, store_tip_cx));
+    }
     return hipGetLastError();
 }
