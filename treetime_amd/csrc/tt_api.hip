@@ -500,7 +500,19 @@ constexpr uint32_t kPlanJoint = 4u;
 int32_t enqueue_joint_tail(tt_tree *T, bool tips, bool given_root, hipStream_t s) {
     const TreeDev &D = T->d;
     const int nb = ttk::root_blocks(T->P);
-    for (auto &L : T->pre_lv) {
+    const std::vector<LvRun> &runs = T->pre_runs[tips ? 1 : 0];
+    size_t ri = 0;
+    for (size_t d = 0; d < T->pre_lv.size();) {
+        if (ri < runs.size() && runs[ri].l0 == (int)d) {  // narrow levels in one launch
+            const LvRun &r = runs[ri++];
+            RunDev rd{(decltype(RunDev::nodes))T->d_pre_nodes, (decltype(RunDev::desc))T->d_pre_desc,
+                      (decltype(RunDev::off))T->d_pre_off, (decltype(RunDev::nint))T->d_pre_nint, r.l0, r.l1, 0, 0,
+                      tips ? 1 : 0};
+            TT_HIP(ttk::backtrack_run(D, rd, s));
+            d = r.l1;
+            continue;
+        }
+        const PreLv &L = T->pre_lv[d++];
         LevelDev ld{(decltype(LevelDev::nodes))(T->d_pre_nodes + L.off),
                     (decltype(LevelDev::desc))(T->d_pre_desc + kPreDesc * L.off), tips ? L.n_all : L.n_int, 1, 0, 0};
         TT_HIP(ttk::backtrack(D, ld, s));
@@ -579,8 +591,23 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
         if (int32_t rc = enqueue_joint_tail(T, tips, true, s)) return rc;
     } else {
         if (!T->site) TT_HIP(ttk::joint_tiptab(D, T->d_tip_nodes, s));
-        for (size_t h = 0; h + 1 < T->post_lv.size(); ++h) {
-            auto &L = T->post_lv[h];
+        const int H = (int)T->post_lv.size() - 1;  // the root level runs on its own
+        size_t ri = 0;
+        for (int h = 0; h < H;) {
+            // narrow levels below the root in one launch (shared model, n <= 8;
+            // the site-specific pass keeps per-level normaliser rows)
+            while (ri < T->post_runs.size() && T->post_runs[ri].l0 < h) ++ri;
+            if (!T->site && T->n <= 8 && ri < T->post_runs.size() && T->post_runs[ri].l0 == h &&
+                std::min(T->post_runs[ri].l1, H) - h >= 2) {
+                const int l1 = std::min(T->post_runs[ri].l1, H);
+                RunDev rd{(decltype(RunDev::nodes))T->d_post_nodes, (decltype(RunDev::desc))T->d_post_desc,
+                          (decltype(RunDev::off))T->d_post_off, nullptr, h, l1, 0, 0, 0};
+                TT_HIP(ttk::post_joint_run(D, rd, tips ? 1 : 0, s));
+                h = l1;
+                ++ri;
+                continue;
+            }
+            auto &L = T->post_lv[h++];
             LevelDev ld{(decltype(LevelDev::nodes))(T->d_post_nodes + L.off),
                         (decltype(LevelDev::desc))(T->d_post_desc + kPostDesc * L.off), L.cnt, L.chunk, L.nchunks,
                         L.row_off};

@@ -131,6 +131,10 @@ hipError_t root_marginal(const TreeDev &T, int S, hipStream_t s);
 hipError_t final_sum(const TreeDev &T, int nblocks, hipStream_t s);
 hipError_t pre_marginal(const TreeDev &T, const LevelDev &L, hipStream_t s);
 hipError_t post_run(const TreeDev &T, const RunDev &R, hipStream_t s);
+// joint pass (n <= 8): runs of narrow postorder levels (no root) and of
+// narrow backtrack levels in one launch each, as post_run / pre_run
+hipError_t post_joint_run(const TreeDev &T, const RunDev &R, int store_tip_cx, hipStream_t s);
+hipError_t backtrack_run(const TreeDev &T, const RunDev &R, hipStream_t s);
 hipError_t pre_run(const TreeDev &T, const RunDev &R, hipStream_t s);
 hipError_t post_run20(const TreeDev &T, const RunDev &R, hipStream_t s);  // MFMA, n = 20
 hipError_t pre_run20(const TreeDev &T, const RunDev &R, hipStream_t s);   // MFMA, n = 20

@@ -219,6 +219,42 @@ __global__ __launch_bounds__(TT_BLOCK) void k_post_joint_wide(TreeDev T, LevelDe
     }
 }
 
+// Runs of narrow joint levels in one launch (as k_post_run): a block of 4
+// waves takes one 64-pattern tile through every level of the run, the
+// level's nodes dealt to the waves, a block barrier between levels.
+template <int N>
+__global__ __launch_bounds__(TT_BLOCK) void k_post_joint_run(TreeDev T, RunDev R, int store_tip_cx) {
+    constexpr int kW = TT_BLOCK / TT_WAVE;
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    const int p = blockIdx.x * TT_WAVE + lane;
+    for (int l = R.l0; l < R.l1; ++l) {
+        const int kb = R.off[l], ke = R.off[l + 1];
+        for (int k = wave_uniform(kb + w); k < ke; k += kW) {
+            const int v = wave_uniform(R.nodes[k]);
+            if (p < T.P) post_joint_node<N>(T, v, wave_uniform(T.int_node[v]), p, store_tip_cx);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(TT_BLOCK) void k_backtrack_run(TreeDev T, RunDev R) {
+    constexpr int kW = TT_BLOCK / TT_WAVE;
+    const int lane = threadIdx.x & (TT_WAVE - 1), w = threadIdx.x / TT_WAVE;
+    const int p = blockIdx.x * TT_WAVE + lane;
+    for (int l = R.l0; l < R.l1; ++l) {
+        const int kb = R.off[l];
+        const int ke = R.tips ? R.off[l + 1] : kb + R.nint[l];
+        for (int k = wave_uniform(kb + w); k < ke; k += kW) {
+            const int c = wave_uniform(R.nodes[k]);
+            if (p < T.P) {
+                const int ps = T.states[(size_t)T.parent[c] * T.P + p];
+                T.states[(size_t)c * T.P + p] = cx_state(T, c, p, ps);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Joint root state (treeanc.py:1015-1037), one thread per pattern.
 template <int N>
 __global__ __launch_bounds__(TT_BLOCK) void k_root_joint(TreeDev T, int store_tip_cx) {
@@ -595,6 +631,18 @@ hipError_t joint_score(const TreeDev &T, int S, int G, hipStream_t s) {
 
 hipError_t joint_final(const TreeDev &T, int S2, hipStream_t s) {
     TT_DISPATCH(T.n, hipLaunchKernelGGL(k_joint_final<N_>, dim3(root_blocks(T.P)), dim3(TT_BLOCK), 0, s, T, S2));
+    return hipGetLastError();
+}
+
+hipError_t post_joint_run(const TreeDev &T, const RunDev &R, int store_tip_cx, hipStream_t s) {
+    if (T.n > 8) return hipErrorInvalidValue;
+    TT_DISPATCH(T.n, hipLaunchKernelGGL(k_post_joint_run<N_>, dim3(n_ptiles(T.P)), dim3(TT_BLOCK), 0, s, T, R,
+                                        store_tip_cx));
+    return hipGetLastError();
+}
+
+hipError_t backtrack_run(const TreeDev &T, const RunDev &R, hipStream_t s) {
+    hipLaunchKernelGGL(k_backtrack_run, dim3(n_ptiles(T.P)), dim3(TT_BLOCK), 0, s, T, R);
     return hipGetLastError();
 }
 
