@@ -93,10 +93,11 @@ int32_t tt_tree_info(const tt_tree *tree, int32_t *n_levels_post, int32_t *n_lev
  * the pattern tile of the one-launch pass (0: level kernels), records_staged
  * 1 when that pass stages its per-node records in LDS, slots_reused 1 when
  * its LDS rows are reused by lifetime (the preorder then re-reads subtree
- * rows from memory).  Introspection for measurement; no counterpart in the
- * reference. */
+ * rows from memory); joint_one_launch 1 when the joint pass also runs in one
+ * launch over the same tiles.  Introspection for measurement; no
+ * counterpart in the reference.  Any output pointer may be NULL. */
 int32_t tt_tree_pass_shape(const tt_tree *tree, int32_t *tile_patterns, int32_t *records_staged,
-                           int32_t *slots_reused);
+                           int32_t *slots_reused, int32_t *joint_one_launch);
 
 /* GTR eigensystem (gtr.py:612-629): P(t) = v diag(exp(mu t lambda)) v_inv,
  * clipped at 0 (gtr.py:1051-1067).  Rebuilds P(t) of every branch in FP64 on

@@ -568,3 +568,51 @@ def test_one_launch_pass_equals_level_kernels(gpu_engine, case, stage, reuse, mo
     assert np.array_equal(one.states(), lv.states())
     lv.close()
     one.close()
+
+
+@pytest.mark.parametrize("case", ONE_LAUNCH_CASES, ids=[c[0] for c in ONE_LAUNCH_CASES])
+@pytest.mark.parametrize("reuse", ["0", "1"], ids=["rows", "reused"])
+def test_one_launch_joint_equals_level_kernels(gpu_engine, case, reuse, monkeypatch):
+    """The one-launch joint pass (postorder max-product rows in LDS, root,
+    backtrack and score in the same launch) gives the level kernels' states,
+    message rows and argmax rows bit for bit and their joint logL to
+    rounding, with and without reconstructed tips, after new branch lengths,
+    and a backtrack from given root states after it equals theirs."""
+    say(f"[one-launch joint] case={case[0]} reuse={reuse}")
+    job, prep, alphabet = make_case(*case)
+    monkeypatch.setenv("TT_TILE", "0")
+    lv = device_tree(gpu_engine, prep)
+    monkeypatch.setenv("TT_TILE", "1")
+    monkeypatch.setenv("TT_TILE_REUSE", reuse)
+    one = device_tree(gpu_engine, prep)
+    assert not lv.joint_one_launch() and one.joint_one_launch()
+    parent = np.asarray(lv.parent)
+    internal = np.isin(np.arange(len(parent)), parent[parent >= 0])
+    for tips in (False, True, False):
+        a, b = lv.joint(reconstruct_tips=tips), one.joint(reconstruct_tips=tips)
+        assert abs(a - b) <= 1e-12 * abs(a), (tips, a, b)
+        assert site_err(one.sequence_lh(), lv.sequence_lh()) <= 1e-12
+        assert np.array_equal(one.states(), lv.states()), tips
+        la, ca = lv.joint_debug()
+        lb, cb = one.joint_debug()
+        # argmax rows exist for every node with a branch: internal ones always,
+        # tips when their states are reconstructed
+        rows = (parent >= 0) & (internal | tips)
+        assert np.array_equal(la, lb) and np.array_equal(ca[rows], cb[rows]), tips
+    nodes = lv.nonroot_nodes()[:3]
+    for dt in (lv, one):
+        dt.set_branch_lengths(nodes, dt.t_cur[nodes] * 1.7)
+    a, b = lv.joint(), one.joint()
+    assert abs(a - b) <= 1e-12 * abs(a)
+    assert np.array_equal(one.states(), lv.states())
+    # the marginal pass of the same tree still equals the level kernels' after a joint pass
+    a, b = lv.marginal(), one.marginal()
+    assert abs(a - b) <= 1e-12 * abs(a)
+    assert np.array_equal(one.profiles(), lv.profiles())
+    a, b = lv.joint(), one.joint()
+    rs = (np.arange(lv.P) % lv.n).astype(np.uint8)
+    a, b = lv.joint_from_root(rs), one.joint_from_root(rs)
+    assert abs(a - b) <= 1e-12 * abs(a)
+    assert np.array_equal(one.states(), lv.states())
+    lv.close()
+    one.close()

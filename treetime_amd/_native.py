@@ -85,7 +85,7 @@ SIGNATURES = {
     "tt_tree_create": (_I32, [_P, C.POINTER(tt_tree_desc), C.POINTER(_P)]),
     "tt_tree_destroy": (_I32, [_P]),
     "tt_tree_info": (_I32, [_P, _PI32, _PI32, C.POINTER(_U64)]),
-    "tt_tree_pass_shape": (_I32, [_P, _PI32, _PI32, _PI32]),
+    "tt_tree_pass_shape": (_I32, [_P, _PI32, _PI32, _PI32, _PI32]),
     "tt_tree_set_model": (_I32, [_P, _PD, _PD, _PD, _D]),
     "tt_tree_set_branch_lengths": (_I32, [_P, _PI32, _PD, _I32]),
     "tt_marginal": (_I32, [_P, _U32, _PD, _P]),

@@ -207,6 +207,7 @@ struct tt_tree {
     bool tile_stale = true;  // the records do not hold the current P(t) yet
     bool tile_staged = false;  // the kernel stages each phase's records in LDS
     bool tile_subg = false;    // LDS slots are reused; the preorder re-reads subtree rows from memory
+    bool tile_joint = false;   // the joint pass runs in one launch over the same tiles (k_tile_joint)
     int tile_slots = 0, tile_root_post = 0, tile_root_pre = 0;
     int32_t *d_tile_post_slot = nullptr, *d_tile_pre_slot = nullptr;  // slot of every internal row / node
     bool have_states = false;  // d.states holds a marginal or joint reconstruction
@@ -589,6 +590,13 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
         }
     } else if (mode == 4) {
         if (int32_t rc = enqueue_joint_tail(T, tips, true, s)) return rc;
+    } else if (T->tile_joint && !T->site) {  // one launch: postorder, root, backtrack and score
+        if (fill) TT_HIP(ttk::fill_states(D, T->d_tip_nodes, T->n_tips, 255, s));
+        ttk::TileArgs A{T->d_tile_post_rec, T->d_tile_pre_rec, T->d_tile_rounds, T->tile_Rp, T->tile_Ri,
+                        T->tile_Ra, T->tile_tp, (int)T->post_nodes.size(), (int)T->pre_nodes.size(),
+                        T->tile_slots, T->tile_root_post, T->tile_root_pre, T->d_tile_post_slot, T->tile_staged,
+                        T->tile_subg, T->d_tile_partial, T->d_tile_counter};
+        TT_HIP(ttk::tile_joint(D, A, T->d_pre_desc, tips ? 1 : 0, s));
     } else {
         if (!T->site) TT_HIP(ttk::joint_tiptab(D, T->d_tip_nodes, s));
         const int H = (int)T->post_lv.size() - 1;  // the root level runs on its own
@@ -649,7 +657,7 @@ int32_t run_pass(tt_tree *T, int mode, uint32_t flags, double *logL, void *strea
     const bool fill = !tips && T->tips_dirty && (mode == 1 || mode == 2);
     flags = (flags & TT_RECONSTRUCT_TIPS) | (fill ? kFillTips : 0u);
     const int key = mode * 8 + (tips ? 1 : 0) + (fill ? 2 : 0);
-    if (T->tile_tp && !T->site && (mode == 1 || mode == 3) && T->tile_stale) {
+    if (T->tile_tp && !T->site && (mode == 1 || mode == 3 || (mode == 2 && T->tile_joint)) && T->tile_stale) {
         TT_HIP(ttk::tile_pack(T->d, T->d_post_desc, (int)T->post_nodes.size(), T->d_pre_desc,
                               (int)T->pre_nodes.size(), T->d_tile_post_slot, T->d_tile_pre_slot, T->tile_subg,
                               T->d_tile_post_rec, T->d_tile_pre_rec, s));
@@ -1089,6 +1097,16 @@ int32_t setup_tile(tt_tree *T) {
     T->tile_staged = staged;
     T->tile_subg = subg;
     T->tile_stale = true;
+    // the joint pass takes the same tiles in one launch when its LDS (message
+    // rows, both phases' descriptors, argmax rows and states) fits
+    // (TT_TILE_JOINT=0: level kernels)
+    T->tile_joint = false;
+    const char *ej = getenv("TT_TILE_JOINT");
+    if (!(ej && ej[0] == '0')) {
+        const size_t jl = ttk::tile_joint_lds_bytes(T->d, tp, T->tile_slots, 2 * (Rp + Ri + Ra),
+                                                    (int)T->post_nodes.size(), (int)T->pre_nodes.size());
+        T->tile_joint = jl <= budget && ttk::tile_joint_blocks_per_cu(T->n, jl) >= 1;
+    }
     return TT_OK;
 }
 
@@ -1324,12 +1342,13 @@ int32_t tt_tree_info(const tt_tree *T, int32_t *n_levels_post, int32_t *n_levels
 }
 
 int32_t tt_tree_pass_shape(const tt_tree *T, int32_t *tile_patterns, int32_t *records_staged,
-                           int32_t *slots_reused) {
+                           int32_t *slots_reused, int32_t *joint_one_launch) {
     if (!T) return fail(TT_EINVAL, "tree is NULL");
     const bool tile = T->tile_tp && !T->site;
     if (tile_patterns) *tile_patterns = tile ? T->tile_tp : 0;
     if (records_staged) *records_staged = tile && T->tile_staged ? 1 : 0;
     if (slots_reused) *slots_reused = tile && T->tile_subg ? 1 : 0;
+    if (joint_one_launch) *joint_one_launch = tile && T->tile_joint ? 1 : 0;
     return TT_OK;
 }
 

@@ -362,6 +362,284 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
     }
 }
 
+// One-launch joint pass (treeanc.py:934-1080) over the same tiles, slots and
+// rounds: the postorder rounds form each internal row's max-product message
+// (joint_children / joint_store arithmetic: the same products, comparisons
+// and first-max rule, so messages, argmax rows and states are the level
+// kernels' own), the message rows live in the LDS slots, the argmax rows and
+// the decoded states in LDS as well (and all of them go to memory as
+// results); the root step picks the root states, the backtrack rounds over
+// the depth-ordered nodes give every node its state and add its FP64 score
+// term, and the workgroups' sums of mult * logL make the total as in the
+// marginal pass.  The descriptors of both phases are staged in LDS.
+template <int N>
+__global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev S, const int32_t *pre_desc,
+                                                             int tips) {
+    using R = JR<N>;
+    constexpr int CB = (int)cx_row_bytes(N);
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    const int TP = S.TP;
+    // LDS layout (tile_joint_lds_bytes): message rows, reduction, round
+    // offsets, postorder descriptors, preorder descriptors, tip codes,
+    // argmax rows, states
+    float *rows = reinterpret_cast<float *>(s_dyn);                                   // [n_slots][TP][N]
+    double *s_red = reinterpret_cast<double *>(rows + align4((size_t)S.n_slots * TP * N));  // [kTileThreads]
+    int *s_rounds = reinterpret_cast<int *>(s_red + kTileThreads);
+    const int n_rounds = 2 * (S.Rp + S.Ri + S.Ra);
+    int4 *s_post = reinterpret_cast<int4 *>(s_rounds + align4(n_rounds));  // [n_post][2]
+    int4 *s_pre = s_post + 2 * (size_t)S.n_post;                          // [n_pre]
+    uint8_t *s_tip = reinterpret_cast<uint8_t *>(s_pre + S.n_pre);        // [n_tips][TP]
+    uint8_t *s_cx = s_tip + (((size_t)T.n_tips * TP + 15) & ~(size_t)15);             // [n_nodes][TP][CB]
+    uint8_t *s_st = s_cx + (((size_t)T.n_nodes * TP * CB + 15) & ~(size_t)15);       // [n_nodes][TP]
+    __shared__ double s_sum[TT_WAVE];
+    __shared__ bool s_last;
+
+    const int t = threadIdx.x;
+    const int q = t & (TP - 1);
+    const int k0 = t / TP;
+    const int p = blockIdx.x * TP + q;
+    const bool act = p < T.P;
+
+    for (int i = t; i < n_rounds; i += kTileThreads) s_rounds[i] = S.rounds[i];
+    for (int i = t; i < 2 * S.n_post; i += kTileThreads)
+        s_post[i] = reinterpret_cast<const int4 *>(S.post_rec + (size_t)post_rec_words(N) * (i >> 1))[i & 1];
+    for (int i = t; i < S.n_pre; i += kTileThreads) s_pre[i] = reinterpret_cast<const int4 *>(pre_desc)[i];
+    for (int i = t; i < T.n_tips * TP; i += kTileThreads) {
+        const int r = i / TP, pp = blockIdx.x * TP + (i & (TP - 1));
+        s_tip[i] = pp < T.P ? T.tip_codes[(size_t)r * T.P + pp] : 0;
+    }
+    lds_barrier();
+
+    auto slot = [&](int sl) { return rows + ((size_t)sl * TP + q) * N; };
+    const auto PJ = joint_P<N>(T);
+    const auto lutj = joint_lut<N>(T);
+    auto put_cx = [&](int node, const int (&cx)[N]) {
+        uint8_t *d = s_cx + ((size_t)node * TP + q) * CB;
+        if constexpr (cx_packed(N)) {
+            uint32_t b = 0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) b |= (uint32_t)cx[j] << (2 * j);
+            d[0] = (uint8_t)b;
+        } else {
+#pragma unroll
+            for (int j = 0; j < N; ++j) d[j] = (uint8_t)cx[j];
+        }
+    };
+    auto get_cx = [&](int node, int ps) -> int {
+        const uint8_t *d = s_cx + ((size_t)node * TP + q) * CB;
+        if constexpr (cx_packed(N))
+            return (d[0] >> (2 * ps)) & 3;
+        else
+            return d[ps];
+    };
+    // tip child (tip_joint_code): msg_j *= max_i max(TINY, P_ij) max(profile_i, TINY)
+    auto tip_child = [&](int cnode, int tr, R (&msg)[N]) {
+        const auto lj = lutj + s_tip[tr * TP + q] * N;
+        const auto P0 = PJ + (size_t)cnode * (N * N);
+        R tl[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) tl[i] = lj[i];
+        int cx[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            R best = P0[j] * tl[0];
+            int bi = 0;
+#pragma unroll
+            for (int i = 1; i < N; ++i) {
+                const R val = P0[i * N + j] * tl[i];
+                if (val > best) {
+                    best = val;
+                    bi = i;
+                }
+            }
+            msg[j] *= best;
+            cx[j] = bi;
+        }
+        put_cx(cnode, cx);
+        if (tips) cx_store<N>(T, cnode, p, cx);
+    };
+    // product over the children of the descriptor's row (joint_children)
+    auto children = [&](const int4 &a, const int4 &b, R (&msg)[N]) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) msg[i] = R(1);
+        if (a.z == 2) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int cref = c ? b.x : a.w;
+                if (cref < 0) {
+                    tip_child(c ? b.z : b.y, -cref - 1, msg);
+                } else {
+                    const float *r = slot(cref);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) msg[i] *= (R)r[i];
+                }
+            }
+        } else {
+            const int e0 = T.child_ptr[a.x], e1 = T.child_ptr[a.x + 1];
+            for (int e = e0; e < e1; ++e) {
+                const int cref = T.child_ref[e];
+                if (cref < 0) {
+                    tip_child(T.child_node[e], -cref - 1, msg);
+                } else {
+                    const float *r = slot(S.post_slot ? S.post_slot[cref] : cref);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) msg[i] *= (R)r[i];
+                }
+                if (e - e0 >= 2) {
+                    R mx = msg[0];
+#pragma unroll
+                    for (int i = 1; i < N; ++i) mx = fmax(mx, msg[i]);
+                    int ex;
+                    (void)frexp(mx, &ex);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) msg[i] = ldexp(msg[i], -ex);
+                }
+            }
+        }
+    };
+
+    // ---- postorder: max-product rounds (the root is the root step's) ---------
+    {
+        const int *rd = s_rounds;
+        for (int r = 0; r < S.Rp; ++r) {
+            const int k = rd[2 * r] + k0;
+            if (k < rd[2 * r + 1] && act) {
+                const int4 a = s_post[2 * k], b = s_post[2 * k + 1];
+                if (a.x != T.root_row) {
+                    R msg[N];
+                    children(a, b, msg);
+                    // joint_store: Lx_j = max_i max(TINY, P_ij) msg_i over the row's own branch
+                    const auto P1 = PJ + (size_t)b.w * (N * N);
+                    R best[N];
+                    int cx[N];
+                    R off = R(0);
+                    int jm = 0;
+#pragma unroll
+                    for (int j = 0; j < N; ++j) {
+                        R bb = P1[j] * msg[0];
+                        int bi = 0;
+#pragma unroll
+                        for (int i = 1; i < N; ++i) {
+                            const R val = P1[i * N + j] * msg[i];
+                            if (val > bb) {
+                                bb = val;
+                                bi = i;
+                            }
+                        }
+                        best[j] = bb;
+                        if (bb > off) {
+                            off = bb;
+                            jm = j;
+                        }
+                        cx[j] = bi;
+                    }
+                    const R inv = R(1) / off;
+                    float out[N];
+#pragma unroll
+                    for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : (float)(best[j] * inv);
+                    float *dst = slot(a.y);
+#pragma unroll
+                    for (int j = 0; j < N; ++j) dst[j] = out[j];
+                    store_row<N>(T.Lx + ((size_t)a.x * T.P + p) * N, out);
+                    cx_store<N>(T, b.w, p, cx);
+                    put_cx(b.w, cx);
+                }
+            }
+            lds_barrier();
+        }
+    }
+
+    // ---- root (treeanc.py:1015-1037): root_joint_state ----------------------
+    if (t < TP && act) {
+        const int k = S.n_post - 1;  // height order: the root comes last
+        R msg[N];
+        children(s_post[2 * k], s_post[2 * k + 1], msg);
+        int jm = 0;
+        R mx = R(-1);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            msg[i] *= (R)T.pi[i];
+            if (msg[i] > mx) {
+                mx = msg[i];
+                jm = i;
+            }
+        }
+        const R inv = R(1) / mx;
+        float out[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) out[i] = (i == jm) ? 1.0f : (float)(msg[i] * inv);
+        store_row<N>(T.Lx + ((size_t)T.root_row * T.P + p) * N, out);
+        T.states[(size_t)T.root * T.P + p] = (uint8_t)jm;
+        s_st[(size_t)T.root * TP + q] = (uint8_t)jm;
+    }
+    lds_barrier();
+
+    // ---- backtrack over every node (treeanc.py:1048-1063) and the score -----
+    double term = 0.0;
+    {
+        const int *rd = s_rounds + 2 * (S.Rp + S.Ri);
+        for (int r = 0; r < S.Ra; ++r) {
+            const int k = rd[2 * r] + k0;
+            if (k < rd[2 * r + 1] && act) {
+                const int4 d = s_pre[k];  // node, parent, internal row or -1, tip row or -1
+                const int c = d.x, sp = s_st[(size_t)d.y * TP + q];
+                if (d.z >= 0) {
+                    const int sc = get_cx(c, sp);
+                    s_st[(size_t)c * TP + q] = (uint8_t)sc;
+                    T.states[(size_t)c * T.P + p] = (uint8_t)sc;
+                    term += joint_int_term<N>(T, c, sc, sp);
+                } else {
+                    if (tips) T.states[(size_t)c * T.P + p] = (uint8_t)get_cx(c, sp);
+                    term += joint_tip_term<N>(T, T.loglut, c, s_tip[d.w * TP + q], sp);
+                }
+            }
+            lds_barrier();
+        }
+    }
+    s_red[t] = act ? term : 0.0;
+    lds_barrier();
+    if (t < TP && act) {
+        double lp = log(T.pi[s_st[(size_t)T.root * TP + q]]);
+        for (int k = q; k < kTileThreads; k += TP) lp += s_red[k];
+        T.jseqlh[p] = lp;
+        s_red[t] = T.mult[p] * lp;
+    }
+    lds_barrier();
+
+    // ---- workgroup sum of mult * logL, then the total over workgroups -------
+    if (t < TT_WAVE) {
+        double a = 0.0;
+        for (int k = t; k < TP; k += TT_WAVE) a += (blockIdx.x * TP + k < T.P) ? s_red[k] : 0.0;
+        s_sum[t] = a;
+    }
+    lds_barrier();
+    if (t == 0) {
+        double a = 0.0;
+        for (int k = 0; k < TT_WAVE; ++k) a += s_sum[k];
+        __hip_atomic_store(&S.partial[blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned done = atomicAdd(S.counter, 1u);
+        s_last = done == (unsigned)S.n_blocks - 1;
+    }
+    lds_barrier();
+    if (s_last) {
+        __threadfence();
+        double a = 0.0;
+        for (int b = t; b < S.n_blocks; b += kTileThreads)
+            a += __hip_atomic_load(&S.partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_red[t] = a;
+        lds_barrier();
+        for (int w = kTileThreads / 2; w > 0; w >>= 1) {
+            if (t < w) s_red[t] += s_red[t + w];
+            lds_barrier();
+        }
+        if (t == 0) {
+            T.total[0] = s_red[0];
+            __hip_atomic_store(S.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 // Records of the tile pass from the level lists and the current P(t): one
 // thread per post entry (g < n_post) or pre entry.
 // post_slot: slot of every internal row; pre_slot: slot of every node's
@@ -426,6 +704,18 @@ size_t tile_lds_bytes(const TreeDev &T, int tp, int n_slots, int n_rounds, size_
     return rows + lut + red + rounds + tips + stage_words * sizeof(int32_t);
 }
 
+// LDS bytes of k_tile_joint (layout in the kernel).
+size_t tile_joint_lds_bytes(const TreeDev &T, int tp, int n_slots, int n_rounds, int n_post, int n_pre) {
+    const size_t rows = align4((size_t)n_slots * tp * T.n) * sizeof(float);
+    const size_t red = (size_t)kTileThreads * sizeof(double);
+    const size_t rounds = align4((size_t)n_rounds) * sizeof(int);
+    const size_t desc = (size_t)n_post * 32 + (size_t)n_pre * 16;
+    const size_t tips = ((size_t)T.n_tips * tp + 15) & ~(size_t)15;
+    const size_t cx = ((size_t)T.n_nodes * tp * cx_row_bytes(T.n) + 15) & ~(size_t)15;
+    const size_t st = (size_t)T.n_nodes * tp;
+    return rows + red + rounds + desc + tips + cx + st;
+}
+
 #define TT_TILE_DISPATCH(n, CALL)                        \
     switch (n) {                                         \
         case 2: { constexpr int N_ = 2; CALL; } break;   \
@@ -453,6 +743,28 @@ int tile_blocks_per_cu(int n, size_t lds, bool staged, bool subg) {
         return 0;
     }
     return b;
+}
+
+// Workgroups of k_tile_joint one CU holds at once.
+int tile_joint_blocks_per_cu(int n, size_t lds) {
+    const void *fn = n == 2 ? (const void *)k_tile_joint<2> : n == 3 ? (const void *)k_tile_joint<3>
+                   : n == 4 ? (const void *)k_tile_joint<4> : n == 5 ? (const void *)k_tile_joint<5> : nullptr;
+    if (!fn) return 0;
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, kTileThreads, lds) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return b;
+}
+
+hipError_t tile_joint(const TreeDev &T, const TileArgs &A, const int32_t *pre_desc, int tips, hipStream_t s) {
+    TileDev S{A.post_rec, A.pre_rec, A.rounds, A.Rp, A.Ri, A.Ra, A.tp, (T.P + A.tp - 1) / A.tp, A.n_post, A.n_pre,
+              A.n_slots, A.root_post, A.root_pre, A.subg ? A.post_slot : nullptr, A.partial, A.counter};
+    const size_t lds = tile_joint_lds_bytes(T, A.tp, A.n_slots, 2 * (A.Rp + A.Ri + A.Ra), A.n_post, A.n_pre);
+    TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL(k_tile_joint<N_>, dim3((unsigned)S.n_blocks), dim3(kTileThreads), lds,
+                                             s, T, S, pre_desc, tips));
+    return hipGetLastError();
 }
 
 hipError_t tile_pack(const TreeDev &T, const int32_t *post_desc, int n_post, const int32_t *pre_desc, int n_pre,

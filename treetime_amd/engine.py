@@ -192,8 +192,14 @@ class DeviceTree:
         whether its records are staged in LDS, whether its LDS rows are
         reused by lifetime)."""
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
-        _native.check(self.lib.tt_tree_pass_shape(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        _native.check(self.lib.tt_tree_pass_shape(self.handle, C.byref(a), C.byref(b), C.byref(c), None))
         return a.value, bool(b.value), bool(c.value)
+
+    def joint_one_launch(self):
+        """Whether the joint pass runs in one launch over the marginal pass's tiles."""
+        j = C.c_int32()
+        _native.check(self.lib.tt_tree_pass_shape(self.handle, None, None, None, C.byref(j)))
+        return bool(j.value)
 
     def marginal(self, reconstruct_tips=False, stream=None, want_logL=True, keep_normalisers=False):
         out = C.c_double(np.nan)
