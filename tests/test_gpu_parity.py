@@ -572,18 +572,21 @@ def test_one_launch_pass_equals_level_kernels(gpu_engine, case, stage, reuse, mo
 
 @pytest.mark.parametrize("case", ONE_LAUNCH_CASES, ids=[c[0] for c in ONE_LAUNCH_CASES])
 @pytest.mark.parametrize("reuse", ["0", "1"], ids=["rows", "reused"])
-def test_one_launch_joint_equals_level_kernels(gpu_engine, case, reuse, monkeypatch):
+@pytest.mark.parametrize("jstage", ["1", "0"], ids=["staged", "streamed"])
+def test_one_launch_joint_equals_level_kernels(gpu_engine, case, reuse, jstage, monkeypatch):
     """The one-launch joint pass (postorder max-product rows in LDS, root,
-    backtrack and score in the same launch) gives the level kernels' states,
+    backtrack and score in the same launch; descriptors staged in LDS or
+    streamed from memory) gives the level kernels' states,
     message rows and argmax rows bit for bit and their joint logL to
     rounding, with and without reconstructed tips, after new branch lengths,
     and a backtrack from given root states after it equals theirs."""
-    say(f"[one-launch joint] case={case[0]} reuse={reuse}")
+    say(f"[one-launch joint] case={case[0]} reuse={reuse} descriptors staged={jstage}")
     job, prep, alphabet = make_case(*case)
     monkeypatch.setenv("TT_TILE", "0")
     lv = device_tree(gpu_engine, prep)
     monkeypatch.setenv("TT_TILE", "1")
     monkeypatch.setenv("TT_TILE_REUSE", reuse)
+    monkeypatch.setenv("TT_TILE_JOINT_STAGE", jstage)
     one = device_tree(gpu_engine, prep)
     assert not lv.joint_one_launch() and one.joint_one_launch()
     parent = np.asarray(lv.parent)

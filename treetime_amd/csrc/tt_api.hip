@@ -208,6 +208,7 @@ struct tt_tree {
     bool tile_staged = false;  // the kernel stages each phase's records in LDS
     bool tile_subg = false;    // LDS slots are reused; the preorder re-reads subtree rows from memory
     bool tile_joint = false;   // the joint pass runs in one launch over the same tiles (k_tile_joint)
+    bool tile_joint_staged = false;  // ... with both phases' descriptors staged in LDS
     int tile_slots = 0, tile_root_post = 0, tile_root_pre = 0;
     int32_t *d_tile_post_slot = nullptr, *d_tile_pre_slot = nullptr;  // slot of every internal row / node
     bool have_states = false;  // d.states holds a marginal or joint reconstruction
@@ -596,7 +597,7 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
                         T->tile_Ra, T->tile_tp, (int)T->post_nodes.size(), (int)T->pre_nodes.size(),
                         T->tile_slots, T->tile_root_post, T->tile_root_pre, T->d_tile_post_slot, T->tile_staged,
                         T->tile_subg, T->d_tile_partial, T->d_tile_counter};
-        TT_HIP(ttk::tile_joint(D, A, T->d_pre_desc, tips ? 1 : 0, s));
+        TT_HIP(ttk::tile_joint(D, A, T->d_pre_desc, tips ? 1 : 0, T->tile_joint_staged, s));
     } else {
         if (!T->site) TT_HIP(ttk::joint_tiptab(D, T->d_tip_nodes, s));
         const int H = (int)T->post_lv.size() - 1;  // the root level runs on its own
@@ -1100,12 +1101,21 @@ int32_t setup_tile(tt_tree *T) {
     // the joint pass takes the same tiles in one launch when its LDS (message
     // rows, both phases' descriptors, argmax rows and states) fits
     // (TT_TILE_JOINT=0: level kernels)
+    // (descriptors staged in LDS when they fit too, else streamed)
     T->tile_joint = false;
     const char *ej = getenv("TT_TILE_JOINT");
+    const char *js = getenv("TT_TILE_JOINT_STAGE");  // 1 / 0: only the staged / streamed descriptors
     if (!(ej && ej[0] == '0')) {
-        const size_t jl = ttk::tile_joint_lds_bytes(T->d, tp, T->tile_slots, 2 * (Rp + Ri + Ra),
-                                                    (int)T->post_nodes.size(), (int)T->pre_nodes.size());
-        T->tile_joint = jl <= budget && ttk::tile_joint_blocks_per_cu(T->n, jl) >= 1;
+        for (bool st : {true, false}) {
+            if (js && js[0] && (js[0] == '1') != st) continue;
+            const size_t jl = ttk::tile_joint_lds_bytes(T->d, tp, T->tile_slots, 2 * (Rp + Ri + Ra),
+                                                        (int)T->post_nodes.size(), (int)T->pre_nodes.size(), st);
+            if (jl <= budget && ttk::tile_joint_blocks_per_cu(T->n, jl, st) >= 1) {
+                T->tile_joint = true;
+                T->tile_joint_staged = st;
+                break;
+            }
+        }
     }
     return TT_OK;
 }

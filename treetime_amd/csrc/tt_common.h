@@ -180,8 +180,8 @@ size_t tile_stage_words(int n, int n_post, int n_pre);
 size_t tile_lds_bytes(const TreeDev &T, int tp, int n_slots, int n_rounds, size_t stage_words);
 int tile_blocks_per_cu(int n, size_t lds, bool staged, bool subg);
 // one-launch joint pass over the same tiles (k_tile_joint)
-size_t tile_joint_lds_bytes(const TreeDev &T, int tp, int n_slots, int n_rounds, int n_post, int n_pre);
-int tile_joint_blocks_per_cu(int n, size_t lds);
+size_t tile_joint_lds_bytes(const TreeDev &T, int tp, int n_slots, int n_rounds, int n_post, int n_pre, bool staged);
+int tile_joint_blocks_per_cu(int n, size_t lds, bool staged);
 hipError_t tile_pack(const TreeDev &T, const int32_t *post_desc, int n_post, const int32_t *pre_desc, int n_pre,
                      const int32_t *post_slot, const int32_t *pre_slot, bool subg, int32_t *post_rec,
                      int32_t *pre_rec, hipStream_t s);
@@ -196,7 +196,8 @@ struct TileArgs {
     unsigned *counter;
 };
 hipError_t tile_marginal(const TreeDev &T, const TileArgs &A, int tips, int pre, hipStream_t s);
-hipError_t tile_joint(const TreeDev &T, const TileArgs &A, const int32_t *pre_desc, int tips, hipStream_t s);
+hipError_t tile_joint(const TreeDev &T, const TileArgs &A, const int32_t *pre_desc, int tips, bool staged,
+                      hipStream_t s);
 // sets tt_last_error()'s text on the calling thread and returns code (tt_api.hip)
 int32_t set_error(int32_t code, const char *msg);
 // joint branch-length optimisation (tt_branch.hip)

@@ -371,8 +371,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
 // results); the root step picks the root states, the backtrack rounds over
 // the depth-ordered nodes give every node its state and add its FP64 score
 // term, and the workgroups' sums of mult * logL make the total as in the
-// marginal pass.  The descriptors of both phases are staged in LDS.
-template <int N>
+// marginal pass.  STAGED: the descriptors of both phases are staged in LDS;
+// otherwise each thread loads its next round's descriptor while the current
+// round runs (two register sets in turn, as the marginal pass streams its
+// records).
+template <int N, bool STAGED>
 __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev S, const int32_t *pre_desc,
                                                              int tips) {
     using R = JR<N>;
@@ -386,9 +389,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
     double *s_red = reinterpret_cast<double *>(rows + align4((size_t)S.n_slots * TP * N));  // [kTileThreads]
     int *s_rounds = reinterpret_cast<int *>(s_red + kTileThreads);
     const int n_rounds = 2 * (S.Rp + S.Ri + S.Ra);
-    int4 *s_post = reinterpret_cast<int4 *>(s_rounds + align4(n_rounds));  // [n_post][2]
-    int4 *s_pre = s_post + 2 * (size_t)S.n_post;                          // [n_pre]
-    uint8_t *s_tip = reinterpret_cast<uint8_t *>(s_pre + S.n_pre);        // [n_tips][TP]
+    int4 *s_post = reinterpret_cast<int4 *>(s_rounds + align4(n_rounds));  // [n_post][2] (STAGED)
+    int4 *s_pre = s_post + (STAGED ? 2 * (size_t)S.n_post : 0);           // [n_pre] (STAGED)
+    uint8_t *s_tip = reinterpret_cast<uint8_t *>(s_pre + (STAGED ? S.n_pre : 0));  // [n_tips][TP]
     uint8_t *s_cx = s_tip + (((size_t)T.n_tips * TP + 15) & ~(size_t)15);             // [n_nodes][TP][CB]
     uint8_t *s_st = s_cx + (((size_t)T.n_nodes * TP * CB + 15) & ~(size_t)15);       // [n_nodes][TP]
     __shared__ double s_sum[TT_WAVE];
@@ -400,10 +403,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
     const int p = blockIdx.x * TP + q;
     const bool act = p < T.P;
 
+    constexpr int W = post_rec_words(N);
     for (int i = t; i < n_rounds; i += kTileThreads) s_rounds[i] = S.rounds[i];
-    for (int i = t; i < 2 * S.n_post; i += kTileThreads)
-        s_post[i] = reinterpret_cast<const int4 *>(S.post_rec + (size_t)post_rec_words(N) * (i >> 1))[i & 1];
-    for (int i = t; i < S.n_pre; i += kTileThreads) s_pre[i] = reinterpret_cast<const int4 *>(pre_desc)[i];
+    if constexpr (STAGED) {
+        for (int i = t; i < 2 * S.n_post; i += kTileThreads)
+            s_post[i] = reinterpret_cast<const int4 *>(S.post_rec + (size_t)W * (i >> 1))[i & 1];
+        for (int i = t; i < S.n_pre; i += kTileThreads) s_pre[i] = reinterpret_cast<const int4 *>(pre_desc)[i];
+    }
     for (int i = t; i < T.n_tips * TP; i += kTileThreads) {
         const int r = i / TP, pp = blockIdx.x * TP + (i & (TP - 1));
         s_tip[i] = pp < T.P ? T.tip_codes[(size_t)r * T.P + pp] : 0;
@@ -499,61 +505,97 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
     };
 
     // ---- postorder: max-product rounds (the root is the root step's) ---------
+    auto post_step = [&](const int4 &a, const int4 &b) {
+        if (a.x != T.root_row) {
+            R msg[N];
+            children(a, b, msg);
+            // joint_store: Lx_j = max_i max(TINY, P_ij) msg_i over the row's own branch
+            const auto P1 = PJ + (size_t)b.w * (N * N);
+            R best[N];
+            int cx[N];
+            R off = R(0);
+            int jm = 0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                R bb = P1[j] * msg[0];
+                int bi = 0;
+#pragma unroll
+                for (int i = 1; i < N; ++i) {
+                    const R val = P1[i * N + j] * msg[i];
+                    if (val > bb) {
+                        bb = val;
+                        bi = i;
+                    }
+                }
+                best[j] = bb;
+                if (bb > off) {
+                    off = bb;
+                    jm = j;
+                }
+                cx[j] = bi;
+            }
+            const R inv = R(1) / off;
+            float out[N];
+#pragma unroll
+            for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : (float)(best[j] * inv);
+            float *dst = slot(a.y);
+#pragma unroll
+            for (int j = 0; j < N; ++j) dst[j] = out[j];
+            store_row<N>(T.Lx + ((size_t)a.x * T.P + p) * N, out);
+            cx_store<N>(T, b.w, p, cx);
+            put_cx(b.w, cx);
+        }
+    };
+    auto post_desc = [&](int k, int4 &a, int4 &b) {
+        if constexpr (STAGED) {
+            a = s_post[2 * k];
+            b = s_post[2 * k + 1];
+        } else {
+            const int4 *g = reinterpret_cast<const int4 *>(S.post_rec + (size_t)W * k);
+            a = g[0];
+            b = g[1];
+        }
+    };
     {
         const int *rd = s_rounds;
-        for (int r = 0; r < S.Rp; ++r) {
-            const int k = rd[2 * r] + k0;
-            if (k < rd[2 * r + 1] && act) {
-                const int4 a = s_post[2 * k], b = s_post[2 * k + 1];
-                if (a.x != T.root_row) {
-                    R msg[N];
-                    children(a, b, msg);
-                    // joint_store: Lx_j = max_i max(TINY, P_ij) msg_i over the row's own branch
-                    const auto P1 = PJ + (size_t)b.w * (N * N);
-                    R best[N];
-                    int cx[N];
-                    R off = R(0);
-                    int jm = 0;
-#pragma unroll
-                    for (int j = 0; j < N; ++j) {
-                        R bb = P1[j] * msg[0];
-                        int bi = 0;
-#pragma unroll
-                        for (int i = 1; i < N; ++i) {
-                            const R val = P1[i * N + j] * msg[i];
-                            if (val > bb) {
-                                bb = val;
-                                bi = i;
-                            }
-                        }
-                        best[j] = bb;
-                        if (bb > off) {
-                            off = bb;
-                            jm = j;
-                        }
-                        cx[j] = bi;
-                    }
-                    const R inv = R(1) / off;
-                    float out[N];
-#pragma unroll
-                    for (int j = 0; j < N; ++j) out[j] = (j == jm) ? 1.0f : (float)(best[j] * inv);
-                    float *dst = slot(a.y);
-#pragma unroll
-                    for (int j = 0; j < N; ++j) dst[j] = out[j];
-                    store_row<N>(T.Lx + ((size_t)a.x * T.P + p) * N, out);
-                    cx_store<N>(T, b.w, p, cx);
-                    put_cx(b.w, cx);
+        if constexpr (STAGED) {
+            for (int r = 0; r < S.Rp; ++r) {
+                const int k = rd[2 * r] + k0;
+                if (k < rd[2 * r + 1] && act) {
+                    int4 a, b;
+                    post_desc(k, a, b);
+                    post_step(a, b);
                 }
+                lds_barrier();
             }
-            lds_barrier();
+        } else {
+            // two descriptor sets in turn, loaded unconditionally (clamped index)
+            auto entry = [&](int r, bool &valid) {
+                const int k = r < S.Rp ? rd[2 * r] + k0 : 0;
+                valid = r < S.Rp && k < rd[2 * r + 1];
+                return min(k, S.n_post - 1);
+            };
+            int4 a0, b0, a1, b1;
+            bool va = false, vb = false;
+            if (S.Rp > 0) post_desc(entry(0, va), a0, b0);
+            for (int r = 0; r < S.Rp; r += 2) {
+                post_desc(entry(r + 1, vb), a1, b1);
+                if (va && act) post_step(a0, b0);
+                lds_barrier();
+                if (r + 1 >= S.Rp) break;
+                post_desc(entry(r + 2, va), a0, b0);
+                if (vb && act) post_step(a1, b1);
+                lds_barrier();
+            }
         }
     }
 
     // ---- root (treeanc.py:1015-1037): root_joint_state ----------------------
     if (t < TP && act) {
-        const int k = S.n_post - 1;  // height order: the root comes last
+        int4 a, b;
+        post_desc(S.n_post - 1, a, b);  // height order: the root comes last
         R msg[N];
-        children(s_post[2 * k], s_post[2 * k + 1], msg);
+        children(a, b, msg);
         int jm = 0;
         R mx = R(-1);
 #pragma unroll
@@ -576,24 +618,45 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
 
     // ---- backtrack over every node (treeanc.py:1048-1063) and the score -----
     double term = 0.0;
+    // d: node, parent, internal row or -1, tip row or -1
+    auto back_step = [&](const int4 &d) {
+        const int c = d.x, sp = s_st[(size_t)d.y * TP + q];
+        if (d.z >= 0) {
+            const int sc = get_cx(c, sp);
+            s_st[(size_t)c * TP + q] = (uint8_t)sc;
+            T.states[(size_t)c * T.P + p] = (uint8_t)sc;
+            term += joint_int_term<N>(T, c, sc, sp);
+        } else {
+            if (tips) T.states[(size_t)c * T.P + p] = (uint8_t)get_cx(c, sp);
+            term += joint_tip_term<N>(T, T.loglut, c, s_tip[d.w * TP + q], sp);
+        }
+    };
     {
         const int *rd = s_rounds + 2 * (S.Rp + S.Ri);
-        for (int r = 0; r < S.Ra; ++r) {
-            const int k = rd[2 * r] + k0;
-            if (k < rd[2 * r + 1] && act) {
-                const int4 d = s_pre[k];  // node, parent, internal row or -1, tip row or -1
-                const int c = d.x, sp = s_st[(size_t)d.y * TP + q];
-                if (d.z >= 0) {
-                    const int sc = get_cx(c, sp);
-                    s_st[(size_t)c * TP + q] = (uint8_t)sc;
-                    T.states[(size_t)c * T.P + p] = (uint8_t)sc;
-                    term += joint_int_term<N>(T, c, sc, sp);
-                } else {
-                    if (tips) T.states[(size_t)c * T.P + p] = (uint8_t)get_cx(c, sp);
-                    term += joint_tip_term<N>(T, T.loglut, c, s_tip[d.w * TP + q], sp);
-                }
+        if constexpr (STAGED) {
+            for (int r = 0; r < S.Ra; ++r) {
+                const int k = rd[2 * r] + k0;
+                if (k < rd[2 * r + 1] && act) back_step(s_pre[k]);
+                lds_barrier();
             }
-            lds_barrier();
+        } else {
+            const int4 *g = reinterpret_cast<const int4 *>(pre_desc);
+            auto entry = [&](int r, bool &valid) {
+                const int k = r < S.Ra ? rd[2 * r] + k0 : 0;
+                valid = r < S.Ra && k < rd[2 * r + 1];
+                return min(k, S.n_pre - 1);
+            };
+            bool va = false, vb = false;
+            int4 d0 = S.Ra > 0 ? g[entry(0, va)] : int4{0, 0, 0, 0}, d1;
+            for (int r = 0; r < S.Ra; r += 2) {
+                d1 = g[entry(r + 1, vb)];
+                if (va && act) back_step(d0);
+                lds_barrier();
+                if (r + 1 >= S.Ra) break;
+                d0 = g[entry(r + 2, va)];
+                if (vb && act) back_step(d1);
+                lds_barrier();
+            }
         }
     }
     s_red[t] = act ? term : 0.0;
@@ -705,11 +768,11 @@ size_t tile_lds_bytes(const TreeDev &T, int tp, int n_slots, int n_rounds, size_
 }
 
 // LDS bytes of k_tile_joint (layout in the kernel).
-size_t tile_joint_lds_bytes(const TreeDev &T, int tp, int n_slots, int n_rounds, int n_post, int n_pre) {
+size_t tile_joint_lds_bytes(const TreeDev &T, int tp, int n_slots, int n_rounds, int n_post, int n_pre, bool staged) {
     const size_t rows = align4((size_t)n_slots * tp * T.n) * sizeof(float);
     const size_t red = (size_t)kTileThreads * sizeof(double);
     const size_t rounds = align4((size_t)n_rounds) * sizeof(int);
-    const size_t desc = (size_t)n_post * 32 + (size_t)n_pre * 16;
+    const size_t desc = staged ? (size_t)n_post * 32 + (size_t)n_pre * 16 : 0;
     const size_t tips = ((size_t)T.n_tips * tp + 15) & ~(size_t)15;
     const size_t cx = ((size_t)T.n_nodes * tp * cx_row_bytes(T.n) + 15) & ~(size_t)15;
     const size_t st = (size_t)T.n_nodes * tp;
@@ -745,10 +808,15 @@ int tile_blocks_per_cu(int n, size_t lds, bool staged, bool subg) {
     return b;
 }
 
+template <int N>
+const void *tile_joint_kernel(bool staged) {
+    return staged ? (const void *)k_tile_joint<N, true> : (const void *)k_tile_joint<N, false>;
+}
+
 // Workgroups of k_tile_joint one CU holds at once.
-int tile_joint_blocks_per_cu(int n, size_t lds) {
-    const void *fn = n == 2 ? (const void *)k_tile_joint<2> : n == 3 ? (const void *)k_tile_joint<3>
-                   : n == 4 ? (const void *)k_tile_joint<4> : n == 5 ? (const void *)k_tile_joint<5> : nullptr;
+int tile_joint_blocks_per_cu(int n, size_t lds, bool staged) {
+    const void *fn = n == 2 ? tile_joint_kernel<2>(staged) : n == 3 ? tile_joint_kernel<3>(staged)
+                   : n == 4 ? tile_joint_kernel<4>(staged) : n == 5 ? tile_joint_kernel<5>(staged) : nullptr;
     if (!fn) return 0;
     int b = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, kTileThreads, lds) != hipSuccess) {
@@ -758,12 +826,17 @@ int tile_joint_blocks_per_cu(int n, size_t lds) {
     return b;
 }
 
-hipError_t tile_joint(const TreeDev &T, const TileArgs &A, const int32_t *pre_desc, int tips, hipStream_t s) {
+hipError_t tile_joint(const TreeDev &T, const TileArgs &A, const int32_t *pre_desc, int tips, bool staged,
+                      hipStream_t s) {
     TileDev S{A.post_rec, A.pre_rec, A.rounds, A.Rp, A.Ri, A.Ra, A.tp, (T.P + A.tp - 1) / A.tp, A.n_post, A.n_pre,
               A.n_slots, A.root_post, A.root_pre, A.subg ? A.post_slot : nullptr, A.partial, A.counter};
-    const size_t lds = tile_joint_lds_bytes(T, A.tp, A.n_slots, 2 * (A.Rp + A.Ri + A.Ra), A.n_post, A.n_pre);
-    TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL(k_tile_joint<N_>, dim3((unsigned)S.n_blocks), dim3(kTileThreads), lds,
-                                             s, T, S, pre_desc, tips));
+    const size_t lds = tile_joint_lds_bytes(T, A.tp, A.n_slots, 2 * (A.Rp + A.Ri + A.Ra), A.n_post, A.n_pre, staged);
+    const dim3 grid((unsigned)S.n_blocks), block(kTileThreads);
+    if (staged) {
+        TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL((k_tile_joint<N_, true>), grid, block, lds, s, T, S, pre_desc, tips));
+    } else {
+        TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL((k_tile_joint<N_, false>), grid, block, lds, s, T, S, pre_desc, tips));
+    }
     return hipGetLastError();
 }
 
