@@ -1014,7 +1014,8 @@ int32_t setup_tile(tt_tree *T) {
     // TT_TILE=1 also allows two waves of workgroups; TT_TILE=0 turns the path
     // off; TT_TILE_STAGE=0 never stages the records; TT_TILE_TP fixes the
     // tile; TT_TILE_REUSE=0 / 1 never / always reuses slots.
-    const int kTileMaxWaves = forced ? 2 : 1;
+    const char *we = getenv("TT_TILE_WAVES");  // waves of workgroups allowed (default 1, 2 with TT_TILE=1)
+    const int kTileMaxWaves = we && atoi(we) > 0 ? atoi(we) : (forced ? 2 : 1);
     const char *se = getenv("TT_TILE_STAGE");
     const bool may_stage = !(se && se[0] == '0');
     const char *te = getenv("TT_TILE_TP");
