@@ -46,8 +46,8 @@ struct TreeDev {
     const double TT_C *P64;          // [n_nodes][n][n]  expQt per branch, FP64
     const double TT_C *Pj64;         // [n_nodes][n][n]  max(TINY, expQt): joint messages
     const double TT_C *logP64;       // [n_nodes][n][n]  log(max(TINY, expQt)): joint path scores
-    const float TT_C *Pj32;          // [n_nodes][n][n]  max(TINY, expQt) in FP32: joint messages, n > 8
-    const float TT_C *lutj32;        // [n_codes][n]   max(profile, TINY) in FP32: joint messages, n > 8
+    const float TT_C *Pj32;          // [n_nodes][n][n]  max(TINY, expQt) in FP32: joint messages, n > 3
+    const float TT_C *lutj32;        // [n_codes][n]   max(profile, TINY) in FP32: joint messages, n > 3
     const float TT_C *P32;           // [n_nodes][n][n]  expQt, FP32
     const float TT_C *D32;           // [n_nodes][n][n]  expQt - I (computed in FP64), FP32
     float TT_G *sub;                 // [n_int][P][n]   subtree partials, max entry = 1

@@ -570,27 +570,30 @@ __device__ __forceinline__ void pre_pair(const TreeDev &T, int c0, int c1, bool 
 // Max and product commute with the log, so the same decisions are taken in
 // product space: messages are stored in FP32, divided by their largest entry
 // (which is therefore exactly 1), and every product and comparison is formed
-// from the tables max(TINY, P) and max(profile, TINY) (FP64, or FP32 for
-// n > 8, see JR).  A stored
-// entry is >= TINY (a row whose largest entry is 1, times max(TINY, P)), so
-// the product of two children stays far inside the FP64 range; from a
+// from the tables max(TINY, P) and max(profile, TINY) (FP32 from 4 states
+// on, FP64 below, see JR).  A stored entry is >= TINY (a row whose largest
+// entry is 1, times max(TINY, P)), so the product of two children's entries
+// times max(TINY, P) is >= 1e-36, inside the normal FP32 range; from a
 // polytomy's third child on, the running product is rescaled by a power of
-// two after each child.  The messages only decide the states: the joint
+// two after each child (an entry that still underflows is more than 1e-24
+// below the entry at the message's maximum and never decides an argmax).  The messages only decide the states: the joint
 // log-likelihood is the FP64 log-space score of the decoded states
 // (joint_tip_term / joint_int_term), summed after the backtrack, so no FP32
 // rounding reaches it.
 
-// Arithmetic of the joint messages: FP64 for small alphabets; FP32 for
-// large ones (n > 8), where the n^2 products and comparisons per child make
-// the pass VALU-bound.  The messages only decide the argmax states (the
-// log-likelihood is the FP64 score of those states), and FP32 keeps every
-// decision whose FP64 margin exceeds ~1e-6.
+// Arithmetic of the joint messages: FP32 from 4 states on, where the n^2
+// products and comparisons per child are most of the pass's instructions
+// (FP64 runs at half the FP32 rate, and its compares and selects take two
+// registers per value); FP64 for binary and ternary alphabets.  The messages
+// only decide the argmax states (the log-likelihood is the FP64 score of
+// those states), and FP32 keeps every decision whose FP64 margin exceeds
+// ~1e-6.
 template <int N>
-using JR = std::conditional_t<(N > 8), float, double>;
+using JR = std::conditional_t<(N > 3), float, double>;
 
 template <int N>
 __device__ __forceinline__ const JR<N> TT_C *joint_P(const TreeDev &T) {
-    if constexpr (N > 8)
+    if constexpr (N > 3)
         return T.Pj32;
     else
         return T.Pj64;
@@ -598,7 +601,7 @@ __device__ __forceinline__ const JR<N> TT_C *joint_P(const TreeDev &T) {
 
 template <int N>
 __device__ __forceinline__ const JR<N> TT_C *joint_lut(const TreeDev &T) {
-    if constexpr (N > 8)
+    if constexpr (N > 3)
         return T.lutj32;
     else
         return T.lutj;
