@@ -70,6 +70,73 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Sum of v over the 64 lanes of a wave, in lane 0 (a fixed shuffle tree).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = TT_WAVE / 2; off > 0; off >>= 1) v += __shfl_down(v, off);
+    return v;
+}
+
+// Sum of v over the threads of one pattern (threads q, q + TP, q + 2 TP, ...,
+// q = t & (TP - 1)), returned to threads t < TP: the lanes of a wave first
+// (shuffles), then the waves' partials from LDS (s_red, one barrier), in a
+// fixed order.  Every thread of the workgroup calls it.
+__device__ __forceinline__ double pattern_sum(double v, double *s_red, int TP) {
+    const int t = threadIdx.x;
+    for (int off = TT_WAVE / 2; off >= TP; off >>= 1) v += __shfl_down(v, off);
+    s_red[t] = v;
+    lds_barrier();
+    double s = 0.0;
+    if (t < TP) {
+        const int step = max(TP, TT_WAVE);  // between the partials of one pattern
+#pragma unroll
+        for (int j = 0; j < kTileThreads / TT_WAVE; ++j) {
+            const int k = t + j * step;
+            const double x = s_red[min(k, kTileThreads - 1)];
+            s += k < kTileThreads ? x : 0.0;
+        }
+    }
+    return s;
+}
+
+// The workgroup's sum of s_red[k] over its patterns k < TP that exist (mult
+// * logL per pattern) goes to partial[blockIdx.x]; the last workgroup to
+// finish sums the partials (every thread a strided share, then wave
+// shuffles and the waves' sums in a fixed order) into total[0] and resets
+// the counter.  Every thread of the workgroup calls it.
+__device__ __forceinline__ void tile_total(const TreeDev &T, const TileDev &S, double *s_red, int TP) {
+    __shared__ bool s_last;
+    const int t = threadIdx.x;
+    if (t < TT_WAVE) {
+        double a = 0.0;
+        for (int k = t; k < TP; k += TT_WAVE) a += (blockIdx.x * TP + k < T.P) ? s_red[k] : 0.0;
+        a = wave_sum(a);
+        if (t == 0) {
+            __hip_atomic_store(&S.partial[blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            const unsigned done = atomicAdd(S.counter, 1u);
+            s_last = done == (unsigned)S.n_blocks - 1;
+        }
+    }
+    lds_barrier();
+    if (s_last) {
+        __threadfence();
+        double a = 0.0;
+        for (int b = t; b < S.n_blocks; b += kTileThreads)
+            a += __hip_atomic_load(&S.partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a = wave_sum(a);
+        if ((t & (TT_WAVE - 1)) == 0) s_red[t / TT_WAVE] = a;
+        lds_barrier();
+        if (t == 0) {
+            double tot = 0.0;
+#pragma unroll
+            for (int w = 0; w < kTileThreads / TT_WAVE; ++w) tot += s_red[w];
+            T.total[0] = tot;
+            __hip_atomic_store(S.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 template <int N>
 struct PostRec {
     int4 a, b;  // v, e0, nchild, cref0 | cref1, cnode0, cnode1, vn
@@ -105,11 +172,40 @@ __device__ __forceinline__ void load_rec(const int32_t *base, PreRec<N> &r) {
     for (int i = 0; i < N * N; ++i) r.M[i] = f[i];
 }
 
+// dst[i] = f(i) for i in [0, n), dst in LDS, f a read from memory: each
+// thread issues kStageLoads reads (clamped index) before it writes any, so
+// staging an array costs one memory latency per kStageLoads * kTileThreads
+// elements instead of one per kTileThreads.
+constexpr int kStageLoads = 8;
+template <class V, class F>
+__device__ __forceinline__ void stage_in(V *dst, int n, F f) {
+    const int t = threadIdx.x;
+    for (int b = 0; b < n; b += kStageLoads * kTileThreads) {
+        V v[kStageLoads];
+#pragma unroll
+        for (int u = 0; u < kStageLoads; ++u) v[u] = f(min(b + u * kTileThreads + t, n - 1));
+#pragma unroll
+        for (int u = 0; u < kStageLoads; ++u) {
+            const int i = b + u * kTileThreads + t;
+            if (i < n) dst[i] = v[u];
+        }
+    }
+}
+
 // Copy words [0, n_words) of a record array into LDS (16-byte pieces).
 __device__ __forceinline__ void stage_records(int32_t *dst, const int32_t *src, size_t n_words) {
     const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
-    uint4 *d4 = reinterpret_cast<uint4 *>(dst);
-    for (size_t i = threadIdx.x; i < n_words / 4; i += kTileThreads) d4[i] = s4[i];
+    stage_in(reinterpret_cast<uint4 *>(dst), (int)(n_words / 4), [&](int i) { return s4[i]; });
+}
+
+// The workgroup's tip characters: s_tip[r TP + j] = code of tip row r at
+// pattern p0 + j (0 past the last pattern).
+__device__ __forceinline__ void stage_tips(uint8_t *s_tip, const TreeDev &T, int TP, int p0) {
+    stage_in(s_tip, T.n_tips * TP, [&](int i) {
+        const int r = i / TP, pp = p0 + (i & (TP - 1));
+        const uint8_t c = T.tip_codes[(size_t)r * T.P + min(pp, T.P - 1)];
+        return pp < T.P ? c : (uint8_t)0;
+    });
 }
 
 // STAGED: each phase's records copied into LDS; SUBG: slots are reused once
@@ -129,8 +225,6 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
     const int n_rounds = 2 * (S.Rp + S.Ri + S.Ra);
     uint8_t *s_tip = reinterpret_cast<uint8_t *>(s_rounds + align4(n_rounds));  // [n_tips][TP]
     int32_t *s_rec = reinterpret_cast<int32_t *>(s_tip + (((size_t)T.n_tips * TP + 15) & ~(size_t)15));
-    __shared__ double s_sum[TT_WAVE];
-    __shared__ bool s_last;
 
     const int t = threadIdx.x;
     const int q = t & (TP - 1);  // this thread's pattern within the tile
@@ -139,12 +233,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
     const bool act = p < T.P;
 
     // stage the profile map, the round offsets and the tile's tip characters
-    for (int i = t; i < T.n_codes * N; i += kTileThreads) s_lut[i] = T.lut[i];
-    for (int i = t; i < n_rounds; i += kTileThreads) s_rounds[i] = S.rounds[i];
-    for (int i = t; i < T.n_tips * TP; i += kTileThreads) {
-        const int r = i / TP, pp = blockIdx.x * TP + (i & (TP - 1));
-        s_tip[i] = pp < T.P ? T.tip_codes[(size_t)r * T.P + pp] : 0;
-    }
+    stage_in(s_lut, T.n_codes * N, [&](int i) { return T.lut[i]; });
+    stage_in(s_rounds, n_rounds, [&](int i) { return S.rounds[i]; });
+    stage_tips(s_tip, T, TP, blockIdx.x * TP);
     if constexpr (STAGED) stage_records(s_rec, S.post_rec, (size_t)S.n_post * post_rec_words(N));
     lds_barrier();  // the staging loads have landed in LDS (each thread waited for its own)
 
@@ -190,38 +281,35 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
     {
         constexpr int W = post_rec_words(N);
         const int *rd = s_rounds;
-        if constexpr (STAGED) {
-            for (int r = 0; r < S.Rp; ++r) {
-                const int k = rd[2 * r] + k0;
-                if (k < rd[2 * r + 1] && act) {
-                    const int32_t *rec = s_rec + (size_t)W * k;
-                    const float *D = reinterpret_cast<const float *>(rec + 8);
-                    post_entry(reinterpret_cast<const int4 *>(rec)[0], reinterpret_cast<const int4 *>(rec)[1], D,
-                               D + N * N);
-                }
-                lds_barrier();
-            }
-        } else {
-            // two record buffers in turn, loaded unconditionally (clamped
-            // index) one round ahead, so a round waits only for the record it
-            // uses and never for the loads just issued for the next round
-            PostRec<N> ra, rb;
-            auto entry = [&](int r, bool &valid) {
-                const int k = r < S.Rp ? rd[2 * r] + k0 : 0;
-                valid = r < S.Rp && k < rd[2 * r + 1];
-                return (size_t)W * min(k, S.n_post - 1);
-            };
-            bool va, vb;
-            if (S.Rp > 0) load_rec<N>(S.post_rec + entry(0, va), ra);
-            for (int r = 0; r < S.Rp; r += 2) {
-                load_rec<N>(S.post_rec + entry(r + 1, vb), rb);
-                if (va && act) post_entry(ra.a, ra.b, ra.D0, ra.D1);
-                lds_barrier();
-                if (r + 1 >= S.Rp) break;
-                load_rec<N>(S.post_rec + entry(r + 2, va), ra);
-                if (vb && act) post_entry(rb.a, rb.b, rb.D0, rb.D1);
-                lds_barrier();
-            }
+        // two record buffers in turn, loaded unconditionally (clamped index)
+        // one round ahead -- from the staged copy in LDS or from memory -- so
+        // a round's dependent reads start from its record in registers and
+        // it waits only for the record it uses, never for the loads just
+        // issued for the next round
+        const int32_t *src;
+        if constexpr (STAGED)
+            src = s_rec;
+        else
+            src = S.post_rec;
+        PostRec<N> ra, rb;
+        // the staged copy is read only for the entries a thread runs (idle
+        // waves take no LDS bandwidth); memory is read unconditionally, so
+        // that the loads' counter never waits on a branch
+        auto fetch = [&](int r, bool &valid, PostRec<N> &rec) {
+            const int k = r < S.Rp ? rd[2 * r] + k0 : 0;
+            valid = r < S.Rp && k < rd[2 * r + 1] && act;
+            if (!STAGED || valid) load_rec<N>(src + (size_t)W * min(k, S.n_post - 1), rec);
+        };
+        bool va, vb;
+        if (S.Rp > 0) fetch(0, va, ra);
+        for (int r = 0; r < S.Rp; r += 2) {
+            fetch(r + 1, vb, rb);
+            if (va) post_entry(ra.a, ra.b, ra.D0, ra.D1);
+            lds_barrier();
+            if (r + 1 >= S.Rp) break;
+            fetch(r + 2, va, ra);
+            if (vb) post_entry(rb.a, rb.b, rb.D0, rb.D1);
+            lds_barrier();
         }
     }
     // the preorder's records replace the postorder's while the root step runs
@@ -230,11 +318,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
     }
 
     // ---- root (treeanc.py:814-829) ------------------------------------------
-    s_red[t] = act ? acc.log_value() : 0.0;
-    lds_barrier();
+    const double off_sum = pattern_sum(act ? acc.log_value() : 0.0, s_red, TP);
     if (t < TP && act) {
-        double off = 0.0;
-        for (int k = q; k < kTileThreads; k += TP) off += s_red[k];
+        const double off = off_sum;
         const float *rr = slot(S.root_post);
         double x[N], s = 0.0;
 #pragma unroll
@@ -297,69 +383,32 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
             for (int i = 0; i < N; ++i) Rw.pp[i] = pp[i];
             pre_finish_m<N>(T, c, ir, p, Rw, s_lut, M, oslot >= 0 ? slot(oslot) : nullptr);
         };
-        if constexpr (STAGED) {
-            for (int r = 0; r < R; ++r) {
-                const int k = rd[2 * r] + k0;
-                if (k < rd[2 * r + 1] && act) {
-                    const int32_t *rec = s_rec + (size_t)W * k;
-                    pre_entry(reinterpret_cast<const int4 *>(rec)[0], reinterpret_cast<const int4 *>(rec)[1],
-                              reinterpret_cast<const float *>(rec + 8));
-                }
-                lds_barrier();
-            }
-        } else {
-            PreRec<N> ra, rb;
-            auto entry = [&](int r, bool &valid) {
-                const int k = r < R ? rd[2 * r] + k0 : 0;
-                valid = r < R && k < rd[2 * r + 1];
-                return (size_t)W * min(k, S.n_pre - 1);
-            };
-            bool va, vb;
-            if (R > 0) load_rec<N>(S.pre_rec + entry(0, va), ra);
-            for (int r = 0; r < R; r += 2) {
-                load_rec<N>(S.pre_rec + entry(r + 1, vb), rb);
-                if (va && act) pre_entry(ra.a, ra.b, ra.M);
-                lds_barrier();
-                if (r + 1 >= R) break;
-                load_rec<N>(S.pre_rec + entry(r + 2, va), ra);
-                if (vb && act) pre_entry(rb.a, rb.b, rb.M);
-                lds_barrier();
-            }
+        const int32_t *src;  // as in the postorder: records one round ahead
+        if constexpr (STAGED)
+            src = s_rec;
+        else
+            src = S.pre_rec;
+        PreRec<N> ra, rb;
+        auto fetch = [&](int r, bool &valid, PreRec<N> &rec) {
+            const int k = r < R ? rd[2 * r] + k0 : 0;
+            valid = r < R && k < rd[2 * r + 1] && act;
+            if (!STAGED || valid) load_rec<N>(src + (size_t)W * min(k, S.n_pre - 1), rec);
+        };
+        bool va, vb;
+        if (R > 0) fetch(0, va, ra);
+        for (int r = 0; r < R; r += 2) {
+            fetch(r + 1, vb, rb);
+            if (va) pre_entry(ra.a, ra.b, ra.M);
+            lds_barrier();
+            if (r + 1 >= R) break;
+            fetch(r + 2, va, ra);
+            if (vb) pre_entry(rb.a, rb.b, rb.M);
+            lds_barrier();
         }
     }
 
     // ---- workgroup sum of mult * logL, then the total over workgroups -------
-    if (t < TT_WAVE) {
-        double a = 0.0;
-        for (int k = t; k < TP; k += TT_WAVE) a += (blockIdx.x * TP + k < T.P) ? s_red[k] : 0.0;
-        s_sum[t] = a;
-    }
-    lds_barrier();
-    if (t == 0) {
-        double a = 0.0;
-        for (int k = 0; k < TT_WAVE; ++k) a += s_sum[k];
-        __hip_atomic_store(&S.partial[blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned done = atomicAdd(S.counter, 1u);
-        s_last = done == (unsigned)S.n_blocks - 1;
-    }
-    lds_barrier();
-    if (s_last) {  // the last workgroup: sum the partials, every thread a strided share, then a fixed tree
-        __threadfence();
-        double a = 0.0;
-        for (int b = t; b < S.n_blocks; b += kTileThreads)
-            a += __hip_atomic_load(&S.partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_red[t] = a;
-        lds_barrier();
-        for (int w = kTileThreads / 2; w > 0; w >>= 1) {
-            if (t < w) s_red[t] += s_red[t + w];
-            lds_barrier();
-        }
-        if (t == 0) {
-            T.total[0] = s_red[0];
-            __hip_atomic_store(S.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    tile_total(T, S, s_red, TP);
 }
 
 // One-launch joint pass (treeanc.py:934-1080) over the same tiles, slots and
@@ -394,8 +443,6 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
     uint8_t *s_tip = reinterpret_cast<uint8_t *>(s_pre + (STAGED ? S.n_pre : 0));  // [n_tips][TP]
     uint8_t *s_cx = s_tip + (((size_t)T.n_tips * TP + 15) & ~(size_t)15);             // [n_nodes][TP][CB]
     uint8_t *s_st = s_cx + (((size_t)T.n_nodes * TP * CB + 15) & ~(size_t)15);       // [n_nodes][TP]
-    __shared__ double s_sum[TT_WAVE];
-    __shared__ bool s_last;
 
     const int t = threadIdx.x;
     const int q = t & (TP - 1);
@@ -404,16 +451,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
     const bool act = p < T.P;
 
     constexpr int W = post_rec_words(N);
-    for (int i = t; i < n_rounds; i += kTileThreads) s_rounds[i] = S.rounds[i];
+    stage_in(s_rounds, n_rounds, [&](int i) { return S.rounds[i]; });
     if constexpr (STAGED) {
-        for (int i = t; i < 2 * S.n_post; i += kTileThreads)
-            s_post[i] = reinterpret_cast<const int4 *>(S.post_rec + (size_t)W * (i >> 1))[i & 1];
-        for (int i = t; i < S.n_pre; i += kTileThreads) s_pre[i] = reinterpret_cast<const int4 *>(pre_desc)[i];
+        stage_in(s_post, 2 * S.n_post,
+                 [&](int i) { return reinterpret_cast<const int4 *>(S.post_rec + (size_t)W * (i >> 1))[i & 1]; });
+        stage_in(s_pre, S.n_pre, [&](int i) { return reinterpret_cast<const int4 *>(pre_desc)[i]; });
     }
-    for (int i = t; i < T.n_tips * TP; i += kTileThreads) {
-        const int r = i / TP, pp = blockIdx.x * TP + (i & (TP - 1));
-        s_tip[i] = pp < T.P ? T.tip_codes[(size_t)r * T.P + pp] : 0;
-    }
+    stage_tips(s_tip, T, TP, blockIdx.x * TP);
     lds_barrier();
 
     auto slot = [&](int sl) { return rows + ((size_t)sl * TP + q) * N; };
@@ -659,48 +703,16 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
             }
         }
     }
-    s_red[t] = act ? term : 0.0;
-    lds_barrier();
+    const double term_sum = pattern_sum(act ? term : 0.0, s_red, TP);
     if (t < TP && act) {
-        double lp = log(T.pi[s_st[(size_t)T.root * TP + q]]);
-        for (int k = q; k < kTileThreads; k += TP) lp += s_red[k];
+        const double lp = log(T.pi[s_st[(size_t)T.root * TP + q]]) + term_sum;
         T.jseqlh[p] = lp;
         s_red[t] = T.mult[p] * lp;
     }
     lds_barrier();
 
     // ---- workgroup sum of mult * logL, then the total over workgroups -------
-    if (t < TT_WAVE) {
-        double a = 0.0;
-        for (int k = t; k < TP; k += TT_WAVE) a += (blockIdx.x * TP + k < T.P) ? s_red[k] : 0.0;
-        s_sum[t] = a;
-    }
-    lds_barrier();
-    if (t == 0) {
-        double a = 0.0;
-        for (int k = 0; k < TT_WAVE; ++k) a += s_sum[k];
-        __hip_atomic_store(&S.partial[blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned done = atomicAdd(S.counter, 1u);
-        s_last = done == (unsigned)S.n_blocks - 1;
-    }
-    lds_barrier();
-    if (s_last) {
-        __threadfence();
-        double a = 0.0;
-        for (int b = t; b < S.n_blocks; b += kTileThreads)
-            a += __hip_atomic_load(&S.partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_red[t] = a;
-        lds_barrier();
-        for (int w = kTileThreads / 2; w > 0; w >>= 1) {
-            if (t < w) s_red[t] += s_red[t + w];
-            lds_barrier();
-        }
-        if (t == 0) {
-            T.total[0] = s_red[0];
-            __hip_atomic_store(S.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    tile_total(T, S, s_red, TP);
 }
 
 // Records of the tile pass from the level lists and the current P(t): one
