@@ -1311,6 +1311,9 @@ int32_t tt_tree_create(tt_engine *eng, const tt_tree_desc *D, tt_tree **out) {
     he = hipMemset(d.states, 255, (size_t)nn * P);
     if (he == hipSuccess) he = hipMemset(d.prof, 0, (size_t)nn * P * n * sizeof(float));
     if (he == hipSuccess && T->d_tile_counter) he = hipMemset(T->d_tile_counter, 0, sizeof(unsigned));
+    // the one-launch passes' per-workgroup partials start empty (all-ones bits)
+    if (he == hipSuccess && T->d_tile_partial)
+        he = hipMemset(T->d_tile_partial, 0xFF, (size_t)((T->P + T->tile_tp - 1) / T->tile_tp) * sizeof(double));
     // the passes run on a non-blocking stream that does not order after the
     // null-stream memsets
     if (he == hipSuccess) he = hipDeviceSynchronize();

@@ -101,9 +101,18 @@ __device__ __forceinline__ double pattern_sum(double v, double *s_red, int TP) {
 
 // The workgroup's sum of s_red[k] over its patterns k < TP that exist (mult
 // * logL per pattern) goes to partial[blockIdx.x]; the last workgroup to
-// finish sums the partials (every thread a strided share, then wave
-// shuffles and the waves' sums in a fixed order) into total[0] and resets
-// the counter.  Every thread of the workgroup calls it.
+// count itself sums the partials (every thread a strided share, then wave
+// shuffles and the waves' sums in a fixed order) into total[0], and puts
+// the counter and the partials back to their empty state for the next pass.
+// The partial and the count are device-scope atomics with no fence between
+// them (a release here would write back the whole L2 of the die): the last
+// workgroup may see the count before a partial, and then waits for that
+// partial -- whose store was issued before its count -- in a bounded loop
+// (a partial still empty at the bound makes the total a NaN).  Every
+// thread of the workgroup calls it.
+constexpr unsigned long long kPartialEmpty = ~0ull;  // the bits of an empty partial (a NaN)
+constexpr int kPartialSpin = 1 << 16;
+
 __device__ __forceinline__ void tile_total(const TreeDev &T, const TileDev &S, double *s_red, int TP) {
     __shared__ bool s_last;
     const int t = threadIdx.x;
@@ -113,17 +122,21 @@ __device__ __forceinline__ void tile_total(const TreeDev &T, const TileDev &S, d
         a = wave_sum(a);
         if (t == 0) {
             __hip_atomic_store(&S.partial[blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
-            const unsigned done = atomicAdd(S.counter, 1u);
+            const unsigned done = __hip_atomic_fetch_add(S.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_last = done == (unsigned)S.n_blocks - 1;
         }
     }
     lds_barrier();
     if (s_last) {
-        __threadfence();
+        auto *bits = reinterpret_cast<unsigned long long *>(S.partial);
         double a = 0.0;
-        for (int b = t; b < S.n_blocks; b += kTileThreads)
-            a += __hip_atomic_load(&S.partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int b = t; b < S.n_blocks; b += kTileThreads) {
+            unsigned long long x = __hip_atomic_load(&bits[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int spin = 0; x == kPartialEmpty && spin < kPartialSpin; ++spin)
+                x = __hip_atomic_load(&bits[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            a += __builtin_bit_cast(double, x);
+            __hip_atomic_store(&bits[b], kPartialEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         a = wave_sum(a);
         if ((t & (TT_WAVE - 1)) == 0) s_red[t / TT_WAVE] = a;
         lds_barrier();
@@ -172,15 +185,15 @@ __device__ __forceinline__ void load_rec(const int32_t *base, PreRec<N> &r) {
     for (int i = 0; i < N * N; ++i) r.M[i] = f[i];
 }
 
-// dst[i] = f(i) for i in [0, n), dst in LDS, f a read from memory: each
+// dst[i] = f(i) for i in [start, n), dst in LDS, f a read from memory: each
 // thread issues kStageLoads reads (clamped index) before it writes any, so
 // staging an array costs one memory latency per kStageLoads * kTileThreads
 // elements instead of one per kTileThreads.
 constexpr int kStageLoads = 8;
 template <class V, class F>
-__device__ __forceinline__ void stage_in(V *dst, int n, F f) {
+__device__ __forceinline__ void stage_in(V *dst, int n, F f, int start = 0) {
     const int t = threadIdx.x;
-    for (int b = 0; b < n; b += kStageLoads * kTileThreads) {
+    for (int b = start; b < n; b += kStageLoads * kTileThreads) {
         V v[kStageLoads];
 #pragma unroll
         for (int u = 0; u < kStageLoads; ++u) v[u] = f(min(b + u * kTileThreads + t, n - 1));
@@ -192,20 +205,44 @@ __device__ __forceinline__ void stage_in(V *dst, int n, F f) {
     }
 }
 
+// Staging in two phases, so that the first reads of several arrays are all
+// in flight before any of them is written to LDS: load() issues a thread's
+// first U reads (clamped index), store() writes them and stages whatever
+// lies beyond the first U * kTileThreads elements.
+template <class V, int U>
+struct Stager {
+    V v[U];
+    template <class F>
+    __device__ __forceinline__ void load(int n, F f) {
+        if (n <= 0) return;
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = f(min(u * kTileThreads + (int)threadIdx.x, n - 1));
+    }
+    template <class F>
+    __device__ __forceinline__ void store(V *dst, int n, F f) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = u * kTileThreads + (int)threadIdx.x;
+            if (i < n) dst[i] = v[u];
+        }
+        stage_in(dst, n, f, U * kTileThreads);
+    }
+};
+
 // Copy words [0, n_words) of a record array into LDS (16-byte pieces).
 __device__ __forceinline__ void stage_records(int32_t *dst, const int32_t *src, size_t n_words) {
     const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
     stage_in(reinterpret_cast<uint4 *>(dst), (int)(n_words / 4), [&](int i) { return s4[i]; });
 }
 
-// The workgroup's tip characters: s_tip[r TP + j] = code of tip row r at
-// pattern p0 + j (0 past the last pattern).
-__device__ __forceinline__ void stage_tips(uint8_t *s_tip, const TreeDev &T, int TP, int p0) {
-    stage_in(s_tip, T.n_tips * TP, [&](int i) {
+// Reader of the workgroup's tip characters: element r TP + j is the code of
+// tip row r at pattern p0 + j (0 past the last pattern).
+__device__ __forceinline__ auto tip_reader(const TreeDev &T, int TP, int p0) {
+    return [&T, TP, p0](int i) {
         const int r = i / TP, pp = p0 + (i & (TP - 1));
         const uint8_t c = T.tip_codes[(size_t)r * T.P + min(pp, T.P - 1)];
         return pp < T.P ? c : (uint8_t)0;
-    });
+    };
 }
 
 // STAGED: each phase's records copied into LDS; SUBG: slots are reused once
@@ -233,10 +270,27 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
     const bool act = p < T.P;
 
     // stage the profile map, the round offsets and the tile's tip characters
-    stage_in(s_lut, T.n_codes * N, [&](int i) { return T.lut[i]; });
-    stage_in(s_rounds, n_rounds, [&](int i) { return S.rounds[i]; });
-    stage_tips(s_tip, T, TP, blockIdx.x * TP);
-    if constexpr (STAGED) stage_records(s_rec, S.post_rec, (size_t)S.n_post * post_rec_words(N));
+    {
+        const int n_lut = T.n_codes * N, n_tipb = T.n_tips * TP;
+        const int n_rec = STAGED ? S.n_post * (post_rec_words(N) / 4) : 0;
+        auto f_lut = [&](int i) { return T.lut[i]; };
+        auto f_rd = [&](int i) { return S.rounds[i]; };
+        auto f_tip = tip_reader(T, TP, blockIdx.x * TP);
+        const uint4 *rec4 = reinterpret_cast<const uint4 *>(S.post_rec);
+        auto f_rec = [&](int i) { return rec4[i]; };
+        Stager<float, 1> g_lut;
+        Stager<int, 1> g_rd;
+        Stager<uint8_t, kStageLoads> g_tip;
+        Stager<uint4, kStageLoads> g_rec;
+        g_lut.load(n_lut, f_lut);
+        g_rd.load(n_rounds, f_rd);
+        g_tip.load(n_tipb, f_tip);
+        g_rec.load(n_rec, f_rec);
+        g_lut.store(s_lut, n_lut, f_lut);
+        g_rd.store(s_rounds, n_rounds, f_rd);
+        g_tip.store(s_tip, n_tipb, f_tip);
+        g_rec.store(reinterpret_cast<uint4 *>(s_rec), n_rec, f_rec);
+    }
     lds_barrier();  // the staging loads have landed in LDS (each thread waited for its own)
 
     auto slot = [&](int row) { return rows + ((size_t)row * TP + q) * N; };
@@ -451,13 +505,24 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
     const bool act = p < T.P;
 
     constexpr int W = post_rec_words(N);
-    stage_in(s_rounds, n_rounds, [&](int i) { return S.rounds[i]; });
-    if constexpr (STAGED) {
-        stage_in(s_post, 2 * S.n_post,
-                 [&](int i) { return reinterpret_cast<const int4 *>(S.post_rec + (size_t)W * (i >> 1))[i & 1]; });
-        stage_in(s_pre, S.n_pre, [&](int i) { return reinterpret_cast<const int4 *>(pre_desc)[i]; });
+    {
+        const int n_post = STAGED ? 2 * S.n_post : 0, n_pre = STAGED ? S.n_pre : 0, n_tipb = T.n_tips * TP;
+        auto f_rd = [&](int i) { return S.rounds[i]; };
+        auto f_post = [&](int i) { return reinterpret_cast<const int4 *>(S.post_rec + (size_t)W * (i >> 1))[i & 1]; };
+        auto f_pre = [&](int i) { return reinterpret_cast<const int4 *>(pre_desc)[i]; };
+        auto f_tip = tip_reader(T, TP, blockIdx.x * TP);
+        Stager<int, 1> g_rd;
+        Stager<int4, 4> g_post, g_pre;
+        Stager<uint8_t, kStageLoads> g_tip;
+        g_rd.load(n_rounds, f_rd);
+        g_post.load(n_post, f_post);
+        g_pre.load(n_pre, f_pre);
+        g_tip.load(n_tipb, f_tip);
+        g_rd.store(s_rounds, n_rounds, f_rd);
+        g_post.store(s_post, n_post, f_post);
+        g_pre.store(s_pre, n_pre, f_pre);
+        g_tip.store(s_tip, n_tipb, f_tip);
     }
-    stage_tips(s_tip, T, TP, blockIdx.x * TP);
     lds_barrier();
 
     auto slot = [&](int sl) { return rows + ((size_t)sl * TP + q) * N; };
