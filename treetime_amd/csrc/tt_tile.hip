@@ -417,13 +417,16 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
         constexpr int W = pre_rec_words(N);
         const int *rd = s_rounds + 2 * (tips ? S.Rp + S.Ri : S.Rp);
         const int R = tips ? S.Ra : S.Ri;
-        auto pre_entry = [&](const int4 &a, const int4 &b, auto M) {
+        // sub: the subtree row of an internal node, read from memory a round
+        // ahead (SUBG)
+        auto pre_entry = [&](const int4 &a, const int4 &b, auto M, const float (&sub)[N]) {
             const int c = a.x, pslot = a.y, oslot = a.z, tr = a.w, ir = b.x, sslot = b.y;
             PreRaw<N> Rw;
             if (ir >= 0) {
                 Rw.code = 0;
                 if constexpr (SUBG) {
-                    load_rowf<N>(T.sub + ((size_t)ir * T.P + p) * N, Rw.sub);
+#pragma unroll
+                    for (int i = 0; i < N; ++i) Rw.sub[i] = sub[i];
                 } else {
                     const float *src = slot(sslot);
 #pragma unroll
@@ -448,16 +451,51 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
             valid = r < R && k < rd[2 * r + 1] && act;
             if (!STAGED || valid) load_rec<N>(src + (size_t)W * min(k, S.n_pre - 1), rec);
         };
-        bool va, vb;
-        if (R > 0) fetch(0, va, ra);
-        for (int r = 0; r < R; r += 2) {
-            fetch(r + 1, vb, rb);
-            if (va) pre_entry(ra.a, ra.b, ra.M);
-            lds_barrier();
-            if (r + 1 >= R) break;
-            fetch(r + 2, va, ra);
-            if (vb) pre_entry(rb.a, rb.b, rb.M);
-            lds_barrier();
+        if constexpr (SUBG) {
+            // records two rounds ahead and the subtree row one round ahead
+            // (its address is in the record), three buffers in turn; a
+            // subtree row is read unconditionally (row 0 for tips and
+            // entries a thread does not run, the last pattern for threads
+            // past it)
+            PreRec<N> rc;
+            float sa[N], sb[N], sc[N];
+            auto fetch_sub = [&](const PreRec<N> &rec, bool valid, float (&sub)[N]) {
+                const int ir = valid ? max(rec.b.x, 0) : 0;
+                load_rowf<N>(T.sub + ((size_t)ir * T.P + min(p, T.P - 1)) * N, sub);
+            };
+            bool va = false, vb = false, vc = false;
+            fetch(0, va, ra);
+            fetch(1, vb, rb);
+            fetch_sub(ra, va, sa);
+            for (int r = 0; r < R; r += 3) {
+                fetch(r + 2, vc, rc);
+                fetch_sub(rb, vb, sb);
+                if (va) pre_entry(ra.a, ra.b, ra.M, sa);
+                lds_barrier();
+                if (r + 1 >= R) break;
+                fetch(r + 3, va, ra);
+                fetch_sub(rc, vc, sc);
+                if (vb) pre_entry(rb.a, rb.b, rb.M, sb);
+                lds_barrier();
+                if (r + 2 >= R) break;
+                fetch(r + 4, vb, rb);
+                fetch_sub(ra, va, sa);
+                if (vc) pre_entry(rc.a, rc.b, rc.M, sc);
+                lds_barrier();
+            }
+        } else {
+            const float none[N] = {};
+            bool va, vb;
+            if (R > 0) fetch(0, va, ra);
+            for (int r = 0; r < R; r += 2) {
+                fetch(r + 1, vb, rb);
+                if (va) pre_entry(ra.a, ra.b, ra.M, none);
+                lds_barrier();
+                if (r + 1 >= R) break;
+                fetch(r + 2, va, ra);
+                if (vb) pre_entry(rb.a, rb.b, rb.M, none);
+                lds_barrier();
+            }
         }
     }
 
