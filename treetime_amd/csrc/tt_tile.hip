@@ -297,26 +297,27 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
 
     // ---- postorder: rounds over height-ordered internal rows ---------------------
     LogAcc acc;
-    // a, b: the record's descriptor words; D0, D1: the children's P - I
-    // (registers, or the staged record in LDS)
-    auto post_entry = [&](const int4 &a, const int4 &b, auto D0, auto D1) {
+    // a, b: the record's descriptor words; D0, D1: the children's P - I.
+    // A binary entry runs in two steps: post_gather reads both children's
+    // rows from LDS (a tip's row is its code's row of the profile map, the
+    // same values bin_finish_m would read), then post_finish computes from
+    // registers -- so that the next round's record can be requested between
+    // the two, behind this round's reads.
+    auto post_gather = [&](const int4 &a, const int4 &b, BinRaw<N> &R) {
+        if (a.z != 2) return;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int cref = c ? b.x : a.w;
+            const float *src = cref < 0 ? s_lut + s_tip[max(-cref - 1, 0) * TP + q] * N : slot(max(cref, 0));
+#pragma unroll
+            for (int i = 0; i < N; ++i) R.row[c][i] = src[i];
+        }
+    };
+    auto post_finish = [&](const int4 &a, const int4 &b, auto D0, auto D1, const BinRaw<N> &R) {
         // row v, its slot vs; children as slots (>= 0) or tips (< 0)
-        const int v = a.x, vs = a.y, nchild = a.z, cref0 = a.w, cref1 = b.x;
+        const int v = a.x, vs = a.y, nchild = a.z;
         if (nchild == 2) {
-            BinRaw<N> R;
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                const int cref = c ? cref1 : cref0;
-                if (cref < 0) {
-                    R.code[c] = s_tip[(-cref - 1) * TP + q];
-                } else {
-                    R.code[c] = 0;
-                    const float *src = slot(cref);
-#pragma unroll
-                    for (int i = 0; i < N; ++i) R.row[c][i] = src[i];
-                }
-            }
-            bin_finish_m<N>(T, v, cref0, cref1, D0, D1, p, R, s_lut, acc, slot(vs));
+            bin_finish_m<N>(T, v, 0, 0, D0, D1, p, R, s_lut, acc, slot(vs));  // both rows gathered
         } else {  // polytomies and unary nodes read their child lists from memory
             auto rowf = [&](int cref, float (&row)[N]) {
                 if (cref < 0) {
@@ -355,14 +356,17 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
             if (!STAGED || valid) load_rec<N>(src + (size_t)W * min(k, S.n_post - 1), rec);
         };
         bool va, vb;
+        BinRaw<N> R;
         if (S.Rp > 0) fetch(0, va, ra);
         for (int r = 0; r < S.Rp; r += 2) {
+            if (va) post_gather(ra.a, ra.b, R);
             fetch(r + 1, vb, rb);
-            if (va) post_entry(ra.a, ra.b, ra.D0, ra.D1);
+            if (va) post_finish(ra.a, ra.b, ra.D0, ra.D1, R);
             lds_barrier();
             if (r + 1 >= S.Rp) break;
+            if (vb) post_gather(rb.a, rb.b, R);
             fetch(r + 2, va, ra);
-            if (vb) post_entry(rb.a, rb.b, rb.D0, rb.D1);
+            if (vb) post_finish(rb.a, rb.b, rb.D0, rb.D1, R);
             lds_barrier();
         }
     }
