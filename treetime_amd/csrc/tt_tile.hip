@@ -767,8 +767,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
     }
     lds_barrier();
 
-    // ---- backtrack over every node (treeanc.py:1048-1063) and the score -----
-    double term = 0.0;
+    // ---- backtrack over every node (treeanc.py:1048-1063) ---------------------
     // d: node, parent, internal row or -1, tip row or -1
     auto back_step = [&](const int4 &d) {
         const int c = d.x, sp = s_st[(size_t)d.y * TP + q];
@@ -776,10 +775,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
             const int sc = get_cx(c, sp);
             s_st[(size_t)c * TP + q] = (uint8_t)sc;
             T.states[(size_t)c * T.P + p] = (uint8_t)sc;
-            term += joint_int_term<N>(T, c, sc, sp);
-        } else {
-            if (tips) T.states[(size_t)c * T.P + p] = (uint8_t)get_cx(c, sp);
-            term += joint_tip_term<N>(T, T.loglut, c, s_tip[d.w * TP + q], sp);
+        } else if (tips) {
+            T.states[(size_t)c * T.P + p] = (uint8_t)get_cx(c, sp);
         }
     };
     {
@@ -810,7 +807,35 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
             }
         }
     }
-    const double term_sum = pattern_sum(act ? term : 0.0, s_red, TP);
+    // ---- the score of the decoded states -----------------------------------------
+    // Every non-root node's term from the states in LDS, the thread's share
+    // of the preorder entries in increasing order; the entries of a chunk
+    // are independent, so their reads of log P are in flight together.
+    double term = 0.0;
+    if (act) {
+        const int4 *g;
+        if constexpr (STAGED)
+            g = s_pre;
+        else
+            g = reinterpret_cast<const int4 *>(pre_desc);
+        constexpr int U = 8;
+        const int step = kTileThreads / TP;
+        for (int k = k0; k < S.n_pre; k += U * step) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int kk = k + u * step;
+                const int4 d = g[min(kk, S.n_pre - 1)];
+                const int sp = s_st[(size_t)d.y * TP + q];
+                const double x = d.z >= 0 ? joint_int_term<N>(T, d.x, min((int)s_st[(size_t)d.x * TP + q], N - 1), sp)
+                                          : joint_tip_term<N>(T, T.loglut, d.x, s_tip[max(d.w, 0) * TP + q], sp);
+                v[u] = kk < S.n_pre ? x : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) term += v[u];
+        }
+    }
+    const double term_sum = pattern_sum(term, s_red, TP);
     if (t < TP && act) {
         const double lp = log(T.pi[s_st[(size_t)T.root * TP + q]]) + term_sum;
         T.jseqlh[p] = lp;
