@@ -19,16 +19,19 @@
 // the rounds issue no global load at all: on this architecture loads and
 // stores share one counter that completes in order, and a load waited on in
 // every round would also wait for the previous round's result stores to be
-// acknowledged.  Otherwise the record of the thread's next round is loaded
-// while the current round computes.  Either way a round costs LDS traffic,
-// arithmetic and a barrier.
+// acknowledged.  Otherwise the records stream from memory.  Either way the
+// record of a thread's next round is read into registers while the current
+// round runs, and a round costs LDS traffic, arithmetic and a barrier.
 //
 // Rows are written to HBM as well (subtree rows, profiles, states: the
-// results), but never read back.  The per-(node, pattern) arithmetic is the
-// level kernels' own device code (tt_device.h) on values read from LDS and
-// registers, so profiles and states are identical to theirs; the
-// per-pattern log normalisers are summed per thread and then across the
-// threads of a pattern, so logL agrees to rounding.
+// results), but never read back (except, with reused slots, the subtree
+// rows the preorder needs, read a round ahead).  The per-(node, pattern)
+// arithmetic is the level kernels' own device code (tt_device.h) on values
+// read from LDS and registers, so profiles and states are identical to
+// theirs; the per-pattern log normalisers are summed per thread and then
+// across the threads of a pattern (wave shuffles, then one value per
+// wave), so logL agrees to rounding.  The workgroups' sums meet in a
+// fence-free partial/count protocol (tile_total).
 //
 // Postorder levels are node heights (treeanc.py:840-878), the root step
 // treeanc.py:814-829, preorder levels node depths (treeanc.py:880-932): a
