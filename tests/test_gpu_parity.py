@@ -525,20 +525,24 @@ ONE_LAUNCH_CASES = [c for c in CASES if c[3] in (4, 5) and c[1] <= 1000]
 @pytest.mark.parametrize("case", ONE_LAUNCH_CASES, ids=[c[0] for c in ONE_LAUNCH_CASES])
 @pytest.mark.parametrize("reuse", ["0", "1"], ids=["rows", "reused"])
 @pytest.mark.parametrize("stage", ["1", "0"], ids=["staged", "streamed"])
-def test_one_launch_pass_equals_level_kernels(gpu_engine, case, stage, reuse, monkeypatch):
+@pytest.mark.parametrize("presep", ["", "0"], ids=["pre-region-auto", "pre-region-shared"])
+def test_one_launch_pass_equals_level_kernels(gpu_engine, case, stage, reuse, presep, monkeypatch):
     """The one-launch pass with LDS-resident rows (TT_TILE=1), with its
     records staged in LDS or streamed from memory, and with one LDS slot per
     internal row or slots reused by lifetime (subtree rows re-read from
-    memory in the preorder), gives the level kernels' profiles, subtree rows
+    memory in the preorder), and with the preorder's staged records in a
+    region of their own where that fits or replacing the postorder's at the
+    root step, gives the level kernels' profiles, subtree rows
     and states bit for bit, and their logL to rounding, with and without
     reconstructed tips and for the log-likelihood pass."""
-    say(f"[one-launch] case={case[0]} stage={stage} reuse={reuse}")
+    say(f"[one-launch] case={case[0]} stage={stage} reuse={reuse} pre-region={presep or 'auto'}")
     job, prep, alphabet = make_case(*case)
     monkeypatch.setenv("TT_TILE", "0")
     lv = device_tree(gpu_engine, prep)
     monkeypatch.setenv("TT_TILE", "1")
     monkeypatch.setenv("TT_TILE_STAGE", stage)
     monkeypatch.setenv("TT_TILE_REUSE", reuse)
+    monkeypatch.setenv("TT_TILE_PRE_SEP", presep)
     one = device_tree(gpu_engine, prep)
     assert lv.pass_shape() == (0, False, False)
     tp, staged, reused = one.pass_shape()

@@ -206,6 +206,7 @@ struct tt_tree {
     unsigned *d_tile_counter = nullptr;
     bool tile_stale = true;  // the records do not hold the current P(t) yet
     bool tile_staged = false;  // the kernel stages each phase's records in LDS
+    bool tile_pre_sep = false;  // staged: the preorder's records in their own LDS region (staged at the start)
     bool tile_subg = false;    // LDS slots are reused; the preorder re-reads subtree rows from memory
     bool tile_joint = false;   // the joint pass runs in one launch over the same tiles (k_tile_joint)
     bool tile_joint_staged = false;  // ... with both phases' descriptors staged in LDS
@@ -544,6 +545,7 @@ int32_t enqueue_pass(tt_tree *T, int mode, uint32_t flags, hipStream_t s) {
                         T->tile_Ra, T->tile_tp, (int)T->post_nodes.size(), (int)T->pre_nodes.size(),
                         T->tile_slots, T->tile_root_post, T->tile_root_pre, T->d_tile_post_slot, T->tile_staged,
                         T->tile_subg, T->d_tile_partial, T->d_tile_counter};
+        A.pre_sep = T->tile_pre_sep;
         TT_HIP(ttk::tile_marginal(D, A, tips ? 1 : 0, mode == 1 ? 1 : 0, s));
     } else if (mode == 1 || mode == 3) {
         size_t ri = 0;
@@ -1070,6 +1072,24 @@ int32_t setup_tile(tt_tree *T) {
     if (!tp) return TT_OK;
     const int blocks = (T->P + tp - 1) / tp;
     tile_rounds(T, ttk::tile_threads() / tp, rounds, Rp, Ri, Ra);
+    // a staged pass keeps the preorder's records in an LDS region of their
+    // own, staged with everything else at the start (no memory read at the
+    // root step), when that fits with no fewer workgroups resident
+    // (TT_TILE_PRE_SEP=0: never)
+    T->tile_pre_sep = false;
+    if (staged) {
+        const char *pe = getenv("TT_TILE_PRE_SEP");
+        const int n_slots = subg ? live_slots : T->n_int;
+        const size_t l0 = ttk::tile_lds_bytes(T->d, tp, n_slots, (int)rounds.size(), stage_words);
+        const size_t l1 = ttk::tile_lds_bytes(
+            T->d, tp, n_slots, (int)rounds.size(),
+            ttk::tile_stage_words(T->n, (int)T->post_nodes.size(), (int)T->pre_nodes.size(), true));
+        if (!(pe && pe[0] == '0') && l1 <= budget) {
+            const int b0 = ttk::tile_blocks_per_cu(T->n, l0, true, subg);
+            const int b1 = ttk::tile_blocks_per_cu(T->n, l1, true, subg);
+            T->tile_pre_sep = b1 >= 1 && (b1 >= b0 || (long)b1 * n_cu >= blocks);
+        }
+    }
     // slot maps: reused by lifetime, or one slot per internal row (the
     // preorder then finds each node's subtree row in its own slot)
     std::vector<int32_t> post_slot, pre_slot;

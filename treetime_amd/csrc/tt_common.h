@@ -176,7 +176,7 @@ bool mfma_enabled();
 int tile_threads();
 int tile_post_rec_words(int n);
 int tile_pre_rec_words(int n);
-size_t tile_stage_words(int n, int n_post, int n_pre);
+size_t tile_stage_words(int n, int n_post, int n_pre, bool sep = false);
 size_t tile_lds_bytes(const TreeDev &T, int tp, int n_slots, int n_rounds, size_t stage_words);
 int tile_blocks_per_cu(int n, size_t lds, bool staged, bool subg);
 // one-launch joint pass over the same tiles (k_tile_joint)
@@ -194,6 +194,7 @@ struct TileArgs {
     bool staged, subg;                 // records staged in LDS; slots reused (subtree rows re-read from memory)
     double *partial;
     unsigned *counter;
+    bool pre_sep = false;              // staged: the preorder's records in their own LDS region, staged at the start
 };
 hipError_t tile_marginal(const TreeDev &T, const TileArgs &A, int tips, int pre, hipStream_t s);
 hipError_t tile_joint(const TreeDev &T, const TileArgs &A, const int32_t *pre_desc, int tips, bool staged,

@@ -63,6 +63,9 @@ struct TileDev {
     const int32_t *post_slot;   // slot of every internal row (polytomies' child lists), or null: slot = row
     double *partial;            // [n_blocks] sum over the workgroup's patterns of mult * logL
     unsigned *counter;          // workgroups finished; reset to 0 by the last one
+    int32_t pre_off = -1;       // STAGED: word offset of the preorder's records in the staged region when
+                                // they have their own (staged at the start), or -1: they replace the
+                                // postorder's at the root step
 };
 
 // Workgroup barrier that orders LDS only: rounds talk through LDS, and their
@@ -232,12 +235,6 @@ struct Stager {
     }
 };
 
-// Copy words [0, n_words) of a record array into LDS (16-byte pieces).
-__device__ __forceinline__ void stage_records(int32_t *dst, const int32_t *src, size_t n_words) {
-    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
-    stage_in(reinterpret_cast<uint4 *>(dst), (int)(n_words / 4), [&](int i) { return s4[i]; });
-}
-
 // Reader of the workgroup's tip characters: element r TP + j is the code of
 // tip row r at pattern p0 + j (0 past the last pattern).
 __device__ __forceinline__ auto tip_reader(const TreeDev &T, int TP, int p0) {
@@ -271,11 +268,23 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
     const int k0 = t / TP;       // this thread's entry within a round
     const int p = blockIdx.x * TP + q;
     const bool act = p < T.P;
+    // read now what the root step needs from memory: a vector load there
+    // would wait for every result store the postorder issued (loads and
+    // stores complete in order)
+    const double mult_p = T.mult[min(p, T.P - 1)];
+    double pi_r[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) pi_r[j] = T.pi[j];
 
-    // stage the profile map, the round offsets and the tile's tip characters
+    // stage the profile map, the round offsets, the tile's tip characters and
+    // the records (the preorder's too when they have their own region)
+    const uint4 *pre4 = reinterpret_cast<const uint4 *>(S.pre_rec);
+    auto f_pre = [&](int i) { return pre4[i]; };
+    const int n_pre4 = S.n_pre * (pre_rec_words(N) / 4);
     {
         const int n_lut = T.n_codes * N, n_tipb = T.n_tips * TP;
         const int n_rec = STAGED ? S.n_post * (post_rec_words(N) / 4) : 0;
+        const int n_pre_now = STAGED && pre && S.pre_off >= 0 ? n_pre4 : 0;
         auto f_lut = [&](int i) { return T.lut[i]; };
         auto f_rd = [&](int i) { return S.rounds[i]; };
         auto f_tip = tip_reader(T, TP, blockIdx.x * TP);
@@ -284,15 +293,17 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
         Stager<float, 1> g_lut;
         Stager<int, 1> g_rd;
         Stager<uint8_t, kStageLoads> g_tip;
-        Stager<uint4, kStageLoads> g_rec;
+        Stager<uint4, kStageLoads> g_rec, g_pre;
         g_lut.load(n_lut, f_lut);
         g_rd.load(n_rounds, f_rd);
         g_tip.load(n_tipb, f_tip);
         g_rec.load(n_rec, f_rec);
+        g_pre.load(n_pre_now, f_pre);
         g_lut.store(s_lut, n_lut, f_lut);
         g_rd.store(s_rounds, n_rounds, f_rd);
         g_tip.store(s_tip, n_tipb, f_tip);
         g_rec.store(reinterpret_cast<uint4 *>(s_rec), n_rec, f_rec);
+        g_pre.store(reinterpret_cast<uint4 *>(s_rec + max(S.pre_off, 0)), n_pre_now, f_pre);
     }
     lds_barrier();  // the staging loads have landed in LDS (each thread waited for its own)
 
@@ -373,9 +384,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
             lds_barrier();
         }
     }
-    // the preorder's records replace the postorder's while the root step runs
+    // the preorder's records replace the postorder's while the root step
+    // runs, unless they were staged at the start
     if constexpr (STAGED) {
-        if (pre) stage_records(s_rec, S.pre_rec, (size_t)S.n_pre * pre_rec_words(N));
+        if (pre && S.pre_off < 0) stage_in(reinterpret_cast<uint4 *>(s_rec), n_pre4, f_pre);
     }
 
     // ---- root (treeanc.py:814-829) ------------------------------------------
@@ -386,7 +398,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
         double x[N], s = 0.0;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            x[j] = T.pi[j] * (double)rr[j];
+            x[j] = pi_r[j] * (double)rr[j];
             s += x[j];
         }
         const double lp = log(s) + off;
@@ -410,7 +422,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
 #pragma unroll
             for (int j = 0; j < N; ++j) rp[j] = out[j];
         }
-        s_red[t] = T.mult[p] * lp;
+        s_red[t] = mult_p * lp;
     }
     // the preorder reads the subtree rows back from memory when slots were
     // reused: a full barrier makes the postorder's row stores visible
@@ -449,7 +461,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_marginal(TreeDev T, TileD
         };
         const int32_t *src;  // as in the postorder: records one round ahead
         if constexpr (STAGED)
-            src = s_rec;
+            src = s_rec + max(S.pre_off, 0);
         else
             src = S.pre_rec;
         PreRec<N> ra, rb;
@@ -548,6 +560,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
     const int k0 = t / TP;
     const int p = blockIdx.x * TP + q;
     const bool act = p < T.P;
+    const double mult_p = T.mult[min(p, T.P - 1)];  // read now, as in the marginal pass
 
     constexpr int W = post_rec_words(N);
     {
@@ -842,7 +855,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_joint(TreeDev T, TileDev 
     if (t < TP && act) {
         const double lp = log(T.pi[s_st[(size_t)T.root * TP + q]]) + term_sum;
         T.jseqlh[p] = lp;
-        s_red[t] = T.mult[p] * lp;
+        s_red[t] = mult_p * lp;
     }
     lds_barrier();
 
@@ -897,9 +910,11 @@ int tile_threads() { return kTileThreads; }
 int tile_post_rec_words(int n) { return post_rec_words(n); }
 int tile_pre_rec_words(int n) { return pre_rec_words(n); }
 
-// Words of the staged-record region: the larger phase's records.
-size_t tile_stage_words(int n, int n_post, int n_pre) {
-    return std::max((size_t)n_post * post_rec_words(n), (size_t)n_pre * pre_rec_words(n));
+// Words of the staged-record region: the larger phase's records, or both
+// phases' (sep: the preorder's records in their own region).
+size_t tile_stage_words(int n, int n_post, int n_pre, bool sep) {
+    const size_t a = (size_t)n_post * post_rec_words(n), b = (size_t)n_pre * pre_rec_words(n);
+    return sep ? a + b : std::max(a, b);
 }
 
 // LDS bytes of k_tile_marginal for tp patterns per workgroup, n_slots rows
@@ -1001,8 +1016,10 @@ hipError_t tile_pack(const TreeDev &T, const int32_t *post_desc, int n_post, con
 hipError_t tile_marginal(const TreeDev &T, const TileArgs &A, int tips, int pre, hipStream_t s) {
     TileDev S{A.post_rec, A.pre_rec, A.rounds, A.Rp, A.Ri, A.Ra, A.tp, (T.P + A.tp - 1) / A.tp, A.n_post, A.n_pre,
               A.n_slots, A.root_post, A.root_pre, A.subg ? A.post_slot : nullptr, A.partial, A.counter};
+    const bool sep = A.staged && A.pre_sep;
+    S.pre_off = sep ? A.n_post * post_rec_words(T.n) : -1;
     const size_t lds = tile_lds_bytes(T, A.tp, A.n_slots, 2 * (A.Rp + A.Ri + A.Ra),
-                                      A.staged ? tile_stage_words(T.n, A.n_post, A.n_pre) : 0);
+                                      A.staged ? tile_stage_words(T.n, A.n_post, A.n_pre, sep) : 0);
     const dim3 grid((unsigned)S.n_blocks), block(kTileThreads);
     if (A.staged && A.subg) {
         TT_TILE_DISPATCH(T.n, hipLaunchKernelGGL((k_tile_marginal<N_, true, true>), grid, block, lds, s, T, S, tips, pre));
